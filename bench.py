@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark of the MARL field path on one MI355X.
+
+Workload (the stencil + light stress grid): 256x256x128 voxels x 12 f32
+species, 5% occupancy in smoothed-noise blobs, per-voxel deposit factor in
+[0.2, 1], D_s in [0.05, 0.16], 8 masked substeps per tick, then the
+Beer-Lambert light scan.  One step = one tick = 8 substeps + 1 light scan on
+HBM-resident state.  Default: 125 steps = 1,000 substeps.
+
+Output: one JSON line.  `value` is voxel-species updates per second of the
+whole step (G/s).  `roofline` is the masked stencil's algorithmic bytes per
+launch (96.125 B per voxel: 12 f32 read + 12 f32 written + the 1-bit mask)
+over its average launch time from hipEvents recorded on the library's stream
+around the same timed launches.  `cpu_baseline` times the f32 CPU oracle
+stencil (OpenMP over z like the reference's rayon loop) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Gvoxel-species/s stencil; Mcell-updates/s plain vs surrogate at ≤1e-3 rel err"
+HBM_PEAK_GBS = 8000.0
+STENCIL_BYTES_PER_VOXEL = 96.125
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(field, occ, dims, sim, substeps=40):
+    from oracle.binding import Oracle
+
+    orc = Oracle()
+    threads = len(os.sched_getaffinity(0))
+    f = np.ascontiguousarray(field, np.float32)
+    dt_sub = np.float32(sim.dt) / np.float32(sim.diffusion_substeps)
+    t0 = time.perf_counter()
+    for _ in range(substeps):
+        f = orc.substep(f, occ, dims, sim, dt_sub, nthreads=threads)
+    dt = time.perf_counter() - t0
+    nvs = dims[0] * dims[1] * dims[2] * 12
+    return {
+        "value": round(substeps * nvs / dt / 1e9, 4),
+        "unit": "Gvoxel-species/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{substeps} masked substeps of the 256x256x128x12 stress grid, f32 C oracle, "
+                  f"OpenMP over z-layers ({dt:.2f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=125)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the grid would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from __graft_entry__ import load_package
+
+    pkg = load_package()
+    from marl_ochre_amd import scenarios  # noqa: E402  (registered by load_package)
+
+    dims = (256, 256, 128)
+    log("generating stress grid", dims)
+    field, occ, sim = scenarios.stencil_stress(dims, seed=0)
+    nvox = dims[0] * dims[1] * dims[2]
+
+    g = pkg.MarlGpu(dims)
+    g.upload_field_aos(field)
+    g.upload_occupancy(occ)
+    tile = g.tile if args.no_autotune else g.autotune(sim)
+    ry, zc = g.tile_shape(tile)
+    log(f"stencil tile {tile}: rows/thread={ry} z-chunk={zc}")
+
+    copy_ms, copy_bytes = 0.0, 0.0
+    import ctypes
+
+    ms = ctypes.c_float()
+    by = ctypes.c_double()
+    g._chk(g.lib.marl_copy_probe(g.h, 20, ctypes.byref(ms), ctypes.byref(by)))
+    copy_gbs = 20 * by.value / (ms.value * 1e-3) / 1e9
+    log(f"copy probe: {copy_gbs:.0f} GB/s")
+
+    for _ in range(args.warmup):
+        g.time_diffuse(sim, sim.diffusion_substeps)
+        g.time_light(sim, 1)
+    g.sync()
+
+    stencil_ms = 0.0
+    light_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stencil_ms += g.time_diffuse(sim, sim.diffusion_substeps)
+        light_ms += g.time_light(sim, 1)
+    g.sync()
+    wall = time.perf_counter() - t0
+
+    n_sub = args.steps * sim.diffusion_substeps
+    ms_per_step = wall * 1e3 / args.steps
+    value = n_sub * nvox * 12 / wall / 1e9
+    per_launch_s = stencil_ms * 1e-3 / n_sub
+    alg_bytes = STENCIL_BYTES_PER_VOXEL * nvox
+    achieved = alg_bytes / per_launch_s / 1e9
+    log(f"stencil {per_launch_s * 1e6:.1f} us/substep ({achieved:.0f} GB/s algorithmic), "
+        f"light {light_ms * 1e3 / args.steps:.1f} us/tick")
+
+    traffic = None
+    tf = ROOT / "profiles" / "stencil_traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gvoxel-species/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "stencil+light stress grid 256x256x128x12, 5% occupancy blobs, deposit factor U[0.2,1], "
+                        "8 masked substeps + 1 light scan per step (1 step = 1 tick)",
+            "grid": list(dims),
+            "substeps_per_step": sim.diffusion_substeps,
+            "stencil_tile": {"id": tile, "rows_per_thread": ry, "z_chunk": zc},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "k_stencil (one masked substep)",
+            "algorithmic_bytes_per_launch": alg_bytes,
+            "avg_launch_us": round(per_launch_s * 1e6, 2),
+            "measured_copy_GBs": round(copy_gbs, 1),
+        },
+        "light_us_per_tick": round(light_ms * 1e3 / args.steps, 2),
+    }
+    g.close()
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(field, occ, dims, sim)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

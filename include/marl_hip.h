@@ -1,0 +1,217 @@
+/*
+ * marl_hip.h — C ABI of the MI355X (gfx950) per-tick field-and-cell path of MARL.
+ *
+ * Every entry point replaces a piece of the reference engine's tick
+ * (alvertremantel/MARL-OCHRE, Rust).  The reference seam is the `gpu` Cargo
+ * feature's `GpuFieldDiffuser` (crates/marl-gpu/src/field_diffusion.rs:31-268),
+ * selected by `--gpu-diffusion` (crates/marl-engine/src/main.rs:3-6).  This ABI
+ * keeps that contract for the drop-in call and adds a GPU-resident tick that
+ * covers the rest of `marl_sim::run`'s loop body (crates/marl-sim/src/lib.rs:160-277).
+ *
+ * Conventions
+ *   - every call returns an int status (MARL_OK or one of MARL_ERR_*);
+ *     the codes mirror `GpuError` (crates/marl-gpu/src/context.rs:4-21);
+ *   - on error host buffers are left untouched (the reference changes the host
+ *     field only after a successful readback, field_diffusion.rs:285-290) and
+ *     marl_last_error() returns a message;
+ *   - host buffers are owned by the caller, device memory by the context;
+ *   - one host thread per context; calls block until the GPU work is done
+ *     unless the name says otherwise (the `_async` timing helpers).
+ *
+ * Layouts at the boundary
+ *   - field AoS: ((z*Y + y)*X + x)*12 + s, f32 (crates/marl-field/src/field.rs:33-35);
+ *   - field SoA: (s*Z + z)*Y*X + y*X + x, f32 (device layout);
+ *   - occupancy: one byte per voxel, (z*Y + y)*X + x, nonzero = occupied
+ *     (crates/marl-sim/src/lib.rs:174-179);
+ *   - light: (z*Y + y)*X + x, f32 (crates/marl-field/src/light.rs:20-24);
+ *   - cells: marl_cell_state[n] + marl_ruleset[n] in the engine's `cells` Vec order.
+ */
+#ifndef MARL_HIP_H
+#define MARL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MARL_S_EXT 12          /* marl-config/src/lib.rs:22 */
+#define MARL_M_INT 16          /* marl-config/src/lib.rs:23 */
+#define MARL_R_MAX 16          /* marl-config/src/lib.rs:26 */
+#define MARL_S_RECEPTORS 8     /* marl-config/src/lib.rs:27 */
+#define MARL_S_TRANSPORTERS 8  /* marl-config/src/lib.rs:28 */
+#define MARL_S_EFFECTORS 8     /* marl-config/src/lib.rs:29 */
+
+/* Status codes (≙ GpuError, crates/marl-gpu/src/context.rs:4-9). */
+enum {
+  MARL_OK = 0,
+  MARL_ERR_NO_DEVICE = 1,     /* ≙ GpuError::NoAdapter */
+  MARL_ERR_DEVICE_INIT = 2,   /* ≙ GpuError::DeviceRequest */
+  MARL_ERR_TRANSFER = 3,      /* ≙ GpuError::BufferMap (and any failed HIP call after init) */
+  MARL_ERR_INVALID_INPUT = 4  /* ≙ GpuError::InvalidInput */
+};
+
+/* Cell events (≙ CellEvent, crates/marl-cell/src/cell.rs:154-160). */
+enum { MARL_EVENT_NONE = 0, MARL_EVENT_DIVISION = 1, MARL_EVENT_DEATH = 2, MARL_EVENT_QUIESCENCE = 3 };
+
+/* Kinetics modes for the resident tick. */
+enum { MARL_KINETICS_PLAIN = 0, MARL_KINETICS_SURROGATE = 1, MARL_KINETICS_SHADOW = 2 };
+
+typedef struct marl_ctx marl_ctx;
+
+/* Grid size.  The reference fixes it at compile time (marl-config/src/lib.rs:17-19);
+ * here it is per context.  x must be a multiple of 32. */
+typedef struct {
+  int32_t x, y, z;
+} marl_dims;
+
+/* The SimulationConfig fields the path reads (marl-config/src/lib.rs:41-106,
+ * defaults :108-168), plus the seed of the counter-based RNG that replaces the
+ * reference's OS-seeded ThreadRng (crates/marl-sim/src/lib.rs:35). */
+typedef struct {
+  float dt;
+  int32_t diffusion_substeps;
+  float d_voxel[MARL_S_EXT];
+  float lambda_decay[MARL_S_EXT];
+  float source_rate_oxidant, source_rate_carbon, source_rate_reductant;
+  float epsilon, c_max, lambda_maintenance, hard_death_floor, reaction_maintenance;
+  float base_division_prep, prep_maintenance_multiplier, rush_penalty_rate;
+  float alpha_eps, k_eps;
+  float light_efficiency, surface_intensity, cell_absorption, chemical_absorption, light_floor;
+  float mutation_stddev, structural_mutation_rate_mult, meta_mutation_rate;
+  float meta_mutation_clamp_low, meta_mutation_clamp_high;
+  float hill_exponent_clamp_low, hill_exponent_clamp_high, active_reaction_threshold;
+  int32_t division_neighbor_distance;
+  uint32_t _pad0;
+  uint64_t rng_seed;
+} marl_sim_params;
+
+/* Ruleset parameter records (crates/marl-cell/src/cell.rs:50-151). */
+typedef struct { float k_half, n_hill, gain; } marl_receptor;                 /* 12 B */
+typedef struct { float uptake_rate, secrete_rate; uint8_t ext_species, int_species, _pad[2]; } marl_transport; /* 12 B */
+typedef struct { uint8_t substrate, product, catalyst, cofactor; float k_m, v_max, k_cat; } marl_reaction;   /* 16 B */
+typedef struct { float threshold, rate; uint8_t int_species, ext_species, _pad[2]; } marl_effector;          /* 12 B */
+
+typedef struct {
+  marl_receptor receptors[MARL_S_RECEPTORS];      /*   0: 96 B */
+  marl_transport transport[MARL_S_TRANSPORTERS];  /*  96: 96 B */
+  marl_reaction reactions[MARL_R_MAX];            /* 192: 256 B */
+  marl_effector effectors[MARL_S_EFFECTORS];      /* 448: 96 B */
+  float division_energy, death_energy, quiescence_energy, division_prep_ticks; /* 544: FateParams */
+  float hgt_propensity, mutation_rate;            /* 560 */
+  float _reserved[2];                             /* 568 → 576 B */
+} marl_ruleset;
+
+/* Cell state (crates/marl-cell/src/cell.rs:168-183), 96 B. */
+typedef struct {
+  float internal[MARL_M_INT];   /*  0: 64 B */
+  uint16_t pos[3];              /* 64 */
+  uint8_t quiescent;            /* 70 */
+  uint8_t starter_type;         /* 71 */
+  uint64_t lineage_id;          /* 72 */
+  uint32_t age;                 /* 80 */
+  uint16_t prep_remaining;      /* 84 */
+  uint16_t _pad;                /* 86 */
+  uint32_t _reserved[2];        /* 88 → 96 B */
+} marl_cell_state;
+
+/* Per-tick counters of the resident tick. */
+typedef struct {
+  int64_t n_cells;        /* population after compaction */
+  int64_t divisions;      /* ≙ tick_divisions (lib.rs:243) */
+  int64_t deaths;         /* ≙ tick_deaths (lib.rs:248) */
+  int64_t births_added;   /* births that survived the duplicate-position check (lib.rs:270-277) */
+  int32_t levels;         /* depth of the conflict schedule of the cell pass */
+  int32_t _pad;
+} marl_tick_stats;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* ≙ GpuFieldDiffuser::new (crates/marl-gpu/src/field_diffusion.rs:32-156).
+ * Selects device 0, allocates the field double buffer, the occupancy bit mask
+ * and the light plane.  Returns MARL_ERR_NO_DEVICE when no GPU is visible. */
+int marl_create(const marl_dims* dims, marl_ctx** out);
+/* ≙ Drop for GpuFieldDiffuser: frees every device buffer. */
+void marl_destroy(marl_ctx* ctx);
+/* Message of the last failed call (≙ GpuError's Display, context.rs:11-20). */
+const char* marl_last_error(const marl_ctx* ctx);
+/* Number of visible GPUs; 0 when none (no error). */
+int marl_device_count(void);
+/* Library version string. */
+const char* marl_version(void);
+
+/* ---- drop-in (same contract as the reference seam) ---------------------- */
+
+/* ≙ GpuFieldDiffuser::diffuse_tick_with_cells (field_diffusion.rs:158-268):
+ * uploads the AoS field and the byte occupancy, runs sim->diffusion_substeps
+ * masked substeps with dt_sub = dt / substeps, and writes the result back into
+ * field_aos.  Length checks as :164-175 (n_floats == X*Y*Z*12, n_voxels ==
+ * X*Y*Z), no-op OK when substeps == 0 (:176-178). */
+int marl_diffuse_tick(marl_ctx* ctx, float* field_aos, size_t n_floats,
+                      const uint8_t* occupancy, size_t n_voxels, const marl_sim_params* sim);
+
+/* ---- resident field ----------------------------------------------------- */
+
+int marl_upload_field_aos(marl_ctx* ctx, const float* field_aos, size_t n_floats);
+int marl_download_field_aos(marl_ctx* ctx, float* field_aos, size_t n_floats);
+int marl_upload_field_soa(marl_ctx* ctx, const float* field_soa, size_t n_floats);
+int marl_download_field_soa(marl_ctx* ctx, float* field_soa, size_t n_floats);
+/* Byte occupancy → device bit mask (for field-only use; the resident tick
+ * rebuilds the mask from the cell list, lib.rs:174-179). */
+int marl_upload_occupancy(marl_ctx* ctx, const uint8_t* occupancy, size_t n_voxels);
+int marl_download_occupancy(marl_ctx* ctx, uint8_t* occupancy, size_t n_voxels);
+
+/* ≙ Field::apply_boundary_sources (crates/marl-field/src/field.rs:231-250). */
+int marl_apply_boundary_sources(marl_ctx* ctx, const marl_sim_params* sim);
+/* ≙ Field::diffuse_tick_with_cells (field.rs:212-217) on the resident field
+ * with the resident occupancy: n_substeps substeps of dt_sub = dt / sim->diffusion_substeps. */
+int marl_diffuse_substeps(marl_ctx* ctx, const marl_sim_params* sim, int32_t n_substeps);
+/* ≙ LightField::update (crates/marl-field/src/light.rs:29-59). */
+int marl_light_update(marl_ctx* ctx, const marl_sim_params* sim);
+int marl_download_light(marl_ctx* ctx, float* light, size_t n_voxels);
+
+/* ---- resident cells and the full tick ------------------------------------ */
+
+/* Upload the population in `cells` Vec order (crates/marl-sim/src/lib.rs:56-57).
+ * Positions must be distinct and in bounds. */
+int marl_upload_cells(marl_ctx* ctx, const marl_cell_state* cells, const marl_ruleset* rulesets, size_t n);
+int marl_cell_count(marl_ctx* ctx, int64_t* n);
+int marl_download_cells(marl_ctx* ctx, marl_cell_state* cells, marl_ruleset* rulesets, size_t capacity, int64_t* n_out);
+/* One or more full ticks (lib.rs:164-277): boundary sources, occupancy rebuild,
+ * diffusion, light, the ordered cell pass, fate (division with mutation,
+ * death) and compaction.  `tick0` is the tick number of the first tick (keys
+ * the RNG).  Stats of the last tick go to *stats when non-null. */
+int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t n_ticks, marl_tick_stats* stats);
+/* Only the cell pass (lib.rs:204-217) on the resident field/light/cells;
+ * events (MARL_EVENT_*) per cell to `events` when non-null (length >= n). */
+int marl_cell_pass(marl_ctx* ctx, const marl_sim_params* sim, uint8_t* events, size_t n_events, int32_t* levels_out);
+/* FP64 per-species totals: out[0..11] empty voxels, out[12..23] occupied
+ * voxels, out[24..39] cell internals (no reference counterpart: the reference
+ * keeps no mass budget, docs/INFO.md:276). */
+int marl_mass_budget(marl_ctx* ctx, double* out, size_t n_out);
+
+/* ---- timing helpers (hipEvents on the context's stream) ------------------ */
+
+/* Launch n substeps back to back on the context stream and return the elapsed
+ * GPU time between hipEvents recorded around them. */
+int marl_time_diffuse(marl_ctx* ctx, const marl_sim_params* sim, int32_t n_substeps, float* ms_out);
+int marl_time_light(marl_ctx* ctx, const marl_sim_params* sim, int32_t n_updates, float* ms_out);
+/* Choose stencil tile shape by timing the candidates on the resident field
+ * (the project's autotuner).  Writes the chosen tile id. */
+int marl_autotune_stencil(marl_ctx* ctx, const marl_sim_params* sim, int32_t* tile_out);
+int marl_set_stencil_tile(marl_ctx* ctx, int32_t tile);
+int marl_get_stencil_tile(marl_ctx* ctx, int32_t* tile);
+int marl_stencil_tile_count(void);
+/* rows-per-thread and z-chunk of a tile id (for reports). */
+int marl_stencil_tile_shape(int32_t tile, int32_t* rows_per_thread, int32_t* z_chunk);
+/* Plain copy of the resident field into the scratch buffer (n_reps times,
+ * float4 loads/stores): the HBM calibration kernel for roofline and counter
+ * scaling.  *bytes_out = bytes read + written per rep. */
+int marl_copy_probe(marl_ctx* ctx, int32_t n_reps, float* ms_out, double* bytes_out);
+int marl_sync(marl_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MARL_HIP_H */

@@ -1,0 +1,240 @@
+// marl_internal.h — shared declarations of the gfx950 MARL library (host + device).
+//
+// Device layout (HBM):
+//   field      SoA planes f32 [s][z][y][x], plane stride X*Y*Z, ping-pong pair
+//   occupancy  1 bit per voxel, u32 words along x: word (z*Y + y)*XW + x/32, XW = X/32
+//   light      f32 [z][y][x]
+//   cells      AoS marl_cell_state (96 B) + marl_ruleset (576 B) in `cells` Vec order
+//   cell_at    i32 per voxel: index of the cell in the voxel, -1 when free
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/marl_hip.h"
+
+#define MARL_VERSION "marl-ochre-amd 0.1.0 (gfx950)"
+
+// ---------------------------------------------------------------------------
+// Math shared by every kernel that must agree bit-for-bit with the CPU oracle.
+// f32 arithmetic in the reference's operator order; this library is built with
+// -ffp-contract=off and IEEE division/sqrt, denormals preserved.
+
+// exp(x) by Cody-Waite reduction (ln2 split 0.693145751953125 + 1.4286068e-6)
+// and the degree-7 Taylor polynomial of e^r on |r| <= ln2/2; results below
+// FLT_MIN flush to zero.  Within 2 ulp of libm expf.
+__host__ __device__ inline float marl_expf(float x) {
+  if (x != x) return x;
+  if (x > 88.72283f) return __builtin_huge_valf();
+  if (x < -87.33654f) return 0.0f;
+  float t = x * 1.44269504f;
+  float n = floorf(t + 0.5f);
+  float r = x - n * 0.693145751953125f;
+  r = r - n * 1.428606765330187e-06f;
+  float p = 1.98412698e-04f;
+  p = p * r + 1.38888889e-03f;
+  p = p * r + 8.33333333e-03f;
+  p = p * r + 4.16666667e-02f;
+  p = p * r + 1.66666667e-01f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  int ni = (int)n;
+  if (ni > 127) { p = p * 2.0f; ni -= 1; }
+  uint32_t b = (uint32_t)(ni + 127) << 23;
+  float s;
+  __builtin_memcpy(&s, &b, 4);
+  return p * s;
+}
+
+// log(x) for normal x > 0: x = m*2^e with m in [sqrt(1/2), sqrt(2)], then
+// log m = 2 atanh(f), f = (m-1)/(m+1), odd series to f^9.
+__host__ __device__ inline float marl_logf(float x) {
+  if (!(x > 0.0f)) return (x == 0.0f) ? -__builtin_huge_valf() : __builtin_nanf("");
+  if (x == __builtin_huge_valf()) return x;
+  uint32_t b;
+  __builtin_memcpy(&b, &x, 4);
+  int e = (int)((b >> 23) & 0xff) - 127;
+  uint32_t mb = (b & 0x007fffffu) | 0x3f800000u;
+  float m;
+  __builtin_memcpy(&m, &mb, 4);
+  if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
+  float f = (m - 1.0f) / (m + 1.0f);
+  float f2 = f * f;
+  float s = 1.11111111e-01f;
+  s = s * f2 + 1.42857143e-01f;
+  s = s * f2 + 2.00000000e-01f;
+  s = s * f2 + 3.33333333e-01f;
+  s = s * f2;
+  float twof = 2.0f * f;
+  float lm = twof + twof * s;
+  float fe = (float)e;
+  return fe * 0.693145751953125f + (fe * 1.428606765330187e-06f + lm);
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x64-10 counter-based RNG (Salmon et al. SC'11), the same bijection as
+// numpy.random.Philox.  Stream layout: key (seed, stream), counter
+// (draw/8, entity, 0, 0); u32 draw j is word (j>>1)&3, low half when j even.
+
+#define MARL_TAG_SEEDING 1ull
+#define MARL_TAG_DIVISION 2ull
+#define MARL_TAG_SCENARIO 3ull
+
+__host__ __device__ inline void marl_mulhilo64(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  *hi = __umul64hi(a, b);
+  *lo = a * b;
+#else
+  __uint128_t p = (__uint128_t)a * b;
+  *hi = (uint64_t)(p >> 64);
+  *lo = (uint64_t)p;
+#endif
+}
+
+__host__ __device__ inline void marl_philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
+#pragma unroll
+  for (int round = 0; round < 10; ++round) {
+    if (round > 0) {
+      k0 += 0x9E3779B97F4A7C15ull;
+      k1 += 0xBB67AE8584CAA73Bull;
+    }
+    uint64_t hi0, lo0, hi1, lo1;
+    marl_mulhilo64(0xD2E7470EE14C6C93ull, c[0], &hi0, &lo0);
+    marl_mulhilo64(0xCA5A826395121157ull, c[2], &hi1, &lo1);
+    uint64_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+  }
+}
+
+// Sequential reader of one entity's stream; caches the current 8-draw block.
+struct MarlRng {
+  uint64_t seed, stream, entity, draw;
+  uint64_t blk[4];
+  uint64_t blk_id;
+  __host__ __device__ void init(uint64_t s, uint64_t st, uint64_t e, uint64_t d0 = 0) {
+    seed = s; stream = st; entity = e; draw = d0; blk_id = ~0ull;
+  }
+  __host__ __device__ uint32_t u32() {
+    uint64_t b = draw >> 3;
+    if (b != blk_id) {
+      blk[0] = b; blk[1] = entity; blk[2] = 0; blk[3] = 0;
+      marl_philox4x64(blk, seed, stream);
+      blk_id = b;
+    }
+    uint64_t w = blk[(draw >> 1) & 3];
+    uint32_t r = (draw & 1) ? (uint32_t)(w >> 32) : (uint32_t)w;
+    ++draw;
+    return r;
+  }
+  // uniform f32 in [0,1) from the top 24 bits
+  __host__ __device__ float u01() { return (float)(u32() >> 8) * 0x1p-24f; }
+  // integer in [0,k) by the widening multiply (k >= 1)
+  __host__ __device__ uint32_t range(uint32_t k) { return (uint32_t)(((uint64_t)u32() * k) >> 32); }
+  __host__ __device__ uint64_t u64() {
+    uint64_t lo = u32();
+    uint64_t hi = u32();
+    return lo | (hi << 32);
+  }
+  // 0 + sd * N(0,1), Marsaglia polar method
+  __host__ __device__ float normal(float sd) {
+    float u, v, s;
+    do {
+      u = 2.0f * u01() - 1.0f;
+      v = 2.0f * u01() - 1.0f;
+      s = u * u + v * v;
+    } while (!(s < 1.0f && s > 0.0f));
+    float z = u * sqrtf((-2.0f * marl_logf(s)) / s);
+    return 0.0f + sd * z;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Stencil parameters passed by value.
+struct StencilParams {
+  float dt_sub, alpha_eps, k_eps;
+  float d_voxel[MARL_S_EXT];
+  float lambda_decay[MARL_S_EXT];
+  // boundary sources fused into the first substep of a tick (field.rs:231-250)
+  int32_t apply_sources;
+  float src_ox, src_c, src_red, c_max;
+};
+
+// ---------------------------------------------------------------------------
+// Host context.
+struct marl_ctx {
+  marl_dims d{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  size_t nvox = 0, nfield = 0, occ_words = 0;
+  int xw = 0;  // occupancy words per row
+
+  float* field[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint32_t* occ = nullptr;
+  float* light = nullptr;
+  float* staging = nullptr;  // AoS staging (nfield floats), allocated on first use
+  uint8_t* occ_bytes = nullptr;
+  int tile = -1;             // stencil tile id; -1 = default
+
+  // cells
+  int64_t n_cells = 0, cap = 0;
+  marl_cell_state* cells[2] = {nullptr, nullptr};
+  marl_ruleset* rules[2] = {nullptr, nullptr};
+  int cbuf = 0;
+  int32_t* cell_at = nullptr;
+  // per-cell scratch (capacity cap)
+  uint8_t* nbmask = nullptr;   // 6-bit empty-neighbour mask
+  int32_t* level = nullptr;
+  int32_t* order = nullptr;    // cells sorted by level
+  uint8_t* events = nullptr;
+  int32_t* flag_scan = nullptr;  // scans (cap+1)
+  int32_t* flag_scan2 = nullptr;
+  int32_t* misc = nullptr;       // small counters
+  int32_t* level_count = nullptr;
+  uint32_t* claim = nullptr;     // per-voxel birth claims
+  marl_cell_state* birth_cells = nullptr;
+  marl_ruleset* birth_rules = nullptr;
+  int32_t* birth_flag = nullptr;
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+  int32_t* pinned = nullptr;   // small pinned host buffer
+  double* budget = nullptr;    // device FP64 partial sums
+  int kin_mode = MARL_KINETICS_PLAIN;
+
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+// helpers implemented in ctx.hip
+int marl_fail(marl_ctx* ctx, int code, const std::string& msg);
+int marl_hip_fail(marl_ctx* ctx, hipError_t e, const char* what);
+#define MARL_HIP(ctx, expr)                                   \
+  do {                                                        \
+    hipError_t _e = (expr);                                   \
+    if (_e != hipSuccess) return marl_hip_fail((ctx), _e, #expr); \
+  } while (0)
+
+StencilParams marl_stencil_params(const marl_sim_params* sim, int apply_sources);
+int marl_check_params(marl_ctx* ctx, const marl_sim_params* sim);
+
+// kernels launched from other translation units
+int marl_launch_substep(marl_ctx* ctx, const StencilParams& p, int tile);
+int marl_launch_boundary(marl_ctx* ctx, const marl_sim_params* sim);
+int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim);
+int marl_launch_occupancy_from_cells(marl_ctx* ctx);
+int marl_ensure_cell_capacity(marl_ctx* ctx, int64_t need);
+int marl_transpose(marl_ctx* ctx, bool to_soa, float* soa);
+int marl_pack_occ(marl_ctx* ctx);
+int marl_unpack_occ(marl_ctx* ctx);
+int marl_ensure_staging(marl_ctx* ctx);
+int marl_launch_copy_probe(marl_ctx* ctx);
+
+// stencil tiles: rows per thread and z-chunk
+struct TileShape { int ry, zc; };
+extern const TileShape kTiles[];
+extern const int kNumTiles;
+int marl_default_tile(const marl_dims& d);

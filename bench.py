@@ -59,6 +59,71 @@ def cpu_baseline(field, occ, dims, sim, substeps=40):
     }
 
 
+KINETICS_BYTES_PER_CELL = 1350.0  # state r/w + ruleset + neighbour gather/RMW at 12% occupancy
+
+
+def kinetics_bench(pkg, ticks, with_cpu):
+    """Plain per-cell kinetics at scale: 256x256x128, 1,000,000 cells, full ticks
+    (boundary, 10 substeps, light, ordered cell pass, fate with mutation,
+    compaction).  Reports Mcell-updates/s of whole ticks and of the per-level
+    kinetics launches alone (hipEvents around them)."""
+    from marl_ochre_amd import scenarios
+
+    dims = (256, 256, 128)
+    log("generating 1M-cell kinetics workload")
+    field, cells, rules, sim = scenarios.kinetics_at_scale(dims, 1_000_000, seed=0)
+    g = pkg.MarlGpu(dims)
+    g.upload_field_aos(field)
+    g.upload_cells(cells, rules)
+    g.tick(sim, 0, 2)  # warm-up ticks (part of the trajectory)
+    g.sync()
+    per = []
+    t0 = time.perf_counter()
+    for t in range(ticks):
+        n_before = g.cell_count()
+        st = g.tick(sim, 2 + t, 1)
+        per.append((n_before, st))
+    wall = time.perf_counter() - t0
+    g.close()
+    cells_total = sum(n for n, _ in per)
+    kin_ms = sum(st["ms_kinetics"] for _, st in per)
+    res = {
+        "workload": "kinetics at scale: 256x256x128, 1,000,000 cells placed uniformly, 16 random reactions/cell "
+                    "(v_max, k_m log-uniform [1e-3,1], p=0.2 light-catalysed), full ticks with mutation",
+        "ticks": ticks,
+        "cells_first": per[0][0],
+        "cells_last": per[-1][1]["n_cells"],
+        "Mcell_updates_per_s_full_tick": round(cells_total / wall / 1e6, 3),
+        "Mcell_updates_per_s_kinetics": round(cells_total / (kin_ms * 1e-3) / 1e6, 3),
+        "ms_per_tick": round(wall * 1e3 / ticks, 3),
+        "ms_breakdown_mean": {k: round(sum(st[k] for _, st in per) / ticks, 3)
+                              for k in ("ms_field", "ms_schedule", "ms_kinetics", "ms_fate")},
+        "levels_mean": round(sum(st["levels"] for _, st in per) / ticks, 2),
+        "roofline_kinetics": {
+            "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "achieved": round(KINETICS_BYTES_PER_CELL * cells_total / (kin_ms * 1e-3) / 1e9, 1),
+            "bytes_per_cell": KINETICS_BYTES_PER_CELL,
+        },
+    }
+    res["roofline_kinetics"]["frac"] = round(res["roofline_kinetics"]["achieved"] / HBM_PEAK_GBS, 4)
+    if with_cpu:
+        from oracle.binding import Oracle
+
+        orc = Oracle()
+        nv = dims[0] * dims[1] * dims[2]
+        sample = 200_000
+        light = np.ones(nv, np.float32)
+        t0 = time.perf_counter()
+        orc.cell_pass(dims, sim, field, light, cells[:sample], rules[:sample])
+        dt = time.perf_counter() - t0
+        res["cpu_baseline_cell_pass"] = {
+            "value": round(sample / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"sequential cell pass (env gather, CellState::tick, delta scatter) over the first {sample} "
+                      f"cells of the 1M-cell workload, one thread as in the reference ({dt:.2f} s)"}
+    log("kinetics:", json.dumps(res))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +131,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--kinetics-ticks", type=int, default=10,
+                    help="full ticks of the 1M-cell kinetics workload (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -161,6 +228,8 @@ def main():
         "light_us_per_tick": round(light_ms * 1e3 / args.steps, 2),
     }
     g.close()
+    if args.kinetics_ticks > 0:
+        out["kinetics"] = kinetics_bench(pkg, args.kinetics_ticks, not args.no_cpu)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(field, occ, dims, sim)

@@ -124,6 +124,10 @@ typedef struct {
   int64_t births_added;   /* births that survived the duplicate-position check (lib.rs:270-277) */
   int32_t levels;         /* depth of the conflict schedule of the cell pass */
   int32_t _pad;
+  float ms_field;         /* occupancy rebuild, sources, substeps, light (hipEvents) */
+  float ms_schedule;      /* empty-neighbour masks and the conflict levels */
+  float ms_kinetics;      /* the per-level kinetics launches */
+  float ms_fate;          /* divisions, mutation, deaths, compaction */
 } marl_tick_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -170,6 +174,7 @@ int marl_diffuse_substeps(marl_ctx* ctx, const marl_sim_params* sim, int32_t n_s
 /* ≙ LightField::update (crates/marl-field/src/light.rs:29-59). */
 int marl_light_update(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_download_light(marl_ctx* ctx, float* light, size_t n_voxels);
+int marl_upload_light(marl_ctx* ctx, const float* light, size_t n_voxels);
 
 /* ---- resident cells and the full tick ------------------------------------ */
 
@@ -190,6 +195,18 @@ int marl_cell_pass(marl_ctx* ctx, const marl_sim_params* sim, uint8_t* events, s
  * voxels, out[24..39] cell internals (no reference counterpart: the reference
  * keeps no mass budget, docs/INFO.md:276). */
 int marl_mass_budget(marl_ctx* ctx, double* out, size_t n_out);
+
+/* ---- setup ----------------------------------------------------------------- */
+
+/* ≙ seed_cells (crates/marl-sim/src/seeding.rs:28-58): up to count*8 attempts
+ * drawing x in [margin, X-margin), y in [margin, Y-margin), z in [z_lo,
+ * min(z_hi, Z)), skipping taken voxels, then a u64 lineage per placed cell —
+ * from the Philox stream (seed, 1<<32 | band).  `cell_at` (host, X*Y*Z int32,
+ * -1 = free) is read and updated; placed cells get indices index_base + k.
+ * Host-only: needs no device. */
+int marl_seed_positions(const marl_dims* dims, uint64_t seed, uint64_t band, int64_t count, int32_t margin,
+                        int32_t z_lo, int32_t z_hi, int32_t* cell_at, uint16_t* pos_out, uint64_t* lineage_out,
+                        int64_t index_base, int64_t* placed);
 
 /* ---- timing helpers (hipEvents on the context's stream) ------------------ */
 

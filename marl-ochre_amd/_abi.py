@@ -74,6 +74,10 @@ class marl_tick_stats(ctypes.Structure):
         ("births_added", ctypes.c_int64),
         ("levels", ctypes.c_int32),
         ("_pad", ctypes.c_int32),
+        ("ms_field", ctypes.c_float),
+        ("ms_schedule", ctypes.c_float),
+        ("ms_kinetics", ctypes.c_float),
+        ("ms_fate", ctypes.c_float),
     ]
 
 
@@ -158,6 +162,7 @@ SIGNATURES = {
     "marl_diffuse_substeps": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I32]),
     "marl_light_update": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params)]),
     "marl_download_light": (ctypes.c_int, [_P, _P, _SZ]),
+    "marl_upload_light": (ctypes.c_int, [_P, _P, _SZ]),
     "marl_upload_cells": (ctypes.c_int, [_P, _P, _P, _SZ]),
     "marl_cell_count": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
     "marl_download_cells": (ctypes.c_int, [_P, _P, _P, _SZ, ctypes.POINTER(_I64)]),
@@ -172,6 +177,8 @@ SIGNATURES = {
     "marl_stencil_tile_count": (ctypes.c_int, []),
     "marl_stencil_tile_shape": (ctypes.c_int, [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "marl_copy_probe": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]),
+    "marl_seed_positions": (ctypes.c_int, [ctypes.POINTER(marl_dims), ctypes.c_uint64, ctypes.c_uint64, _I64, _I32,
+                                           _I32, _I32, _P, _P, _P, _I64, ctypes.POINTER(_I64)]),
     "marl_sync": (ctypes.c_int, [_P]),
 }
 

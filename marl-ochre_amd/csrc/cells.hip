@@ -1,24 +1,1051 @@
-// cells.hip — resident cell path (placeholder entry points until the kernels land).
+// cells.hip — the GPU-resident cell path and the full tick.
+//
+// Reference: crates/marl-sim/src/lib.rs:160-277 (tick loop), spatial.rs:8-154
+// (neighbour helpers, division placement), crates/marl-cell/src/cell.rs:193-524
+// (CellState::tick, Ruleset::mutate).
+//
+// Exact sequential semantics of the cell pass (lib.rs:206-217: cells in Vec
+// order, each one's deltas land in the field before the next reads it) on a
+// data-parallel machine: two cells interact only through an empty voxel that
+// is a face neighbour of both, i.e. when they are L1 distance 2 apart and a
+// shared face neighbour is free.  Occupancy is fixed during the pass, so the
+// conflict graph is fixed; level(j) = 1 + max level(i) over conflicting i < j.
+// Running one launch per level, cells within a level touch disjoint voxels,
+// and every voxel sees its writers in index order — the same result as the
+// sequential loop, bit for bit.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include "marl_internal.h"
 
+namespace {
+
+constexpr int kMaxLevels = 4096;
+constexpr int kMaxPreds = 18;
+
+struct CellParams {
+  float dt, epsilon, c_max, lambda_maintenance, hard_death_floor, reaction_maintenance;
+  float base_division_prep, prep_maintenance_multiplier, rush_penalty_rate, active_reaction_threshold;
+  float mutation_stddev, structural_mutation_rate_mult, meta_mutation_rate, meta_lo, meta_hi, hill_lo, hill_hi;
+  int32_t div_dist;
+  int32_t X, Y, Z;
+  uint64_t seed, stream;
+};
+
+CellParams cell_params(const marl_ctx* ctx, const marl_sim_params* s, uint64_t tick) {
+  CellParams p;
+  p.dt = s->dt;
+  p.epsilon = s->epsilon;
+  p.c_max = s->c_max;
+  p.lambda_maintenance = s->lambda_maintenance;
+  p.hard_death_floor = s->hard_death_floor;
+  p.reaction_maintenance = s->reaction_maintenance;
+  p.base_division_prep = s->base_division_prep;
+  p.prep_maintenance_multiplier = s->prep_maintenance_multiplier;
+  p.rush_penalty_rate = s->rush_penalty_rate;
+  p.active_reaction_threshold = s->active_reaction_threshold;
+  p.mutation_stddev = s->mutation_stddev;
+  p.structural_mutation_rate_mult = s->structural_mutation_rate_mult;
+  p.meta_mutation_rate = s->meta_mutation_rate;
+  p.meta_lo = s->meta_mutation_clamp_low;
+  p.meta_hi = s->meta_mutation_clamp_high;
+  p.hill_lo = s->hill_exponent_clamp_low;
+  p.hill_hi = s->hill_exponent_clamp_high;
+  p.div_dist = s->division_neighbor_distance;
+  p.X = ctx->d.x;
+  p.Y = ctx->d.y;
+  p.Z = ctx->d.z;
+  p.seed = s->rng_seed;
+  p.stream = (MARL_TAG_DIVISION << 32) | (tick & 0xffffffffull);
+  return p;
+}
+
+__host__ __device__ inline float clamp_rs(float x, float lo, float hi) {  // Rust f32::clamp
+  if (x < lo) x = lo;
+  if (x > hi) x = hi;
+  return x;
+}
+
+__device__ inline uint16_t f32_as_u16(float f) {  // Rust `as u16`: truncate, saturate, NaN -> 0
+  if (!(f > 0.0f)) return 0;
+  if (f >= 65535.0f) return 65535;
+  return (uint16_t)f;
+}
+
+// FACE_OFFSETS (spatial.rs:8-15): +x, -x, +y, -y, +z, -z
+__constant__ int8_t kFace[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+
+__device__ inline bool inb(int x, int y, int z, const CellParams& p) {
+  return x >= 0 && x < p.X && y >= 0 && y < p.Y && z >= 0 && z < p.Z;
+}
+__device__ inline int64_t vox(int x, int y, int z, const CellParams& p) {
+  return ((int64_t)z * p.Y + y) * p.X + x;
+}
+
+// ---------------------------------------------------------------------------
+// utility kernels
+__global__ void k_fill_i32(int32_t* a, int32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) a[i] = v;
+}
+
+// K3: occupancy bits and the voxel -> cell index map (≙ cell_map + the
+// occupancy Vec<bool> of lib.rs:174-179).
+__global__ void k_scatter_cells(const marl_cell_state* __restrict__ cells, int n, int32_t* __restrict__ cell_at,
+                                uint32_t* __restrict__ occ, CellParams p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint16_t* ps = cells[i].pos;
+  const int64_t v = vox(ps[0], ps[1], ps[2], p);
+  cell_at[v] = i;
+  atomicOr(&occ[v >> 5], 1u << (v & 31));
+}
+
+// K4: empty-neighbour mask (spatial.rs:18-43) and the conflict predecessors.
+// Conflicting partner at L1 distance 2 with a lower index that shares at least
+// one free face neighbour.
+__global__ void k_cell_prep(const marl_cell_state* __restrict__ cells, int n, const int32_t* __restrict__ cell_at,
+                            uint8_t* __restrict__ nbmask, int32_t* __restrict__ preds, uint8_t* __restrict__ npred,
+                            int32_t* __restrict__ level, CellParams p) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint16_t* ps = cells[j].pos;
+  const int x = ps[0], y = ps[1], z = ps[2];
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int nx = x + kFace[k][0], ny = y + kFace[k][1], nz = z + kFace[k][2];
+    if (inb(nx, ny, nz, p) && cell_at[vox(nx, ny, nz, p)] < 0) m |= 1u << k;
+  }
+  nbmask[j] = (uint8_t)m;
+  int np = 0;
+  int32_t* pj = preds + (size_t)j * kMaxPreds;
+  // +-2 along one axis: shared voxel is the midpoint (a face neighbour of j)
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (!((m >> k) & 1u)) continue;
+    const int ox = x + 2 * kFace[k][0], oy = y + 2 * kFace[k][1], oz = z + 2 * kFace[k][2];
+    if (!inb(ox, oy, oz, p)) continue;
+    const int32_t i = cell_at[vox(ox, oy, oz, p)];
+    if (i >= 0 && i < j) pj[np++] = i;
+  }
+  // one step along two different axes: shared voxels are the two corners
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+#pragma unroll
+    for (int b = a + 1; b < 6; ++b) {
+      if ((a >> 1) == (b >> 1)) continue;  // same axis
+      if (!(((m >> a) | (m >> b)) & 1u)) continue;  // neither corner free
+      const int ox = x + kFace[a][0] + kFace[b][0], oy = y + kFace[a][1] + kFace[b][1],
+                oz = z + kFace[a][2] + kFace[b][2];
+      if (!inb(ox, oy, oz, p)) continue;
+      const int32_t i = cell_at[vox(ox, oy, oz, p)];
+      if (i >= 0 && i < j) pj[np++] = i;
+    }
+  }
+  npred[j] = (uint8_t)np;
+  level[j] = 0;
+}
+
+// Longest-path relaxation on the conflict DAG; monotone, converges to the
+// exact level in (depth + 1) sweeps or fewer.
+__global__ void k_level_relax(int n, const int32_t* __restrict__ preds, const uint8_t* __restrict__ npred,
+                              int32_t* level, int32_t* changed) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int np = npred[j];
+  if (np == 0) return;
+  const int32_t* pj = preds + (size_t)j * kMaxPreds;
+  int L = 0;
+  for (int k = 0; k < np; ++k) L = max(L, level[pj[k]] + 1);
+  if (L > level[j]) {
+    level[j] = L;
+    *changed = 1;
+  }
+}
+
+__global__ void k_level_hist(int n, const int32_t* __restrict__ level, int32_t* hist /* [0]=max, [1+L]=count */) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int L = level[j];
+  atomicMax(&hist[0], L);
+  if (L < kMaxLevels) atomicAdd(&hist[1 + L], 1);
+}
+
+__global__ void k_level_scatter(int n, const int32_t* __restrict__ level, int32_t* cursor, int32_t* __restrict__ order) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int pos = atomicAdd(&cursor[level[j]], 1);
+  order[pos] = j;
+}
+
+// ---------------------------------------------------------------------------
+// K5/K6: one level of the cell pass.  Per cell: mean environment of the free
+// face neighbours (spatial.rs:56-77), CellState::tick (cell.rs:193-369), then
+// split deltas with the max(0) clamp into those neighbours (spatial.rs:87-110,
+// field.rs:57-62).  Dynamically indexed per-cell arrays live in LDS (one
+// private slot per thread, odd strides), so nothing goes to scratch.
+constexpr int kKinBlock = 128;
+
+__global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restrict__ order, int count,
+                                                      marl_cell_state* __restrict__ cells,
+                                                      const marl_ruleset* __restrict__ rules,
+                                                      float* __restrict__ field, const float* __restrict__ light,
+                                                      const uint8_t* __restrict__ nbmask,
+                                                      uint8_t* __restrict__ events, CellParams p) {
+  __shared__ float sI[kKinBlock * 17];
+  __shared__ float sE[kKinBlock * 13];
+  __shared__ float sD[kKinBlock * 13];
+  const int t = blockIdx.x * kKinBlock + threadIdx.x;
+  if (t >= count) return;
+  const int i = order[t];
+  float* I = sI + threadIdx.x * 17;
+  float* E = sE + threadIdx.x * 13;
+  float* D = sD + threadIdx.x * 13;
+  marl_cell_state c = cells[i];
+  const marl_ruleset& r = rules[i];
+  const float dt = p.dt;
+#pragma unroll
+  for (int k = 0; k < MARL_M_INT; ++k) I[k] = c.internal[k];
+
+  const int x = c.pos[0], y = c.pos[1], z = c.pos[2];
+  const size_t XY = (size_t)p.X * p.Y, P = XY * p.Z;
+  const size_t v = ((size_t)z * p.Y + y) * p.X + x;
+  const uint32_t m = nbmask[i];
+  const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
+  size_t nv[6];
+  int n = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    if ((m >> k) & 1u) nv[n++] = (size_t)((int64_t)v + off[k]);
+
+  // environment (sum in face order, then divide)
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) {
+    float acc = 0.0f;
+    for (int k = 0; k < n; ++k) acc += field[(size_t)s * P + nv[k]];
+    E[s] = n ? acc / (float)n : 0.0f;
+    D[s] = 0.0f;
+  }
+  const float lt = light[v];
+
+  // transport (cell.rs:223-238)
+#pragma unroll
+  for (int q = 0; q < MARL_S_TRANSPORTERS; ++q) {
+    const marl_transport tp = r.transport[q];
+    const int e = tp.ext_species, k = tp.int_species;
+    if (e >= MARL_S_EXT || k >= MARL_M_INT) continue;
+    const float ex = E[e];
+    const float ik = I[k];
+    const float up = tp.uptake_rate * ex / (1.0f + ex);
+    const float sec = tp.secrete_rate * ik / (1.0f + ik);
+    I[k] = clamp_rs(ik + (up - sec) * dt, 0.0f, p.c_max);
+    D[e] += (sec - up) * dt;
+  }
+  I[MARL_M_INT - 1] = lt;  // :243
+
+  // reactions (cell.rs:246-285)
+  int active = 0;
+#pragma unroll 4
+  for (int q = 0; q < MARL_R_MAX; ++q) {
+    const marl_reaction rx = r.reactions[q];
+    if (fabsf(rx.v_max) > p.active_reaction_threshold && rx.substrate != rx.product) ++active;
+    if (fabsf(rx.v_max) < p.active_reaction_threshold) continue;
+    if (rx.substrate == rx.product) continue;
+    const int si = rx.substrate, pi = rx.product, ci = rx.catalyst;
+    if (si >= MARL_M_INT || pi >= MARL_M_INT || ci >= MARL_M_INT) continue;
+    const float s = I[si];
+    const float cc = I[ci];
+    const float st = s / (rx.k_m + s + 1.1920929e-07f);
+    const float ct = p.epsilon + cc / (rx.k_cat + cc + 1.1920929e-07f);
+    float rate = rx.v_max * st * ct;
+    if (rx.cofactor != 0xFF) {
+      const int fi = rx.cofactor;
+      if (fi < MARL_M_INT) {
+        const float cof = I[fi];
+        rate *= cof / (1.0f + cof);
+        const float used = fminf(0.5f * rate * dt, I[fi]);
+        I[fi] -= used;
+      }
+    }
+    const float flux = fminf(rate * dt, I[si]);
+    I[si] -= flux;
+    I[pi] = fminf(I[pi] + flux, p.c_max);
+  }
+
+  // maintenance (cell.rs:291-311)
+  float pm = 1.0f;
+  if (c.prep_remaining > 0) {
+    const float evolved = fmaxf(r.division_prep_ticks, 1.0f);
+    const float rush = fmaxf(p.base_division_prep - evolved, 0.0f);
+    pm = p.prep_maintenance_multiplier + rush * p.rush_penalty_rate;
+  }
+  float e0 = I[0];
+  e0 *= 1.0f - p.lambda_maintenance * pm * dt;
+  e0 = fmaxf(e0 - (float)active * p.reaction_maintenance * dt, 0.0f);
+  I[0] = e0;
+
+  // effectors (cell.rs:314-329)
+  if (!c.quiescent) {
+#pragma unroll
+    for (int q = 0; q < MARL_S_EFFECTORS; ++q) {
+      const marl_effector ef = r.effectors[q];
+      const int k = ef.int_species, e = ef.ext_species;
+      if (k >= MARL_M_INT || e >= MARL_S_EXT) continue;
+      const float ik = I[k];
+      if (ik > ef.threshold) {
+        const float amount = ef.rate * ik / (1.0f + ik) * dt;
+        I[k] = fmaxf(ik - amount, 0.0f);
+        D[e] += amount;
+      }
+    }
+  }
+
+  // fate (cell.rs:332-366)
+  const float energy = I[0];
+  c.age += 1;
+  const float eff_death = fmaxf(r.death_energy, p.hard_death_floor);
+  uint8_t ev;
+  if (energy < eff_death) {
+    c.prep_remaining = 0;
+    ev = MARL_EVENT_DEATH;
+  } else if (c.prep_remaining > 0) {
+    c.prep_remaining -= 1;
+    ev = c.prep_remaining == 0 ? MARL_EVENT_DIVISION : MARL_EVENT_NONE;
+  } else if (energy > r.division_energy) {
+    c.prep_remaining = f32_as_u16(fmaxf(r.division_prep_ticks, 1.0f));
+    c.quiescent = 0;
+    ev = MARL_EVENT_NONE;
+  } else if (energy < r.quiescence_energy) {
+    c.quiescent = 1;
+    ev = MARL_EVENT_QUIESCENCE;
+  } else {
+    c.quiescent = 0;
+    ev = MARL_EVENT_NONE;
+  }
+#pragma unroll
+  for (int k = 0; k < MARL_M_INT; ++k) c.internal[k] = I[k];
+  cells[i] = c;
+  events[i] = ev;
+
+  // deltas to the free neighbours (spatial.rs:87-110)
+  if (n > 0) {
+    const float nf = (float)n;
+#pragma unroll
+    for (int s = 0; s < MARL_S_EXT; ++s) {
+      const float split = D[s] / nf;
+      for (int k = 0; k < n; ++k) {
+        float* fp = field + (size_t)s * P + nv[k];
+        *fp = fmaxf(*fp + split, 0.0f);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K7: fate.  Division targets (find_empty_neighbor, spatial.rs:115-154) in the
+// pre-birth map; one RNG stream per dividing cell keyed (seed, tick, index).
+__global__ void k_div_target(int n, const marl_cell_state* __restrict__ cells, const uint8_t* __restrict__ events,
+                             const int32_t* __restrict__ cell_at, int32_t* __restrict__ target,
+                             int32_t* __restrict__ bflag, CellParams p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t tgt = -1;
+  if (events[i] == MARL_EVENT_DIVISION) {
+    const uint16_t* ps = cells[i].pos;
+    const int x = ps[0], y = ps[1], z = ps[2];
+    int32_t cand[6];
+    int nc = 0;
+    const int dd = p.div_dist;
+    for (int k = 0; k < 6; ++k) {
+      const int nx = x + kFace[k][0] * dd, ny = y + kFace[k][1] * dd, nz = z + kFace[k][2] * dd;
+      if (inb(nx, ny, nz, p)) {
+        const int64_t w = vox(nx, ny, nz, p);
+        if (cell_at[w] < 0) cand[nc++] = (int32_t)w;
+      }
+    }
+    if (nc == 0) {
+      for (int k = 0; k < 6; ++k) {
+        const int nx = x + kFace[k][0], ny = y + kFace[k][1], nz = z + kFace[k][2];
+        if (inb(nx, ny, nz, p)) {
+          const int64_t w = vox(nx, ny, nz, p);
+          if (cell_at[w] < 0) cand[nc++] = (int32_t)w;
+        }
+      }
+    }
+    if (nc > 0) {
+      MarlRng rng;
+      rng.init(p.seed, p.stream, (uint64_t)i);
+      tgt = cand[rng.range((uint32_t)nc)];
+    }
+  }
+  target[i] = tgt;
+  bflag[i] = tgt >= 0 ? 1 : 0;
+}
+
+// Ruleset::mutate (cell.rs:406-524) in its draw order, on the daughter's copy.
+__device__ inline void maybe_mutate(float& v, float rate, float sd, MarlRng& rng) {
+  if (rng.u01() < rate) {
+    v += rng.normal(sd);
+    v = fmaxf(v, 0.0f);
+  }
+}
+
+__device__ void mutate_ruleset(marl_ruleset* r, const CellParams& p, MarlRng& rng) {
+  const float rate = r->mutation_rate;
+  const float sd = p.mutation_stddev;
+  const float sr = rate * p.structural_mutation_rate_mult;
+  for (int q = 0; q < MARL_S_RECEPTORS; ++q) {
+    marl_receptor* rc = &r->receptors[q];
+    maybe_mutate(rc->k_half, rate, sd, rng);
+    maybe_mutate(rc->n_hill, rate, sd, rng);
+    rc->n_hill = clamp_rs(rc->n_hill, p.hill_lo, p.hill_hi);
+    maybe_mutate(rc->gain, rate, sd, rng);
+  }
+  for (int q = 0; q < MARL_S_TRANSPORTERS; ++q) {
+    marl_transport* t = &r->transport[q];
+    maybe_mutate(t->uptake_rate, rate, sd, rng);
+    maybe_mutate(t->secrete_rate, rate, sd, rng);
+    if (rng.u01() < sr) t->ext_species = (uint8_t)rng.range(MARL_S_EXT);
+    if (rng.u01() < sr) t->int_species = (uint8_t)rng.range(MARL_M_INT);
+  }
+  int active[MARL_R_MAX];
+  int na = 0;
+  for (int q = 0; q < MARL_R_MAX; ++q)
+    if (fabsf(r->reactions[q].v_max) > p.active_reaction_threshold) active[na++] = q;
+  for (int q = 0; q < MARL_R_MAX; ++q) {
+    marl_reaction* rx = &r->reactions[q];
+    maybe_mutate(rx->k_m, rate, sd, rng);
+    maybe_mutate(rx->v_max, rate, sd, rng);
+    maybe_mutate(rx->k_cat, rate, sd, rng);
+    for (int fld = 0; fld < 3; ++fld) {
+      if (rng.u01() < sr) {
+        const uint32_t idx = rng.range((uint32_t)(na > 1 ? na : 1));
+        if ((int)idx < na) {
+          const int donor = active[idx];
+          if (donor != q) {
+            if (fld == 0) rx->substrate = r->reactions[donor].substrate;
+            else if (fld == 1) rx->product = r->reactions[donor].product;
+            else rx->catalyst = r->reactions[donor].catalyst;
+          }
+        } else {
+          const uint8_t vv = (uint8_t)rng.range(MARL_M_INT);
+          if (fld == 0) rx->substrate = vv;
+          else if (fld == 1) rx->product = vv;
+          else rx->catalyst = vv;
+        }
+      }
+    }
+  }
+  for (int q = 0; q < MARL_S_EFFECTORS; ++q) {
+    maybe_mutate(r->effectors[q].threshold, rate, sd, rng);
+    maybe_mutate(r->effectors[q].rate, rate, sd, rng);
+  }
+  maybe_mutate(r->division_energy, rate, sd, rng);
+  maybe_mutate(r->death_energy, rate, sd, rng);
+  maybe_mutate(r->quiescence_energy, rate, sd, rng);
+  maybe_mutate(r->division_prep_ticks, rate, sd, rng);
+  r->division_prep_ticks = fmaxf(r->division_prep_ticks, 1.0f);
+  maybe_mutate(r->hgt_propensity, rate, sd, rng);
+  if (rng.u01() < p.meta_mutation_rate) {
+    r->mutation_rate += rng.normal(sd) * p.meta_mutation_rate;
+    r->mutation_rate = clamp_rs(r->mutation_rate, p.meta_lo, p.meta_hi);
+  }
+}
+
+// Births (lib.rs:226-243): daughter = clone of the ticked parent at the target,
+// age 0, prep 0, new lineage; both halve all 16 internals; daughter mutates.
+// Birth slot = rank of the division in event order; claims keep the first
+// birth per voxel (lib.rs:270-277).
+__global__ void k_births(int n, marl_cell_state* __restrict__ cells, const marl_ruleset* __restrict__ rules,
+                         const int32_t* __restrict__ target, const int32_t* __restrict__ brank,
+                         marl_cell_state* __restrict__ bcells, marl_ruleset* __restrict__ brules,
+                         int32_t* __restrict__ btarget, uint32_t* __restrict__ claim, CellParams p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t tgt = target[i];
+  if (tgt < 0) return;
+  const int32_t rk = brank[i];
+  MarlRng rng;
+  rng.init(p.seed, p.stream, (uint64_t)i, 1);  // draw 0 picked the target
+  marl_cell_state d = cells[i];
+  const int64_t XY = (int64_t)p.X * p.Y;
+  d.pos[0] = (uint16_t)(tgt % p.X);
+  d.pos[1] = (uint16_t)((tgt / p.X) % p.Y);
+  d.pos[2] = (uint16_t)(tgt / XY);
+  d.age = 0;
+  d.prep_remaining = 0;
+  d.lineage_id = rng.u64();
+  marl_cell_state par = cells[i];
+#pragma unroll
+  for (int k = 0; k < MARL_M_INT; ++k) {
+    d.internal[k] *= 0.5f;
+    par.internal[k] *= 0.5f;
+  }
+  cells[i] = par;
+  bcells[rk] = d;
+  marl_ruleset* dr = &brules[rk];
+  *dr = rules[i];
+  mutate_ruleset(dr, p, rng);
+  btarget[rk] = tgt;
+  atomicMin(&claim[tgt], (uint32_t)rk);
+}
+
+__global__ void k_birth_winners(int nb, const int32_t* __restrict__ btarget, const uint32_t* __restrict__ claim,
+                                int32_t* __restrict__ win) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nb) return;
+  win[r] = claim[btarget[r]] == (uint32_t)r ? 1 : 0;
+}
+
+__global__ void k_claim_reset(int nb, const int32_t* __restrict__ btarget, uint32_t* __restrict__ claim) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nb) return;
+  claim[btarget[r]] = 0xffffffffu;
+}
+
+__global__ void k_death_flags(int n, const uint8_t* __restrict__ events, int32_t* __restrict__ dflag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dflag[i] = events[i] == MARL_EVENT_DEATH ? 1 : 0;
+}
+
+// K8: swap_remove compaction (lib.rs:257-267) in closed form.  With P the
+// exclusive prefix count of deaths, level j covers [lo_j, n) with head end
+// m_j = n - (P[n] - P[lo_j]) and lo_{j+1} = m_j.  Output slot s < m_0 holds
+// s itself if it survives; otherwise the q-th smallest head hole of level j
+// (q = P[s] - P[lo_j]) receives what lands in slot m_j + q of level j+1.
+__global__ void k_compact_gather(int n, int m0, const uint8_t* __restrict__ events, const int32_t* __restrict__ P,
+                                 const marl_cell_state* __restrict__ src_c, const marl_ruleset* __restrict__ src_r,
+                                 marl_cell_state* __restrict__ dst_c, marl_ruleset* __restrict__ dst_r) {
+  const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s0 >= m0) return;
+  const int total = P[n];
+  int s = s0, lo = 0, m = m0;
+  while (events[s] == MARL_EVENT_DEATH) {
+    const int q = P[s] - P[lo];
+    const int nlo = m;
+    s = m + q;
+    lo = nlo;
+    m = n - (total - P[lo]);
+  }
+  dst_c[s0] = src_c[s];
+  dst_r[s0] = src_r[s];
+}
+
+__global__ void k_append_births(int nb, int base, const int32_t* __restrict__ win, const int32_t* __restrict__ wrank,
+                                const marl_cell_state* __restrict__ bc, const marl_ruleset* __restrict__ br,
+                                marl_cell_state* __restrict__ dst_c, marl_ruleset* __restrict__ dst_r) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nb || !win[r]) return;
+  dst_c[base + wrank[r]] = bc[r];
+  dst_r[base + wrank[r]] = br[r];
+}
+
+// K9: FP64 mass budget, deterministic two-pass reduction.
+constexpr int kBudget = 40;
+__global__ void __launch_bounds__(256) k_budget_field(const float* __restrict__ f, const uint32_t* __restrict__ occ,
+                                                      size_t nvox, double* __restrict__ partial) {
+  double acc[24];
+#pragma unroll
+  for (int k = 0; k < 24; ++k) acc[k] = 0.0;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
+    const int o = (occ[v >> 5] >> (v & 31)) & 1u;
+#pragma unroll
+    for (int s = 0; s < MARL_S_EXT; ++s) {
+      const double x = f[(size_t)s * nvox + v];
+      acc[s] += o ? 0.0 : x;
+      acc[12 + s] += o ? x : 0.0;
+    }
+  }
+  __shared__ double red[256];
+  for (int k = 0; k < 24; ++k) {
+    red[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * kBudget + k] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) k_budget_cells(const marl_cell_state* __restrict__ c, int n,
+                                                      double* __restrict__ partial) {
+  double acc[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] += (double)c[i].internal[k];
+  __shared__ double red[256];
+  for (int k = 0; k < 16; ++k) {
+    red[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * kBudget + 24 + k] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_budget_final(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= kBudget) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * kBudget + k];
+  out[k] = s;
+}
+
+constexpr int kBudgetBlocks = 512;
+
+inline unsigned grid1(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// host orchestration
+
+template <typename T>
+int dev_alloc(marl_ctx* ctx, T** p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  MARL_HIP(ctx, hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+  return MARL_OK;
+}
+
+int scan_i32(marl_ctx* ctx, const int32_t* in, int32_t* out, int n) {
+  size_t need = 0;
+  MARL_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, n, ctx->stream));
+  if (need > ctx->scan_tmp_bytes) {
+    if (ctx->scan_tmp) (void)hipFree(ctx->scan_tmp);
+    ctx->scan_tmp = nullptr;
+    MARL_HIP(ctx, hipMalloc(&ctx->scan_tmp, need * 2));
+    ctx->scan_tmp_bytes = need * 2;
+  }
+  size_t have = ctx->scan_tmp_bytes;
+  MARL_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp, have, in, out, n, ctx->stream));
+  return MARL_OK;
+}
+
+int fill_i32(marl_ctx* ctx, int32_t* a, int32_t v, size_t n) {
+  if (n == 0) return MARL_OK;
+  const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_i32, dim3(g), dim3(256), 0, ctx->stream, a, v, n);
+  MARL_HIP(ctx, hipGetLastError());
+  return MARL_OK;
+}
+
+int d2h_sync(marl_ctx* ctx, void* host, const void* dev, size_t bytes) {
+  MARL_HIP(ctx, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MARL_OK;
+}
+
+// Per-context scratch that is sized by voxel count (allocated once).
+int ensure_voxel_buffers(marl_ctx* ctx) {
+  if (!ctx->cell_at) {
+    if (int rc = dev_alloc(ctx, &ctx->cell_at, ctx->nvox)) return rc;
+    if (int rc = fill_i32(ctx, ctx->cell_at, -1, ctx->nvox)) return rc;
+  }
+  if (!ctx->claim) {
+    if (int rc = dev_alloc(ctx, &ctx->claim, ctx->nvox)) return rc;
+    if (int rc = fill_i32(ctx, reinterpret_cast<int32_t*>(ctx->claim), -1, ctx->nvox)) return rc;
+  }
+  if (!ctx->level_count) {
+    if (int rc = dev_alloc(ctx, &ctx->level_count, 2 * (kMaxLevels + 1))) return rc;
+  }
+  if (!ctx->misc) {
+    if (int rc = dev_alloc(ctx, &ctx->misc, 64)) return rc;
+  }
+  return MARL_OK;
+}
+
+}  // namespace
+
+// Grows every per-cell buffer to hold `need` cells, preserving the live ones.
+int marl_ensure_cell_capacity(marl_ctx* ctx, int64_t need) {
+  if (need <= ctx->cap) return MARL_OK;
+  int64_t cap = std::max<int64_t>(need, 4096);
+  cap = std::min<int64_t>(std::max<int64_t>(cap, ctx->cap * 2), (int64_t)ctx->nvox * 2 + 16);
+  cap = std::max<int64_t>(cap, need);
+  marl_cell_state* nc[2] = {nullptr, nullptr};
+  marl_ruleset* nr[2] = {nullptr, nullptr};
+  for (int b = 0; b < 2; ++b) {
+    MARL_HIP(ctx, hipMalloc(&nc[b], cap * sizeof(marl_cell_state)));
+    MARL_HIP(ctx, hipMalloc(&nr[b], cap * sizeof(marl_ruleset)));
+  }
+  if (ctx->n_cells > 0) {
+    MARL_HIP(ctx, hipMemcpyAsync(nc[0], ctx->cells[ctx->cbuf], ctx->n_cells * sizeof(marl_cell_state),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+    MARL_HIP(ctx, hipMemcpyAsync(nr[0], ctx->rules[ctx->cbuf], ctx->n_cells * sizeof(marl_ruleset),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int b = 0; b < 2; ++b) {
+    if (ctx->cells[b]) (void)hipFree(ctx->cells[b]);
+    if (ctx->rules[b]) (void)hipFree(ctx->rules[b]);
+    ctx->cells[b] = nc[b];
+    ctx->rules[b] = nr[b];
+  }
+  ctx->cbuf = 0;
+  if (int rc = dev_alloc(ctx, &ctx->nbmask, cap)) return rc;
+  if (int rc = dev_alloc(ctx, &ctx->level, cap)) return rc;
+  if (int rc = dev_alloc(ctx, &ctx->order, cap)) return rc;
+  if (int rc = dev_alloc(ctx, &ctx->events, cap)) return rc;
+  if (int rc = dev_alloc(ctx, &ctx->flag_scan, 4 * (cap + 1))) return rc;   // target, bflag, brank, dflag/P
+  if (int rc = dev_alloc(ctx, &ctx->flag_scan2, 3 * (cap + 1))) return rc;  // btarget, win, wrank
+  if (int rc = dev_alloc(ctx, &ctx->preds, (size_t)cap * kMaxPreds)) return rc;
+  if (int rc = dev_alloc(ctx, &ctx->npred, cap)) return rc;
+  if (int rc = dev_alloc(ctx, &ctx->birth_cells, cap)) return rc;
+  if (int rc = dev_alloc(ctx, &ctx->birth_rules, cap)) return rc;
+  ctx->cap = cap;
+  return MARL_OK;
+}
+
+int marl_launch_occupancy_from_cells(marl_ctx* ctx) {
+  if (int rc = ensure_voxel_buffers(ctx)) return rc;
+  if (int rc = fill_i32(ctx, ctx->cell_at, -1, ctx->nvox)) return rc;
+  if (int rc = fill_i32(ctx, reinterpret_cast<int32_t*>(ctx->occ), 0, ctx->occ_words)) return rc;
+  const int n = (int)ctx->n_cells;
+  if (n > 0) {
+    const CellParams p = cell_params(ctx, &ctx->last_sim, 0);
+    hipLaunchKernelGGL(k_scatter_cells, dim3(grid1(n)), dim3(256), 0, ctx->stream, ctx->cells[ctx->cbuf], n,
+                       ctx->cell_at, ctx->occ, p);
+    MARL_HIP(ctx, hipGetLastError());
+  }
+  return MARL_OK;
+}
+
+namespace {
+
+// The ordered cell pass (lib.rs:204-217) on the resident field, light and
+// occupancy.  Leaves events in ctx->events; *levels_out = schedule depth.
+int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEvent_t ev_sched) {
+  const int n = (int)ctx->n_cells;
+  if (levels_out) *levels_out = 0;
+  if (n == 0) {
+    if (ev_sched) MARL_HIP(ctx, hipEventRecord(ev_sched, ctx->stream));
+    return MARL_OK;
+  }
+  marl_cell_state* cells = ctx->cells[ctx->cbuf];
+  const marl_ruleset* rules = ctx->rules[ctx->cbuf];
+  hipLaunchKernelGGL(k_cell_prep, dim3(grid1(n)), dim3(256), 0, ctx->stream, cells, n, ctx->cell_at, ctx->nbmask,
+                     ctx->preds, ctx->npred, ctx->level, p);
+  MARL_HIP(ctx, hipGetLastError());
+  int32_t* changed = ctx->misc;
+  int32_t* h = ctx->pinned;
+  for (int sweep = 0;; ++sweep) {
+    if (sweep > kMaxLevels + 2) return marl_fail(ctx, MARL_ERR_TRANSFER, "conflict schedule did not converge");
+    // a few sweeps per host check
+    if (int rc = fill_i32(ctx, changed, 0, 1)) return rc;
+    for (int k = 0; k < 4; ++k)
+      hipLaunchKernelGGL(k_level_relax, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->preds, ctx->npred,
+                         ctx->level, changed);
+    MARL_HIP(ctx, hipGetLastError());
+    // if the last of the four sweeps changed nothing the levels are final
+    if (int rc = fill_i32(ctx, changed + 1, 0, 1)) return rc;
+    hipLaunchKernelGGL(k_level_relax, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->preds, ctx->npred,
+                       ctx->level, changed + 1);
+    MARL_HIP(ctx, hipGetLastError());
+    if (int rc = d2h_sync(ctx, h, changed + 1, sizeof(int32_t))) return rc;
+    if (h[0] == 0) break;
+  }
+  int32_t* hist = ctx->level_count;
+  if (int rc = fill_i32(ctx, hist, 0, kMaxLevels + 1)) return rc;
+  hipLaunchKernelGGL(k_level_hist, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->level, hist);
+  MARL_HIP(ctx, hipGetLastError());
+  std::vector<int32_t> hh(kMaxLevels + 1);
+  if (int rc = d2h_sync(ctx, hh.data(), hist, sizeof(int32_t) * (kMaxLevels + 1))) return rc;
+  const int nlev = hh[0] + 1;
+  if (nlev > kMaxLevels) return marl_fail(ctx, MARL_ERR_TRANSFER, "cell-pass schedule deeper than 4096 levels");
+  std::vector<int32_t> off(nlev + 1, 0);
+  for (int L = 0; L < nlev; ++L) off[L + 1] = off[L] + hh[1 + L];
+  int32_t* cursor = ctx->level_count + (kMaxLevels + 1);
+  MARL_HIP(ctx, hipMemcpyAsync(cursor, off.data(), sizeof(int32_t) * nlev, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_level_scatter, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->level, cursor, ctx->order);
+  MARL_HIP(ctx, hipGetLastError());
+  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));  // `off` must outlive the copy
+  if (ev_sched) MARL_HIP(ctx, hipEventRecord(ev_sched, ctx->stream));
+  float* field = ctx->field[ctx->cur];
+  for (int L = 0; L < nlev; ++L) {
+    const int cnt = off[L + 1] - off[L];
+    if (cnt == 0) continue;
+    hipLaunchKernelGGL(k_kinetics, dim3(grid1(cnt, kKinBlock)), dim3(kKinBlock), 0, ctx->stream,
+                       ctx->order + off[L], cnt, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p);
+  }
+  MARL_HIP(ctx, hipGetLastError());
+  if (levels_out) *levels_out = nlev;
+  return MARL_OK;
+}
+
+// Fate, births with mutation, deaths and compaction (lib.rs:220-277).
+int run_fate(marl_ctx* ctx, const CellParams& p, marl_tick_stats* st) {
+  const int n = (int)ctx->n_cells;
+  if (n == 0) return MARL_OK;
+  const int64_t cap = ctx->cap;
+  int32_t* target = ctx->flag_scan;
+  int32_t* bflag = target + (cap + 1);
+  int32_t* brank = bflag + (cap + 1);
+  int32_t* P = brank + (cap + 1);
+  int32_t* btarget = ctx->flag_scan2;
+  int32_t* win = btarget + (cap + 1);
+  int32_t* wrank = win + (cap + 1);
+  marl_cell_state* cells = ctx->cells[ctx->cbuf];
+  marl_ruleset* rules = ctx->rules[ctx->cbuf];
+
+  hipLaunchKernelGGL(k_div_target, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, cells, ctx->events, ctx->cell_at,
+                     target, bflag, p);
+  MARL_HIP(ctx, hipGetLastError());
+  if (int rc = fill_i32(ctx, bflag + n, 0, 1)) return rc;
+  if (int rc = scan_i32(ctx, bflag, brank, n + 1)) return rc;  // brank[n] = number of births
+  hipLaunchKernelGGL(k_births, dim3(grid1(n, 64)), dim3(64), 0, ctx->stream, n, cells, rules, target, brank,
+                     ctx->birth_cells, ctx->birth_rules, btarget, ctx->claim, p);
+  MARL_HIP(ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_death_flags, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->events, win);
+  MARL_HIP(ctx, hipGetLastError());
+  if (int rc = fill_i32(ctx, win + n, 0, 1)) return rc;
+  if (int rc = scan_i32(ctx, win, P, n + 1)) return rc;  // P[n] = deaths
+  int32_t* h = ctx->pinned;
+  MARL_HIP(ctx, hipMemcpyAsync(h, brank + n, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  MARL_HIP(ctx, hipMemcpyAsync(h + 1, P + n, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int nb = h[0], nd = h[1];
+  int nw = 0;
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_birth_winners, dim3(grid1(nb)), dim3(256), 0, ctx->stream, nb, btarget, ctx->claim, win);
+    MARL_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_claim_reset, dim3(grid1(nb)), dim3(256), 0, ctx->stream, nb, btarget, ctx->claim);
+    MARL_HIP(ctx, hipGetLastError());
+    if (int rc = fill_i32(ctx, win + nb, 0, 1)) return rc;
+    if (int rc = scan_i32(ctx, win, wrank, nb + 1)) return rc;
+    if (int rc = d2h_sync(ctx, h + 2, wrank + nb, sizeof(int32_t))) return rc;
+    nw = h[2];
+  }
+  const int m0 = n - nd;
+  const int64_t n_new = (int64_t)m0 + nw;
+  // the destination buffer must hold n_new cells
+  marl_cell_state* dc = ctx->cells[ctx->cbuf ^ 1];
+  marl_ruleset* dr = ctx->rules[ctx->cbuf ^ 1];
+  if (m0 > 0) {
+    hipLaunchKernelGGL(k_compact_gather, dim3(grid1(m0)), dim3(256), 0, ctx->stream, n, m0, ctx->events, P, cells,
+                       rules, dc, dr);
+    MARL_HIP(ctx, hipGetLastError());
+  }
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_append_births, dim3(grid1(nb)), dim3(256), 0, ctx->stream, nb, m0, win, wrank,
+                       ctx->birth_cells, ctx->birth_rules, dc, dr);
+    MARL_HIP(ctx, hipGetLastError());
+  }
+  ctx->cbuf ^= 1;
+  ctx->n_cells = n_new;
+  if (st) {
+    st->divisions = nb;
+    st->deaths = nd;
+    st->births_added = nw;
+    st->n_cells = n_new;
+  }
+  return MARL_OK;
+}
+
+int check_cells_host(marl_ctx* ctx, const marl_cell_state* cells, size_t n) {
+  std::vector<uint8_t> seen(ctx->nvox, 0);
+  for (size_t i = 0; i < n; ++i) {
+    const uint16_t* ps = cells[i].pos;
+    if (ps[0] >= ctx->d.x || ps[1] >= ctx->d.y || ps[2] >= ctx->d.z)
+      return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "cell position out of bounds");
+    const size_t v = ((size_t)ps[2] * ctx->d.y + ps[1]) * ctx->d.x + ps[0];
+    if (seen[v]) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "two cells share a voxel");
+    seen[v] = 1;
+  }
+  return MARL_OK;
+}
+
+}  // namespace
+
 extern "C" {
-int marl_upload_cells(marl_ctx* ctx, const marl_cell_state*, const marl_ruleset*, size_t) {
-  return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "cell path not built");
+
+int marl_upload_cells(marl_ctx* ctx, const marl_cell_state* cells, const marl_ruleset* rules, size_t n) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  if (n > 0 && (!cells || !rules)) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "null cell buffers");
+  if (n > ctx->nvox) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "more cells than voxels");
+  if (int rc = check_cells_host(ctx, cells, n)) return rc;
+  if (int rc = ensure_voxel_buffers(ctx)) return rc;
+  ctx->n_cells = 0;
+  if (int rc = marl_ensure_cell_capacity(ctx, std::max<int64_t>(2 * (int64_t)n, 1024))) return rc;
+  if (n > 0) {
+    MARL_HIP(ctx, hipMemcpyAsync(ctx->cells[ctx->cbuf], cells, n * sizeof(marl_cell_state), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    MARL_HIP(ctx, hipMemcpyAsync(ctx->rules[ctx->cbuf], rules, n * sizeof(marl_ruleset), hipMemcpyHostToDevice,
+                                 ctx->stream));
+  }
+  ctx->n_cells = (int64_t)n;
+  if (int rc = marl_launch_occupancy_from_cells(ctx)) return rc;
+  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MARL_OK;
 }
+
 int marl_cell_count(marl_ctx* ctx, int64_t* n) {
-  if (n) *n = 0;
-  return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "cell path not built");
+  if (!ctx || !n) return MARL_ERR_INVALID_INPUT;
+  *n = ctx->n_cells;
+  return MARL_OK;
 }
-int marl_download_cells(marl_ctx* ctx, marl_cell_state*, marl_ruleset*, size_t, int64_t*) {
-  return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "cell path not built");
+
+int marl_download_cells(marl_ctx* ctx, marl_cell_state* cells, marl_ruleset* rules, size_t capacity,
+                        int64_t* n_out) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  const size_t n = (size_t)ctx->n_cells;
+  if (n_out) *n_out = (int64_t)n;
+  if (n == 0) return MARL_OK;
+  if (capacity < n || !cells) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "download capacity below cell count");
+  std::vector<marl_cell_state> tc(n);
+  MARL_HIP(ctx, hipMemcpyAsync(tc.data(), ctx->cells[ctx->cbuf], n * sizeof(marl_cell_state), hipMemcpyDeviceToHost,
+                               ctx->stream));
+  std::vector<marl_ruleset> tr;
+  if (rules) {
+    tr.resize(n);
+    MARL_HIP(ctx, hipMemcpyAsync(tr.data(), ctx->rules[ctx->cbuf], n * sizeof(marl_ruleset), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  }
+  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  memcpy(cells, tc.data(), n * sizeof(marl_cell_state));
+  if (rules) memcpy(rules, tr.data(), n * sizeof(marl_ruleset));
+  return MARL_OK;
 }
-int marl_tick(marl_ctx* ctx, const marl_sim_params*, int64_t, int32_t, marl_tick_stats*) {
-  return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "cell path not built");
+
+int marl_upload_light(marl_ctx* ctx, const float* light, size_t n) {
+  if (!ctx || !light) return MARL_ERR_INVALID_INPUT;
+  if (n != ctx->nvox) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "light length differs from voxel count");
+  MARL_HIP(ctx, hipMemcpyAsync(ctx->light, light, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MARL_OK;
 }
-int marl_cell_pass(marl_ctx* ctx, const marl_sim_params*, uint8_t*, size_t, int32_t*) {
-  return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "cell path not built");
+
+int marl_cell_pass(marl_ctx* ctx, const marl_sim_params* sim, uint8_t* events, size_t n_events, int32_t* levels) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  if (int rc = marl_check_params(ctx, sim)) return rc;
+  const size_t n = (size_t)ctx->n_cells;
+  if (events && n_events < n) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "events buffer shorter than cell count");
+  ctx->last_sim = *sim;
+  if (int rc = marl_launch_occupancy_from_cells(ctx)) return rc;
+  const CellParams p = cell_params(ctx, sim, 0);
+  if (int rc = run_cell_pass(ctx, p, levels, nullptr)) return rc;
+  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (events && n > 0) {
+    std::vector<uint8_t> tmp(n);
+    if (int rc = d2h_sync(ctx, tmp.data(), ctx->events, n)) return rc;
+    memcpy(events, tmp.data(), n);
+  }
+  return MARL_OK;
 }
-int marl_mass_budget(marl_ctx* ctx, double*, size_t) {
-  return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "cell path not built");
+
+int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t n_ticks, marl_tick_stats* stats) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  if (int rc = marl_check_params(ctx, sim)) return rc;
+  if (n_ticks < 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "negative tick count");
+  ctx->last_sim = *sim;
+  if (int rc = ensure_voxel_buffers(ctx)) return rc;
+  hipEvent_t ev[5] = {};
+  for (auto& e : ev) MARL_HIP(ctx, hipEventCreate(&e));
+  struct Guard {
+    hipEvent_t* e;
+    ~Guard() {
+      for (int k = 0; k < 5; ++k)
+        if (e[k]) (void)hipEventDestroy(e[k]);
+    }
+  } guard{ev};
+  marl_tick_stats st{};
+  for (int t = 0; t < n_ticks; ++t) {
+    st = marl_tick_stats{};
+    if (int rc = marl_ensure_cell_capacity(ctx, 2 * ctx->n_cells + 1024)) return rc;
+    const CellParams p = cell_params(ctx, sim, (uint64_t)(tick0 + t));
+    MARL_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+    // occupancy from the live cells (lib.rs:174-179); sources fused into the
+    // first substep (field.rs:231-250 then :212-217); light (lib.rs:199)
+    if (int rc = marl_launch_occupancy_from_cells(ctx)) return rc;
+    if (sim->diffusion_substeps > 0) {
+      for (int k = 0; k < sim->diffusion_substeps; ++k) {
+        const StencilParams sp = marl_stencil_params(sim, k == 0 ? 1 : 0);
+        if (int rc = marl_launch_substep(ctx, sp, ctx->tile)) return rc;
+      }
+    } else {
+      if (int rc = marl_launch_boundary(ctx, sim)) return rc;
+    }
+    if (int rc = marl_launch_light(ctx, sim)) return rc;
+    MARL_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+    int32_t levels = 0;
+    if (int rc = run_cell_pass(ctx, p, &levels, ev[2])) return rc;
+    if (ctx->n_cells == 0) MARL_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+    MARL_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+    if (int rc = run_fate(ctx, p, &st)) return rc;
+    MARL_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
+    MARL_HIP(ctx, hipEventSynchronize(ev[4]));
+    st.levels = levels;
+    st.n_cells = ctx->n_cells;
+    (void)hipEventElapsedTime(&st.ms_field, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&st.ms_schedule, ev[1], ev[2]);
+    (void)hipEventElapsedTime(&st.ms_kinetics, ev[2], ev[3]);
+    (void)hipEventElapsedTime(&st.ms_fate, ev[3], ev[4]);
+  }
+  if (stats) *stats = st;
+  return MARL_OK;
 }
+
+int marl_seed_positions(const marl_dims* d, uint64_t seed, uint64_t band, int64_t count, int32_t margin,
+                        int32_t z_lo, int32_t z_hi, int32_t* cell_at, uint16_t* pos_out, uint64_t* lineage_out,
+                        int64_t index_base, int64_t* placed) {
+  if (!d || !cell_at || !placed || count < 0 || d->x - 2 * margin <= 0 || d->y - 2 * margin <= 0)
+    return MARL_ERR_INVALID_INPUT;
+  const int32_t zh = std::min(z_hi, d->z);
+  if (zh - z_lo <= 0 || z_lo < 0) return MARL_ERR_INVALID_INPUT;
+  if (count > 0 && (!pos_out || !lineage_out)) return MARL_ERR_INVALID_INPUT;
+  MarlRng rng;
+  rng.init(seed, (MARL_TAG_SEEDING << 32) | band, 0);
+  int64_t seeded = 0;
+  for (int64_t a = 0; a < count * 8; ++a) {
+    if (seeded >= count) break;
+    const int x = margin + (int)rng.range((uint32_t)(d->x - 2 * margin));
+    const int y = margin + (int)rng.range((uint32_t)(d->y - 2 * margin));
+    const int z = z_lo + (int)rng.range((uint32_t)(zh - z_lo));
+    const size_t v = ((size_t)z * d->y + y) * d->x + x;
+    if (cell_at[v] >= 0) continue;
+    const uint64_t lin = rng.u64();
+    cell_at[v] = (int32_t)(index_base + seeded);
+    pos_out[3 * seeded + 0] = (uint16_t)x;
+    pos_out[3 * seeded + 1] = (uint16_t)y;
+    pos_out[3 * seeded + 2] = (uint16_t)z;
+    lineage_out[seeded] = lin;
+    ++seeded;
+  }
+  *placed = seeded;
+  return MARL_OK;
 }
+
+int marl_mass_budget(marl_ctx* ctx, double* out, size_t n_out) {
+  if (!ctx || !out) return MARL_ERR_INVALID_INPUT;
+  if (n_out < (size_t)kBudget) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "mass budget needs 40 doubles");
+  if (!ctx->budget) MARL_HIP(ctx, hipMalloc(&ctx->budget, sizeof(double) * (kBudgetBlocks * kBudget + kBudget)));
+  MARL_HIP(ctx, hipMemsetAsync(ctx->budget, 0, sizeof(double) * kBudgetBlocks * kBudget, ctx->stream));
+  hipLaunchKernelGGL(k_budget_field, dim3(kBudgetBlocks), dim3(256), 0, ctx->stream, ctx->field[ctx->cur], ctx->occ,
+                     ctx->nvox, ctx->budget);
+  MARL_HIP(ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_budget_cells, dim3(kBudgetBlocks), dim3(256), 0, ctx->stream, ctx->cells[ctx->cbuf],
+                     (int)ctx->n_cells, ctx->budget);
+  MARL_HIP(ctx, hipGetLastError());
+  double* fin = ctx->budget + kBudgetBlocks * kBudget;
+  hipLaunchKernelGGL(k_budget_final, dim3(1), dim3(64), 0, ctx->stream, ctx->budget, kBudgetBlocks, fin);
+  MARL_HIP(ctx, hipGetLastError());
+  double tmp[kBudget];
+  if (int rc = d2h_sync(ctx, tmp, fin, sizeof(tmp))) return rc;
+  memcpy(out, tmp, sizeof(tmp));
+  return MARL_OK;
+}
+
+}  // extern "C"

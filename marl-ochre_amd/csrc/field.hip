@@ -70,8 +70,10 @@ __global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, 
   constexpr int TY = 4 * RY;
   constexpr int LW = 66;
   constexpr int S = MARL_S_EXT;
-  __shared__ float lds[S][TY + 2][LW];
-  __shared__ uint32_t mlds[TY + 2][4];
+  // one spare LDS row (TY + 2) and spare mask slots take the writes of lanes
+  // that own no halo value, so those writes need no branch
+  __shared__ float lds[S][TY + 3][LW];
+  __shared__ uint32_t mlds[(TY + 2) * 4 + 256];
 
   // XCD-aware tile order: hardware dispatches block b to XCD b % 8; give each
   // XCD a contiguous run of logical tiles so y-neighbour tiles (which share
@@ -92,133 +94,153 @@ __global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, 
   const bool xin = x < X;
   const uint32_t XY = (uint32_t)X * (uint32_t)Y;
   const uint32_t XYb = XY * 4u, Pb = XYb * (uint32_t)Z;
-  const int wx0 = x0 >> 5;
+  const uint32_t XWb = (uint32_t)XW * 4u, MZb = XWb * (uint32_t)Y;
   const rsrc_t rs = make_rsrc(src, Pb * S);
   const rsrc_t rd = make_rsrc(dst, Pb * S);
+  const rsrc_t rm = make_rsrc(occ, MZb * (uint32_t)Z);
 
-  float prv[RY][S], cur[RY][S], nxt[RY][S];
-  uint32_t mp[RY], mn[RY];
+  // Every load below is unconditional: lanes outside the grid (partial tiles)
+  // read voxel 0 of the plane instead, a value they never use, so every access
+  // stays inside the buffer and the loop body has no branches around loads.
   bool valid[RY];
   int yr[RY];
-  uint32_t vo[RY];
-
+  uint32_t vo[RY], mo[RY];
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
     yr[r] = y0 + wy + 4 * r;
     valid[r] = xin && yr[r] < Y;
-    vo[r] = ((uint32_t)yr[r] * (uint32_t)X + (uint32_t)x) * 4u;
-    mp[r] = 0u;
-    mn[r] = 0u;
-    if (valid[r]) {
-      const size_t wrow = (size_t)yr[r] * XW + (x >> 5);
-#pragma unroll
-      for (int s = 0; s < S; ++s) cur[r][s] = ld_field<SRC>(rs, p, s, Pb, XYb, Z, vo[r], z0);
-      if (z0 > 0) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) prv[r][s] = ld_field<SRC>(rs, p, s, Pb, XYb, Z, vo[r], z0 - 1);
-        mp[r] = occ[(size_t)(z0 - 1) * Y * XW + wrow];
-      }
-      if (z0 + 1 < Z) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) nxt[r][s] = ld_field<SRC>(rs, p, s, Pb, XYb, Z, vo[r], z0 + 1);
-        mn[r] = occ[(size_t)(z0 + 1) * Y * XW + wrow];
-      }
-    }
+    vo[r] = valid[r] ? ((uint32_t)yr[r] * (uint32_t)X + (uint32_t)x) * 4u : 0u;
+    mo[r] = valid[r] ? (uint32_t)yr[r] * XWb + (uint32_t)(x >> 5) * 4u : 0u;
   }
-
-  // halo source offsets of this thread (rows for waves 0-1, columns for 2-3)
-  bool hv;
-  uint32_t hvo;
+  // halo source offset of this thread: rows for waves 0-1, columns for waves 2-3
   int hrow, hcol;
+  uint32_t hvo;
+  bool hw;  // lanes that own a halo slot
   if (wy < 2) {
     const int yy = (wy == 0) ? y0 - 1 : y0 + TY;
     hrow = (wy == 0) ? 0 : TY + 1;
     hcol = lx + 1;
-    hv = xin && yy >= 0 && yy < Y;
-    hvo = ((uint32_t)(yy < 0 ? 0 : yy) * (uint32_t)X + (uint32_t)x) * 4u;
+    hw = yy >= 0 && yy < Y && xin;
+    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)x) * 4u : 0u;
   } else {
     const int xx = (wy == 2) ? x0 - 1 : x0 + 64;
-    const int yy = y0 + lx;
-    hrow = lx + 1;
+    const int yy = y0 + (lx < TY ? lx : 0);
+    hrow = (lx < TY ? lx : 0) + 1;
     hcol = (wy == 2) ? 0 : LW - 1;
-    hv = lx < TY && xx >= 0 && xx < X && yy < Y;
-    hvo = ((uint32_t)yy * (uint32_t)X + (uint32_t)(xx < 0 ? 0 : xx)) * 4u;
+    hw = lx < TY && xx >= 0 && xx < X && yy < Y;
+    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)xx) * 4u : 0u;
+  }
+  // mask words of plane z for rows y0-1 .. y0+TY, word columns x0/32-1 .. +2
+  const int mrow = tid >> 2, mwc = tid & 3;
+  const int myy = y0 - 1 + mrow, mwx = (x0 >> 5) - 1 + mwc;
+  const bool mw = tid < (TY + 2) * 4;
+  const bool mvalid = mw && myy >= 0 && myy < Y && mwx >= 0 && mwx < XW;
+  const uint32_t mvo = mvalid ? (uint32_t)myy * XWb + (uint32_t)mwx * 4u : 0u;
+  const int mslot = mw ? tid : (TY + 2) * 4 + tid;
+  if (!hw) {  // park the halo write of lanes without a halo slot in the spare row
+    hrow = TY + 2;
+    hcol = lx;
+  }
+
+  auto ldq = [&](float (&q)[RY][S], uint32_t (&m)[RY], int zz) {
+    // planes beyond the grid are never consumed; clamp the offset into range
+    const int zl = zz < Z ? zz : Z - 1;
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) q[r][s] = ld_field<SRC>(rs, p, s, Pb, XYb, Z, vo[r], zl);
+      m[r] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mo[r], (int)((uint32_t)zl * MZb), 0);
+    }
+  };
+
+  float prv[RY][S], cur[RY][S], nxt[RY][S], nn[RY][S];
+  uint32_t mp[RY], mc[RY], mn[RY], mnn[RY];
+  ldq(cur, mc, z0);
+  ldq(nxt, mn, z0 + 1);
+  ldq(nn, mnn, z0 + 2);
+  if (z0 > 0) {
+    ldq(prv, mp, z0 - 1);
+  } else {
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      mp[r] = 0u;
+#pragma unroll
+      for (int s = 0; s < S; ++s) prv[r][s] = 0.0f;
+    }
   }
 
   for (int z = z0; z < z1; ++z) {
-    // plane z into LDS: own values, halos, mask words
+    // halo values and mask words of plane z (issued first, waited on first)
+    float hv[S];
+    const int zl = z;
+#pragma unroll
+    for (int s = 0; s < S; ++s) hv[s] = ld_field<SRC>(rs, p, s, Pb, XYb, Z, hvo, zl);
+    const uint32_t mword = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zl * MZb), 0);
+    // keep the halo loads ahead of the prefetch so the wait before the barrier
+    // leaves the prefetch in flight
+    __builtin_amdgcn_sched_barrier(0);
+    // prefetch plane z+3
+    float n3[RY][S];
+    uint32_t mn3[RY];
+    ldq(n3, mn3, z + 3);
+    // plane z into LDS
 #pragma unroll
     for (int r = 0; r < RY; ++r)
-      if (valid[r]) {
 #pragma unroll
-        for (int s = 0; s < S; ++s) lds[s][wy + 4 * r + 1][lx + 1] = cur[r][s];
-      }
-    if (hv) {
+      for (int s = 0; s < S; ++s) lds[s][wy + 4 * r + 1][lx + 1] = cur[r][s];
 #pragma unroll
-      for (int s = 0; s < S; ++s) lds[s][hrow][hcol] = ld_field<SRC>(rs, p, s, Pb, XYb, Z, hvo, z);
-    }
-    if (tid < (TY + 2) * 4) {
-      const int row = tid >> 2, wc = tid & 3;
-      const int yy = y0 - 1 + row, wxx = wx0 - 1 + wc;
-      mlds[row][wc] = (yy >= 0 && yy < Y && wxx >= 0 && wxx < XW) ? occ[((size_t)z * Y + yy) * XW + wxx] : 0u;
-    }
-    // prefetch plane z+2 (next iteration's z+1)
-    float nn[RY][S];
-    uint32_t mnn[RY];
-    const bool pre = (z + 2 < Z) && (z + 1 < z1);
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      mnn[r] = 0u;
-      if (valid[r] && pre) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) nn[r][s] = ld_field<SRC>(rs, p, s, Pb, XYb, Z, vo[r], z + 2);
-        mnn[r] = occ[((size_t)(z + 2) * Y + yr[r]) * XW + (x >> 5)];
-      } else {
-#pragma unroll
-        for (int s = 0; s < S; ++s) nn[r][s] = 0.0f;
-      }
-    }
+    for (int s = 0; s < S; ++s) lds[s][hrow][hcol] = hv[s];
+    mlds[mslot] = mvalid ? mword : 0u;
     __syncthreads();
 
-    uint32_t mc[RY];
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       const int row = wy + 4 * r + 1;
-      mc[r] = mlds[row][(lx >> 5) + 1];
-      if (!valid[r]) continue;
       const int y = yr[r];
       auto bitat = [&](int rw, int c) -> uint32_t {
         const int i = c + 32;
-        return (mlds[rw][i >> 5] >> (i & 31)) & 1u;
+        return (mlds[rw * 4 + (i >> 5)] >> (i & 31)) & 1u;
       };
       const uint32_t zo = (uint32_t)z * XYb;
-      if (bitat(row, lx)) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) bstore(rd, cur[r][s], vo[r], (uint32_t)s * Pb + zo);
-        continue;
-      }
-      const bool vxm = (x > 0) && !bitat(row, lx - 1);
-      const bool vxp = (x < X - 1) && !bitat(row, lx + 1);
-      const bool vym = (y > 0) && !bitat(row - 1, lx);
-      const bool vyp = (y < Y - 1) && !bitat(row + 1, lx);
-      const bool vzm = (z > 0) && !((mp[r] >> (x & 31)) & 1u);
-      const bool vzp = (z < Z - 1) && !((mn[r] >> (x & 31)) & 1u);
+      // neighbour flags with bitwise operators: every mask read is unconditional
+      const uint32_t bc = bitat(row, lx), bxm = bitat(row, lx - 1), bxp = bitat(row, lx + 1);
+      const uint32_t bym = bitat(row - 1, lx), byp = bitat(row + 1, lx);
+      const uint32_t bzm = (mp[r] >> (x & 31)) & 1u, bzp = (mn[r] >> (x & 31)) & 1u;
+      const bool vxm = (uint32_t)(x > 0) & (bxm ^ 1u);
+      const bool vxp = (uint32_t)(x < X - 1) & (bxp ^ 1u);
+      const bool vym = (uint32_t)(y > 0) & (bym ^ 1u);
+      const bool vyp = (uint32_t)(y < Y - 1) & (byp ^ 1u);
+      const bool vzm = (uint32_t)(z > 0) & (bzm ^ 1u);
+      const bool vzp = (uint32_t)(z < Z - 1) & (bzp ^ 1u);
       const float structural = cur[r][7];
       const float niche = 1.0f - p.alpha_eps * structural / (p.k_eps + structural);
+      float out[S];
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         const float c = cur[r][s];
         const float dd = p.d_voxel[s] * niche;
-        const float xm = vxm ? lds[s][row][lx] : c;
-        const float xp = vxp ? lds[s][row][lx + 2] : c;
-        const float ym = vym ? lds[s][row - 1][lx + 1] : c;
-        const float yp = vyp ? lds[s][row + 1][lx + 1] : c;
+        const float lxm = lds[s][row][lx];
+        const float lxp = lds[s][row][lx + 2];
+        const float lym = lds[s][row - 1][lx + 1];
+        const float lyp = lds[s][row + 1][lx + 1];
+        const float xm = vxm ? lxm : c;
+        const float xp = vxp ? lxp : c;
+        const float ym = vym ? lym : c;
+        const float yp = vyp ? lyp : c;
         const float zm = vzm ? prv[r][s] : c;
         const float zp = vzp ? nxt[r][s] : c;
         const float lap = xm + xp + ym + yp + zm + zp - 6.0f * c;
         const float decay = p.lambda_decay[s] * c;
         const float nc = c + p.dt_sub * (dd * lap - decay);
-        bstore(rd, fmaxf(nc, 0.0f), vo[r], (uint32_t)s * Pb + zo);
+        // occupied centre keeps c; a bit select keeps the compiler from
+        // turning this into a branch around the neighbour reads
+        const uint32_t keep = 0u - bc;
+        out[s] = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, c) & keep) |
+                                              (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
+      }
+      if (valid[r]) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) bstore(rd, out[s], vo[r], (uint32_t)s * Pb + zo);
       }
     }
     __syncthreads();
@@ -229,9 +251,12 @@ __global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, 
         prv[r][s] = cur[r][s];
         cur[r][s] = nxt[r][s];
         nxt[r][s] = nn[r][s];
+        nn[r][s] = n3[r][s];
       }
       mp[r] = mc[r];
+      mc[r] = mn[r];
       mn[r] = mnn[r];
+      mnn[r] = mn3[r];
     }
   }
 }
@@ -417,13 +442,20 @@ int marl_unpack_occ(marl_ctx* ctx) {
 // Copy probe: a plain float4 streaming copy of known size, used to calibrate
 // HBM bandwidth and the FETCH_SIZE / WRITE_SIZE counters.
 __global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b, size_t n4) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) b[i] = a[i];
+  // each thread moves 4 float4 spaced one block-width apart (4 loads in flight)
+  const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+  float4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + 256 * k < n4) v[k] = a[base + 256 * k];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + 256 * k < n4) b[base + 256 * k] = v[k];
 }
 
 int marl_launch_copy_probe(marl_ctx* ctx) {
   const size_t n4 = ctx->nfield / 4;
-  hipLaunchKernelGGL(k_copy_probe, dim3(256 * 16), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(k_copy_probe, dim3((unsigned)((n4 + 1023) / 1024)), dim3(256), 0, ctx->stream,
                      reinterpret_cast<const float4*>(ctx->field[ctx->cur]),
                      reinterpret_cast<float4*>(ctx->field[ctx->cur ^ 1]), n4);
   MARL_HIP(ctx, hipGetLastError());

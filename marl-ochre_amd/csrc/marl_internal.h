@@ -190,6 +190,8 @@ struct marl_ctx {
   // per-cell scratch (capacity cap)
   uint8_t* nbmask = nullptr;   // 6-bit empty-neighbour mask
   int32_t* level = nullptr;
+  int32_t* preds = nullptr;    // conflict predecessors, 18 slots per cell
+  uint8_t* npred = nullptr;
   int32_t* order = nullptr;    // cells sorted by level
   uint8_t* events = nullptr;
   int32_t* flag_scan = nullptr;  // scans (cap+1)
@@ -205,6 +207,7 @@ struct marl_ctx {
   int32_t* pinned = nullptr;   // small pinned host buffer
   double* budget = nullptr;    // device FP64 partial sums
   int kin_mode = MARL_KINETICS_PLAIN;
+  marl_sim_params last_sim{};
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

@@ -112,6 +112,10 @@ class MarlGpu:
         self._chk(self.lib.marl_download_light(self.h, ptr(out), out.size))
         return out
 
+    def upload_light(self, light: np.ndarray) -> None:
+        lt = np.ascontiguousarray(light, dtype=np.float32).reshape(-1)
+        self._chk(self.lib.marl_upload_light(self.h, ptr(lt), lt.size))
+
     # -- timing / tuning ----------------------------------------------------
     def time_diffuse(self, sim, n: int) -> float:
         p = _params(sim)

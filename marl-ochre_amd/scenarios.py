@@ -53,3 +53,142 @@ def stencil_stress(dims=(256, 256, 128), seed=0, occ_frac=0.05, sigma=1.5):
     factor = rng.uniform(0.201, 1.0, nv)
     field[:, 7] = deposit_species7(factor, sim)
     return field.reshape(-1), occ, sim
+
+
+# ---------------------------------------------------------------------------
+# Cell workloads
+
+def seed_positions(dims, seed, band, count, margin, z_lo, z_hi, cell_at=None, index_base=0):
+    """seed_cells placement (crates/marl-sim/src/seeding.rs:28-58) through the C ABI."""
+    import ctypes
+
+    from . import _abi
+
+    lib = _abi.load_library()
+    nv = dims[0] * dims[1] * dims[2]
+    ca = np.full(nv, -1, np.int32) if cell_at is None else cell_at
+    pos = np.zeros((max(count, 1), 3), np.uint16)
+    lin = np.zeros(max(count, 1), np.uint64)
+    placed = ctypes.c_int64()
+    rc = lib.marl_seed_positions(ctypes.byref(_abi.marl_dims(*dims)), seed, band, count, margin, z_lo, z_hi,
+                                 _abi.ptr(ca), _abi.ptr(pos), _abi.ptr(lin), index_base, ctypes.byref(placed))
+    _abi.check(rc, what="marl_seed_positions")
+    return pos[: placed.value], lin[: placed.value], ca
+
+
+def random_reactions(rules, rng, vmax_range, km_range, light_p=0.0, k_cat=0.1):
+    """Overwrite the 16 reactions of each ruleset with random ones.
+
+    The workload's "stoichiometry +-1 over 1-3 species" maps onto the reference
+    Reaction (cell.rs:84-92) as: 2 species -> {substrate, product}; 3 species ->
+    {substrate, product, cofactor}.  One species has no encoding (substrate ==
+    product is skipped, cell.rs:250), so the count is drawn from {2, 3}.  The
+    catalyst is species 15 (light) with probability light_p, else uniform over
+    0..14; k_cat = 0.1; v_max and k_m are log-uniform over the given ranges.
+    """
+    n = rules.shape[0]
+    R = 16
+    lv = np.log(vmax_range)
+    lk = np.log(km_range)
+    rx = rules["reactions"]
+    vmax = np.exp(rng.uniform(lv[0], lv[1], (n, R))).astype(F32)
+    km = np.exp(rng.uniform(lk[0], lk[1], (n, R))).astype(F32)
+    nspec = rng.integers(2, 4, (n, R))
+    sub = rng.integers(0, 15, (n, R))
+    prod = (sub + rng.integers(1, 15, (n, R))) % 15
+    cof = (sub + rng.integers(1, 15, (n, R))) % 15
+    cof = np.where(cof == prod, (cof + 1) % 15, cof)
+    cof = np.where(cof == sub, (cof + 1) % 15, cof)
+    cof = np.where(nspec == 3, cof, 0xFF)
+    cat = np.where(rng.random((n, R)) < light_p, 15, rng.integers(0, 15, (n, R)))
+    rx["substrate"] = sub
+    rx["product"] = prod
+    rx["catalyst"] = cat
+    rx["cofactor"] = cof
+    rx["k_m"] = km
+    rx["v_max"] = vmax
+    rx["k_cat"] = k_cat
+
+
+def linear_gradient_field(dims):
+    """Species 1 and 3 fall linearly from 1 at z=0 to 0 at z=Z-1, species 2 rises
+    from 0 to 1; the other nine are 0."""
+    X, Y, Z = dims
+    z = np.arange(Z, dtype=np.float64) / max(Z - 1, 1)
+    f = np.zeros((Z, Y, X, 12), F32)
+    f[..., 1] = (1.0 - z)[:, None, None]
+    f[..., 3] = (1.0 - z)[:, None, None]
+    f[..., 2] = z[:, None, None]
+    return f.reshape(-1)
+
+
+def winogradsky_column(dims=(128, 128, 64), seed=0, cells_per_band=1366,
+                       bands=((4, 12), (24, 40), (52, 60))):
+    """The default-size column with about 4,096 cells.
+
+    Linear-gradient field; cells seeded by seed_cells in three depth bands, one
+    metabolism per band (phototroph, chemolithotroph, anaerobe, in that order)
+    with that starter's receptors, transport, effectors and fate; 16 internals
+    uniform in [0, 0.1]; 16 random reactions (v_max log-U[1e-3, 1e-1], k_m
+    log-U[1e-2, 1], catalyst uniform).  Reference defaults otherwise
+    (10 substeps).
+    """
+    from .config import SimulationConfig
+    from .starters import make_cell, make_ruleset
+    from ._abi import CELL_DTYPE, RULESET_DTYPE
+
+    sim = SimulationConfig(rng_seed=seed)
+    rng = np.random.Generator(np.random.Philox(seed + 1))
+    nv = dims[0] * dims[1] * dims[2]
+    ca = np.full(nv, -1, np.int32)
+    cells, rules = [], []
+    for kind, (lo, hi) in enumerate(bands):
+        pos, lin, ca = seed_positions(dims, seed, kind, cells_per_band, sim.seed_margin, lo, hi, ca,
+                                      index_base=sum(len(c) for c in cells))
+        c = np.zeros(len(pos), CELL_DTYPE)
+        r = np.zeros(len(pos), RULESET_DTYPE)
+        r[:] = make_ruleset(kind)
+        for k in range(len(pos)):
+            c[k] = make_cell(kind, pos[k], int(lin[k]))
+        cells.append(c)
+        rules.append(r)
+    cells = np.concatenate(cells)
+    rules = np.concatenate(rules)
+    cells["internal"] = rng.uniform(0.0, 0.1, (len(cells), 16)).astype(F32)
+    random_reactions(rules, rng, (1e-3, 1e-1), (1e-2, 1.0))
+    return linear_gradient_field(dims), cells, rules, sim
+
+
+def kinetics_at_scale(dims=(256, 256, 128), n_cells=1_000_000, seed=0):
+    """Plain per-cell kinetics at scale.
+
+    n_cells placed uniformly (seed_cells with margin 0 over all z); starter type
+    uniform over the three metabolisms (their transport, effectors and fate
+    thresholds, i.e. "as in the reference"); 16 internals uniform in [0, 1];
+    16 random reactions (v_max log-U[1e-3, 1], k_m log-U[1e-3, 1], 2-3 species,
+    light-catalysed with p = 0.2); field species uniform in [0, 1] with species 7
+    in [0, 0.1].  Reference defaults otherwise (10 substeps, mutation_rate 0.05).
+    """
+    from .config import SimulationConfig
+    from .starters import make_ruleset
+    from ._abi import CELL_DTYPE, RULESET_DTYPE
+
+    sim = SimulationConfig(rng_seed=seed)
+    rng = np.random.Generator(np.random.Philox(seed + 2))
+    pos, lin, _ = seed_positions(dims, seed, 7, n_cells, 0, 0, dims[2])
+    n = len(pos)
+    kinds = rng.integers(0, 3, n)
+    cells = np.zeros(n, CELL_DTYPE)
+    rules = np.zeros(n, RULESET_DTYPE)
+    for kind in range(3):
+        sel = kinds == kind
+        rules[sel] = make_ruleset(kind)
+    cells["pos"] = pos
+    cells["lineage_id"] = lin
+    cells["starter_type"] = kinds
+    cells["internal"] = rng.random((n, 16), dtype=F32)
+    random_reactions(rules, rng, (1e-3, 1.0), (1e-3, 1.0), light_p=0.2)
+    nv = dims[0] * dims[1] * dims[2]
+    field = rng.random((nv, 12), dtype=F32)
+    field[:, 7] *= 0.1
+    return field.reshape(-1), cells, rules, sim

@@ -1,0 +1,374 @@
+"""Pins the oracle's cell path (CPU only): CellState::tick known answers and
+fate truth table, neighbour helpers, division placement, mutate's draw
+contract, swap_remove compaction, seeding, and the closed-form compaction the
+GPU uses, against brute-force sequential emulation."""
+import numpy as np
+import pytest
+
+F = np.float32
+SMALL = (16, 12, 10)
+
+
+@pytest.fixture(scope="module")
+def A(pkg):
+    return pkg._abi
+
+
+def blank(A):
+    """A cell with every transporter, reaction and effector inactive."""
+    from marl_ochre_amd import starters
+
+    r = starters._inactive()
+    c = np.zeros((), A.CELL_DTYPE)
+    c["pos"] = (5, 5, 5)
+    r["division_energy"] = 100.0
+    r["death_energy"] = 0.0
+    r["quiescence_energy"] = 0.0
+    r["division_prep_ticks"] = 20.0
+    return c, r
+
+
+def sim_no_maint(pkg, **kw):
+    return pkg.SimulationConfig(lambda_maintenance=0.0, reaction_maintenance=0.0, hard_death_floor=0.0, **kw)
+
+
+def test_transport_known_answer(pkg, orc, A):
+    # cell.rs:231-237 with E = 1, I = 0: uptake = rate/2, secretion 0
+    c, r = blank(A)
+    c["internal"][0] = 1.0
+    r["transport"][0] = (0.8, 0.0, 3, 2)
+    ext = np.zeros(12, F)
+    ext[3] = 1.0
+    c2, D, ev = orc.cell_tick(c, r, ext, 0.5, sim_no_maint(pkg))
+    assert c2["internal"][2] == F(0.4)
+    assert D[3] == F(-0.4) and np.count_nonzero(D) == 1
+    assert c2["internal"][15] == F(0.5)  # light stored as pseudo-species 15 (cell.rs:243)
+
+
+def test_transport_is_sequential_and_clamped(pkg, orc, A):
+    # the second transporter sees the first one's update; internal clamps to c_max
+    c, r = blank(A)
+    c["internal"][0] = 1.0
+    c["internal"][4] = 9.9
+    r["transport"][0] = (2.0, 0.0, 1, 4)   # uptake from E1 = 1 -> +1.0 -> clamps to 10
+    r["transport"][1] = (0.0, 1.1, 5, 4)   # secretes 1.1 * 10/11 = 1.0 from the clamped value
+    ext = np.zeros(12, F)
+    ext[1] = 1.0
+    c2, D, _ = orc.cell_tick(c, r, ext, 0.0, sim_no_maint(pkg))
+    assert c2["internal"][4] == pytest.approx(9.0, rel=1e-6)
+    assert D[1] == F(-1.0) and D[5] == pytest.approx(1.0, rel=1e-6)
+
+
+def test_reaction_known_answer_and_epsilon_background(pkg, orc, A):
+    # cell.rs:261-284: no catalyst -> rate = vmax * s/(km+s+eps) * epsilon
+    c, r = blank(A)
+    c["internal"][0] = 1.0
+    c["internal"][1] = 1.0
+    r["reactions"][0] = (1, 2, 3, 0xFF, 0.5, 2.0, 0.1)
+    sim = sim_no_maint(pkg)
+    c2, _, _ = orc.cell_tick(c, r, np.zeros(12, F), 0.0, sim)
+    rate = 2.0 * (1.0 / (0.5 + 1.0 + 2.0**-23)) * 0.001
+    assert c2["internal"][1] == pytest.approx(1.0 - rate, rel=1e-6)
+    assert c2["internal"][2] == pytest.approx(rate, rel=1e-6)
+    # with catalyst present: factor epsilon + c/(kcat + c)
+    c["internal"][3] = 0.3
+    r["reactions"][0] = (1, 2, 3, 0xFF, 0.5, 1.0, 0.1)
+    c3, _, _ = orc.cell_tick(c, r, np.zeros(12, F), 0.0, sim)
+    rate3 = 1.0 * (1.0 / 1.5) * (0.001 + 0.3 / 0.4)
+    assert c3["internal"][2] == pytest.approx(rate3, rel=1e-5)
+
+
+def test_reaction_cofactor_and_caps(pkg, orc, A):
+    c, r = blank(A)
+    c["internal"][0] = 1.0
+    c["internal"][1] = 0.25   # substrate smaller than rate*dt -> flux capped at substrate
+    c["internal"][3] = 1.0    # cofactor: rate *= 1/2, consumes min(rate/2, cof)
+    c["internal"][2] = 9.9    # product capped at c_max
+    c["internal"][6] = 1e6    # catalyst saturates: ct = eps + ~1
+    r["reactions"][0] = (1, 2, 6, 3, 1e-6, 50.0, 1e-6)
+    c2, _, _ = orc.cell_tick(c, r, np.zeros(12, F), 0.0, sim_no_maint(pkg))
+    rate = 50.0 * (0.25 / (0.25 + 1e-6 + 2**-23)) * (0.001 + 1.0) * 0.5
+    assert c2["internal"][3] == pytest.approx(1.0 - min(0.5 * rate, 1.0), abs=1e-6)
+    assert c2["internal"][1] == 0.0
+    assert c2["internal"][2] == F(10.0)
+
+
+@pytest.mark.parametrize("skip", ["inactive", "self", "out_of_range"])
+def test_reaction_skips(pkg, orc, A, skip):
+    c, r = blank(A)
+    c["internal"][:] = 0.5
+    c["internal"][0] = 1.0
+    row = {"inactive": (1, 2, 3, 0xFF, 0.5, 1e-10, 0.1), "self": (2, 2, 3, 0xFF, 0.5, 1.0, 0.1),
+           "out_of_range": (1, 2, 16, 0xFF, 0.5, 1.0, 0.1)}[skip]
+    r["reactions"][0] = row
+    c2, _, _ = orc.cell_tick(c, r, np.zeros(12, F), 0.5, sim_no_maint(pkg))
+    assert np.array_equal(c2["internal"][1:15], c["internal"][1:15])
+
+
+def test_maintenance_known_answer(pkg, orc, A):
+    c, r = blank(A)
+    c["internal"][0] = 2.0
+    c["prep_remaining"] = 5
+    r["division_prep_ticks"] = 15.0
+    r["reactions"][0] = (1, 2, 3, 0xFF, 0.5, 1e-9 * 2, 0.1)    # active, counts for maintenance
+    r["reactions"][1] = (4, 4, 3, 0xFF, 0.5, 1.0, 0.1)          # sub == prod: not counted
+    r["reactions"][2] = (1, 2, 3, 0xFF, 0.5, 1e-9, 0.1)         # |vmax| == threshold: not counted (strict >)
+    sim = pkg.SimulationConfig(hard_death_floor=0.0)
+    c2, _, ev = orc.cell_tick(c, r, np.zeros(12, F), 0.0, sim)
+    pm = 2.0 + max(20.0 - 15.0, 0.0) * 0.05
+    want = 2.0 * (1.0 - 0.12 * pm) - 1 * 0.003
+    assert c2["internal"][0] == pytest.approx(want, rel=1e-6)
+    assert c2["prep_remaining"] == 4 and ev == A.EVENT_NONE
+
+
+def test_effectors_and_quiescence_skip(pkg, orc, A):
+    c, r = blank(A)
+    c["internal"][0] = 1.0
+    c["internal"][5] = 1.0
+    r["effectors"][0] = (0.5, 0.6, 5, 7)
+    c2, D, _ = orc.cell_tick(c, r, np.zeros(12, F), 0.0, sim_no_maint(pkg))
+    assert c2["internal"][5] == pytest.approx(0.7, rel=1e-6) and D[7] == pytest.approx(0.3, rel=1e-6)
+    c["quiescent"] = 1
+    c3, D3, _ = orc.cell_tick(c, r, np.zeros(12, F), 0.0, sim_no_maint(pkg))
+    assert c3["internal"][5] == F(1.0) and not D3.any()
+
+
+@pytest.mark.parametrize(
+    "energy,prep,death_e,floor,div_e,quiesc_e,prep_ticks,want_ev,want_prep,want_q",
+    [
+        (0.02, 0, 0.05, 0.01, 1.0, 0.1, 20.0, "DEATH", 0, None),
+        (0.02, 7, 0.01, 0.03, 1.0, 0.1, 20.0, "DEATH", 0, None),      # hard floor wins, prep reset
+        (0.50, 1, 0.05, 0.01, 1.0, 0.1, 20.0, "DIVISION", 0, None),
+        (0.50, 3, 0.05, 0.01, 1.0, 0.1, 20.0, "NONE", 2, None),
+        (1.50, 0, 0.05, 0.01, 1.0, 0.1, 20.7, "NONE", 20, 0),          # enter prep, truncation
+        (1.50, 0, 0.05, 0.01, 1.0, 0.1, 0.3, "NONE", 1, 0),            # max(prep_ticks, 1)
+        (1.50, 0, 0.05, 0.01, 1.0, 0.1, 1e9, "NONE", 65535, 0),        # `as u16` saturates
+        (0.08, 0, 0.05, 0.01, 1.0, 0.1, 20.0, "QUIESCENCE", 0, 1),
+        (0.50, 0, 0.05, 0.01, 1.0, 0.1, 20.0, "NONE", 0, 0),
+    ])
+def test_fate_truth_table(pkg, orc, A, energy, prep, death_e, floor, div_e, quiesc_e, prep_ticks, want_ev,
+                          want_prep, want_q):
+    c, r = blank(A)
+    c["internal"][0] = energy
+    c["prep_remaining"] = prep
+    c["quiescent"] = 1
+    c["age"] = 41
+    r["death_energy"], r["division_energy"], r["quiescence_energy"] = death_e, div_e, quiesc_e
+    r["division_prep_ticks"] = prep_ticks
+    sim = pkg.SimulationConfig(lambda_maintenance=0.0, reaction_maintenance=0.0, hard_death_floor=floor,
+                               base_division_prep=0.0)
+    c2, _, ev = orc.cell_tick(c, r, np.zeros(12, F), 0.0, sim)
+    assert ev == getattr(A, "EVENT_" + want_ev)
+    assert c2["prep_remaining"] == want_prep and c2["age"] == 42
+    if want_q is not None:
+        assert c2["quiescent"] == want_q
+
+
+def _cell_at(dims, cells):
+    ca = np.full(dims[0] * dims[1] * dims[2], -1, np.int32)
+    for i, c in enumerate(cells):
+        x, y, z = (int(v) for v in c["pos"])
+        ca[(z * dims[1] + y) * dims[0] + x] = i
+    return ca
+
+
+def _rand_population(A, dims, n, seed, starters=True):
+    from marl_ochre_amd import starters as st
+
+    rng = np.random.default_rng(seed)
+    nv = dims[0] * dims[1] * dims[2]
+    vox = rng.choice(nv, n, replace=False)
+    cells = np.zeros(n, A.CELL_DTYPE)
+    rules = np.zeros(n, A.RULESET_DTYPE)
+    for i, v in enumerate(vox):
+        kind = int(rng.integers(0, 3))
+        rules[i] = st.make_ruleset(kind)
+        cells[i] = st.make_cell(kind, (v % dims[0], (v // dims[0]) % dims[1], v // (dims[0] * dims[1])), i)
+    cells["internal"] = rng.random((n, 16)).astype(F) * 2.0
+    cells["prep_remaining"] = rng.integers(0, 3, n)
+    return cells, rules
+
+
+def test_cell_pass_matches_python_sequential_emulation(pkg, orc, A):
+    """The oracle's pass against a Python restatement of lib.rs:206-217 built from
+    the oracle's own per-cell tick: checks the gather / scatter / ordering."""
+    dims = SMALL
+    cells, rules = _rand_population(A, dims, 300, 1)
+    rng = np.random.default_rng(2)
+    nv = dims[0] * dims[1] * dims[2]
+    field = rng.random(nv * 12).astype(F)
+    light = rng.random(nv).astype(F)
+    sim = pkg.SimulationConfig()
+    f_or, c_or, ev_or = orc.cell_pass(dims, sim, field, light, cells, rules)
+    ca = _cell_at(dims, cells)
+    f = field.copy().reshape(nv, 12)
+    X, Y, Z = dims
+    for i in range(len(cells)):
+        x, y, z = (int(v) for v in cells[i]["pos"])
+        nb = []
+        for dx, dy, dz in [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]:
+            nx, ny, nz = x + dx, y + dy, z + dz
+            if 0 <= nx < X and 0 <= ny < Y and 0 <= nz < Z and ca[(nz * Y + ny) * X + nx] < 0:
+                nb.append((nz * Y + ny) * X + nx)
+        ext = np.zeros(12, F)
+        for v in nb:
+            ext = (ext + f[v]).astype(F)
+        if nb:
+            ext = (ext / F(len(nb))).astype(F)
+        c2, D, ev = orc.cell_tick(cells[i], rules[i], ext, light[(z * Y + y) * X + x], sim)
+        assert c2 == c_or[i] and ev == ev_or[i]
+        if nb:
+            split = (D / F(len(nb))).astype(F)
+            for v in nb:
+                f[v] = np.maximum((f[v] + split).astype(F), F(0))
+    assert np.array_equal(f.reshape(-1), f_or)
+
+
+def test_empty_neighbors_and_enclosed_cell(pkg, orc, A):
+    dims = SMALL
+    cells = np.zeros(7, A.CELL_DTYPE)
+    cells["pos"] = [(5, 5, 5), (6, 5, 5), (4, 5, 5), (5, 6, 5), (5, 4, 5), (5, 5, 6), (5, 5, 4)]
+    rules = np.zeros(7, A.RULESET_DTYPE)
+    from marl_ochre_amd import starters as st
+
+    rules[:] = st.make_ruleset(0)
+    cells["internal"] = 1.0
+    nv = np.prod(dims)
+    field = np.full(nv * 12, 0.5, F)
+    light = np.ones(nv, F)
+    f2, c2, ev = orc.cell_pass(dims, pkg.SimulationConfig(), field, light, cells, rules)
+    # the centre cell is enclosed: it sees zeros and its deltas are lost (spatial.rs:62-64, 94-96)
+    c0, D0, _ = orc.cell_tick(cells[0], rules[0], np.zeros(12, F), 1.0, pkg.SimulationConfig())
+    assert c2[0] == c0
+    assert D0.any()
+
+
+def test_find_empty_neighbor_prefers_distance_two(pkg, orc, A):
+    import ctypes
+
+    dims = SMALL
+    sim = pkg.SimulationConfig()
+    ca = np.full(np.prod(dims), -1, np.int32)
+    pos = np.array([0, 5, 5], np.uint16)   # -x side out of bounds
+    out = np.zeros(3, np.uint16)
+    d = np.zeros(1, np.uint64)
+    got = orc.L.orc_find_empty_neighbor(pos.ctypes.data, ca.ctypes.data, ctypes.byref(A.marl_dims(*dims)),
+                                        ctypes.byref(sim.to_params()), 9, 3, 4, d.ctypes.data, out.ctypes.data)
+    cands = [(2, 5, 5), (0, 7, 5), (0, 3, 5), (0, 5, 7), (0, 5, 3)]
+    k = (orc.draw_u32(9, 3, 4, 0) * len(cands)) >> 32
+    assert got == 1 and tuple(out) == cands[k] and d[0] == 1
+    # all distance-2 targets taken -> adjacent fallback; all taken -> no draw
+    for (x, y, z) in cands:
+        ca[(z * dims[1] + y) * dims[0] + x] = 0
+    d[0] = 0
+    got = orc.L.orc_find_empty_neighbor(pos.ctypes.data, ca.ctypes.data, ctypes.byref(A.marl_dims(*dims)),
+                                        ctypes.byref(sim.to_params()), 9, 3, 4, d.ctypes.data, out.ctypes.data)
+    adj = [(1, 5, 5), (0, 6, 5), (0, 4, 5), (0, 5, 6), (0, 5, 4)]
+    assert got == 1 and tuple(out) == adj[(orc.draw_u32(9, 3, 4, 0) * 5) >> 32]
+    for (x, y, z) in adj:
+        ca[(z * dims[1] + y) * dims[0] + x] = 0
+    d[0] = 0
+    got = orc.L.orc_find_empty_neighbor(pos.ctypes.data, ca.ctypes.data, ctypes.byref(A.marl_dims(*dims)),
+                                        ctypes.byref(sim.to_params()), 9, 3, 4, d.ctypes.data, out.ctypes.data)
+    assert got == 0 and d[0] == 0
+
+
+def test_mutate_draw_count_and_identity_at_rate_zero(pkg, orc, A):
+    from marl_ochre_amd import starters as st
+
+    r = st.make_ruleset(1)
+    r["mutation_rate"] = 0.0
+    sim = pkg.SimulationConfig(meta_mutation_rate=0.0)
+    r2, draws = orc.mutate(r, sim, 1, 2, 3)
+    assert draws == 174   # receptors 24 + transport 32 + reactions 96 + effectors 16 + fate 5 + meta 1
+    assert r2 == r
+
+
+def test_mutate_at_rate_one_changes_and_clamps(pkg, orc, A):
+    from marl_ochre_amd import starters as st
+
+    r = st.make_ruleset(2)
+    r["mutation_rate"] = 1.0
+    sim = pkg.SimulationConfig(structural_mutation_rate_mult=1.0, meta_mutation_rate=1.0)
+    r2, draws = orc.mutate(r, sim, 5, 6, 7)
+    assert draws > 174 + 2 * 100
+    assert (r2["reactions"]["k_m"] != r["reactions"]["k_m"]).all()
+    for fld in ["k_m", "v_max", "k_cat"]:
+        assert (r2["reactions"][fld] >= 0).all()
+    assert (r2["receptors"]["n_hill"] >= 0.5).all() and (r2["receptors"]["n_hill"] <= 8.0).all()
+    assert r2["division_prep_ticks"] >= 1.0
+    assert 0.001 <= r2["mutation_rate"] <= 0.5
+    assert (r2["transport"]["ext_species"] < 12).all() and (r2["transport"]["int_species"] < 16).all()
+
+
+def test_mutate_structural_copies_from_active_donors(pkg, orc, A):
+    from marl_ochre_amd import starters as st
+
+    r = st.make_ruleset(0)   # active reactions 0..4
+    r["mutation_rate"] = 1.0
+    sim = pkg.SimulationConfig(structural_mutation_rate_mult=1.0, meta_mutation_rate=0.0, mutation_stddev=0.0)
+    r2, _ = orc.mutate(r, sim, 11, 12, 13)
+    donors_sub = set(int(v) for v in r["reactions"]["substrate"][:5])
+    for i in range(16):
+        assert int(r2["reactions"]["substrate"][i]) in donors_sub | {int(r["reactions"]["substrate"][i])}
+
+
+# ---------------------------------------------------------------------------
+# compaction
+
+def python_swap_remove(n, deaths):
+    """lib.rs:257-267 literally: sort, dedup, swap_remove in descending order."""
+    cells = list(range(n))
+    for i in sorted(set(deaths), reverse=True):
+        cells[i] = cells[-1]
+        cells.pop()
+    return cells
+
+
+def closed_form(n, deaths):
+    """The level-chasing formula of the GPU kernel (k_compact_gather)."""
+    dead = np.zeros(n, bool)
+    dead[list(deaths)] = True
+    P = np.concatenate([[0], np.cumsum(dead)])
+    m0 = n - P[n]
+    out = []
+    for s0 in range(m0):
+        s, lo, m = s0, 0, m0
+        while dead[s]:
+            q = P[s] - P[lo]
+            s, lo = m + q, m
+            m = n - (P[n] - P[lo])
+        out.append(s)
+    return out
+
+
+def test_compaction_counterexample_to_naive_form(orc):
+    n, deaths = 28, [0, 3, 7, 10, 18, 21, 23, 25]
+    want = python_swap_remove(n, deaths)
+    assert list(orc.compact_indices(n, deaths)) == want
+    assert closed_form(n, deaths) == want
+
+
+def test_compaction_random_against_sequential_emulation(orc):
+    rng = np.random.default_rng(0)
+    for trial in range(3000):
+        n = int(rng.integers(0, 60))
+        frac = rng.random()
+        deaths = [i for i in range(n) if rng.random() < frac]
+        want = python_swap_remove(n, deaths)
+        assert closed_form(n, deaths) == want, (n, deaths)
+        if trial % 10 == 0:
+            assert list(orc.compact_indices(n, deaths)) == want
+
+
+def test_seeding_product_matches_oracle(pkg, orc):
+    from marl_ochre_amd import scenarios
+
+    dims = (64, 48, 32)
+    for band, (count, margin, lo, hi) in enumerate([(300, 5, 4, 12), (2000, 0, 0, 32), (50, 5, 30, 40)]):
+        p1, l1, ca1 = scenarios.seed_positions(dims, 42, band, count, margin, lo, hi)
+        p2, l2, ca2 = orc.seed_positions(dims, 42, band, count, margin, lo, hi)
+        assert np.array_equal(p1, p2) and np.array_equal(l1, l2) and np.array_equal(ca1, ca2)
+        assert (p1[:, 0] >= margin).all() and (p1[:, 0] < dims[0] - margin).all()
+        assert (p1[:, 2] >= lo).all() and (p1[:, 2] < min(hi, dims[2])).all()
+        assert len({tuple(p) for p in p1}) == len(p1)

@@ -220,8 +220,8 @@ int marl_autotune_stencil(marl_ctx* ctx, const marl_sim_params* sim, int32_t* ti
 int marl_set_stencil_tile(marl_ctx* ctx, int32_t tile);
 int marl_get_stencil_tile(marl_ctx* ctx, int32_t* tile);
 int marl_stencil_tile_count(void);
-/* rows-per-thread and z-chunk of a tile id (for reports). */
-int marl_stencil_tile_shape(int32_t tile, int32_t* rows_per_thread, int32_t* z_chunk);
+/* Rows per thread, z-chunk and prefetch distance (planes) of a tile id. */
+int marl_stencil_tile_shape(int32_t tile, int32_t* rows_per_thread, int32_t* z_chunk, int32_t* prefetch);
 /* Plain copy of the resident field into the scratch buffer (n_reps times,
  * float4 loads/stores): the HBM calibration kernel for roofline and counter
  * scaling.  *bytes_out = bytes read + written per rep. */

@@ -168,19 +168,47 @@ __global__ void k_level_relax(int n, const int32_t* __restrict__ preds, const ui
   }
 }
 
-__global__ void k_level_hist(int n, const int32_t* __restrict__ level, int32_t* hist /* [0]=max, [1+L]=count */) {
+// Level histogram and counting-sort scatter.  Counts are aggregated per block
+// in LDS (one global atomic per block and level) — a million same-address
+// global atomics would serialize in L2.
+constexpr int kLocalLevels = 64;
+
+__global__ void __launch_bounds__(256) k_level_hist(int n, const int32_t* __restrict__ level,
+                                                    int32_t* hist /* [0]=max, [1+L]=count */) {
+  __shared__ int32_t lh[kLocalLevels];
+  __shared__ int32_t lmax;
+  for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x) lh[t] = 0;
+  if (threadIdx.x == 0) lmax = 0;
+  __syncthreads();
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const int L = level[j];
-  atomicMax(&hist[0], L);
-  if (L < kMaxLevels) atomicAdd(&hist[1 + L], 1);
+  if (j < n) {
+    const int L = level[j];
+    atomicMax(&lmax, L);
+    if (L < kLocalLevels) atomicAdd(&lh[L], 1);
+    else if (L < kMaxLevels) atomicAdd(&hist[1 + L], 1);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x)
+    if (lh[t]) atomicAdd(&hist[1 + t], lh[t]);
+  if (threadIdx.x == 0) atomicMax(&hist[0], lmax);
 }
 
-__global__ void k_level_scatter(int n, const int32_t* __restrict__ level, int32_t* cursor, int32_t* __restrict__ order) {
+__global__ void __launch_bounds__(256) k_level_scatter(int n, const int32_t* __restrict__ level, int32_t* cursor,
+                                                       int32_t* __restrict__ order) {
+  __shared__ int32_t cnt[kLocalLevels], base[kLocalLevels];
+  for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x) cnt[t] = 0;
+  __syncthreads();
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const int pos = atomicAdd(&cursor[level[j]], 1);
-  order[pos] = j;
+  const int L = j < n ? level[j] : -1;
+  const int local = (L >= 0 && L < kLocalLevels) ? atomicAdd(&cnt[L], 1) : -1;
+  __syncthreads();
+  for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x)
+    base[t] = cnt[t] ? atomicAdd(&cursor[t], cnt[t]) : 0;
+  __syncthreads();
+  if (j < n) {
+    const int pos = (L < kLocalLevels) ? base[L] + local : atomicAdd(&cursor[L], 1);
+    order[pos] = j;
+  }
 }
 
 // ---------------------------------------------------------------------------

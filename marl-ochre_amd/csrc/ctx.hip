@@ -301,10 +301,11 @@ int marl_copy_probe(marl_ctx* ctx, int32_t n, float* ms, double* bytes) {
 
 int marl_stencil_tile_count(void) { return kNumTiles; }
 
-int marl_stencil_tile_shape(int32_t tile, int32_t* ry, int32_t* zc) {
+int marl_stencil_tile_shape(int32_t tile, int32_t* ry, int32_t* zc, int32_t* pd) {
   if (tile < 0 || tile >= kNumTiles) return MARL_ERR_INVALID_INPUT;
   if (ry) *ry = kTiles[tile].ry;
   if (zc) *zc = kTiles[tile].zc;
+  if (pd) *pd = kTiles[tile].pd;
   return MARL_OK;
 }
 

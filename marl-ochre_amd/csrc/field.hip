@@ -13,23 +13,28 @@
 //
 // A block of 256 threads (4 wave64) owns a 64 (x) x 4*RY (y) tile and marches
 // it through a z-chunk of zc planes.  Each thread keeps, for each of its RY
-// rows, the 12 species of planes z-1, z, z+1 (and a prefetch of z+2) in
-// registers; x/y neighbours come from an LDS copy of plane z with a one-voxel
-// halo.  HBM traffic per substep is one read and one write of every species
-// plane plus the 1-bit mask, plus two halo planes per z-chunk.  The niche
-// factor is computed in registers from species 7 of the centre voxel.
+// rows, the 12 species of planes z-1 .. z+PD in registers (a queue refilled
+// PD+1 planes ahead); x/y neighbours come from an LDS copy of plane z with a
+// one-voxel halo.  Halo rows and columns are prefetched at the same distance
+// as the tile's own planes, so the tile and its neighbours (same XCD, same z
+// progress) touch the shared cache lines together and the halo hits L2.
+// HBM traffic per substep is one read and one write of every species plane
+// plus the 1-bit mask, plus one halo plane on each side of a z-chunk.  The
+// niche factor is computed in registers from species 7 of the centre voxel.
 //
 // The per-voxel arithmetic is the reference's, in its order and without
 // contraction, so the result is bit-identical to the f32 CPU statement.
 
+// {rows per thread, z-chunk, prefetch distance in planes, species per block}
 const TileShape kTiles[] = {
-    {1, 8}, {1, 16}, {1, 32}, {1, 64}, {2, 8}, {2, 16}, {2, 32}, {2, 64}, {4, 16}, {4, 32},
+    {1, 32, 1, 12}, {2, 16, 1, 12}, {2, 32, 1, 12}, {2, 16, 1, 6}, {2, 32, 1, 6}, {2, 64, 1, 6},
+    {2, 32, 2, 6},  {4, 32, 1, 6},  {4, 64, 1, 6},
 };
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 int marl_default_tile(const marl_dims& d) {
   (void)d;
-  return 2;  // RY=1, zc=32 until the autotuner says otherwise
+  return 2;  // 2 rows per thread, z-chunk 32, prefetch 1, all species, until the autotuner says otherwise
 }
 
 // Raw buffer accesses: the voxel's byte offset lives in one VGPR, the
@@ -45,6 +50,11 @@ __device__ __forceinline__ float bload(rsrc_t r, uint32_t voff, uint32_t soff) {
 }
 __device__ __forceinline__ void bstore(rsrc_t r, float v, uint32_t voff, uint32_t soff) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+}
+// non-temporal store (cache policy nt): the substep's output is not re-read
+// before the next launch, so it should not displace halo lines in L2
+__device__ __forceinline__ void bstore_nt(rsrc_t r, float v, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 2);
 }
 
 // Load species s at plane zz; with SRC the boundary sources of field.rs:231-250
@@ -63,28 +73,38 @@ __device__ __forceinline__ float ld_field(rsrc_t r, const StencilParams& p, int 
   return v;
 }
 
-template <int RY, bool SRC>
+template <int RY, bool SRC, int PD, int SG>
 __global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, float* __restrict__ dst,
                                                  const uint32_t* __restrict__ occ, int X, int Y, int Z, int XW,
                                                  int zc, int ntx, int nty, int ntz, StencilParams p) {
   constexpr int TY = 4 * RY;
   constexpr int LW = 66;
-  constexpr int S = MARL_S_EXT;
+  constexpr int S = SG;              // species handled by this block
+  constexpr int NG = MARL_S_EXT / SG;
+  constexpr bool OWN7 = false;       // species 7 is in the last group only when SG == 12 (see s7 below)
+  constexpr int QN = PD + 2;  // own planes held: z-1, z, z+1 .. z+PD
+  constexpr int HN = PD + 1;  // halo planes held: z .. z+PD
   // one spare LDS row (TY + 2) and spare mask slots take the writes of lanes
   // that own no halo value, so those writes need no branch
   __shared__ float lds[S][TY + 3][LW];
   __shared__ uint32_t mlds[(TY + 2) * 4 + 256];
 
   // XCD-aware tile order: hardware dispatches block b to XCD b % 8; give each
-  // XCD a contiguous run of logical tiles so y-neighbour tiles (which share
-  // halo rows) sit in the same L2.
-  const int nb = ntx * nty * ntz;
+  // XCD a contiguous run of logical tiles so x/y-neighbour tiles (which share
+  // halo lines) sit in the same L2 and march through z together.
+  const int nb = ntx * nty * ntz * NG;
   int b = blockIdx.x;
   const int nb8 = nb & ~7;
   if (b < nb8) b = (b & 7) * (nb8 >> 3) + (b >> 3);
+  // species group fastest: the groups of one tile run side by side on one XCD
+  const int grp = b % NG;
+  b /= NG;
   const int tx = b % ntx;
   const int ty = (b / ntx) % nty;
   const int tz = b / (ntx * nty);
+  const int s0 = grp * SG;          // first species of this block
+  const bool has7 = s0 <= 7 && 7 < s0 + SG;
+  (void)OWN7;
 
   const int tid = threadIdx.x;
   const int lx = tid & 63, wy = tid >> 6;
@@ -95,13 +115,13 @@ __global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, 
   const uint32_t XY = (uint32_t)X * (uint32_t)Y;
   const uint32_t XYb = XY * 4u, Pb = XYb * (uint32_t)Z;
   const uint32_t XWb = (uint32_t)XW * 4u, MZb = XWb * (uint32_t)Y;
-  const rsrc_t rs = make_rsrc(src, Pb * S);
-  const rsrc_t rd = make_rsrc(dst, Pb * S);
+  const rsrc_t rs = make_rsrc(src, Pb * MARL_S_EXT);
+  const rsrc_t rd = make_rsrc(dst, Pb * MARL_S_EXT);
   const rsrc_t rm = make_rsrc(occ, MZb * (uint32_t)Z);
 
-  // Every load below is unconditional: lanes outside the grid (partial tiles)
-  // read voxel 0 of the plane instead, a value they never use, so every access
-  // stays inside the buffer and the loop body has no branches around loads.
+  // All loads are unconditional per lane: lanes outside the grid (partial
+  // tiles) read element 0 of the plane, a value they never use, so every access
+  // stays inside its buffer and there are no branches around loads.
   bool valid[RY];
   int yr[RY];
   uint32_t vo[RY], mo[RY];
@@ -112,85 +132,120 @@ __global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, 
     vo[r] = valid[r] ? ((uint32_t)yr[r] * (uint32_t)X + (uint32_t)x) * 4u : 0u;
     mo[r] = valid[r] ? (uint32_t)yr[r] * XWb + (uint32_t)(x >> 5) * 4u : 0u;
   }
-  // halo source offset of this thread: rows for waves 0-1, columns for waves 2-3
+  // halo slot of this lane: rows y0-1 / y0+TY for waves 0-1, columns x0-1 /
+  // x0+64 for the first TY lanes of waves 2-3; other lanes park in the spare row
   int hrow, hcol;
   uint32_t hvo;
-  bool hw;  // lanes that own a halo slot
+  bool hw;
   if (wy < 2) {
     const int yy = (wy == 0) ? y0 - 1 : y0 + TY;
+    hw = yy >= 0 && yy < Y && xin;
     hrow = (wy == 0) ? 0 : TY + 1;
     hcol = lx + 1;
-    hw = yy >= 0 && yy < Y && xin;
     hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)x) * 4u : 0u;
   } else {
     const int xx = (wy == 2) ? x0 - 1 : x0 + 64;
     const int yy = y0 + (lx < TY ? lx : 0);
+    hw = lx < TY && xx >= 0 && xx < X && yy < Y;
     hrow = (lx < TY ? lx : 0) + 1;
     hcol = (wy == 2) ? 0 : LW - 1;
-    hw = lx < TY && xx >= 0 && xx < X && yy < Y;
-    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)xx) * 4u : 0u;
+    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)(xx < 0 ? 0 : xx)) * 4u : 0u;
   }
-  // mask words of plane z for rows y0-1 .. y0+TY, word columns x0/32-1 .. +2
+  if (!hw) {
+    hrow = TY + 2;
+    hcol = lx;
+  }
+  // mask words of a plane for rows y0-1 .. y0+TY, word columns x0/32-1 .. +2
   const int mrow = tid >> 2, mwc = tid & 3;
   const int myy = y0 - 1 + mrow, mwx = (x0 >> 5) - 1 + mwc;
   const bool mw = tid < (TY + 2) * 4;
   const bool mvalid = mw && myy >= 0 && myy < Y && mwx >= 0 && mwx < XW;
   const uint32_t mvo = mvalid ? (uint32_t)myy * XWb + (uint32_t)mwx * 4u : 0u;
   const int mslot = mw ? tid : (TY + 2) * 4 + tid;
-  if (!hw) {  // park the halo write of lanes without a halo slot in the spare row
-    hrow = TY + 2;
-    hcol = lx;
-  }
 
-  auto ldq = [&](float (&q)[RY][S], uint32_t (&m)[RY], int zz) {
-    // planes beyond the grid are never consumed; clamp the offset into range
-    const int zl = zz < Z ? zz : Z - 1;
+  float q[QN][RY][S];   // q[0] = z-1, q[1] = z, q[2] = z+1, ...
+  uint32_t qm[QN][RY];
+  float h[HN][S];       // h[0] = halo of plane z, ...
+  uint32_t hm[HN];      // this lane's mask word of plane z, ...
+  float q7[HN][RY];     // species 7 of planes z .. z+PD for the niche factor (groups without it)
+
+  auto load_own = [&](int k, int zz) {
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
 #pragma unroll
-      for (int s = 0; s < S; ++s) q[r][s] = ld_field<SRC>(rs, p, s, Pb, XYb, Z, vo[r], zl);
-      m[r] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mo[r], (int)((uint32_t)zl * MZb), 0);
+      for (int s = 0; s < S; ++s) q[k][r][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, vo[r], zz);
+      qm[k][r] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mo[r], (int)((uint32_t)zz * MZb), 0);
     }
   };
-
-  float prv[RY][S], cur[RY][S], nxt[RY][S], nn[RY][S];
-  uint32_t mp[RY], mc[RY], mn[RY], mnn[RY];
-  ldq(cur, mc, z0);
-  ldq(nxt, mn, z0 + 1);
-  ldq(nn, mnn, z0 + 2);
-  if (z0 > 0) {
-    ldq(prv, mp, z0 - 1);
-  } else {
+  auto load_halo = [&](int k, int zz) {
 #pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      mp[r] = 0u;
+    for (int s = 0; s < S; ++s) h[k][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, hvo, zz);
+    hm[k] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zz * MZb), 0);
+  };
+  auto load_s7 = [&](int k, int zz) {
+    if (SG < MARL_S_EXT && !has7) {
 #pragma unroll
-      for (int s = 0; s < S; ++s) prv[r][s] = 0.0f;
+      for (int r = 0; r < RY; ++r) q7[k][r] = ld_field<SRC>(rs, p, 7, Pb, XYb, Z, vo[r], zz);
     }
+  };
+  // prologue: planes z0-1 .. z0+PD (own) and z0 .. z0+PD (halo), clamped into the grid
+#pragma unroll
+  for (int k = 0; k < QN; ++k) {
+    const int zz = z0 - 1 + k;
+    load_own(k, zz < 0 ? 0 : (zz < Z ? zz : Z - 1));
+  }
+#pragma unroll
+  for (int k = 0; k < HN; ++k) {
+    const int zz = z0 + k;
+    load_halo(k, zz < Z ? zz : Z - 1);
+    load_s7(k, zz < Z ? zz : Z - 1);
   }
 
   for (int z = z0; z < z1; ++z) {
-    // halo values and mask words of plane z (issued first, waited on first)
-    float hv[S];
-    const int zl = z;
+    // prefetch plane z+PD+1: own rows up to z1 (the last zp neighbour),
+    // halo up to z1-1; the condition is uniform across the block
+    const int zf = z + PD + 1;
+    float qn[RY][S];
+    uint32_t qmn[RY];
+    float hn[S];
+    float q7n[RY];
+    uint32_t hmn = 0u;
+    if (zf <= z1 && zf < Z) {
 #pragma unroll
-    for (int s = 0; s < S; ++s) hv[s] = ld_field<SRC>(rs, p, s, Pb, XYb, Z, hvo, zl);
-    const uint32_t mword = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zl * MZb), 0);
-    // keep the halo loads ahead of the prefetch so the wait before the barrier
-    // leaves the prefetch in flight
-    __builtin_amdgcn_sched_barrier(0);
-    // prefetch plane z+3
-    float n3[RY][S];
-    uint32_t mn3[RY];
-    ldq(n3, mn3, z + 3);
-    // plane z into LDS
+      for (int r = 0; r < RY; ++r) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) qn[r][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, vo[r], zf);
+        qmn[r] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mo[r], (int)((uint32_t)zf * MZb), 0);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        qmn[r] = 0u;
+#pragma unroll
+        for (int s = 0; s < S; ++s) qn[r][s] = 0.0f;
+      }
+    }
+    if (zf < z1) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) hn[s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, hvo, zf);
+      hmn = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zf * MZb), 0);
+#pragma unroll
+      for (int r = 0; r < RY; ++r)
+        q7n[r] = (SG < MARL_S_EXT && !has7) ? ld_field<SRC>(rs, p, 7, Pb, XYb, Z, vo[r], zf) : 0.0f;
+    } else {
+#pragma unroll
+      for (int s = 0; s < S; ++s) hn[s] = 0.0f;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) q7n[r] = 0.0f;
+    }
+    // plane z into LDS (own values, halo values loaded PD+1 iterations ago)
 #pragma unroll
     for (int r = 0; r < RY; ++r)
 #pragma unroll
-      for (int s = 0; s < S; ++s) lds[s][wy + 4 * r + 1][lx + 1] = cur[r][s];
+      for (int s = 0; s < S; ++s) lds[s][wy + 4 * r + 1][lx + 1] = q[1][r][s];
 #pragma unroll
-    for (int s = 0; s < S; ++s) lds[s][hrow][hcol] = hv[s];
-    mlds[mslot] = mvalid ? mword : 0u;
+    for (int s = 0; s < S; ++s) lds[s][hrow][hcol] = h[0][s];
+    mlds[mslot] = mvalid ? hm[0] : 0u;
     __syncthreads();
 
 #pragma unroll
@@ -205,20 +260,21 @@ __global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, 
       // neighbour flags with bitwise operators: every mask read is unconditional
       const uint32_t bc = bitat(row, lx), bxm = bitat(row, lx - 1), bxp = bitat(row, lx + 1);
       const uint32_t bym = bitat(row - 1, lx), byp = bitat(row + 1, lx);
-      const uint32_t bzm = (mp[r] >> (x & 31)) & 1u, bzp = (mn[r] >> (x & 31)) & 1u;
+      const uint32_t bzm = (qm[0][r] >> (x & 31)) & 1u, bzp = (qm[2][r] >> (x & 31)) & 1u;
       const bool vxm = (uint32_t)(x > 0) & (bxm ^ 1u);
       const bool vxp = (uint32_t)(x < X - 1) & (bxp ^ 1u);
       const bool vym = (uint32_t)(y > 0) & (bym ^ 1u);
       const bool vyp = (uint32_t)(y < Y - 1) & (byp ^ 1u);
       const bool vzm = (uint32_t)(z > 0) & (bzm ^ 1u);
       const bool vzp = (uint32_t)(z < Z - 1) & (bzp ^ 1u);
-      const float structural = cur[r][7];
+      const float structural = (SG == MARL_S_EXT || has7) ? q[1][r][7 - (SG == MARL_S_EXT ? 0 : MARL_S_EXT - SG)] : q7[0][r];
       const float niche = 1.0f - p.alpha_eps * structural / (p.k_eps + structural);
+      const uint32_t keep = 0u - bc;  // occupied centre: copied unchanged
       float out[S];
 #pragma unroll
       for (int s = 0; s < S; ++s) {
-        const float c = cur[r][s];
-        const float dd = p.d_voxel[s] * niche;
+        const float c = q[1][r][s];
+        const float dd = p.d_voxel[s0 + s] * niche;
         const float lxm = lds[s][row][lx];
         const float lxp = lds[s][row][lx + 2];
         const float lym = lds[s][row - 1][lx + 1];
@@ -227,50 +283,70 @@ __global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, 
         const float xp = vxp ? lxp : c;
         const float ym = vym ? lym : c;
         const float yp = vyp ? lyp : c;
-        const float zm = vzm ? prv[r][s] : c;
-        const float zp = vzp ? nxt[r][s] : c;
+        const float zm = vzm ? q[0][r][s] : c;
+        const float zp = vzp ? q[2][r][s] : c;
         const float lap = xm + xp + ym + yp + zm + zp - 6.0f * c;
-        const float decay = p.lambda_decay[s] * c;
+        const float decay = p.lambda_decay[s0 + s] * c;
         const float nc = c + p.dt_sub * (dd * lap - decay);
-        // occupied centre keeps c; a bit select keeps the compiler from
-        // turning this into a branch around the neighbour reads
-        const uint32_t keep = 0u - bc;
+        // a bit select keeps the compiler from branching around the reads
         out[s] = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, c) & keep) |
-                                              (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
+                                               (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
       }
       if (valid[r]) {
 #pragma unroll
-        for (int s = 0; s < S; ++s) bstore(rd, out[s], vo[r], (uint32_t)s * Pb + zo);
+        for (int s = 0; s < S; ++s) bstore_nt(rd, out[s], vo[r], (uint32_t)(s0 + s) * Pb + zo);
       }
     }
     __syncthreads();
+    // rotate the queues
+#pragma unroll
+    for (int k = 0; k + 1 < QN; ++k)
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        qm[k][r] = qm[k + 1][r];
+#pragma unroll
+        for (int s = 0; s < S; ++s) q[k][r][s] = q[k + 1][r][s];
+      }
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
+      qm[QN - 1][r] = qmn[r];
 #pragma unroll
-      for (int s = 0; s < S; ++s) {
-        prv[r][s] = cur[r][s];
-        cur[r][s] = nxt[r][s];
-        nxt[r][s] = nn[r][s];
-        nn[r][s] = n3[r][s];
-      }
-      mp[r] = mc[r];
-      mc[r] = mn[r];
-      mn[r] = mnn[r];
-      mnn[r] = mn3[r];
+      for (int s = 0; s < S; ++s) q[QN - 1][r][s] = qn[r][s];
     }
+#pragma unroll
+    for (int k = 0; k + 1 < HN; ++k) {
+      hm[k] = hm[k + 1];
+#pragma unroll
+      for (int s = 0; s < S; ++s) h[k][s] = h[k + 1][s];
+    }
+    hm[HN - 1] = hmn;
+#pragma unroll
+    for (int s = 0; s < S; ++s) h[HN - 1][s] = hn[s];
+#pragma unroll
+    for (int k = 0; k + 1 < HN; ++k)
+#pragma unroll
+      for (int r = 0; r < RY; ++r) q7[k][r] = q7[k + 1][r];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) q7[HN - 1][r] = q7n[r];
   }
 }
 
-template <int RY, bool SRC>
+template <int RY, bool SRC, int PD, int SG>
 static hipError_t launch_stencil_t(marl_ctx* ctx, const StencilParams& p, int zc) {
   const int X = ctx->d.x, Y = ctx->d.y, Z = ctx->d.z;
   const int ntx = (X + 63) / 64, nty = (Y + 4 * RY - 1) / (4 * RY), ntz = (Z + zc - 1) / zc;
-  const int nb = ntx * nty * ntz;
+  const int nb = ntx * nty * ntz * (MARL_S_EXT / SG);
   const float* src = ctx->field[ctx->cur];
   float* dst = ctx->field[ctx->cur ^ 1];
-  hipLaunchKernelGGL((k_stencil<RY, SRC>), dim3(nb), dim3(256), 0, ctx->stream, src, dst, ctx->occ, X, Y, Z,
+  hipLaunchKernelGGL((k_stencil<RY, SRC, PD, SG>), dim3(nb), dim3(256), 0, ctx->stream, src, dst, ctx->occ, X, Y, Z,
                      ctx->xw, zc, ntx, nty, ntz, p);
   return hipGetLastError();
+}
+
+template <int RY, int PD, int SG>
+static hipError_t launch_stencil_s(marl_ctx* ctx, const StencilParams& p, int zc) {
+  return p.apply_sources ? launch_stencil_t<RY, true, PD, SG>(ctx, p, zc)
+                         : launch_stencil_t<RY, false, PD, SG>(ctx, p, zc);
 }
 
 int marl_launch_substep(marl_ctx* ctx, const StencilParams& p, int tile) {
@@ -278,11 +354,13 @@ int marl_launch_substep(marl_ctx* ctx, const StencilParams& p, int tile) {
   const TileShape t = kTiles[tile];
   const int zc = t.zc < ctx->d.z ? t.zc : ctx->d.z;
   hipError_t e = hipErrorInvalidValue;
-  const bool src = p.apply_sources != 0;
-  switch (t.ry) {
-    case 1: e = src ? launch_stencil_t<1, true>(ctx, p, zc) : launch_stencil_t<1, false>(ctx, p, zc); break;
-    case 2: e = src ? launch_stencil_t<2, true>(ctx, p, zc) : launch_stencil_t<2, false>(ctx, p, zc); break;
-    case 4: e = src ? launch_stencil_t<4, true>(ctx, p, zc) : launch_stencil_t<4, false>(ctx, p, zc); break;
+  if (t.sg == 12) {
+    if (t.ry == 1 && t.pd == 1) e = launch_stencil_s<1, 1, 12>(ctx, p, zc);
+    else if (t.ry == 2 && t.pd == 1) e = launch_stencil_s<2, 1, 12>(ctx, p, zc);
+  } else if (t.sg == 6) {
+    if (t.ry == 2 && t.pd == 1) e = launch_stencil_s<2, 1, 6>(ctx, p, zc);
+    else if (t.ry == 2 && t.pd == 2) e = launch_stencil_s<2, 2, 6>(ctx, p, zc);
+    else if (t.ry == 4 && t.pd == 1) e = launch_stencil_s<4, 1, 6>(ctx, p, zc);
   }
   MARL_HIP(ctx, e);
   ctx->cur ^= 1;

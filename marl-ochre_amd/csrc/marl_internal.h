@@ -237,7 +237,7 @@ int marl_ensure_staging(marl_ctx* ctx);
 int marl_launch_copy_probe(marl_ctx* ctx);
 
 // stencil tiles: rows per thread and z-chunk
-struct TileShape { int ry, zc; };
+struct TileShape { int ry, zc, pd, sg; };
 extern const TileShape kTiles[];
 extern const int kNumTiles;
 int marl_default_tile(const marl_dims& d);

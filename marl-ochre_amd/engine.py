@@ -145,11 +145,12 @@ class MarlGpu:
     def tile(self, t: int) -> None:
         self._chk(self.lib.marl_set_stencil_tile(self.h, int(t)))
 
-    def tile_shape(self, t: int | None = None) -> tuple[int, int]:
+    def tile_shape(self, t: int | None = None) -> tuple[int, int, int]:
+        """(rows per thread, z-chunk, prefetch distance) of a stencil tile id."""
         t = self.tile if t is None else t
-        ry, zc = ctypes.c_int32(), ctypes.c_int32()
-        self._chk(self.lib.marl_stencil_tile_shape(int(t), ctypes.byref(ry), ctypes.byref(zc)))
-        return ry.value, zc.value
+        ry, zc, pd = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self.lib.marl_stencil_tile_shape(int(t), ctypes.byref(ry), ctypes.byref(zc), ctypes.byref(pd)))
+        return ry.value, zc.value, pd.value
 
     def sync(self) -> None:
         self._chk(self.lib.marl_sync(self.h))

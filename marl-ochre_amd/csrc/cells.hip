@@ -109,17 +109,23 @@ __global__ void k_scatter_cells(const marl_cell_state* __restrict__ cells, int n
 // Conflicting partner at L1 distance 2 with a lower index that shares at least
 // one free face neighbour.
 __global__ void k_cell_prep(const marl_cell_state* __restrict__ cells, int n, const int32_t* __restrict__ cell_at,
-                            uint8_t* __restrict__ nbmask, int32_t* __restrict__ preds, uint8_t* __restrict__ npred,
-                            int32_t* __restrict__ level, CellParams p) {
+                            const uint32_t* __restrict__ occ, uint8_t* __restrict__ nbmask,
+                            int32_t* __restrict__ preds, uint8_t* __restrict__ npred, int32_t* __restrict__ level,
+                            CellParams p) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint16_t* ps = cells[j].pos;
   const int x = ps[0], y = ps[1], z = ps[2];
+  // emptiness from the 1-bit mask (L2-resident); cell_at only for occupied voxels
+  auto occupied = [&](int ox, int oy, int oz) -> bool {
+    const int64_t w = vox(ox, oy, oz, p);
+    return (occ[w >> 5] >> (w & 31)) & 1u;
+  };
   uint32_t m = 0;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const int nx = x + kFace[k][0], ny = y + kFace[k][1], nz = z + kFace[k][2];
-    if (inb(nx, ny, nz, p) && cell_at[vox(nx, ny, nz, p)] < 0) m |= 1u << k;
+    if (inb(nx, ny, nz, p) && !occupied(nx, ny, nz)) m |= 1u << k;
   }
   nbmask[j] = (uint8_t)m;
   int np = 0;
@@ -129,7 +135,7 @@ __global__ void k_cell_prep(const marl_cell_state* __restrict__ cells, int n, co
   for (int k = 0; k < 6; ++k) {
     if (!((m >> k) & 1u)) continue;
     const int ox = x + 2 * kFace[k][0], oy = y + 2 * kFace[k][1], oz = z + 2 * kFace[k][2];
-    if (!inb(ox, oy, oz, p)) continue;
+    if (!inb(ox, oy, oz, p) || !occupied(ox, oy, oz)) continue;
     const int32_t i = cell_at[vox(ox, oy, oz, p)];
     if (i >= 0 && i < j) pj[np++] = i;
   }
@@ -142,7 +148,7 @@ __global__ void k_cell_prep(const marl_cell_state* __restrict__ cells, int n, co
       if (!(((m >> a) | (m >> b)) & 1u)) continue;  // neither corner free
       const int ox = x + kFace[a][0] + kFace[b][0], oy = y + kFace[a][1] + kFace[b][1],
                 oz = z + kFace[a][2] + kFace[b][2];
-      if (!inb(ox, oy, oz, p)) continue;
+      if (!inb(ox, oy, oz, p) || !occupied(ox, oy, oz)) continue;
       const int32_t i = cell_at[vox(ox, oy, oz, p)];
       if (i >= 0 && i < j) pj[np++] = i;
     }
@@ -215,8 +221,11 @@ __global__ void __launch_bounds__(256) k_level_scatter(int n, const int32_t* __r
 // K5/K6: one level of the cell pass.  Per cell: mean environment of the free
 // face neighbours (spatial.rs:56-77), CellState::tick (cell.rs:193-369), then
 // split deltas with the max(0) clamp into those neighbours (spatial.rs:87-110,
-// field.rs:57-62).  Dynamically indexed per-cell arrays live in LDS (one
-// private slot per thread, odd strides), so nothing goes to scratch.
+// field.rs:57-62).  The field is read and written through an AoS view (one
+// 48-byte record per voxel, transposed around the pass) so a neighbour costs
+// three float4 accesses instead of twelve scattered floats.  Dynamically
+// indexed per-cell arrays live in LDS (one private slot per thread, odd
+// strides), so nothing goes to scratch.
 constexpr int kKinBlock = 128;
 
 __global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restrict__ order, int count,
@@ -241,7 +250,7 @@ __global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restric
   for (int k = 0; k < MARL_M_INT; ++k) I[k] = c.internal[k];
 
   const int x = c.pos[0], y = c.pos[1], z = c.pos[2];
-  const size_t XY = (size_t)p.X * p.Y, P = XY * p.Z;
+  const size_t XY = (size_t)p.X * p.Y;
   const size_t v = ((size_t)z * p.Y + y) * p.X + x;
   const uint32_t m = nbmask[i];
   const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
@@ -251,12 +260,20 @@ __global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restric
   for (int k = 0; k < 6; ++k)
     if ((m >> k) & 1u) nv[n++] = (size_t)((int64_t)v + off[k]);
 
-  // environment (sum in face order, then divide)
+  // environment from the AoS view (3 float4 per voxel): sum in face order, then divide
+  float acc[MARL_S_EXT];
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) acc[s] = 0.0f;
+  for (int k = 0; k < n; ++k) {
+    const float4* fv = reinterpret_cast<const float4*>(field + nv[k] * MARL_S_EXT);
+    const float4 a = fv[0], b = fv[1], cc = fv[2];
+    acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
+    acc[4] += b.x; acc[5] += b.y; acc[6] += b.z; acc[7] += b.w;
+    acc[8] += cc.x; acc[9] += cc.y; acc[10] += cc.z; acc[11] += cc.w;
+  }
 #pragma unroll
   for (int s = 0; s < MARL_S_EXT; ++s) {
-    float acc = 0.0f;
-    for (int k = 0; k < n; ++k) acc += field[(size_t)s * P + nv[k]];
-    E[s] = n ? acc / (float)n : 0.0f;
+    E[s] = n ? acc[s] / (float)n : 0.0f;
     D[s] = 0.0f;
   }
   const float lt = light[v];
@@ -360,16 +377,22 @@ __global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restric
   cells[i] = c;
   events[i] = ev;
 
-  // deltas to the free neighbours (spatial.rs:87-110)
+  // deltas to the free neighbours (spatial.rs:87-110), on the AoS view
   if (n > 0) {
     const float nf = (float)n;
+    float split[MARL_S_EXT];
 #pragma unroll
-    for (int s = 0; s < MARL_S_EXT; ++s) {
-      const float split = D[s] / nf;
-      for (int k = 0; k < n; ++k) {
-        float* fp = field + (size_t)s * P + nv[k];
-        *fp = fmaxf(*fp + split, 0.0f);
-      }
+    for (int s = 0; s < MARL_S_EXT; ++s) split[s] = D[s] / nf;
+    for (int k = 0; k < n; ++k) {
+      float4* fv = reinterpret_cast<float4*>(field + nv[k] * MARL_S_EXT);
+      float4 a = fv[0], b = fv[1], cc = fv[2];
+      a.x = fmaxf(a.x + split[0], 0.0f); a.y = fmaxf(a.y + split[1], 0.0f);
+      a.z = fmaxf(a.z + split[2], 0.0f); a.w = fmaxf(a.w + split[3], 0.0f);
+      b.x = fmaxf(b.x + split[4], 0.0f); b.y = fmaxf(b.y + split[5], 0.0f);
+      b.z = fmaxf(b.z + split[6], 0.0f); b.w = fmaxf(b.w + split[7], 0.0f);
+      cc.x = fmaxf(cc.x + split[8], 0.0f); cc.y = fmaxf(cc.y + split[9], 0.0f);
+      cc.z = fmaxf(cc.z + split[10], 0.0f); cc.w = fmaxf(cc.w + split[11], 0.0f);
+      fv[0] = a; fv[1] = b; fv[2] = cc;
     }
   }
 }
@@ -547,11 +570,13 @@ __global__ void k_death_flags(int n, const uint8_t* __restrict__ events, int32_t
 // m_j = n - (P[n] - P[lo_j]) and lo_{j+1} = m_j.  Output slot s < m_0 holds
 // s itself if it survives; otherwise the q-th smallest head hole of level j
 // (q = P[s] - P[lo_j]) receives what lands in slot m_j + q of level j+1.
-__global__ void k_compact_gather(int n, int m0, const uint8_t* __restrict__ events, const int32_t* __restrict__ P,
-                                 const marl_cell_state* __restrict__ src_c, const marl_ruleset* __restrict__ src_r,
-                                 marl_cell_state* __restrict__ dst_c, marl_ruleset* __restrict__ dst_r) {
+// Survivors below m_0 keep their slot, so only the head holes are written:
+// each one's source (a tail survivor at or past m_0) is found here and the
+// records are moved in place by k_move_records.
+__global__ void k_hole_sources(int n, int m0, const uint8_t* __restrict__ events, const int32_t* __restrict__ P,
+                               int32_t* __restrict__ hole_dst, int32_t* __restrict__ hole_src) {
   const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s0 >= m0) return;
+  if (s0 >= m0 || events[s0] != MARL_EVENT_DEATH) return;
   const int total = P[n];
   int s = s0, lo = 0, m = m0;
   while (events[s] == MARL_EVENT_DEATH) {
@@ -561,17 +586,41 @@ __global__ void k_compact_gather(int n, int m0, const uint8_t* __restrict__ even
     lo = nlo;
     m = n - (total - P[lo]);
   }
-  dst_c[s0] = src_c[s];
-  dst_r[s0] = src_r[s];
+  const int h = P[s0];  // every death before s0 is a head hole too
+  hole_dst[h] = s0;
+  hole_src[h] = s;
 }
 
-__global__ void k_append_births(int nb, int base, const int32_t* __restrict__ win, const int32_t* __restrict__ wrank,
-                                const marl_cell_state* __restrict__ bc, const marl_ruleset* __restrict__ br,
-                                marl_cell_state* __restrict__ dst_c, marl_ruleset* __restrict__ dst_r) {
+// Moves cell + ruleset records in 16-byte chunks, one chunk per lane, so a
+// record's 42 chunks are copied by consecutive lanes.
+constexpr int kCellChunks = sizeof(marl_cell_state) / 16;
+constexpr int kRuleChunks = sizeof(marl_ruleset) / 16;
+constexpr int kRecChunks = kCellChunks + kRuleChunks;
+static_assert(sizeof(marl_cell_state) % 16 == 0 && sizeof(marl_ruleset) % 16 == 0, "record sizes");
+
+__global__ void k_move_records(int count, const int32_t* __restrict__ dst_idx, const int32_t* __restrict__ src_idx,
+                               const marl_cell_state* __restrict__ src_c, const marl_ruleset* __restrict__ src_r,
+                               marl_cell_state* __restrict__ dst_c, marl_ruleset* __restrict__ dst_r,
+                               const int32_t* __restrict__ select) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)count * kRecChunks) return;
+  const int h = (int)(t / kRecChunks), c = (int)(t % kRecChunks);
+  if (select && !select[h]) return;
+  const int d = dst_idx ? dst_idx[h] : h;
+  const int sidx = src_idx ? src_idx[h] : h;
+  if (c < kCellChunks) {
+    reinterpret_cast<uint4*>(dst_c + d)[c] = reinterpret_cast<const uint4*>(src_c + sidx)[c];
+  } else {
+    reinterpret_cast<uint4*>(dst_r + d)[c - kCellChunks] = reinterpret_cast<const uint4*>(src_r + sidx)[c - kCellChunks];
+  }
+}
+
+// Destination slot of each surviving birth: m0 + its rank among the winners.
+__global__ void k_birth_slots(int nb, int base, const int32_t* __restrict__ win, const int32_t* __restrict__ wrank,
+                              int32_t* __restrict__ slot) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nb || !win[r]) return;
-  dst_c[base + wrank[r]] = bc[r];
-  dst_r[base + wrank[r]] = br[r];
+  if (r >= nb) return;
+  slot[r] = win[r] ? base + wrank[r] : 0;
 }
 
 // K9: FP64 mass budget, deterministic two-pass reduction.
@@ -765,8 +814,8 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
   }
   marl_cell_state* cells = ctx->cells[ctx->cbuf];
   const marl_ruleset* rules = ctx->rules[ctx->cbuf];
-  hipLaunchKernelGGL(k_cell_prep, dim3(grid1(n)), dim3(256), 0, ctx->stream, cells, n, ctx->cell_at, ctx->nbmask,
-                     ctx->preds, ctx->npred, ctx->level, p);
+  hipLaunchKernelGGL(k_cell_prep, dim3(grid1(n)), dim3(256), 0, ctx->stream, cells, n, ctx->cell_at, ctx->occ,
+                     ctx->nbmask, ctx->preds, ctx->npred, ctx->level, p);
   MARL_HIP(ctx, hipGetLastError());
   int32_t* changed = ctx->misc;
   int32_t* h = ctx->pinned;
@@ -802,7 +851,10 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
   MARL_HIP(ctx, hipGetLastError());
   MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));  // `off` must outlive the copy
   if (ev_sched) MARL_HIP(ctx, hipEventRecord(ev_sched, ctx->stream));
-  float* field = ctx->field[ctx->cur];
+  // AoS view of the field for the pass (the resident layout is SoA planes)
+  if (int rc = marl_ensure_staging(ctx)) return rc;
+  if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
+  float* field = ctx->staging;
   for (int L = 0; L < nlev; ++L) {
     const int cnt = off[L + 1] - off[L];
     if (cnt == 0) continue;
@@ -810,6 +862,7 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
                        ctx->order + off[L], cnt, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p);
   }
   MARL_HIP(ctx, hipGetLastError());
+  if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
   if (levels_out) *levels_out = nlev;
   return MARL_OK;
 }
@@ -859,20 +912,32 @@ int run_fate(marl_ctx* ctx, const CellParams& p, marl_tick_stats* st) {
   }
   const int m0 = n - nd;
   const int64_t n_new = (int64_t)m0 + nw;
-  // the destination buffer must hold n_new cells
-  marl_cell_state* dc = ctx->cells[ctx->cbuf ^ 1];
-  marl_ruleset* dr = ctx->rules[ctx->cbuf ^ 1];
-  if (m0 > 0) {
-    hipLaunchKernelGGL(k_compact_gather, dim3(grid1(m0)), dim3(256), 0, ctx->stream, n, m0, ctx->events, P, cells,
-                       rules, dc, dr);
+  // deaths: fill the head holes in place from the tail (lib.rs:257-267)
+  if (nd > 0 && m0 > 0) {
+    // scratch reuse: per-cell targets and per-birth targets are consumed by now
+    int32_t* hole_dst = btarget;
+    int32_t* hole_src = target;
+    // hole count = deaths below m0 = P[m0]
+    if (int rc = d2h_sync(ctx, h + 3, P + m0, sizeof(int32_t))) return rc;
+    const int holes = h[3];
+    if (holes > 0) {
+      hipLaunchKernelGGL(k_hole_sources, dim3(grid1(m0)), dim3(256), 0, ctx->stream, n, m0, ctx->events, P,
+                         hole_dst, hole_src);
+      MARL_HIP(ctx, hipGetLastError());
+      hipLaunchKernelGGL(k_move_records, dim3(grid1((int64_t)holes * kRecChunks)), dim3(256), 0, ctx->stream,
+                         holes, hole_dst, hole_src, cells, rules, cells, rules, nullptr);
+      MARL_HIP(ctx, hipGetLastError());
+    }
+  }
+  // births appended in event order after the survivors (lib.rs:270-277)
+  if (nb > 0 && nw > 0) {
+    int32_t* slot = bflag;  // birth flags were consumed by the rank scan
+    hipLaunchKernelGGL(k_birth_slots, dim3(grid1(nb)), dim3(256), 0, ctx->stream, nb, m0, win, wrank, slot);
+    MARL_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_move_records, dim3(grid1((int64_t)nb * kRecChunks)), dim3(256), 0, ctx->stream, nb, slot,
+                       nullptr, ctx->birth_cells, ctx->birth_rules, cells, rules, win);
     MARL_HIP(ctx, hipGetLastError());
   }
-  if (nb > 0) {
-    hipLaunchKernelGGL(k_append_births, dim3(grid1(nb)), dim3(256), 0, ctx->stream, nb, m0, win, wrank,
-                       ctx->birth_cells, ctx->birth_rules, dc, dr);
-    MARL_HIP(ctx, hipGetLastError());
-  }
-  ctx->cbuf ^= 1;
   ctx->n_cells = n_new;
   if (st) {
     st->divisions = nb;

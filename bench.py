@@ -203,10 +203,14 @@ def main():
     log(f"stencil {per_launch_s * 1e6:.1f} us/substep ({achieved:.0f} GB/s algorithmic), "
         f"light {light_ms * 1e3 / args.steps:.1f} us/tick")
 
+    # HBM bytes per launch from the committed rocprofv3 PMC summary, valid for
+    # the tile it was measured on
     traffic = None
     tf = ROOT / "profiles" / "stencil_traffic.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+        meas = json.loads(tf.read_text())
+        if meas.get("tile") == tile and tuple(meas.get("grid", ())) == dims:
+            traffic = meas.get("bytes_per_launch")
 
     out = {
         "metric": METRIC,

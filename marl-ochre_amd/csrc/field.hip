@@ -34,7 +34,7 @@ const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 int marl_default_tile(const marl_dims& d) {
   (void)d;
-  return 2;  // 2 rows per thread, z-chunk 32, prefetch 1, all species, until the autotuner says otherwise
+  return 3;  // 2 rows per thread, z-chunk 16, prefetch 1, 6 species; the autotuner may pick another
 }
 
 // Raw buffer accesses: the voxel's byte offset lives in one VGPR, the

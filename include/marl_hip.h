@@ -228,6 +228,76 @@ int marl_stencil_tile_shape(int32_t tile, int32_t* rows_per_thread, int32_t* z_c
 int marl_copy_probe(marl_ctx* ctx, int32_t n_reps, float* ms_out, double* bytes_out);
 int marl_sync(marl_ctx* ctx);
 
+/* ---- learned kinetics surrogate ------------------------------------------- */
+/* No reference counterpart: the reference integrates every cell's kinetics
+ * exactly (crates/marl-cell/src/cell.rs:223-329).  The surrogate is an MLP
+ * (80 -> 256 -> 256 -> 256 -> 28, SiLU) trained on the GPU from pairs the plain
+ * kinetics pass harvests, then used for cells whose reaction topology is one
+ * of the trained classes, whose standardised features lie within zmax and
+ * whose predicted energy is at least energy_margin from every fate
+ * threshold; all other cells run the plain path. */
+
+typedef struct {
+  int64_t steps;       /* Adam steps */
+  int32_t batch;       /* minibatch rows */
+  int32_t reinit;      /* 1: fresh He-uniform initialisation even if trained */
+  float lr, lr_final;  /* cosine schedule from lr to lr_final */
+  float zmax;          /* out-of-distribution bound (<= 0: 6) */
+  float energy_margin; /* fate margin on the predicted energy (< 0: 0.05) */
+  uint64_t seed;
+} marl_train_config;
+
+typedef struct {
+  float loss_first, loss_last, val_loss; /* MSE on standardised targets */
+  float ms;                              /* wall time of the call */
+  int64_t pairs, pairs_train;
+  int32_t classes, _pad;
+} marl_train_report;
+
+typedef struct {
+  int64_t cells;              /* cells that reached the surrogate pass */
+  int64_t surrogate;          /* advanced by the surrogate */
+  int64_t fallback_topology;  /* unknown reaction topology */
+  int64_t fallback_ood;       /* feature outside zmax */
+  int64_t fallback_margin;    /* predicted energy near a fate threshold */
+  int64_t shadow;             /* cells run both ways and compared */
+  int64_t shadow_within_1e3;  /* max relative internal error <= 1e-3 */
+  int64_t shadow_fate_agree;  /* same fate event and fate state */
+  float shadow_max_rel_err;
+  int32_t _pad;
+  int64_t err_hist[24];       /* log10 max relative error, half-decade bins from 1e-9 */
+} marl_surr_stats;
+
+/* MARL_KINETICS_PLAIN (exact), _SURROGATE (learned, with fallback) or
+ * _SHADOW (exact, with every eligible cell also predicted and compared). */
+int marl_set_kinetics_mode(marl_ctx* ctx, int32_t mode);
+/* Start appending (features, targets, topology hash) of a sample_rate share
+ * of cells from each plain kinetics pass, up to capacity pairs. */
+int marl_harvest_begin(marl_ctx* ctx, int64_t capacity, float sample_rate);
+int marl_harvest_end(marl_ctx* ctx, int64_t* n_pairs);
+/* feat [n][80], targ [n][28] (internal deltas 16, exchange fluxes 12), hash [n]. */
+int marl_harvest_download(marl_ctx* ctx, float* feat, float* targ, uint64_t* hash, size_t cap, int64_t* n_out);
+int marl_harvest_upload(marl_ctx* ctx, const float* feat, const float* targ, const uint64_t* hash, size_t n);
+int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train_report* report);
+int marl_surrogate_param_count(void);
+/* Flat FP32 parameters: per layer W [in x out] row-major then b [out];
+ * norm = feature mean[80], std[80], target mean[28], std[28]. */
+int marl_surrogate_get_params(marl_ctx* ctx, float* params, size_t n, float* norm, size_t n_norm);
+int marl_surrogate_set_params(marl_ctx* ctx, const float* params, size_t n, const float* norm, size_t n_norm,
+                              const uint64_t* classes, int32_t n_classes);
+int marl_surrogate_classes(marl_ctx* ctx, uint64_t* classes, int32_t* n);
+/* Network output for standardised inputs x [rows][80] -> y [rows][28]:
+ * fused = 1 the inference kernel, 0 the training forward. */
+int marl_surrogate_forward(marl_ctx* ctx, const float* x, size_t rows, float* y, int32_t fused);
+/* MSE loss and its gradient (flat, as the parameters) for x, targets t. */
+int marl_surrogate_grad(marl_ctx* ctx, const float* x, const float* t, size_t rows, float* loss, float* grads);
+/* One Adam step (b1 0.9, b2 0.999, eps 1e-8) with the given gradient. */
+int marl_surrogate_adam_step(marl_ctx* ctx, const float* grads, float lr);
+/* zmax, energy margin, share of surrogate cells also run exactly for monitoring. */
+int marl_surrogate_configure(marl_ctx* ctx, float zmax, float energy_margin, float shadow_rate);
+int marl_surrogate_stats(marl_ctx* ctx, marl_surr_stats* out);
+int marl_surrogate_reset_stats(marl_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

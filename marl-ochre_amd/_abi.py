@@ -81,6 +81,51 @@ class marl_tick_stats(ctypes.Structure):
     ]
 
 
+class marl_train_config(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int64),
+        ("batch", ctypes.c_int32),
+        ("reinit", ctypes.c_int32),
+        ("lr", ctypes.c_float),
+        ("lr_final", ctypes.c_float),
+        ("zmax", ctypes.c_float),
+        ("energy_margin", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+class marl_train_report(ctypes.Structure):
+    _fields_ = [
+        ("loss_first", ctypes.c_float),
+        ("loss_last", ctypes.c_float),
+        ("val_loss", ctypes.c_float),
+        ("ms", ctypes.c_float),
+        ("pairs", ctypes.c_int64),
+        ("pairs_train", ctypes.c_int64),
+        ("classes", ctypes.c_int32),
+        ("_pad", ctypes.c_int32),
+    ]
+
+
+class marl_surr_stats(ctypes.Structure):
+    _fields_ = [
+        ("cells", ctypes.c_int64),
+        ("surrogate", ctypes.c_int64),
+        ("fallback_topology", ctypes.c_int64),
+        ("fallback_ood", ctypes.c_int64),
+        ("fallback_margin", ctypes.c_int64),
+        ("shadow", ctypes.c_int64),
+        ("shadow_within_1e3", ctypes.c_int64),
+        ("shadow_fate_agree", ctypes.c_int64),
+        ("shadow_max_rel_err", ctypes.c_float),
+        ("_pad", ctypes.c_int32),
+        ("err_hist", ctypes.c_int64 * 24),
+    ]
+
+
+# surrogate network sizes (80 features -> 3 x 256 SiLU -> 28 targets)
+SURR_FEAT, SURR_HIDDEN, SURR_TARG = 80, 256, 28
+
 # numpy record layouts of marl_cell_state (96 B) and marl_ruleset (576 B)
 CELL_DTYPE = np.dtype(
     {
@@ -180,6 +225,22 @@ SIGNATURES = {
     "marl_seed_positions": (ctypes.c_int, [ctypes.POINTER(marl_dims), ctypes.c_uint64, ctypes.c_uint64, _I64, _I32,
                                            _I32, _I32, _P, _P, _P, _I64, ctypes.POINTER(_I64)]),
     "marl_sync": (ctypes.c_int, [_P]),
+    "marl_set_kinetics_mode": (ctypes.c_int, [_P, _I32]),
+    "marl_harvest_begin": (ctypes.c_int, [_P, _I64, ctypes.c_float]),
+    "marl_harvest_end": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
+    "marl_harvest_download": (ctypes.c_int, [_P, _P, _P, _P, _SZ, ctypes.POINTER(_I64)]),
+    "marl_harvest_upload": (ctypes.c_int, [_P, _P, _P, _P, _SZ]),
+    "marl_surrogate_train": (ctypes.c_int, [_P, ctypes.POINTER(marl_train_config), ctypes.POINTER(marl_train_report)]),
+    "marl_surrogate_param_count": (ctypes.c_int, []),
+    "marl_surrogate_get_params": (ctypes.c_int, [_P, _P, _SZ, _P, _SZ]),
+    "marl_surrogate_set_params": (ctypes.c_int, [_P, _P, _SZ, _P, _SZ, _P, _I32]),
+    "marl_surrogate_classes": (ctypes.c_int, [_P, _P, ctypes.POINTER(_I32)]),
+    "marl_surrogate_forward": (ctypes.c_int, [_P, _P, _SZ, _P, _I32]),
+    "marl_surrogate_grad": (ctypes.c_int, [_P, _P, _P, _SZ, ctypes.POINTER(ctypes.c_float), _P]),
+    "marl_surrogate_adam_step": (ctypes.c_int, [_P, _P, ctypes.c_float]),
+    "marl_surrogate_configure": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "marl_surrogate_stats": (ctypes.c_int, [_P, ctypes.POINTER(marl_surr_stats)]),
+    "marl_surrogate_reset_stats": (ctypes.c_int, [_P]),
 }
 
 _lib = None

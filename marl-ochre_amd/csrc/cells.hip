@@ -19,21 +19,14 @@
 #include <cstring>
 #include <vector>
 
+#include "kinetics.h"
 #include "marl_internal.h"
+#include "surrogate.h"
 
 namespace {
 
 constexpr int kMaxLevels = 4096;
 constexpr int kMaxPreds = 18;
-
-struct CellParams {
-  float dt, epsilon, c_max, lambda_maintenance, hard_death_floor, reaction_maintenance;
-  float base_division_prep, prep_maintenance_multiplier, rush_penalty_rate, active_reaction_threshold;
-  float mutation_stddev, structural_mutation_rate_mult, meta_mutation_rate, meta_lo, meta_hi, hill_lo, hill_hi;
-  int32_t div_dist;
-  int32_t X, Y, Z;
-  uint64_t seed, stream;
-};
 
 CellParams cell_params(const marl_ctx* ctx, const marl_sim_params* s, uint64_t tick) {
   CellParams p;
@@ -61,18 +54,6 @@ CellParams cell_params(const marl_ctx* ctx, const marl_sim_params* s, uint64_t t
   p.seed = s->rng_seed;
   p.stream = (MARL_TAG_DIVISION << 32) | (tick & 0xffffffffull);
   return p;
-}
-
-__host__ __device__ inline float clamp_rs(float x, float lo, float hi) {  // Rust f32::clamp
-  if (x < lo) x = lo;
-  if (x > hi) x = hi;
-  return x;
-}
-
-__device__ inline uint16_t f32_as_u16(float f) {  // Rust `as u16`: truncate, saturate, NaN -> 0
-  if (!(f > 0.0f)) return 0;
-  if (f >= 65535.0f) return 65535;
-  return (uint16_t)f;
 }
 
 // FACE_OFFSETS (spatial.rs:8-15): +x, -x, +y, -y, +z, -z
@@ -225,7 +206,8 @@ __global__ void __launch_bounds__(256) k_level_scatter(int n, const int32_t* __r
 // 48-byte record per voxel, transposed around the pass) so a neighbour costs
 // three float4 accesses instead of twelve scattered floats.  Dynamically
 // indexed per-cell arrays live in LDS (one private slot per thread, odd
-// strides), so nothing goes to scratch.
+// strides), so nothing goes to scratch.  With a harvest sink, sampled cells
+// also append their (features, post-kinetics deltas and fluxes) pair.
 constexpr int kKinBlock = 128;
 
 __global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restrict__ order, int count,
@@ -233,168 +215,47 @@ __global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restric
                                                       const marl_ruleset* __restrict__ rules,
                                                       float* __restrict__ field, const float* __restrict__ light,
                                                       const uint8_t* __restrict__ nbmask,
-                                                      uint8_t* __restrict__ events, CellParams p) {
+                                                      uint8_t* __restrict__ events, CellParams p, HarvestSink hv) {
   __shared__ float sI[kKinBlock * 17];
   __shared__ float sE[kKinBlock * 13];
   __shared__ float sD[kKinBlock * 13];
   const int t = blockIdx.x * kKinBlock + threadIdx.x;
   if (t >= count) return;
-  const int i = order[t];
+  const int i = order ? order[t] : t;
   float* I = sI + threadIdx.x * 17;
   float* E = sE + threadIdx.x * 13;
   float* D = sD + threadIdx.x * 13;
   marl_cell_state c = cells[i];
   const marl_ruleset& r = rules[i];
-  const float dt = p.dt;
 #pragma unroll
   for (int k = 0; k < MARL_M_INT; ++k) I[k] = c.internal[k];
-
-  const int x = c.pos[0], y = c.pos[1], z = c.pos[2];
-  const size_t XY = (size_t)p.X * p.Y;
-  const size_t v = ((size_t)z * p.Y + y) * p.X + x;
-  const uint32_t m = nbmask[i];
-  const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
-  size_t nv[6];
-  int n = 0;
-#pragma unroll
-  for (int k = 0; k < 6; ++k)
-    if ((m >> k) & 1u) nv[n++] = (size_t)((int64_t)v + off[k]);
-
-  // environment from the AoS view (3 float4 per voxel): sum in face order, then divide
-  float acc[MARL_S_EXT];
-#pragma unroll
-  for (int s = 0; s < MARL_S_EXT; ++s) acc[s] = 0.0f;
-  for (int k = 0; k < n; ++k) {
-    const float4* fv = reinterpret_cast<const float4*>(field + nv[k] * MARL_S_EXT);
-    const float4 a = fv[0], b = fv[1], cc = fv[2];
-    acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
-    acc[4] += b.x; acc[5] += b.y; acc[6] += b.z; acc[7] += b.w;
-    acc[8] += cc.x; acc[9] += cc.y; acc[10] += cc.z; acc[11] += cc.w;
-  }
-#pragma unroll
-  for (int s = 0; s < MARL_S_EXT; ++s) {
-    E[s] = n ? acc[s] / (float)n : 0.0f;
-    D[s] = 0.0f;
-  }
+  size_t v, nv[6];
+  const int n = cell_neighbours(c, nbmask[i], p, &v, nv);
+  gather_env(field, nv, n, E);
   const float lt = light[v];
-
-  // transport (cell.rs:223-238)
+  cell_kinetics(c, r, I, E, D, lt, p);
+  if (hv.count && (uint64_t)sample_hash(hv.tick, (uint32_t)i) < hv.threshold) {
+    const unsigned long long slot = atomicAdd(hv.count, 1ull);
+    if ((int64_t)slot < hv.cap) {
+      float f[kFeat];
+      make_features(c.internal, c, E, lt, r, 0, f);
+      float* fo = hv.feat + slot * kFeat;
 #pragma unroll
-  for (int q = 0; q < MARL_S_TRANSPORTERS; ++q) {
-    const marl_transport tp = r.transport[q];
-    const int e = tp.ext_species, k = tp.int_species;
-    if (e >= MARL_S_EXT || k >= MARL_M_INT) continue;
-    const float ex = E[e];
-    const float ik = I[k];
-    const float up = tp.uptake_rate * ex / (1.0f + ex);
-    const float sec = tp.secrete_rate * ik / (1.0f + ik);
-    I[k] = clamp_rs(ik + (up - sec) * dt, 0.0f, p.c_max);
-    D[e] += (sec - up) * dt;
-  }
-  I[MARL_M_INT - 1] = lt;  // :243
-
-  // reactions (cell.rs:246-285)
-  int active = 0;
-#pragma unroll 4
-  for (int q = 0; q < MARL_R_MAX; ++q) {
-    const marl_reaction rx = r.reactions[q];
-    if (fabsf(rx.v_max) > p.active_reaction_threshold && rx.substrate != rx.product) ++active;
-    if (fabsf(rx.v_max) < p.active_reaction_threshold) continue;
-    if (rx.substrate == rx.product) continue;
-    const int si = rx.substrate, pi = rx.product, ci = rx.catalyst;
-    if (si >= MARL_M_INT || pi >= MARL_M_INT || ci >= MARL_M_INT) continue;
-    const float s = I[si];
-    const float cc = I[ci];
-    const float st = s / (rx.k_m + s + 1.1920929e-07f);
-    const float ct = p.epsilon + cc / (rx.k_cat + cc + 1.1920929e-07f);
-    float rate = rx.v_max * st * ct;
-    if (rx.cofactor != 0xFF) {
-      const int fi = rx.cofactor;
-      if (fi < MARL_M_INT) {
-        const float cof = I[fi];
-        rate *= cof / (1.0f + cof);
-        const float used = fminf(0.5f * rate * dt, I[fi]);
-        I[fi] -= used;
-      }
-    }
-    const float flux = fminf(rate * dt, I[si]);
-    I[si] -= flux;
-    I[pi] = fminf(I[pi] + flux, p.c_max);
-  }
-
-  // maintenance (cell.rs:291-311)
-  float pm = 1.0f;
-  if (c.prep_remaining > 0) {
-    const float evolved = fmaxf(r.division_prep_ticks, 1.0f);
-    const float rush = fmaxf(p.base_division_prep - evolved, 0.0f);
-    pm = p.prep_maintenance_multiplier + rush * p.rush_penalty_rate;
-  }
-  float e0 = I[0];
-  e0 *= 1.0f - p.lambda_maintenance * pm * dt;
-  e0 = fmaxf(e0 - (float)active * p.reaction_maintenance * dt, 0.0f);
-  I[0] = e0;
-
-  // effectors (cell.rs:314-329)
-  if (!c.quiescent) {
+      for (int k = 0; k < kFeat; ++k) fo[k] = f[k];
+      float* to = hv.targ + slot * kTarg;
 #pragma unroll
-    for (int q = 0; q < MARL_S_EFFECTORS; ++q) {
-      const marl_effector ef = r.effectors[q];
-      const int k = ef.int_species, e = ef.ext_species;
-      if (k >= MARL_M_INT || e >= MARL_S_EXT) continue;
-      const float ik = I[k];
-      if (ik > ef.threshold) {
-        const float amount = ef.rate * ik / (1.0f + ik) * dt;
-        I[k] = fmaxf(ik - amount, 0.0f);
-        D[e] += amount;
-      }
+      for (int k = 0; k < MARL_M_INT; ++k) to[k] = I[k] - c.internal[k];
+#pragma unroll
+      for (int s2 = 0; s2 < MARL_S_EXT; ++s2) to[16 + s2] = D[s2];
+      hv.hash[slot] = topology_hash(r);
     }
   }
-
-  // fate (cell.rs:332-366)
-  const float energy = I[0];
-  c.age += 1;
-  const float eff_death = fmaxf(r.death_energy, p.hard_death_floor);
-  uint8_t ev;
-  if (energy < eff_death) {
-    c.prep_remaining = 0;
-    ev = MARL_EVENT_DEATH;
-  } else if (c.prep_remaining > 0) {
-    c.prep_remaining -= 1;
-    ev = c.prep_remaining == 0 ? MARL_EVENT_DIVISION : MARL_EVENT_NONE;
-  } else if (energy > r.division_energy) {
-    c.prep_remaining = f32_as_u16(fmaxf(r.division_prep_ticks, 1.0f));
-    c.quiescent = 0;
-    ev = MARL_EVENT_NONE;
-  } else if (energy < r.quiescence_energy) {
-    c.quiescent = 1;
-    ev = MARL_EVENT_QUIESCENCE;
-  } else {
-    c.quiescent = 0;
-    ev = MARL_EVENT_NONE;
-  }
+  const uint8_t ev = cell_fate(c, r, I[0], p);
 #pragma unroll
   for (int k = 0; k < MARL_M_INT; ++k) c.internal[k] = I[k];
   cells[i] = c;
   events[i] = ev;
-
-  // deltas to the free neighbours (spatial.rs:87-110), on the AoS view
-  if (n > 0) {
-    const float nf = (float)n;
-    float split[MARL_S_EXT];
-#pragma unroll
-    for (int s = 0; s < MARL_S_EXT; ++s) split[s] = D[s] / nf;
-    for (int k = 0; k < n; ++k) {
-      float4* fv = reinterpret_cast<float4*>(field + nv[k] * MARL_S_EXT);
-      float4 a = fv[0], b = fv[1], cc = fv[2];
-      a.x = fmaxf(a.x + split[0], 0.0f); a.y = fmaxf(a.y + split[1], 0.0f);
-      a.z = fmaxf(a.z + split[2], 0.0f); a.w = fmaxf(a.w + split[3], 0.0f);
-      b.x = fmaxf(b.x + split[4], 0.0f); b.y = fmaxf(b.y + split[5], 0.0f);
-      b.z = fmaxf(b.z + split[6], 0.0f); b.w = fmaxf(b.w + split[7], 0.0f);
-      cc.x = fmaxf(cc.x + split[8], 0.0f); cc.y = fmaxf(cc.y + split[9], 0.0f);
-      cc.z = fmaxf(cc.z + split[10], 0.0f); cc.w = fmaxf(cc.w + split[11], 0.0f);
-      fv[0] = a; fv[1] = b; fv[2] = cc;
-    }
-  }
+  scatter_deltas(field, nv, n, D);
 }
 
 // ---------------------------------------------------------------------------
@@ -855,11 +716,16 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
   if (int rc = marl_ensure_staging(ctx)) return rc;
   if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
   float* field = ctx->staging;
+  const HarvestSink hv = surrogate_harvest_sink(ctx, p);
   for (int L = 0; L < nlev; ++L) {
     const int cnt = off[L + 1] - off[L];
     if (cnt == 0) continue;
-    hipLaunchKernelGGL(k_kinetics, dim3(grid1(cnt, kKinBlock)), dim3(kKinBlock), 0, ctx->stream,
-                       ctx->order + off[L], cnt, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p);
+    if (ctx->kin_mode != MARL_KINETICS_PLAIN && ctx->surr) {
+      if (int rc = surrogate_level(ctx, p, ctx->order + off[L], cnt, cells, rules, field)) return rc;
+    } else {
+      hipLaunchKernelGGL(k_kinetics, dim3(grid1(cnt, kKinBlock)), dim3(kKinBlock), 0, ctx->stream,
+                         ctx->order + off[L], cnt, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p, hv);
+    }
   }
   MARL_HIP(ctx, hipGetLastError());
   if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;

@@ -1,0 +1,253 @@
+// kinetics.h — per-cell device functions shared by the plain kinetics kernel,
+// the pair harvester and the surrogate's fallback path.
+//
+// Reference: crates/marl-sim/src/spatial.rs:56-110 (environment, deltas),
+// crates/marl-cell/src/cell.rs:193-369 (CellState::tick).
+#pragma once
+
+#include "marl_internal.h"
+
+struct CellParams {
+  float dt, epsilon, c_max, lambda_maintenance, hard_death_floor, reaction_maintenance;
+  float base_division_prep, prep_maintenance_multiplier, rush_penalty_rate, active_reaction_threshold;
+  float mutation_stddev, structural_mutation_rate_mult, meta_mutation_rate, meta_lo, meta_hi, hill_lo, hill_hi;
+  int32_t div_dist;
+  int32_t X, Y, Z;
+  uint64_t seed, stream;
+};
+
+__host__ __device__ inline float clamp_rs(float x, float lo, float hi) {  // Rust f32::clamp
+  if (x < lo) x = lo;
+  if (x > hi) x = hi;
+  return x;
+}
+
+__device__ inline uint16_t f32_as_u16(float f) {  // Rust `as u16`: truncate, saturate, NaN -> 0
+  if (!(f > 0.0f)) return 0;
+  if (f >= 65535.0f) return 65535;
+  return (uint16_t)f;
+}
+
+// Free face neighbours (spatial.rs:18-43, FACE_OFFSETS order +x,-x,+y,-y,+z,-z)
+// of the cell's voxel from its 6-bit mask; returns their count.
+__device__ inline int cell_neighbours(const marl_cell_state& c, uint32_t m, const CellParams& p, size_t* v_out,
+                                      size_t nv[6]) {
+  const size_t XY = (size_t)p.X * p.Y;
+  const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
+  const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
+  int n = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    if ((m >> k) & 1u) nv[n++] = (size_t)((int64_t)v + off[k]);
+  *v_out = v;
+  return n;
+}
+
+// Mean environment over the free neighbours (spatial.rs:56-77) from the AoS
+// field view: summed in face order, then divided; zeros when enclosed.
+__device__ inline void gather_env(const float* __restrict__ field, const size_t nv[6], int n, float* E) {
+  float acc[MARL_S_EXT];
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) acc[s] = 0.0f;
+  for (int k = 0; k < n; ++k) {
+    const float4* fv = reinterpret_cast<const float4*>(field + nv[k] * MARL_S_EXT);
+    const float4 a = fv[0], b = fv[1], cc = fv[2];
+    acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
+    acc[4] += b.x; acc[5] += b.y; acc[6] += b.z; acc[7] += b.w;
+    acc[8] += cc.x; acc[9] += cc.y; acc[10] += cc.z; acc[11] += cc.w;
+  }
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) E[s] = n ? acc[s] / (float)n : 0.0f;
+}
+
+// Split deltas into the free neighbours with the max(0) clamp
+// (spatial.rs:87-110, field.rs:57-62); lost when enclosed.
+__device__ inline void scatter_deltas(float* __restrict__ field, const size_t nv[6], int n, const float* D) {
+  if (n == 0) return;
+  const float nf = (float)n;
+  float split[MARL_S_EXT];
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) split[s] = D[s] / nf;
+  for (int k = 0; k < n; ++k) {
+    float4* fv = reinterpret_cast<float4*>(field + nv[k] * MARL_S_EXT);
+    float4 a = fv[0], b = fv[1], cc = fv[2];
+    a.x = fmaxf(a.x + split[0], 0.0f); a.y = fmaxf(a.y + split[1], 0.0f);
+    a.z = fmaxf(a.z + split[2], 0.0f); a.w = fmaxf(a.w + split[3], 0.0f);
+    b.x = fmaxf(b.x + split[4], 0.0f); b.y = fmaxf(b.y + split[5], 0.0f);
+    b.z = fmaxf(b.z + split[6], 0.0f); b.w = fmaxf(b.w + split[7], 0.0f);
+    cc.x = fmaxf(cc.x + split[8], 0.0f); cc.y = fmaxf(cc.y + split[9], 0.0f);
+    cc.z = fmaxf(cc.z + split[10], 0.0f); cc.w = fmaxf(cc.w + split[11], 0.0f);
+    fv[0] = a; fv[1] = b; fv[2] = cc;
+  }
+}
+
+// Phase 5 (cell.rs:332-366) from the post-effector energy.
+__device__ inline uint8_t cell_fate(marl_cell_state& c, const marl_ruleset& r, float energy, const CellParams& p) {
+  c.age += 1;
+  const float eff_death = fmaxf(r.death_energy, p.hard_death_floor);
+  if (energy < eff_death) {
+    c.prep_remaining = 0;
+    return MARL_EVENT_DEATH;
+  } else if (c.prep_remaining > 0) {
+    c.prep_remaining -= 1;
+    return c.prep_remaining == 0 ? MARL_EVENT_DIVISION : MARL_EVENT_NONE;
+  } else if (energy > r.division_energy) {
+    c.prep_remaining = f32_as_u16(fmaxf(r.division_prep_ticks, 1.0f));
+    c.quiescent = 0;
+    return MARL_EVENT_NONE;
+  } else if (energy < r.quiescence_energy) {
+    c.quiescent = 1;
+    return MARL_EVENT_QUIESCENCE;
+  }
+  c.quiescent = 0;
+  return MARL_EVENT_NONE;
+}
+
+// Phases 2-4 of CellState::tick (cell.rs:223-329) on I[16] (LDS slot, updated
+// in place) with environment E[12], accumulating exchange fluxes into D[12].
+// The receptor pass (:206-220) only writes an unused array and is omitted.
+__device__ inline void cell_kinetics(const marl_cell_state& c, const marl_ruleset& r, float* I, const float* E,
+                                     float* D, float lt, const CellParams& p) {
+  const float dt = p.dt;
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) D[s] = 0.0f;
+  // transport (cell.rs:223-238)
+#pragma unroll
+  for (int q = 0; q < MARL_S_TRANSPORTERS; ++q) {
+    const marl_transport tp = r.transport[q];
+    const int e = tp.ext_species, k = tp.int_species;
+    if (e >= MARL_S_EXT || k >= MARL_M_INT) continue;
+    const float ex = E[e];
+    const float ik = I[k];
+    const float up = tp.uptake_rate * ex / (1.0f + ex);
+    const float sec = tp.secrete_rate * ik / (1.0f + ik);
+    I[k] = clamp_rs(ik + (up - sec) * dt, 0.0f, p.c_max);
+    D[e] += (sec - up) * dt;
+  }
+  I[MARL_M_INT - 1] = lt;  // :243
+  // reactions (cell.rs:246-285)
+  int active = 0;
+#pragma unroll 4
+  for (int q = 0; q < MARL_R_MAX; ++q) {
+    const marl_reaction rx = r.reactions[q];
+    if (fabsf(rx.v_max) > p.active_reaction_threshold && rx.substrate != rx.product) ++active;
+    if (fabsf(rx.v_max) < p.active_reaction_threshold) continue;
+    if (rx.substrate == rx.product) continue;
+    const int si = rx.substrate, pi = rx.product, ci = rx.catalyst;
+    if (si >= MARL_M_INT || pi >= MARL_M_INT || ci >= MARL_M_INT) continue;
+    const float s = I[si];
+    const float cc = I[ci];
+    const float st = s / (rx.k_m + s + 1.1920929e-07f);
+    const float ct = p.epsilon + cc / (rx.k_cat + cc + 1.1920929e-07f);
+    float rate = rx.v_max * st * ct;
+    if (rx.cofactor != 0xFF) {
+      const int fi = rx.cofactor;
+      if (fi < MARL_M_INT) {
+        const float cof = I[fi];
+        rate *= cof / (1.0f + cof);
+        const float used = fminf(0.5f * rate * dt, I[fi]);
+        I[fi] -= used;
+      }
+    }
+    const float flux = fminf(rate * dt, I[si]);
+    I[si] -= flux;
+    I[pi] = fminf(I[pi] + flux, p.c_max);
+  }
+  // maintenance (cell.rs:291-311)
+  float pm = 1.0f;
+  if (c.prep_remaining > 0) {
+    const float evolved = fmaxf(r.division_prep_ticks, 1.0f);
+    const float rush = fmaxf(p.base_division_prep - evolved, 0.0f);
+    pm = p.prep_maintenance_multiplier + rush * p.rush_penalty_rate;
+  }
+  float e0 = I[0];
+  e0 *= 1.0f - p.lambda_maintenance * pm * dt;
+  e0 = fmaxf(e0 - (float)active * p.reaction_maintenance * dt, 0.0f);
+  I[0] = e0;
+  // effectors (cell.rs:314-329)
+  if (!c.quiescent) {
+#pragma unroll
+    for (int q = 0; q < MARL_S_EFFECTORS; ++q) {
+      const marl_effector ef = r.effectors[q];
+      const int k = ef.int_species, e = ef.ext_species;
+      if (k >= MARL_M_INT || e >= MARL_S_EXT) continue;
+      const float ik = I[k];
+      if (ik > ef.threshold) {
+        const float amount = ef.rate * ik / (1.0f + ik) * dt;
+        I[k] = fmaxf(ik - amount, 0.0f);
+        D[e] += amount;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Surrogate features (80): internal[0..14], quiescent, environment[12], light,
+// in-prep, then per reaction (log v_max, log k_m, catalyst == light), then the
+// topology class as two one-hot slots (class 0 = both zero).  The first 77
+// follow the workload's feature list with internal[15] (overwritten by light
+// every tick) and the prep flag replaced by the two state bits the tick
+// depends on; the last two identify which known reaction topology applies.
+constexpr int kFeat = 80;
+constexpr int kTarg = 28;
+
+__device__ inline float safe_log(float v) { return __logf(fmaxf(fabsf(v), 1e-12f)); }
+
+__device__ inline void make_features(const float* I_pre, const marl_cell_state& c, const float* E, float lt,
+                                     const marl_ruleset& r, int cls, float* f) {
+#pragma unroll
+  for (int k = 0; k < 15; ++k) f[k] = I_pre[k];
+  f[15] = c.quiescent ? 1.0f : 0.0f;
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) f[16 + s] = E[s];
+  f[28] = lt;
+  f[29] = c.prep_remaining > 0 ? 1.0f : 0.0f;
+#pragma unroll 4
+  for (int q = 0; q < MARL_R_MAX; ++q) {
+    const marl_reaction rx = r.reactions[q];
+    f[30 + 3 * q] = safe_log(rx.v_max);
+    f[31 + 3 * q] = safe_log(rx.k_m);
+    f[32 + 3 * q] = rx.catalyst == 15 ? 1.0f : 0.0f;
+  }
+  f[78] = cls == 1 ? 1.0f : 0.0f;
+  f[79] = cls == 2 ? 1.0f : 0.0f;
+}
+
+// 64-bit FNV-1a over the ruleset's integer wiring (reaction species,
+// transporter and effector species): cells with the same hash share a
+// reaction topology.
+__device__ inline uint64_t topology_hash(const marl_ruleset& r) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint32_t b) {
+    h ^= b;
+    h *= 1099511628211ull;
+  };
+#pragma unroll 4
+  for (int q = 0; q < MARL_R_MAX; ++q) {
+    const marl_reaction rx = r.reactions[q];
+    mix(rx.substrate | (rx.product << 8) | (rx.catalyst << 16) | ((uint32_t)rx.cofactor << 24));
+  }
+#pragma unroll
+  for (int q = 0; q < MARL_S_TRANSPORTERS; ++q) mix(r.transport[q].ext_species | (r.transport[q].int_species << 8));
+#pragma unroll
+  for (int q = 0; q < MARL_S_EFFECTORS; ++q) mix(r.effectors[q].int_species | (r.effectors[q].ext_species << 8));
+  return h;
+}
+
+// Pair harvest: sampled cells of the plain pass append (features, targets).
+struct HarvestSink {
+  float* feat;        // [cap][kFeat] (class slots zero; class from hash at training time)
+  float* targ;        // [cap][kTarg]: internal deltas [16], exchange fluxes [12]
+  uint64_t* hash;     // [cap] topology hash
+  unsigned long long* count;
+  int64_t cap;
+  uint64_t threshold;  // sample if hash32(tick, cell) < threshold (2^32: every cell)
+  uint64_t tick;
+};
+
+__device__ inline uint32_t sample_hash(uint64_t tick, uint32_t i) {
+  uint64_t z = tick * 0x9E3779B97F4A7C15ull + i;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return (uint32_t)(z ^ (z >> 31));
+}

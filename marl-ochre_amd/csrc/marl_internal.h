@@ -165,6 +165,9 @@ struct StencilParams {
 
 // ---------------------------------------------------------------------------
 // Host context.
+struct SurrogateState;
+void surrogate_free(SurrogateState* s);
+
 struct marl_ctx {
   marl_dims d{};
   int device = 0;
@@ -207,6 +210,7 @@ struct marl_ctx {
   int32_t* pinned = nullptr;   // small pinned host buffer
   double* budget = nullptr;    // device FP64 partial sums
   int kin_mode = MARL_KINETICS_PLAIN;
+  SurrogateState* surr = nullptr;  // learned kinetics surrogate, created on first use
   marl_sim_params last_sim{};
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -195,6 +195,105 @@ class MarlGpu:
         self._chk(self.lib.marl_mass_budget(self.h, ptr(out), out.size))
         return out
 
+    # -- learned kinetics surrogate --------------------------------------------
+    def set_kinetics_mode(self, mode: int) -> None:
+        self._chk(self.lib.marl_set_kinetics_mode(self.h, int(mode)))
+
+    def harvest_begin(self, capacity: int, sample_rate: float = 1.0) -> None:
+        self._chk(self.lib.marl_harvest_begin(self.h, int(capacity), float(sample_rate)))
+
+    def harvest_end(self) -> int:
+        n = ctypes.c_int64()
+        self._chk(self.lib.marl_harvest_end(self.h, ctypes.byref(n)))
+        return n.value
+
+    def harvest_download(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        n = ctypes.c_int64()
+        self._chk(self.lib.marl_harvest_download(self.h, None, None, None, 0, ctypes.byref(n)))
+        m = max(n.value, 1)
+        f = np.zeros((m, _abi.SURR_FEAT), np.float32)
+        t = np.zeros((m, _abi.SURR_TARG), np.float32)
+        h = np.zeros(m, np.uint64)
+        self._chk(self.lib.marl_harvest_download(self.h, ptr(f), ptr(t), ptr(h), m, ctypes.byref(n)))
+        return f[: n.value], t[: n.value], h[: n.value]
+
+    def harvest_upload(self, feat: np.ndarray, targ: np.ndarray, hashes: np.ndarray) -> None:
+        f = np.ascontiguousarray(feat, np.float32)
+        t = np.ascontiguousarray(targ, np.float32)
+        h = np.ascontiguousarray(hashes, np.uint64)
+        if f.shape != (h.size, _abi.SURR_FEAT) or t.shape != (h.size, _abi.SURR_TARG):
+            raise _abi.InvalidInput("pair arrays have inconsistent shapes")
+        self._chk(self.lib.marl_harvest_upload(self.h, ptr(f), ptr(t), ptr(h), h.size))
+
+    def surrogate_train(self, steps: int, batch: int = 4096, lr: float = 2e-3, lr_final: float = 1e-5,
+                        seed: int = 1, reinit: bool = False, zmax: float = 6.0, energy_margin: float = 0.05) -> dict:
+        cfg = _abi.marl_train_config(steps=int(steps), batch=int(batch), reinit=int(reinit), lr=lr,
+                                     lr_final=lr_final, zmax=zmax, energy_margin=energy_margin, seed=int(seed))
+        rep = _abi.marl_train_report()
+        self._chk(self.lib.marl_surrogate_train(self.h, ctypes.byref(cfg), ctypes.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in rep._fields_ if not k.startswith("_")}
+
+    @property
+    def surrogate_param_count(self) -> int:
+        return int(self.lib.marl_surrogate_param_count())
+
+    def surrogate_get_params(self) -> tuple[np.ndarray, np.ndarray]:
+        w = np.zeros(self.surrogate_param_count, np.float32)
+        nm = np.zeros(2 * (_abi.SURR_FEAT + _abi.SURR_TARG), np.float32)
+        self._chk(self.lib.marl_surrogate_get_params(self.h, ptr(w), w.size, ptr(nm), nm.size))
+        return w, nm
+
+    def surrogate_set_params(self, params: np.ndarray, norm: np.ndarray | None = None, classes=()) -> None:
+        w = np.ascontiguousarray(params, np.float32)
+        cl = np.ascontiguousarray(np.asarray(classes, np.uint64).reshape(-1))
+        nm = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        self._chk(self.lib.marl_surrogate_set_params(self.h, ptr(w), w.size, None if nm is None else ptr(nm),
+                                                     0 if nm is None else nm.size,
+                                                     ptr(cl) if cl.size else None, cl.size))
+
+    def surrogate_classes(self) -> np.ndarray:
+        out = np.zeros(8, np.uint64)
+        n = ctypes.c_int32()
+        self._chk(self.lib.marl_surrogate_classes(self.h, ptr(out), ctypes.byref(n)))
+        return out[: n.value]
+
+    def surrogate_forward(self, x: np.ndarray, fused: bool = True) -> np.ndarray:
+        xx = np.ascontiguousarray(x, np.float32)
+        if xx.ndim != 2 or xx.shape[1] != _abi.SURR_FEAT:
+            raise _abi.InvalidInput("x must be [rows, 80]")
+        y = np.zeros((xx.shape[0], _abi.SURR_TARG), np.float32)
+        self._chk(self.lib.marl_surrogate_forward(self.h, ptr(xx), xx.shape[0], ptr(y), int(fused)))
+        return y
+
+    def surrogate_grad(self, x: np.ndarray, t: np.ndarray) -> tuple[float, np.ndarray]:
+        xx = np.ascontiguousarray(x, np.float32)
+        tt = np.ascontiguousarray(t, np.float32)
+        if xx.ndim != 2 or xx.shape[1] != _abi.SURR_FEAT or tt.shape != (xx.shape[0], _abi.SURR_TARG):
+            raise _abi.InvalidInput("x must be [rows, 80] and t [rows, 28]")
+        g = np.zeros(self.surrogate_param_count, np.float32)
+        loss = ctypes.c_float()
+        self._chk(self.lib.marl_surrogate_grad(self.h, ptr(xx), ptr(tt), xx.shape[0], ctypes.byref(loss), ptr(g)))
+        return loss.value, g
+
+    def surrogate_adam_step(self, grads: np.ndarray, lr: float) -> None:
+        g = np.ascontiguousarray(grads, np.float32)
+        if g.size != self.surrogate_param_count:
+            raise _abi.InvalidInput("gradient size mismatch")
+        self._chk(self.lib.marl_surrogate_adam_step(self.h, ptr(g), float(lr)))
+
+    def surrogate_configure(self, zmax: float = 6.0, energy_margin: float = 0.05, shadow_rate: float = 0.0) -> None:
+        self._chk(self.lib.marl_surrogate_configure(self.h, float(zmax), float(energy_margin), float(shadow_rate)))
+
+    def surrogate_stats(self) -> dict:
+        st = _abi.marl_surr_stats()
+        self._chk(self.lib.marl_surrogate_stats(self.h, ctypes.byref(st)))
+        d = {k: getattr(st, k) for k, _ in st._fields_ if not k.startswith("_") and k != "err_hist"}
+        d["err_hist"] = list(st.err_hist)
+        return d
+
+    def surrogate_reset_stats(self) -> None:
+        self._chk(self.lib.marl_surrogate_reset_stats(self.h))
+
 
 class GpuFieldDiffuser:
     """≙ marl_gpu::GpuFieldDiffuser (crates/marl-gpu/src/field_diffusion.rs:20-268)."""

@@ -76,7 +76,7 @@ def seed_positions(dims, seed, band, count, margin, z_lo, z_hi, cell_at=None, in
     return pos[: placed.value], lin[: placed.value], ca
 
 
-def random_reactions(rules, rng, vmax_range, km_range, light_p=0.0, k_cat=0.1):
+def random_reactions(rules, rng, vmax_range, km_range, light_p=0.0, k_cat=0.1, groups=None):
     """Overwrite the 16 reactions of each ruleset with random ones.
 
     The workload's "stoichiometry +-1 over 1-3 species" maps onto the reference
@@ -85,6 +85,11 @@ def random_reactions(rules, rng, vmax_range, km_range, light_p=0.0, k_cat=0.1):
     product is skipped, cell.rs:250), so the count is drawn from {2, 3}.  The
     catalyst is species 15 (light) with probability light_p, else uniform over
     0..14; k_cat = 0.1; v_max and k_m are log-uniform over the given ranges.
+
+    groups: optional int array (one entry per ruleset).  When given, the wiring
+    (substrate, product, cofactor, catalyst) is drawn once per group and shared
+    by its rulesets, as in a population grown from a few starter metabolisms;
+    v_max and k_m stay per ruleset.
     """
     n = rules.shape[0]
     R = 16
@@ -93,14 +98,18 @@ def random_reactions(rules, rng, vmax_range, km_range, light_p=0.0, k_cat=0.1):
     rx = rules["reactions"]
     vmax = np.exp(rng.uniform(lv[0], lv[1], (n, R))).astype(F32)
     km = np.exp(rng.uniform(lk[0], lk[1], (n, R))).astype(F32)
-    nspec = rng.integers(2, 4, (n, R))
-    sub = rng.integers(0, 15, (n, R))
-    prod = (sub + rng.integers(1, 15, (n, R))) % 15
-    cof = (sub + rng.integers(1, 15, (n, R))) % 15
+    m = n if groups is None else int(np.max(groups)) + 1
+    nspec = rng.integers(2, 4, (m, R))
+    sub = rng.integers(0, 15, (m, R))
+    prod = (sub + rng.integers(1, 15, (m, R))) % 15
+    cof = (sub + rng.integers(1, 15, (m, R))) % 15
     cof = np.where(cof == prod, (cof + 1) % 15, cof)
     cof = np.where(cof == sub, (cof + 1) % 15, cof)
     cof = np.where(nspec == 3, cof, 0xFF)
-    cat = np.where(rng.random((n, R)) < light_p, 15, rng.integers(0, 15, (n, R)))
+    cat = np.where(rng.random((m, R)) < light_p, 15, rng.integers(0, 15, (m, R)))
+    if groups is not None:
+        g = np.asarray(groups)
+        sub, prod, cof, cat = sub[g], prod[g], cof[g], cat[g]
     rx["substrate"] = sub
     rx["product"] = prod
     rx["catalyst"] = cat
@@ -159,7 +168,7 @@ def winogradsky_column(dims=(128, 128, 64), seed=0, cells_per_band=1366,
     return linear_gradient_field(dims), cells, rules, sim
 
 
-def kinetics_at_scale(dims=(256, 256, 128), n_cells=1_000_000, seed=0):
+def kinetics_at_scale(dims=(256, 256, 128), n_cells=1_000_000, seed=0, shared_wiring=False):
     """Plain per-cell kinetics at scale.
 
     n_cells placed uniformly (seed_cells with margin 0 over all z); starter type
@@ -168,6 +177,8 @@ def kinetics_at_scale(dims=(256, 256, 128), n_cells=1_000_000, seed=0):
     16 random reactions (v_max log-U[1e-3, 1], k_m log-U[1e-3, 1], 2-3 species,
     light-catalysed with p = 0.2); field species uniform in [0, 1] with species 7
     in [0, 0.1].  Reference defaults otherwise (10 substeps, mutation_rate 0.05).
+    shared_wiring: the reaction wiring is drawn once per metabolism (three
+    reaction topologies) instead of once per cell — the surrogate's workload.
     """
     from .config import SimulationConfig
     from .starters import make_ruleset
@@ -187,8 +198,28 @@ def kinetics_at_scale(dims=(256, 256, 128), n_cells=1_000_000, seed=0):
     cells["lineage_id"] = lin
     cells["starter_type"] = kinds
     cells["internal"] = rng.random((n, 16), dtype=F32)
-    random_reactions(rules, rng, (1e-3, 1.0), (1e-3, 1.0), light_p=0.2)
+    random_reactions(rules, rng, (1e-3, 1.0), (1e-3, 1.0), light_p=0.2, groups=kinds if shared_wiring else None)
     nv = dims[0] * dims[1] * dims[2]
     field = rng.random((nv, 12), dtype=F32)
     field[:, 7] *= 0.1
     return field.reshape(-1), cells, rules, sim
+
+
+def topology_hash(rules) -> np.ndarray:
+    """64-bit FNV-1a over each ruleset's wiring: per reaction the word
+    substrate | product << 8 | catalyst << 16 | cofactor << 24, then per
+    transporter ext | int << 8, then per effector int | ext << 8 — the key the
+    surrogate's topology classes use."""
+    r = np.atleast_1d(rules)
+    rx = r["reactions"]
+    words = [rx["substrate"].astype(np.uint64) | rx["product"].astype(np.uint64) << np.uint64(8)
+             | rx["catalyst"].astype(np.uint64) << np.uint64(16) | rx["cofactor"].astype(np.uint64) << np.uint64(24)]
+    tp, ef = r["transport"], r["effectors"]
+    words.append(tp["ext_species"].astype(np.uint64) | tp["int_species"].astype(np.uint64) << np.uint64(8))
+    words.append(ef["int_species"].astype(np.uint64) | ef["ext_species"].astype(np.uint64) << np.uint64(8))
+    w = np.concatenate(words, axis=1)
+    h = np.full(len(r), 1469598103934665603, np.uint64)
+    with np.errstate(over="ignore"):
+        for k in range(w.shape[1]):
+            h = (h ^ w[:, k]) * np.uint64(1099511628211)
+    return h

@@ -1,0 +1,121 @@
+"""NumPy restatement of the kinetics surrogate's network — test infrastructure only.
+
+The surrogate has no counterpart in the reference (it replaces the reaction
+phase of CellState::tick, crates/marl-cell/src/cell.rs:223-329, for eligible
+cells).  This module is the checker for the HIP network kernels:
+
+* ``forward`` — 80 -> 256 -> 256 -> 256 -> 28, SiLU on the hidden layers, in
+  float64; with ``bf16=True`` every matrix-product operand is rounded to
+  bfloat16 (round to nearest even) first, as the MFMA kernels do, while sums,
+  biases and activations stay wide;
+* ``loss_and_grad`` — mean squared error over all outputs and its gradient by
+  the chain rule (hand-written, pinned against torch autograd goldens and
+  float64 central differences in tests/test_oracle_mlp.py), with the same
+  optional operand rounding;
+* ``adam`` — Kingma & Ba (2015), float32 elementwise, bias-corrected moments.
+
+The flat parameter vector holds, per layer, W [in x out] row-major, then b.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+DIMS = (80, 256, 256, 256, 28)
+
+
+def layout():
+    """[(w_offset, b_offset, n_in, n_out)] per layer and the total length."""
+    out, off = [], 0
+    for l in range(4):
+        i, o = DIMS[l], DIMS[l + 1]
+        out.append((off, off + i * o, i, o))
+        off += i * o + o
+    return out, off
+
+
+N_PARAMS = layout()[1]
+
+
+def unpack(params: np.ndarray):
+    lay, total = layout()
+    assert params.size == total
+    return [(params[w:w + i * o].reshape(i, o), params[b:b + o]) for w, b, i, o in lay]
+
+
+def pack(layers) -> np.ndarray:
+    return np.concatenate([np.concatenate([W.reshape(-1), b]) for W, b in layers]).astype(np.float32)
+
+
+def bf16(a: np.ndarray) -> np.ndarray:
+    """float32 -> nearest-even bfloat16, returned as float32 (finite inputs)."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).reshape(np.shape(a))
+
+
+def _op(a, rounded):
+    return bf16(a).astype(np.float64) if rounded else np.asarray(a, np.float64)
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def silu_grad(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+def forward(params: np.ndarray, x: np.ndarray, bf16_ops: bool = False):
+    """Outputs [M, 28] (float64) and the (Z, H) cache of the hidden layers."""
+    layers = unpack(np.asarray(params, np.float32))
+    h = np.asarray(x, np.float32)
+    zs, hs = [], []
+    for l, (W, b) in enumerate(layers):
+        z = _op(h, bf16_ops) @ _op(W, bf16_ops) + b.astype(np.float64)
+        if l < 3:
+            zs.append(z)
+            h = silu(z)
+            if bf16_ops:
+                h = h.astype(np.float32)  # the kernels keep f32 activations between products
+            hs.append(h)
+        else:
+            return z, (zs, hs)
+
+
+def loss_and_grad(params: np.ndarray, x: np.ndarray, t: np.ndarray, bf16_ops: bool = False):
+    """MSE = mean((y - t)^2) and its gradient as a flat float64 vector."""
+    layers = unpack(np.asarray(params, np.float32))
+    y, (zs, hs) = forward(params, x, bf16_ops)
+    M, O = y.shape
+    d = y - np.asarray(t, np.float64)
+    loss = float(np.mean(d * d))
+    dz = 2.0 * d / (M * O)
+    if bf16_ops:
+        dz = dz.astype(np.float32)
+    ins = [np.asarray(x, np.float32)] + hs
+    grads = [None] * 4
+    for l in range(3, -1, -1):
+        W = layers[l][0]
+        gW = _op(ins[l], bf16_ops).T @ _op(dz, bf16_ops)
+        gb = np.asarray(dz, np.float64).sum(axis=0)
+        grads[l] = (gW, gb)
+        if l > 0:
+            dh = _op(dz, bf16_ops) @ _op(W, bf16_ops).T
+            dz = dh * silu_grad(zs[l - 1])
+            if bf16_ops:
+                dz = dz.astype(np.float32)
+    flat = np.concatenate([np.concatenate([gW.reshape(-1), gb]) for gW, gb in grads])
+    return loss, flat
+
+
+def adam(params, grads, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step in float32; returns (params, m, v)."""
+    f = np.float32
+    g = np.asarray(grads, f)
+    m = f(b1) * np.asarray(m, f) + f(1.0 - b1) * g
+    v = f(b2) * np.asarray(v, f) + f(1.0 - b2) * g * g
+    c1 = f(1.0) - f(b1) ** f(step)
+    c2 = f(1.0) - f(b2) ** f(step)
+    w = np.asarray(params, f) - f(lr) * (m / c1) / (np.sqrt(v / c2) + f(eps))
+    return w.astype(f), m.astype(f), v.astype(f)

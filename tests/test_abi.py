@@ -31,16 +31,20 @@ def test_record_layouts_match_header(pkg, tmp_path):
     src = tmp_path / "sizes.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "marl_hip.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(marl_sim_params), sizeof(marl_cell_state),"
-        " sizeof(marl_ruleset), offsetof(marl_ruleset, division_energy), offsetof(marl_cell_state, lineage_id),"
-        " sizeof(marl_tick_stats), offsetof(marl_sim_params, rng_seed)); return 0;}\n")
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(marl_sim_params),"
+        " sizeof(marl_cell_state), sizeof(marl_ruleset), offsetof(marl_ruleset, division_energy),"
+        " offsetof(marl_cell_state, lineage_id), sizeof(marl_tick_stats), offsetof(marl_sim_params, rng_seed),"
+        " sizeof(marl_train_config), sizeof(marl_train_report), sizeof(marl_surr_stats),"
+        " offsetof(marl_surr_stats, err_hist)); return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     A = pkg._abi
     want = [ctypes.sizeof(A.marl_sim_params), A.CELL_DTYPE.itemsize, A.RULESET_DTYPE.itemsize,
             A.RULESET_DTYPE.fields["division_energy"][1], A.CELL_DTYPE.fields["lineage_id"][1],
-            ctypes.sizeof(A.marl_tick_stats), A.marl_sim_params.rng_seed.offset]
+            ctypes.sizeof(A.marl_tick_stats), A.marl_sim_params.rng_seed.offset,
+            ctypes.sizeof(A.marl_train_config), ctypes.sizeof(A.marl_train_report), ctypes.sizeof(A.marl_surr_stats),
+            A.marl_surr_stats.err_hist.offset]
     assert got == want
 
 

@@ -1,0 +1,291 @@
+"""GPU checks of the learned kinetics surrogate through the C ABI.
+
+* network kernels: the BF16 MFMA forward (training GEMM path and fused
+  inference), the backward and Adam, against oracle/mlp_ref.py (which is
+  pinned to torch autograd goldens and float64 central differences);
+* harvest: pairs the plain kinetics pass appends agree with the oracle's cell
+  pass for the same cells;
+* fallback and shadow: cells the surrogate does not take are advanced by the
+  plain path bit for bit; shadow mode leaves the state bit-identical to plain;
+* training: the network trained on harvested pairs predicts held-out pairs far
+  better than the untrained network and than "no reaction".
+"""
+import importlib.util
+import pathlib
+
+import numpy as np
+import pytest
+
+from oracle import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+F = np.float32
+GOLDEN = pathlib.Path(__file__).parent / "golden"
+
+
+def _golden():
+    spec = importlib.util.spec_from_file_location("make_golden", GOLDEN / "make_golden.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="module")
+def net():
+    mg = _golden()
+    params, x, t = mg.mlp_inputs()
+    return mg, params, x, t
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("rows", [64, 1, 200])
+def test_forward_matches_oracle(pkg, net, rows):
+    _, params, x0, _ = net
+    rng = np.random.default_rng(rows)
+    x = x0 if rows == 64 else rng.standard_normal((rows, 80)).astype(F)
+    emu, _ = mlp_ref.forward(params, x, bf16_ops=True)
+    exact, _ = mlp_ref.forward(params, x)
+    with pkg.MarlGpu((32, 32, 32)) as g:
+        g.surrogate_set_params(params)
+        yf = g.surrogate_forward(x, fused=True)
+        yt = g.surrogate_forward(x, fused=False)
+    scale = max(1.0, float(np.abs(emu).max()))
+    # against the bf16-operand restatement: only f32 summation order and exp differ
+    assert np.abs(yf - emu).max() <= 4e-3 * scale, np.abs(yf - emu).max()
+    assert np.abs(yt - emu).max() <= 4e-3 * scale, np.abs(yt - emu).max()
+    assert np.abs(yf - yt).max() <= 2e-3 * scale
+    # and the cost of bf16 operands against the exact network
+    assert _rel(yf, exact) < 3e-2
+
+
+@pytest.mark.parametrize("rows", [64, 200])
+def test_backward_matches_oracle(pkg, net, rows):
+    _, params, x0, t0 = net
+    rng = np.random.default_rng(100 + rows)
+    x = x0 if rows == 64 else rng.standard_normal((rows, 80)).astype(F)
+    t = t0 if rows == 64 else rng.standard_normal((rows, 28)).astype(F)
+    l_emu, g_emu = mlp_ref.loss_and_grad(params, x, t, bf16_ops=True)
+    l_ex, g_ex = mlp_ref.loss_and_grad(params, x, t)
+    with pkg.MarlGpu((32, 32, 32)) as g:
+        g.surrogate_set_params(params)
+        loss, grad = g.surrogate_grad(x, t)
+    assert abs(loss - l_emu) <= 2e-3 * l_emu
+    for w, b, i, o in mlp_ref.layout()[0]:
+        for lo, hi in ((w, w + i * o), (b, b + o)):
+            assert _rel(grad[lo:hi], g_emu[lo:hi]) < 1e-2, (lo, _rel(grad[lo:hi], g_emu[lo:hi]))
+            assert _rel(grad[lo:hi], g_ex[lo:hi]) < 6e-2, (lo, _rel(grad[lo:hi], g_ex[lo:hi]))
+    cos = float(grad @ g_ex / (np.linalg.norm(grad) * np.linalg.norm(g_ex)))
+    assert cos > 0.998
+
+
+def test_backward_matches_torch_golden(pkg, net):
+    _, params, x, t = net
+    gold = dict(np.load(GOLDEN / "mlp_torch.npz"))
+    with pkg.MarlGpu((32, 32, 32)) as g:
+        g.surrogate_set_params(params)
+        loss, grad = g.surrogate_grad(x, t)
+    assert abs(loss - float(gold["loss"])) <= 1e-2 * float(gold["loss"])
+    idx = gold["grad_idx"]
+    assert _rel(grad[idx], gold["grad"]) < 6e-2
+
+
+def test_adam_matches_oracle(pkg, net):
+    mg, params, x, t = net
+    _, grad = mlp_ref.loss_and_grad(params, x, t)
+    grad = grad.astype(F)
+    w, m, v = params.copy(), np.zeros_like(params), np.zeros_like(params)
+    with pkg.MarlGpu((32, 32, 32)) as g:
+        g.surrogate_set_params(params)
+        for step, sc in enumerate(mg.ADAM_SCALES, start=1):
+            g.surrogate_adam_step(grad * F(sc), mg.ADAM_LR)
+            w, m, v = mlp_ref.adam(w, grad * F(sc), m, v, step, mg.ADAM_LR)
+        got, _ = g.surrogate_get_params()
+    np.testing.assert_allclose(got - params, w - params, rtol=1e-4, atol=1e-9)
+
+
+def _population(A, dims, n, seed, shared=True):
+    from marl_ochre_amd import scenarios
+    from marl_ochre_amd import starters as st
+
+    rng = np.random.default_rng(seed)
+    nv = dims[0] * dims[1] * dims[2]
+    vox = np.sort(rng.choice(nv, n, replace=False))
+    kinds = rng.integers(0, 3, n)
+    cells = np.zeros(n, A.CELL_DTYPE)
+    rules = np.zeros(n, A.RULESET_DTYPE)
+    for k in range(3):
+        rules[kinds == k] = st.make_ruleset(k)
+    cells["pos"] = np.stack([vox % dims[0], (vox // dims[0]) % dims[1], vox // (dims[0] * dims[1])], 1)
+    cells["starter_type"] = kinds
+    cells["lineage_id"] = rng.integers(0, 2**62, n)
+    cells["internal"] = rng.random((n, 16), dtype=F)
+    scenarios.random_reactions(rules, rng, (1e-3, 1.0), (1e-3, 1.0), light_p=0.2, groups=kinds if shared else None)
+    field = rng.random(nv * 12, dtype=F)
+    light = rng.random(nv, dtype=F)
+    return cells, rules, field, light
+
+
+def test_harvest_pairs_match_oracle(pkg, orc):
+    from marl_ochre_amd import scenarios
+
+    A = pkg._abi
+    dims = (64, 32, 32)
+    cells, rules, field, light = _population(A, dims, 3000, 7)
+    sim = pkg.SimulationConfig()
+    _, c_w, _ = orc.cell_pass(dims, sim, field, light, cells, rules)
+    with pkg.MarlGpu(dims) as g:
+        g.upload_field_aos(field)
+        g.upload_cells(cells, rules)
+        g.upload_light(light)
+        g.harvest_begin(10000, 1.0)
+        g.cell_pass(sim)
+        n = g.harvest_end()
+        feat, targ, hsh = g.harvest_download()
+    assert n == len(cells)
+    # pairs arrive in completion order: match them to cells by their internal state
+    key = {cells["internal"][i].tobytes()[:60]: i for i in range(len(cells))}
+    idx = np.array([key[feat[j, :15].astype(F).tobytes()] for j in range(n)])
+    assert np.unique(idx).size == n
+    np.testing.assert_array_equal(hsh, scenarios.topology_hash(rules)[idx])
+    np.testing.assert_array_equal(feat[:, 15], cells["quiescent"][idx].astype(F))
+    np.testing.assert_array_equal(feat[:, 29], (cells["prep_remaining"][idx] > 0).astype(F))
+    vox = (cells["pos"][:, 2].astype(np.int64) * dims[1] + cells["pos"][:, 1]) * dims[0] + cells["pos"][:, 0]
+    np.testing.assert_array_equal(feat[:, 28], light[vox[idx]])
+    rx = rules["reactions"][idx]
+    np.testing.assert_allclose(feat[:, 30:78:3], np.log(np.maximum(np.abs(rx["v_max"]), 1e-12)), rtol=0, atol=1e-5)
+    np.testing.assert_allclose(feat[:, 31:78:3], np.log(np.maximum(np.abs(rx["k_m"]), 1e-12)), rtol=0, atol=1e-5)
+    np.testing.assert_array_equal(feat[:, 32:78:3], (rx["catalyst"] == 15).astype(F))
+    # internal-delta targets: post-kinetics internals minus pre, bit for bit in f32
+    want = c_w["internal"][idx] - cells["internal"][idx]
+    np.testing.assert_array_equal(targ[:, :16], want)
+
+
+def _trained(pkg, dims, cells, rules, field, light, sim, steps=400):
+    g = pkg.MarlGpu(dims)
+    g.upload_field_aos(field)
+    g.upload_light(light)
+    g.upload_cells(cells, rules)
+    g.harvest_begin(200000, 1.0)
+    for _ in range(3):  # three passes over the same cells from evolving states
+        g.cell_pass(sim)
+    g.harvest_end()
+    rep = g.surrogate_train(steps=steps, batch=2048, lr=2e-3, seed=3)
+    return g, rep
+
+
+def test_training_beats_untrained_and_no_reaction(pkg):
+    A = pkg._abi
+    dims = (64, 48, 32)
+    cells, rules, field, light = _population(A, dims, 16000, 11)
+    sim = pkg.SimulationConfig()
+    g, rep = _trained(pkg, dims, cells, rules, field, light, sim)
+    try:
+        assert rep["classes"] == 3 and rep["pairs"] == 3 * len(cells)
+        assert rep["loss_last"] < 0.2 * rep["loss_first"]
+        # standardised targets: predicting the mean scores 1.0; a useful fit is far below
+        assert rep["val_loss"] < 0.1, rep
+        # held-out cells (a fresh population) in raw units: trained network vs zero deltas
+        c2, r2, f2, l2 = _population(A, dims, 4000, 12)
+        for name in ("substrate", "product", "catalyst", "cofactor"):
+            r2["reactions"][name] = rules["reactions"][name][:4000]
+        r2["transport"] = rules["transport"][:4000]
+        r2["effectors"] = rules["effectors"][:4000]
+        g.upload_field_aos(f2)
+        g.upload_light(l2)
+        g.upload_cells(c2, r2)
+        g.harvest_begin(8000, 1.0)
+        g.cell_pass(sim)
+        g.harvest_end()
+        feat, targ, hsh = g.harvest_download()
+        w, norm = g.surrogate_get_params()
+        classes = g.surrogate_classes()
+        cls = np.select([hsh == classes[1], hsh == classes[2]], [1, 2], 0)
+        feat[:, 78] = cls == 1
+        feat[:, 79] = cls == 2
+        z = (feat - norm[:80]) / norm[80:160]
+        pred = g.surrogate_forward(z, fused=True) * norm[188:216] + norm[160:188]
+        err_net = np.sqrt(np.mean((pred[:, :15] - targ[:, :15]) ** 2))
+        err_zero = np.sqrt(np.mean(targ[:, :15] ** 2))
+        assert err_net < 0.2 * err_zero, (err_net, err_zero)
+    finally:
+        g.close()
+
+
+def test_fallback_and_shadow_are_bit_exact(pkg):
+    from marl_ochre_amd import scenarios
+
+    A = pkg._abi
+    dims = (64, 48, 32)
+    cells, rules, field, light = _population(A, dims, 12000, 21)
+    sim = pkg.SimulationConfig()
+
+    def run(mode, classes=None, margin=0.05, shadow=0.0):
+        with pkg.MarlGpu(dims) as g:
+            g.upload_field_aos(field)
+            g.upload_light(light)
+            g.upload_cells(cells, rules)
+            if mode != A.KINETICS_PLAIN:
+                rng = np.random.default_rng(0)
+                g.surrogate_set_params(rng.standard_normal(g.surrogate_param_count).astype(F) * F(0.05),
+                                       None, classes)
+                g.surrogate_configure(zmax=1e9, energy_margin=margin, shadow_rate=shadow)
+                g.set_kinetics_mode(mode)
+            ev, _ = g.cell_pass(sim)
+            c, r = g.download_cells()
+            st = g.surrogate_stats()
+            return ev, c, g.download_field_aos(), st
+
+    ev0, c0, f0, _ = run(A.KINETICS_PLAIN)
+    known = np.unique(scenarios.topology_hash(rules))
+    assert known.size == 3
+    # unknown topology: every cell falls back
+    ev1, c1, f1, st1 = run(A.KINETICS_SURROGATE, classes=[12345])
+    assert st1["cells"] == len(cells) and st1["fallback_topology"] == len(cells) and st1["surrogate"] == 0
+    # known topologies but an energy margin wider than any energy: every cell falls back
+    ev2, c2, f2, st2 = run(A.KINETICS_SURROGATE, classes=known, margin=1e9)
+    assert st2["fallback_margin"] == len(cells) and st2["surrogate"] == 0
+    # shadow: exact path authoritative, every eligible cell compared
+    ev3, c3, f3, st3 = run(A.KINETICS_SHADOW, classes=known)
+    assert st3["shadow"] == len(cells) and st3["surrogate"] == 0
+    assert sum(st3["err_hist"]) == st3["shadow"]
+    for ev, c, f in ((ev1, c1, f1), (ev2, c2, f2), (ev3, c3, f3)):
+        np.testing.assert_array_equal(ev, ev0)
+        np.testing.assert_array_equal(c["internal"].view(np.uint32), c0["internal"].view(np.uint32))
+        np.testing.assert_array_equal(c["prep_remaining"], c0["prep_remaining"])
+        np.testing.assert_array_equal(f.view(np.uint32), f0.view(np.uint32))
+
+
+def test_surrogate_mode_accounting_and_bounds(pkg):
+    A = pkg._abi
+    dims = (64, 48, 32)
+    cells, rules, field, light = _population(A, dims, 16000, 31)
+    sim = pkg.SimulationConfig()
+    g, rep = _trained(pkg, dims, cells, rules, field, light, sim, steps=300)
+    try:
+        g.upload_field_aos(field)
+        g.upload_light(light)
+        g.upload_cells(cells, rules)
+        g.surrogate_configure(zmax=6.0, energy_margin=0.05, shadow_rate=0.5)
+        g.surrogate_reset_stats()
+        g.set_kinetics_mode(A.KINETICS_SURROGATE)
+        for _ in range(3):
+            g.cell_pass(sim)
+        st = g.surrogate_stats()
+        c, _ = g.download_cells()
+        f = g.download_field_aos()
+    finally:
+        g.close()
+    n = 3 * len(cells)
+    assert st["cells"] == n
+    assert st["surrogate"] + st["fallback_topology"] + st["fallback_ood"] + st["fallback_margin"] == n
+    assert st["surrogate"] > 0.5 * n
+    assert 0.4 * st["surrogate"] < st["shadow"] < 0.6 * st["surrogate"]
+    assert sum(st["err_hist"]) == st["shadow"] and st["shadow_within_1e3"] <= st["shadow"]
+    assert st["shadow_fate_agree"] <= st["shadow"]
+    ci = c["internal"]
+    assert np.isfinite(ci).all() and (ci >= 0).all() and (ci <= sim.c_max).all()
+    assert np.isfinite(f).all() and (f >= 0).all()

@@ -136,6 +136,97 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
     return res
 
 
+SURR_FLOP_PER_CELL = 2.0 * (80 * 256 + 256 * 256 * 2 + 256 * 28)  # forward multiply-adds x 2
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3):
+    """The learned kinetics surrogate on the 1M-cell workload with the reaction
+    wiring shared per metabolism (three topologies; rates per cell).
+
+    Pairs are harvested from `harvest_ticks` plain ticks of the trajectory, the
+    network is trained on the GPU (batch 8192), then the first tick is run
+    `reps` times from the same state in plain mode and in surrogate mode
+    (kinetics launches timed with hipEvents), once in shadow mode (plain
+    authoritative, every eligible cell also predicted) for the error
+    distribution, and the fused inference kernel is timed alone on 1M rows."""
+    from marl_ochre_amd import scenarios
+
+    A = pkg._abi
+    dims = (256, 256, 128)
+    log("generating 1M-cell surrogate workload")
+    field, cells, rules, sim = scenarios.kinetics_at_scale(dims, 1_000_000, seed=0, shared_wiring=True)
+    n0 = len(cells)
+    g = pkg.MarlGpu(dims)
+    try:
+        g.upload_field_aos(field)
+        g.upload_cells(cells, rules)
+        g.harvest_begin(harvest_ticks * n0 + 1, 1.0)
+        t0 = time.perf_counter()
+        for t in range(harvest_ticks):
+            g.tick(sim, t, 1)
+        n_pairs = g.harvest_end()
+        harvest_s = time.perf_counter() - t0
+        log(f"harvested {n_pairs} pairs in {harvest_s:.2f} s; training {train_steps} steps")
+        rep = g.surrogate_train(steps=train_steps, batch=8192, lr=2e-3, lr_final=1e-5, seed=1, reinit=True)
+        log("train:", json.dumps(rep))
+
+        def first_tick(mode, shadow_rate=0.0):
+            out = []
+            for r in range(reps + 1):
+                g.set_kinetics_mode(A.KINETICS_PLAIN)
+                g.upload_field_aos(field)
+                g.upload_cells(cells, rules)
+                if mode != A.KINETICS_PLAIN:
+                    g.surrogate_configure(zmax=6.0, energy_margin=0.05, shadow_rate=shadow_rate)
+                    g.set_kinetics_mode(mode)
+                g.surrogate_reset_stats()
+                st = g.tick(sim, 0, 1)
+                if r > 0:
+                    out.append(st)
+            stats = g.surrogate_stats()
+            g.set_kinetics_mode(A.KINETICS_PLAIN)
+            return out, stats
+
+        plain, _ = first_tick(A.KINETICS_PLAIN)
+        surr, surr_stats = first_tick(A.KINETICS_SURROGATE)
+        shadow, shadow_stats = first_tick(A.KINETICS_SHADOW)
+        infer_ms = g.surrogate_time_infer(n0, 10)
+    finally:
+        g.close()
+    kin_plain = sum(s["ms_kinetics"] for s in plain) / reps
+    kin_surr = sum(s["ms_kinetics"] for s in surr) / reps
+    sh = shadow_stats
+    hist = sh["err_hist"]
+    med_bin = next(b for b in range(24) if sum(hist[:b + 1]) * 2 >= max(sh["shadow"], 1))
+    res = {
+        "workload": "surrogate: 256x256x128, 1,000,000 cells, 16 reactions with wiring drawn once per metabolism "
+                    "(3 topologies), v_max/k_m per cell log-uniform [1e-3,1]; first full tick",
+        "pairs_harvested": n_pairs, "harvest_ticks": harvest_ticks, "train": rep,
+        "train_pairs_per_s": round(train_steps * 8192 / (rep["ms"] * 1e-3), 0),
+        "Mcell_updates_per_s_plain": round(n0 / (kin_plain * 1e-3) / 1e6, 3),
+        "Mcell_updates_per_s_surrogate": round(n0 / (kin_surr * 1e-3) / 1e6, 3),
+        "speedup_vs_plain": round(kin_plain / kin_surr, 3),
+        "ms_kinetics_plain": round(kin_plain, 3), "ms_kinetics_surrogate": round(kin_surr, 3),
+        "surrogate_share": round(surr_stats["surrogate"] / max(surr_stats["cells"], 1), 4),
+        "fallbacks": {k: surr_stats[k] for k in ("fallback_topology", "fallback_ood", "fallback_margin")},
+        "fate_events_plain": {k: plain[-1][k] for k in ("divisions", "deaths", "births_added", "n_cells")},
+        "fate_events_surrogate": {k: surr[-1][k] for k in ("divisions", "deaths", "births_added", "n_cells")},
+        "shadow": {"cells": sh["shadow"],
+                   "within_1e-3_rel": round(sh["shadow_within_1e3"] / max(sh["shadow"], 1), 5),
+                   "fate_agreement": round(sh["shadow_fate_agree"] / max(sh["shadow"], 1), 5),
+                   "median_max_rel_err_bin": [10 ** (med_bin / 2 - 9), 10 ** ((med_bin + 1) / 2 - 9)],
+                   "max_rel_err": sh["shadow_max_rel_err"], "err_hist_half_decades_from_1e-9": hist},
+        "infer_kernel": {"rows": n0, "ms": round(infer_ms, 4),
+                         "TFLOPs": round(SURR_FLOP_PER_CELL * n0 / (infer_ms * 1e-3) / 1e12, 1),
+                         "mfma_frac": round(SURR_FLOP_PER_CELL * n0 / (infer_ms * 1e-3) / 1e12
+                                            / MFMA_BF16_PEAK_TFLOPS, 4),
+                         "flop_per_cell": SURR_FLOP_PER_CELL, "peak_TFLOPs": MFMA_BF16_PEAK_TFLOPS},
+    }
+    log("surrogate:", json.dumps(res))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +237,10 @@ def main():
     ap.add_argument("--tile", type=int, default=-1, help="use this stencil tile id instead of autotuning")
     ap.add_argument("--kinetics-ticks", type=int, default=10,
                     help="full ticks of the 1M-cell kinetics workload (0 = skip)")
+    ap.add_argument("--surrogate-steps", type=int, default=3000,
+                    help="training steps of the surrogate leg (0 = skip)")
+    ap.add_argument("--harvest-ticks", type=int, default=10,
+                    help="plain ticks whose pairs train the surrogate")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -249,6 +344,8 @@ def main():
     g.close()
     if args.kinetics_ticks > 0:
         out["kinetics"] = kinetics_bench(pkg, args.kinetics_ticks, not args.no_cpu)
+    if args.surrogate_steps > 0:
+        out["surrogate"] = surrogate_bench(pkg, args.surrogate_steps, args.harvest_ticks)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(field, occ, dims, sim)

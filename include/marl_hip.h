@@ -296,6 +296,9 @@ int marl_surrogate_adam_step(marl_ctx* ctx, const float* grads, float lr);
 /* zmax, energy margin, share of surrogate cells also run exactly for monitoring. */
 int marl_surrogate_configure(marl_ctx* ctx, float zmax, float energy_margin, float shadow_rate);
 int marl_surrogate_stats(marl_ctx* ctx, marl_surr_stats* out);
+/* Average time of the fused inference kernel over `rows` synthetic rows
+ * (hipEvents around `reps` back-to-back launches on the context stream). */
+int marl_surrogate_time_infer(marl_ctx* ctx, int64_t rows, int32_t reps, float* ms_out);
 int marl_surrogate_reset_stats(marl_ctx* ctx);
 
 #ifdef __cplusplus

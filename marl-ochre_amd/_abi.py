@@ -241,6 +241,7 @@ SIGNATURES = {
     "marl_surrogate_configure": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
     "marl_surrogate_stats": (ctypes.c_int, [_P, ctypes.POINTER(marl_surr_stats)]),
     "marl_surrogate_reset_stats": (ctypes.c_int, [_P]),
+    "marl_surrogate_time_infer": (ctypes.c_int, [_P, _I64, _I32, ctypes.POINTER(ctypes.c_float)]),
 }
 
 _lib = None

@@ -249,10 +249,20 @@ __global__ void k_gather_batch(const int32_t* __restrict__ idx, int n, uint64_t 
   for (int k = 0; k < kFeat; ++k) X[(size_t)b * kFeat + k] = (f[k] - norm[k]) / norm[kFeat + k];
   const float* tm = norm + 2 * kFeat;
   const float* ts = tm + kTarg;
-  for (int k = 0; k < kTarg; ++k) Yt[(size_t)b * kTarg + k] = (T[(size_t)row * kTarg + k] - tm[k]) / ts[k];
+  for (int k = 0; k < kTarg; ++k)
+    Yt[(size_t)b * kTarg + k] = ts[k] > 0.0f ? (T[(size_t)row * kTarg + k] - tm[k]) / ts[k] : 0.0f;
 }
 
 inline unsigned gs(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// Copies on the context stream (which does not synchronise with the null
+// stream) and waits for them, so earlier kernels on the stream are ordered
+// before the copy and the copy is complete on return.
+hipError_t xfer(marl_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(ctx->stream);
+}
 
 int ensure_surr(marl_ctx* ctx) {
   if (!ctx->surr) {
@@ -287,6 +297,12 @@ Classes classes_of(const SurrogateState* s) {
   cl.n = s->n_classes;
   for (int j = 0; j < kMaxClasses; ++j) cl.h[j] = s->classes[j];
   return cl;
+}
+
+// deterministic standardised-looking bf16 inputs for timing the network alone
+__global__ void k_fill_timing_input(size_t n, uint16_t* __restrict__ x) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = bf16_bits(((float)((i * 2654435761u) & 1023u) - 512.0f) * (1.0f / 256.0f));
 }
 
 }  // namespace
@@ -395,9 +411,9 @@ int marl_harvest_download(marl_ctx* ctx, float* feat, float* targ, uint64_t* has
   const size_t m = std::min<size_t>(cap, (size_t)n);
   if (m == 0) return MARL_OK;
   SurrogateState* s = ctx->surr;
-  if (feat) MARL_HIP(ctx, hipMemcpy(feat, s->hfeat, sizeof(float) * m * kFeat, hipMemcpyDeviceToHost));
-  if (targ) MARL_HIP(ctx, hipMemcpy(targ, s->htarg, sizeof(float) * m * kTarg, hipMemcpyDeviceToHost));
-  if (hash) MARL_HIP(ctx, hipMemcpy(hash, s->hhash, sizeof(uint64_t) * m, hipMemcpyDeviceToHost));
+  if (feat) MARL_HIP(ctx, xfer(ctx, feat, s->hfeat, sizeof(float) * m * kFeat, hipMemcpyDeviceToHost));
+  if (targ) MARL_HIP(ctx, xfer(ctx, targ, s->htarg, sizeof(float) * m * kTarg, hipMemcpyDeviceToHost));
+  if (hash) MARL_HIP(ctx, xfer(ctx, hash, s->hhash, sizeof(uint64_t) * m, hipMemcpyDeviceToHost));
   return MARL_OK;
 }
 
@@ -406,11 +422,11 @@ int marl_harvest_upload(marl_ctx* ctx, const float* feat, const float* targ, con
   if (int rc = marl_harvest_begin(ctx, (int64_t)n, 1.0f)) return rc;
   SurrogateState* s = ctx->surr;
   s->harvesting = false;
-  MARL_HIP(ctx, hipMemcpy(s->hfeat, feat, sizeof(float) * n * kFeat, hipMemcpyHostToDevice));
-  MARL_HIP(ctx, hipMemcpy(s->htarg, targ, sizeof(float) * n * kTarg, hipMemcpyHostToDevice));
-  MARL_HIP(ctx, hipMemcpy(s->hhash, hash, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, s->hfeat, feat, sizeof(float) * n * kFeat, hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, s->htarg, targ, sizeof(float) * n * kTarg, hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, s->hhash, hash, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
   const unsigned long long c = n;
-  MARL_HIP(ctx, hipMemcpy(s->hcount, &c, sizeof(c), hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, s->hcount, &c, sizeof(c), hipMemcpyHostToDevice));
   return MARL_OK;
 }
 
@@ -426,7 +442,7 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
   if (n <= 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "no harvested pairs");
   // topology classes: the most frequent hashes holding >= 0.5% of the pairs
   std::vector<uint64_t> hh(n);
-  MARL_HIP(ctx, hipMemcpy(hh.data(), s->hhash, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+  MARL_HIP(ctx, xfer(ctx, hh.data(), s->hhash, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
   std::unordered_map<uint64_t, int64_t> freq;
   for (uint64_t h : hh) freq[h]++;
   std::vector<std::pair<int64_t, uint64_t>> byf;
@@ -461,8 +477,8 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
   fr.p.push_back(dva);
   MARL_HIP(ctx, hipMalloc(&dacc, sizeof(double) * 2 * (kFeat + kTarg)));
   fr.p.push_back(dacc);
-  MARL_HIP(ctx, hipMemcpy(dtr, tr.data(), sizeof(int32_t) * tr.size(), hipMemcpyHostToDevice));
-  MARL_HIP(ctx, hipMemcpy(dva, va.data(), sizeof(int32_t) * va.size(), hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, dtr, tr.data(), sizeof(int32_t) * tr.size(), hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, dva, va.data(), sizeof(int32_t) * va.size(), hipMemcpyHostToDevice));
   const int B = cfg->batch;
   if (int rc = mlp_alloc(ctx, s->mlp, B)) return rc;
   // normalisation from the training pairs
@@ -472,7 +488,7 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
                      (int)tr.size(), s->hfeat, s->htarg, s->hhash, cl, dacc);
   MARL_HIP(ctx, hipGetLastError());
   std::vector<double> acc(2 * (kFeat + kTarg));
-  MARL_HIP(ctx, hipMemcpy(acc.data(), dacc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost));
+  MARL_HIP(ctx, xfer(ctx, acc.data(), dacc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost));
   std::vector<float> norm(2 * (kFeat + kTarg));
   const double nt = (double)tr.size();
   for (int k = 0; k < kFeat + kTarg; ++k) {
@@ -481,14 +497,20 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
     double sd = std::sqrt(var);
     const bool bin = k < kFeat && binary_feature(k);
     if (bin) sd = 1.0;
-    if (sd < 1e-6) sd = 1.0;
+    if (k < kFeat) {
+      if (sd < 1e-6) sd = 1.0;
+    } else if (sd < 1e-12 || k - kFeat == 15) {
+      // constant targets, and internal[15] (set to the voxel's light by the
+      // tick itself), are not learned: std 0 makes the output their mean
+      sd = 0.0;
+    }
     const int base = k < kFeat ? 0 : 2 * kFeat;
     const int kk = k < kFeat ? k : k - kFeat;
     const int width = k < kFeat ? kFeat : kTarg;
-    norm[base + kk] = bin ? 0.0f : (float)mean;
+    norm[base + kk] = bin || (k == kFeat + 15) ? 0.0f : (float)mean;
     norm[base + width + kk] = (float)sd;
   }
-  MARL_HIP(ctx, hipMemcpy(s->mlp.norm, norm.data(), sizeof(float) * norm.size(), hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, s->mlp.norm, norm.data(), sizeof(float) * norm.size(), hipMemcpyHostToDevice));
   // initialisation: He-uniform weights from the seed, zero biases (fresh model only)
   const MlpLayout L = mlp_layout();
   if (!s->trained || cfg->reinit) {
@@ -499,9 +521,9 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
       const float lim = std::sqrt(6.0f / (float)L.in[l]);
       for (size_t q = 0; q < (size_t)L.in[l] * L.out[l]; ++q) w[L.w[l] + q] = (2.0f * rng.u01() - 1.0f) * lim;
     }
-    MARL_HIP(ctx, hipMemcpy(s->mlp.w, w.data(), sizeof(float) * L.total, hipMemcpyHostToDevice));
-    MARL_HIP(ctx, hipMemset(s->mlp.m, 0, sizeof(float) * L.total));
-    MARL_HIP(ctx, hipMemset(s->mlp.v, 0, sizeof(float) * L.total));
+    MARL_HIP(ctx, xfer(ctx, s->mlp.w, w.data(), sizeof(float) * L.total, hipMemcpyHostToDevice));
+    MARL_HIP(ctx, hipMemsetAsync(s->mlp.m, 0, sizeof(float) * L.total, ctx->stream));
+    MARL_HIP(ctx, hipMemsetAsync(s->mlp.v, 0, sizeof(float) * L.total, ctx->stream));
     s->mlp.step = 0;
   }
   float first = -1.0f, last = 0.0f;
@@ -552,11 +574,11 @@ int marl_surrogate_get_params(marl_ctx* ctx, float* params, size_t n, float* nor
   const MlpLayout L = mlp_layout();
   if (params) {
     if (n < L.total) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "params buffer too small");
-    MARL_HIP(ctx, hipMemcpy(params, ctx->surr->mlp.w, sizeof(float) * L.total, hipMemcpyDeviceToHost));
+    MARL_HIP(ctx, xfer(ctx, params, ctx->surr->mlp.w, sizeof(float) * L.total, hipMemcpyDeviceToHost));
   }
   if (norm) {
     if (n_norm < (size_t)2 * (kFeat + kTarg)) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "norm buffer too small");
-    MARL_HIP(ctx, hipMemcpy(norm, ctx->surr->mlp.norm, sizeof(float) * 2 * (kFeat + kTarg), hipMemcpyDeviceToHost));
+    MARL_HIP(ctx, xfer(ctx, norm, ctx->surr->mlp.norm, sizeof(float) * 2 * (kFeat + kTarg), hipMemcpyDeviceToHost));
   }
   return MARL_OK;
 }
@@ -570,9 +592,9 @@ int marl_surrogate_set_params(marl_ctx* ctx, const float* params, size_t n, cons
   if (int rc = ensure_surr(ctx)) return rc;
   SurrogateState* s = ctx->surr;
   if (int rc = mlp_alloc(ctx, s->mlp, std::max(s->mlp.batch, 64))) return rc;
-  MARL_HIP(ctx, hipMemcpy(s->mlp.w, params, sizeof(float) * L.total, hipMemcpyHostToDevice));
-  MARL_HIP(ctx, hipMemset(s->mlp.m, 0, sizeof(float) * L.total));
-  MARL_HIP(ctx, hipMemset(s->mlp.v, 0, sizeof(float) * L.total));
+  MARL_HIP(ctx, xfer(ctx, s->mlp.w, params, sizeof(float) * L.total, hipMemcpyHostToDevice));
+  MARL_HIP(ctx, hipMemsetAsync(s->mlp.m, 0, sizeof(float) * L.total, ctx->stream));
+  MARL_HIP(ctx, hipMemsetAsync(s->mlp.v, 0, sizeof(float) * L.total, ctx->stream));
   s->mlp.step = 0;
   std::vector<float> nm(2 * (kFeat + kTarg));
   for (int k = 0; k < kFeat; ++k) { nm[k] = 0.0f; nm[kFeat + k] = 1.0f; }
@@ -581,7 +603,7 @@ int marl_surrogate_set_params(marl_ctx* ctx, const float* params, size_t n, cons
     if (n_norm != nm.size()) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "norm size mismatch");
     std::copy(norm, norm + n_norm, nm.begin());
   }
-  MARL_HIP(ctx, hipMemcpy(s->mlp.norm, nm.data(), sizeof(float) * nm.size(), hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, s->mlp.norm, nm.data(), sizeof(float) * nm.size(), hipMemcpyHostToDevice));
   s->n_classes = n_classes;
   for (int j = 0; j < n_classes; ++j) s->classes[j] = classes[j];
   if (int rc = mlp_pack(ctx, s->mlp)) return rc;
@@ -596,7 +618,7 @@ int marl_surrogate_forward(marl_ctx* ctx, const float* x, size_t rows, float* y,
   MlpState& m = ctx->surr->mlp;
   if (rows > (size_t)(1 << 20)) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "too many rows");
   if (int rc = mlp_alloc(ctx, m, (int)rows)) return rc;
-  MARL_HIP(ctx, hipMemcpy(m.x, x, sizeof(float) * rows * kIn, hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, m.x, x, sizeof(float) * rows * kIn, hipMemcpyHostToDevice));
   float* dst = m.y;
   if (fused) {
     if (int rc = mlp_infer(ctx, m, (int)rows, m.x, dst)) return rc;
@@ -604,7 +626,7 @@ int marl_surrogate_forward(marl_ctx* ctx, const float* x, size_t rows, float* y,
     if (int rc = mlp_forward_train(ctx, m, (int)rows)) return rc;
   }
   MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  MARL_HIP(ctx, hipMemcpy(y, dst, sizeof(float) * rows * kOut, hipMemcpyDeviceToHost));
+  MARL_HIP(ctx, xfer(ctx, y, dst, sizeof(float) * rows * kOut, hipMemcpyDeviceToHost));
   return MARL_OK;
 }
 
@@ -614,15 +636,15 @@ int marl_surrogate_grad(marl_ctx* ctx, const float* x, const float* t, size_t ro
   if (!ctx->surr || !ctx->surr->trained) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "no surrogate");
   MlpState& m = ctx->surr->mlp;
   if (int rc = mlp_alloc(ctx, m, (int)rows)) return rc;
-  MARL_HIP(ctx, hipMemcpy(m.x, x, sizeof(float) * rows * kIn, hipMemcpyHostToDevice));
-  MARL_HIP(ctx, hipMemcpy(m.t, t, sizeof(float) * rows * kOut, hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, m.x, x, sizeof(float) * rows * kIn, hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, m.t, t, sizeof(float) * rows * kOut, hipMemcpyHostToDevice));
   if (int rc = mlp_forward_train(ctx, m, (int)rows)) return rc;
   float l = 0.0f;
   if (int rc = mlp_loss(ctx, m, (int)rows, &l)) return rc;
   if (int rc = mlp_backward(ctx, m, (int)rows)) return rc;
   MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (loss) *loss = l;
-  if (grads) MARL_HIP(ctx, hipMemcpy(grads, m.g, sizeof(float) * mlp_layout().total, hipMemcpyDeviceToHost));
+  if (grads) MARL_HIP(ctx, xfer(ctx, grads, m.g, sizeof(float) * mlp_layout().total, hipMemcpyDeviceToHost));
   return MARL_OK;
 }
 
@@ -630,7 +652,7 @@ int marl_surrogate_adam_step(marl_ctx* ctx, const float* grads, float lr) {
   if (!ctx || !grads) return MARL_ERR_INVALID_INPUT;
   if (!ctx->surr || !ctx->surr->trained) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "no surrogate");
   MlpState& m = ctx->surr->mlp;
-  MARL_HIP(ctx, hipMemcpy(m.g, grads, sizeof(float) * mlp_layout().total, hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, m.g, grads, sizeof(float) * mlp_layout().total, hipMemcpyHostToDevice));
   if (int rc = mlp_adam(ctx, m, lr)) return rc;
   if (int rc = mlp_pack(ctx, m)) return rc;
   MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -644,6 +666,26 @@ int marl_surrogate_configure(marl_ctx* ctx, float zmax, float energy_margin, flo
   ctx->surr->zmax = zmax;
   ctx->surr->margin = energy_margin;
   ctx->surr->shadow_threshold = (uint64_t)((double)shadow_rate * 4294967296.0);
+  return MARL_OK;
+}
+
+int marl_surrogate_time_infer(marl_ctx* ctx, int64_t rows, int32_t reps, float* ms_out) {
+  if (!ctx || rows <= 0 || rows > (1 << 24) || reps <= 0 || !ms_out) return MARL_ERR_INVALID_INPUT;
+  if (!ctx->surr || !ctx->surr->trained) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "no surrogate");
+  SurrogateState* s = ctx->surr;
+  if (int rc = ensure_level_scratch(ctx, rows)) return rc;
+  hipLaunchKernelGGL(k_fill_timing_input, dim3(gs((size_t)rows * kFeat)), dim3(256), 0, ctx->stream,
+                     (size_t)rows * kFeat, s->featb);
+  MARL_HIP(ctx, hipGetLastError());
+  if (int rc = mlp_infer_bf16(ctx, s->mlp, (int)rows, s->featb, s->out)) return rc;  // warm-up
+  MARL_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int r = 0; r < reps; ++r)
+    if (int rc = mlp_infer_bf16(ctx, s->mlp, (int)rows, s->featb, s->out)) return rc;
+  MARL_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  MARL_HIP(ctx, hipEventSynchronize(ctx->ev1));
+  float ms = 0.0f;
+  MARL_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  *ms_out = ms / (float)reps;
   return MARL_OK;
 }
 

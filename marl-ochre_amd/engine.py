@@ -294,6 +294,12 @@ class MarlGpu:
     def surrogate_reset_stats(self) -> None:
         self._chk(self.lib.marl_surrogate_reset_stats(self.h))
 
+    def surrogate_time_infer(self, rows: int, reps: int = 10) -> float:
+        """Milliseconds per fused inference launch over `rows` synthetic rows."""
+        ms = ctypes.c_float()
+        self._chk(self.lib.marl_surrogate_time_infer(self.h, int(rows), int(reps), ctypes.byref(ms)))
+        return ms.value
+
 
 class GpuFieldDiffuser:
     """≙ marl_gpu::GpuFieldDiffuser (crates/marl-gpu/src/field_diffusion.rs:20-268)."""

@@ -7,8 +7,8 @@
   pass for the same cells;
 * fallback and shadow: cells the surrogate does not take are advanced by the
   plain path bit for bit; shadow mode leaves the state bit-identical to plain;
-* training: the network trained on harvested pairs predicts held-out pairs far
-  better than the untrained network and than "no reaction".
+* training: the network trained on harvested pairs predicts held-out cells
+  better than the untrained network and than "no reaction" (zero deltas).
 """
 import importlib.util
 import pathlib
@@ -164,55 +164,105 @@ def test_harvest_pairs_match_oracle(pkg, orc):
     np.testing.assert_array_equal(targ[:, :16], want)
 
 
-def _trained(pkg, dims, cells, rules, field, light, sim, steps=400):
+@pytest.fixture(scope="module")
+def trained(pkg):
+    """A context whose network was trained on pairs harvested from two plain
+    passes over 200k cells of three reaction topologies, plus the same network
+    before training (parameters and normalisation)."""
+    A = pkg._abi
+    dims = (128, 128, 64)
+    cells, rules, field, light = _population(A, dims, 200000, 11)
+    sim = pkg.SimulationConfig()
     g = pkg.MarlGpu(dims)
     g.upload_field_aos(field)
     g.upload_light(light)
     g.upload_cells(cells, rules)
-    g.harvest_begin(200000, 1.0)
-    for _ in range(3):  # three passes over the same cells from evolving states
+    g.harvest_begin(500000, 1.0)
+    for _ in range(2):
         g.cell_pass(sim)
+    n_pairs = g.harvest_end()
+    g.surrogate_train(steps=0, batch=4096, seed=3, reinit=True)
+    untrained = g.surrogate_get_params()
+    rep = g.surrogate_train(steps=800, batch=4096, lr=2e-3, seed=3)
+    yield dict(g=g, rep=rep, n_pairs=n_pairs, untrained=untrained, dims=dims, rules=rules, sim=sim)
+    g.close()
+
+
+def _held_out(pkg, dims, rules, n, seed):
+    """A fresh population whose cells reuse the trained reaction topologies."""
+    c2, r2, f2, l2 = _population(pkg._abi, dims, n, seed)
+    for name in ("substrate", "product", "catalyst", "cofactor"):
+        r2["reactions"][name] = rules["reactions"][name][:n]
+    r2["transport"] = rules["transport"][:n]
+    r2["effectors"] = rules["effectors"][:n]
+    return c2, r2, f2, l2
+
+
+def test_training_beats_untrained_and_no_reaction(pkg, trained):
+    g, rep = trained["g"], trained["rep"]
+    assert trained["n_pairs"] == 400000
+    assert rep["classes"] == 3 and rep["pairs"] == 400000
+    assert rep["loss_last"] < 0.1 * rep["loss_first"]
+    # standardised targets: predicting the mean scores 1.0
+    assert rep["val_loss"] < 0.1, rep
+    # held-out cells in raw units: internal-delta RMS error of the trained
+    # network, the untrained one and "no reaction" (zero deltas)
+    c2, r2, f2, l2 = _held_out(pkg, trained["dims"], trained["rules"], 20000, 12)
+    g.upload_field_aos(f2)
+    g.upload_light(l2)
+    g.upload_cells(c2, r2)
+    g.harvest_begin(30000, 1.0)
+    g.cell_pass(trained["sim"])
     g.harvest_end()
-    rep = g.surrogate_train(steps=steps, batch=2048, lr=2e-3, seed=3)
-    return g, rep
+    feat, targ, hsh = g.harvest_download()
+    classes = g.surrogate_classes()
+    cls = np.select([hsh == classes[1], hsh == classes[2]], [1, 2], 0)
+    feat[:, 78] = cls == 1
+    feat[:, 79] = cls == 2
+
+    def rms_err(w, norm):
+        with pkg.MarlGpu((32, 32, 32)) as h:
+            h.surrogate_set_params(w, norm, classes)
+            z = (feat - norm[:80]) / norm[80:160]
+            pred = h.surrogate_forward(z, fused=True) * norm[188:216] + norm[160:188]
+        return np.sqrt(np.mean((pred[:, :15] - targ[:, :15]) ** 2))
+
+    err_net = rms_err(*g.surrogate_get_params())
+    err_untrained = rms_err(*trained["untrained"])
+    err_zero = np.sqrt(np.mean(targ[:, :15] ** 2))
+    assert err_net < 0.5 * err_zero, (err_net, err_zero)
+    assert err_net < 0.4 * err_untrained, (err_net, err_untrained)
 
 
-def test_training_beats_untrained_and_no_reaction(pkg):
+def test_surrogate_mode_accounting_and_bounds(pkg, trained):
     A = pkg._abi
-    dims = (64, 48, 32)
-    cells, rules, field, light = _population(A, dims, 16000, 11)
-    sim = pkg.SimulationConfig()
-    g, rep = _trained(pkg, dims, cells, rules, field, light, sim)
+    g, sim = trained["g"], trained["sim"]
+    c2, r2, f2, l2 = _held_out(pkg, trained["dims"], trained["rules"], 50000, 13)
+    g.upload_field_aos(f2)
+    g.upload_light(l2)
+    g.upload_cells(c2, r2)
+    g.surrogate_configure(zmax=6.0, energy_margin=0.05, shadow_rate=0.5)
+    g.surrogate_reset_stats()
+    g.set_kinetics_mode(A.KINETICS_SURROGATE)
     try:
-        assert rep["classes"] == 3 and rep["pairs"] == 3 * len(cells)
-        assert rep["loss_last"] < 0.2 * rep["loss_first"]
-        # standardised targets: predicting the mean scores 1.0; a useful fit is far below
-        assert rep["val_loss"] < 0.1, rep
-        # held-out cells (a fresh population) in raw units: trained network vs zero deltas
-        c2, r2, f2, l2 = _population(A, dims, 4000, 12)
-        for name in ("substrate", "product", "catalyst", "cofactor"):
-            r2["reactions"][name] = rules["reactions"][name][:4000]
-        r2["transport"] = rules["transport"][:4000]
-        r2["effectors"] = rules["effectors"][:4000]
-        g.upload_field_aos(f2)
-        g.upload_light(l2)
-        g.upload_cells(c2, r2)
-        g.harvest_begin(8000, 1.0)
-        g.cell_pass(sim)
-        g.harvest_end()
-        feat, targ, hsh = g.harvest_download()
-        w, norm = g.surrogate_get_params()
-        classes = g.surrogate_classes()
-        cls = np.select([hsh == classes[1], hsh == classes[2]], [1, 2], 0)
-        feat[:, 78] = cls == 1
-        feat[:, 79] = cls == 2
-        z = (feat - norm[:80]) / norm[80:160]
-        pred = g.surrogate_forward(z, fused=True) * norm[188:216] + norm[160:188]
-        err_net = np.sqrt(np.mean((pred[:, :15] - targ[:, :15]) ** 2))
-        err_zero = np.sqrt(np.mean(targ[:, :15] ** 2))
-        assert err_net < 0.2 * err_zero, (err_net, err_zero)
+        for _ in range(3):
+            g.cell_pass(sim)
+        st = g.surrogate_stats()
+        c, _ = g.download_cells()
+        f = g.download_field_aos()
     finally:
-        g.close()
+        g.set_kinetics_mode(A.KINETICS_PLAIN)
+    n = 3 * len(c2)
+    assert st["cells"] == n
+    assert st["surrogate"] + st["fallback_topology"] + st["fallback_ood"] + st["fallback_margin"] == n
+    assert st["fallback_topology"] == 0
+    assert st["surrogate"] > 0.5 * n
+    assert 0.45 * st["surrogate"] < st["shadow"] < 0.55 * st["surrogate"]
+    assert sum(st["err_hist"]) == st["shadow"] and st["shadow_within_1e3"] <= st["shadow"]
+    assert 0.9 * st["shadow"] <= st["shadow_fate_agree"] <= st["shadow"]
+    ci = c["internal"]
+    assert np.isfinite(ci).all() and (ci >= 0).all() and (ci <= sim.c_max).all()
+    assert np.isfinite(f).all() and (f >= 0).all()
 
 
 def test_fallback_and_shadow_are_bit_exact(pkg):
@@ -257,35 +307,3 @@ def test_fallback_and_shadow_are_bit_exact(pkg):
         np.testing.assert_array_equal(c["internal"].view(np.uint32), c0["internal"].view(np.uint32))
         np.testing.assert_array_equal(c["prep_remaining"], c0["prep_remaining"])
         np.testing.assert_array_equal(f.view(np.uint32), f0.view(np.uint32))
-
-
-def test_surrogate_mode_accounting_and_bounds(pkg):
-    A = pkg._abi
-    dims = (64, 48, 32)
-    cells, rules, field, light = _population(A, dims, 16000, 31)
-    sim = pkg.SimulationConfig()
-    g, rep = _trained(pkg, dims, cells, rules, field, light, sim, steps=300)
-    try:
-        g.upload_field_aos(field)
-        g.upload_light(light)
-        g.upload_cells(cells, rules)
-        g.surrogate_configure(zmax=6.0, energy_margin=0.05, shadow_rate=0.5)
-        g.surrogate_reset_stats()
-        g.set_kinetics_mode(A.KINETICS_SURROGATE)
-        for _ in range(3):
-            g.cell_pass(sim)
-        st = g.surrogate_stats()
-        c, _ = g.download_cells()
-        f = g.download_field_aos()
-    finally:
-        g.close()
-    n = 3 * len(cells)
-    assert st["cells"] == n
-    assert st["surrogate"] + st["fallback_topology"] + st["fallback_ood"] + st["fallback_margin"] == n
-    assert st["surrogate"] > 0.5 * n
-    assert 0.4 * st["surrogate"] < st["shadow"] < 0.6 * st["surrogate"]
-    assert sum(st["err_hist"]) == st["shadow"] and st["shadow_within_1e3"] <= st["shadow"]
-    assert st["shadow_fate_agree"] <= st["shadow"]
-    ci = c["internal"]
-    assert np.isfinite(ci).all() and (ci >= 0).all() and (ci <= sim.c_max).all()
-    assert np.isfinite(f).all() and (f >= 0).all()

@@ -136,6 +136,86 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
     return res
 
 
+def drift_report(pkg, n_ticks, nthreads):
+    """Long-run accuracy on the default 128x128x64 column (~4,096 cells).
+
+    (1) Full ticks on the GPU and in the FP32 oracle in lockstep for n_ticks:
+        field, light and every cell field compared at checkpoints.
+    (2) Field-only ticks (boundary sources + 10 substeps, occupancy of the
+        initial cells held fixed) on the GPU in FP32 against the oracle's FP64
+        statement: relative L2 / max errors and per-species mass at checkpoints."""
+    from marl_ochre_amd import scenarios
+    from oracle.binding import Oracle
+
+    orc = Oracle()
+    dims = (128, 128, 64)
+    field, cells, rules, sim = scenarios.winogradsky_column(dims)
+    checks = sorted({c for c in (1, 10, 100, 1000, n_ticks) if c <= n_ticks})
+    out = {"workload": "default column 128x128x64, %d cells, 10 substeps/tick" % len(cells), "ticks": n_ticks}
+    # (1) full ticks, FP32 GPU vs FP32 oracle
+    g = pkg.MarlGpu(dims)
+    g.upload_field_aos(field)
+    g.upload_cells(cells, rules)
+    f_o, c_o, r_o = field.copy(), cells.copy(), rules.copy()
+    rows = []
+    t0 = time.perf_counter()
+    for t in range(n_ticks):
+        g.tick(sim, t, 1)
+        f_o, l_o, c_o, r_o, _, _ = orc.tick(dims, sim, t, f_o, c_o, r_o, nthreads=nthreads)
+        if t + 1 in checks:
+            f_g = g.download_field_aos()
+            l_g = g.download_light()
+            c_g, r_g = g.download_cells()
+            same_cells = len(c_g) == len(c_o) and all(
+                np.array_equal(c_g[k], c_o[k]) for k in ("pos", "lineage_id", "age", "prep_remaining", "quiescent"))
+            same_rules = len(r_g) == len(r_o) and all(
+                np.array_equal(r_g["reactions"][k], r_o["reactions"][k]) for k in r_g["reactions"].dtype.names) and \
+                all(np.array_equal(r_g[k], r_o[k]) for k in ("division_energy", "death_energy", "mutation_rate"))
+            rows.append({"tick": t + 1, "cells": len(c_g), "field_max_abs_diff": float(np.abs(f_g - f_o).max()),
+                         "light_max_abs_diff": float(np.abs(l_g - l_o).max()),
+                         "internal_max_abs_diff": float(np.abs(c_g["internal"] - c_o["internal"]).max())
+                         if same_cells else None,
+                         "bit_identical": bool(same_cells and same_rules
+                                               and np.array_equal(f_g.view(np.uint32), f_o.view(np.uint32))
+                                               and np.array_equal(l_g.view(np.uint32), l_o.view(np.uint32))
+                                               and np.array_equal(c_g["internal"].view(np.uint32),
+                                                                  c_o["internal"].view(np.uint32)))})
+            log("drift fp32:", json.dumps(rows[-1]), f"({time.perf_counter() - t0:.1f} s)")
+        elif (t + 1) % 100 == 0:
+            log(f"drift fp32: tick {t + 1}")
+    g.close()
+    out["fp32_full_tick_vs_oracle"] = rows
+    # (2) field-only ticks, FP32 GPU vs FP64 oracle
+    occ = np.zeros(dims[0] * dims[1] * dims[2], np.uint8)
+    p = cells["pos"].astype(np.int64)
+    occ[(p[:, 2] * dims[1] + p[:, 1]) * dims[0] + p[:, 0]] = 1
+    g = pkg.MarlGpu(dims)
+    g.upload_field_aos(field)
+    g.upload_occupancy(occ)
+    f64 = field.astype(np.float64)
+    rows, done = [], 0
+    for c in checks:
+        for _ in range(c - done):
+            g.apply_boundary_sources(sim)
+            g.diffuse(sim)
+        orc.field_ticks_f64(f64, occ, dims, sim, c - done)
+        done = c
+        f32 = g.download_field_aos().astype(np.float64)
+        d = f32 - f64
+        big = np.abs(f64) > 1e-3
+        m32 = f32.reshape(-1, 12).sum(0)
+        m64 = f64.reshape(-1, 12).sum(0)
+        live = m64 > 1e-9
+        rows.append({"tick": c, "rel_l2": float(np.linalg.norm(d) / np.linalg.norm(f64)),
+                     "max_abs": float(np.abs(d).max()),
+                     "max_rel_where_gt_1e-3": float((np.abs(d[big]) / np.abs(f64[big])).max()),
+                     "mass_max_rel_per_species": float((np.abs(m32 - m64)[live] / m64[live]).max())})
+        log("drift fp64:", json.dumps(rows[-1]))
+    g.close()
+    out["fp32_field_vs_fp64_oracle"] = rows
+    return out
+
+
 SURR_FLOP_PER_CELL = 2.0 * (80 * 256 + 256 * 256 * 2 + 256 * 28)  # forward multiply-adds x 2
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
@@ -237,6 +317,8 @@ def main():
     ap.add_argument("--tile", type=int, default=-1, help="use this stencil tile id instead of autotuning")
     ap.add_argument("--kinetics-ticks", type=int, default=10,
                     help="full ticks of the 1M-cell kinetics workload (0 = skip)")
+    ap.add_argument("--drift-ticks", type=int, default=0,
+                    help="long-run accuracy report over this many ticks of the default column (0 = skip)")
     ap.add_argument("--surrogate-steps", type=int, default=3000,
                     help="training steps of the surrogate leg (0 = skip)")
     ap.add_argument("--harvest-ticks", type=int, default=10,
@@ -344,6 +426,8 @@ def main():
     g.close()
     if args.kinetics_ticks > 0:
         out["kinetics"] = kinetics_bench(pkg, args.kinetics_ticks, not args.no_cpu)
+    if args.drift_ticks > 0:
+        out["drift"] = drift_report(pkg, args.drift_ticks, min(os.cpu_count() or 1, 16))
     if args.surrogate_steps > 0:
         out["surrogate"] = surrogate_bench(pkg, args.surrogate_steps, args.harvest_ticks)
     if not args.no_cpu:

@@ -190,6 +190,19 @@ int marl_download_cells(marl_ctx* ctx, marl_cell_state* cells, marl_ruleset* rul
 int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t n_ticks, marl_tick_stats* stats);
 /* Only the cell pass (lib.rs:204-217) on the resident field/light/cells;
  * events (MARL_EVENT_*) per cell to `events` when non-null (length >= n). */
+/* The same tick as marl_tick, captured once as a hipGraph and replayed: every
+ * count (cells, schedule depth, births, deaths) lives on the device, so a
+ * tick costs one graph launch and one small readback instead of a host round
+ * trip per phase.  Results are bit-identical to marl_tick.  Requires an even
+ * diffusion_substeps, plain kinetics and no harvest.  A tick whose schedule
+ * needs more relaxation sweeps or levels than the captured budget (64 each,
+ * see marl_set_graph_limits) is finished by the host-driven path;
+ * *host_finished (may be NULL) counts such ticks.  stats->ms_field holds the
+ * graph's GPU time; the other phase times are 0. */
+int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t n_ticks,
+                    marl_tick_stats* stats, int32_t* host_finished);
+/* Budget of relaxation sweeps and level launches in the captured tick (1..64). */
+int marl_set_graph_limits(marl_ctx* ctx, int32_t sweeps, int32_t levels);
 int marl_cell_pass(marl_ctx* ctx, const marl_sim_params* sim, uint8_t* events, size_t n_events, int32_t* levels_out);
 /* FP64 per-species totals: out[0..11] empty voxels, out[12..23] occupied
  * voxels, out[24..39] cell internals (no reference counterpart: the reference

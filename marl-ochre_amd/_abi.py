@@ -212,6 +212,9 @@ SIGNATURES = {
     "marl_cell_count": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
     "marl_download_cells": (ctypes.c_int, [_P, _P, _P, _SZ, ctypes.POINTER(_I64)]),
     "marl_tick": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I64, _I32, ctypes.POINTER(marl_tick_stats)]),
+    "marl_tick_graph": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I64, _I32,
+                                       ctypes.POINTER(marl_tick_stats), ctypes.POINTER(_I32)]),
+    "marl_set_graph_limits": (ctypes.c_int, [_P, _I32, _I32]),
     "marl_cell_pass": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _P, _SZ, ctypes.POINTER(_I32)]),
     "marl_mass_budget": (ctypes.c_int, [_P, _P, _SZ]),
     "marl_time_diffuse": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I32, ctypes.POINTER(ctypes.c_float)]),

@@ -89,12 +89,11 @@ __global__ void k_scatter_cells(const marl_cell_state* __restrict__ cells, int n
 // K4: empty-neighbour mask (spatial.rs:18-43) and the conflict predecessors.
 // Conflicting partner at L1 distance 2 with a lower index that shares at least
 // one free face neighbour.
-__global__ void k_cell_prep(const marl_cell_state* __restrict__ cells, int n, const int32_t* __restrict__ cell_at,
-                            const uint32_t* __restrict__ occ, uint8_t* __restrict__ nbmask,
-                            int32_t* __restrict__ preds, uint8_t* __restrict__ npred, int32_t* __restrict__ level,
-                            CellParams p) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+__device__ __forceinline__ void cell_prep_cell(int j, const marl_cell_state* __restrict__ cells,
+                                               const int32_t* __restrict__ cell_at, const uint32_t* __restrict__ occ,
+                                               uint8_t* __restrict__ nbmask, int32_t* __restrict__ preds,
+                                               uint8_t* __restrict__ npred, int32_t* __restrict__ level,
+                                               const CellParams& p) {
   const uint16_t* ps = cells[j].pos;
   const int x = ps[0], y = ps[1], z = ps[2];
   // emptiness from the 1-bit mask (L2-resident); cell_at only for occupied voxels
@@ -136,6 +135,15 @@ __global__ void k_cell_prep(const marl_cell_state* __restrict__ cells, int n, co
   }
   npred[j] = (uint8_t)np;
   level[j] = 0;
+}
+
+__global__ void k_cell_prep(const marl_cell_state* __restrict__ cells, int n, const int32_t* __restrict__ cell_at,
+                            const uint32_t* __restrict__ occ, uint8_t* __restrict__ nbmask,
+                            int32_t* __restrict__ preds, uint8_t* __restrict__ npred, int32_t* __restrict__ level,
+                            CellParams p) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  cell_prep_cell(j, cells, cell_at, occ, nbmask, preds, npred, level, p);
 }
 
 // Longest-path relaxation on the conflict DAG; monotone, converges to the
@@ -210,21 +218,11 @@ __global__ void __launch_bounds__(256) k_level_scatter(int n, const int32_t* __r
 // also append their (features, post-kinetics deltas and fluxes) pair.
 constexpr int kKinBlock = 128;
 
-__global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restrict__ order, int count,
-                                                      marl_cell_state* __restrict__ cells,
-                                                      const marl_ruleset* __restrict__ rules,
-                                                      float* __restrict__ field, const float* __restrict__ light,
-                                                      const uint8_t* __restrict__ nbmask,
-                                                      uint8_t* __restrict__ events, CellParams p, HarvestSink hv) {
-  __shared__ float sI[kKinBlock * 17];
-  __shared__ float sE[kKinBlock * 13];
-  __shared__ float sD[kKinBlock * 13];
-  const int t = blockIdx.x * kKinBlock + threadIdx.x;
-  if (t >= count) return;
-  const int i = order ? order[t] : t;
-  float* I = sI + threadIdx.x * 17;
-  float* E = sE + threadIdx.x * 13;
-  float* D = sD + threadIdx.x * 13;
+__device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict__ cells,
+                                              const marl_ruleset* __restrict__ rules, float* __restrict__ field,
+                                              const float* __restrict__ light, const uint8_t* __restrict__ nbmask,
+                                              uint8_t* __restrict__ events, const CellParams& p,
+                                              const HarvestSink& hv, float* I, float* E, float* D) {
   marl_cell_state c = cells[i];
   const marl_ruleset& r = rules[i];
 #pragma unroll
@@ -258,14 +256,29 @@ __global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restric
   scatter_deltas(field, nv, n, D);
 }
 
+__global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restrict__ order, int count,
+                                                      marl_cell_state* __restrict__ cells,
+                                                      const marl_ruleset* __restrict__ rules,
+                                                      float* __restrict__ field, const float* __restrict__ light,
+                                                      const uint8_t* __restrict__ nbmask,
+                                                      uint8_t* __restrict__ events, CellParams p, HarvestSink hv) {
+  __shared__ float sI[kKinBlock * 17];
+  __shared__ float sE[kKinBlock * 13];
+  __shared__ float sD[kKinBlock * 13];
+  const int t = blockIdx.x * kKinBlock + threadIdx.x;
+  if (t >= count) return;
+  const int i = order ? order[t] : t;
+  kinetics_cell(i, cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17, sE + threadIdx.x * 13,
+                sD + threadIdx.x * 13);
+}
+
 // ---------------------------------------------------------------------------
 // K7: fate.  Division targets (find_empty_neighbor, spatial.rs:115-154) in the
 // pre-birth map; one RNG stream per dividing cell keyed (seed, tick, index).
-__global__ void k_div_target(int n, const marl_cell_state* __restrict__ cells, const uint8_t* __restrict__ events,
-                             const int32_t* __restrict__ cell_at, int32_t* __restrict__ target,
-                             int32_t* __restrict__ bflag, CellParams p) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void div_target_cell(int i, const marl_cell_state* __restrict__ cells,
+                                                const uint8_t* __restrict__ events,
+                                                const int32_t* __restrict__ cell_at, int32_t* __restrict__ target,
+                                                int32_t* __restrict__ bflag, const CellParams& p) {
   int32_t tgt = -1;
   if (events[i] == MARL_EVENT_DIVISION) {
     const uint16_t* ps = cells[i].pos;
@@ -297,6 +310,14 @@ __global__ void k_div_target(int n, const marl_cell_state* __restrict__ cells, c
   }
   target[i] = tgt;
   bflag[i] = tgt >= 0 ? 1 : 0;
+}
+
+__global__ void k_div_target(int n, const marl_cell_state* __restrict__ cells, const uint8_t* __restrict__ events,
+                             const int32_t* __restrict__ cell_at, int32_t* __restrict__ target,
+                             int32_t* __restrict__ bflag, CellParams p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  div_target_cell(i, cells, events, cell_at, target, bflag, p);
 }
 
 // Ruleset::mutate (cell.rs:406-524) in its draw order, on the daughter's copy.
@@ -373,12 +394,12 @@ __device__ void mutate_ruleset(marl_ruleset* r, const CellParams& p, MarlRng& rn
 // age 0, prep 0, new lineage; both halve all 16 internals; daughter mutates.
 // Birth slot = rank of the division in event order; claims keep the first
 // birth per voxel (lib.rs:270-277).
-__global__ void k_births(int n, marl_cell_state* __restrict__ cells, const marl_ruleset* __restrict__ rules,
-                         const int32_t* __restrict__ target, const int32_t* __restrict__ brank,
-                         marl_cell_state* __restrict__ bcells, marl_ruleset* __restrict__ brules,
-                         int32_t* __restrict__ btarget, uint32_t* __restrict__ claim, CellParams p) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void birth_cell(int i, marl_cell_state* __restrict__ cells,
+                                           const marl_ruleset* __restrict__ rules,
+                                           const int32_t* __restrict__ target, const int32_t* __restrict__ brank,
+                                           marl_cell_state* __restrict__ bcells, marl_ruleset* __restrict__ brules,
+                                           int32_t* __restrict__ btarget, uint32_t* __restrict__ claim,
+                                           const CellParams& p) {
   const int32_t tgt = target[i];
   if (tgt < 0) return;
   const int32_t rk = brank[i];
@@ -405,6 +426,15 @@ __global__ void k_births(int n, marl_cell_state* __restrict__ cells, const marl_
   mutate_ruleset(dr, p, rng);
   btarget[rk] = tgt;
   atomicMin(&claim[tgt], (uint32_t)rk);
+}
+
+__global__ void k_births(int n, marl_cell_state* __restrict__ cells, const marl_ruleset* __restrict__ rules,
+                         const int32_t* __restrict__ target, const int32_t* __restrict__ brank,
+                         marl_cell_state* __restrict__ bcells, marl_ruleset* __restrict__ brules,
+                         int32_t* __restrict__ btarget, uint32_t* __restrict__ claim, CellParams p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  birth_cell(i, cells, rules, target, brank, bcells, brules, btarget, claim, p);
 }
 
 __global__ void k_birth_winners(int nb, const int32_t* __restrict__ btarget, const uint32_t* __restrict__ claim,
@@ -434,10 +464,10 @@ __global__ void k_death_flags(int n, const uint8_t* __restrict__ events, int32_t
 // Survivors below m_0 keep their slot, so only the head holes are written:
 // each one's source (a tail survivor at or past m_0) is found here and the
 // records are moved in place by k_move_records.
-__global__ void k_hole_sources(int n, int m0, const uint8_t* __restrict__ events, const int32_t* __restrict__ P,
-                               int32_t* __restrict__ hole_dst, int32_t* __restrict__ hole_src) {
-  const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s0 >= m0 || events[s0] != MARL_EVENT_DEATH) return;
+__device__ __forceinline__ void hole_source(int s0, int n, int m0, const uint8_t* __restrict__ events,
+                                            const int32_t* __restrict__ P, int32_t* __restrict__ hole_dst,
+                                            int32_t* __restrict__ hole_src) {
+  if (events[s0] != MARL_EVENT_DEATH) return;
   const int total = P[n];
   int s = s0, lo = 0, m = m0;
   while (events[s] == MARL_EVENT_DEATH) {
@@ -452,6 +482,13 @@ __global__ void k_hole_sources(int n, int m0, const uint8_t* __restrict__ events
   hole_src[h] = s;
 }
 
+__global__ void k_hole_sources(int n, int m0, const uint8_t* __restrict__ events, const int32_t* __restrict__ P,
+                               int32_t* __restrict__ hole_dst, int32_t* __restrict__ hole_src) {
+  const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s0 >= m0) return;
+  hole_source(s0, n, m0, events, P, hole_dst, hole_src);
+}
+
 // Moves cell + ruleset records in 16-byte chunks, one chunk per lane, so a
 // record's 42 chunks are copied by consecutive lanes.
 constexpr int kCellChunks = sizeof(marl_cell_state) / 16;
@@ -459,12 +496,11 @@ constexpr int kRuleChunks = sizeof(marl_ruleset) / 16;
 constexpr int kRecChunks = kCellChunks + kRuleChunks;
 static_assert(sizeof(marl_cell_state) % 16 == 0 && sizeof(marl_ruleset) % 16 == 0, "record sizes");
 
-__global__ void k_move_records(int count, const int32_t* __restrict__ dst_idx, const int32_t* __restrict__ src_idx,
-                               const marl_cell_state* __restrict__ src_c, const marl_ruleset* __restrict__ src_r,
-                               marl_cell_state* __restrict__ dst_c, marl_ruleset* __restrict__ dst_r,
-                               const int32_t* __restrict__ select) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)count * kRecChunks) return;
+__device__ __forceinline__ void move_chunk(int64_t t, const int32_t* __restrict__ dst_idx,
+                                           const int32_t* __restrict__ src_idx,
+                                           const marl_cell_state* __restrict__ src_c,
+                                           const marl_ruleset* __restrict__ src_r, marl_cell_state* __restrict__ dst_c,
+                                           marl_ruleset* __restrict__ dst_r, const int32_t* __restrict__ select) {
   const int h = (int)(t / kRecChunks), c = (int)(t % kRecChunks);
   if (select && !select[h]) return;
   const int d = dst_idx ? dst_idx[h] : h;
@@ -474,6 +510,15 @@ __global__ void k_move_records(int count, const int32_t* __restrict__ dst_idx, c
   } else {
     reinterpret_cast<uint4*>(dst_r + d)[c - kCellChunks] = reinterpret_cast<const uint4*>(src_r + sidx)[c - kCellChunks];
   }
+}
+
+__global__ void k_move_records(int count, const int32_t* __restrict__ dst_idx, const int32_t* __restrict__ src_idx,
+                               const marl_cell_state* __restrict__ src_c, const marl_ruleset* __restrict__ src_r,
+                               marl_cell_state* __restrict__ dst_c, marl_ruleset* __restrict__ dst_r,
+                               const int32_t* __restrict__ select) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)count * kRecChunks) return;
+  move_chunk(t, dst_idx, src_idx, src_c, src_r, dst_c, dst_r, select);
 }
 
 // Destination slot of each surviving birth: m0 + its rank among the winners.
@@ -541,6 +586,267 @@ __global__ void k_budget_final(const double* __restrict__ partial, int nblocks, 
   for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * kBudget + k];
   out[k] = s;
 }
+
+// ---------------------------------------------------------------------------
+// Device-driven tick.  The same per-cell functions as above, but every count
+// (live cells, schedule depth, births, deaths, holes) is read from a small
+// device record instead of the host, kernels loop over their work with a grid
+// stride, and scans run over the buffer capacity.  A whole tick then needs no
+// host round trip and is captured once as a hipGraph and replayed.  The
+// schedule gets a fixed budget of relaxation sweeps and level launches (spare
+// ones exit at once); a tick that would exceed it, or the cell capacity, sets
+// `err`, skips the cell phase and is finished by the host-driven path.
+constexpr int kGraphSweeps = 64;
+constexpr int kGraphLevels = 64;  // <= kLocalLevels: every level is block-aggregated
+static_assert(kGraphLevels <= kLocalLevels, "graph levels");
+
+struct TickDev {
+  int32_t n, err, nlev, nb, nd, nw, m0, holes;
+  int32_t divisions, deaths, births_added, levels;
+  uint64_t tick;
+  int32_t chg[kGraphSweeps];
+  int32_t off[kGraphLevels + 1];
+};
+
+__device__ __forceinline__ uint64_t div_stream(const TickDev* ts) {
+  return (MARL_TAG_DIVISION << 32) | (ts->tick & 0xffffffffull);
+}
+
+#define GS_LOOP(i, n) for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
+                           i += (int64_t)gridDim.x * blockDim.x)
+
+__global__ void k_g_begin(TickDev* ts, int64_t cap) {
+  if (threadIdx.x == 0) {
+    ts->err = (2 * (int64_t)ts->n + 16 > cap) ? 1 : 0;
+    ts->nlev = ts->nb = ts->nd = ts->nw = ts->m0 = ts->holes = 0;
+    ts->divisions = ts->deaths = ts->births_added = ts->levels = 0;
+  }
+  for (int k = threadIdx.x; k < kGraphSweeps; k += blockDim.x) ts->chg[k] = 0;
+}
+
+__global__ void k_g_scatter_cells(const marl_cell_state* __restrict__ cells, const TickDev* ts,
+                                  int32_t* __restrict__ cell_at, uint32_t* __restrict__ occ, CellParams p) {
+  const int n = ts->n;
+  GS_LOOP(i, n) {
+    const uint16_t* ps = cells[i].pos;
+    const int64_t v = vox(ps[0], ps[1], ps[2], p);
+    cell_at[v] = (int32_t)i;
+    atomicOr(&occ[v >> 5], 1u << (v & 31));
+  }
+}
+
+__global__ void k_g_cell_prep(const marl_cell_state* __restrict__ cells, const TickDev* ts,
+                              const int32_t* __restrict__ cell_at, const uint32_t* __restrict__ occ,
+                              uint8_t* __restrict__ nbmask, int32_t* __restrict__ preds, uint8_t* __restrict__ npred,
+                              int32_t* __restrict__ level, CellParams p) {
+  if (ts->err) return;
+  const int n = ts->n;
+  GS_LOOP(j, n) cell_prep_cell((int)j, cells, cell_at, occ, nbmask, preds, npred, level, p);
+}
+
+// sweep k runs only if sweep k-1 changed a level
+__global__ void k_g_level_relax(int k, TickDev* ts, const int32_t* __restrict__ preds,
+                                const uint8_t* __restrict__ npred, int32_t* level) {
+  if (ts->err || (k > 0 && ts->chg[k - 1] == 0)) return;
+  const int n = ts->n;
+  bool any = false;
+  GS_LOOP(j, n) {
+    const int np = npred[j];
+    if (np == 0) continue;
+    const int32_t* pj = preds + (size_t)j * kMaxPreds;
+    int L = 0;
+    for (int q = 0; q < np; ++q) L = max(L, level[pj[q]] + 1);
+    if (L > level[j]) {
+      level[j] = L;
+      any = true;
+    }
+  }
+  if (any) ts->chg[k] = 1;
+}
+
+__global__ void k_g_relax_check(TickDev* ts, int sweeps) {
+  if (ts->err == 0 && ts->chg[sweeps - 1] != 0) ts->err = 2;  // not known to have converged
+}
+
+__global__ void __launch_bounds__(256) k_g_level_hist(const TickDev* ts, const int32_t* __restrict__ level,
+                                                      int32_t* hist) {
+  if (ts->err) return;
+  __shared__ int32_t lh[kLocalLevels];
+  __shared__ int32_t lmax;
+  for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x) lh[t] = 0;
+  if (threadIdx.x == 0) lmax = 0;
+  __syncthreads();
+  const int n = ts->n;
+  GS_LOOP(j, n) {
+    const int L = level[j];
+    atomicMax(&lmax, L);
+    if (L < kLocalLevels) atomicAdd(&lh[L], 1);
+    else if (L < kMaxLevels) atomicAdd(&hist[1 + L], 1);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x)
+    if (lh[t]) atomicAdd(&hist[1 + t], lh[t]);
+  if (threadIdx.x == 0) atomicMax(&hist[0], lmax);
+}
+
+__global__ void k_g_level_offsets(TickDev* ts, const int32_t* __restrict__ hist, int32_t* __restrict__ cursor,
+                                  int max_levels) {
+  if (threadIdx.x != 0 || ts->err) return;
+  const int n = ts->n;
+  const int nlev = n > 0 ? hist[0] + 1 : 0;
+  if (nlev > max_levels) {
+    ts->err = 3;
+    return;
+  }
+  int acc = 0;
+  for (int L = 0; L < nlev; ++L) {
+    ts->off[L] = acc;
+    cursor[L] = acc;
+    acc += hist[1 + L];
+  }
+  ts->off[nlev] = acc;
+  ts->nlev = nlev;
+  ts->levels = nlev;
+}
+
+__global__ void __launch_bounds__(256) k_g_level_scatter(const TickDev* ts, const int32_t* __restrict__ level,
+                                                         int32_t* cursor, int32_t* __restrict__ order) {
+  if (ts->err) return;
+  __shared__ int32_t cnt[kLocalLevels], base[kLocalLevels];
+  const int n = ts->n;
+  for (int b0 = blockIdx.x * 256; b0 < n; b0 += gridDim.x * 256) {
+    for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x) cnt[t] = 0;
+    __syncthreads();
+    const int j = b0 + threadIdx.x;
+    const int L = j < n ? level[j] : -1;
+    const int local = L >= 0 ? atomicAdd(&cnt[L], 1) : -1;
+    __syncthreads();
+    for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x)
+      base[t] = cnt[t] ? atomicAdd(&cursor[t], cnt[t]) : 0;
+    __syncthreads();
+    if (j < n) order[base[L] + local] = j;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kKinBlock) k_g_kinetics(int L, const TickDev* ts, const int32_t* __restrict__ order,
+                                                        marl_cell_state* __restrict__ cells,
+                                                        const marl_ruleset* __restrict__ rules,
+                                                        float* __restrict__ field, const float* __restrict__ light,
+                                                        const uint8_t* __restrict__ nbmask,
+                                                        uint8_t* __restrict__ events, CellParams p) {
+  if (ts->err || L >= ts->nlev) return;
+  __shared__ float sI[kKinBlock * 17];
+  __shared__ float sE[kKinBlock * 13];
+  __shared__ float sD[kKinBlock * 13];
+  const int lo = ts->off[L], cnt = ts->off[L + 1] - lo;
+  const HarvestSink hv{};
+  GS_LOOP(t, cnt)
+  kinetics_cell(order[lo + t], cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17,
+                sE + threadIdx.x * 13, sD + threadIdx.x * 13);
+}
+
+// division targets for i < n; zero flags up to cap for the fixed-size scan
+__global__ void k_g_div_target(const TickDev* ts, int64_t cap, const marl_cell_state* __restrict__ cells,
+                               const uint8_t* __restrict__ events, const int32_t* __restrict__ cell_at,
+                               int32_t* __restrict__ target, int32_t* __restrict__ bflag, CellParams p) {
+  const int n = ts->err ? 0 : ts->n;
+  p.stream = div_stream(ts);
+  GS_LOOP(i, cap + 1) {
+    if (i < n) {
+      div_target_cell((int)i, cells, events, cell_at, target, bflag, p);
+    } else {
+      target[i] = -1;
+      bflag[i] = 0;
+    }
+  }
+}
+
+__global__ void k_g_births(const TickDev* ts, marl_cell_state* __restrict__ cells,
+                           const marl_ruleset* __restrict__ rules, const int32_t* __restrict__ target,
+                           const int32_t* __restrict__ brank, marl_cell_state* __restrict__ bcells,
+                           marl_ruleset* __restrict__ brules, int32_t* __restrict__ btarget,
+                           uint32_t* __restrict__ claim, CellParams p) {
+  if (ts->err) return;
+  const int n = ts->n;
+  p.stream = div_stream(ts);
+  GS_LOOP(i, n) birth_cell((int)i, cells, rules, target, brank, bcells, brules, btarget, claim, p);
+}
+
+__global__ void k_g_death_flags(const TickDev* ts, int64_t cap, const uint8_t* __restrict__ events,
+                                int32_t* __restrict__ dflag) {
+  const int n = ts->err ? 0 : ts->n;
+  GS_LOOP(i, cap + 1) dflag[i] = (i < n && events[i] == MARL_EVENT_DEATH) ? 1 : 0;
+}
+
+__global__ void k_g_counts(TickDev* ts, const int32_t* __restrict__ brank, const int32_t* __restrict__ P) {
+  if (threadIdx.x != 0 || ts->err) return;
+  const int n = ts->n;
+  const int nb = brank[n], nd = P[n], m0 = n - nd;
+  ts->nb = nb;
+  ts->nd = nd;
+  ts->m0 = m0;
+  ts->holes = (nd > 0 && m0 > 0) ? P[m0] : 0;
+}
+
+__global__ void k_g_birth_winners(const TickDev* ts, int64_t cap, const int32_t* __restrict__ btarget,
+                                  const uint32_t* __restrict__ claim, int32_t* __restrict__ win) {
+  const int nb = ts->err ? 0 : ts->nb;
+  GS_LOOP(r, cap + 1) win[r] = (r < nb && claim[btarget[r]] == (uint32_t)r) ? 1 : 0;
+}
+
+__global__ void k_g_claim_reset(const TickDev* ts, const int32_t* __restrict__ btarget,
+                                uint32_t* __restrict__ claim) {
+  if (ts->err) return;
+  const int nb = ts->nb;
+  GS_LOOP(r, nb) claim[btarget[r]] = 0xffffffffu;
+}
+
+__global__ void k_g_hole_sources(TickDev* ts, const uint8_t* __restrict__ events, const int32_t* __restrict__ P,
+                                 const int32_t* __restrict__ wrank, int32_t* __restrict__ hole_dst,
+                                 int32_t* __restrict__ hole_src) {
+  if (ts->err) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ts->nw = wrank[ts->nb];
+  if (ts->holes == 0) return;
+  const int n = ts->n, m0 = ts->m0;
+  GS_LOOP(s0, m0) hole_source((int)s0, n, m0, events, P, hole_dst, hole_src);
+}
+
+__global__ void k_g_move_holes(const TickDev* ts, const int32_t* __restrict__ hole_dst,
+                               const int32_t* __restrict__ hole_src, marl_cell_state* cells, marl_ruleset* rules) {
+  if (ts->err) return;
+  const int64_t total = (int64_t)ts->holes * kRecChunks;
+  GS_LOOP(t, total) move_chunk(t, hole_dst, hole_src, cells, rules, cells, rules, nullptr);
+}
+
+__global__ void k_g_birth_slots(const TickDev* ts, const int32_t* __restrict__ win,
+                                const int32_t* __restrict__ wrank, int32_t* __restrict__ slot) {
+  if (ts->err) return;
+  const int nb = ts->nb, m0 = ts->m0;
+  GS_LOOP(r, nb) slot[r] = win[r] ? m0 + wrank[r] : 0;
+}
+
+__global__ void k_g_move_births(const TickDev* ts, const int32_t* __restrict__ slot,
+                                const marl_cell_state* __restrict__ bcells, const marl_ruleset* __restrict__ brules,
+                                marl_cell_state* __restrict__ cells, marl_ruleset* __restrict__ rules,
+                                const int32_t* __restrict__ win) {
+  if (ts->err || ts->nw == 0) return;
+  const int64_t total = (int64_t)ts->nb * kRecChunks;
+  GS_LOOP(t, total) move_chunk(t, slot, nullptr, bcells, brules, cells, rules, win);
+}
+
+__global__ void k_g_finish(TickDev* ts) {
+  if (threadIdx.x != 0) return;
+  if (ts->err == 0) {
+    ts->divisions = ts->nb;
+    ts->deaths = ts->nd;
+    ts->births_added = ts->nw;
+    ts->n = ts->m0 + ts->nw;
+  }
+  ts->tick += 1;
+}
+
+#undef GS_LOOP
 
 constexpr int kBudgetBlocks = 512;
 
@@ -814,6 +1120,157 @@ int run_fate(marl_ctx* ctx, const CellParams& p, marl_tick_stats* st) {
   return MARL_OK;
 }
 
+// Reserves hipcub scan storage for n items (no allocation may happen while a
+// graph is being captured).
+int scan_reserve(marl_ctx* ctx, int n) {
+  size_t need = 0;
+  MARL_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, need, (const int32_t*)nullptr, (int32_t*)nullptr, n,
+                                                 ctx->stream));
+  if (need > ctx->scan_tmp_bytes) {
+    if (ctx->scan_tmp) (void)hipFree(ctx->scan_tmp);
+    ctx->scan_tmp = nullptr;
+    MARL_HIP(ctx, hipMalloc(&ctx->scan_tmp, need * 2));
+    ctx->scan_tmp_bytes = need * 2;
+  }
+  return MARL_OK;
+}
+
+inline unsigned gs_grid(int64_t n, int block = 256, int cap_blocks = 2048) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, cap_blocks));
+}
+
+// Enqueues one device-driven tick on ctx->stream (capturable: no host
+// synchronisation, no allocation, no memset nodes).
+int enqueue_device_tick(marl_ctx* ctx, const marl_sim_params* sim) {
+  TickDev* ts = static_cast<TickDev*>(ctx->tdev);
+  const int64_t cap = ctx->cap;
+  const CellParams p = cell_params(ctx, sim, 0);  // the RNG stream comes from ts->tick
+  hipStream_t st = ctx->stream;
+  marl_cell_state* cells = ctx->cells[ctx->cbuf];
+  marl_ruleset* rules = ctx->rules[ctx->cbuf];
+  const unsigned gc = gs_grid(cap), gk = gs_grid(cap, kKinBlock, 4096), gf = gs_grid(cap + 1);
+  hipLaunchKernelGGL(k_g_begin, dim3(1), dim3(64), 0, st, ts, cap);
+  // field phase: occupancy (lib.rs:174-179), substeps with sources fused, light
+  if (int rc = fill_i32(ctx, ctx->cell_at, -1, ctx->nvox)) return rc;
+  if (int rc = fill_i32(ctx, reinterpret_cast<int32_t*>(ctx->occ), 0, ctx->occ_words)) return rc;
+  hipLaunchKernelGGL(k_g_scatter_cells, dim3(gc), dim3(256), 0, st, cells, ts, ctx->cell_at, ctx->occ, p);
+  if (sim->diffusion_substeps > 0) {
+    for (int k = 0; k < sim->diffusion_substeps; ++k) {
+      const StencilParams sp = marl_stencil_params(sim, k == 0 ? 1 : 0);
+      if (int rc = marl_launch_substep(ctx, sp, ctx->tile)) return rc;
+    }
+  } else {
+    if (int rc = marl_launch_boundary(ctx, sim)) return rc;
+  }
+  if (int rc = marl_launch_light(ctx, sim)) return rc;
+  // schedule
+  hipLaunchKernelGGL(k_g_cell_prep, dim3(gc), dim3(256), 0, st, cells, ts, ctx->cell_at, ctx->occ, ctx->nbmask,
+                     ctx->preds, ctx->npred, ctx->level, p);
+  for (int k = 0; k < ctx->graph_sweeps; ++k)
+    hipLaunchKernelGGL(k_g_level_relax, dim3(gc), dim3(256), 0, st, k, ts, ctx->preds, ctx->npred, ctx->level);
+  hipLaunchKernelGGL(k_g_relax_check, dim3(1), dim3(1), 0, st, ts, ctx->graph_sweeps);
+  int32_t* hist = ctx->level_count;
+  int32_t* cursor = ctx->level_count + (kMaxLevels + 1);
+  if (int rc = fill_i32(ctx, hist, 0, kMaxLevels + 1)) return rc;
+  hipLaunchKernelGGL(k_g_level_hist, dim3(gc), dim3(256), 0, st, ts, ctx->level, hist);
+  hipLaunchKernelGGL(k_g_level_offsets, dim3(1), dim3(1), 0, st, ts, hist, cursor, ctx->graph_levels);
+  hipLaunchKernelGGL(k_g_level_scatter, dim3(gc), dim3(256), 0, st, ts, ctx->level, cursor, ctx->order);
+  MARL_HIP(ctx, hipGetLastError());
+  // the ordered pass on the AoS view
+  if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
+  for (int L = 0; L < ctx->graph_levels; ++L)
+    hipLaunchKernelGGL(k_g_kinetics, dim3(gk), dim3(kKinBlock), 0, st, L, ts, ctx->order, cells, rules,
+                       ctx->staging, ctx->light, ctx->nbmask, ctx->events, p);
+  MARL_HIP(ctx, hipGetLastError());
+  if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
+  // fate, births with mutation, deaths and compaction
+  int32_t* target = ctx->flag_scan;
+  int32_t* bflag = target + (cap + 1);
+  int32_t* brank = bflag + (cap + 1);
+  int32_t* P = brank + (cap + 1);
+  int32_t* btarget = ctx->flag_scan2;
+  int32_t* win = btarget + (cap + 1);
+  int32_t* wrank = win + (cap + 1);
+  hipLaunchKernelGGL(k_g_div_target, dim3(gf), dim3(256), 0, st, ts, cap, cells, ctx->events, ctx->cell_at, target,
+                     bflag, p);
+  if (int rc = scan_i32(ctx, bflag, brank, (int)cap + 1)) return rc;
+  hipLaunchKernelGGL(k_g_births, dim3(gs_grid(cap, 64, 8192)), dim3(64), 0, st, ts, cells, rules, target, brank,
+                     ctx->birth_cells, ctx->birth_rules, btarget, ctx->claim, p);
+  hipLaunchKernelGGL(k_g_death_flags, dim3(gf), dim3(256), 0, st, ts, cap, ctx->events, win);
+  if (int rc = scan_i32(ctx, win, P, (int)cap + 1)) return rc;
+  hipLaunchKernelGGL(k_g_counts, dim3(1), dim3(1), 0, st, ts, brank, P);
+  hipLaunchKernelGGL(k_g_birth_winners, dim3(gf), dim3(256), 0, st, ts, cap, btarget, ctx->claim, win);
+  hipLaunchKernelGGL(k_g_claim_reset, dim3(gc), dim3(256), 0, st, ts, btarget, ctx->claim);
+  if (int rc = scan_i32(ctx, win, wrank, (int)cap + 1)) return rc;
+  int32_t* hole_dst = btarget;
+  int32_t* hole_src = target;
+  hipLaunchKernelGGL(k_g_hole_sources, dim3(gc), dim3(256), 0, st, ts, ctx->events, P, wrank, hole_dst, hole_src);
+  hipLaunchKernelGGL(k_g_move_holes, dim3(gs_grid(cap * kRecChunks, 256, 8192)), dim3(256), 0, st, ts, hole_dst,
+                     hole_src, cells, rules);
+  int32_t* slot = bflag;
+  hipLaunchKernelGGL(k_g_birth_slots, dim3(gc), dim3(256), 0, st, ts, win, wrank, slot);
+  hipLaunchKernelGGL(k_g_move_births, dim3(gs_grid(cap * kRecChunks, 256, 8192)), dim3(256), 0, st, ts, slot,
+                     ctx->birth_cells, ctx->birth_rules, cells, rules, win);
+  hipLaunchKernelGGL(k_g_finish, dim3(1), dim3(1), 0, st, ts);
+  MARL_HIP(ctx, hipGetLastError());
+  return MARL_OK;
+}
+
+// What a captured tick depends on: parameters, tile, buffers and their parity.
+std::vector<uint8_t> graph_key(const marl_ctx* ctx, const marl_sim_params* sim) {
+  struct K {
+    marl_sim_params sim;
+    int64_t cap;
+    const void *cells, *rules, *f0, *f1, *staging;
+    int32_t tile, cur, cbuf, sweeps, levels;
+  } k;
+  memset(&k, 0, sizeof(k));
+  k.sim = *sim;
+  k.cap = ctx->cap;
+  k.cells = ctx->cells[ctx->cbuf];
+  k.rules = ctx->rules[ctx->cbuf];
+  k.f0 = ctx->field[0];
+  k.f1 = ctx->field[1];
+  k.staging = ctx->staging;
+  k.tile = ctx->tile;
+  k.cur = ctx->cur;
+  k.cbuf = ctx->cbuf;
+  k.sweeps = ctx->graph_sweeps;
+  k.levels = ctx->graph_levels;
+  const uint8_t* b = reinterpret_cast<const uint8_t*>(&k);
+  return std::vector<uint8_t>(b, b + sizeof(k));
+}
+
+int drop_graph(marl_ctx* ctx) {
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->gexec = nullptr;
+  ctx->graph = nullptr;
+  ctx->graph_key.clear();
+  return MARL_OK;
+}
+
+// Capture one device-driven tick; the host-side field parity is restored
+// afterwards because capture records work without running it.
+int capture_tick(marl_ctx* ctx, const marl_sim_params* sim) {
+  drop_graph(ctx);
+  const int cur0 = ctx->cur;
+  MARL_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_device_tick(ctx, sim);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  ctx->cur = cur0;
+  if (rc != MARL_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  MARL_HIP(ctx, e);
+  ctx->graph = g;
+  MARL_HIP(ctx, hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
+  ctx->graph_key = graph_key(ctx, sim);
+  return MARL_OK;
+}
+
 int check_cells_host(marl_ctx* ctx, const marl_cell_state* cells, size_t n) {
   std::vector<uint8_t> seen(ctx->nvox, 0);
   for (size_t i = 0; i < n; ++i) {
@@ -954,6 +1411,76 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
     (void)hipEventElapsedTime(&st.ms_fate, ev[3], ev[4]);
   }
   if (stats) *stats = st;
+  return MARL_OK;
+}
+
+int marl_set_graph_limits(marl_ctx* ctx, int32_t sweeps, int32_t levels) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  if (sweeps < 1 || sweeps > kGraphSweeps || levels < 1 || levels > kGraphLevels)
+    return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "graph limits out of range");
+  ctx->graph_sweeps = sweeps;
+  ctx->graph_levels = levels;
+  return MARL_OK;
+}
+
+int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t n_ticks,
+                    marl_tick_stats* stats, int32_t* host_finished) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  if (int rc = marl_check_params(ctx, sim)) return rc;
+  if (n_ticks < 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "negative tick count");
+  if (sim->diffusion_substeps % 2 != 0 || ctx->kin_mode != MARL_KINETICS_PLAIN ||
+      (ctx->surr && ctx->surr->harvesting))
+    return marl_fail(ctx, MARL_ERR_INVALID_INPUT,
+                     "the captured tick needs an even substep count, plain kinetics and no harvest");
+  ctx->last_sim = *sim;
+  if (int rc = ensure_voxel_buffers(ctx)) return rc;
+  if (int rc = marl_ensure_staging(ctx)) return rc;
+  if (!ctx->tdev) MARL_HIP(ctx, hipMalloc(&ctx->tdev, sizeof(TickDev)));
+  TickDev* hts = reinterpret_cast<TickDev*>(reinterpret_cast<uint8_t*>(ctx->pinned) + 2048);
+  static_assert(sizeof(TickDev) <= 2048, "TickDev fits the pinned scratch");
+  TickDev* ts = static_cast<TickDev*>(ctx->tdev);
+  marl_tick_stats st{};
+  int32_t finished_by_host = 0;
+  for (int t = 0; t < n_ticks; ++t) {
+    st = marl_tick_stats{};
+    const int64_t n = ctx->n_cells;
+    // room for this tick's births; generous so that growth rarely re-captures
+    if (int rc = marl_ensure_cell_capacity(ctx, std::max<int64_t>(2 * n + 1024, 4 * n))) return rc;
+    if (int rc = scan_reserve(ctx, (int)ctx->cap + 1)) return rc;
+    if (ctx->graph_key != graph_key(ctx, sim))
+      if (int rc = capture_tick(ctx, sim)) return rc;
+    // the device record carries n and the tick index between replays
+    hts->n = (int32_t)n;
+    hts->tick = (uint64_t)(tick0 + t);
+    MARL_HIP(ctx, hipMemcpyAsync(&ts->n, &hts->n, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    MARL_HIP(ctx, hipMemcpyAsync(&ts->tick, &hts->tick, sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+    MARL_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    MARL_HIP(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
+    MARL_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    MARL_HIP(ctx, hipMemcpyAsync(hts, ts, sizeof(TickDev), hipMemcpyDeviceToHost, ctx->stream));
+    MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipEventElapsedTime(&st.ms_field, ctx->ev0, ctx->ev1);
+    if (hts->err == 0) {
+      ctx->n_cells = hts->n;
+      st.divisions = hts->divisions;
+      st.deaths = hts->deaths;
+      st.births_added = hts->births_added;
+      st.levels = hts->levels;
+    } else {
+      // the field phase ran; the cell phase was skipped: finish it host-driven
+      ++finished_by_host;
+      if (int rc = marl_ensure_cell_capacity(ctx, 2 * n + 1024)) return rc;
+      const CellParams p = cell_params(ctx, sim, (uint64_t)(tick0 + t));
+      int32_t levels = 0;
+      if (int rc = run_cell_pass(ctx, p, &levels, nullptr)) return rc;
+      if (int rc = run_fate(ctx, p, &st)) return rc;
+      MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      st.levels = levels;
+    }
+    st.n_cells = ctx->n_cells;
+  }
+  if (stats) *stats = st;
+  if (host_finished) *host_finished = finished_by_host;
   return MARL_OK;
 }
 

@@ -103,6 +103,9 @@ void marl_destroy(marl_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   surrogate_free(c->surr);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->tdev) (void)hipFree(c->tdev);
   void* bufs[] = {c->field[0], c->field[1], c->occ, c->light, c->staging, c->occ_bytes, c->cells[0], c->cells[1],
                   c->rules[0], c->rules[1], c->cell_at, c->nbmask, c->level, c->order, c->events, c->flag_scan,
                   c->flag_scan2, c->misc, c->level_count, c->claim, c->birth_cells, c->birth_rules,

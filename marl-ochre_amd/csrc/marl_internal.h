@@ -211,6 +211,13 @@ struct marl_ctx {
   double* budget = nullptr;    // device FP64 partial sums
   int kin_mode = MARL_KINETICS_PLAIN;
   SurrogateState* surr = nullptr;  // learned kinetics surrogate, created on first use
+
+  // device-driven tick captured as a hipGraph
+  void* tdev = nullptr;            // per-tick counters on the device (TickDev)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint8_t> graph_key;  // inputs the captured graph was recorded for
+  int graph_sweeps = 64, graph_levels = 64;
   marl_sim_params last_sim{};
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -182,6 +182,21 @@ class MarlGpu:
         self._chk(self.lib.marl_tick(self.h, ctypes.byref(p), int(tick0), int(n_ticks), ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in st._fields_ if not k.startswith("_")}
 
+    def tick_graph(self, sim, tick0: int = 0, n_ticks: int = 1) -> dict:
+        """marl_tick through the captured hipGraph; 'host_finished' counts ticks
+        whose cell phase exceeded the captured schedule budget."""
+        p = _params(sim)
+        st = _abi.marl_tick_stats()
+        hf = ctypes.c_int32()
+        self._chk(self.lib.marl_tick_graph(self.h, ctypes.byref(p), int(tick0), int(n_ticks), ctypes.byref(st),
+                                           ctypes.byref(hf)))
+        d = {k: getattr(st, k) for k, _ in st._fields_ if not k.startswith("_")}
+        d["host_finished"] = hf.value
+        return d
+
+    def set_graph_limits(self, sweeps: int = 64, levels: int = 64) -> None:
+        self._chk(self.lib.marl_set_graph_limits(self.h, int(sweeps), int(levels)))
+
     def cell_pass(self, sim) -> tuple[np.ndarray, int]:
         p = _params(sim)
         n = self.cell_count()

@@ -132,7 +132,7 @@ def linear_gradient_field(dims):
 
 
 def winogradsky_column(dims=(128, 128, 64), seed=0, cells_per_band=1366,
-                       bands=((4, 12), (24, 40), (52, 60))):
+                       bands=((4, 12), (24, 40), (52, 60)), starter_state=False):
     """The default-size column with about 4,096 cells.
 
     Linear-gradient field; cells seeded by seed_cells in three depth bands, one
@@ -141,6 +141,10 @@ def winogradsky_column(dims=(128, 128, 64), seed=0, cells_per_band=1366,
     uniform in [0, 0.1]; 16 random reactions (v_max log-U[1e-3, 1e-1], k_m
     log-U[1e-2, 1], catalyst uniform).  Reference defaults otherwise
     (10 substeps).
+
+    starter_state: keep each starter's own initial internals and reactions
+    (starter_metabolisms.rs) instead of the random ones — the reference's own
+    default population, which lives and divides for many ticks.
     """
     from .config import SimulationConfig
     from .starters import make_cell, make_ruleset
@@ -163,8 +167,9 @@ def winogradsky_column(dims=(128, 128, 64), seed=0, cells_per_band=1366,
         rules.append(r)
     cells = np.concatenate(cells)
     rules = np.concatenate(rules)
-    cells["internal"] = rng.uniform(0.0, 0.1, (len(cells), 16)).astype(F32)
-    random_reactions(rules, rng, (1e-3, 1e-1), (1e-2, 1.0))
+    if not starter_state:
+        cells["internal"] = rng.uniform(0.0, 0.1, (len(cells), 16)).astype(F32)
+        random_reactions(rules, rng, (1e-3, 1e-1), (1e-2, 1.0))
     return linear_gradient_field(dims), cells, rules, sim
 
 

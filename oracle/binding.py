@@ -49,6 +49,7 @@ class Oracle:
             "orc_diffusion_substep": (None, [P, P, P, D, SP, f32, ctypes.c_int]),
             "orc_diffuse_tick": (None, [P, P, P, D, SP, ctypes.c_int]),
             "orc_diffusion_substep_f64": (None, [P, P, P, D, SP, ctypes.c_double]),
+            "orc_field_ticks_f64": (None, [P, P, P, D, SP, ctypes.c_int]),
             "orc_light_update": (None, [P, P, P, D, SP]),
             "orc_light_update_f64": (None, [P, P, P, D, SP]),
             "orc_cell_tick": (ctypes.c_int, [P, P, P, f32, SP, P]),
@@ -129,6 +130,16 @@ class Oracle:
         self.L.orc_diffusion_substep_f64(self._p(f), self._p(out), None if o is None else self._p(o),
                                          self._dims(dims), s, float(dt_sub))
         return out
+
+    def field_ticks_f64(self, field64, occ, dims, sim, n_ticks):
+        """Advance the float64 field in place by n_ticks field-only ticks."""
+        assert field64.dtype == np.float64 and field64.flags.c_contiguous
+        scratch = np.empty_like(field64)
+        o = None if occ is None else np.ascontiguousarray(occ, np.uint8)
+        s, _keep = self._sim(sim)
+        self.L.orc_field_ticks_f64(self._p(field64), self._p(scratch), None if o is None else self._p(o),
+                                   self._dims(dims), s, int(n_ticks))
+        return field64
 
     def light(self, field, occ, dims, sim):
         f = np.ascontiguousarray(field, np.float32)

@@ -281,6 +281,33 @@ void orc_diffusion_substep_f64(const double* src, double* dst, const uint8_t* oc
       }
 }
 
+/* Field-only ticks in FP64: boundary sources (field.rs:231-250) then
+ * sim->diffusion_substeps substeps (field.rs:107-217), n_ticks times, with a
+ * fixed occupancy — the FP64 error reference for the FP32 field path. */
+void orc_field_ticks_f64(double* field, double* scratch, const uint8_t* occ, const marl_dims* d,
+                         const marl_sim_params* sim, int n_ticks) {
+  const size_t n = (size_t)d->x * d->y * d->z * S_EXT;
+  const int subs = sim->diffusion_substeps;
+  const double dt_sub = subs > 0 ? (double)sim->dt / (double)subs : 0.0;
+  for (int t = 0; t < n_ticks; ++t) {
+    for (int y = 0; y < d->y; ++y)
+      for (int x = 0; x < d->x; ++x) {
+        double* f0 = field + vidx(d, x, y, 0) * S_EXT;
+        double* f1 = field + vidx(d, x, y, d->z - 1) * S_EXT;
+        f0[1] = fmin(f0[1] + (double)sim->source_rate_oxidant, (double)sim->c_max);
+        f0[3] = fmin(f0[3] + (double)sim->source_rate_carbon, (double)sim->c_max);
+        f1[2] = fmin(f1[2] + (double)sim->source_rate_reductant, (double)sim->c_max);
+      }
+    double* a = field;
+    double* b = scratch;
+    for (int k = 0; k < subs; ++k) {
+      orc_diffusion_substep_f64(a, b, occ, d, sim, dt_sub);
+      double* t2 = a; a = b; b = t2;
+    }
+    if (a != field) memcpy(field, a, n * sizeof(double));
+  }
+}
+
 /* LightField::update — crates/marl-field/src/light.rs:29-59.  The constant
  * exp(-cell_absorption) is evaluated once (it does not depend on the voxel). */
 void orc_light_update(float* light, const float* f, const uint8_t* occ, const marl_dims* d,

@@ -52,6 +52,9 @@ void orc_diffuse_tick(float* field, float* scratch, const uint8_t* occ, const ma
                       const marl_sim_params* sim, int nthreads);
 void orc_diffusion_substep_f64(const double* src, double* dst, const uint8_t* occ, const marl_dims* d,
                                const marl_sim_params* sim, double dt_sub);
+/* n_ticks field-only ticks (boundary sources + substeps) in FP64, fixed occupancy. */
+void orc_field_ticks_f64(double* field, double* scratch, const uint8_t* occ, const marl_dims* d,
+                         const marl_sim_params* sim, int n_ticks);
 void orc_light_update(float* light, const float* field, const uint8_t* occ, const marl_dims* d,
                       const marl_sim_params* sim);
 void orc_light_update_f64(double* light, const double* field, const uint8_t* occ, const marl_dims* d,

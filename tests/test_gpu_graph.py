@@ -1,0 +1,106 @@
+"""The device-driven tick captured as a hipGraph (marl_tick_graph) against the
+host-driven tick (marl_tick) and the oracle: every tick bit-identical —
+fields, light and every cell field — including ticks whose cell phase exceeds
+the captured schedule budget and is finished by the host-driven path."""
+import numpy as np
+import pytest
+
+from test_gpu_cells import assert_cells_equal, population
+
+pytestmark = pytest.mark.gpu
+
+F = np.float32
+
+
+def _compare_runs(pkg, dims, field, cells, rules, sim, n_ticks, limits=None, orc=None):
+    seen = {"divisions": 0, "deaths": 0, "host_finished": 0}
+    with pkg.MarlGpu(dims) as ge, pkg.MarlGpu(dims) as gg:
+        for g in (ge, gg):
+            g.upload_field_aos(field)
+            g.upload_cells(cells, rules)
+        if limits:
+            gg.set_graph_limits(*limits)
+        f, c, r = field, cells, rules
+        for t in range(n_ticks):
+            se = ge.tick(sim, t, 1)
+            sg = gg.tick_graph(sim, t, 1)
+            for k in ("divisions", "deaths", "births_added", "n_cells", "levels"):
+                assert sg[k] == se[k], (t, k, sg[k], se[k])
+            seen["divisions"] += se["divisions"]
+            seen["deaths"] += se["deaths"]
+            seen["host_finished"] += sg["host_finished"]
+            c_e, r_e = ge.download_cells()
+            c_g, r_g = gg.download_cells()
+            assert_cells_equal(c_g, r_g, c_e, r_e)
+            assert np.array_equal(gg.download_field_aos().view(np.uint32), ge.download_field_aos().view(np.uint32)), t
+            assert np.array_equal(gg.download_light().view(np.uint32), ge.download_light().view(np.uint32)), t
+            if orc is not None:
+                f, lt, c, r, _, st = orc.tick(dims, sim, t, f, c, r)
+                assert_cells_equal(c_g, r_g, c, r)
+                assert np.array_equal(gg.download_field_aos(), f), t
+    return seen
+
+
+def test_graph_tick_bit_identical_with_divisions_and_deaths(pkg, orc):
+    A = pkg._abi
+    dims = (64, 32, 24)
+    cells, rules = population(A, dims, 4000, 2)
+    rng = np.random.default_rng(4)
+    field = rng.random(np.prod(dims) * 12).astype(F)
+    sim = pkg.SimulationConfig(rng_seed=77, diffusion_substeps=4)
+    seen = _compare_runs(pkg, dims, field, cells, rules, sim, 6, orc=orc)
+    assert seen["divisions"] > 50 and seen["deaths"] > 50, seen
+    assert seen["host_finished"] == 0
+
+
+def test_graph_tick_host_finishes_over_budget_ticks(pkg):
+    """Budgets too small for the schedule force the skip-and-finish path:
+    one sweep (relaxation not known to converge) and two levels (too deep)."""
+    A = pkg._abi
+    dims = (64, 32, 24)
+    cells, rules = population(A, dims, 6000, 3)
+    rng = np.random.default_rng(5)
+    field = rng.random(np.prod(dims) * 12).astype(F)
+    sim = pkg.SimulationConfig(rng_seed=9, diffusion_substeps=2)
+    for limits in ((1, 64), (64, 2)):
+        seen = _compare_runs(pkg, dims, field, cells, rules, sim, 3, limits=limits)
+        assert seen["host_finished"] == 3, (limits, seen)
+
+
+def test_graph_tick_crowded_and_empty(pkg):
+    A = pkg._abi
+    from marl_ochre_amd import starters as st
+
+    dims = (32, 16, 12)
+    pos = [(x, y, z) for z in range(3, 8) for y in range(4, 9) for x in range(8, 14)]
+    pos += [(0, 0, 0), (31, 15, 11), (0, 15, 5), (31, 0, 6), (20, 8, 5), (22, 8, 5), (24, 8, 5), (21, 10, 5)]
+    n = len(pos)
+    cells = np.zeros(n, A.CELL_DTYPE)
+    rules = np.zeros(n, A.RULESET_DTYPE)
+    rng = np.random.default_rng(9)
+    for i, p in enumerate(pos):
+        rules[i] = st.make_ruleset(i % 3)
+        cells[i] = st.make_cell(i % 3, p, i)
+    cells["internal"] = (rng.random((n, 16)) * 2).astype(F)
+    cells["prep_remaining"] = 1
+    field = rng.random(np.prod(dims) * 12).astype(F)
+    seen = _compare_runs(pkg, dims, field, cells, rules, pkg.SimulationConfig(rng_seed=5), 3)
+    assert seen["divisions"] > 10
+    empty_c = np.zeros(0, A.CELL_DTYPE)
+    empty_r = np.zeros(0, A.RULESET_DTYPE)
+    _compare_runs(pkg, dims, field, empty_c, empty_r, pkg.SimulationConfig(), 2)
+
+
+def test_graph_tick_starter_column(pkg):
+    """The reference's default column with its own starter states: 20 ticks."""
+    from marl_ochre_amd import scenarios
+
+    field, cells, rules, sim = scenarios.winogradsky_column(starter_state=True)
+    _compare_runs(pkg, (128, 128, 64), field, cells, rules, sim, 20)
+
+
+def test_graph_tick_rejects_odd_substeps(pkg):
+    dims = (32, 16, 8)
+    with pkg.MarlGpu(dims) as g:
+        with pytest.raises(pkg.InvalidInput):
+            g.tick_graph(pkg.SimulationConfig(diffusion_substeps=3), 0, 1)

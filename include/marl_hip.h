@@ -59,6 +59,7 @@ enum { MARL_EVENT_NONE = 0, MARL_EVENT_DIVISION = 1, MARL_EVENT_DEATH = 2, MARL_
 enum { MARL_KINETICS_PLAIN = 0, MARL_KINETICS_SURROGATE = 1, MARL_KINETICS_SHADOW = 2 };
 
 typedef struct marl_ctx marl_ctx;
+typedef struct marl_ensemble marl_ensemble;
 
 /* Grid size.  The reference fixes it at compile time (marl-config/src/lib.rs:17-19);
  * here it is per context.  x must be a multiple of 32. */
@@ -240,6 +241,30 @@ int marl_stencil_tile_shape(int32_t tile, int32_t* rows_per_thread, int32_t* z_c
  * scaling.  *bytes_out = bytes read + written per rep. */
 int marl_copy_probe(marl_ctx* ctx, int32_t n_reps, float* ms_out, double* bytes_out);
 int marl_sync(marl_ctx* ctx);
+
+/* ---- ensembles ------------------------------------------------------------- */
+/* No reference counterpart: the reference runs one column per process
+ * (crates/marl-sim/src/lib.rs:160-277).  An ensemble advances independent
+ * columns of uneven sizes together: each column is an ordinary context
+ * (created, uploaded and downloaded as usual; not owned by the ensemble).
+ * A tick is one hipGraph: every column's field phase in one persistent
+ * stencil launch per substep and one light launch, and the columns' cell
+ * phases side by side on their own streams.  Column c uses seeds[c] as its
+ * rng_seed (sim->rng_seed + c when seeds is NULL); results are bit-identical
+ * to marl_tick on each column alone.  Needs an even, positive substep count,
+ * plain kinetics and no harvest. */
+int marl_ensemble_create(marl_ctx* const* columns, int32_t n_columns, marl_ensemble** out);
+void marl_ensemble_destroy(marl_ensemble* e);
+const char* marl_ensemble_last_error(const marl_ensemble* e);
+/* stats: n_columns entries (may be NULL); *host_finished counts column-ticks
+ * whose cell phase was finished host-driven; *ms_out = GPU time of the graphs. */
+int marl_ensemble_tick(marl_ensemble* e, const marl_sim_params* sim, const uint64_t* seeds, int64_t tick0,
+                       int32_t n_ticks, marl_tick_stats* stats, int32_t* host_finished, float* ms_out);
+int marl_ensemble_set_tile(marl_ensemble* e, int32_t tile);
+/* Average time of one batched substep over all columns (fields unchanged). */
+int marl_ensemble_time_field(marl_ensemble* e, const marl_sim_params* sim, int32_t n_substeps, float* ms_out);
+/* Times every stencil tile shape on the resident columns and keeps the fastest. */
+int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t* tile_out);
 
 /* ---- learned kinetics surrogate ------------------------------------------- */
 /* No reference counterpart: the reference integrates every cell's kinetics

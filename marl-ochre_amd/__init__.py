@@ -8,9 +8,9 @@ importlib under the name `marl_ochre_amd` (see __graft_entry__.load_package).
 from ._abi import (CELL_DTYPE, RULESET_DTYPE, BufferMap, DeviceRequest, GpuError, InvalidInput, NoAdapter,
                    load_library)
 from .config import SimulationConfig
-from .engine import GpuFieldDiffuser, MarlGpu
+from .engine import GpuFieldDiffuser, MarlEnsemble, MarlGpu
 
 __all__ = [
     "CELL_DTYPE", "RULESET_DTYPE", "GpuError", "NoAdapter", "DeviceRequest", "BufferMap", "InvalidInput",
-    "load_library", "SimulationConfig", "GpuFieldDiffuser", "MarlGpu",
+    "load_library", "SimulationConfig", "GpuFieldDiffuser", "MarlGpu", "MarlEnsemble",
 ]

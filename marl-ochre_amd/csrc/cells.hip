@@ -1139,21 +1139,26 @@ inline unsigned gs_grid(int64_t n, int block = 256, int cap_blocks = 2048) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, cap_blocks));
 }
 
-// Enqueues one device-driven tick on ctx->stream (capturable: no host
-// synchronisation, no allocation, no memset nodes).
-int enqueue_device_tick(marl_ctx* ctx, const marl_sim_params* sim) {
+}  // namespace
+
+// The device-driven tick in three parts, each enqueued on ctx->stream and
+// capturable (no host synchronisation, no allocation, no memset nodes):
+// counters and occupancy; the field phase; the cell phase.
+int marl_enqueue_tick_begin(marl_ctx* ctx, const marl_sim_params* sim) {
   TickDev* ts = static_cast<TickDev*>(ctx->tdev);
-  const int64_t cap = ctx->cap;
-  const CellParams p = cell_params(ctx, sim, 0);  // the RNG stream comes from ts->tick
-  hipStream_t st = ctx->stream;
-  marl_cell_state* cells = ctx->cells[ctx->cbuf];
-  marl_ruleset* rules = ctx->rules[ctx->cbuf];
-  const unsigned gc = gs_grid(cap), gk = gs_grid(cap, kKinBlock, 4096), gf = gs_grid(cap + 1);
-  hipLaunchKernelGGL(k_g_begin, dim3(1), dim3(64), 0, st, ts, cap);
-  // field phase: occupancy (lib.rs:174-179), substeps with sources fused, light
+  const CellParams p = cell_params(ctx, sim, 0);
+  hipLaunchKernelGGL(k_g_begin, dim3(1), dim3(64), 0, ctx->stream, ts, ctx->cap);
+  // occupancy from the live cells (lib.rs:174-179)
   if (int rc = fill_i32(ctx, ctx->cell_at, -1, ctx->nvox)) return rc;
   if (int rc = fill_i32(ctx, reinterpret_cast<int32_t*>(ctx->occ), 0, ctx->occ_words)) return rc;
-  hipLaunchKernelGGL(k_g_scatter_cells, dim3(gc), dim3(256), 0, st, cells, ts, ctx->cell_at, ctx->occ, p);
+  hipLaunchKernelGGL(k_g_scatter_cells, dim3(gs_grid(ctx->cap)), dim3(256), 0, ctx->stream, ctx->cells[ctx->cbuf],
+                     ts, ctx->cell_at, ctx->occ, p);
+  MARL_HIP(ctx, hipGetLastError());
+  return MARL_OK;
+}
+
+// Substeps with the boundary sources fused into the first, then light.
+int marl_enqueue_field_phase(marl_ctx* ctx, const marl_sim_params* sim) {
   if (sim->diffusion_substeps > 0) {
     for (int k = 0; k < sim->diffusion_substeps; ++k) {
       const StencilParams sp = marl_stencil_params(sim, k == 0 ? 1 : 0);
@@ -1162,7 +1167,18 @@ int enqueue_device_tick(marl_ctx* ctx, const marl_sim_params* sim) {
   } else {
     if (int rc = marl_launch_boundary(ctx, sim)) return rc;
   }
-  if (int rc = marl_launch_light(ctx, sim)) return rc;
+  return marl_launch_light(ctx, sim);
+}
+
+// Schedule, the ordered pass, fate, births, deaths and compaction.
+int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
+  TickDev* ts = static_cast<TickDev*>(ctx->tdev);
+  const int64_t cap = ctx->cap;
+  const CellParams p = cell_params(ctx, sim, 0);  // the RNG stream comes from ts->tick
+  hipStream_t st = ctx->stream;
+  marl_cell_state* cells = ctx->cells[ctx->cbuf];
+  marl_ruleset* rules = ctx->rules[ctx->cbuf];
+  const unsigned gc = gs_grid(cap), gk = gs_grid(cap, kKinBlock, 4096), gf = gs_grid(cap + 1);
   // schedule
   hipLaunchKernelGGL(k_g_cell_prep, dim3(gc), dim3(256), 0, st, cells, ts, ctx->cell_at, ctx->occ, ctx->nbmask,
                      ctx->preds, ctx->npred, ctx->level, p);
@@ -1217,7 +1233,7 @@ int enqueue_device_tick(marl_ctx* ctx, const marl_sim_params* sim) {
 }
 
 // What a captured tick depends on: parameters, tile, buffers and their parity.
-std::vector<uint8_t> graph_key(const marl_ctx* ctx, const marl_sim_params* sim) {
+std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* sim) {
   struct K {
     marl_sim_params sim;
     int64_t cap;
@@ -1241,6 +1257,75 @@ std::vector<uint8_t> graph_key(const marl_ctx* ctx, const marl_sim_params* sim) 
   return std::vector<uint8_t>(b, b + sizeof(k));
 }
 
+// Buffers, capacity for this tick's births, scan storage and the device
+// counters of a device-driven tick (allocations happen here, never during
+// capture).
+int marl_graph_prepare(marl_ctx* ctx, const marl_sim_params* sim) {
+  if (int rc = marl_check_params(ctx, sim)) return rc;
+  if (sim->diffusion_substeps % 2 != 0 || ctx->kin_mode != MARL_KINETICS_PLAIN ||
+      (ctx->surr && ctx->surr->harvesting))
+    return marl_fail(ctx, MARL_ERR_INVALID_INPUT,
+                     "the captured tick needs an even substep count, plain kinetics and no harvest");
+  ctx->last_sim = *sim;
+  if (int rc = ensure_voxel_buffers(ctx)) return rc;
+  if (int rc = marl_ensure_staging(ctx)) return rc;
+  if (!ctx->tdev) MARL_HIP(ctx, hipMalloc(&ctx->tdev, sizeof(TickDev)));
+  const int64_t n = ctx->n_cells;
+  // room for this tick's births; generous so that growth rarely re-captures
+  if (int rc = marl_ensure_cell_capacity(ctx, std::max<int64_t>(2 * n + 1024, 4 * n))) return rc;
+  return scan_reserve(ctx, (int)ctx->cap + 1);
+}
+
+static TickDev* host_tickdev(marl_ctx* ctx) {
+  static_assert(sizeof(TickDev) <= 2048, "TickDev fits the pinned scratch");
+  return reinterpret_cast<TickDev*>(reinterpret_cast<uint8_t*>(ctx->pinned) + 2048);
+}
+
+// Before a replay: the live count and tick index into the device counters.
+int marl_graph_stage(marl_ctx* ctx, hipStream_t st, int64_t tick) {
+  TickDev* hts = host_tickdev(ctx);
+  TickDev* ts = static_cast<TickDev*>(ctx->tdev);
+  hts->n = (int32_t)ctx->n_cells;
+  hts->tick = (uint64_t)tick;
+  MARL_HIP(ctx, hipMemcpyAsync(&ts->n, &hts->n, sizeof(int32_t), hipMemcpyHostToDevice, st));
+  MARL_HIP(ctx, hipMemcpyAsync(&ts->tick, &hts->tick, sizeof(uint64_t), hipMemcpyHostToDevice, st));
+  return MARL_OK;
+}
+
+// After a replay: the counters back (the caller synchronises st).
+int marl_graph_fetch(marl_ctx* ctx, hipStream_t st) {
+  MARL_HIP(ctx, hipMemcpyAsync(host_tickdev(ctx), ctx->tdev, sizeof(TickDev), hipMemcpyDeviceToHost, st));
+  return MARL_OK;
+}
+
+// After the fetch completed: the tick's outcome, finishing the cell phase on
+// the host-driven path if the device skipped it.
+int marl_graph_post(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick, marl_tick_stats* st,
+                    int32_t* host_finished) {
+  const TickDev* hts = host_tickdev(ctx);
+  *st = marl_tick_stats{};
+  if (hts->err == 0) {
+    ctx->n_cells = hts->n;
+    st->divisions = hts->divisions;
+    st->deaths = hts->deaths;
+    st->births_added = hts->births_added;
+    st->levels = hts->levels;
+  } else {
+    ++*host_finished;
+    if (int rc = marl_ensure_cell_capacity(ctx, 2 * ctx->n_cells + 1024)) return rc;
+    const CellParams p = cell_params(ctx, sim, (uint64_t)tick);
+    int32_t levels = 0;
+    if (int rc = run_cell_pass(ctx, p, &levels, nullptr)) return rc;
+    if (int rc = run_fate(ctx, p, st)) return rc;
+    MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    st->levels = levels;
+  }
+  st->n_cells = ctx->n_cells;
+  return MARL_OK;
+}
+
+namespace {
+
 int drop_graph(marl_ctx* ctx) {
   if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
@@ -1256,7 +1341,9 @@ int capture_tick(marl_ctx* ctx, const marl_sim_params* sim) {
   drop_graph(ctx);
   const int cur0 = ctx->cur;
   MARL_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = enqueue_device_tick(ctx, sim);
+  int rc = marl_enqueue_tick_begin(ctx, sim);
+  if (rc == MARL_OK) rc = marl_enqueue_field_phase(ctx, sim);
+  if (rc == MARL_OK) rc = marl_enqueue_cell_phase(ctx, sim);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
   ctx->cur = cur0;
@@ -1267,7 +1354,7 @@ int capture_tick(marl_ctx* ctx, const marl_sim_params* sim) {
   MARL_HIP(ctx, e);
   ctx->graph = g;
   MARL_HIP(ctx, hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
-  ctx->graph_key = graph_key(ctx, sim);
+  ctx->graph_key = marl_graph_key(ctx, sim);
   return MARL_OK;
 }
 
@@ -1426,61 +1513,26 @@ int marl_set_graph_limits(marl_ctx* ctx, int32_t sweeps, int32_t levels) {
 int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t n_ticks,
                     marl_tick_stats* stats, int32_t* host_finished) {
   if (!ctx) return MARL_ERR_INVALID_INPUT;
-  if (int rc = marl_check_params(ctx, sim)) return rc;
   if (n_ticks < 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "negative tick count");
-  if (sim->diffusion_substeps % 2 != 0 || ctx->kin_mode != MARL_KINETICS_PLAIN ||
-      (ctx->surr && ctx->surr->harvesting))
-    return marl_fail(ctx, MARL_ERR_INVALID_INPUT,
-                     "the captured tick needs an even substep count, plain kinetics and no harvest");
-  ctx->last_sim = *sim;
-  if (int rc = ensure_voxel_buffers(ctx)) return rc;
-  if (int rc = marl_ensure_staging(ctx)) return rc;
-  if (!ctx->tdev) MARL_HIP(ctx, hipMalloc(&ctx->tdev, sizeof(TickDev)));
-  TickDev* hts = reinterpret_cast<TickDev*>(reinterpret_cast<uint8_t*>(ctx->pinned) + 2048);
-  static_assert(sizeof(TickDev) <= 2048, "TickDev fits the pinned scratch");
-  TickDev* ts = static_cast<TickDev*>(ctx->tdev);
   marl_tick_stats st{};
-  int32_t finished_by_host = 0;
+  int32_t finished = 0;
   for (int t = 0; t < n_ticks; ++t) {
-    st = marl_tick_stats{};
-    const int64_t n = ctx->n_cells;
-    // room for this tick's births; generous so that growth rarely re-captures
-    if (int rc = marl_ensure_cell_capacity(ctx, std::max<int64_t>(2 * n + 1024, 4 * n))) return rc;
-    if (int rc = scan_reserve(ctx, (int)ctx->cap + 1)) return rc;
-    if (ctx->graph_key != graph_key(ctx, sim))
+    if (int rc = marl_graph_prepare(ctx, sim)) return rc;
+    if (ctx->graph_key != marl_graph_key(ctx, sim))
       if (int rc = capture_tick(ctx, sim)) return rc;
-    // the device record carries n and the tick index between replays
-    hts->n = (int32_t)n;
-    hts->tick = (uint64_t)(tick0 + t);
-    MARL_HIP(ctx, hipMemcpyAsync(&ts->n, &hts->n, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    MARL_HIP(ctx, hipMemcpyAsync(&ts->tick, &hts->tick, sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+    if (int rc = marl_graph_stage(ctx, ctx->stream, tick0 + t)) return rc;
     MARL_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     MARL_HIP(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
     MARL_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    MARL_HIP(ctx, hipMemcpyAsync(hts, ts, sizeof(TickDev), hipMemcpyDeviceToHost, ctx->stream));
+    if (int rc = marl_graph_fetch(ctx, ctx->stream)) return rc;
     MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    (void)hipEventElapsedTime(&st.ms_field, ctx->ev0, ctx->ev1);
-    if (hts->err == 0) {
-      ctx->n_cells = hts->n;
-      st.divisions = hts->divisions;
-      st.deaths = hts->deaths;
-      st.births_added = hts->births_added;
-      st.levels = hts->levels;
-    } else {
-      // the field phase ran; the cell phase was skipped: finish it host-driven
-      ++finished_by_host;
-      if (int rc = marl_ensure_cell_capacity(ctx, 2 * n + 1024)) return rc;
-      const CellParams p = cell_params(ctx, sim, (uint64_t)(tick0 + t));
-      int32_t levels = 0;
-      if (int rc = run_cell_pass(ctx, p, &levels, nullptr)) return rc;
-      if (int rc = run_fate(ctx, p, &st)) return rc;
-      MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      st.levels = levels;
-    }
-    st.n_cells = ctx->n_cells;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (int rc = marl_graph_post(ctx, sim, tick0 + t, &st, &finished)) return rc;
+    st.ms_field = ms;
   }
   if (stats) *stats = st;
-  if (host_finished) *host_finished = finished_by_host;
+  if (host_finished) *host_finished = finished;
   return MARL_OK;
 }
 

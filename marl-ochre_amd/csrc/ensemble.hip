@@ -1,0 +1,284 @@
+// ensemble.hip — ensembles of independent columns of uneven sizes and seeds,
+// advanced together.
+//
+// No reference counterpart: the reference runs one column per process
+// (crates/marl-sim/src/lib.rs:160-277).  Every column is an ordinary context
+// (its own dims, cells and seed); the ensemble owns a stream and one hipGraph
+// per tick shape:
+//   - fork: each column's counters and occupancy on the column's stream;
+//   - join, then the field phase of all columns in one persistent stencil
+//     launch per substep and one light launch (k_stencil_ens, k_light_ens);
+//   - fork: each column's cell phase (schedule, ordered pass, fate, compaction)
+//     on its own stream, so small columns run side by side; join.
+// Results are bit-identical to ticking each column alone.
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "marl_internal.h"
+
+struct marl_ensemble {
+  std::vector<marl_ctx*> cols;
+  hipStream_t stream = nullptr;
+  hipEvent_t fork[2] = {nullptr, nullptr};
+  std::vector<hipEvent_t> join;  // two per column
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<EnsColumn> hcols;
+  EnsColumn* dcols = nullptr;
+  int tile = 3;
+  int total_tiles = 0, total_light = 0, grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint8_t> key;
+  std::string err;
+};
+
+namespace {
+
+int efail(marl_ensemble* e, int code, const std::string& msg) {
+  if (e) e->err = msg;
+  return code;
+}
+
+#define ENS_HIP(e, call)                                                                            \
+  do {                                                                                              \
+    hipError_t _err = (call);                                                                       \
+    if (_err != hipSuccess) return efail((e), MARL_ERR_TRANSFER, std::string(#call ": ") + hipGetErrorString(_err)); \
+  } while (0)
+
+// a column's failure, reported on the ensemble
+int col_fail(marl_ensemble* e, int c, int rc) {
+  return efail(e, rc, "column " + std::to_string(c) + ": " + marl_last_error(e->cols[c]));
+}
+
+void drop(marl_ensemble* e) {
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->gexec = nullptr;
+  e->graph = nullptr;
+  e->key.clear();
+}
+
+// The batched kernels' view of the columns for the ensemble's tile shape.
+int build_table(marl_ensemble* e) {
+  const int n = (int)e->cols.size();
+  e->hcols.assign(n, EnsColumn{});
+  for (int c = 0; c < n; ++c) {
+    marl_ctx* x = e->cols[c];
+    EnsColumn& t = e->hcols[c];
+    t.f[0] = x->field[0];
+    t.f[1] = x->field[1];
+    t.occ = x->occ;
+    t.light = x->light;
+    t.X = x->d.x;
+    t.Y = x->d.y;
+    t.Z = x->d.z;
+    t.XW = x->xw;
+    t.cur = x->cur;
+  }
+  marl_ens_layout(e->tile, e->hcols.data(), n, &e->total_tiles, &e->total_light);
+  e->grid = marl_stencil_ens_grid(e->tile, e->total_tiles);
+  if (!e->dcols) ENS_HIP(e, hipMalloc(&e->dcols, sizeof(EnsColumn) * n));
+  ENS_HIP(e, hipMemcpyAsync(e->dcols, e->hcols.data(), sizeof(EnsColumn) * n, hipMemcpyHostToDevice, e->stream));
+  ENS_HIP(e, hipStreamSynchronize(e->stream));
+  return MARL_OK;
+}
+
+std::vector<uint8_t> ens_key(marl_ensemble* e, const std::vector<marl_sim_params>& sims) {
+  std::vector<uint8_t> k;
+  for (size_t c = 0; c < e->cols.size(); ++c) {
+    const std::vector<uint8_t> kc = marl_graph_key(e->cols[c], &sims[c]);
+    k.insert(k.end(), kc.begin(), kc.end());
+  }
+  k.push_back((uint8_t)e->tile);
+  return k;
+}
+
+int enqueue_ensemble_tick(marl_ensemble* e, const std::vector<marl_sim_params>& sims) {
+  const int n = (int)e->cols.size();
+  const marl_sim_params& sim = sims[0];
+  ENS_HIP(e, hipEventRecord(e->fork[0], e->stream));
+  for (int c = 0; c < n; ++c) {
+    marl_ctx* x = e->cols[c];
+    ENS_HIP(e, hipStreamWaitEvent(x->stream, e->fork[0], 0));
+    if (int rc = marl_enqueue_tick_begin(x, &sims[c])) return col_fail(e, c, rc);
+    ENS_HIP(e, hipEventRecord(e->join[2 * c], x->stream));
+    ENS_HIP(e, hipStreamWaitEvent(e->stream, e->join[2 * c], 0));
+  }
+  for (int k = 0; k < sim.diffusion_substeps; ++k)
+    ENS_HIP(e, marl_launch_substep_ens(e->stream, e->dcols, n, e->total_tiles, k & 1,
+                                       marl_stencil_params(&sim, k == 0 ? 1 : 0), e->tile, e->grid));
+  ENS_HIP(e, marl_launch_light_ens(e->stream, e->dcols, n, e->total_light, &sim));
+  ENS_HIP(e, hipEventRecord(e->fork[1], e->stream));
+  for (int c = 0; c < n; ++c) {
+    marl_ctx* x = e->cols[c];
+    ENS_HIP(e, hipStreamWaitEvent(x->stream, e->fork[1], 0));
+    if (int rc = marl_enqueue_cell_phase(x, &sims[c])) return col_fail(e, c, rc);
+    ENS_HIP(e, hipEventRecord(e->join[2 * c + 1], x->stream));
+    ENS_HIP(e, hipStreamWaitEvent(e->stream, e->join[2 * c + 1], 0));
+  }
+  return MARL_OK;
+}
+
+int capture(marl_ensemble* e, const std::vector<marl_sim_params>& sims) {
+  drop(e);
+  if (int rc = build_table(e)) return rc;
+  ENS_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_ensemble_tick(e, sims);
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(e->stream, &g);
+  if (rc != MARL_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  ENS_HIP(e, ee);
+  e->graph = g;
+  ENS_HIP(e, hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+  e->key = ens_key(e, sims);
+  return MARL_OK;
+}
+
+std::vector<marl_sim_params> column_params(const marl_ensemble* e, const marl_sim_params* sim, const uint64_t* seeds) {
+  std::vector<marl_sim_params> sims(e->cols.size(), *sim);
+  for (size_t c = 0; c < sims.size(); ++c) sims[c].rng_seed = seeds ? seeds[c] : sim->rng_seed + c;
+  return sims;
+}
+
+}  // namespace
+
+extern "C" {
+
+int marl_ensemble_create(marl_ctx* const* columns, int32_t n_columns, marl_ensemble** out) {
+  if (!out) return MARL_ERR_INVALID_INPUT;
+  *out = nullptr;
+  if (!columns || n_columns < 1 || n_columns > 1024) return MARL_ERR_INVALID_INPUT;
+  for (int c = 0; c < n_columns; ++c) {
+    if (!columns[c]) return MARL_ERR_INVALID_INPUT;
+    for (int d = 0; d < c; ++d)
+      if (columns[d] == columns[c]) return MARL_ERR_INVALID_INPUT;
+  }
+  marl_ensemble* e = new marl_ensemble();
+  e->cols.assign(columns, columns + n_columns);
+  auto fail = [&](hipError_t) {
+    marl_ensemble_destroy(e);
+    return MARL_ERR_DEVICE_INIT;
+  };
+  hipError_t r;
+  if ((r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) return fail(r);
+  for (auto& f : e->fork)
+    if ((r = hipEventCreateWithFlags(&f, hipEventDisableTiming)) != hipSuccess) return fail(r);
+  e->join.assign(2 * n_columns, nullptr);
+  for (auto& j : e->join)
+    if ((r = hipEventCreateWithFlags(&j, hipEventDisableTiming)) != hipSuccess) return fail(r);
+  if ((r = hipEventCreate(&e->ev0)) != hipSuccess) return fail(r);
+  if ((r = hipEventCreate(&e->ev1)) != hipSuccess) return fail(r);
+  *out = e;
+  return MARL_OK;
+}
+
+void marl_ensemble_destroy(marl_ensemble* e) {
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop(e);
+  if (e->dcols) (void)hipFree(e->dcols);
+  for (auto f : e->fork)
+    if (f) (void)hipEventDestroy(f);
+  for (auto j : e->join)
+    if (j) (void)hipEventDestroy(j);
+  if (e->ev0) (void)hipEventDestroy(e->ev0);
+  if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+const char* marl_ensemble_last_error(const marl_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int marl_ensemble_set_tile(marl_ensemble* e, int32_t tile) {
+  if (!e) return MARL_ERR_INVALID_INPUT;
+  if (tile < 0 || tile >= marl_stencil_tile_count()) return efail(e, MARL_ERR_INVALID_INPUT, "unknown tile id");
+  if (tile != e->tile) drop(e);
+  e->tile = tile;
+  return MARL_OK;
+}
+
+int marl_ensemble_tick(marl_ensemble* e, const marl_sim_params* sim, const uint64_t* seeds, int64_t tick0,
+                       int32_t n_ticks, marl_tick_stats* stats, int32_t* host_finished, float* ms_out) {
+  if (!e || !sim) return MARL_ERR_INVALID_INPUT;
+  if (n_ticks < 0) return efail(e, MARL_ERR_INVALID_INPUT, "negative tick count");
+  if (sim->diffusion_substeps < 2 || sim->diffusion_substeps % 2 != 0)
+    return efail(e, MARL_ERR_INVALID_INPUT, "ensembles need an even, positive substep count");
+  const int n = (int)e->cols.size();
+  const std::vector<marl_sim_params> sims = column_params(e, sim, seeds);
+  int32_t finished = 0;
+  float ms_total = 0.0f;
+  std::vector<marl_tick_stats> st(n);
+  for (int t = 0; t < n_ticks; ++t) {
+    for (int c = 0; c < n; ++c)
+      if (int rc = marl_graph_prepare(e->cols[c], &sims[c])) return col_fail(e, c, rc);
+    if (e->key != ens_key(e, sims))
+      if (int rc = capture(e, sims)) return rc;
+    for (int c = 0; c < n; ++c)
+      if (int rc = marl_graph_stage(e->cols[c], e->stream, tick0 + t)) return col_fail(e, c, rc);
+    ENS_HIP(e, hipEventRecord(e->ev0, e->stream));
+    ENS_HIP(e, hipGraphLaunch(e->gexec, e->stream));
+    ENS_HIP(e, hipEventRecord(e->ev1, e->stream));
+    for (int c = 0; c < n; ++c)
+      if (int rc = marl_graph_fetch(e->cols[c], e->stream)) return col_fail(e, c, rc);
+    ENS_HIP(e, hipStreamSynchronize(e->stream));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e->ev0, e->ev1);
+    ms_total += ms;
+    for (int c = 0; c < n; ++c) {
+      if (int rc = marl_graph_post(e->cols[c], &sims[c], tick0 + t, &st[c], &finished)) return col_fail(e, c, rc);
+      st[c].ms_field = ms;
+    }
+  }
+  if (stats)
+    for (int c = 0; c < n; ++c) stats[c] = st[c];
+  if (host_finished) *host_finished = finished;
+  if (ms_out) *ms_out = ms_total;
+  return MARL_OK;
+}
+
+int marl_ensemble_time_field(marl_ensemble* e, const marl_sim_params* sim, int32_t n_substeps, float* ms_out) {
+  if (!e || !sim || n_substeps <= 0 || !ms_out) return MARL_ERR_INVALID_INPUT;
+  drop(e);  // the table is rebuilt for the current tile
+  if (int rc = build_table(e)) return rc;
+  const int n = (int)e->cols.size();
+  const StencilParams sp = marl_stencil_params(sim, 0);
+  // every launch reads each column's current buffer and writes the other one,
+  // so the resident fields are left unchanged
+  ENS_HIP(e, marl_launch_substep_ens(e->stream, e->dcols, n, e->total_tiles, 0, sp, e->tile, e->grid));
+  ENS_HIP(e, hipEventRecord(e->ev0, e->stream));
+  for (int k = 0; k < n_substeps; ++k)
+    ENS_HIP(e, marl_launch_substep_ens(e->stream, e->dcols, n, e->total_tiles, 0, sp, e->tile, e->grid));
+  ENS_HIP(e, hipEventRecord(e->ev1, e->stream));
+  ENS_HIP(e, hipEventSynchronize(e->ev1));
+  float ms = 0.0f;
+  ENS_HIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+  *ms_out = ms / (float)n_substeps;
+  return MARL_OK;
+}
+
+// The ensemble's tile shape, picked by timing every candidate on the resident
+// columns (the fields are left unchanged).
+int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t* tile_out) {
+  if (!e || !sim) return MARL_ERR_INVALID_INPUT;
+  int best = e->tile;
+  float best_ms = 1e30f;
+  for (int t = 0; t < marl_stencil_tile_count(); ++t) {
+    e->tile = t;
+    float ms = 0.0f;
+    if (int rc = marl_ensemble_time_field(e, sim, 5, &ms)) return rc;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  drop(e);
+  e->tile = best;
+  if (tile_out) *tile_out = best;
+  return MARL_OK;
+}
+
+}  // extern "C"

@@ -73,38 +73,38 @@ __device__ __forceinline__ float ld_field(rsrc_t r, const StencilParams& p, int 
   return v;
 }
 
+// LDS of one tile: species planes with a one-voxel halo (one spare row takes
+// the writes of lanes that own no halo value, so those writes need no branch)
+// and the mask words of the tile's rows.
+template <int RY, int SG>
+struct StencilLds {
+  float f[SG][4 * RY + 3][66];
+  uint32_t m[(4 * RY + 2) * 4 + 256];
+};
+
+// XCD-aware tile order: hardware dispatches block b to XCD b % 8; give each
+// XCD a contiguous run of logical tiles so x/y-neighbour tiles (which share
+// halo lines) sit in the same L2 and march through z together.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int nb8 = nb & ~7;
+  return b < nb8 ? (b & 7) * (nb8 >> 3) + (b >> 3) : b;
+}
+
+// One tile (tx, ty, tz) of species group grp of one substep.
 template <int RY, bool SRC, int PD, int SG>
-__global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, float* __restrict__ dst,
-                                                 const uint32_t* __restrict__ occ, int X, int Y, int Z, int XW,
-                                                 int zc, int ntx, int nty, int ntz, StencilParams p) {
+__device__ __forceinline__ void stencil_tile(const float* __restrict__ src, float* __restrict__ dst,
+                                             const uint32_t* __restrict__ occ, int X, int Y, int Z, int XW, int zc,
+                                             int tx, int ty, int tz, int grp, const StencilParams& p,
+                                             StencilLds<RY, SG>& sh) {
   constexpr int TY = 4 * RY;
-  constexpr int LW = 66;
   constexpr int S = SG;              // species handled by this block
-  constexpr int NG = MARL_S_EXT / SG;
-  constexpr bool OWN7 = false;       // species 7 is in the last group only when SG == 12 (see s7 below)
   constexpr int QN = PD + 2;  // own planes held: z-1, z, z+1 .. z+PD
   constexpr int HN = PD + 1;  // halo planes held: z .. z+PD
-  // one spare LDS row (TY + 2) and spare mask slots take the writes of lanes
-  // that own no halo value, so those writes need no branch
-  __shared__ float lds[S][TY + 3][LW];
-  __shared__ uint32_t mlds[(TY + 2) * 4 + 256];
-
-  // XCD-aware tile order: hardware dispatches block b to XCD b % 8; give each
-  // XCD a contiguous run of logical tiles so x/y-neighbour tiles (which share
-  // halo lines) sit in the same L2 and march through z together.
-  const int nb = ntx * nty * ntz * NG;
-  int b = blockIdx.x;
-  const int nb8 = nb & ~7;
-  if (b < nb8) b = (b & 7) * (nb8 >> 3) + (b >> 3);
-  // species group fastest: the groups of one tile run side by side on one XCD
-  const int grp = b % NG;
-  b /= NG;
-  const int tx = b % ntx;
-  const int ty = (b / ntx) % nty;
-  const int tz = b / (ntx * nty);
+  constexpr int LW = 66;
+  auto& lds = sh.f;
+  auto& mlds = sh.m;
   const int s0 = grp * SG;          // first species of this block
   const bool has7 = s0 <= 7 && 7 < s0 + SG;
-  (void)OWN7;
 
   const int tid = threadIdx.x;
   const int lx = tid & 63, wy = tid >> 6;
@@ -332,6 +332,43 @@ __global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, 
 }
 
 template <int RY, bool SRC, int PD, int SG>
+__global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, float* __restrict__ dst,
+                                                 const uint32_t* __restrict__ occ, int X, int Y, int Z, int XW,
+                                                 int zc, int ntx, int nty, int ntz, StencilParams p) {
+  __shared__ StencilLds<RY, SG> sh;
+  constexpr int NG = MARL_S_EXT / SG;
+  int b = xcd_remap(blockIdx.x, ntx * nty * ntz * NG);
+  // species group fastest: the groups of one tile run side by side on one XCD
+  const int grp = b % NG;
+  b /= NG;
+  stencil_tile<RY, SRC, PD, SG>(src, dst, occ, X, Y, Z, XW, zc, b % ntx, (b / ntx) % nty, b / (ntx * nty), grp, p, sh);
+}
+
+// Ensembles: one persistent launch covers every column of the ensemble.  The
+// logical tiles of all columns are numbered back to back (column after
+// column, each in the single-column order), spread over the XCDs by the same
+// remap, and each block walks its share with a grid stride.
+template <int RY, bool SRC, int PD, int SG>
+__global__ void __launch_bounds__(256) k_stencil_ens(const EnsColumn* __restrict__ cols, int ncols, int total,
+                                                     int parity, StencilParams p) {
+  __shared__ StencilLds<RY, SG> sh;
+  constexpr int NG = MARL_S_EXT / SG;
+  for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    int b = xcd_remap(w, total);
+    int c = 0;
+    while (c + 1 < ncols && cols[c + 1].tile_base <= b) ++c;
+    const EnsColumn& col = cols[c];
+    b -= col.tile_base;
+    const int grp = b % NG;
+    b /= NG;
+    const int cur = col.cur ^ parity;
+    stencil_tile<RY, SRC, PD, SG>(col.f[cur], col.f[cur ^ 1], col.occ, col.X, col.Y, col.Z, col.XW, col.zc,
+                                  b % col.ntx, (b / col.ntx) % col.nty, b / (col.ntx * col.nty), grp, p, sh);
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
+template <int RY, bool SRC, int PD, int SG>
 static hipError_t launch_stencil_t(marl_ctx* ctx, const StencilParams& p, int zc) {
   const int X = ctx->d.x, Y = ctx->d.y, Z = ctx->d.z;
   const int ntx = (X + 63) / 64, nty = (Y + 4 * RY - 1) / (4 * RY), ntz = (Z + zc - 1) / zc;
@@ -367,6 +404,74 @@ int marl_launch_substep(marl_ctx* ctx, const StencilParams& p, int tile) {
   return MARL_OK;
 }
 
+template <int RY, int PD, int SG>
+static hipError_t launch_stencil_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total, int parity,
+                                     const StencilParams& p, int grid) {
+  if (p.apply_sources)
+    hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG>), dim3(grid), dim3(256), 0, st, dcols, ncols, total, parity, p);
+  else
+    hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG>), dim3(grid), dim3(256), 0, st, dcols, ncols, total, parity,
+                       p);
+  return hipGetLastError();
+}
+
+void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int* total_light_blocks) {
+  if (tile < 0 || tile >= kNumTiles) tile = 3;
+  const TileShape t = kTiles[tile];
+  const int NG = MARL_S_EXT / t.sg;
+  int base = 0, lbase = 0;
+  for (int c = 0; c < ncols; ++c) {
+    EnsColumn& e = cols[c];
+    e.zc = t.zc < e.Z ? t.zc : e.Z;
+    e.ntx = (e.X + 63) / 64;
+    e.nty = (e.Y + 4 * t.ry - 1) / (4 * t.ry);
+    e.ntz = (e.Z + e.zc - 1) / e.zc;
+    e.tile_base = base;
+    base += e.ntx * e.nty * e.ntz * NG;
+    e.lbx = (e.X + 63) / 64;
+    e.light_base = lbase;
+    lbase += e.lbx * e.Y;
+  }
+  *total_tiles = base;
+  *total_light_blocks = lbase;
+}
+
+// Persistent grid: as many blocks as fit the chip at once (occupancy of the
+// instantiation x CUs), a multiple of 8 so every XCD gets the same share.
+int marl_stencil_ens_grid(int tile, int total_tiles) {
+  if (tile < 0 || tile >= kNumTiles) tile = 3;
+  const TileShape t = kTiles[tile];
+  int per_cu = 0;
+  hipError_t e = hipErrorInvalidValue;
+  if (t.sg == 12 && t.ry == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stencil_ens<1, false, 1, 12>, 256, 0);
+  else if (t.sg == 12) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stencil_ens<2, false, 1, 12>, 256, 0);
+  else if (t.ry == 2 && t.pd == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stencil_ens<2, false, 1, 6>, 256, 0);
+  else if (t.ry == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stencil_ens<2, false, 2, 6>, 256, 0);
+  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stencil_ens<4, false, 1, 6>, 256, 0);
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess || per_cu < 1) per_cu = 1;
+  int grid = per_cu * cus;
+  grid = grid < total_tiles ? grid : total_tiles;
+  if (grid > 8) grid &= ~7;
+  return grid < 1 ? 1 : grid;
+}
+
+hipError_t marl_launch_substep_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total, int parity,
+                                   const StencilParams& p, int tile, int grid) {
+  if (tile < 0 || tile >= kNumTiles) tile = 3;
+  const TileShape t = kTiles[tile];
+  if (t.sg == 12) {
+    if (t.ry == 1 && t.pd == 1) return launch_stencil_ens<1, 1, 12>(st, dcols, ncols, total, parity, p, grid);
+    if (t.ry == 2 && t.pd == 1) return launch_stencil_ens<2, 1, 12>(st, dcols, ncols, total, parity, p, grid);
+  } else if (t.sg == 6) {
+    if (t.ry == 2 && t.pd == 1) return launch_stencil_ens<2, 1, 6>(st, dcols, ncols, total, parity, p, grid);
+    if (t.ry == 2 && t.pd == 2) return launch_stencil_ens<2, 2, 6>(st, dcols, ncols, total, parity, p, grid);
+    if (t.ry == 4 && t.pd == 1) return launch_stencil_ens<4, 1, 6>(st, dcols, ncols, total, parity, p, grid);
+  }
+  return hipErrorInvalidValue;
+}
+
 // ---------------------------------------------------------------------------
 // Boundary sources in place (field.rs:231-250): one thread per (x, y).
 __global__ void k_boundary(float* __restrict__ f, int X, int Y, int Z, float src_ox, float src_c, float src_red,
@@ -393,12 +498,9 @@ int marl_launch_boundary(marl_ctx* ctx, const marl_sim_params* sim) {
 // K2: Beer-Lambert light (light.rs:29-59).  One lane per (x, y) column,
 // coalesced across x; the z loop carries the intensity in a register.  The
 // sequential multiply order of the reference is kept per column.
-__global__ void __launch_bounds__(64) k_light(const float* __restrict__ s4, const uint32_t* __restrict__ occ,
-                                              float* __restrict__ light, int X, int Y, int Z, int XW, float I0,
-                                              float e_cell, float nchem, float floor_) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y;
-  if (x >= X) return;
+__device__ __forceinline__ void light_column(const float* __restrict__ s4, const uint32_t* __restrict__ occ,
+                                             float* __restrict__ light, int x, int y, int X, int Y, int Z, int XW,
+                                             float I0, float e_cell, float nchem, float floor_) {
   const size_t XY = (size_t)X * Y;
   const size_t col = (size_t)y * X + x;
   const uint32_t bit = 1u << (x & 31);
@@ -428,6 +530,37 @@ __global__ void __launch_bounds__(64) k_light(const float* __restrict__ s4, cons
     I *= marl_expf(nchem * s4[(size_t)z * XY + col]);
     if (I < floor_) I = 0.0f;
   }
+}
+
+__global__ void __launch_bounds__(64) k_light(const float* __restrict__ s4, const uint32_t* __restrict__ occ,
+                                              float* __restrict__ light, int X, int Y, int Z, int XW, float I0,
+                                              float e_cell, float nchem, float floor_) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  if (x >= X) return;
+  light_column(s4, occ, light, x, blockIdx.y, X, Y, Z, XW, I0, e_cell, nchem, floor_);
+}
+
+// Ensembles: one launch over the (x, y) columns of every column of the ensemble.
+__global__ void __launch_bounds__(64) k_light_ens(const EnsColumn* __restrict__ cols, int ncols, float I0,
+                                                  float e_cell, float nchem, float floor_) {
+  int c = 0;
+  const int b = blockIdx.x;
+  while (c + 1 < ncols && cols[c + 1].light_base <= b) ++c;
+  const EnsColumn& e = cols[c];
+  const int local = b - e.light_base;
+  const int x = (local % e.lbx) * 64 + threadIdx.x;
+  if (x >= e.X) return;
+  const size_t nvox = (size_t)e.X * e.Y * e.Z;
+  light_column(e.f[e.cur] + 4 * nvox, e.occ, e.light, x, local / e.lbx, e.X, e.Y, e.Z, e.XW, I0, e_cell, nchem,
+               floor_);
+}
+
+hipError_t marl_launch_light_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total_blocks,
+                                 const marl_sim_params* sim) {
+  const float e_cell = marl_expf(-sim->cell_absorption);
+  hipLaunchKernelGGL(k_light_ens, dim3(total_blocks), dim3(64), 0, st, dcols, ncols, sim->surface_intensity, e_cell,
+                     -sim->chemical_absorption, sim->light_floor);
+  return hipGetLastError();
 }
 
 int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim) {

@@ -164,6 +164,20 @@ struct StencilParams {
 };
 
 // ---------------------------------------------------------------------------
+// One column of an ensemble as the batched field kernels see it.
+struct EnsColumn {
+  float* f[2];           // the column's field ping-pong buffers (SoA planes)
+  const uint32_t* occ;   // its occupancy bits
+  float* light;          // its light plane
+  int X, Y, Z, XW;
+  int ntx, nty, ntz, zc; // stencil tiles of the ensemble's tile shape
+  int tile_base;         // first logical stencil tile (species groups included)
+  int cur;               // buffer holding the field at the start of a tick
+  int light_base;        // first 64-lane light block
+  int lbx;               // light blocks along x
+};
+
+// ---------------------------------------------------------------------------
 // Host context.
 struct SurrogateState;
 void surrogate_free(SurrogateState* s);
@@ -239,6 +253,26 @@ int marl_check_params(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_launch_substep(marl_ctx* ctx, const StencilParams& p, int tile);
 int marl_launch_boundary(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim);
+// Ensembles: fill the tile fields of `cols` for a tile id (returns total
+// logical tiles and light blocks), and launch one batched substep / light scan.
+void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int* total_light_blocks);
+hipError_t marl_launch_substep_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total_tiles, int parity,
+                                   const StencilParams& p, int tile, int grid);
+hipError_t marl_launch_light_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total_blocks,
+                                 const marl_sim_params* sim);
+int marl_stencil_ens_grid(int tile, int total_tiles);
+
+// Device-driven tick pieces (cells.hip), shared by the single-column graph and
+// ensembles.  prepare allocates; the enqueue calls are capturable.
+int marl_graph_prepare(marl_ctx* ctx, const marl_sim_params* sim);
+int marl_enqueue_tick_begin(marl_ctx* ctx, const marl_sim_params* sim);
+int marl_enqueue_field_phase(marl_ctx* ctx, const marl_sim_params* sim);
+int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim);
+std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* sim);
+int marl_graph_stage(marl_ctx* ctx, hipStream_t st, int64_t tick);
+int marl_graph_fetch(marl_ctx* ctx, hipStream_t st);
+int marl_graph_post(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick, marl_tick_stats* st,
+                    int32_t* host_finished);
 int marl_launch_occupancy_from_cells(marl_ctx* ctx);
 int marl_ensure_cell_capacity(marl_ctx* ctx, int64_t need);
 int marl_transpose(marl_ctx* ctx, bool to_soa, float* soa);

@@ -337,3 +337,70 @@ class GpuFieldDiffuser:
 
     def close(self) -> None:
         self._gpu.close()
+
+
+class MarlEnsemble:
+    """Independent columns of uneven sizes and seeds advanced together: one
+    hipGraph per tick with every column's field phase batched into single
+    launches and the cell phases side by side.  The columns are MarlGpu
+    contexts the caller keeps (upload, download and close them as usual;
+    close the ensemble first)."""
+
+    def __init__(self, columns):
+        self.lib = _abi.load_library()
+        self.columns = list(columns)
+        arr = (ctypes.c_void_p * len(self.columns))(*[c.h.value for c in self.columns])
+        h = ctypes.c_void_p()
+        rc = self.lib.marl_ensemble_create(arr, len(self.columns), ctypes.byref(h))
+        check(rc, what="marl_ensemble_create")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.marl_ensemble_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc: int) -> None:
+        if rc != _abi.OK:
+            m = self.lib.marl_ensemble_last_error(self.h)
+            raise _abi._ERRORS.get(rc, _abi.GpuError)(m.decode(errors="replace") if m else f"status {rc}")
+
+    def tick(self, sim, seeds=None, tick0: int = 0, n_ticks: int = 1):
+        """Returns (per-column stats dicts, GPU ms of the graphs, host-finished count)."""
+        p = _params(sim)
+        n = len(self.columns)
+        st = (_abi.marl_tick_stats * n)()
+        sd = None if seeds is None else np.ascontiguousarray(seeds, np.uint64)
+        hf = ctypes.c_int32()
+        ms = ctypes.c_float()
+        self._chk(self.lib.marl_ensemble_tick(self.h, ctypes.byref(p), None if sd is None else ptr(sd), int(tick0),
+                                              int(n_ticks), st, ctypes.byref(hf), ctypes.byref(ms)))
+        stats = [{k: getattr(s, k) for k, _ in s._fields_ if not k.startswith("_")} for s in st]
+        return stats, ms.value, hf.value
+
+    def set_tile(self, tile: int) -> None:
+        self._chk(self.lib.marl_ensemble_set_tile(self.h, int(tile)))
+
+    def time_field(self, sim, n_substeps: int = 20) -> float:
+        p = _params(sim)
+        ms = ctypes.c_float()
+        self._chk(self.lib.marl_ensemble_time_field(self.h, ctypes.byref(p), int(n_substeps), ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, sim) -> int:
+        p = _params(sim)
+        t = ctypes.c_int32()
+        self._chk(self.lib.marl_ensemble_autotune(self.h, ctypes.byref(p), ctypes.byref(t)))
+        return t.value

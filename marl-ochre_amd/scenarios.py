@@ -228,3 +228,29 @@ def topology_hash(rules) -> np.ndarray:
         for k in range(w.shape[1]):
             h = (h ^ w[:, k]) * np.uint64(1099511628211)
     return h
+
+
+def ensemble_columns(n_columns=32, seed=0, sizes=(64, 96, 128, 192), cells_range=(1000, 200000), max_fill=0.15):
+    """Independent columns of uneven sizes and seeds.
+
+    Column c: x = y drawn from `sizes`, z = x / 2; its own rng seed
+    (seed * 1000 + c); a cell count log-uniform over `cells_range`, capped at
+    `max_fill` of the voxels; cells placed uniformly, each one of the three
+    starter metabolisms with that starter's own initial state; linear-gradient
+    field.  Returns [(dims, field, cells, rules, rng_seed)]."""
+    from .starters import make_cells
+
+    out = []
+    for c in range(n_columns):
+        rng = np.random.Generator(np.random.Philox(seed * 1000 + c + 17))
+        x = int(rng.choice(sizes))
+        dims = (x, x, x // 2)
+        nv = dims[0] * dims[1] * dims[2]
+        lo, hi = np.log(cells_range[0]), np.log(cells_range[1])
+        n = min(int(np.exp(rng.uniform(lo, hi))), int(max_fill * nv))
+        vox = np.sort(rng.choice(nv, n, replace=False))
+        pos = np.stack([vox % dims[0], (vox // dims[0]) % dims[1], vox // (dims[0] * dims[1])], 1)
+        kinds = rng.integers(0, 3, n)
+        cells, rules = make_cells(kinds, pos, rng.integers(0, 2**63, n, dtype=np.uint64))
+        out.append((dims, linear_gradient_field(dims), cells, rules, seed * 1000 + c))
+    return out

@@ -112,3 +112,24 @@ def make_cell(kind: int, pos, lineage_id: int) -> np.ndarray:
         c["internal"][k] = v
     c["starter_type"] = kind
     return c
+
+
+def make_cells(kinds: np.ndarray, pos: np.ndarray, lineage_ids: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Vectorised make_cell / make_ruleset: (cells, rulesets) of the given starter kinds."""
+    kinds = np.asarray(kinds)
+    n = kinds.size
+    cells = np.zeros(n, CELL_DTYPE)
+    rules = np.zeros(n, RULESET_DTYPE)
+    cells["pos"] = pos
+    cells["lineage_id"] = lineage_ids
+    cells["starter_type"] = kinds
+    for kind in (PHOTOTROPH, CHEMOLITHOTROPH, ANAEROBE):
+        sel = kinds == kind
+        if not sel.any():
+            continue
+        rules[sel] = make_ruleset(kind)
+        internal = np.zeros(16, np.float32)
+        for k, v in _START_INTERNAL[kind].items():
+            internal[k] = v
+        cells["internal"][sel] = internal
+    return cells, rules

@@ -1,0 +1,94 @@
+"""Ensembles of uneven columns (marl_ensemble_*) against each column ticked
+alone with marl_tick: fields, light and every cell field bit-identical after
+every tick, for every stencil tile shape; the autotuner leaves fields alone."""
+import numpy as np
+import pytest
+
+from test_gpu_cells import assert_cells_equal, population
+
+pytestmark = pytest.mark.gpu
+
+F = np.float32
+
+
+def _columns(pkg):
+    """Five columns of different shapes; three with busy random populations
+    (divisions and deaths from the first tick), two with starter states."""
+    from marl_ochre_amd import scenarios
+
+    A = pkg._abi
+    cols = []
+    for k, dims in enumerate([(64, 32, 24), (32, 32, 16), (96, 64, 40)]):
+        cells, rules = population(A, dims, int(0.08 * np.prod(dims)), 40 + k)
+        field = np.random.default_rng(50 + k).random(np.prod(dims) * 12).astype(F)
+        cols.append((dims, field, cells, rules, 1000 + k))
+    for dims, field, cells, rules, seed in scenarios.ensemble_columns(2, seed=3, sizes=(64, 96),
+                                                                       cells_range=(500, 5000)):
+        cols.append((dims, field, cells, rules, seed))
+    return cols
+
+
+def _run(pkg, cols, sim, n_ticks, tile=None):
+    seeds = np.array([c[4] for c in cols], np.uint64)
+    alone = [pkg.MarlGpu(c[0]) for c in cols]
+    batched = [pkg.MarlGpu(c[0]) for c in cols]
+    try:
+        for g, h, (dims, field, cells, rules, _) in zip(alone, batched, cols):
+            for x in (g, h):
+                x.upload_field_aos(field)
+                x.upload_cells(cells, rules)
+        events = 0
+        with pkg.MarlEnsemble(batched) as ens:
+            if tile is not None:
+                ens.set_tile(tile)
+            for t in range(n_ticks):
+                stats, ms, hf = ens.tick(sim, seeds, t, 1)
+                assert ms > 0 and hf == 0
+                for c, (g, h) in enumerate(zip(alone, batched)):
+                    se = g.tick(sim.replace(rng_seed=int(seeds[c])), t, 1)
+                    for k in ("divisions", "deaths", "births_added", "n_cells"):
+                        assert stats[c][k] == se[k], (t, c, k, stats[c][k], se[k])
+                    events += se["divisions"] + se["deaths"]
+                    ca, ra = g.download_cells()
+                    cb, rb = h.download_cells()
+                    assert_cells_equal(cb, rb, ca, ra)
+                    assert np.array_equal(h.download_field_aos().view(np.uint32),
+                                          g.download_field_aos().view(np.uint32)), (t, c)
+                    assert np.array_equal(h.download_light().view(np.uint32), g.download_light().view(np.uint32)), (t, c)
+        return events
+    finally:
+        for x in alone + batched:
+            x.close()
+
+
+def test_ensemble_bit_identical_to_single_columns(pkg):
+    cols = _columns(pkg)
+    events = _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=4), 4)
+    assert events > 50
+
+
+@pytest.mark.parametrize("tile", [0, 1, 4, 6, 7])
+def test_ensemble_every_tile_shape(pkg, tile):
+    cols = _columns(pkg)[:3]
+    _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=2), 2, tile=tile)
+
+
+def test_ensemble_autotune_keeps_fields(pkg):
+    cols = _columns(pkg)
+    gs = [pkg.MarlGpu(c[0]) for c in cols]
+    try:
+        for g, (dims, field, cells, rules, _) in zip(gs, cols):
+            g.upload_field_aos(field)
+            g.upload_cells(cells, rules)
+        before = [g.download_field_aos() for g in gs]
+        with pkg.MarlEnsemble(gs) as ens:
+            tile = ens.autotune(pkg.SimulationConfig())
+            assert 0 <= tile < pkg.load_library().marl_stencil_tile_count()
+            assert ens.time_field(pkg.SimulationConfig(), 4) > 0
+            with pytest.raises(pkg.InvalidInput):
+                ens.tick(pkg.SimulationConfig(diffusion_substeps=3))
+        for g, b in zip(gs, before):
+            assert np.array_equal(g.download_field_aos(), b)
+    finally:
+        for g in gs:
+            g.close()

@@ -88,6 +88,20 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
         wall = time.perf_counter() - t0
         if rep > 0:  # the first is a warm-up
             runs.append((wall, st))
+    graph_runs = []
+    for rep in range(reps + 1):
+        g.upload_field_aos(field)
+        g.upload_cells(cells, rules)
+        g.sync()
+        t0 = time.perf_counter()
+        st = g.tick_graph(sim, 0, 1)
+        wall = time.perf_counter() - t0
+        if rep > 0:  # the first captures the graph
+            graph_runs.append((wall, st))
+    for (_, a), (_, b) in zip(runs, graph_runs):
+        assert all(a[k] == b[k] for k in ("divisions", "deaths", "births_added", "n_cells")), (a, b)
+    g.upload_field_aos(field)
+    g.upload_cells(cells, rules)
     traj = []
     t0 = time.perf_counter()
     for t in range(ticks):
@@ -105,6 +119,8 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
         "Mcell_updates_per_s_kinetics": round(n0 / (kin_ms * 1e-3) / 1e6, 3),
         "Mcell_updates_per_s_full_tick": round(n0 / (tick_ms * 1e-3) / 1e6, 3),
         "ms_full_tick": round(tick_ms, 3),
+        "ms_full_tick_graph": round(sum(w for w, _ in graph_runs) * 1e3 / reps, 3),
+        "ms_full_tick_graph_gpu": round(sum(st["ms_field"] for _, st in graph_runs) / reps, 3),
         "ms_breakdown": {k: round(sum(st[k] for _, st in runs) / reps, 3)
                          for k in ("ms_field", "ms_schedule", "ms_kinetics", "ms_fate")},
         "levels": runs[-1][1]["levels"],
@@ -149,9 +165,10 @@ def drift_report(pkg, n_ticks, nthreads):
 
     orc = Oracle()
     dims = (128, 128, 64)
-    field, cells, rules, sim = scenarios.winogradsky_column(dims)
+    field, cells, rules, sim = scenarios.winogradsky_column(dims, starter_state=True)
     checks = sorted({c for c in (1, 10, 100, 1000, n_ticks) if c <= n_ticks})
-    out = {"workload": "default column 128x128x64, %d cells, 10 substeps/tick" % len(cells), "ticks": n_ticks}
+    out = {"workload": "default column 128x128x64 with the starters' own states, %d cells at start, 10 substeps/tick, "
+                       "GPU ticks through the captured graph" % len(cells), "ticks": n_ticks}
     # (1) full ticks, FP32 GPU vs FP32 oracle
     g = pkg.MarlGpu(dims)
     g.upload_field_aos(field)
@@ -160,7 +177,7 @@ def drift_report(pkg, n_ticks, nthreads):
     rows = []
     t0 = time.perf_counter()
     for t in range(n_ticks):
-        g.tick(sim, t, 1)
+        g.tick_graph(sim, t, 1)
         f_o, l_o, c_o, r_o, _, _ = orc.tick(dims, sim, t, f_o, c_o, r_o, nthreads=nthreads)
         if t + 1 in checks:
             f_g = g.download_field_aos()
@@ -214,6 +231,87 @@ def drift_report(pkg, n_ticks, nthreads):
     g.close()
     out["fp32_field_vs_fp64_oracle"] = rows
     return out
+
+
+def ensemble_bench(pkg, ticks=5):
+    """32 independent columns, x = y in {64, 96, 128, 192}, z = x/2, own seeds,
+    starter populations of log-uniform size in [1k, 200k] (capped at 15 % of
+    the voxels).  Field phase: one batched substep over all columns against
+    the sum of per-column substeps.  Whole ticks: the ensemble's hipGraph
+    against ticking the columns one after another (host-driven and graphed)."""
+    from marl_ochre_amd import scenarios
+
+    log("generating 32-column ensemble")
+    cols = scenarios.ensemble_columns(32, seed=0)
+    sim = pkg.SimulationConfig()
+    seeds = np.array([c[4] for c in cols], np.uint64)
+    nvox = sum(d[0] * d[1] * d[2] for d, *_ in cols)
+    ncells = sum(len(c[2]) for c in cols)
+
+    def fresh():
+        gs = [pkg.MarlGpu(d) for d, *_ in cols]
+        for g, (dims, field, cells, rules, _) in zip(gs, cols):
+            g.upload_field_aos(field)
+            g.upload_cells(cells, rules)
+        return gs
+
+    gs = fresh()
+    res = {"workload": "32 columns x=y in {64,96,128,192}, z=x/2, own seeds, starter populations log-U[1k,200k] "
+                       "(<=15% of voxels), 10 substeps/tick",
+           "columns": len(cols), "voxels": nvox, "cells": ncells}
+    try:
+        with pkg.MarlEnsemble(gs) as ens:
+            tile = ens.autotune(sim)
+            ms_sub = ens.time_field(sim, 20)
+            ens.tick(sim, seeds, 0, 1)  # capture
+            for g, (dims, field, cells, rules, _) in zip(gs, cols):
+                g.upload_field_aos(field)
+                g.upload_cells(cells, rules)
+            t0 = time.perf_counter()
+            _, gpu_ms, hf = ens.tick(sim, seeds, 0, ticks)
+            ens_wall = (time.perf_counter() - t0) * 1e3 / ticks
+        per_col_sub = 0.0
+        for g in gs:
+            g.tile = tile
+            per_col_sub += g.time_diffuse(sim, 20) / 20
+    finally:
+        for g in gs:
+            g.close()
+    seq = {}
+    for mode in ("eager", "graph"):
+        gs = fresh()
+        try:
+            if mode == "graph":
+                for g, c in zip(gs, cols):
+                    g.tick_graph(sim.replace(rng_seed=c[4]), 0, 1)  # capture
+                for g, (dims, field, cells, rules, _) in zip(gs, cols):
+                    g.upload_field_aos(field)
+                    g.upload_cells(cells, rules)
+            t0 = time.perf_counter()
+            for t in range(ticks):
+                for g, c in zip(gs, cols):
+                    s2 = sim.replace(rng_seed=c[4])
+                    (g.tick_graph if mode == "graph" else g.tick)(s2, t, 1)
+            seq[mode] = (time.perf_counter() - t0) * 1e3 / ticks
+        finally:
+            for g in gs:
+                g.close()
+    alg = 96.125 * nvox
+    res.update({
+        "tile": tile,
+        "batched_substep_us": round(ms_sub * 1e3, 1),
+        "per_column_substeps_us": round(per_col_sub * 1e3, 1),
+        "batched_substep_Gvoxel_species_per_s": round(nvox * 12 / (ms_sub * 1e-3) / 1e9, 1),
+        "batched_substep_hbm_frac": round(alg / (ms_sub * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "ms_per_tick_ensemble_graph": round(ens_wall, 3),
+        "ms_per_tick_ensemble_graph_gpu": round(gpu_ms / ticks, 3),
+        "ms_per_tick_columns_one_by_one_eager": round(seq["eager"], 3),
+        "ms_per_tick_columns_one_by_one_graph": round(seq["graph"], 3),
+        "speedup_ensemble_vs_eager": round(seq["eager"] / ens_wall, 2),
+        "host_finished_column_ticks": hf,
+    })
+    log("ensemble:", json.dumps(res))
+    return res
 
 
 SURR_FLOP_PER_CELL = 2.0 * (80 * 256 + 256 * 256 * 2 + 256 * 28)  # forward multiply-adds x 2
@@ -317,6 +415,8 @@ def main():
     ap.add_argument("--tile", type=int, default=-1, help="use this stencil tile id instead of autotuning")
     ap.add_argument("--kinetics-ticks", type=int, default=10,
                     help="full ticks of the 1M-cell kinetics workload (0 = skip)")
+    ap.add_argument("--ensemble-ticks", type=int, default=5,
+                    help="ticks of the 32-column ensemble leg (0 = skip)")
     ap.add_argument("--drift-ticks", type=int, default=0,
                     help="long-run accuracy report over this many ticks of the default column (0 = skip)")
     ap.add_argument("--surrogate-steps", type=int, default=3000,
@@ -426,6 +526,8 @@ def main():
     g.close()
     if args.kinetics_ticks > 0:
         out["kinetics"] = kinetics_bench(pkg, args.kinetics_ticks, not args.no_cpu)
+    if args.ensemble_ticks > 0:
+        out["ensemble"] = ensemble_bench(pkg, args.ensemble_ticks)
     if args.drift_ticks > 0:
         out["drift"] = drift_report(pkg, args.drift_ticks, min(os.cpu_count() or 1, 16))
     if args.surrogate_steps > 0:

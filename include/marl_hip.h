@@ -234,6 +234,9 @@ int marl_autotune_stencil(marl_ctx* ctx, const marl_sim_params* sim, int32_t* ti
 int marl_set_stencil_tile(marl_ctx* ctx, int32_t tile);
 int marl_get_stencil_tile(marl_ctx* ctx, int32_t* tile);
 int marl_stencil_tile_count(void);
+/* Substeps one launch of a tile id performs: 1, or 2 for the temporally
+ * blocked tiles (which keep the intermediate substep on chip). */
+int marl_stencil_tile_substeps(int32_t tile);
 /* Rows per thread, z-chunk and prefetch distance (planes) of a tile id. */
 int marl_stencil_tile_shape(int32_t tile, int32_t* rows_per_thread, int32_t* z_chunk, int32_t* prefetch);
 /* Plain copy of the resident field into the scratch buffer (n_reps times,

@@ -1160,10 +1160,7 @@ int marl_enqueue_tick_begin(marl_ctx* ctx, const marl_sim_params* sim) {
 // Substeps with the boundary sources fused into the first, then light.
 int marl_enqueue_field_phase(marl_ctx* ctx, const marl_sim_params* sim) {
   if (sim->diffusion_substeps > 0) {
-    for (int k = 0; k < sim->diffusion_substeps; ++k) {
-      const StencilParams sp = marl_stencil_params(sim, k == 0 ? 1 : 0);
-      if (int rc = marl_launch_substep(ctx, sp, ctx->tile)) return rc;
-    }
+    if (int rc = marl_launch_substeps(ctx, sim, sim->diffusion_substeps, true, ctx->tile)) return rc;
   } else {
     if (int rc = marl_launch_boundary(ctx, sim)) return rc;
   }
@@ -1262,10 +1259,8 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
 // capture).
 int marl_graph_prepare(marl_ctx* ctx, const marl_sim_params* sim) {
   if (int rc = marl_check_params(ctx, sim)) return rc;
-  if (sim->diffusion_substeps % 2 != 0 || ctx->kin_mode != MARL_KINETICS_PLAIN ||
-      (ctx->surr && ctx->surr->harvesting))
-    return marl_fail(ctx, MARL_ERR_INVALID_INPUT,
-                     "the captured tick needs an even substep count, plain kinetics and no harvest");
+  if (ctx->kin_mode != MARL_KINETICS_PLAIN || (ctx->surr && ctx->surr->harvesting))
+    return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "the captured tick needs plain kinetics and no harvest");
   ctx->last_sim = *sim;
   if (int rc = ensure_voxel_buffers(ctx)) return rc;
   if (int rc = marl_ensure_staging(ctx)) return rc;
@@ -1326,35 +1321,38 @@ int marl_graph_post(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick, mar
 
 namespace {
 
-int drop_graph(marl_ctx* ctx) {
-  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
-  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-  ctx->gexec = nullptr;
-  ctx->graph = nullptr;
-  ctx->graph_key.clear();
+int drop_graph(marl_ctx* ctx, int slot) {
+  if (ctx->gexec[slot]) (void)hipGraphExecDestroy(ctx->gexec[slot]);
+  if (ctx->graph[slot]) (void)hipGraphDestroy(ctx->graph[slot]);
+  ctx->gexec[slot] = nullptr;
+  ctx->graph[slot] = nullptr;
+  ctx->graph_key[slot].clear();
   return MARL_OK;
 }
 
-// Capture one device-driven tick; the host-side field parity is restored
-// afterwards because capture records work without running it.
+// Capture one device-driven tick for the current field parity.  Capture
+// records work without running it, so the host-side parity is restored and
+// the number of buffer swaps the graph performs is remembered.
 int capture_tick(marl_ctx* ctx, const marl_sim_params* sim) {
-  drop_graph(ctx);
-  const int cur0 = ctx->cur;
+  const int slot = ctx->cur;
+  drop_graph(ctx, slot);
   MARL_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   int rc = marl_enqueue_tick_begin(ctx, sim);
   if (rc == MARL_OK) rc = marl_enqueue_field_phase(ctx, sim);
+  const int cur_after = ctx->cur;
   if (rc == MARL_OK) rc = marl_enqueue_cell_phase(ctx, sim);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-  ctx->cur = cur0;
+  ctx->cur = slot;
   if (rc != MARL_OK) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   MARL_HIP(ctx, e);
-  ctx->graph = g;
-  MARL_HIP(ctx, hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
-  ctx->graph_key = marl_graph_key(ctx, sim);
+  ctx->graph[slot] = g;
+  MARL_HIP(ctx, hipGraphInstantiate(&ctx->gexec[slot], g, nullptr, nullptr, 0));
+  ctx->graph_key[slot] = marl_graph_key(ctx, sim);
+  ctx->graph_flip[slot] = cur_after != slot;
   return MARL_OK;
 }
 
@@ -1474,10 +1472,7 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
     // first substep (field.rs:231-250 then :212-217); light (lib.rs:199)
     if (int rc = marl_launch_occupancy_from_cells(ctx)) return rc;
     if (sim->diffusion_substeps > 0) {
-      for (int k = 0; k < sim->diffusion_substeps; ++k) {
-        const StencilParams sp = marl_stencil_params(sim, k == 0 ? 1 : 0);
-        if (int rc = marl_launch_substep(ctx, sp, ctx->tile)) return rc;
-      }
+      if (int rc = marl_launch_substeps(ctx, sim, sim->diffusion_substeps, true, ctx->tile)) return rc;
     } else {
       if (int rc = marl_launch_boundary(ctx, sim)) return rc;
     }
@@ -1518,14 +1513,16 @@ int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, in
   int32_t finished = 0;
   for (int t = 0; t < n_ticks; ++t) {
     if (int rc = marl_graph_prepare(ctx, sim)) return rc;
-    if (ctx->graph_key != marl_graph_key(ctx, sim))
+    const int slot = ctx->cur;
+    if (ctx->graph_key[slot] != marl_graph_key(ctx, sim))
       if (int rc = capture_tick(ctx, sim)) return rc;
     if (int rc = marl_graph_stage(ctx, ctx->stream, tick0 + t)) return rc;
     MARL_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    MARL_HIP(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
+    MARL_HIP(ctx, hipGraphLaunch(ctx->gexec[slot], ctx->stream));
     MARL_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     if (int rc = marl_graph_fetch(ctx, ctx->stream)) return rc;
     MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->cur ^= ctx->graph_flip[slot];
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     if (int rc = marl_graph_post(ctx, sim, tick0 + t, &st, &finished)) return rc;

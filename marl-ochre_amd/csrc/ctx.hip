@@ -103,8 +103,10 @@ void marl_destroy(marl_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   surrogate_free(c->surr);
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (int k = 0; k < 2; ++k) {
+    if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+    if (c->graph[k]) (void)hipGraphDestroy(c->graph[k]);
+  }
   if (c->tdev) (void)hipFree(c->tdev);
   void* bufs[] = {c->field[0], c->field[1], c->occ, c->light, c->staging, c->occ_bytes, c->cells[0], c->cells[1],
                   c->rules[0], c->rules[1], c->cell_at, c->nbmask, c->level, c->order, c->events, c->flag_scan,
@@ -229,9 +231,7 @@ int marl_diffuse_substeps(marl_ctx* ctx, const marl_sim_params* sim, int32_t n) 
   if (!ctx) return MARL_ERR_INVALID_INPUT;
   if (int rc = marl_check_params(ctx, sim)) return rc;
   if (sim->diffusion_substeps == 0 || n <= 0) return MARL_OK;
-  const StencilParams p = marl_stencil_params(sim, 0);
-  for (int k = 0; k < n; ++k)
-    if (int rc = marl_launch_substep(ctx, p, ctx->tile)) return rc;
+  if (int rc = marl_launch_substeps(ctx, sim, n, false, ctx->tile)) return rc;
   MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MARL_OK;
 }
@@ -250,9 +250,7 @@ int marl_diffuse_tick(marl_ctx* ctx, float* aos, size_t n_floats, const uint8_t*
   MARL_HIP(ctx, hipMemcpyAsync(ctx->occ_bytes, occ, n_voxels, hipMemcpyHostToDevice, ctx->stream));
   if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
   if (int rc = marl_pack_occ(ctx)) return rc;
-  const StencilParams p = marl_stencil_params(sim, 0);
-  for (int k = 0; k < sim->diffusion_substeps; ++k)
-    if (int rc = marl_launch_substep(ctx, p, ctx->tile)) return rc;
+  if (int rc = marl_launch_substeps(ctx, sim, sim->diffusion_substeps, false, ctx->tile)) return rc;
   if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
   return d2h_checked(ctx, aos, ctx->staging, n_floats * sizeof(float));         // :270-292
 }
@@ -269,10 +267,8 @@ int marl_time_diffuse(marl_ctx* ctx, const marl_sim_params* sim, int32_t n, floa
   if (!ctx || !ms) return MARL_ERR_INVALID_INPUT;
   if (int rc = marl_check_params(ctx, sim)) return rc;
   if (sim->diffusion_substeps == 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "diffusion_substeps == 0");
-  const StencilParams p = marl_stencil_params(sim, 0);
   MARL_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  for (int k = 0; k < n; ++k)
-    if (int rc = marl_launch_substep(ctx, p, ctx->tile)) return rc;
+  if (int rc = marl_launch_substeps(ctx, sim, n, false, ctx->tile)) return rc;
   MARL_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   MARL_HIP(ctx, hipEventSynchronize(ctx->ev1));
   MARL_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
@@ -305,6 +301,11 @@ int marl_copy_probe(marl_ctx* ctx, int32_t n, float* ms, double* bytes) {
 
 int marl_stencil_tile_count(void) { return kNumTiles; }
 
+int marl_stencil_tile_substeps(int32_t tile) {
+  if (tile < 0 || tile >= kNumTiles) return -MARL_ERR_INVALID_INPUT;
+  return kTiles[tile].tb;
+}
+
 int marl_stencil_tile_shape(int32_t tile, int32_t* ry, int32_t* zc, int32_t* pd) {
   if (tile < 0 || tile >= kNumTiles) return MARL_ERR_INVALID_INPUT;
   if (ry) *ry = kTiles[tile].ry;
@@ -336,15 +337,12 @@ int marl_autotune_stencil(marl_ctx* ctx, const marl_sim_params* sim, int32_t* ti
   const int cur0 = ctx->cur;
   MARL_HIP(ctx, hipMemcpyAsync(ctx->staging, ctx->field[cur0], ctx->nfield * sizeof(float),
                                hipMemcpyDeviceToDevice, ctx->stream));
-  const StencilParams p = marl_stencil_params(sim, 0);
   int best = ctx->tile;
   float best_ms = 1e30f;
   for (int t = 0; t < kNumTiles; ++t) {
-    for (int k = 0; k < 2; ++k)
-      if (int rc = marl_launch_substep(ctx, p, t)) return rc;
+    if (int rc = marl_launch_substeps(ctx, sim, 2, false, t)) return rc;
     MARL_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    for (int k = 0; k < 8; ++k)
-      if (int rc = marl_launch_substep(ctx, p, t)) return rc;
+    if (int rc = marl_launch_substeps(ctx, sim, 8, false, t)) return rc;
     MARL_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     MARL_HIP(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0.f;

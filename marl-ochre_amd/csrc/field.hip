@@ -25,10 +25,13 @@
 // The per-voxel arithmetic is the reference's, in its order and without
 // contraction, so the result is bit-identical to the f32 CPU statement.
 
-// {rows per thread, z-chunk, prefetch distance in planes, species per block}
+// {rows per thread, z-chunk, prefetch distance in planes, species per block,
+//  substeps per pass, waves per block}
 const TileShape kTiles[] = {
-    {1, 32, 1, 12}, {2, 16, 1, 12}, {2, 32, 1, 12}, {2, 16, 1, 6}, {2, 32, 1, 6}, {2, 64, 1, 6},
-    {2, 32, 2, 6},  {4, 32, 1, 6},  {4, 64, 1, 6},
+    {1, 32, 1, 12, 1, 4}, {2, 16, 1, 12, 1, 4}, {2, 32, 1, 12, 1, 4}, {2, 16, 1, 6, 1, 4}, {2, 32, 1, 6, 1, 4},
+    {2, 64, 1, 6, 1, 4},  {2, 32, 2, 6, 1, 4},  {4, 32, 1, 6, 1, 4},  {4, 64, 1, 6, 1, 4},
+    // temporal blocking (K1b): two substeps per launch
+    {1, 16, 1, 12, 2, 8}, {1, 16, 1, 6, 2, 8},
 };
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
@@ -368,6 +371,325 @@ __global__ void __launch_bounds__(256) k_stencil_ens(const EnsColumn* __restrict
   }
 }
 
+// ---------------------------------------------------------------------------
+// K1b: two substeps per pass (temporal blocking).  A block owns an output
+// tile of 62 (x) x (TY - 2) (y) voxels and computes the first substep on the
+// 64 x TY tile around it (one voxel of overlap on every side).  The
+// intermediate field never reaches HBM: each thread keeps its own column's
+// intermediate planes z-2, z-1, z in registers and the block keeps plane z-1
+// in LDS for the x/y neighbours of the second substep, which runs one plane
+// behind the first.  Per voxel both substeps are the single-substep kernel's
+// arithmetic in the same order, so the result is bit-identical to two
+// launches of K1; HBM traffic per substep is halved (one read and one write
+// of the field per two substeps, plus the overlap).
+template <int TY>
+struct Stencil2Lds {
+  float in[MARL_S_EXT][TY + 3][66];   // input plane z with halo (+ spare row)
+  float mid[MARL_S_EXT][TY][64];      // intermediate plane z-1
+  uint32_t m[2][(TY + 2) * 4 + 64];   // mask words of planes z and z-1 (by parity) + spare slots
+};
+
+// NW waves per species group, RY rows per thread (the tile is TY = NW * RY
+// rows tall), SG species per thread: 12 / SG groups of NW waves share the
+// tile, each handling SG species; niche factors read species 7 from LDS.
+template <int RY, bool SRC, int PD, int NW, int SG>
+__global__ void __launch_bounds__(64 * NW * (MARL_S_EXT / SG))
+    k_stencil2(const float* __restrict__ src, float* __restrict__ dst, const uint32_t* __restrict__ occ, int X,
+               int Y, int Z, int XW, int zc, int ntx, int nty, int ntz, StencilParams p) {
+  constexpr int S = SG;
+  constexpr int TY = NW * RY;
+  constexpr int NT = 64 * NW * (MARL_S_EXT / SG);
+  constexpr int LW = 66;
+  constexpr int QN = PD + 2;  // input planes held: z-1, z, z+1 .. z+PD
+  constexpr int HN = PD + 1;  // halo planes held: z .. z+PD
+  static_assert(TY <= 64 && (TY + 2) * 4 <= NT, "tile rows");
+  __shared__ Stencil2Lds<TY> sh;
+  auto& lds = sh.in;
+  auto& mid = sh.mid;
+
+  int b = xcd_remap(blockIdx.x, ntx * nty * ntz);
+  const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
+  const int tid = threadIdx.x;
+  const int grp = tid / (64 * NW), gt = tid % (64 * NW);
+  const int s0 = grp * SG;
+  const int lx = gt & 63, wy = gt >> 6;
+  const int x0 = tx * 62 - 1, y0 = ty * (TY - 2) - 1;  // origin of the first-substep tile
+  const int z0 = tz * zc, z1 = min(z0 + zc, Z);          // output planes [z0, z1)
+  const int zs = max(z0 - 1, 0), ze = min(z1, Z - 1);    // intermediate planes [zs, ze]
+  const int x = x0 + lx;
+  const bool xin = x >= 0 && x < X;
+  const uint32_t XY = (uint32_t)X * (uint32_t)Y;
+  const uint32_t XYb = XY * 4u, Pb = XYb * (uint32_t)Z;
+  const uint32_t XWb = (uint32_t)XW * 4u, MZb = XWb * (uint32_t)Y;
+  const rsrc_t rs = make_rsrc(src, Pb * MARL_S_EXT);
+  const rsrc_t rd = make_rsrc(dst, Pb * MARL_S_EXT);
+  const rsrc_t rm = make_rsrc(occ, MZb * (uint32_t)Z);
+
+  bool valid[RY], outv[RY];
+  int yr[RY];
+  uint32_t vo[RY], mo[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int row = wy + NW * r;
+    yr[r] = y0 + row;
+    valid[r] = xin && yr[r] >= 0 && yr[r] < Y;
+    outv[r] = valid[r] && lx >= 1 && lx <= 62 && row >= 1 && row <= TY - 2;
+    vo[r] = valid[r] ? ((uint32_t)yr[r] * (uint32_t)X + (uint32_t)x) * 4u : 0u;
+    mo[r] = valid[r] ? (uint32_t)yr[r] * XWb + (uint32_t)(x >> 5) * 4u : 0u;
+  }
+  // halo slot of this lane (as in K1, around the first-substep tile)
+  int hrow, hcol;
+  uint32_t hvo;
+  bool hw;
+  if (wy < 2) {
+    const int yy = (wy == 0) ? y0 - 1 : y0 + TY;
+    hw = yy >= 0 && yy < Y && xin;
+    hrow = (wy == 0) ? 0 : TY + 1;
+    hcol = lx + 1;
+    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)x) * 4u : 0u;
+  } else if (wy >= 4) {
+    hw = false;
+    hrow = TY + 2;
+    hcol = lx;
+    hvo = 0u;
+  } else {
+    const int xx = (wy == 2) ? x0 - 1 : x0 + 64;
+    const int yy = y0 + (lx < TY ? lx : 0);
+    hw = lx < TY && xx >= 0 && xx < X && yy >= 0 && yy < Y;
+    hrow = (lx < TY ? lx : 0) + 1;
+    hcol = (wy == 2) ? 0 : LW - 1;
+    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)xx) * 4u : 0u;
+  }
+  if (!hw) {
+    hrow = TY + 2;
+    hcol = lx;
+  }
+  // mask words of a plane: rows y0-1 .. y0+TY, word columns wx0-1 .. wx0+2
+  const int wx0 = x0 >> 5;  // floor
+  const int mrow = gt >> 2, mwc = gt & 3;
+  const int myy = y0 - 1 + mrow, mwx = wx0 - 1 + mwc;
+  const bool mw = grp == 0 && gt < (TY + 2) * 4;
+  const bool mvalid = mw && myy >= 0 && myy < Y && mwx >= 0 && mwx < XW;
+  const uint32_t mvo = mvalid ? (uint32_t)myy * XWb + (uint32_t)mwx * 4u : 0u;
+  const int mslot = mw ? gt : (TY + 2) * 4 + (tid & 63);  // lanes that write no mask word share spare slots
+  // occupancy bit of absolute column xa in mask row rw of buffer mb
+  auto bitat = [&](int mb, int rw, int xa) -> uint32_t {
+    return (sh.m[mb][rw * 4 + ((xa >> 5) - (wx0 - 1))] >> (xa & 31)) & 1u;
+  };
+
+  float q[QN][RY][S];
+  uint32_t qm[QN][RY];
+  float h[HN][S];
+  uint32_t hm[HN];
+  float p1[2][RY][S];   // intermediate planes z-2, z-1 of this thread's column
+  uint32_t qmz2[RY];    // mask word of plane z-2 of this thread's column
+
+  auto load_own = [&](int k, int zz) {
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) q[k][r][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, vo[r], zz);
+      qm[k][r] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mo[r], (int)((uint32_t)zz * MZb), 0);
+    }
+  };
+  auto load_halo = [&](int k, int zz) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) h[k][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, hvo, zz);
+    hm[k] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zz * MZb), 0);
+  };
+#pragma unroll
+  for (int k = 0; k < QN; ++k) {
+    const int zz = zs - 1 + k;
+    load_own(k, zz < 0 ? 0 : (zz < Z ? zz : Z - 1));
+  }
+#pragma unroll
+  for (int k = 0; k < HN; ++k) {
+    const int zz = zs + k;
+    load_halo(k, zz < Z ? zz : Z - 1);
+  }
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    qmz2[r] = 0u;
+#pragma unroll
+    for (int s = 0; s < S; ++s) p1[0][r][s] = p1[1][r][s] = 0.0f;
+  }
+
+  for (int z = zs; z <= z1; ++z) {
+    const bool first = z <= ze;  // this iteration computes intermediate plane z
+    // prefetch input plane z+PD+1: own rows up to ze+1, halo up to ze
+    const int zf = z + PD + 1;
+    float qn[RY][S];
+    uint32_t qmn[RY];
+    float hn[S];
+    uint32_t hmn = 0u;
+    if (zf <= ze + 1 && zf < Z) {
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) qn[r][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, vo[r], zf);
+        qmn[r] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mo[r], (int)((uint32_t)zf * MZb), 0);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        qmn[r] = 0u;
+#pragma unroll
+        for (int s = 0; s < S; ++s) qn[r][s] = 0.0f;
+      }
+    }
+    if (zf <= ze) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) hn[s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, hvo, zf);
+      hmn = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zf * MZb), 0);
+    } else {
+#pragma unroll
+      for (int s = 0; s < S; ++s) hn[s] = 0.0f;
+    }
+    if (first) {
+#pragma unroll
+      for (int r = 0; r < RY; ++r)
+#pragma unroll
+        for (int s = 0; s < S; ++s) lds[s0 + s][wy + NW * r + 1][lx + 1] = q[1][r][s];
+#pragma unroll
+      for (int s = 0; s < S; ++s) lds[s0 + s][hrow][hcol] = h[0][s];
+      sh.m[z & 1][mslot] = mvalid ? hm[0] : 0u;
+    }
+    __syncthreads();
+
+    // first substep on plane z (registers only; mid still holds plane z-1)
+    float p1n[RY][S];
+    if (first) {
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const int row = wy + NW * r + 1;
+        const int y = yr[r];
+        const uint32_t bc = bitat(z & 1, row, x), bxm = bitat(z & 1, row, x - 1), bxp = bitat(z & 1, row, x + 1);
+        const uint32_t bym = bitat(z & 1, row - 1, x), byp = bitat(z & 1, row + 1, x);
+        const uint32_t bzm = (qm[0][r] >> (x & 31)) & 1u, bzp = (qm[2][r] >> (x & 31)) & 1u;
+        const bool vxm = (uint32_t)(x > 0) & (bxm ^ 1u);
+        const bool vxp = (uint32_t)(x < X - 1) & (bxp ^ 1u);
+        const bool vym = (uint32_t)(y > 0) & (bym ^ 1u);
+        const bool vyp = (uint32_t)(y < Y - 1) & (byp ^ 1u);
+        const bool vzm = (uint32_t)(z > 0) & (bzm ^ 1u);
+        const bool vzp = (uint32_t)(z < Z - 1) & (bzp ^ 1u);
+        const float structural = lds[7][row][lx + 1];
+        const float niche = 1.0f - p.alpha_eps * structural / (p.k_eps + structural);
+        const uint32_t keep = 0u - bc;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const float c = q[1][r][s];
+          const float dd = p.d_voxel[s0 + s] * niche;
+          const float xm = vxm ? lds[s0 + s][row][lx] : c;
+          const float xp = vxp ? lds[s0 + s][row][lx + 2] : c;
+          const float ym = vym ? lds[s0 + s][row - 1][lx + 1] : c;
+          const float yp = vyp ? lds[s0 + s][row + 1][lx + 1] : c;
+          const float zm = vzm ? q[0][r][s] : c;
+          const float zp = vzp ? q[2][r][s] : c;
+          const float lap = xm + xp + ym + yp + zm + zp - 6.0f * c;
+          const float decay = p.lambda_decay[s0 + s] * c;
+          const float nc = c + p.dt_sub * (dd * lap - decay);
+          p1n[r][s] = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, c) & keep) |
+                                                    (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RY; ++r)
+#pragma unroll
+        for (int s = 0; s < S; ++s) p1n[r][s] = 0.0f;
+    }
+
+    // second substep on plane zo = z-1 from the intermediate planes
+    const int zo = z - 1;
+    if (zo >= z0) {
+      const int mb = zo & 1;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const int mrw = wy + NW * r;         // row in mid
+        const int row = mrw + 1;             // row in the mask LDS
+        const int y = yr[r];
+        const uint32_t bc = bitat(mb, row, x), bxm = bitat(mb, row, x - 1), bxp = bitat(mb, row, x + 1);
+        const uint32_t bym = bitat(mb, row - 1, x), byp = bitat(mb, row + 1, x);
+        const uint32_t bzm = (qmz2[r] >> (x & 31)) & 1u, bzp = (qm[1][r] >> (x & 31)) & 1u;
+        const bool vxm = (uint32_t)(x > 0) & (bxm ^ 1u);
+        const bool vxp = (uint32_t)(x < X - 1) & (bxp ^ 1u);
+        const bool vym = (uint32_t)(y > 0) & (bym ^ 1u);
+        const bool vyp = (uint32_t)(y < Y - 1) & (byp ^ 1u);
+        const bool vzm = (uint32_t)(zo > 0) & (bzm ^ 1u);
+        const bool vzp = (uint32_t)(zo < Z - 1) & (bzp ^ 1u);
+        // edge lanes and rows (no output) read clamped slots
+        const int cxm = lx > 0 ? lx - 1 : 0, cxp = lx < 63 ? lx + 1 : 63;
+        const int rym = mrw > 0 ? mrw - 1 : 0, ryp = mrw < TY - 1 ? mrw + 1 : TY - 1;
+        const float structural = mid[7][mrw][lx];
+        const float niche = 1.0f - p.alpha_eps * structural / (p.k_eps + structural);
+        const uint32_t keep = 0u - bc;
+        float out[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const float c = p1[1][r][s];
+          const float dd = p.d_voxel[s0 + s] * niche;
+          const float xm = vxm ? mid[s0 + s][mrw][cxm] : c;
+          const float xp = vxp ? mid[s0 + s][mrw][cxp] : c;
+          const float ym = vym ? mid[s0 + s][rym][lx] : c;
+          const float yp = vyp ? mid[s0 + s][ryp][lx] : c;
+          const float zm = vzm ? p1[0][r][s] : c;
+          const float zp = vzp ? p1n[r][s] : c;
+          const float lap = xm + xp + ym + yp + zm + zp - 6.0f * c;
+          const float decay = p.lambda_decay[s0 + s] * c;
+          const float nc = c + p.dt_sub * (dd * lap - decay);
+          out[s] = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, c) & keep) |
+                                                 (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
+        }
+        if (outv[r]) {
+          const uint32_t zo_b = (uint32_t)zo * XYb;
+#pragma unroll
+          for (int s = 0; s < S; ++s) bstore_nt(rd, out[s], vo[r], (uint32_t)(s0 + s) * Pb + zo_b);
+        }
+      }
+    }
+    __syncthreads();
+    if (first) {
+#pragma unroll
+      for (int r = 0; r < RY; ++r)
+#pragma unroll
+        for (int s = 0; s < S; ++s) mid[s0 + s][wy + NW * r][lx] = p1n[r][s];
+    }
+    // rotate the queues
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      qmz2[r] = qm[0][r];
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        p1[0][r][s] = p1[1][r][s];
+        p1[1][r][s] = p1n[r][s];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k + 1 < QN; ++k)
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        qm[k][r] = qm[k + 1][r];
+#pragma unroll
+        for (int s = 0; s < S; ++s) q[k][r][s] = q[k + 1][r][s];
+      }
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      qm[QN - 1][r] = qmn[r];
+#pragma unroll
+      for (int s = 0; s < S; ++s) q[QN - 1][r][s] = qn[r][s];
+    }
+#pragma unroll
+    for (int k = 0; k + 1 < HN; ++k) {
+      hm[k] = hm[k + 1];
+#pragma unroll
+      for (int s = 0; s < S; ++s) h[k][s] = h[k + 1][s];
+    }
+    hm[HN - 1] = hmn;
+#pragma unroll
+    for (int s = 0; s < S; ++s) h[HN - 1][s] = hn[s];
+  }
+}
+
 template <int RY, bool SRC, int PD, int SG>
 static hipError_t launch_stencil_t(marl_ctx* ctx, const StencilParams& p, int zc) {
   const int X = ctx->d.x, Y = ctx->d.y, Z = ctx->d.z;
@@ -404,6 +726,56 @@ int marl_launch_substep(marl_ctx* ctx, const StencilParams& p, int tile) {
   return MARL_OK;
 }
 
+template <int RY, bool SRC, int PD, int NW, int SG>
+static hipError_t launch_stencil2_t(marl_ctx* ctx, const StencilParams& p, int zc) {
+  constexpr int TY = NW * RY;
+  const int X = ctx->d.x, Y = ctx->d.y, Z = ctx->d.z;
+  const int ntx = (X + 61) / 62, nty = (Y + TY - 3) / (TY - 2), ntz = (Z + zc - 1) / zc;
+  hipLaunchKernelGGL((k_stencil2<RY, SRC, PD, NW, SG>), dim3(ntx * nty * ntz), dim3(64 * NW * (MARL_S_EXT / SG)), 0,
+                     ctx->stream, ctx->field[ctx->cur], ctx->field[ctx->cur ^ 1], ctx->occ, X, Y, Z, ctx->xw, zc, ntx,
+                     nty, ntz, p);
+  return hipGetLastError();
+}
+
+template <int RY, int PD, int NW, int SG>
+static hipError_t launch_stencil2_s(marl_ctx* ctx, const StencilParams& p, int zc) {
+  return p.apply_sources ? launch_stencil2_t<RY, true, PD, NW, SG>(ctx, p, zc)
+                         : launch_stencil2_t<RY, false, PD, NW, SG>(ctx, p, zc);
+}
+
+// Two substeps in one launch with a temporal tile.
+static int launch_double(marl_ctx* ctx, const StencilParams& p, const TileShape& t) {
+  const int zc = t.zc < ctx->d.z ? t.zc : ctx->d.z;
+  hipError_t e = hipErrorInvalidValue;
+  if (t.sg == 12 && t.ry == 1 && t.pd == 1 && t.nw == 8) e = launch_stencil2_s<1, 1, 8, 12>(ctx, p, zc);
+  else if (t.sg == 6 && t.ry == 1 && t.pd == 1 && t.nw == 8) e = launch_stencil2_s<1, 1, 8, 6>(ctx, p, zc);
+  MARL_HIP(ctx, e);
+  ctx->cur ^= 1;
+  return MARL_OK;
+}
+
+int marl_launch_substeps(marl_ctx* ctx, const marl_sim_params* sim, int n, bool sources_first, int tile,
+                         int* launches) {
+  if (tile < 0 || tile >= kNumTiles) tile = marl_default_tile(ctx->d);
+  const TileShape t = kTiles[tile];
+  int k = 0, nl = 0;
+  bool src = sources_first;
+  if (t.tb == 2) {
+    for (; k + 2 <= n; k += 2, ++nl) {
+      if (int rc = launch_double(ctx, marl_stencil_params(sim, src ? 1 : 0), t)) return rc;
+      src = false;
+    }
+  }
+  // single substeps (an odd remainder of a temporal tile runs on the default tile)
+  const int single = t.tb == 2 ? marl_default_tile(ctx->d) : tile;
+  for (; k < n; ++k, ++nl) {
+    if (int rc = marl_launch_substep(ctx, marl_stencil_params(sim, src ? 1 : 0), single)) return rc;
+    src = false;
+  }
+  if (launches) *launches = nl;
+  return MARL_OK;
+}
+
 template <int RY, int PD, int SG>
 static hipError_t launch_stencil_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total, int parity,
                                      const StencilParams& p, int grid) {
@@ -416,7 +788,7 @@ static hipError_t launch_stencil_ens(hipStream_t st, const EnsColumn* dcols, int
 }
 
 void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int* total_light_blocks) {
-  if (tile < 0 || tile >= kNumTiles) tile = 3;
+  if (tile < 0 || tile >= kNumTiles || kTiles[tile].tb != 1) tile = 3;
   const TileShape t = kTiles[tile];
   const int NG = MARL_S_EXT / t.sg;
   int base = 0, lbase = 0;
@@ -439,7 +811,7 @@ void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int
 // Persistent grid: as many blocks as fit the chip at once (occupancy of the
 // instantiation x CUs), a multiple of 8 so every XCD gets the same share.
 int marl_stencil_ens_grid(int tile, int total_tiles) {
-  if (tile < 0 || tile >= kNumTiles) tile = 3;
+  if (tile < 0 || tile >= kNumTiles || kTiles[tile].tb != 1) tile = 3;
   const TileShape t = kTiles[tile];
   int per_cu = 0;
   hipError_t e = hipErrorInvalidValue;
@@ -459,7 +831,7 @@ int marl_stencil_ens_grid(int tile, int total_tiles) {
 
 hipError_t marl_launch_substep_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total, int parity,
                                    const StencilParams& p, int tile, int grid) {
-  if (tile < 0 || tile >= kNumTiles) tile = 3;
+  if (tile < 0 || tile >= kNumTiles || kTiles[tile].tb != 1) tile = 3;
   const TileShape t = kTiles[tile];
   if (t.sg == 12) {
     if (t.ry == 1 && t.pd == 1) return launch_stencil_ens<1, 1, 12>(st, dcols, ncols, total, parity, p, grid);

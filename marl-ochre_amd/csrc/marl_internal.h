@@ -228,9 +228,12 @@ struct marl_ctx {
 
   // device-driven tick captured as a hipGraph
   void* tdev = nullptr;            // per-tick counters on the device (TickDev)
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  std::vector<uint8_t> graph_key;  // inputs the captured graph was recorded for
+  // one captured tick per field parity (a tick with an odd number of stencil
+  // launches alternates between them)
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::vector<uint8_t> graph_key[2];  // inputs each captured graph was recorded for
+  int graph_flip[2] = {0, 0};         // the graph swaps the field buffers
   int graph_sweeps = 64, graph_levels = 64;
   marl_sim_params last_sim{};
 
@@ -251,6 +254,11 @@ int marl_check_params(marl_ctx* ctx, const marl_sim_params* sim);
 
 // kernels launched from other translation units
 int marl_launch_substep(marl_ctx* ctx, const StencilParams& p, int tile);
+// n substeps of sim with the given tile (two per launch for temporal tiles),
+// boundary sources fused into the first when sources_first; *launches (may
+// be null) = kernel launches, i.e. field buffer swaps.
+int marl_launch_substeps(marl_ctx* ctx, const marl_sim_params* sim, int n, bool sources_first, int tile,
+                         int* launches = nullptr);
 int marl_launch_boundary(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim);
 // Ensembles: fill the tile fields of `cols` for a tile id (returns total
@@ -282,7 +290,7 @@ int marl_ensure_staging(marl_ctx* ctx);
 int marl_launch_copy_probe(marl_ctx* ctx);
 
 // stencil tiles: rows per thread and z-chunk
-struct TileShape { int ry, zc, pd, sg; };
+struct TileShape { int ry, zc, pd, sg, tb, nw; };  // tb: substeps per pass (2 = temporal blocking); nw: waves/block
 extern const TileShape kTiles[];
 extern const int kNumTiles;
 int marl_default_tile(const marl_dims& d);

@@ -152,6 +152,10 @@ class MarlGpu:
         self._chk(self.lib.marl_stencil_tile_shape(int(t), ctypes.byref(ry), ctypes.byref(zc), ctypes.byref(pd)))
         return ry.value, zc.value, pd.value
 
+    def tile_substeps(self, t: int | None = None) -> int:
+        """Substeps per launch of a tile (2 for the temporally blocked tiles)."""
+        return int(self.lib.marl_stencil_tile_substeps(int(self.tile if t is None else t)))
+
     def sync(self) -> None:
         self._chk(self.lib.marl_sync(self.h))
 

@@ -151,3 +151,39 @@ def test_autotune_keeps_field_and_picks_valid_tile(pkg, orc):
         assert np.array_equal(g.download_field_aos(), f)
         g.diffuse(sim)
         assert np.array_equal(g.download_field_aos(), orc.diffuse_tick(f, occ, dims, sim))
+
+
+@pytest.mark.parametrize("dims", [(128, 70, 37), (64, 64, 16), (32, 5, 3)])
+def test_temporal_tiles_bit_identical(pkg, orc, dims):
+    """Two-substep tiles: even and odd substep counts (an odd remainder runs a
+    single-substep launch), and a full tick with the boundary sources fused into
+    the first launch, against the oracle."""
+    f, occ = _rand_case(dims, 7)
+    with pkg.MarlGpu(dims) as g:
+        temporal = [t for t in range(g.lib.marl_stencil_tile_count()) if g.tile_substeps(t) == 2]
+        assert temporal
+        for n in (1, 2, 4, 5):
+            sim = pkg.SimulationConfig(diffusion_substeps=n)
+            want = orc.diffuse_tick(f, occ, dims, sim)
+            for t in temporal:
+                g.upload_field_aos(f)
+                g.upload_occupancy(occ)
+                g.tile = t
+                g.diffuse(sim)
+                got = g.download_field_aos()
+                assert np.array_equal(got, want), f"tile {t}, {n} substeps: {np.count_nonzero(got != want)} differ"
+        A = pkg._abi
+        empty_c = np.zeros(0, A.CELL_DTYPE)
+        empty_r = np.zeros(0, A.RULESET_DTYPE)
+        sim = pkg.SimulationConfig(diffusion_substeps=4)
+        fw, lw, _, _, _, _ = orc.tick(dims, sim, 0, f, empty_c, empty_r)
+        for t in temporal:
+            g.upload_field_aos(f)
+            g.upload_cells(empty_c, empty_r)
+            g.tile = t
+            g.tick(sim, 0, 1)
+            assert np.array_equal(g.download_field_aos(), fw), t
+            assert np.array_equal(g.download_light(), lw), t
+            g.upload_field_aos(f)
+            g.tick_graph(sim, 0, 1)
+            assert np.array_equal(g.download_field_aos(), fw), t

@@ -218,44 +218,48 @@ __global__ void __launch_bounds__(256) k_level_scatter(int n, const int32_t* __r
 // also append their (features, post-kinetics deltas and fluxes) pair.
 constexpr int kKinBlock = 128;
 
+template <bool HARVEST>
 __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict__ cells,
                                               const marl_ruleset* __restrict__ rules, float* __restrict__ field,
                                               const float* __restrict__ light, const uint8_t* __restrict__ nbmask,
                                               uint8_t* __restrict__ events, const CellParams& p,
-                                              const HarvestSink& hv, float* I, float* E, float* D) {
-  marl_cell_state c = cells[i];
+                                              const HarvestSink& hv, float* sI, float* sE, float* sD) {
+  marl_cell_state c = load_cell(cells + i);
   const marl_ruleset& r = rules[i];
+  const uint32_t m = nbmask[i];
+  const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
 #pragma unroll
-  for (int k = 0; k < MARL_M_INT; ++k) I[k] = c.internal[k];
-  size_t v, nv[6];
-  const int n = cell_neighbours(c, nbmask[i], p, &v, nv);
-  gather_env(field, nv, n, E);
+  for (int k = 0; k < MARL_M_INT; ++k) sI[k] = c.internal[k];
+  gather_env6(field, v, m, p, sE);
   const float lt = light[v];
+  PtrArr I{sI}, D{sD};
+  const PtrArr E{sE};
   cell_kinetics(c, r, I, E, D, lt, p);
-  if (hv.count && (uint64_t)sample_hash(hv.tick, (uint32_t)i) < hv.threshold) {
+  if (HARVEST && (uint64_t)sample_hash(hv.tick, (uint32_t)i) < hv.threshold) {
     const unsigned long long slot = atomicAdd(hv.count, 1ull);
     if ((int64_t)slot < hv.cap) {
       float f[kFeat];
-      make_features(c.internal, c, E, lt, r, 0, f);
+      make_features(c.internal, c, sE, lt, r, 0, f);
       float* fo = hv.feat + slot * kFeat;
 #pragma unroll
       for (int k = 0; k < kFeat; ++k) fo[k] = f[k];
       float* to = hv.targ + slot * kTarg;
 #pragma unroll
-      for (int k = 0; k < MARL_M_INT; ++k) to[k] = I[k] - c.internal[k];
+      for (int k = 0; k < MARL_M_INT; ++k) to[k] = sI[k] - c.internal[k];
 #pragma unroll
-      for (int s2 = 0; s2 < MARL_S_EXT; ++s2) to[16 + s2] = D[s2];
+      for (int s2 = 0; s2 < MARL_S_EXT; ++s2) to[16 + s2] = sD[s2];
       hv.hash[slot] = topology_hash(r);
     }
   }
-  const uint8_t ev = cell_fate(c, r, I[0], p);
+  const uint8_t ev = cell_fate(c, r, sI[0], p);
 #pragma unroll
-  for (int k = 0; k < MARL_M_INT; ++k) c.internal[k] = I[k];
+  for (int k = 0; k < MARL_M_INT; ++k) c.internal[k] = sI[k];
   cells[i] = c;
   events[i] = ev;
-  scatter_deltas(field, nv, n, D);
+  scatter_deltas6(field, v, m, p, sD);
 }
 
+template <bool HARVEST>
 __global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restrict__ order, int count,
                                                       marl_cell_state* __restrict__ cells,
                                                       const marl_ruleset* __restrict__ rules,
@@ -268,8 +272,8 @@ __global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restric
   const int t = blockIdx.x * kKinBlock + threadIdx.x;
   if (t >= count) return;
   const int i = order ? order[t] : t;
-  kinetics_cell(i, cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17, sE + threadIdx.x * 13,
-                sD + threadIdx.x * 13);
+  kinetics_cell<HARVEST>(i, cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17,
+                         sE + threadIdx.x * 13, sD + threadIdx.x * 13);
 }
 
 // ---------------------------------------------------------------------------
@@ -742,8 +746,8 @@ __global__ void __launch_bounds__(kKinBlock) k_g_kinetics(int L, const TickDev* 
   const int lo = ts->off[L], cnt = ts->off[L + 1] - lo;
   const HarvestSink hv{};
   GS_LOOP(t, cnt)
-  kinetics_cell(order[lo + t], cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17,
-                sE + threadIdx.x * 13, sD + threadIdx.x * 13);
+  kinetics_cell<false>(order[lo + t], cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17,
+                       sE + threadIdx.x * 13, sD + threadIdx.x * 13);
 }
 
 // division targets for i < n; zero flags up to cap for the fixed-size scan
@@ -1029,8 +1033,12 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
     if (ctx->kin_mode != MARL_KINETICS_PLAIN && ctx->surr) {
       if (int rc = surrogate_level(ctx, p, ctx->order + off[L], cnt, cells, rules, field)) return rc;
     } else {
-      hipLaunchKernelGGL(k_kinetics, dim3(grid1(cnt, kKinBlock)), dim3(kKinBlock), 0, ctx->stream,
-                         ctx->order + off[L], cnt, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p, hv);
+      if (hv.count)
+        hipLaunchKernelGGL(k_kinetics<true>, dim3(grid1(cnt, kKinBlock)), dim3(kKinBlock), 0, ctx->stream,
+                           ctx->order + off[L], cnt, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p, hv);
+      else
+        hipLaunchKernelGGL(k_kinetics<false>, dim3(grid1(cnt, kKinBlock)), dim3(kKinBlock), 0, ctx->stream,
+                           ctx->order + off[L], cnt, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p, hv);
     }
   }
   MARL_HIP(ctx, hipGetLastError());

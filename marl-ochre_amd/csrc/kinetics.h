@@ -28,6 +28,42 @@ __device__ inline uint16_t f32_as_u16(float f) {  // Rust `as u16`: truncate, sa
   return (uint16_t)f;
 }
 
+// The parts of a ruleset the tick reads (transporters .. fate thresholds; the
+// receptors' result is unused), copied into registers with 29 independent
+// 16-byte loads at the start of a cell's update so that their latency is paid
+// once instead of once per reaction.
+struct RuleRegs {
+  marl_transport transport[MARL_S_TRANSPORTERS];
+  marl_reaction reactions[MARL_R_MAX];
+  marl_effector effectors[MARL_S_EFFECTORS];
+  float division_energy, death_energy, quiescence_energy, division_prep_ticks;
+};
+static_assert(sizeof(RuleRegs) == 464 && sizeof(RuleRegs) % 16 == 0, "RuleRegs layout");
+static_assert(offsetof(marl_ruleset, transport) == 96 && offsetof(marl_ruleset, hgt_propensity) == 96 + 464,
+              "RuleRegs mirrors marl_ruleset from the transporters on");
+
+__device__ __forceinline__ RuleRegs load_rules(const marl_ruleset* r) {
+  constexpr int N = sizeof(RuleRegs) / 16;
+  const uint4* s = reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(r) + 96);
+  uint4 buf[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) buf[k] = s[k];
+  RuleRegs out;
+  __builtin_memcpy(&out, buf, sizeof(out));
+  return out;
+}
+
+__device__ __forceinline__ marl_cell_state load_cell(const marl_cell_state* c) {
+  constexpr int N = sizeof(marl_cell_state) / 16;
+  const uint4* s = reinterpret_cast<const uint4*>(c);
+  uint4 buf[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) buf[k] = s[k];
+  marl_cell_state out;
+  __builtin_memcpy(&out, buf, sizeof(out));
+  return out;
+}
+
 // Free face neighbours (spatial.rs:18-43, FACE_OFFSETS order +x,-x,+y,-y,+z,-z)
 // of the cell's voxel from its 6-bit mask; returns their count.
 __device__ inline int cell_neighbours(const marl_cell_state& c, uint32_t m, const CellParams& p, size_t* v_out,
@@ -60,6 +96,39 @@ __device__ inline void gather_env(const float* __restrict__ field, const size_t 
   for (int s = 0; s < MARL_S_EXT; ++s) E[s] = n ? acc[s] / (float)n : 0.0f;
 }
 
+// The same mean with all six face records requested at once (non-free faces
+// read the cell's own voxel and are ignored); the sum still runs over the free
+// faces in face order, so the result is identical to gather_env.
+__device__ inline void gather_env6(const float* __restrict__ field, size_t v, uint32_t m, const CellParams& p,
+                                   float* E) {
+  const size_t XY = (size_t)p.X * p.Y;
+  const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
+  float4 a[6], b[6], cc[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const size_t w = ((m >> k) & 1u) ? (size_t)((int64_t)v + off[k]) : v;
+    const float4* fv = reinterpret_cast<const float4*>(field + w * MARL_S_EXT);
+    a[k] = fv[0];
+    b[k] = fv[1];
+    cc[k] = fv[2];
+  }
+  float acc[MARL_S_EXT];
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) acc[s] = 0.0f;
+  int n = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if ((m >> k) & 1u) {
+      ++n;
+      acc[0] += a[k].x; acc[1] += a[k].y; acc[2] += a[k].z; acc[3] += a[k].w;
+      acc[4] += b[k].x; acc[5] += b[k].y; acc[6] += b[k].z; acc[7] += b[k].w;
+      acc[8] += cc[k].x; acc[9] += cc[k].y; acc[10] += cc[k].z; acc[11] += cc[k].w;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) E[s] = n ? acc[s] / (float)n : 0.0f;
+}
+
 // Split deltas into the free neighbours with the max(0) clamp
 // (spatial.rs:87-110, field.rs:57-62); lost when enclosed.
 __device__ inline void scatter_deltas(float* __restrict__ field, const size_t nv[6], int n, const float* D) {
@@ -81,8 +150,37 @@ __device__ inline void scatter_deltas(float* __restrict__ field, const size_t nv
   }
 }
 
+// The same scatter over the six faces by mask bit, without a neighbour list
+// (each free face is a different voxel, so the order of the updates does not
+// matter).
+__device__ inline void scatter_deltas6(float* __restrict__ field, size_t v, uint32_t m, const CellParams& p,
+                                       const float* D) {
+  const int n = __builtin_popcount(m);
+  if (n == 0) return;
+  const size_t XY = (size_t)p.X * p.Y;
+  const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
+  const float nf = (float)n;
+  float split[MARL_S_EXT];
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) split[s] = D[s] / nf;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (!((m >> k) & 1u)) continue;
+    float4* fv = reinterpret_cast<float4*>(field + (size_t)((int64_t)v + off[k]) * MARL_S_EXT);
+    float4 a = fv[0], b = fv[1], cc = fv[2];
+    a.x = fmaxf(a.x + split[0], 0.0f); a.y = fmaxf(a.y + split[1], 0.0f);
+    a.z = fmaxf(a.z + split[2], 0.0f); a.w = fmaxf(a.w + split[3], 0.0f);
+    b.x = fmaxf(b.x + split[4], 0.0f); b.y = fmaxf(b.y + split[5], 0.0f);
+    b.z = fmaxf(b.z + split[6], 0.0f); b.w = fmaxf(b.w + split[7], 0.0f);
+    cc.x = fmaxf(cc.x + split[8], 0.0f); cc.y = fmaxf(cc.y + split[9], 0.0f);
+    cc.z = fmaxf(cc.z + split[10], 0.0f); cc.w = fmaxf(cc.w + split[11], 0.0f);
+    fv[0] = a; fv[1] = b; fv[2] = cc;
+  }
+}
+
 // Phase 5 (cell.rs:332-366) from the post-effector energy.
-__device__ inline uint8_t cell_fate(marl_cell_state& c, const marl_ruleset& r, float energy, const CellParams& p) {
+template <typename R>
+__device__ inline uint8_t cell_fate(marl_cell_state& c, const R& r, float energy, const CellParams& p) {
   c.age += 1;
   const float eff_death = fmaxf(r.death_energy, p.hard_death_floor);
   if (energy < eff_death) {
@@ -103,31 +201,65 @@ __device__ inline uint8_t cell_fate(marl_cell_state& c, const marl_ruleset& r, f
   return MARL_EVENT_NONE;
 }
 
-// Phases 2-4 of CellState::tick (cell.rs:223-329) on I[16] (LDS slot, updated
-// in place) with environment E[12], accumulating exchange fluxes into D[12].
-// The receptor pass (:206-220) only writes an unused array and is omitted.
-__device__ inline void cell_kinetics(const marl_cell_state& c, const marl_ruleset& r, float* I, const float* E,
-                                     float* D, float lt, const CellParams& p) {
+// Per-cell arrays indexed by data (species numbers taken from the ruleset).
+// PtrArr views a thread's LDS slot.  RegArr<N> keeps the array in registers:
+// a read is a select tree over the index bits and a write one select per
+// element, so a data-dependent index costs a few dependent VALU operations
+// instead of an LDS round trip (the cell update is a chain of such accesses).
+struct PtrArr {
+  float* p;
+  __device__ __forceinline__ float get(int i) const { return p[i]; }
+  __device__ __forceinline__ void set(int i, float x) { p[i] = x; }
+};
+
+template <int N>
+struct RegArr {
+  static_assert(N <= 16, "RegArr size");
+  float v[N];
+  __device__ __forceinline__ float get(int i) const {
+    float a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = k < N ? v[k] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = (i & 1) ? a[2 * k + 1] : a[2 * k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = (i & 2) ? a[2 * k + 1] : a[2 * k];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) a[k] = (i & 4) ? a[2 * k + 1] : a[2 * k];
+    return (i & 8) ? a[1] : a[0];
+  }
+  __device__ __forceinline__ void set(int i, float x) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = (i == k) ? x : v[k];
+  }
+};
+
+// Phases 2-4 of CellState::tick (cell.rs:223-329) on I[16] (updated in place)
+// with environment E[12], accumulating exchange fluxes into D[12].  The
+// receptor pass (:206-220) only writes an unused array and is omitted.
+template <typename R, typename AI, typename AE, typename AD>
+__device__ inline void cell_kinetics(const marl_cell_state& c, const R& r, AI& I, const AE& E, AD& D, float lt,
+                                     const CellParams& p) {
   const float dt = p.dt;
 #pragma unroll
-  for (int s = 0; s < MARL_S_EXT; ++s) D[s] = 0.0f;
+  for (int s = 0; s < MARL_S_EXT; ++s) D.set(s, 0.0f);
   // transport (cell.rs:223-238)
 #pragma unroll
   for (int q = 0; q < MARL_S_TRANSPORTERS; ++q) {
     const marl_transport tp = r.transport[q];
     const int e = tp.ext_species, k = tp.int_species;
     if (e >= MARL_S_EXT || k >= MARL_M_INT) continue;
-    const float ex = E[e];
-    const float ik = I[k];
+    const float ex = E.get(e);
+    const float ik = I.get(k);
     const float up = tp.uptake_rate * ex / (1.0f + ex);
     const float sec = tp.secrete_rate * ik / (1.0f + ik);
-    I[k] = clamp_rs(ik + (up - sec) * dt, 0.0f, p.c_max);
-    D[e] += (sec - up) * dt;
+    I.set(k, clamp_rs(ik + (up - sec) * dt, 0.0f, p.c_max));
+    D.set(e, D.get(e) + (sec - up) * dt);
   }
-  I[MARL_M_INT - 1] = lt;  // :243
+  I.set(MARL_M_INT - 1, lt);  // :243
   // reactions (cell.rs:246-285)
   int active = 0;
-#pragma unroll 4
+#pragma unroll
   for (int q = 0; q < MARL_R_MAX; ++q) {
     const marl_reaction rx = r.reactions[q];
     if (fabsf(rx.v_max) > p.active_reaction_threshold && rx.substrate != rx.product) ++active;
@@ -135,23 +267,23 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const marl_rulese
     if (rx.substrate == rx.product) continue;
     const int si = rx.substrate, pi = rx.product, ci = rx.catalyst;
     if (si >= MARL_M_INT || pi >= MARL_M_INT || ci >= MARL_M_INT) continue;
-    const float s = I[si];
-    const float cc = I[ci];
+    const float s = I.get(si);
+    const float cc = I.get(ci);
     const float st = s / (rx.k_m + s + 1.1920929e-07f);
     const float ct = p.epsilon + cc / (rx.k_cat + cc + 1.1920929e-07f);
     float rate = rx.v_max * st * ct;
     if (rx.cofactor != 0xFF) {
       const int fi = rx.cofactor;
       if (fi < MARL_M_INT) {
-        const float cof = I[fi];
+        const float cof = I.get(fi);
         rate *= cof / (1.0f + cof);
-        const float used = fminf(0.5f * rate * dt, I[fi]);
-        I[fi] -= used;
+        const float used = fminf(0.5f * rate * dt, I.get(fi));
+        I.set(fi, I.get(fi) - used);
       }
     }
-    const float flux = fminf(rate * dt, I[si]);
-    I[si] -= flux;
-    I[pi] = fminf(I[pi] + flux, p.c_max);
+    const float flux = fminf(rate * dt, I.get(si));
+    I.set(si, I.get(si) - flux);
+    I.set(pi, fminf(I.get(pi) + flux, p.c_max));
   }
   // maintenance (cell.rs:291-311)
   float pm = 1.0f;
@@ -160,10 +292,10 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const marl_rulese
     const float rush = fmaxf(p.base_division_prep - evolved, 0.0f);
     pm = p.prep_maintenance_multiplier + rush * p.rush_penalty_rate;
   }
-  float e0 = I[0];
+  float e0 = I.get(0);
   e0 *= 1.0f - p.lambda_maintenance * pm * dt;
   e0 = fmaxf(e0 - (float)active * p.reaction_maintenance * dt, 0.0f);
-  I[0] = e0;
+  I.set(0, e0);
   // effectors (cell.rs:314-329)
   if (!c.quiescent) {
 #pragma unroll
@@ -171,11 +303,11 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const marl_rulese
       const marl_effector ef = r.effectors[q];
       const int k = ef.int_species, e = ef.ext_species;
       if (k >= MARL_M_INT || e >= MARL_S_EXT) continue;
-      const float ik = I[k];
+      const float ik = I.get(k);
       if (ik > ef.threshold) {
         const float amount = ef.rate * ik / (1.0f + ik) * dt;
-        I[k] = fmaxf(ik - amount, 0.0f);
-        D[e] += amount;
+        I.set(k, fmaxf(ik - amount, 0.0f));
+        D.set(e, D.get(e) + amount);
       }
     }
   }
@@ -193,8 +325,9 @@ constexpr int kTarg = 28;
 
 __device__ inline float safe_log(float v) { return __logf(fmaxf(fabsf(v), 1e-12f)); }
 
+template <typename R>
 __device__ inline void make_features(const float* I_pre, const marl_cell_state& c, const float* E, float lt,
-                                     const marl_ruleset& r, int cls, float* f) {
+                                     const R& r, int cls, float* f) {
 #pragma unroll
   for (int k = 0; k < 15; ++k) f[k] = I_pre[k];
   f[15] = c.quiescent ? 1.0f : 0.0f;
@@ -216,7 +349,8 @@ __device__ inline void make_features(const float* I_pre, const marl_cell_state& 
 // 64-bit FNV-1a over the ruleset's integer wiring (reaction species,
 // transporter and effector species): cells with the same hash share a
 // reaction topology.
-__device__ inline uint64_t topology_hash(const marl_ruleset& r) {
+template <typename R>
+__device__ inline uint64_t topology_hash(const R& r) {
   uint64_t h = 1469598103934665603ull;
   auto mix = [&](uint32_t b) {
     h ^= b;

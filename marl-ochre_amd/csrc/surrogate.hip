@@ -170,7 +170,11 @@ __global__ void __launch_bounds__(kSBlock) k_surr_apply(const int32_t* __restric
   }
   const bool shadow = have && (mode == MARL_KINETICS_SHADOW ||
                                (use_s && (uint64_t)sample_hash(p.stream, (uint32_t)i) < shadow_threshold));
-  if (!use_s || shadow) cell_kinetics(c, r, I, E, D, lt, p);
+  if (!use_s || shadow) {
+    PtrArr aI{I}, aD{D};
+    const PtrArr aE{E};
+    cell_kinetics(c, r, aI, aE, aD, lt, p);
+  }
   if (shadow) {
     record_error(Ip, I, stats);
     marl_cell_state cs = c, cp = c;

@@ -99,8 +99,11 @@ def test_graph_tick_starter_column(pkg):
     _compare_runs(pkg, (128, 128, 64), field, cells, rules, sim, 20)
 
 
-def test_graph_tick_rejects_odd_substeps(pkg):
-    dims = (32, 16, 8)
-    with pkg.MarlGpu(dims) as g:
-        with pytest.raises(pkg.InvalidInput):
-            g.tick_graph(pkg.SimulationConfig(diffusion_substeps=3), 0, 1)
+def test_graph_tick_odd_substeps(pkg):
+    """An odd number of stencil launches per tick swaps the field buffers each
+    tick: the captured ticks alternate between two graphs, one per parity."""
+    A = pkg._abi
+    dims = (64, 32, 24)
+    cells, rules = population(A, dims, 3000, 8)
+    field = np.random.default_rng(6).random(np.prod(dims) * 12).astype(F)
+    _compare_runs(pkg, dims, field, cells, rules, pkg.SimulationConfig(rng_seed=3, diffusion_substeps=3), 4)

@@ -85,13 +85,6 @@ struct StencilLds {
   uint32_t m[(4 * RY + 2) * 4 + 256];
 };
 
-// XCD-aware tile order: hardware dispatches block b to XCD b % 8; give each
-// XCD a contiguous run of logical tiles so x/y-neighbour tiles (which share
-// halo lines) sit in the same L2 and march through z together.
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-  const int nb8 = nb & ~7;
-  return b < nb8 ? (b & 7) * (nb8 >> 3) + (b >> 3) : b;
-}
 
 // One tile (tx, ty, tz) of species group grp of one substep.
 template <int RY, bool SRC, int PD, int SG>

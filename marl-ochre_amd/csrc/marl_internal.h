@@ -164,6 +164,16 @@ struct StencilParams {
 };
 
 // ---------------------------------------------------------------------------
+// XCD-aware block order: hardware dispatches block b to XCD b % 8; give each
+// XCD a contiguous run of logical blocks so that blocks touching neighbouring
+// data (stencil tiles sharing halo lines, spatially adjacent cells) sit in the
+// same L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int nb8 = nb & ~7;
+  return b < nb8 ? (b & 7) * (nb8 >> 3) + (b >> 3) : b;
+}
+
+// ---------------------------------------------------------------------------
 // One column of an ensemble as the batched field kernels see it.
 struct EnsColumn {
   float* f[2];           // the column's field ping-pong buffers (SoA planes)

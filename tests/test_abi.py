@@ -72,3 +72,48 @@ def test_sim_params_defaults_match_reference(pkg):
     assert list(np.float32(p.d_voxel)) == list(np.float32([0.0, 1.5, 1.0, 1.2, 0.8, 0.5, 0.5, 0.1, 0.3, 0.3, 0.3, 0.3]))
     assert np.float32(p.alpha_eps) == np.float32(0.8) and np.float32(p.k_eps) == np.float32(2.0)
     assert np.float32(p.light_floor) == np.float32(1e-7) and p.division_neighbor_distance == 2
+
+
+def test_entry_points_validate_arguments_without_device(pkg):
+    """Argument checks that return before touching a device."""
+    A = pkg._abi
+    lib = pkg.load_library()
+    out = ctypes.c_void_p()
+    assert lib.marl_ensemble_create(None, 0, ctypes.byref(out)) == A.ERR_INVALID_INPUT
+    assert lib.marl_ensemble_create(None, 3, ctypes.byref(out)) == A.ERR_INVALID_INPUT
+    arr = (ctypes.c_void_p * 2)(None, None)
+    assert lib.marl_ensemble_create(arr, 2, ctypes.byref(out)) == A.ERR_INVALID_INPUT
+    assert lib.marl_ensemble_tick(None, None, None, 0, 1, None, None, None) == A.ERR_INVALID_INPUT
+    assert lib.marl_set_graph_limits(None, 8, 8) == A.ERR_INVALID_INPUT
+    assert lib.marl_tick_graph(None, None, 0, 1, None, None) == A.ERR_INVALID_INPUT
+    assert lib.marl_set_kinetics_mode(None, 0) == A.ERR_INVALID_INPUT
+    assert lib.marl_surrogate_param_count() == 159516
+
+
+def test_tile_table(pkg):
+    """Tile ids: single-substep shapes first, then the two-substep (temporally
+    blocked) ones; every id reports its shape."""
+    lib = pkg.load_library()
+    n = lib.marl_stencil_tile_count()
+    subs = [lib.marl_stencil_tile_substeps(t) for t in range(n)]
+    assert set(subs) == {1, 2} and subs == sorted(subs)
+    assert lib.marl_stencil_tile_substeps(-1) < 0 and lib.marl_stencil_tile_substeps(n) < 0
+    ry, zc, pd = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    for t in range(n):
+        assert lib.marl_stencil_tile_shape(t, ctypes.byref(ry), ctypes.byref(zc), ctypes.byref(pd)) == 0
+        assert ry.value in (1, 2, 4) and zc.value in (16, 32, 64) and pd.value in (1, 2)
+
+
+def test_topology_hash_groups_wiring(pkg):
+    """Host restatement of the surrogate's topology key: rulesets differing only
+    in rates share a hash, any wiring change separates them."""
+    from marl_ochre_amd import scenarios, starters
+
+    r = np.zeros(4, pkg.RULESET_DTYPE)
+    for k in range(4):
+        r[k] = starters.make_ruleset(k % 3)
+    r["reactions"]["v_max"][3] *= 2.0  # same wiring as ruleset 0
+    h = scenarios.topology_hash(r)
+    assert h[0] == h[3] and len(set(h[:3].tolist())) == 3
+    r["reactions"]["substrate"][3, 5] = (int(r["reactions"]["substrate"][3, 5]) + 1) % 15
+    assert scenarios.topology_hash(r)[3] != h[0]

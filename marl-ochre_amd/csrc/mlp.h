@@ -22,6 +22,7 @@ struct MlpState {
   uint16_t* packed = nullptr;                                     // bf16 MFMA fragment images
   uint16_t* pk[4] = {nullptr, nullptr, nullptr, nullptr};
   float* norm = nullptr;  // feature mean[kIn], std[kIn], target mean[kOut], std[kOut]
+  float* ws = nullptr;    // split-K partial products of the weight gradients
   int step = 0;
   int batch = 0;
   float *x = nullptr, *t = nullptr, *y = nullptr, *dy = nullptr;

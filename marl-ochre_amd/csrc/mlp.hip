@@ -7,6 +7,7 @@
 // template flags) with fused epilogues, plus bias-gradient, SiLU-derivative,
 // loss and Adam kernels.  Inference uses a fused kernel (all four layers, the
 // activations of a 64-cell tile kept in LDS, weights pre-packed per MFMA lane).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -45,18 +46,23 @@ __device__ __forceinline__ float silu_grad(float z) {
 
 enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_SILU = 2 };
 
+// Split-K: block z covers k in [z * kchunk, min(K, (z + 1) * kchunk)) and
+// writes its partial product to C + z * M * ldc (kchunk = K: one split).
 template <bool TA, bool TB, int EPI>
 __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const float* __restrict__ A, int lda,
                                               const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
-                                              const float* __restrict__ bias, float* __restrict__ Zout) {
+                                              const float* __restrict__ bias, float* __restrict__ Zout,
+                                              int kchunk) {
   constexpr int BM = 64, BN = 64, BK = 32, LP = BK + 8;  // +8 bf16 pad per row
   __shared__ uint16_t As[BM][LP];
   __shared__ uint16_t Bs[BN][LP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int k_lo = blockIdx.z * kchunk, k_hi = min(K, k_lo + kchunk);
+  C += (size_t)blockIdx.z * M * ldc;
   f32x16 acc = {};
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  for (int k0 = k_lo; k0 < k_hi; k0 += BK) {
     // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -66,14 +72,14 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const float* 
       else { mm = idx / BK; kk = idx % BK; }
       const int gm = m0 + mm, gk = k0 + kk;
       float v = 0.0f;
-      if (gm < M && gk < K) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+      if (gm < M && gk < k_hi) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
       As[mm][kk] = to_bf16_bits(v);
       int nn, kb;
       if (TB) { nn = idx / BK; kb = idx % BK; }   // B[n][k] rows contiguous in k
       else { kb = idx / BN; nn = idx % BN; }
       const int gn = n0 + nn, gkb = k0 + kb;
       float w = 0.0f;
-      if (gn < N && gkb < K) w = TB ? B[(size_t)gn * ldb + gkb] : B[(size_t)gkb * ldb + gn];
+      if (gn < N && gkb < k_hi) w = TB ? B[(size_t)gn * ldb + gkb] : B[(size_t)gkb * ldb + gn];
       Bs[nn][kb] = to_bf16_bits(w);
     }
     __syncthreads();
@@ -106,7 +112,40 @@ template <bool TA, bool TB, int EPI>
 hipError_t gemm(hipStream_t st, int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
                 int ldc, const float* bias = nullptr, float* Z = nullptr) {
   dim3 grid((N + 63) / 64, (M + 63) / 64);
-  hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, bias, Z);
+  hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, bias, Z, K);
+  return hipGetLastError();
+}
+
+// C[M x N] (ldc = N) = sum over splits of the partials in ws, in split order
+__global__ void k_sum_splits(size_t mn, int splits, const float* __restrict__ ws, float* __restrict__ C) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= mn) return;
+  float acc = ws[i];
+  for (int z = 1; z < splits; ++z) acc += ws[(size_t)z * mn + i];
+  C[i] = acc;
+}
+
+constexpr int kMaxSplits = 64;
+
+// Weight gradients have a small output (<= 256 x 256) and K = the batch: the
+// K range is split over enough blocks to fill the chip, partials summed in a
+// fixed order (deterministic).
+template <bool TA, bool TB>
+hipError_t gemm_splitk(hipStream_t st, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
+                       float* C, float* ws) {
+  const int tiles = ((N + 63) / 64) * ((M + 63) / 64);
+  int splits = std::min(kMaxSplits, std::max(1, 1024 / tiles));
+  int kchunk = (K + splits - 1) / splits;
+  kchunk = (kchunk + 31) / 32 * 32;
+  splits = (K + kchunk - 1) / kchunk;
+  if (splits <= 1) return gemm<TA, TB, EPI_STORE>(st, M, N, K, A, lda, B, ldb, C, N);
+  dim3 grid((N + 63) / 64, (M + 63) / 64, splits);
+  hipLaunchKernelGGL((k_gemm<TA, TB, EPI_STORE>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, ws, N, nullptr,
+                     nullptr, kchunk);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const size_t mn = (size_t)M * N;
+  hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, mn, splits, ws, C);
   return hipGetLastError();
 }
 
@@ -318,6 +357,7 @@ int mlp_alloc(marl_ctx* ctx, MlpState& s, int batch) {
     s.pk[2] = s.pk[1] + p12;
     s.pk[3] = s.pk[2] + p12;
     MARL_HIP(ctx, hipMalloc(&s.norm, sizeof(float) * 2 * (kIn + kOut)));
+    MARL_HIP(ctx, hipMalloc(&s.ws, sizeof(float) * kMaxSplits * kHidden * kHidden));
     s.step = 0;
   }
   if (batch > s.batch) {
@@ -348,7 +388,7 @@ int mlp_alloc(marl_ctx* ctx, MlpState& s, int batch) {
 }
 
 void mlp_free(MlpState& s) {
-  void* bufs[] = {s.w, s.g, s.m, s.v, s.packed, s.norm, s.x, s.t, s.y, s.dy, s.dh, s.dz, s.loss_part,
+  void* bufs[] = {s.w, s.g, s.m, s.v, s.packed, s.norm, s.ws, s.x, s.t, s.y, s.dy, s.dh, s.dz, s.loss_part,
                   s.z[0], s.z[1], s.z[2], s.h[0], s.h[1], s.h[2], s.xb, s.yinf};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -375,8 +415,8 @@ int mlp_forward_train(marl_ctx* ctx, MlpState& s, int M) {
 int mlp_backward(marl_ctx* ctx, MlpState& s, int M) {
   const MlpLayout L = mlp_layout();
   // layer 3
-  MARL_HIP(ctx, (gemm<true, false, EPI_STORE>(ctx->stream, kHidden, kOut, M, s.h[2], kHidden, s.dy, kOut,
-                                              s.g + L.w[3], kOut)));
+  MARL_HIP(ctx, (gemm_splitk<true, false>(ctx->stream, kHidden, kOut, M, s.h[2], kHidden, s.dy, kOut,
+                                          s.g + L.w[3], s.ws)));
   hipLaunchKernelGGL(k_bias_grad, dim3(kOut), dim3(256), 0, ctx->stream, M, kOut, s.dy, s.g + L.b[3]);
   MARL_HIP(ctx, (gemm<false, true, EPI_STORE>(ctx->stream, M, kHidden, kOut, s.dy, kOut, s.w + L.w[3], kOut, s.dh,
                                               kHidden)));
@@ -386,8 +426,8 @@ int mlp_backward(marl_ctx* ctx, MlpState& s, int M) {
     MARL_HIP(ctx, hipGetLastError());
     const float* hin = l > 0 ? s.h[l - 1] : s.x;
     const int K = l > 0 ? kHidden : kIn;
-    MARL_HIP(ctx, (gemm<true, false, EPI_STORE>(ctx->stream, K, kHidden, M, hin, K, s.dz, kHidden, s.g + L.w[l],
-                                                kHidden)));
+    MARL_HIP(ctx, (gemm_splitk<true, false>(ctx->stream, K, kHidden, M, hin, K, s.dz, kHidden, s.g + L.w[l],
+                                            s.ws)));
     hipLaunchKernelGGL(k_bias_grad, dim3(kHidden), dim3(256), 0, ctx->stream, M, kHidden, s.dz, s.g + L.b[l]);
     MARL_HIP(ctx, hipGetLastError());
     if (l > 0)

@@ -419,9 +419,9 @@ def main():
                     help="ticks of the 32-column ensemble leg (0 = skip)")
     ap.add_argument("--drift-ticks", type=int, default=0,
                     help="long-run accuracy report over this many ticks of the default column (0 = skip)")
-    ap.add_argument("--surrogate-steps", type=int, default=3000,
+    ap.add_argument("--surrogate-steps", type=int, default=20000,
                     help="training steps of the surrogate leg (0 = skip)")
-    ap.add_argument("--harvest-ticks", type=int, default=10,
+    ap.add_argument("--harvest-ticks", type=int, default=20,
                     help="plain ticks whose pairs train the surrogate")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -235,8 +235,19 @@ __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict
   PtrArr I{sI}, D{sD};
   const PtrArr E{sE};
   cell_kinetics(c, r, I, E, D, lt, p);
-  if (HARVEST && (uint64_t)sample_hash(hv.tick, (uint32_t)i) < hv.threshold) {
-    const unsigned long long slot = atomicAdd(hv.count, 1ull);
+  // one counter update per wave: a per-cell atomic would serialise on one address
+  const bool take = HARVEST && (uint64_t)sample_hash(hv.tick, (uint32_t)i) < hv.threshold;
+  unsigned long long slot = 0;
+  if (HARVEST) {
+    const unsigned long long mask = __ballot(take);
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (mask && lane == leader) base = atomicAdd(hv.count, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+  }
+  if (take) {
     if ((int64_t)slot < hv.cap) {
       float f[kFeat];
       make_features(c.internal, c, sE, lt, r, 0, f);

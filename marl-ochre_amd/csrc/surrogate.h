@@ -24,6 +24,7 @@ struct SurrogateState {
   float zmax = 6.0f;           // out-of-distribution bound on |standardised feature|
   float margin = 0.05f;        // fate margin on the predicted energy
   uint64_t shadow_threshold = 0;  // sampled surrogate cells also run the plain path for monitoring
+  float* inv = nullptr;        // 1 / feature std (device, kFeat), kept with the normalisation
   // per-level scratch
   uint16_t* featb = nullptr;
   float* out = nullptr;

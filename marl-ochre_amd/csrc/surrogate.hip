@@ -46,81 +46,91 @@ __device__ __host__ inline bool binary_feature(int k) { return k == 15 || k == 2
 
 constexpr int kSBlock = 128;
 
+// Statistics are counted per block in LDS and flushed with one global atomic
+// per non-zero counter: a global atomic per cell would serialise a million
+// updates per tick on a handful of addresses.
+__device__ __forceinline__ void stats_init(unsigned int* s) {
+  for (int k = threadIdx.x; k < ST_COUNT; k += blockDim.x) s[k] = 0u;
+  __syncthreads();
+}
+__device__ __forceinline__ void stats_flush(const unsigned int* s, unsigned long long* __restrict__ g) {
+  __syncthreads();
+  for (int k = threadIdx.x; k < ST_COUNT; k += blockDim.x) {
+    if (!s[k]) continue;
+    if (k == ST_MAXERR) atomicMax(&g[k], (unsigned long long)s[k]);
+    else atomicAdd(&g[k], (unsigned long long)s[k]);
+  }
+}
+
 __global__ void __launch_bounds__(kSBlock) k_surr_features(const int32_t* __restrict__ order, int count,
                                                          const marl_cell_state* __restrict__ cells,
                                                          const marl_ruleset* __restrict__ rules,
                                                          const float* __restrict__ field,
                                                          const float* __restrict__ light,
                                                          const uint8_t* __restrict__ nbmask, CellParams p,
-                                                         Classes cl, const float* __restrict__ norm, float zmax,
+                                                         Classes cl, const float* __restrict__ norm,
+                                                         const float* __restrict__ inv, float zmax,
                                                          uint16_t* __restrict__ featb, uint8_t* __restrict__ elig,
                                                          unsigned long long* __restrict__ stats) {
   __shared__ float sE[kSBlock * 13];
+  __shared__ unsigned int sst[ST_COUNT];
+  stats_init(sst);
   const int t = blockIdx.x * kSBlock + threadIdx.x;
-  if (t >= count) return;
-  const int i = order[t];
-  const marl_cell_state c = cells[i];
-  const marl_ruleset& r = rules[i];
-  float* E = sE + threadIdx.x * 13;
-  size_t v, nv[6];
-  const int n = cell_neighbours(c, nbmask[i], p, &v, nv);
-  gather_env(field, nv, n, E);
-  const int cls = class_of(topology_hash(r), cl);
-  float f[kFeat];
-  make_features(c.internal, c, E, light[v], r, cls < 0 ? 0 : cls, f);
-  bool ok = cls >= 0;
-  uint16_t* fo = featb + (size_t)t * kFeat;
-#pragma unroll 8
-  for (int k = 0; k < kFeat; ++k) {
-    const float z = (f[k] - norm[k]) / norm[kFeat + k];
-    if (!binary_feature(k) && !(fabsf(z) <= zmax)) ok = false;
-    fo[k] = bf16_bits(z);
+  if (t < count) {
+    const int i = order[t];
+    const marl_cell_state c = load_cell(cells + i);
+    const marl_ruleset& r = rules[i];
+    float* E = sE + threadIdx.x * 13;
+    const uint32_t m = nbmask[i];
+    const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
+    gather_env6(field, v, m, p, E);
+    const int cls = class_of(topology_hash(r), cl);
+    float f[kFeat];
+    make_features(c.internal, c, E, light[v], r, cls < 0 ? 0 : cls, f);
+    bool ok = cls >= 0;
+    uint16_t* fo = featb + (size_t)t * kFeat;
+#pragma unroll
+    for (int k = 0; k < kFeat; ++k) {
+      const float z = (f[k] - norm[k]) * inv[k];
+      if (!binary_feature(k) && !(fabsf(z) <= zmax)) ok = false;
+      fo[k] = bf16_bits(z);
+    }
+    elig[t] = ok ? 1 : 0;
+    atomicAdd(&sst[ST_CELLS], 1u);
+    if (cls < 0) atomicAdd(&sst[ST_FB_TOPO], 1u);
+    else if (!ok) atomicAdd(&sst[ST_FB_OOD], 1u);
   }
-  elig[t] = ok ? 1 : 0;
-  atomicAdd(&stats[ST_CELLS], 1ull);
-  if (cls < 0) atomicAdd(&stats[ST_FB_TOPO], 1ull);
-  else if (!ok) atomicAdd(&stats[ST_FB_OOD], 1ull);
+  stats_flush(sst, stats);
 }
 
-__device__ inline void record_error(const float* Is, const float* Ip, unsigned long long* stats) {
+__device__ inline void record_error(const float* Is, const float* Ip, unsigned int* sst) {
   float err = 0.0f;
 #pragma unroll
   for (int k = 0; k < 15; ++k) err = fmaxf(err, fabsf(Is[k] - Ip[k]) / fmaxf(fabsf(Ip[k]), 1e-3f));
-  atomicAdd(&stats[ST_SHADOW], 1ull);
-  if (err <= 1e-3f) atomicAdd(&stats[ST_WITHIN], 1ull);
-  atomicMax(&stats[ST_MAXERR], (unsigned long long)__builtin_bit_cast(uint32_t, err));
+  atomicAdd(&sst[ST_SHADOW], 1u);
+  if (err <= 1e-3f) atomicAdd(&sst[ST_WITHIN], 1u);
+  atomicMax(&sst[ST_MAXERR], __builtin_bit_cast(uint32_t, err));
   int bin = (int)floorf((log10f(fmaxf(err, 1e-9f)) + 9.0f) * 2.0f);
   bin = bin < 0 ? 0 : (bin >= kErrBins ? kErrBins - 1 : bin);
-  atomicAdd(&stats[ST_HIST + bin], 1ull);
+  atomicAdd(&sst[ST_HIST + bin], 1u);
 }
 
-// Applies one level: surrogate result where eligible and not near a fate
-// threshold, the plain tick otherwise; shadow cells compare both.
-__global__ void __launch_bounds__(kSBlock) k_surr_apply(const int32_t* __restrict__ order, int count,
-                                                      marl_cell_state* __restrict__ cells,
-                                                      const marl_ruleset* __restrict__ rules,
-                                                      float* __restrict__ field, const float* __restrict__ light,
-                                                      const uint8_t* __restrict__ nbmask,
-                                                      uint8_t* __restrict__ events, CellParams p,
-                                                      const float* __restrict__ out, const uint8_t* __restrict__ elig,
-                                                      const float* __restrict__ norm, int mode, float margin,
-                                                      uint64_t shadow_threshold, unsigned long long* __restrict__ stats) {
-  __shared__ float sI[kSBlock * 17];
-  __shared__ float sE[kSBlock * 13];
-  __shared__ float sD[kSBlock * 13];
-  const int t = blockIdx.x * kSBlock + threadIdx.x;
-  if (t >= count) return;
-  const int i = order[t];
-  float* I = sI + threadIdx.x * 17;
-  float* E = sE + threadIdx.x * 13;
-  float* D = sD + threadIdx.x * 13;
-  marl_cell_state c = cells[i];
+// One cell of a surrogate level (see k_surr_apply).
+__device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __restrict__ cells,
+                                           const marl_ruleset* __restrict__ rules, float* __restrict__ field,
+                                           const float* __restrict__ light, const uint8_t* __restrict__ nbmask,
+                                           uint8_t* __restrict__ events, const CellParams& p,
+                                           const float* __restrict__ out, const uint8_t* __restrict__ elig,
+                                           const float* __restrict__ norm, int mode, float margin,
+                                           uint64_t shadow_threshold, unsigned int* sst, float* I, float* E,
+                                           float* D) {
+  marl_cell_state c = load_cell(cells + i);
   const marl_ruleset& r = rules[i];
 #pragma unroll
   for (int k = 0; k < MARL_M_INT; ++k) I[k] = c.internal[k];
-  size_t v, nv[6];
-  const int n = cell_neighbours(c, nbmask[i], p, &v, nv);
-  gather_env(field, nv, n, E);
+  const uint32_t m = nbmask[i];
+  const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
+  gather_env6(field, v, m, p, E);
   const float lt = light[v];
 
   const float* tmean = norm + 2 * kFeat;
@@ -165,7 +175,7 @@ __global__ void __launch_bounds__(kSBlock) k_surr_apply(const int32_t* __restric
     bool near = false;
 #pragma unroll
     for (int q = 0; q < 3; ++q) near |= fabsf(e1 - thr[q]) < margin;
-    if (near) atomicAdd(&stats[ST_FB_MARGIN], 1ull);
+    if (near) atomicAdd(&sst[ST_FB_MARGIN], 1u);
     else if (mode == MARL_KINETICS_SURROGATE) use_s = true;
   }
   const bool shadow = have && (mode == MARL_KINETICS_SHADOW ||
@@ -176,14 +186,14 @@ __global__ void __launch_bounds__(kSBlock) k_surr_apply(const int32_t* __restric
     cell_kinetics(c, r, aI, aE, aD, lt, p);
   }
   if (shadow) {
-    record_error(Ip, I, stats);
+    record_error(Ip, I, sst);
     marl_cell_state cs = c, cp = c;
     const uint8_t es = cell_fate(cs, r, Ip[0], p), ep = cell_fate(cp, r, I[0], p);
     if (es == ep && cs.prep_remaining == cp.prep_remaining && cs.quiescent == cp.quiescent)
-      atomicAdd(&stats[ST_FATE_AGREE], 1ull);
+      atomicAdd(&sst[ST_FATE_AGREE], 1u);
   }
   if (use_s) {
-    atomicAdd(&stats[ST_SURR], 1ull);
+    atomicAdd(&sst[ST_SURR], 1u);
 #pragma unroll
     for (int k = 0; k < MARL_M_INT; ++k) I[k] = Ip[k];
 #pragma unroll
@@ -194,7 +204,30 @@ __global__ void __launch_bounds__(kSBlock) k_surr_apply(const int32_t* __restric
   for (int k = 0; k < MARL_M_INT; ++k) c.internal[k] = I[k];
   cells[i] = c;
   events[i] = ev;
-  scatter_deltas(field, nv, n, D);
+  scatter_deltas6(field, v, m, p, D);
+}
+
+// Applies one level: surrogate result where eligible and not near a fate
+// threshold, the plain tick otherwise; shadow cells compare both.
+__global__ void __launch_bounds__(kSBlock) k_surr_apply(const int32_t* __restrict__ order, int count,
+                                                      marl_cell_state* __restrict__ cells,
+                                                      const marl_ruleset* __restrict__ rules,
+                                                      float* __restrict__ field, const float* __restrict__ light,
+                                                      const uint8_t* __restrict__ nbmask,
+                                                      uint8_t* __restrict__ events, CellParams p,
+                                                      const float* __restrict__ out, const uint8_t* __restrict__ elig,
+                                                      const float* __restrict__ norm, int mode, float margin,
+                                                      uint64_t shadow_threshold, unsigned long long* __restrict__ stats) {
+  __shared__ float sI[kSBlock * 17];
+  __shared__ float sE[kSBlock * 13];
+  __shared__ float sD[kSBlock * 13];
+  __shared__ unsigned int sst[ST_COUNT];
+  stats_init(sst);
+  const int t = blockIdx.x * kSBlock + threadIdx.x;
+  if (t < count) apply_cell(order[t], t, cells, rules, field, light, nbmask, events, p, out, elig, norm, mode, margin,
+                            shadow_threshold, sst, sI + threadIdx.x * 17, sE + threadIdx.x * 13,
+                            sD + threadIdx.x * 13);
+  stats_flush(sst, stats);
 }
 
 // ---------------------------------------------------------------------------
@@ -235,6 +268,7 @@ __global__ void __launch_bounds__(256) k_colstats(const int32_t* __restrict__ id
 __global__ void k_gather_batch(const int32_t* __restrict__ idx, int n, uint64_t seed, uint64_t step, int B,
                                const float* __restrict__ F, const float* __restrict__ T,
                                const uint64_t* __restrict__ H, Classes cl, const float* __restrict__ norm,
+                               const float* __restrict__ inv,
                                float* __restrict__ X, float* __restrict__ Yt, int sequential) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
@@ -250,7 +284,7 @@ __global__ void k_gather_batch(const int32_t* __restrict__ idx, int n, uint64_t 
   float f[kFeat];
   for (int k = 0; k < kFeat; ++k) f[k] = F[(size_t)row * kFeat + k];
   class_slots(f, class_of(H[row], cl));
-  for (int k = 0; k < kFeat; ++k) X[(size_t)b * kFeat + k] = (f[k] - norm[k]) / norm[kFeat + k];
+  for (int k = 0; k < kFeat; ++k) X[(size_t)b * kFeat + k] = (f[k] - norm[k]) * inv[k];
   const float* tm = norm + 2 * kFeat;
   const float* ts = tm + kTarg;
   for (int k = 0; k < kTarg; ++k)
@@ -296,6 +330,17 @@ int ensure_level_scratch(marl_ctx* ctx, int64_t n) {
   return MARL_OK;
 }
 
+// The normalisation (feature mean/std, target mean/std) and the feature
+// reciprocals the kernels multiply by.
+int upload_norm(marl_ctx* ctx, SurrogateState* s, const std::vector<float>& nm) {
+  if (!s->inv) MARL_HIP(ctx, hipMalloc(&s->inv, sizeof(float) * kFeat));
+  std::vector<float> inv(kFeat);
+  for (int k = 0; k < kFeat; ++k) inv[k] = 1.0f / nm[kFeat + k];
+  MARL_HIP(ctx, xfer(ctx, s->mlp.norm, nm.data(), sizeof(float) * nm.size(), hipMemcpyHostToDevice));
+  MARL_HIP(ctx, xfer(ctx, s->inv, inv.data(), sizeof(float) * kFeat, hipMemcpyHostToDevice));
+  return MARL_OK;
+}
+
 Classes classes_of(const SurrogateState* s) {
   Classes cl;
   cl.n = s->n_classes;
@@ -314,7 +359,7 @@ __global__ void k_fill_timing_input(size_t n, uint16_t* __restrict__ x) {
 void surrogate_free(SurrogateState* s) {
   if (!s) return;
   mlp_free(s->mlp);
-  void* bufs[] = {s->hfeat, s->htarg, s->hhash, s->hcount, s->featb, s->out, s->elig, s->stats};
+  void* bufs[] = {s->hfeat, s->htarg, s->hhash, s->hcount, s->featb, s->out, s->elig, s->stats, s->inv};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete s;
@@ -340,7 +385,7 @@ int surrogate_level(marl_ctx* ctx, const CellParams& p, const int32_t* order, in
   if (!s->trained) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "surrogate mode needs a trained surrogate");
   if (int rc = ensure_level_scratch(ctx, cnt)) return rc;
   hipLaunchKernelGGL(k_surr_features, dim3(gs(cnt, kSBlock)), dim3(kSBlock), 0, ctx->stream, order, cnt, cells,
-                     rules, field, ctx->light, ctx->nbmask, p, classes_of(s), s->mlp.norm, s->zmax, s->featb,
+                     rules, field, ctx->light, ctx->nbmask, p, classes_of(s), s->mlp.norm, s->inv, s->zmax, s->featb,
                      s->elig, s->stats);
   MARL_HIP(ctx, hipGetLastError());
   if (int rc = mlp_infer_bf16(ctx, s->mlp, cnt, s->featb, s->out)) return rc;
@@ -514,7 +559,7 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
     norm[base + kk] = bin || (k == kFeat + 15) ? 0.0f : (float)mean;
     norm[base + width + kk] = (float)sd;
   }
-  MARL_HIP(ctx, xfer(ctx, s->mlp.norm, norm.data(), sizeof(float) * norm.size(), hipMemcpyHostToDevice));
+  if (int rc = upload_norm(ctx, s, norm)) return rc;
   // initialisation: He-uniform weights from the seed, zero biases (fresh model only)
   const MlpLayout L = mlp_layout();
   if (!s->trained || cfg->reinit) {
@@ -533,7 +578,7 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
   float first = -1.0f, last = 0.0f;
   for (int64_t st = 0; st < cfg->steps; ++st) {
     hipLaunchKernelGGL(k_gather_batch, dim3(gs(B)), dim3(256), 0, ctx->stream, dtr, (int)tr.size(), cfg->seed,
-                       (uint64_t)st, B, s->hfeat, s->htarg, s->hhash, cl, s->mlp.norm, s->mlp.x, s->mlp.t, 0);
+                       (uint64_t)st, B, s->hfeat, s->htarg, s->hhash, cl, s->mlp.norm, s->inv, s->mlp.x, s->mlp.t, 0);
     MARL_HIP(ctx, hipGetLastError());
     if (int rc = mlp_forward_train(ctx, s->mlp, B)) return rc;
     const bool report = st == 0 || st == cfg->steps - 1;
@@ -551,7 +596,7 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
   // validation loss on up to one batch of held-out pairs (in order)
   const int VB = (int)std::min<size_t>(va.size(), (size_t)B);
   hipLaunchKernelGGL(k_gather_batch, dim3(gs(VB)), dim3(256), 0, ctx->stream, dva, (int)va.size(), cfg->seed, 0ull,
-                     VB, s->hfeat, s->htarg, s->hhash, cl, s->mlp.norm, s->mlp.x, s->mlp.t, 1);
+                     VB, s->hfeat, s->htarg, s->hhash, cl, s->mlp.norm, s->inv, s->mlp.x, s->mlp.t, 1);
   MARL_HIP(ctx, hipGetLastError());
   if (int rc = mlp_forward_train(ctx, s->mlp, VB)) return rc;
   float vloss = 0.0f;
@@ -607,7 +652,7 @@ int marl_surrogate_set_params(marl_ctx* ctx, const float* params, size_t n, cons
     if (n_norm != nm.size()) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "norm size mismatch");
     std::copy(norm, norm + n_norm, nm.begin());
   }
-  MARL_HIP(ctx, xfer(ctx, s->mlp.norm, nm.data(), sizeof(float) * nm.size(), hipMemcpyHostToDevice));
+  if (int rc = upload_norm(ctx, s, nm)) return rc;
   s->n_classes = n_classes;
   for (int j = 0; j < n_classes; ++j) s->classes[j] = classes[j];
   if (int rc = mlp_pack(ctx, s->mlp)) return rc;

@@ -245,6 +245,33 @@ int marl_stencil_tile_shape(int32_t tile, int32_t* rows_per_thread, int32_t* z_c
 int marl_copy_probe(marl_ctx* ctx, int32_t n_reps, float* ms_out, double* bytes_out);
 int marl_sync(marl_ctx* ctx);
 
+/* ---- outputs: reductions and packing on the resident state ---------------- */
+
+/* The population figures of DataLogger::log_tick (crates/marl-output/src/data.rs:195-275)
+ * and print_stats (crates/marl-sim/src/stats.rs:9-82): FP64 sums of internal[0]
+ * (energy), [5] and [6] (enzymes a/b), the number of reactions with
+ * |v_max| > 1e-9, and cells with z < Z/3, < 2*(Z/3) and the rest (Z/3 in
+ * integers, stats.rs:39).  The FP64 sums are deterministic but summed in a
+ * different order from the reference's sequential loop. */
+typedef struct {
+  int64_t n_cells;
+  double sum_energy, sum_enzyme_a, sum_enzyme_b;
+  int64_t sum_active_reactions;
+  int64_t z_thirds[3];
+} marl_population_summary;
+/* z_counts (may be NULL): cells per z-layer, nz >= Z. */
+int marl_cell_summary(marl_ctx* ctx, marl_population_summary* out, int64_t* z_counts, size_t nz);
+/* ≙ DataLogger::snapshot_chemistry (data.rs:292-335): per z-layer, the FP64 mean
+ * over the XY plane of each of the 12 species, then of light: out[z*13 + c]. */
+int marl_chemistry_profile(marl_ctx* ctx, double* out, size_t n);
+/* ≙ write_cell_dump's records (crates/marl-output/src/binary_dump.rs:34-41,
+ * crates/marl-format/src/lib.rs:350-364): 25 bytes per cell, packed on the GPU,
+ * {pos: f32[3], lineage_id: u64, starter_type: u8, energy: f32}, little-endian. */
+int marl_download_viewer_cells(marl_ctx* ctx, uint8_t* out, size_t cap_bytes, int64_t* n_out);
+/* Point reads (≙ Field::get / LightField::get at stats.rs:51-56, 86-103):
+ * queries[4k..4k+3] = x, y, z, species (-1 = light). */
+int marl_sample(marl_ctx* ctx, const int32_t* queries, size_t n, float* out);
+
 /* ---- ensembles ------------------------------------------------------------- */
 /* No reference counterpart: the reference runs one column per process
  * (crates/marl-sim/src/lib.rs:160-277).  An ensemble advances independent

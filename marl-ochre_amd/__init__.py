@@ -9,8 +9,9 @@ from ._abi import (CELL_DTYPE, RULESET_DTYPE, BufferMap, DeviceRequest, GpuError
                    load_library)
 from .config import SimulationConfig
 from .engine import GpuFieldDiffuser, MarlEnsemble, MarlGpu
+from . import output
 
 __all__ = [
     "CELL_DTYPE", "RULESET_DTYPE", "GpuError", "NoAdapter", "DeviceRequest", "BufferMap", "InvalidInput",
-    "load_library", "SimulationConfig", "GpuFieldDiffuser", "MarlGpu", "MarlEnsemble",
+    "load_library", "SimulationConfig", "GpuFieldDiffuser", "MarlGpu", "MarlEnsemble", "output",
 ]

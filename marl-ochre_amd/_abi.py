@@ -123,6 +123,17 @@ class marl_surr_stats(ctypes.Structure):
     ]
 
 
+class marl_population_summary(ctypes.Structure):
+    _fields_ = [
+        ("n_cells", ctypes.c_int64),
+        ("sum_energy", ctypes.c_double),
+        ("sum_enzyme_a", ctypes.c_double),
+        ("sum_enzyme_b", ctypes.c_double),
+        ("sum_active_reactions", ctypes.c_int64),
+        ("z_thirds", ctypes.c_int64 * 3),
+    ]
+
+
 # surrogate network sizes (80 features -> 3 x 256 SiLU -> 28 targets)
 SURR_FEAT, SURR_HIDDEN, SURR_TARG = 80, 256, 28
 
@@ -229,6 +240,10 @@ SIGNATURES = {
     "marl_seed_positions": (ctypes.c_int, [ctypes.POINTER(marl_dims), ctypes.c_uint64, ctypes.c_uint64, _I64, _I32,
                                            _I32, _I32, _P, _P, _P, _I64, ctypes.POINTER(_I64)]),
     "marl_sync": (ctypes.c_int, [_P]),
+    "marl_cell_summary": (ctypes.c_int, [_P, ctypes.POINTER(marl_population_summary), _P, _SZ]),
+    "marl_chemistry_profile": (ctypes.c_int, [_P, _P, _SZ]),
+    "marl_download_viewer_cells": (ctypes.c_int, [_P, _P, _SZ, ctypes.POINTER(_I64)]),
+    "marl_sample": (ctypes.c_int, [_P, _P, _SZ, _P]),
     "marl_ensemble_create": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_P)]),
     "marl_ensemble_destroy": (None, [_P]),
     "marl_ensemble_last_error": (ctypes.c_char_p, [_P]),

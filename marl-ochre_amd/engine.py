@@ -214,6 +214,37 @@ class MarlGpu:
         self._chk(self.lib.marl_mass_budget(self.h, ptr(out), out.size))
         return out
 
+    # -- outputs ------------------------------------------------------------------
+    def cell_summary(self) -> tuple[dict, np.ndarray]:
+        """Population sums (FP64) and z-thirds, plus cells per z-layer."""
+        st = _abi.marl_population_summary()
+        zc = np.zeros(self.dims[2], np.int64)
+        self._chk(self.lib.marl_cell_summary(self.h, ctypes.byref(st), ptr(zc), zc.size))
+        d = {k: getattr(st, k) for k, _ in st._fields_ if k != "z_thirds"}
+        d["z_thirds"] = list(st.z_thirds)
+        return d, zc
+
+    def chemistry_profile(self) -> np.ndarray:
+        """[Z, 13]: per z-layer XY means of the 12 species, then of light (FP64)."""
+        out = np.zeros((self.dims[2], S_EXT + 1), np.float64)
+        self._chk(self.lib.marl_chemistry_profile(self.h, ptr(out), out.size))
+        return out
+
+    def download_viewer_cells(self) -> bytes:
+        """The 25-byte packed viewer records of every cell, in cell order."""
+        n = self.cell_count()
+        buf = np.zeros(max(n, 1) * 25, np.uint8)
+        got = ctypes.c_int64()
+        self._chk(self.lib.marl_download_viewer_cells(self.h, ptr(buf), buf.size, ctypes.byref(got)))
+        return buf[: got.value * 25].tobytes()
+
+    def sample(self, queries) -> np.ndarray:
+        """Point reads: rows (x, y, z, species), species -1 = light."""
+        q = np.ascontiguousarray(np.asarray(queries, np.int32).reshape(-1, 4))
+        out = np.zeros(len(q), np.float32)
+        self._chk(self.lib.marl_sample(self.h, ptr(q), len(q), ptr(out)))
+        return out
+
     # -- learned kinetics surrogate --------------------------------------------
     def set_kinetics_mode(self, mode: int) -> None:
         self._chk(self.lib.marl_set_kinetics_mode(self.h, int(mode)))

@@ -446,7 +446,8 @@ def main():
         g.tile = args.tile
     tile = g.tile if (args.no_autotune or args.tile >= 0) else g.autotune(sim)
     ry, zc, pd = g.tile_shape(tile)
-    log(f"stencil tile {tile}: rows/thread={ry} z-chunk={zc} prefetch={pd}")
+    sg, nw = g.tile_split(tile)
+    log(f"stencil tile {tile}: rows/thread={ry} z-chunk={zc} prefetch={pd} species/thread={sg} waves/block={nw}")
 
     copy_ms, copy_bytes = 0.0, 0.0
     import ctypes

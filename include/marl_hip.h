@@ -239,6 +239,10 @@ int marl_stencil_tile_count(void);
 int marl_stencil_tile_substeps(int32_t tile);
 /* Rows per thread, z-chunk and prefetch distance (planes) of a tile id. */
 int marl_stencil_tile_shape(int32_t tile, int32_t* rows_per_thread, int32_t* z_chunk, int32_t* prefetch);
+/* Species per thread and waves per block of a tile id: a block of W waves
+ * holds W/4 groups of 4 waves sharing one LDS plane, each taking that many
+ * species. */
+int marl_stencil_tile_split(int32_t tile, int32_t* species_per_thread, int32_t* waves_per_block);
 /* Plain copy of the resident field into the scratch buffer (n_reps times,
  * float4 loads/stores): the HBM calibration kernel for roofline and counter
  * scaling.  *bytes_out = bytes read + written per rep. */

@@ -314,6 +314,13 @@ int marl_stencil_tile_shape(int32_t tile, int32_t* ry, int32_t* zc, int32_t* pd)
   return MARL_OK;
 }
 
+int marl_stencil_tile_split(int32_t tile, int32_t* sg, int32_t* nw) {
+  if (tile < 0 || tile >= kNumTiles) return MARL_ERR_INVALID_INPUT;
+  if (sg) *sg = kTiles[tile].sg;
+  if (nw) *nw = kTiles[tile].nw;
+  return MARL_OK;
+}
+
 int marl_set_stencil_tile(marl_ctx* ctx, int32_t tile) {
   if (!ctx) return MARL_ERR_INVALID_INPUT;
   if (tile < 0 || tile >= kNumTiles) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "tile id out of range");

@@ -196,8 +196,8 @@ const char* marl_ensemble_last_error(const marl_ensemble* e) { return e ? e->err
 int marl_ensemble_set_tile(marl_ensemble* e, int32_t tile) {
   if (!e) return MARL_ERR_INVALID_INPUT;
   if (tile < 0 || tile >= marl_stencil_tile_count()) return efail(e, MARL_ERR_INVALID_INPUT, "unknown tile id");
-  if (marl_stencil_tile_substeps(tile) != 1)
-    return efail(e, MARL_ERR_INVALID_INPUT, "ensembles use single-substep tiles");
+  if (!marl_tile_ensemble_ok(tile))
+    return efail(e, MARL_ERR_INVALID_INPUT, "ensembles use single-substep, single-group tiles");
   if (tile != e->tile) drop(e);
   e->tile = tile;
   return MARL_OK;
@@ -269,7 +269,7 @@ int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t
   int best = e->tile;
   float best_ms = 1e30f;
   for (int t = 0; t < marl_stencil_tile_count(); ++t) {
-    if (marl_stencil_tile_substeps(t) != 1) continue;
+    if (!marl_tile_ensemble_ok(t)) continue;
     e->tile = t;
     float ms = 0.0f;
     if (int rc = marl_ensemble_time_field(e, sim, 5, &ms)) return rc;

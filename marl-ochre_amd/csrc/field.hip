@@ -25,11 +25,15 @@
 // The per-voxel arithmetic is the reference's, in its order and without
 // contraction, so the result is bit-identical to the f32 CPU statement.
 
-// {rows per thread, z-chunk, prefetch distance in planes, species per block,
+// {rows per thread, z-chunk, prefetch distance in planes, species per thread,
 //  substeps per pass, waves per block}
 const TileShape kTiles[] = {
     {1, 32, 1, 12, 1, 4}, {2, 16, 1, 12, 1, 4}, {2, 32, 1, 12, 1, 4}, {2, 16, 1, 6, 1, 4}, {2, 32, 1, 6, 1, 4},
     {2, 64, 1, 6, 1, 4},  {2, 32, 2, 6, 1, 4},  {4, 32, 1, 6, 1, 4},  {4, 64, 1, 6, 1, 4},
+    // split blocks: 4-wave groups of one block share the tile's LDS plane,
+    // each group taking 6 (or 4) of the 12 species
+    {2, 16, 1, 6, 1, 8}, {2, 32, 1, 6, 1, 8}, {1, 32, 1, 4, 1, 12}, {2, 16, 1, 4, 1, 12}, {2, 32, 1, 4, 1, 12},
+    {1, 32, 1, 6, 1, 4}, {1, 32, 1, 6, 1, 8}, {1, 64, 1, 4, 1, 12}, {1, 32, 2, 4, 1, 12}, {1, 16, 1, 4, 1, 12},
     // temporal blocking (K1b): two substeps per launch
     {1, 16, 1, 12, 2, 8}, {1, 16, 1, 6, 2, 8},
 };
@@ -78,31 +82,37 @@ __device__ __forceinline__ float ld_field(rsrc_t r, const StencilParams& p, int 
 
 // LDS of one tile: species planes with a one-voxel halo (one spare row takes
 // the writes of lanes that own no halo value, so those writes need no branch)
-// and the mask words of the tile's rows.
-template <int RY, int SG>
+// and the mask words of the tile's rows.  SB = species held by the block.
+template <int RY, int SB>
 struct StencilLds {
-  float f[SG][4 * RY + 3][66];
+  float f[SB][4 * RY + 3][66];
   uint32_t m[(4 * RY + 2) * 4 + 256];
 };
 
 
-// One tile (tx, ty, tz) of species group grp of one substep.
-template <int RY, bool SRC, int PD, int SG>
+// One tile (tx, ty, tz) of block species group grp of one substep.  The
+// block has HB groups of 4 waves; each takes SG species of the block's HB*SG,
+// and all of them share the LDS plane (so a group without species 7 reads the
+// niche factor's species 7 from LDS when the block holds it).
+template <int RY, bool SRC, int PD, int SG, int HB = 1>
 __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, float* __restrict__ dst,
                                              const uint32_t* __restrict__ occ, int X, int Y, int Z, int XW, int zc,
                                              int tx, int ty, int tz, int grp, const StencilParams& p,
-                                             StencilLds<RY, SG>& sh) {
+                                             StencilLds<RY, SG * HB>& sh) {
   constexpr int TY = 4 * RY;
-  constexpr int S = SG;              // species handled by this block
-  constexpr int QN = PD + 2;  // own planes held: z-1, z, z+1 .. z+PD
-  constexpr int HN = PD + 1;  // halo planes held: z .. z+PD
+  constexpr int S = SG;              // species handled by this thread
   constexpr int LW = 66;
   auto& lds = sh.f;
   auto& mlds = sh.m;
-  const int s0 = grp * SG;          // first species of this block
+  // wave group within the block (wave-uniform: kept in a scalar register)
+  const int half = HB == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 8));
+  const int bs0 = grp * SG * HB;    // first species of this block
+  const int s0 = bs0 + half * SG;   // first species of this wave group
+  const int ls0 = half * SG;        // its first LDS species slot
   const bool has7 = s0 <= 7 && 7 < s0 + SG;
+  const bool blk7 = bs0 <= 7 && 7 < bs0 + SG * HB;  // species 7 is in the block's LDS
 
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x & 255;
   const int lx = tid & 63, wy = tid >> 6;
   const int x0 = tx * 64, y0 = ty * TY, z0 = tz * zc;
   const int z1 = min(z0 + zc, Z);
@@ -159,11 +169,17 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
   const uint32_t mvo = mvalid ? (uint32_t)myy * XWb + (uint32_t)mwx * 4u : 0u;
   const int mslot = mw ? tid : (TY + 2) * 4 + tid;
 
-  float q[QN][RY][S];   // q[0] = z-1, q[1] = z, q[2] = z+1, ...
-  uint32_t qm[QN][RY];
-  float h[HN][S];       // h[0] = halo of plane z, ...
-  uint32_t hm[HN];      // this lane's mask word of plane z, ...
-  float q7[HN][RY];     // species 7 of planes z .. z+PD for the niche factor (groups without it)
+  // Register ring: plane pl of the own rows lives in slot (pl - z0 + 1) mod R,
+  // holding planes z-1 .. z+PD+1 at step z.  The z loop is unrolled by R, so
+  // every slot index is a compile-time constant: nothing is copied between
+  // steps and a prefetch is waited for only where it is first used, one step
+  // later, instead of at the end of the step that issued it.
+  constexpr int R = PD + 3;
+  float q[R][RY][S];
+  uint32_t qm[R][RY];
+  float h[R][S];    // halo of plane pl (slots of planes z .. z+PD+1)
+  uint32_t hm[R];   // this lane's mask word of plane pl
+  float q7[R][RY];  // species 7 of plane pl for the niche factor (blocks without it)
 
   auto load_own = [&](int k, int zz) {
 #pragma unroll
@@ -177,167 +193,110 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
 #pragma unroll
     for (int s = 0; s < S; ++s) h[k][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, hvo, zz);
     hm[k] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zz * MZb), 0);
-  };
-  auto load_s7 = [&](int k, int zz) {
-    if (SG < MARL_S_EXT && !has7) {
+    if (SG < MARL_S_EXT && !blk7) {
 #pragma unroll
       for (int r = 0; r < RY; ++r) q7[k][r] = ld_field<SRC>(rs, p, 7, Pb, XYb, Z, vo[r], zz);
     }
   };
   // prologue: planes z0-1 .. z0+PD (own) and z0 .. z0+PD (halo), clamped into the grid
 #pragma unroll
-  for (int k = 0; k < QN; ++k) {
+  for (int k = 0; k < PD + 2; ++k) {
     const int zz = z0 - 1 + k;
     load_own(k, zz < 0 ? 0 : (zz < Z ? zz : Z - 1));
-  }
-#pragma unroll
-  for (int k = 0; k < HN; ++k) {
-    const int zz = z0 + k;
-    load_halo(k, zz < Z ? zz : Z - 1);
-    load_s7(k, zz < Z ? zz : Z - 1);
+    if (k > 0) load_halo(k, zz < Z ? zz : Z - 1);
   }
 
-  for (int z = z0; z < z1; ++z) {
-    // prefetch plane z+PD+1: own rows up to z1 (the last zp neighbour),
-    // halo up to z1-1; the condition is uniform across the block
-    const int zf = z + PD + 1;
-    float qn[RY][S];
-    uint32_t qmn[RY];
-    float hn[S];
-    float q7n[RY];
-    uint32_t hmn = 0u;
-    if (zf <= z1 && zf < Z) {
+  for (int zb = z0; zb < z1; zb += R) {
 #pragma unroll
-      for (int r = 0; r < RY; ++r) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) qn[r][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, vo[r], zf);
-        qmn[r] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mo[r], (int)((uint32_t)zf * MZb), 0);
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < RY; ++r) {
-        qmn[r] = 0u;
-#pragma unroll
-        for (int s = 0; s < S; ++s) qn[r][s] = 0.0f;
-      }
-    }
-    if (zf < z1) {
-#pragma unroll
-      for (int s = 0; s < S; ++s) hn[s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, hvo, zf);
-      hmn = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zf * MZb), 0);
+    for (int u = 0; u < R; ++u) {
+      const int z = zb + u;
+      if (z >= z1) break;
+      const int km = u % R, kc = (u + 1) % R, kp = (u + 2) % R, kf = (u + PD + 2) % R;
+      // prefetch plane z+PD+1 without a branch: past the chunk the own rows
+      // re-read plane z1 and the halo plane z1-1 (both already fetched), so
+      // the clamp costs no HBM traffic
+      const int zf = z + PD + 1;
+      const int zo_f = min(min(zf, z1), Z - 1);
+      const int zh_f = min(min(zf, z1 - 1), Z - 1);
+      load_own(kf, zo_f);
+      load_halo(kf, zh_f);
+      // plane z into LDS (own values and halo values loaded PD+1 steps ago)
 #pragma unroll
       for (int r = 0; r < RY; ++r)
-        q7n[r] = (SG < MARL_S_EXT && !has7) ? ld_field<SRC>(rs, p, 7, Pb, XYb, Z, vo[r], zf) : 0.0f;
-    } else {
 #pragma unroll
-      for (int s = 0; s < S; ++s) hn[s] = 0.0f;
+        for (int s = 0; s < S; ++s) lds[ls0 + s][wy + 4 * r + 1][lx + 1] = q[kc][r][s];
 #pragma unroll
-      for (int r = 0; r < RY; ++r) q7n[r] = 0.0f;
-    }
-    // plane z into LDS (own values, halo values loaded PD+1 iterations ago)
-#pragma unroll
-    for (int r = 0; r < RY; ++r)
-#pragma unroll
-      for (int s = 0; s < S; ++s) lds[s][wy + 4 * r + 1][lx + 1] = q[1][r][s];
-#pragma unroll
-    for (int s = 0; s < S; ++s) lds[s][hrow][hcol] = h[0][s];
-    mlds[mslot] = mvalid ? hm[0] : 0u;
-    __syncthreads();
+      for (int s = 0; s < S; ++s) lds[ls0 + s][hrow][hcol] = h[kc][s];
+      if (half == 0) mlds[mslot] = mvalid ? hm[kc] : 0u;
+      __syncthreads();
 
 #pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      const int row = wy + 4 * r + 1;
-      const int y = yr[r];
-      auto bitat = [&](int rw, int c) -> uint32_t {
-        const int i = c + 32;
-        return (mlds[rw * 4 + (i >> 5)] >> (i & 31)) & 1u;
-      };
-      const uint32_t zo = (uint32_t)z * XYb;
-      // neighbour flags with bitwise operators: every mask read is unconditional
-      const uint32_t bc = bitat(row, lx), bxm = bitat(row, lx - 1), bxp = bitat(row, lx + 1);
-      const uint32_t bym = bitat(row - 1, lx), byp = bitat(row + 1, lx);
-      const uint32_t bzm = (qm[0][r] >> (x & 31)) & 1u, bzp = (qm[2][r] >> (x & 31)) & 1u;
-      const bool vxm = (uint32_t)(x > 0) & (bxm ^ 1u);
-      const bool vxp = (uint32_t)(x < X - 1) & (bxp ^ 1u);
-      const bool vym = (uint32_t)(y > 0) & (bym ^ 1u);
-      const bool vyp = (uint32_t)(y < Y - 1) & (byp ^ 1u);
-      const bool vzm = (uint32_t)(z > 0) & (bzm ^ 1u);
-      const bool vzp = (uint32_t)(z < Z - 1) & (bzp ^ 1u);
-      const float structural = (SG == MARL_S_EXT || has7) ? q[1][r][7 - (SG == MARL_S_EXT ? 0 : MARL_S_EXT - SG)] : q7[0][r];
-      const float niche = 1.0f - p.alpha_eps * structural / (p.k_eps + structural);
-      const uint32_t keep = 0u - bc;  // occupied centre: copied unchanged
-      float out[S];
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        const float c = q[1][r][s];
-        const float dd = p.d_voxel[s0 + s] * niche;
-        const float lxm = lds[s][row][lx];
-        const float lxp = lds[s][row][lx + 2];
-        const float lym = lds[s][row - 1][lx + 1];
-        const float lyp = lds[s][row + 1][lx + 1];
-        const float xm = vxm ? lxm : c;
-        const float xp = vxp ? lxp : c;
-        const float ym = vym ? lym : c;
-        const float yp = vyp ? lyp : c;
-        const float zm = vzm ? q[0][r][s] : c;
-        const float zp = vzp ? q[2][r][s] : c;
-        const float lap = xm + xp + ym + yp + zm + zp - 6.0f * c;
-        const float decay = p.lambda_decay[s0 + s] * c;
-        const float nc = c + p.dt_sub * (dd * lap - decay);
-        // a bit select keeps the compiler from branching around the reads
-        out[s] = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, c) & keep) |
-                                               (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
-      }
-      if (valid[r]) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) bstore_nt(rd, out[s], vo[r], (uint32_t)(s0 + s) * Pb + zo);
-      }
-    }
-    __syncthreads();
-    // rotate the queues
-#pragma unroll
-    for (int k = 0; k + 1 < QN; ++k)
-#pragma unroll
       for (int r = 0; r < RY; ++r) {
-        qm[k][r] = qm[k + 1][r];
+        const int row = wy + 4 * r + 1;
+        const int y = yr[r];
+        auto bitat = [&](int rw, int c) -> uint32_t {
+          const int i = c + 32;
+          return (mlds[rw * 4 + (i >> 5)] >> (i & 31)) & 1u;
+        };
+        const uint32_t zo = (uint32_t)z * XYb;
+        // neighbour flags with bitwise operators: every mask read is unconditional
+        const uint32_t bc = bitat(row, lx), bxm = bitat(row, lx - 1), bxp = bitat(row, lx + 1);
+        const uint32_t bym = bitat(row - 1, lx), byp = bitat(row + 1, lx);
+        const uint32_t bzm = (qm[km][r] >> (x & 31)) & 1u, bzp = (qm[kp][r] >> (x & 31)) & 1u;
+        const bool vxm = (uint32_t)(x > 0) & (bxm ^ 1u);
+        const bool vxp = (uint32_t)(x < X - 1) & (bxp ^ 1u);
+        const bool vym = (uint32_t)(y > 0) & (bym ^ 1u);
+        const bool vyp = (uint32_t)(y < Y - 1) & (byp ^ 1u);
+        const bool vzm = (uint32_t)(z > 0) & (bzm ^ 1u);
+        const bool vzp = (uint32_t)(z < Z - 1) & (bzp ^ 1u);
+        // s0 is a multiple of SG, so species 7 sits at register slot 7 % SG
+        const float structural = has7 ? q[kc][r][7 % SG] : (blk7 ? lds[7 - bs0][row][lx + 1] : q7[kc][r]);
+        const float niche = 1.0f - p.alpha_eps * structural / (p.k_eps + structural);
+        const uint32_t keep = 0u - bc;  // occupied centre: copied unchanged
+        float out[S];
 #pragma unroll
-        for (int s = 0; s < S; ++s) q[k][r][s] = q[k + 1][r][s];
+        for (int s = 0; s < S; ++s) {
+          const float c = q[kc][r][s];
+          const float dd = p.d_voxel[s0 + s] * niche;
+          const float lxm = lds[ls0 + s][row][lx];
+          const float lxp = lds[ls0 + s][row][lx + 2];
+          const float lym = lds[ls0 + s][row - 1][lx + 1];
+          const float lyp = lds[ls0 + s][row + 1][lx + 1];
+          const float xm = vxm ? lxm : c;
+          const float xp = vxp ? lxp : c;
+          const float ym = vym ? lym : c;
+          const float yp = vyp ? lyp : c;
+          const float zm = vzm ? q[km][r][s] : c;
+          const float zp = vzp ? q[kp][r][s] : c;
+          const float lap = xm + xp + ym + yp + zm + zp - 6.0f * c;
+          const float decay = p.lambda_decay[s0 + s] * c;
+          const float nc = c + p.dt_sub * (dd * lap - decay);
+          // a bit select keeps the compiler from branching around the reads
+          out[s] = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, c) & keep) |
+                                                 (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
+        }
+        if (valid[r]) {
+#pragma unroll
+          for (int s = 0; s < S; ++s) bstore_nt(rd, out[s], vo[r], (uint32_t)(s0 + s) * Pb + zo);
+        }
       }
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      qm[QN - 1][r] = qmn[r];
-#pragma unroll
-      for (int s = 0; s < S; ++s) q[QN - 1][r][s] = qn[r][s];
+      __syncthreads();
     }
-#pragma unroll
-    for (int k = 0; k + 1 < HN; ++k) {
-      hm[k] = hm[k + 1];
-#pragma unroll
-      for (int s = 0; s < S; ++s) h[k][s] = h[k + 1][s];
-    }
-    hm[HN - 1] = hmn;
-#pragma unroll
-    for (int s = 0; s < S; ++s) h[HN - 1][s] = hn[s];
-#pragma unroll
-    for (int k = 0; k + 1 < HN; ++k)
-#pragma unroll
-      for (int r = 0; r < RY; ++r) q7[k][r] = q7[k + 1][r];
-#pragma unroll
-    for (int r = 0; r < RY; ++r) q7[HN - 1][r] = q7n[r];
   }
 }
 
-template <int RY, bool SRC, int PD, int SG>
-__global__ void __launch_bounds__(256) k_stencil(const float* __restrict__ src, float* __restrict__ dst,
-                                                 const uint32_t* __restrict__ occ, int X, int Y, int Z, int XW,
-                                                 int zc, int ntx, int nty, int ntz, StencilParams p) {
-  __shared__ StencilLds<RY, SG> sh;
-  constexpr int NG = MARL_S_EXT / SG;
+template <int RY, bool SRC, int PD, int SG, int HB>
+__global__ void __launch_bounds__(256 * HB) k_stencil(const float* __restrict__ src, float* __restrict__ dst,
+                                                      const uint32_t* __restrict__ occ, int X, int Y, int Z, int XW,
+                                                      int zc, int ntx, int nty, int ntz, StencilParams p) {
+  __shared__ StencilLds<RY, SG * HB> sh;
+  constexpr int NG = MARL_S_EXT / (SG * HB);
   int b = xcd_remap(blockIdx.x, ntx * nty * ntz * NG);
   // species group fastest: the groups of one tile run side by side on one XCD
   const int grp = b % NG;
   b /= NG;
-  stencil_tile<RY, SRC, PD, SG>(src, dst, occ, X, Y, Z, XW, zc, b % ntx, (b / ntx) % nty, b / (ntx * nty), grp, p, sh);
+  stencil_tile<RY, SRC, PD, SG, HB>(src, dst, occ, X, Y, Z, XW, zc, b % ntx, (b / ntx) % nty, b / (ntx * nty), grp,
+                                    p, sh);
 }
 
 // Ensembles: one persistent launch covers every column of the ensemble.  The
@@ -393,8 +352,6 @@ __global__ void __launch_bounds__(64 * NW * (MARL_S_EXT / SG))
   constexpr int TY = NW * RY;
   constexpr int NT = 64 * NW * (MARL_S_EXT / SG);
   constexpr int LW = 66;
-  constexpr int QN = PD + 2;  // input planes held: z-1, z, z+1 .. z+PD
-  constexpr int HN = PD + 1;  // halo planes held: z .. z+PD
   static_assert(TY <= 64 && (TY + 2) * 4 <= NT, "tile rows");
   __shared__ Stencil2Lds<TY> sh;
   auto& lds = sh.in;
@@ -403,7 +360,8 @@ __global__ void __launch_bounds__(64 * NW * (MARL_S_EXT / SG))
   int b = xcd_remap(blockIdx.x, ntx * nty * ntz);
   const int tx = b % ntx, ty = (b / ntx) % nty, tz = b / (ntx * nty);
   const int tid = threadIdx.x;
-  const int grp = tid / (64 * NW), gt = tid % (64 * NW);
+  // species group of this wave (wave-uniform: kept in a scalar register)
+  const int grp = __builtin_amdgcn_readfirstlane(tid / (64 * NW)), gt = tid % (64 * NW);
   const int s0 = grp * SG;
   const int lx = gt & 63, wy = gt >> 6;
   const int x0 = tx * 62 - 1, y0 = ty * (TY - 2) - 1;  // origin of the first-substep tile
@@ -470,11 +428,17 @@ __global__ void __launch_bounds__(64 * NW * (MARL_S_EXT / SG))
     return (sh.m[mb][rw * 4 + ((xa >> 5) - (wx0 - 1))] >> (xa & 31)) & 1u;
   };
 
-  float q[QN][RY][S];
-  uint32_t qm[QN][RY];
-  float h[HN][S];
-  uint32_t hm[HN];
-  float p1[2][RY][S];   // intermediate planes z-2, z-1 of this thread's column
+  // Register rings as in K1: input plane pl lives in slot (pl - zs + 1) mod R
+  // (planes z-1 .. z+PD+1 at step z), intermediate plane pl in the same slot
+  // of p1 (planes z-2, z-1, z); the z loop is unrolled by R so no register
+  // holding a prefetch in flight is ever copied.
+  constexpr int R = PD + 3;
+  static_assert(R >= 3, "intermediate ring");
+  float q[R][RY][S];
+  uint32_t qm[R][RY];
+  float h[R][S];
+  uint32_t hm[R];
+  float p1[2][RY][S];   // intermediate planes z-2, z-1 of this thread's column (computed, so copied freely)
   uint32_t qmz2[RY];    // mask word of plane z-2 of this thread's column
 
   auto load_own = [&](int k, int zz) {
@@ -491,214 +455,173 @@ __global__ void __launch_bounds__(64 * NW * (MARL_S_EXT / SG))
     hm[k] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zz * MZb), 0);
   };
 #pragma unroll
-  for (int k = 0; k < QN; ++k) {
+  for (int k = 0; k < PD + 2; ++k) {
     const int zz = zs - 1 + k;
     load_own(k, zz < 0 ? 0 : (zz < Z ? zz : Z - 1));
+    if (k > 0) load_halo(k, zz < Z ? zz : Z - 1);
   }
 #pragma unroll
-  for (int k = 0; k < HN; ++k) {
-    const int zz = zs + k;
-    load_halo(k, zz < Z ? zz : Z - 1);
-  }
+  for (int k = 0; k < 2; ++k)
 #pragma unroll
-  for (int r = 0; r < RY; ++r) {
-    qmz2[r] = 0u;
+    for (int r = 0; r < RY; ++r)
 #pragma unroll
-    for (int s = 0; s < S; ++s) p1[0][r][s] = p1[1][r][s] = 0.0f;
-  }
+      for (int s = 0; s < S; ++s) p1[k][r][s] = 0.0f;
+#pragma unroll
+  for (int r = 0; r < RY; ++r) qmz2[r] = 0u;
 
-  for (int z = zs; z <= z1; ++z) {
-    const bool first = z <= ze;  // this iteration computes intermediate plane z
-    // prefetch input plane z+PD+1: own rows up to ze+1, halo up to ze
-    const int zf = z + PD + 1;
-    float qn[RY][S];
-    uint32_t qmn[RY];
-    float hn[S];
-    uint32_t hmn = 0u;
-    if (zf <= ze + 1 && zf < Z) {
+  for (int zb = zs; zb <= z1; zb += R) {
 #pragma unroll
-      for (int r = 0; r < RY; ++r) {
+    for (int u = 0; u < R; ++u) {
+      const int z = zb + u;
+      if (z > z1) break;
+      const int km = u % R, kc = (u + 1) % R, kp = (u + 2) % R, kf = (u + PD + 2) % R;
+      const bool first = z <= ze;       // this step computes intermediate plane z
+      // prefetch input plane z+PD+1 without a branch: own rows are needed up
+      // to ze+1 and the halo up to ze; past that the loads repeat a plane
+      // already fetched
+      const int zf = z + PD + 1;
+      load_own(kf, min(min(zf, ze + 1), Z - 1));
+      load_halo(kf, min(min(zf, ze), Z - 1));
+      if (first) {
 #pragma unroll
-        for (int s = 0; s < S; ++s) qn[r][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, vo[r], zf);
-        qmn[r] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mo[r], (int)((uint32_t)zf * MZb), 0);
+        for (int r = 0; r < RY; ++r)
+#pragma unroll
+          for (int s = 0; s < S; ++s) lds[s0 + s][wy + NW * r + 1][lx + 1] = q[kc][r][s];
+#pragma unroll
+        for (int s = 0; s < S; ++s) lds[s0 + s][hrow][hcol] = h[kc][s];
+        sh.m[z & 1][mslot] = mvalid ? hm[kc] : 0u;
       }
-    } else {
-#pragma unroll
-      for (int r = 0; r < RY; ++r) {
-        qmn[r] = 0u;
-#pragma unroll
-        for (int s = 0; s < S; ++s) qn[r][s] = 0.0f;
-      }
-    }
-    if (zf <= ze) {
-#pragma unroll
-      for (int s = 0; s < S; ++s) hn[s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, hvo, zf);
-      hmn = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zf * MZb), 0);
-    } else {
-#pragma unroll
-      for (int s = 0; s < S; ++s) hn[s] = 0.0f;
-    }
-    if (first) {
-#pragma unroll
-      for (int r = 0; r < RY; ++r)
-#pragma unroll
-        for (int s = 0; s < S; ++s) lds[s0 + s][wy + NW * r + 1][lx + 1] = q[1][r][s];
-#pragma unroll
-      for (int s = 0; s < S; ++s) lds[s0 + s][hrow][hcol] = h[0][s];
-      sh.m[z & 1][mslot] = mvalid ? hm[0] : 0u;
-    }
-    __syncthreads();
+      __syncthreads();
 
-    // first substep on plane z (registers only; mid still holds plane z-1)
-    float p1n[RY][S];
-    if (first) {
+      // first substep on plane z (registers only; mid still holds plane z-1)
+      float p1n[RY][S];
+      if (first) {
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+          const int row = wy + NW * r + 1;
+          const int y = yr[r];
+          const uint32_t bc = bitat(z & 1, row, x), bxm = bitat(z & 1, row, x - 1), bxp = bitat(z & 1, row, x + 1);
+          const uint32_t bym = bitat(z & 1, row - 1, x), byp = bitat(z & 1, row + 1, x);
+          const uint32_t bzm = (qm[km][r] >> (x & 31)) & 1u, bzp = (qm[kp][r] >> (x & 31)) & 1u;
+          const bool vxm = (uint32_t)(x > 0) & (bxm ^ 1u);
+          const bool vxp = (uint32_t)(x < X - 1) & (bxp ^ 1u);
+          const bool vym = (uint32_t)(y > 0) & (bym ^ 1u);
+          const bool vyp = (uint32_t)(y < Y - 1) & (byp ^ 1u);
+          const bool vzm = (uint32_t)(z > 0) & (bzm ^ 1u);
+          const bool vzp = (uint32_t)(z < Z - 1) & (bzp ^ 1u);
+          const float structural = lds[7][row][lx + 1];
+          const float niche = 1.0f - p.alpha_eps * structural / (p.k_eps + structural);
+          const uint32_t keep = 0u - bc;
+#pragma unroll
+          for (int s = 0; s < S; ++s) {
+            const float c = q[kc][r][s];
+            const float dd = p.d_voxel[s0 + s] * niche;
+            const float xm = vxm ? lds[s0 + s][row][lx] : c;
+            const float xp = vxp ? lds[s0 + s][row][lx + 2] : c;
+            const float ym = vym ? lds[s0 + s][row - 1][lx + 1] : c;
+            const float yp = vyp ? lds[s0 + s][row + 1][lx + 1] : c;
+            const float zm = vzm ? q[km][r][s] : c;
+            const float zp = vzp ? q[kp][r][s] : c;
+            const float lap = xm + xp + ym + yp + zm + zp - 6.0f * c;
+            const float decay = p.lambda_decay[s0 + s] * c;
+            const float nc = c + p.dt_sub * (dd * lap - decay);
+            p1n[r][s] = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, c) & keep) |
+                                                      (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < RY; ++r)
+#pragma unroll
+          for (int s = 0; s < S; ++s) p1n[r][s] = 0.0f;
+      }
+
+      // second substep on plane zo = z-1 from the intermediate planes
+      const int zo = z - 1;
+      if (zo >= z0) {
+        const int mb = zo & 1;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+          const int mrw = wy + NW * r;         // row in mid
+          const int row = mrw + 1;             // row in the mask LDS
+          const int y = yr[r];
+          const uint32_t bc = bitat(mb, row, x), bxm = bitat(mb, row, x - 1), bxp = bitat(mb, row, x + 1);
+          const uint32_t bym = bitat(mb, row - 1, x), byp = bitat(mb, row + 1, x);
+          const uint32_t bzm = (qmz2[r] >> (x & 31)) & 1u, bzp = (qm[kc][r] >> (x & 31)) & 1u;
+          const bool vxm = (uint32_t)(x > 0) & (bxm ^ 1u);
+          const bool vxp = (uint32_t)(x < X - 1) & (bxp ^ 1u);
+          const bool vym = (uint32_t)(y > 0) & (bym ^ 1u);
+          const bool vyp = (uint32_t)(y < Y - 1) & (byp ^ 1u);
+          const bool vzm = (uint32_t)(zo > 0) & (bzm ^ 1u);
+          const bool vzp = (uint32_t)(zo < Z - 1) & (bzp ^ 1u);
+          // edge lanes and rows (no output) read clamped slots
+          const int cxm = lx > 0 ? lx - 1 : 0, cxp = lx < 63 ? lx + 1 : 63;
+          const int rym = mrw > 0 ? mrw - 1 : 0, ryp = mrw < TY - 1 ? mrw + 1 : TY - 1;
+          const float structural = mid[7][mrw][lx];
+          const float niche = 1.0f - p.alpha_eps * structural / (p.k_eps + structural);
+          const uint32_t keep = 0u - bc;
+          float out[S];
+#pragma unroll
+          for (int s = 0; s < S; ++s) {
+            const float c = p1[1][r][s];
+            const float dd = p.d_voxel[s0 + s] * niche;
+            const float xm = vxm ? mid[s0 + s][mrw][cxm] : c;
+            const float xp = vxp ? mid[s0 + s][mrw][cxp] : c;
+            const float ym = vym ? mid[s0 + s][rym][lx] : c;
+            const float yp = vyp ? mid[s0 + s][ryp][lx] : c;
+            const float zm = vzm ? p1[0][r][s] : c;
+            const float zp = vzp ? p1n[r][s] : c;
+            const float lap = xm + xp + ym + yp + zm + zp - 6.0f * c;
+            const float decay = p.lambda_decay[s0 + s] * c;
+            const float nc = c + p.dt_sub * (dd * lap - decay);
+            out[s] = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, c) & keep) |
+                                                   (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
+          }
+          if (outv[r]) {
+            const uint32_t zo_b = (uint32_t)zo * XYb;
+#pragma unroll
+            for (int s = 0; s < S; ++s) bstore_nt(rd, out[s], vo[r], (uint32_t)(s0 + s) * Pb + zo_b);
+          }
+        }
+      }
+      __syncthreads();
+      if (first) {
+#pragma unroll
+        for (int r = 0; r < RY; ++r)
+#pragma unroll
+          for (int s = 0; s < S; ++s) mid[s0 + s][wy + NW * r][lx] = p1n[r][s];
+      }
+      // the mask of plane z-1 (fetched steps ago) becomes plane z-2's
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
-        const int row = wy + NW * r + 1;
-        const int y = yr[r];
-        const uint32_t bc = bitat(z & 1, row, x), bxm = bitat(z & 1, row, x - 1), bxp = bitat(z & 1, row, x + 1);
-        const uint32_t bym = bitat(z & 1, row - 1, x), byp = bitat(z & 1, row + 1, x);
-        const uint32_t bzm = (qm[0][r] >> (x & 31)) & 1u, bzp = (qm[2][r] >> (x & 31)) & 1u;
-        const bool vxm = (uint32_t)(x > 0) & (bxm ^ 1u);
-        const bool vxp = (uint32_t)(x < X - 1) & (bxp ^ 1u);
-        const bool vym = (uint32_t)(y > 0) & (bym ^ 1u);
-        const bool vyp = (uint32_t)(y < Y - 1) & (byp ^ 1u);
-        const bool vzm = (uint32_t)(z > 0) & (bzm ^ 1u);
-        const bool vzp = (uint32_t)(z < Z - 1) & (bzp ^ 1u);
-        const float structural = lds[7][row][lx + 1];
-        const float niche = 1.0f - p.alpha_eps * structural / (p.k_eps + structural);
-        const uint32_t keep = 0u - bc;
+        qmz2[r] = qm[km][r];
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-          const float c = q[1][r][s];
-          const float dd = p.d_voxel[s0 + s] * niche;
-          const float xm = vxm ? lds[s0 + s][row][lx] : c;
-          const float xp = vxp ? lds[s0 + s][row][lx + 2] : c;
-          const float ym = vym ? lds[s0 + s][row - 1][lx + 1] : c;
-          const float yp = vyp ? lds[s0 + s][row + 1][lx + 1] : c;
-          const float zm = vzm ? q[0][r][s] : c;
-          const float zp = vzp ? q[2][r][s] : c;
-          const float lap = xm + xp + ym + yp + zm + zp - 6.0f * c;
-          const float decay = p.lambda_decay[s0 + s] * c;
-          const float nc = c + p.dt_sub * (dd * lap - decay);
-          p1n[r][s] = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, c) & keep) |
-                                                    (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
-        }
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < RY; ++r)
-#pragma unroll
-        for (int s = 0; s < S; ++s) p1n[r][s] = 0.0f;
-    }
-
-    // second substep on plane zo = z-1 from the intermediate planes
-    const int zo = z - 1;
-    if (zo >= z0) {
-      const int mb = zo & 1;
-#pragma unroll
-      for (int r = 0; r < RY; ++r) {
-        const int mrw = wy + NW * r;         // row in mid
-        const int row = mrw + 1;             // row in the mask LDS
-        const int y = yr[r];
-        const uint32_t bc = bitat(mb, row, x), bxm = bitat(mb, row, x - 1), bxp = bitat(mb, row, x + 1);
-        const uint32_t bym = bitat(mb, row - 1, x), byp = bitat(mb, row + 1, x);
-        const uint32_t bzm = (qmz2[r] >> (x & 31)) & 1u, bzp = (qm[1][r] >> (x & 31)) & 1u;
-        const bool vxm = (uint32_t)(x > 0) & (bxm ^ 1u);
-        const bool vxp = (uint32_t)(x < X - 1) & (bxp ^ 1u);
-        const bool vym = (uint32_t)(y > 0) & (bym ^ 1u);
-        const bool vyp = (uint32_t)(y < Y - 1) & (byp ^ 1u);
-        const bool vzm = (uint32_t)(zo > 0) & (bzm ^ 1u);
-        const bool vzp = (uint32_t)(zo < Z - 1) & (bzp ^ 1u);
-        // edge lanes and rows (no output) read clamped slots
-        const int cxm = lx > 0 ? lx - 1 : 0, cxp = lx < 63 ? lx + 1 : 63;
-        const int rym = mrw > 0 ? mrw - 1 : 0, ryp = mrw < TY - 1 ? mrw + 1 : TY - 1;
-        const float structural = mid[7][mrw][lx];
-        const float niche = 1.0f - p.alpha_eps * structural / (p.k_eps + structural);
-        const uint32_t keep = 0u - bc;
-        float out[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-          const float c = p1[1][r][s];
-          const float dd = p.d_voxel[s0 + s] * niche;
-          const float xm = vxm ? mid[s0 + s][mrw][cxm] : c;
-          const float xp = vxp ? mid[s0 + s][mrw][cxp] : c;
-          const float ym = vym ? mid[s0 + s][rym][lx] : c;
-          const float yp = vyp ? mid[s0 + s][ryp][lx] : c;
-          const float zm = vzm ? p1[0][r][s] : c;
-          const float zp = vzp ? p1n[r][s] : c;
-          const float lap = xm + xp + ym + yp + zm + zp - 6.0f * c;
-          const float decay = p.lambda_decay[s0 + s] * c;
-          const float nc = c + p.dt_sub * (dd * lap - decay);
-          out[s] = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, c) & keep) |
-                                                 (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
-        }
-        if (outv[r]) {
-          const uint32_t zo_b = (uint32_t)zo * XYb;
-#pragma unroll
-          for (int s = 0; s < S; ++s) bstore_nt(rd, out[s], vo[r], (uint32_t)(s0 + s) * Pb + zo_b);
+          p1[0][r][s] = p1[1][r][s];
+          p1[1][r][s] = p1n[r][s];
         }
       }
     }
-    __syncthreads();
-    if (first) {
-#pragma unroll
-      for (int r = 0; r < RY; ++r)
-#pragma unroll
-        for (int s = 0; s < S; ++s) mid[s0 + s][wy + NW * r][lx] = p1n[r][s];
-    }
-    // rotate the queues
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      qmz2[r] = qm[0][r];
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        p1[0][r][s] = p1[1][r][s];
-        p1[1][r][s] = p1n[r][s];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k + 1 < QN; ++k)
-#pragma unroll
-      for (int r = 0; r < RY; ++r) {
-        qm[k][r] = qm[k + 1][r];
-#pragma unroll
-        for (int s = 0; s < S; ++s) q[k][r][s] = q[k + 1][r][s];
-      }
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      qm[QN - 1][r] = qmn[r];
-#pragma unroll
-      for (int s = 0; s < S; ++s) q[QN - 1][r][s] = qn[r][s];
-    }
-#pragma unroll
-    for (int k = 0; k + 1 < HN; ++k) {
-      hm[k] = hm[k + 1];
-#pragma unroll
-      for (int s = 0; s < S; ++s) h[k][s] = h[k + 1][s];
-    }
-    hm[HN - 1] = hmn;
-#pragma unroll
-    for (int s = 0; s < S; ++s) h[HN - 1][s] = hn[s];
   }
 }
 
-template <int RY, bool SRC, int PD, int SG>
+template <int RY, bool SRC, int PD, int SG, int HB>
 static hipError_t launch_stencil_t(marl_ctx* ctx, const StencilParams& p, int zc) {
+  static_assert(MARL_S_EXT % (SG * HB) == 0, "species split");
   const int X = ctx->d.x, Y = ctx->d.y, Z = ctx->d.z;
   const int ntx = (X + 63) / 64, nty = (Y + 4 * RY - 1) / (4 * RY), ntz = (Z + zc - 1) / zc;
-  const int nb = ntx * nty * ntz * (MARL_S_EXT / SG);
+  const int nb = ntx * nty * ntz * (MARL_S_EXT / (SG * HB));
   const float* src = ctx->field[ctx->cur];
   float* dst = ctx->field[ctx->cur ^ 1];
-  hipLaunchKernelGGL((k_stencil<RY, SRC, PD, SG>), dim3(nb), dim3(256), 0, ctx->stream, src, dst, ctx->occ, X, Y, Z,
-                     ctx->xw, zc, ntx, nty, ntz, p);
+  hipLaunchKernelGGL((k_stencil<RY, SRC, PD, SG, HB>), dim3(nb), dim3(256 * HB), 0, ctx->stream, src, dst, ctx->occ,
+                     X, Y, Z, ctx->xw, zc, ntx, nty, ntz, p);
   return hipGetLastError();
 }
 
-template <int RY, int PD, int SG>
+template <int RY, int PD, int SG, int HB = 1>
 static hipError_t launch_stencil_s(marl_ctx* ctx, const StencilParams& p, int zc) {
-  return p.apply_sources ? launch_stencil_t<RY, true, PD, SG>(ctx, p, zc)
-                         : launch_stencil_t<RY, false, PD, SG>(ctx, p, zc);
+  return p.apply_sources ? launch_stencil_t<RY, true, PD, SG, HB>(ctx, p, zc)
+                         : launch_stencil_t<RY, false, PD, SG, HB>(ctx, p, zc);
 }
 
 int marl_launch_substep(marl_ctx* ctx, const StencilParams& p, int tile) {
@@ -706,13 +629,23 @@ int marl_launch_substep(marl_ctx* ctx, const StencilParams& p, int tile) {
   const TileShape t = kTiles[tile];
   const int zc = t.zc < ctx->d.z ? t.zc : ctx->d.z;
   hipError_t e = hipErrorInvalidValue;
-  if (t.sg == 12) {
-    if (t.ry == 1 && t.pd == 1) e = launch_stencil_s<1, 1, 12>(ctx, p, zc);
-    else if (t.ry == 2 && t.pd == 1) e = launch_stencil_s<2, 1, 12>(ctx, p, zc);
-  } else if (t.sg == 6) {
-    if (t.ry == 2 && t.pd == 1) e = launch_stencil_s<2, 1, 6>(ctx, p, zc);
-    else if (t.ry == 2 && t.pd == 2) e = launch_stencil_s<2, 2, 6>(ctx, p, zc);
-    else if (t.ry == 4 && t.pd == 1) e = launch_stencil_s<4, 1, 6>(ctx, p, zc);
+  if (t.nw == 4) {
+    if (t.sg == 12) {
+      if (t.ry == 1 && t.pd == 1) e = launch_stencil_s<1, 1, 12>(ctx, p, zc);
+      else if (t.ry == 2 && t.pd == 1) e = launch_stencil_s<2, 1, 12>(ctx, p, zc);
+    } else if (t.sg == 6) {
+      if (t.ry == 1 && t.pd == 1) e = launch_stencil_s<1, 1, 6>(ctx, p, zc);
+      else if (t.ry == 2 && t.pd == 1) e = launch_stencil_s<2, 1, 6>(ctx, p, zc);
+      else if (t.ry == 2 && t.pd == 2) e = launch_stencil_s<2, 2, 6>(ctx, p, zc);
+      else if (t.ry == 4 && t.pd == 1) e = launch_stencil_s<4, 1, 6>(ctx, p, zc);
+    }
+  } else if (t.nw == 8 && t.sg == 6 && t.pd == 1) {
+    if (t.ry == 1) e = launch_stencil_s<1, 1, 6, 2>(ctx, p, zc);
+    else if (t.ry == 2) e = launch_stencil_s<2, 1, 6, 2>(ctx, p, zc);
+  } else if (t.nw == 12 && t.sg == 4) {
+    if (t.ry == 1 && t.pd == 1) e = launch_stencil_s<1, 1, 4, 3>(ctx, p, zc);
+    else if (t.ry == 1 && t.pd == 2) e = launch_stencil_s<1, 2, 4, 3>(ctx, p, zc);
+    else if (t.ry == 2 && t.pd == 1) e = launch_stencil_s<2, 1, 4, 3>(ctx, p, zc);
   }
   MARL_HIP(ctx, e);
   ctx->cur ^= 1;
@@ -780,8 +713,13 @@ static hipError_t launch_stencil_ens(hipStream_t st, const EnsColumn* dcols, int
   return hipGetLastError();
 }
 
+// The persistent ensemble kernel runs single-substep tiles of one 4-wave group.
+bool marl_tile_ensemble_ok(int tile) {
+  return tile >= 0 && tile < kNumTiles && kTiles[tile].tb == 1 && kTiles[tile].nw == 4;
+}
+
 void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int* total_light_blocks) {
-  if (tile < 0 || tile >= kNumTiles || kTiles[tile].tb != 1) tile = 3;
+  if (!marl_tile_ensemble_ok(tile)) tile = 3;
   const TileShape t = kTiles[tile];
   const int NG = MARL_S_EXT / t.sg;
   int base = 0, lbase = 0;
@@ -804,7 +742,7 @@ void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int
 // Persistent grid: as many blocks as fit the chip at once (occupancy of the
 // instantiation x CUs), a multiple of 8 so every XCD gets the same share.
 int marl_stencil_ens_grid(int tile, int total_tiles) {
-  if (tile < 0 || tile >= kNumTiles || kTiles[tile].tb != 1) tile = 3;
+  if (!marl_tile_ensemble_ok(tile)) tile = 3;
   const TileShape t = kTiles[tile];
   int per_cu = 0;
   hipError_t e = hipErrorInvalidValue;
@@ -824,7 +762,7 @@ int marl_stencil_ens_grid(int tile, int total_tiles) {
 
 hipError_t marl_launch_substep_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total, int parity,
                                    const StencilParams& p, int tile, int grid) {
-  if (tile < 0 || tile >= kNumTiles || kTiles[tile].tb != 1) tile = 3;
+  if (!marl_tile_ensemble_ok(tile)) tile = 3;
   const TileShape t = kTiles[tile];
   if (t.sg == 12) {
     if (t.ry == 1 && t.pd == 1) return launch_stencil_ens<1, 1, 12>(st, dcols, ncols, total, parity, p, grid);

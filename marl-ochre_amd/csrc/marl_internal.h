@@ -303,4 +303,5 @@ int marl_launch_copy_probe(marl_ctx* ctx);
 struct TileShape { int ry, zc, pd, sg, tb, nw; };  // tb: substeps per pass (2 = temporal blocking); nw: waves/block
 extern const TileShape kTiles[];
 extern const int kNumTiles;
+bool marl_tile_ensemble_ok(int tile);
 int marl_default_tile(const marl_dims& d);

@@ -152,6 +152,13 @@ class MarlGpu:
         self._chk(self.lib.marl_stencil_tile_shape(int(t), ctypes.byref(ry), ctypes.byref(zc), ctypes.byref(pd)))
         return ry.value, zc.value, pd.value
 
+    def tile_split(self, t: int | None = None) -> tuple[int, int]:
+        """(species per thread, waves per block) of a stencil tile id."""
+        t = self.tile if t is None else t
+        sg, nw = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self.lib.marl_stencil_tile_split(int(t), ctypes.byref(sg), ctypes.byref(nw)))
+        return sg.value, nw.value
+
     def tile_substeps(self, t: int | None = None) -> int:
         """Substeps per launch of a tile (2 for the temporally blocked tiles)."""
         return int(self.lib.marl_stencil_tile_substeps(int(self.tile if t is None else t)))

@@ -99,9 +99,15 @@ def test_tile_table(pkg):
     assert set(subs) == {1, 2} and subs == sorted(subs)
     assert lib.marl_stencil_tile_substeps(-1) < 0 and lib.marl_stencil_tile_substeps(n) < 0
     ry, zc, pd = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    sg, nw = ctypes.c_int32(), ctypes.c_int32()
     for t in range(n):
         assert lib.marl_stencil_tile_shape(t, ctypes.byref(ry), ctypes.byref(zc), ctypes.byref(pd)) == 0
         assert ry.value in (1, 2, 4) and zc.value in (16, 32, 64) and pd.value in (1, 2)
+        assert lib.marl_stencil_tile_split(t, ctypes.byref(sg), ctypes.byref(nw)) == 0
+        assert nw.value in (4, 8, 12) and 12 % sg.value == 0
+        if subs[t] == 1:  # a block's wave groups split the 12 species evenly
+            assert 12 % (sg.value * nw.value // 4) == 0
+    assert lib.marl_stencil_tile_split(n, None, None) != 0
 
 
 def test_topology_hash_groups_wiring(pkg):

@@ -73,6 +73,23 @@ def test_ensemble_every_tile_shape(pkg, tile):
     _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=2), 2, tile=tile)
 
 
+def test_ensemble_rejects_split_and_temporal_tiles(pkg):
+    """The persistent ensemble kernel runs 4-wave, single-substep tiles only."""
+    cols = _columns(pkg)[:2]
+    gs = [pkg.MarlGpu(c[0]) for c in cols]
+    try:
+        with pkg.MarlEnsemble(gs) as ens:
+            n = gs[0].lib.marl_stencil_tile_count()
+            bad = [t for t in range(n) if gs[0].tile_substeps(t) != 1 or gs[0].tile_split(t)[1] != 4]
+            assert bad
+            for t in bad:
+                with pytest.raises(pkg.InvalidInput):
+                    ens.set_tile(t)
+    finally:
+        for g in gs:
+            g.close()
+
+
 def test_ensemble_autotune_keeps_fields(pkg):
     cols = _columns(pkg)
     gs = [pkg.MarlGpu(c[0]) for c in cols]

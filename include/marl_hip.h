@@ -20,7 +20,8 @@
  *
  * Layouts at the boundary
  *   - field AoS: ((z*Y + y)*X + x)*12 + s, f32 (crates/marl-field/src/field.rs:33-35);
- *   - field SoA: (s*Z + z)*Y*X + y*X + x, f32 (device layout);
+ *   - field SoA: (s*Z + z)*Y*X + y*X + x, f32 (species planes; the device
+ *     holds species quads, ((s/4)*Z*Y*X + (z*Y + y)*X + x)*4 + s%4);
  *   - occupancy: one byte per voxel, (z*Y + y)*X + x, nonzero = occupied
  *     (crates/marl-sim/src/lib.rs:174-179);
  *   - light: (z*Y + y)*X + x, f32 (crates/marl-field/src/light.rs:20-24);

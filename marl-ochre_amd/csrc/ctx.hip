@@ -184,7 +184,9 @@ int marl_download_field_aos(marl_ctx* ctx, float* aos, size_t n) {
 int marl_upload_field_soa(marl_ctx* ctx, const float* soa, size_t n) {
   if (!ctx || !soa) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "null argument");
   if (int rc = check_len(ctx, n, ctx->nfield, "field")) return rc;
-  MARL_HIP(ctx, hipMemcpyAsync(ctx->field[ctx->cur], soa, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  if (int rc = marl_ensure_staging(ctx)) return rc;
+  MARL_HIP(ctx, hipMemcpyAsync(ctx->staging, soa, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  if (int rc = marl_planes(ctx, true, ctx->field[ctx->cur])) return rc;
   MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MARL_OK;
 }
@@ -192,7 +194,9 @@ int marl_upload_field_soa(marl_ctx* ctx, const float* soa, size_t n) {
 int marl_download_field_soa(marl_ctx* ctx, float* soa, size_t n) {
   if (!ctx || !soa) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "null argument");
   if (int rc = check_len(ctx, n, ctx->nfield, "field")) return rc;
-  return d2h_checked(ctx, soa, ctx->field[ctx->cur], n * sizeof(float));
+  if (int rc = marl_ensure_staging(ctx)) return rc;
+  if (int rc = marl_planes(ctx, false, ctx->field[ctx->cur])) return rc;
+  return d2h_checked(ctx, soa, ctx->staging, n * sizeof(float));
 }
 
 int marl_upload_occupancy(marl_ctx* ctx, const uint8_t* occ, size_t n) {

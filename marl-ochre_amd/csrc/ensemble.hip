@@ -25,7 +25,7 @@ struct marl_ensemble {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<EnsColumn> hcols;
   EnsColumn* dcols = nullptr;
-  int tile = 3;
+  int tile = 7;
   int total_tiles = 0, total_light = 0, grid = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -197,7 +197,7 @@ int marl_ensemble_set_tile(marl_ensemble* e, int32_t tile) {
   if (!e) return MARL_ERR_INVALID_INPUT;
   if (tile < 0 || tile >= marl_stencil_tile_count()) return efail(e, MARL_ERR_INVALID_INPUT, "unknown tile id");
   if (!marl_tile_ensemble_ok(tile))
-    return efail(e, MARL_ERR_INVALID_INPUT, "ensembles use single-substep, single-group tiles");
+    return efail(e, MARL_ERR_INVALID_INPUT, "ensembles use single-substep tiles");
   if (tile != e->tile) drop(e);
   e->tile = tile;
   return MARL_OK;

@@ -28,12 +28,15 @@
 // {rows per thread, z-chunk, prefetch distance in planes, species per thread,
 //  substeps per pass, waves per block}
 const TileShape kTiles[] = {
-    {1, 32, 1, 12, 1, 4}, {2, 16, 1, 12, 1, 4}, {2, 32, 1, 12, 1, 4}, {2, 16, 1, 6, 1, 4}, {2, 32, 1, 6, 1, 4},
-    {2, 64, 1, 6, 1, 4},  {2, 32, 2, 6, 1, 4},  {4, 32, 1, 6, 1, 4},  {4, 64, 1, 6, 1, 4},
-    // split blocks: 4-wave groups of one block share the tile's LDS plane,
-    // each group taking 6 (or 4) of the 12 species
+    // one 4-wave group per block, all 12 species per thread (b128 access)
+    {1, 32, 1, 12, 1, 4}, {2, 16, 1, 12, 1, 4}, {2, 32, 1, 12, 1, 4},
+    // split blocks: the 4-wave groups of one block share the tile's LDS plane,
+    // each group taking 6 (b64 access) or 4 (one quad, b128 access) species
     {2, 16, 1, 6, 1, 8}, {2, 32, 1, 6, 1, 8}, {1, 32, 1, 4, 1, 12}, {2, 16, 1, 4, 1, 12}, {2, 32, 1, 4, 1, 12},
-    {1, 32, 1, 6, 1, 4}, {1, 32, 1, 6, 1, 8}, {1, 64, 1, 4, 1, 12}, {1, 32, 2, 4, 1, 12}, {1, 16, 1, 4, 1, 12},
+    {1, 64, 1, 4, 1, 12}, {1, 32, 2, 4, 1, 12}, {2, 64, 1, 4, 1, 12},
+    // 6 species per 4-wave block: the group without species 7 reads it for
+    // the niche factor from HBM
+    {2, 16, 1, 6, 1, 4},
     // temporal blocking (K1b): two substeps per launch
     {1, 16, 1, 12, 2, 8}, {1, 16, 1, 6, 2, 8},
 };
@@ -41,7 +44,30 @@ const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 int marl_default_tile(const marl_dims& d) {
   (void)d;
-  return 3;  // 2 rows per thread, z-chunk 16, prefetch 1, 6 species; the autotuner may pick another
+  return 7;  // 2 rows per thread, z-chunk 32, 3 groups of one quad each; the autotuner may pick another
+}
+
+// The kernel instantiation of a single-substep tile shape: calls f(TileT<...>)
+// for the shape's template parameters (rows, prefetch, species per thread,
+// groups per block), or returns hipErrorInvalidValue for a shape without one.
+template <int RY_, int PD_, int SG_, int HB_>
+struct TileT {
+  static constexpr int RY = RY_, PD = PD_, SG = SG_, HB = HB_;
+};
+template <typename F>
+static hipError_t dispatch_tile(const TileShape& t, F&& f) {
+  const int hb = t.nw / 4;
+#define MARL_TILE(R_, P_, S_, H_) \
+  if (t.ry == R_ && t.pd == P_ && t.sg == S_ && hb == H_) return f(TileT<R_, P_, S_, H_>{});
+  MARL_TILE(1, 1, 12, 1)
+  MARL_TILE(2, 1, 12, 1)
+  MARL_TILE(2, 1, 6, 2)
+  MARL_TILE(1, 1, 4, 3)
+  MARL_TILE(1, 2, 4, 3)
+  MARL_TILE(2, 1, 4, 3)
+  MARL_TILE(2, 1, 6, 1)
+#undef MARL_TILE
+  return hipErrorInvalidValue;
 }
 
 // Raw buffer accesses: the voxel's byte offset lives in one VGPR, the
@@ -64,12 +90,13 @@ __device__ __forceinline__ void bstore_nt(rsrc_t r, float v, uint32_t voff, uint
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 2);
 }
 
-// Load species s at plane zz; with SRC the boundary sources of field.rs:231-250
-// are applied on the fly (the first substep of a tick reads the sourced field).
+// Byte offset of species s within the quad field (voxel and z-plane offsets
+// excluded): its quad plane (Qb bytes each) and its lane of the 16-byte quad.
+__device__ __forceinline__ uint32_t qoff(int s, uint32_t Qb) { return (uint32_t)(s >> 2) * Qb + (uint32_t)(s & 3) * 4u; }
+
+// A species of a voxel with the boundary sources applied when SRC.
 template <bool SRC>
-__device__ __forceinline__ float ld_field(rsrc_t r, const StencilParams& p, int s, uint32_t Pb, uint32_t XYb,
-                                          int Z, uint32_t voff, int zz) {
-  float v = bload(r, voff, (uint32_t)s * Pb + (uint32_t)zz * XYb);
+__device__ __forceinline__ float with_sources(float v, const StencilParams& p, int s, int Z, int zz) {
   if (SRC) {
     if (zz == 0) {
       if (s == 1) v = fminf(v + p.src_ox, p.c_max);
@@ -79,6 +106,68 @@ __device__ __forceinline__ float ld_field(rsrc_t r, const StencilParams& p, int 
   }
   return v;
 }
+
+// Load species s at plane zz (voff = voxel-in-plane * 16, XYq = bytes per
+// z-plane of a quad plane); with SRC the boundary sources of field.rs:231-250
+// are applied on the fly (the first substep of a tick reads the sourced field).
+template <bool SRC>
+__device__ __forceinline__ float ld_field(rsrc_t r, const StencilParams& p, int s, uint32_t Qb, uint32_t XYq,
+                                          int Z, uint32_t voff, int zz) {
+  return with_sources<SRC>(bload(r, voff, qoff(s, Qb) + (uint32_t)zz * XYq), p, s, Z, zz);
+}
+
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// A consecutive species s .. s+A-1 of one voxel (s a multiple of A, A = 1, 2
+// or 4) with one 4/8/16-byte load: in the quad layout they are adjacent.  The
+// wide loads are ordinary global loads from a scalar base (the plane) plus the
+// 32-bit voxel offset, not buffer-load builtins: this compiler shrinks a wide
+// buffer load to one dword when its elements are re-packed into a vector.
+// Offsets stay inside the field (planes are clamped, out-of-grid lanes read
+// voxel 0), so no buffer range check is needed.
+template <bool SRC, int A>
+__device__ __forceinline__ void ld_vec(const float* __restrict__ base, const StencilParams& p, int s, uint32_t Qb,
+                                       uint32_t XYq, int Z, uint32_t voff, int zz, float* out) {
+  const uint32_t so = qoff(s, Qb) + (uint32_t)zz * XYq;
+  const char* a = reinterpret_cast<const char*>(base) + so;
+  if constexpr (A == 4) {
+    const f32x4 t = *reinterpret_cast<const f32x4*>(a + voff);
+    out[0] = t.x;
+    out[1] = t.y;
+    out[2] = t.z;
+    out[3] = t.w;
+  } else if constexpr (A == 2) {
+    const f32x2 t = *reinterpret_cast<const f32x2*>(a + voff);
+    out[0] = t.x;
+    out[1] = t.y;
+  } else {
+    out[0] = *reinterpret_cast<const float*>(a + voff);
+  }
+#pragma unroll
+  for (int k = 0; k < A; ++k) out[k] = with_sources<SRC>(out[k], p, s + k, Z, zz);
+}
+
+// Non-temporal store of species s .. s+A-1 of one voxel.
+template <int A>
+__device__ __forceinline__ void st_vec_nt(rsrc_t r, const float* v, uint32_t voff, uint32_t soff) {
+  if constexpr (A == 4) {
+    const u32x4 t = {__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1]),
+                     __builtin_bit_cast(uint32_t, v[2]), __builtin_bit_cast(uint32_t, v[3])};
+    __builtin_amdgcn_raw_buffer_store_b128(t, r, (int)voff, (int)soff, 2);
+  } else if constexpr (A == 2) {
+    const u32x2 t = {__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1])};
+    __builtin_amdgcn_raw_buffer_store_b64(t, r, (int)voff, (int)soff, 2);
+  } else {
+    bstore_nt(r, v[0], voff, soff);
+  }
+}
+
+// Vector width of a thread's species group: s0 is a multiple of SG.
+template <int SG>
+constexpr int group_vec() { return SG % 4 == 0 ? 4 : (SG % 2 == 0 ? 2 : 1); }
 
 // LDS of one tile: species planes with a one-voxel halo (one spare row takes
 // the writes of lanes that own no halo value, so those writes need no branch)
@@ -119,10 +208,10 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
   const int x = x0 + lx;
   const bool xin = x < X;
   const uint32_t XY = (uint32_t)X * (uint32_t)Y;
-  const uint32_t XYb = XY * 4u, Pb = XYb * (uint32_t)Z;
+  const uint32_t XYq = XY * 16u, Qb = XYq * (uint32_t)Z;  // field bytes per quad z-plane, per quad plane
   const uint32_t XWb = (uint32_t)XW * 4u, MZb = XWb * (uint32_t)Y;
-  const rsrc_t rs = make_rsrc(src, Pb * MARL_S_EXT);
-  const rsrc_t rd = make_rsrc(dst, Pb * MARL_S_EXT);
+  const rsrc_t rs = make_rsrc(src, Qb * 3u);
+  const rsrc_t rd = make_rsrc(dst, Qb * 3u);
   const rsrc_t rm = make_rsrc(occ, MZb * (uint32_t)Z);
 
   // All loads are unconditional per lane: lanes outside the grid (partial
@@ -135,7 +224,7 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
   for (int r = 0; r < RY; ++r) {
     yr[r] = y0 + wy + 4 * r;
     valid[r] = xin && yr[r] < Y;
-    vo[r] = valid[r] ? ((uint32_t)yr[r] * (uint32_t)X + (uint32_t)x) * 4u : 0u;
+    vo[r] = valid[r] ? ((uint32_t)yr[r] * (uint32_t)X + (uint32_t)x) * 16u : 0u;
     mo[r] = valid[r] ? (uint32_t)yr[r] * XWb + (uint32_t)(x >> 5) * 4u : 0u;
   }
   // halo slot of this lane: rows y0-1 / y0+TY for waves 0-1, columns x0-1 /
@@ -148,14 +237,14 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
     hw = yy >= 0 && yy < Y && xin;
     hrow = (wy == 0) ? 0 : TY + 1;
     hcol = lx + 1;
-    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)x) * 4u : 0u;
+    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)x) * 16u : 0u;
   } else {
     const int xx = (wy == 2) ? x0 - 1 : x0 + 64;
     const int yy = y0 + (lx < TY ? lx : 0);
     hw = lx < TY && xx >= 0 && xx < X && yy < Y;
     hrow = (lx < TY ? lx : 0) + 1;
     hcol = (wy == 2) ? 0 : LW - 1;
-    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)(xx < 0 ? 0 : xx)) * 4u : 0u;
+    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)(xx < 0 ? 0 : xx)) * 16u : 0u;
   }
   if (!hw) {
     hrow = TY + 2;
@@ -181,21 +270,22 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
   uint32_t hm[R];   // this lane's mask word of plane pl
   float q7[R][RY];  // species 7 of plane pl for the niche factor (blocks without it)
 
+  constexpr int A = group_vec<SG>();
   auto load_own = [&](int k, int zz) {
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
 #pragma unroll
-      for (int s = 0; s < S; ++s) q[k][r][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, vo[r], zz);
+      for (int s = 0; s < S; s += A) ld_vec<SRC, A>(src, p, s0 + s, Qb, XYq, Z, vo[r], zz, &q[k][r][s]);
       qm[k][r] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mo[r], (int)((uint32_t)zz * MZb), 0);
     }
   };
   auto load_halo = [&](int k, int zz) {
 #pragma unroll
-    for (int s = 0; s < S; ++s) h[k][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, hvo, zz);
+    for (int s = 0; s < S; s += A) ld_vec<SRC, A>(src, p, s0 + s, Qb, XYq, Z, hvo, zz, &h[k][s]);
     hm[k] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zz * MZb), 0);
     if (SG < MARL_S_EXT && !blk7) {
 #pragma unroll
-      for (int r = 0; r < RY; ++r) q7[k][r] = ld_field<SRC>(rs, p, 7, Pb, XYb, Z, vo[r], zz);
+      for (int r = 0; r < RY; ++r) q7[k][r] = ld_field<SRC>(rs, p, 7, Qb, XYq, Z, vo[r], zz);
     }
   };
   // prologue: planes z0-1 .. z0+PD (own) and z0 .. z0+PD (halo), clamped into the grid
@@ -238,7 +328,7 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
           const int i = c + 32;
           return (mlds[rw * 4 + (i >> 5)] >> (i & 31)) & 1u;
         };
-        const uint32_t zo = (uint32_t)z * XYb;
+        const uint32_t zo = (uint32_t)z * XYq;
         // neighbour flags with bitwise operators: every mask read is unconditional
         const uint32_t bc = bitat(row, lx), bxm = bitat(row, lx - 1), bxp = bitat(row, lx + 1);
         const uint32_t bym = bitat(row - 1, lx), byp = bitat(row + 1, lx);
@@ -277,7 +367,7 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
         }
         if (valid[r]) {
 #pragma unroll
-          for (int s = 0; s < S; ++s) bstore_nt(rd, out[s], vo[r], (uint32_t)(s0 + s) * Pb + zo);
+          for (int s = 0; s < S; s += A) st_vec_nt<A>(rd, &out[s], vo[r], qoff(s0 + s, Qb) + zo);
         }
       }
       __syncthreads();
@@ -303,11 +393,11 @@ __global__ void __launch_bounds__(256 * HB) k_stencil(const float* __restrict__ 
 // logical tiles of all columns are numbered back to back (column after
 // column, each in the single-column order), spread over the XCDs by the same
 // remap, and each block walks its share with a grid stride.
-template <int RY, bool SRC, int PD, int SG>
-__global__ void __launch_bounds__(256) k_stencil_ens(const EnsColumn* __restrict__ cols, int ncols, int total,
-                                                     int parity, StencilParams p) {
-  __shared__ StencilLds<RY, SG> sh;
-  constexpr int NG = MARL_S_EXT / SG;
+template <int RY, bool SRC, int PD, int SG, int HB>
+__global__ void __launch_bounds__(256 * HB) k_stencil_ens(const EnsColumn* __restrict__ cols, int ncols, int total,
+                                                          int parity, StencilParams p) {
+  __shared__ StencilLds<RY, SG * HB> sh;
+  constexpr int NG = MARL_S_EXT / (SG * HB);
   for (int w = blockIdx.x; w < total; w += gridDim.x) {
     int b = xcd_remap(w, total);
     int c = 0;
@@ -317,7 +407,7 @@ __global__ void __launch_bounds__(256) k_stencil_ens(const EnsColumn* __restrict
     const int grp = b % NG;
     b /= NG;
     const int cur = col.cur ^ parity;
-    stencil_tile<RY, SRC, PD, SG>(col.f[cur], col.f[cur ^ 1], col.occ, col.X, col.Y, col.Z, col.XW, col.zc,
+    stencil_tile<RY, SRC, PD, SG, HB>(col.f[cur], col.f[cur ^ 1], col.occ, col.X, col.Y, col.Z, col.XW, col.zc,
                                   b % col.ntx, (b / col.ntx) % col.nty, b / (col.ntx * col.nty), grp, p, sh);
     __syncthreads();  // the next tile reuses the LDS
   }
@@ -370,10 +460,9 @@ __global__ void __launch_bounds__(64 * NW * (MARL_S_EXT / SG))
   const int x = x0 + lx;
   const bool xin = x >= 0 && x < X;
   const uint32_t XY = (uint32_t)X * (uint32_t)Y;
-  const uint32_t XYb = XY * 4u, Pb = XYb * (uint32_t)Z;
+  const uint32_t XYq = XY * 16u, Qb = XYq * (uint32_t)Z;  // field bytes per quad z-plane, per quad plane
   const uint32_t XWb = (uint32_t)XW * 4u, MZb = XWb * (uint32_t)Y;
-  const rsrc_t rs = make_rsrc(src, Pb * MARL_S_EXT);
-  const rsrc_t rd = make_rsrc(dst, Pb * MARL_S_EXT);
+  const rsrc_t rd = make_rsrc(dst, Qb * 3u);
   const rsrc_t rm = make_rsrc(occ, MZb * (uint32_t)Z);
 
   bool valid[RY], outv[RY];
@@ -385,7 +474,7 @@ __global__ void __launch_bounds__(64 * NW * (MARL_S_EXT / SG))
     yr[r] = y0 + row;
     valid[r] = xin && yr[r] >= 0 && yr[r] < Y;
     outv[r] = valid[r] && lx >= 1 && lx <= 62 && row >= 1 && row <= TY - 2;
-    vo[r] = valid[r] ? ((uint32_t)yr[r] * (uint32_t)X + (uint32_t)x) * 4u : 0u;
+    vo[r] = valid[r] ? ((uint32_t)yr[r] * (uint32_t)X + (uint32_t)x) * 16u : 0u;
     mo[r] = valid[r] ? (uint32_t)yr[r] * XWb + (uint32_t)(x >> 5) * 4u : 0u;
   }
   // halo slot of this lane (as in K1, around the first-substep tile)
@@ -397,7 +486,7 @@ __global__ void __launch_bounds__(64 * NW * (MARL_S_EXT / SG))
     hw = yy >= 0 && yy < Y && xin;
     hrow = (wy == 0) ? 0 : TY + 1;
     hcol = lx + 1;
-    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)x) * 4u : 0u;
+    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)x) * 16u : 0u;
   } else if (wy >= 4) {
     hw = false;
     hrow = TY + 2;
@@ -409,7 +498,7 @@ __global__ void __launch_bounds__(64 * NW * (MARL_S_EXT / SG))
     hw = lx < TY && xx >= 0 && xx < X && yy >= 0 && yy < Y;
     hrow = (lx < TY ? lx : 0) + 1;
     hcol = (wy == 2) ? 0 : LW - 1;
-    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)xx) * 4u : 0u;
+    hvo = hw ? ((uint32_t)yy * (uint32_t)X + (uint32_t)xx) * 16u : 0u;
   }
   if (!hw) {
     hrow = TY + 2;
@@ -441,17 +530,18 @@ __global__ void __launch_bounds__(64 * NW * (MARL_S_EXT / SG))
   float p1[2][RY][S];   // intermediate planes z-2, z-1 of this thread's column (computed, so copied freely)
   uint32_t qmz2[RY];    // mask word of plane z-2 of this thread's column
 
+  constexpr int A = group_vec<SG>();
   auto load_own = [&](int k, int zz) {
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
 #pragma unroll
-      for (int s = 0; s < S; ++s) q[k][r][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, vo[r], zz);
+      for (int s = 0; s < S; s += A) ld_vec<SRC, A>(src, p, s0 + s, Qb, XYq, Z, vo[r], zz, &q[k][r][s]);
       qm[k][r] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mo[r], (int)((uint32_t)zz * MZb), 0);
     }
   };
   auto load_halo = [&](int k, int zz) {
 #pragma unroll
-    for (int s = 0; s < S; ++s) h[k][s] = ld_field<SRC>(rs, p, s0 + s, Pb, XYb, Z, hvo, zz);
+    for (int s = 0; s < S; s += A) ld_vec<SRC, A>(src, p, s0 + s, Qb, XYq, Z, hvo, zz, &h[k][s]);
     hm[k] = __builtin_amdgcn_raw_buffer_load_b32(rm, (int)mvo, (int)((uint32_t)zz * MZb), 0);
   };
 #pragma unroll
@@ -578,9 +668,9 @@ __global__ void __launch_bounds__(64 * NW * (MARL_S_EXT / SG))
                                                    (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
           }
           if (outv[r]) {
-            const uint32_t zo_b = (uint32_t)zo * XYb;
+            const uint32_t zo_b = (uint32_t)zo * XYq;
 #pragma unroll
-            for (int s = 0; s < S; ++s) bstore_nt(rd, out[s], vo[r], (uint32_t)(s0 + s) * Pb + zo_b);
+            for (int s = 0; s < S; s += A) st_vec_nt<A>(rd, &out[s], vo[r], qoff(s0 + s, Qb) + zo_b);
           }
         }
       }
@@ -625,28 +715,13 @@ static hipError_t launch_stencil_s(marl_ctx* ctx, const StencilParams& p, int zc
 }
 
 int marl_launch_substep(marl_ctx* ctx, const StencilParams& p, int tile) {
-  if (tile < 0 || tile >= kNumTiles) tile = marl_default_tile(ctx->d);
+  if (tile < 0 || tile >= kNumTiles || kTiles[tile].tb != 1) tile = marl_default_tile(ctx->d);
   const TileShape t = kTiles[tile];
   const int zc = t.zc < ctx->d.z ? t.zc : ctx->d.z;
-  hipError_t e = hipErrorInvalidValue;
-  if (t.nw == 4) {
-    if (t.sg == 12) {
-      if (t.ry == 1 && t.pd == 1) e = launch_stencil_s<1, 1, 12>(ctx, p, zc);
-      else if (t.ry == 2 && t.pd == 1) e = launch_stencil_s<2, 1, 12>(ctx, p, zc);
-    } else if (t.sg == 6) {
-      if (t.ry == 1 && t.pd == 1) e = launch_stencil_s<1, 1, 6>(ctx, p, zc);
-      else if (t.ry == 2 && t.pd == 1) e = launch_stencil_s<2, 1, 6>(ctx, p, zc);
-      else if (t.ry == 2 && t.pd == 2) e = launch_stencil_s<2, 2, 6>(ctx, p, zc);
-      else if (t.ry == 4 && t.pd == 1) e = launch_stencil_s<4, 1, 6>(ctx, p, zc);
-    }
-  } else if (t.nw == 8 && t.sg == 6 && t.pd == 1) {
-    if (t.ry == 1) e = launch_stencil_s<1, 1, 6, 2>(ctx, p, zc);
-    else if (t.ry == 2) e = launch_stencil_s<2, 1, 6, 2>(ctx, p, zc);
-  } else if (t.nw == 12 && t.sg == 4) {
-    if (t.ry == 1 && t.pd == 1) e = launch_stencil_s<1, 1, 4, 3>(ctx, p, zc);
-    else if (t.ry == 1 && t.pd == 2) e = launch_stencil_s<1, 2, 4, 3>(ctx, p, zc);
-    else if (t.ry == 2 && t.pd == 1) e = launch_stencil_s<2, 1, 4, 3>(ctx, p, zc);
-  }
+  const hipError_t e = dispatch_tile(t, [&](auto T) {
+    using TT = decltype(T);
+    return launch_stencil_s<TT::RY, TT::PD, TT::SG, TT::HB>(ctx, p, zc);
+  });
   MARL_HIP(ctx, e);
   ctx->cur ^= 1;
   return MARL_OK;
@@ -702,26 +777,27 @@ int marl_launch_substeps(marl_ctx* ctx, const marl_sim_params* sim, int n, bool 
   return MARL_OK;
 }
 
-template <int RY, int PD, int SG>
+template <int RY, int PD, int SG, int HB>
 static hipError_t launch_stencil_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total, int parity,
                                      const StencilParams& p, int grid) {
   if (p.apply_sources)
-    hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG>), dim3(grid), dim3(256), 0, st, dcols, ncols, total, parity, p);
+    hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG, HB>), dim3(grid), dim3(256 * HB), 0, st, dcols, ncols, total,
+                       parity, p);
   else
-    hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG>), dim3(grid), dim3(256), 0, st, dcols, ncols, total, parity,
-                       p);
+    hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG, HB>), dim3(grid), dim3(256 * HB), 0, st, dcols, ncols,
+                       total, parity, p);
   return hipGetLastError();
 }
 
-// The persistent ensemble kernel runs single-substep tiles of one 4-wave group.
+// The persistent ensemble kernel runs the single-substep tiles.
 bool marl_tile_ensemble_ok(int tile) {
-  return tile >= 0 && tile < kNumTiles && kTiles[tile].tb == 1 && kTiles[tile].nw == 4;
+  return tile >= 0 && tile < kNumTiles && kTiles[tile].tb == 1;
 }
 
 void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int* total_light_blocks) {
-  if (!marl_tile_ensemble_ok(tile)) tile = 3;
+  if (!marl_tile_ensemble_ok(tile)) tile = 7;
   const TileShape t = kTiles[tile];
-  const int NG = MARL_S_EXT / t.sg;
+  const int NG = MARL_S_EXT / (t.sg * (t.nw / 4));
   int base = 0, lbase = 0;
   for (int c = 0; c < ncols; ++c) {
     EnsColumn& e = cols[c];
@@ -742,15 +818,14 @@ void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int
 // Persistent grid: as many blocks as fit the chip at once (occupancy of the
 // instantiation x CUs), a multiple of 8 so every XCD gets the same share.
 int marl_stencil_ens_grid(int tile, int total_tiles) {
-  if (!marl_tile_ensemble_ok(tile)) tile = 3;
+  if (!marl_tile_ensemble_ok(tile)) tile = 7;
   const TileShape t = kTiles[tile];
   int per_cu = 0;
-  hipError_t e = hipErrorInvalidValue;
-  if (t.sg == 12 && t.ry == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stencil_ens<1, false, 1, 12>, 256, 0);
-  else if (t.sg == 12) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stencil_ens<2, false, 1, 12>, 256, 0);
-  else if (t.ry == 2 && t.pd == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stencil_ens<2, false, 1, 6>, 256, 0);
-  else if (t.ry == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stencil_ens<2, false, 2, 6>, 256, 0);
-  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stencil_ens<4, false, 1, 6>, 256, 0);
+  const hipError_t e = dispatch_tile(t, [&](auto T) {
+    using TT = decltype(T);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stencil_ens<TT::RY, false, TT::PD, TT::SG, TT::HB>,
+                                                        256 * TT::HB, 0);
+  });
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess || per_cu < 1) per_cu = 1;
@@ -762,17 +837,11 @@ int marl_stencil_ens_grid(int tile, int total_tiles) {
 
 hipError_t marl_launch_substep_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total, int parity,
                                    const StencilParams& p, int tile, int grid) {
-  if (!marl_tile_ensemble_ok(tile)) tile = 3;
-  const TileShape t = kTiles[tile];
-  if (t.sg == 12) {
-    if (t.ry == 1 && t.pd == 1) return launch_stencil_ens<1, 1, 12>(st, dcols, ncols, total, parity, p, grid);
-    if (t.ry == 2 && t.pd == 1) return launch_stencil_ens<2, 1, 12>(st, dcols, ncols, total, parity, p, grid);
-  } else if (t.sg == 6) {
-    if (t.ry == 2 && t.pd == 1) return launch_stencil_ens<2, 1, 6>(st, dcols, ncols, total, parity, p, grid);
-    if (t.ry == 2 && t.pd == 2) return launch_stencil_ens<2, 2, 6>(st, dcols, ncols, total, parity, p, grid);
-    if (t.ry == 4 && t.pd == 1) return launch_stencil_ens<4, 1, 6>(st, dcols, ncols, total, parity, p, grid);
-  }
-  return hipErrorInvalidValue;
+  if (!marl_tile_ensemble_ok(tile)) tile = 7;
+  return dispatch_tile(kTiles[tile], [&](auto T) {
+    using TT = decltype(T);
+    return launch_stencil_ens<TT::RY, TT::PD, TT::SG, TT::HB>(st, dcols, ncols, total, parity, p, grid);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -780,12 +849,13 @@ hipError_t marl_launch_substep_ens(hipStream_t st, const EnsColumn* dcols, int n
 __global__ void k_boundary(float* __restrict__ f, int X, int Y, int Z, float src_ox, float src_c, float src_red,
                            float c_max) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t XY = (size_t)X * Y, P = XY * Z;
+  const size_t XY = (size_t)X * Y, nvox = XY * Z;
   if (i >= (int)XY) return;
-  f[1 * P + i] = fminf(f[1 * P + i] + src_ox, c_max);
-  f[3 * P + i] = fminf(f[3 * P + i] + src_c, c_max);
-  const size_t b = (size_t)(Z - 1) * XY + i;
-  f[2 * P + b] = fminf(f[2 * P + b] + src_red, c_max);
+  const size_t i1 = fq_index(1, i, nvox), i3 = fq_index(3, i, nvox);
+  f[i1] = fminf(f[i1] + src_ox, c_max);
+  f[i3] = fminf(f[i3] + src_c, c_max);
+  const size_t b = fq_index(2, (size_t)(Z - 1) * XY + i, nvox);
+  f[b] = fminf(f[b] + src_red, c_max);
 }
 
 int marl_launch_boundary(marl_ctx* ctx, const marl_sim_params* sim) {
@@ -800,7 +870,9 @@ int marl_launch_boundary(marl_ctx* ctx, const marl_sim_params* sim) {
 // ---------------------------------------------------------------------------
 // K2: Beer-Lambert light (light.rs:29-59).  One lane per (x, y) column,
 // coalesced across x; the z loop carries the intensity in a register.  The
-// sequential multiply order of the reference is kept per column.
+// sequential multiply order of the reference is kept per column.  s4 points at
+// species 4 of voxel 0 in the quad field (quad plane 1, lane 0): voxel v's
+// value is s4[4 v].
 __device__ __forceinline__ void light_column(const float* __restrict__ s4, const uint32_t* __restrict__ occ,
                                              float* __restrict__ light, int x, int y, int X, int Y, int Z, int XW,
                                              float I0, float e_cell, float nchem, float floor_) {
@@ -816,7 +888,7 @@ __device__ __forceinline__ void light_column(const float* __restrict__ s4, const
     uint32_t w[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      a[u] = s4[(size_t)(z + u) * XY + col];
+      a[u] = s4[4 * ((size_t)(z + u) * XY + col)];
       w[u] = occ[(size_t)(z + u) * Y * XW + wcol];
     }
 #pragma unroll
@@ -830,7 +902,7 @@ __device__ __forceinline__ void light_column(const float* __restrict__ s4, const
   for (; z < Z; ++z) {
     light[(size_t)z * XY + col] = I;
     if (occ[(size_t)z * Y * XW + wcol] & bit) I *= e_cell;
-    I *= marl_expf(nchem * s4[(size_t)z * XY + col]);
+    I *= marl_expf(nchem * s4[4 * ((size_t)z * XY + col)]);
     if (I < floor_) I = 0.0f;
   }
 }
@@ -877,27 +949,43 @@ int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim) {
 }
 
 // ---------------------------------------------------------------------------
-// Layout kernels.
-__global__ void k_aos_to_soa(const float* __restrict__ aos, float* __restrict__ soa, size_t nvox) {
+// Layout kernels.  AoS (the boundary's field.rs layout, 12 floats per voxel)
+// and the resident quads hold the same 16-byte groups, so the conversion is a
+// float4 permutation; species planes (SoA) go through per-float moves.
+__global__ void k_aos_to_quad(const float4* __restrict__ aos, float4* __restrict__ q, size_t nvox) {
   const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nvox) return;
-  const float4* a = reinterpret_cast<const float4*>(aos + v * MARL_S_EXT);
-  const float4 q0 = a[0], q1 = a[1], q2 = a[2];
-  const float vals[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
 #pragma unroll
-  for (int s = 0; s < MARL_S_EXT; ++s) soa[(size_t)s * nvox + v] = vals[s];
+  for (int g = 0; g < 3; ++g) q[(size_t)g * nvox + v] = aos[3 * v + g];
 }
 
-__global__ void k_soa_to_aos(const float* __restrict__ soa, float* __restrict__ aos, size_t nvox) {
+__global__ void k_quad_to_aos(const float4* __restrict__ q, float4* __restrict__ aos, size_t nvox) {
   const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nvox) return;
-  float vals[12];
 #pragma unroll
-  for (int s = 0; s < MARL_S_EXT; ++s) vals[s] = soa[(size_t)s * nvox + v];
-  float4* a = reinterpret_cast<float4*>(aos + v * MARL_S_EXT);
-  a[0] = make_float4(vals[0], vals[1], vals[2], vals[3]);
-  a[1] = make_float4(vals[4], vals[5], vals[6], vals[7]);
-  a[2] = make_float4(vals[8], vals[9], vals[10], vals[11]);
+  for (int g = 0; g < 3; ++g) aos[3 * v + g] = q[(size_t)g * nvox + v];
+}
+
+__global__ void k_soa_to_quad(const float* __restrict__ soa, float4* __restrict__ q, size_t nvox) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvox) return;
+#pragma unroll
+  for (int g = 0; g < 3; ++g)
+    q[(size_t)g * nvox + v] = make_float4(soa[(size_t)(4 * g) * nvox + v], soa[(size_t)(4 * g + 1) * nvox + v],
+                                          soa[(size_t)(4 * g + 2) * nvox + v], soa[(size_t)(4 * g + 3) * nvox + v]);
+}
+
+__global__ void k_quad_to_soa(const float4* __restrict__ q, float* __restrict__ soa, size_t nvox) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvox) return;
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    const float4 a = q[(size_t)g * nvox + v];
+    soa[(size_t)(4 * g) * nvox + v] = a.x;
+    soa[(size_t)(4 * g + 1) * nvox + v] = a.y;
+    soa[(size_t)(4 * g + 2) * nvox + v] = a.z;
+    soa[(size_t)(4 * g + 3) * nvox + v] = a.w;
+  }
 }
 
 // byte occupancy -> bit words (X % 32 == 0, so word w holds voxels 32w..32w+31)
@@ -925,13 +1013,27 @@ __global__ void k_unpack_occ(const uint32_t* __restrict__ words, uint8_t* __rest
   bytes[v] = (uint8_t)((words[v >> 5] >> (v & 31)) & 1u);
 }
 
-int marl_transpose(marl_ctx* ctx, bool to_soa, float* soa) {
+int marl_transpose(marl_ctx* ctx, bool to_quads, float* quads) {
   const size_t n = ctx->nvox;
   const dim3 g((unsigned)((n + 255) / 256));
-  if (to_soa)
-    hipLaunchKernelGGL(k_aos_to_soa, g, dim3(256), 0, ctx->stream, ctx->staging, soa, n);
+  float4* q = reinterpret_cast<float4*>(quads);
+  float4* a = reinterpret_cast<float4*>(ctx->staging);
+  if (to_quads)
+    hipLaunchKernelGGL(k_aos_to_quad, g, dim3(256), 0, ctx->stream, a, q, n);
   else
-    hipLaunchKernelGGL(k_soa_to_aos, g, dim3(256), 0, ctx->stream, soa, ctx->staging, n);
+    hipLaunchKernelGGL(k_quad_to_aos, g, dim3(256), 0, ctx->stream, q, a, n);
+  MARL_HIP(ctx, hipGetLastError());
+  return MARL_OK;
+}
+
+int marl_planes(marl_ctx* ctx, bool to_quads, float* quads) {
+  const size_t n = ctx->nvox;
+  const dim3 g((unsigned)((n + 255) / 256));
+  float4* q = reinterpret_cast<float4*>(quads);
+  if (to_quads)
+    hipLaunchKernelGGL(k_soa_to_quad, g, dim3(256), 0, ctx->stream, ctx->staging, q, n);
+  else
+    hipLaunchKernelGGL(k_quad_to_soa, g, dim3(256), 0, ctx->stream, q, ctx->staging, n);
   MARL_HIP(ctx, hipGetLastError());
   return MARL_OK;
 }

@@ -64,53 +64,23 @@ __device__ __forceinline__ marl_cell_state load_cell(const marl_cell_state* c) {
   return out;
 }
 
-// Free face neighbours (spatial.rs:18-43, FACE_OFFSETS order +x,-x,+y,-y,+z,-z)
-// of the cell's voxel from its 6-bit mask; returns their count.
-__device__ inline int cell_neighbours(const marl_cell_state& c, uint32_t m, const CellParams& p, size_t* v_out,
-                                      size_t nv[6]) {
-  const size_t XY = (size_t)p.X * p.Y;
-  const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
-  const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
-  int n = 0;
-#pragma unroll
-  for (int k = 0; k < 6; ++k)
-    if ((m >> k) & 1u) nv[n++] = (size_t)((int64_t)v + off[k]);
-  *v_out = v;
-  return n;
-}
-
-// Mean environment over the free neighbours (spatial.rs:56-77) from the AoS
-// field view: summed in face order, then divided; zeros when enclosed.
-__device__ inline void gather_env(const float* __restrict__ field, const size_t nv[6], int n, float* E) {
-  float acc[MARL_S_EXT];
-#pragma unroll
-  for (int s = 0; s < MARL_S_EXT; ++s) acc[s] = 0.0f;
-  for (int k = 0; k < n; ++k) {
-    const float4* fv = reinterpret_cast<const float4*>(field + nv[k] * MARL_S_EXT);
-    const float4 a = fv[0], b = fv[1], cc = fv[2];
-    acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
-    acc[4] += b.x; acc[5] += b.y; acc[6] += b.z; acc[7] += b.w;
-    acc[8] += cc.x; acc[9] += cc.y; acc[10] += cc.z; acc[11] += cc.w;
-  }
-#pragma unroll
-  for (int s = 0; s < MARL_S_EXT; ++s) E[s] = n ? acc[s] / (float)n : 0.0f;
-}
-
-// The same mean with all six face records requested at once (non-free faces
-// read the cell's own voxel and are ignored); the sum still runs over the free
-// faces in face order, so the result is identical to gather_env.
+// Mean environment over the free face neighbours (spatial.rs:56-77; faces in
+// FACE_OFFSETS order +x,-x,+y,-y,+z,-z, spatial.rs:18-43) from the resident
+// quad field: all six face voxels are requested at once (non-free faces read
+// the cell's own voxel and are ignored), the sum runs over the free faces in
+// face order, then divides; zeros when enclosed.
 __device__ inline void gather_env6(const float* __restrict__ field, size_t v, uint32_t m, const CellParams& p,
                                    float* E) {
-  const size_t XY = (size_t)p.X * p.Y;
+  const size_t XY = (size_t)p.X * p.Y, nvox = XY * (size_t)p.Z;
   const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
+  const float4* f4 = reinterpret_cast<const float4*>(field);
   float4 a[6], b[6], cc[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const size_t w = ((m >> k) & 1u) ? (size_t)((int64_t)v + off[k]) : v;
-    const float4* fv = reinterpret_cast<const float4*>(field + w * MARL_S_EXT);
-    a[k] = fv[0];
-    b[k] = fv[1];
-    cc[k] = fv[2];
+    a[k] = f4[w];
+    b[k] = f4[nvox + w];
+    cc[k] = f4[2 * nvox + w];
   }
   float acc[MARL_S_EXT];
 #pragma unroll
@@ -129,35 +99,14 @@ __device__ inline void gather_env6(const float* __restrict__ field, size_t v, ui
   for (int s = 0; s < MARL_S_EXT; ++s) E[s] = n ? acc[s] / (float)n : 0.0f;
 }
 
-// Split deltas into the free neighbours with the max(0) clamp
-// (spatial.rs:87-110, field.rs:57-62); lost when enclosed.
-__device__ inline void scatter_deltas(float* __restrict__ field, const size_t nv[6], int n, const float* D) {
-  if (n == 0) return;
-  const float nf = (float)n;
-  float split[MARL_S_EXT];
-#pragma unroll
-  for (int s = 0; s < MARL_S_EXT; ++s) split[s] = D[s] / nf;
-  for (int k = 0; k < n; ++k) {
-    float4* fv = reinterpret_cast<float4*>(field + nv[k] * MARL_S_EXT);
-    float4 a = fv[0], b = fv[1], cc = fv[2];
-    a.x = fmaxf(a.x + split[0], 0.0f); a.y = fmaxf(a.y + split[1], 0.0f);
-    a.z = fmaxf(a.z + split[2], 0.0f); a.w = fmaxf(a.w + split[3], 0.0f);
-    b.x = fmaxf(b.x + split[4], 0.0f); b.y = fmaxf(b.y + split[5], 0.0f);
-    b.z = fmaxf(b.z + split[6], 0.0f); b.w = fmaxf(b.w + split[7], 0.0f);
-    cc.x = fmaxf(cc.x + split[8], 0.0f); cc.y = fmaxf(cc.y + split[9], 0.0f);
-    cc.z = fmaxf(cc.z + split[10], 0.0f); cc.w = fmaxf(cc.w + split[11], 0.0f);
-    fv[0] = a; fv[1] = b; fv[2] = cc;
-  }
-}
-
-// The same scatter over the six faces by mask bit, without a neighbour list
-// (each free face is a different voxel, so the order of the updates does not
-// matter).
+// Split deltas into the free face neighbours with the max(0) clamp
+// (spatial.rs:87-110, field.rs:57-62), lost when enclosed.  Each free face is
+// a different voxel, so the order of the updates does not matter.
 __device__ inline void scatter_deltas6(float* __restrict__ field, size_t v, uint32_t m, const CellParams& p,
                                        const float* D) {
   const int n = __builtin_popcount(m);
   if (n == 0) return;
-  const size_t XY = (size_t)p.X * p.Y;
+  const size_t XY = (size_t)p.X * p.Y, nvox = XY * (size_t)p.Z;
   const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
   const float nf = (float)n;
   float split[MARL_S_EXT];
@@ -166,15 +115,15 @@ __device__ inline void scatter_deltas6(float* __restrict__ field, size_t v, uint
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     if (!((m >> k) & 1u)) continue;
-    float4* fv = reinterpret_cast<float4*>(field + (size_t)((int64_t)v + off[k]) * MARL_S_EXT);
-    float4 a = fv[0], b = fv[1], cc = fv[2];
+    float4* f4 = reinterpret_cast<float4*>(field) + (size_t)((int64_t)v + off[k]);
+    float4 a = f4[0], b = f4[nvox], cc = f4[2 * nvox];
     a.x = fmaxf(a.x + split[0], 0.0f); a.y = fmaxf(a.y + split[1], 0.0f);
     a.z = fmaxf(a.z + split[2], 0.0f); a.w = fmaxf(a.w + split[3], 0.0f);
     b.x = fmaxf(b.x + split[4], 0.0f); b.y = fmaxf(b.y + split[5], 0.0f);
     b.z = fmaxf(b.z + split[6], 0.0f); b.w = fmaxf(b.w + split[7], 0.0f);
     cc.x = fmaxf(cc.x + split[8], 0.0f); cc.y = fmaxf(cc.y + split[9], 0.0f);
     cc.z = fmaxf(cc.z + split[10], 0.0f); cc.w = fmaxf(cc.w + split[11], 0.0f);
-    fv[0] = a; fv[1] = b; fv[2] = cc;
+    f4[0] = a; f4[nvox] = b; f4[2 * nvox] = cc;
   }
 }
 

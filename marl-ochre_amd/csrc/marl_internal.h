@@ -1,7 +1,8 @@
 // marl_internal.h — shared declarations of the gfx950 MARL library (host + device).
 //
 // Device layout (HBM):
-//   field      SoA planes f32 [s][z][y][x], plane stride X*Y*Z, ping-pong pair
+//   field      species quads: three planes of float4 [g][z][y][x][4], plane g holding
+//              species 4g..4g+3 of every voxel (fq_index), ping-pong pair
 //   occupancy  1 bit per voxel, u32 words along x: word (z*Y + y)*XW + x/32, XW = X/32
 //   light      f32 [z][y][x]
 //   cells      AoS marl_cell_state (96 B) + marl_ruleset (576 B) in `cells` Vec order
@@ -10,11 +11,19 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stddef.h>
 
 #include <string>
 #include <vector>
 
 #include "../../include/marl_hip.h"
+
+// Species s of voxel v in the resident field: float index of the quad layout.
+// A voxel's four species of one quad are adjacent (one 16-byte load), and the
+// quad planes are nvox float4 apart.
+__host__ __device__ inline size_t fq_index(int s, size_t v, size_t nvox) {
+  return ((size_t)(s >> 2) * nvox + v) * 4 + (size_t)(s & 3);
+}
 
 #define MARL_VERSION "marl-ochre-amd 0.1.0 (gfx950)"
 
@@ -176,7 +185,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 // ---------------------------------------------------------------------------
 // One column of an ensemble as the batched field kernels see it.
 struct EnsColumn {
-  float* f[2];           // the column's field ping-pong buffers (SoA planes)
+  float* f[2];           // the column's field ping-pong buffers (species quads)
   const uint32_t* occ;   // its occupancy bits
   float* light;          // its light plane
   int X, Y, Z, XW;
@@ -293,7 +302,10 @@ int marl_graph_post(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick, mar
                     int32_t* host_finished);
 int marl_launch_occupancy_from_cells(marl_ctx* ctx);
 int marl_ensure_cell_capacity(marl_ctx* ctx, int64_t need);
-int marl_transpose(marl_ctx* ctx, bool to_soa, float* soa);
+// AoS (ctx->staging) <-> resident quads (dst/src)
+int marl_transpose(marl_ctx* ctx, bool to_quads, float* quads);
+// species planes (ctx->staging) <-> resident quads
+int marl_planes(marl_ctx* ctx, bool to_quads, float* quads);
 int marl_pack_occ(marl_ctx* ctx);
 int marl_unpack_occ(marl_ctx* ctx);
 int marl_ensure_staging(marl_ctx* ctx);

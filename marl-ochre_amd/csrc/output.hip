@@ -76,9 +76,11 @@ __global__ void __launch_bounds__(256) k_chem_profile(const float* __restrict__ 
   const int z = blockIdx.x, s = blockIdx.y;
   const size_t XY = (size_t)X * Y;
   const size_t nvox = XY * Z;
-  const float* plane = (s < MARL_S_EXT ? field + (size_t)s * nvox : light) + (size_t)z * XY;
+  // species s of the quad field sits every 4th float; light is dense
+  const float* plane = s < MARL_S_EXT ? field + fq_index(s, (size_t)z * XY, nvox) : light + (size_t)z * XY;
+  const size_t stride = s < MARL_S_EXT ? 4 : 1;
   double acc = 0.0;
-  for (size_t k = threadIdx.x; k < XY; k += blockDim.x) acc += (double)plane[k];
+  for (size_t k = threadIdx.x; k < XY; k += blockDim.x) acc += (double)plane[k * stride];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -112,7 +114,7 @@ __global__ void k_sample(const float* __restrict__ field, const float* __restric
   const int x = q[4 * k], y = q[4 * k + 1], z = q[4 * k + 2], s = q[4 * k + 3];
   const size_t nvox = (size_t)X * Y * Z;
   const size_t v = ((size_t)z * Y + y) * X + x;
-  out[k] = s < 0 ? light[v] : field[(size_t)s * nvox + v];
+  out[k] = s < 0 ? light[v] : field[fq_index(s, v, nvox)];
 }
 
 template <typename T>

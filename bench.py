@@ -487,8 +487,9 @@ def main():
     tf = ROOT / "profiles" / "stencil_traffic.json"
     if tf.exists():
         meas = json.loads(tf.read_text())
-        if meas.get("tile") == tile and tuple(meas.get("grid", ())) == dims:
-            traffic = meas.get("bytes_per_launch")
+        ent = meas.get("tiles", {}).get(str(tile))
+        if ent and tuple(meas.get("grid", ())) == dims:
+            traffic = ent.get("bytes_per_launch")
 
     out = {
         "metric": METRIC,

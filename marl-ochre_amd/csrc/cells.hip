@@ -1036,8 +1036,12 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
   MARL_HIP(ctx, hipGetLastError());
   MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));  // `off` must outlive the copy
   if (ev_sched) MARL_HIP(ctx, hipEventRecord(ev_sched, ctx->stream));
-  // the pass reads and writes the resident quad field in place
-  float* field = ctx->field[ctx->cur];
+  // the pass works on an AoS view of the field (the 48-byte voxel records keep
+  // a neighbour's 12 species in one or two cache lines; the resident layout is
+  // species quads, best for the stencil)
+  if (int rc = marl_ensure_staging(ctx)) return rc;
+  if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
+  float* field = ctx->staging;
   const HarvestSink hv = surrogate_harvest_sink(ctx, p);
   for (int L = 0; L < nlev; ++L) {
     const int cnt = off[L + 1] - off[L];
@@ -1054,6 +1058,7 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
     }
   }
   MARL_HIP(ctx, hipGetLastError());
+  if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
   if (levels_out) *levels_out = nlev;
   return MARL_OK;
 }
@@ -1208,11 +1213,13 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
   hipLaunchKernelGGL(k_g_level_offsets, dim3(1), dim3(1), 0, st, ts, hist, cursor, ctx->graph_levels);
   hipLaunchKernelGGL(k_g_level_scatter, dim3(gc), dim3(256), 0, st, ts, ctx->level, cursor, ctx->order);
   MARL_HIP(ctx, hipGetLastError());
-  // the ordered pass on the resident quad field
+  // the ordered pass on the AoS view
+  if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
   for (int L = 0; L < ctx->graph_levels; ++L)
     hipLaunchKernelGGL(k_g_kinetics, dim3(gk), dim3(kKinBlock), 0, st, L, ts, ctx->order, cells, rules,
-                       ctx->field[ctx->cur], ctx->light, ctx->nbmask, ctx->events, p);
+                       ctx->staging, ctx->light, ctx->nbmask, ctx->events, p);
   MARL_HIP(ctx, hipGetLastError());
+  if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
   // fate, births with mutation, deaths and compaction
   int32_t* target = ctx->flag_scan;
   int32_t* bflag = target + (cap + 1);
@@ -1251,7 +1258,7 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
   struct K {
     marl_sim_params sim;
     int64_t cap;
-    const void *cells, *rules, *f0, *f1;
+    const void *cells, *rules, *f0, *f1, *staging;
     int32_t tile, cur, cbuf, sweeps, levels;
   } k;
   memset(&k, 0, sizeof(k));
@@ -1261,6 +1268,7 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
   k.rules = ctx->rules[ctx->cbuf];
   k.f0 = ctx->field[0];
   k.f1 = ctx->field[1];
+  k.staging = ctx->staging;
   k.tile = ctx->tile;
   k.cur = ctx->cur;
   k.cbuf = ctx->cbuf;
@@ -1279,6 +1287,7 @@ int marl_graph_prepare(marl_ctx* ctx, const marl_sim_params* sim) {
     return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "the captured tick needs plain kinetics and no harvest");
   ctx->last_sim = *sim;
   if (int rc = ensure_voxel_buffers(ctx)) return rc;
+  if (int rc = marl_ensure_staging(ctx)) return rc;
   if (!ctx->tdev) MARL_HIP(ctx, hipMalloc(&ctx->tdev, sizeof(TickDev)));
   const int64_t n = ctx->n_cells;
   // room for this tick's births; generous so that growth rarely re-captures

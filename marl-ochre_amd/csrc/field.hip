@@ -33,7 +33,7 @@ const TileShape kTiles[] = {
     // split blocks: the 4-wave groups of one block share the tile's LDS plane,
     // each group taking 6 (b64 access) or 4 (one quad, b128 access) species
     {2, 16, 1, 6, 1, 8}, {2, 32, 1, 6, 1, 8}, {1, 32, 1, 4, 1, 12}, {2, 16, 1, 4, 1, 12}, {2, 32, 1, 4, 1, 12},
-    {1, 64, 1, 4, 1, 12}, {1, 32, 2, 4, 1, 12}, {2, 64, 1, 4, 1, 12},
+    {1, 64, 1, 4, 1, 12}, {1, 32, 2, 4, 1, 12}, {2, 64, 1, 4, 1, 12}, {1, 128, 1, 4, 1, 12}, {2, 128, 1, 4, 1, 12},
     // 6 species per 4-wave block: the group without species 7 reads it for
     // the niche factor from HBM
     {2, 16, 1, 6, 1, 4},
@@ -952,18 +952,21 @@ int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim) {
 // Layout kernels.  AoS (the boundary's field.rs layout, 12 floats per voxel)
 // and the resident quads hold the same 16-byte groups, so the conversion is a
 // float4 permutation; species planes (SoA) go through per-float moves.
-__global__ void k_aos_to_quad(const float4* __restrict__ aos, float4* __restrict__ q, size_t nvox) {
-  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nvox) return;
-#pragma unroll
-  for (int g = 0; g < 3; ++g) q[(size_t)g * nvox + v] = aos[3 * v + g];
+// One float4 per thread, in AoS order: the AoS side is a linear stream and the
+// quad side three interleaved runs (one per plane) of consecutive voxels.
+// 3 * nvox < 2^32 (the field fits 2 GiB).
+__global__ void k_aos_to_quad(const float4* __restrict__ aos, float4* __restrict__ q, uint32_t nvox) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3u * nvox) return;
+  const uint32_t v = i / 3u, g = i - 3u * v;
+  q[(size_t)g * nvox + v] = aos[i];
 }
 
-__global__ void k_quad_to_aos(const float4* __restrict__ q, float4* __restrict__ aos, size_t nvox) {
-  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nvox) return;
-#pragma unroll
-  for (int g = 0; g < 3; ++g) aos[3 * v + g] = q[(size_t)g * nvox + v];
+__global__ void k_quad_to_aos(const float4* __restrict__ q, float4* __restrict__ aos, uint32_t nvox) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3u * nvox) return;
+  const uint32_t v = i / 3u, g = i - 3u * v;
+  aos[i] = q[(size_t)g * nvox + v];
 }
 
 __global__ void k_soa_to_quad(const float* __restrict__ soa, float4* __restrict__ q, size_t nvox) {
@@ -1014,8 +1017,8 @@ __global__ void k_unpack_occ(const uint32_t* __restrict__ words, uint8_t* __rest
 }
 
 int marl_transpose(marl_ctx* ctx, bool to_quads, float* quads) {
-  const size_t n = ctx->nvox;
-  const dim3 g((unsigned)((n + 255) / 256));
+  const uint32_t n = (uint32_t)ctx->nvox;
+  const dim3 g((3u * n + 255u) / 256u);
   float4* q = reinterpret_cast<float4*>(quads);
   float4* a = reinterpret_cast<float4*>(ctx->staging);
   if (to_quads)

@@ -66,21 +66,21 @@ __device__ __forceinline__ marl_cell_state load_cell(const marl_cell_state* c) {
 
 // Mean environment over the free face neighbours (spatial.rs:56-77; faces in
 // FACE_OFFSETS order +x,-x,+y,-y,+z,-z, spatial.rs:18-43) from the resident
-// quad field: all six face voxels are requested at once (non-free faces read
-// the cell's own voxel and are ignored), the sum runs over the free faces in
-// face order, then divides; zeros when enclosed.
+// AoS field (one 48-byte record per voxel): all six face records are requested
+// at once (non-free faces read the cell's own voxel and are ignored), the sum
+// runs over the free faces in face order, then divides; zeros when enclosed.
 __device__ inline void gather_env6(const float* __restrict__ field, size_t v, uint32_t m, const CellParams& p,
                                    float* E) {
-  const size_t XY = (size_t)p.X * p.Y, nvox = XY * (size_t)p.Z;
+  const size_t XY = (size_t)p.X * p.Y;
   const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
-  const float4* f4 = reinterpret_cast<const float4*>(field);
   float4 a[6], b[6], cc[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const size_t w = ((m >> k) & 1u) ? (size_t)((int64_t)v + off[k]) : v;
-    a[k] = f4[w];
-    b[k] = f4[nvox + w];
-    cc[k] = f4[2 * nvox + w];
+    const float4* fv = reinterpret_cast<const float4*>(field + w * MARL_S_EXT);
+    a[k] = fv[0];
+    b[k] = fv[1];
+    cc[k] = fv[2];
   }
   float acc[MARL_S_EXT];
 #pragma unroll
@@ -106,7 +106,7 @@ __device__ inline void scatter_deltas6(float* __restrict__ field, size_t v, uint
                                        const float* D) {
   const int n = __builtin_popcount(m);
   if (n == 0) return;
-  const size_t XY = (size_t)p.X * p.Y, nvox = XY * (size_t)p.Z;
+  const size_t XY = (size_t)p.X * p.Y;
   const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
   const float nf = (float)n;
   float split[MARL_S_EXT];
@@ -115,15 +115,15 @@ __device__ inline void scatter_deltas6(float* __restrict__ field, size_t v, uint
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     if (!((m >> k) & 1u)) continue;
-    float4* f4 = reinterpret_cast<float4*>(field) + (size_t)((int64_t)v + off[k]);
-    float4 a = f4[0], b = f4[nvox], cc = f4[2 * nvox];
+    float4* fv = reinterpret_cast<float4*>(field + (size_t)((int64_t)v + off[k]) * MARL_S_EXT);
+    float4 a = fv[0], b = fv[1], cc = fv[2];
     a.x = fmaxf(a.x + split[0], 0.0f); a.y = fmaxf(a.y + split[1], 0.0f);
     a.z = fmaxf(a.z + split[2], 0.0f); a.w = fmaxf(a.w + split[3], 0.0f);
     b.x = fmaxf(b.x + split[4], 0.0f); b.y = fmaxf(b.y + split[5], 0.0f);
     b.z = fmaxf(b.z + split[6], 0.0f); b.w = fmaxf(b.w + split[7], 0.0f);
     cc.x = fmaxf(cc.x + split[8], 0.0f); cc.y = fmaxf(cc.y + split[9], 0.0f);
     cc.z = fmaxf(cc.z + split[10], 0.0f); cc.w = fmaxf(cc.w + split[11], 0.0f);
-    f4[0] = a; f4[nvox] = b; f4[2 * nvox] = cc;
+    fv[0] = a; fv[1] = b; fv[2] = cc;
   }
 }
 

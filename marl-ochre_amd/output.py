@@ -278,3 +278,300 @@ def z_profile_text(gpu) -> str:
         l, ox, re, ca, og = v[z]
         lines.append(f"{z:>3} {int(zc[z]):>6} {l:>6.3f} {ox:>6.3f} {re:>6.3f} {ca:>6.3f} {og:>6.3f}")
     return "\n".join(lines) + "\n"
+
+
+# ---------------------------------------------------------------------------
+# Cell and reaction CSV snapshots (crates/marl-output/src/data.rs:18-130, 347-466)
+
+def rust_f32(v) -> str:
+    """Rust's `Display` of an f32: the shortest decimal that reads back as the
+    same f32, never in exponent form."""
+    return np.format_float_positional(np.float32(v), unique=True, trim="-")
+
+
+def active_reactions(rules) -> np.ndarray:
+    """Reactions with |v_max| > 1e-9 per ruleset (the f32 comparison of data.rs:219-224)."""
+    return (np.abs(rules["reactions"]["v_max"]) > np.float32(1e-9)).sum(axis=1)
+
+
+class ReactionRegistry:
+    """Reaction topology (substrate, product, catalyst, cofactor) -> id in
+    first-seen order (≙ ReactionRegistry, data.rs:40-130)."""
+
+    def __init__(self):
+        self.ids = {}
+
+    def register(self, topo) -> int:
+        return self.ids.setdefault(tuple(int(t) for t in topo), len(self.ids))
+
+    def count(self) -> int:
+        return len(self.ids)
+
+    def write(self, out_dir) -> Path:
+        path = Path(out_dir) / "reaction_registry.csv"
+        lines = ["reaction_id,substrate,product,catalyst,cofactor"]
+        lines += [f"{i},{a},{b},{c},{d}" for (a, b, c, d), i in sorted(self.ids.items(), key=lambda kv: kv[1])]
+        path.write_text("\n".join(lines) + "\n")
+        return path
+
+
+def cells_csv(cells, rules) -> str:
+    """cells_<T>.csv rows (data.rs:347-395): internals with {:.6}."""
+    act = active_reactions(rules)
+    lines = ["x,y,z,energy,enzyme_a,enzyme_b,active_rxns,lineage_id,age,quiescent,starter_type"]
+    for c, a in zip(cells, act):
+        i = c["internal"]
+        lines.append(f"{c['pos'][0]},{c['pos'][1]},{c['pos'][2]},{float(i[0]):.6f},{float(i[5]):.6f},"
+                     f"{float(i[6]):.6f},{int(a)},{int(c['lineage_id'])},{int(c['age'])},{int(c['quiescent'] != 0)},"
+                     f"{int(c['starter_type'])}")
+    return "\n".join(lines) + "\n"
+
+
+def snapshot_cells(cells, rules, tick: int, out_dir) -> Path:
+    path = Path(out_dir) / f"cells_{tick}.csv"
+    path.write_text(cells_csv(cells, rules))
+    return path
+
+
+def reactions_csv(cells, rules, registry: ReactionRegistry) -> str:
+    """reactions_<T>.csv rows (data.rs:399-433): active reactions' registry ids,
+    registered in cell order then slot order."""
+    rx = rules["reactions"]
+    lines = ["x,y,z,starter_type,lineage_id,reaction_ids"]
+    for k, c in enumerate(cells):
+        ids = [registry.register((r["substrate"], r["product"], r["catalyst"], r["cofactor"]))
+               for r in rx[k] if abs(np.float32(r["v_max"])) > np.float32(1e-9)]
+        lines.append(f"{c['pos'][0]},{c['pos'][1]},{c['pos'][2]},{int(c['starter_type'])},{int(c['lineage_id'])},"
+                     + ";".join(str(i) for i in ids))
+    return "\n".join(lines) + "\n"
+
+
+def snapshot_reactions(cells, rules, tick: int, out_dir, registry: ReactionRegistry) -> Path:
+    path = Path(out_dir) / f"reactions_{tick}.csv"
+    path.write_text(reactions_csv(cells, rules, registry))
+    return path
+
+
+# ---------------------------------------------------------------------------
+# Ruleset dumps (crates/marl-output/src/binary_dump.rs:122-328; record layouts
+# crates/marl-format/src/lib.rs:52-93)
+
+RULESET_FULL_MAGIC = b"MRSF"
+RULESET_FULL_FORMAT_VERSION = 1
+RULESET_FULL_HEADER_SIZE = 24
+RULESET_FULL_CELL_REF_STRIDE = 10
+RULESET_FULL_CANONICAL_SIZE = 536
+
+
+def _canonical_dtype():
+    names, formats, offsets = [], [], []
+    off = 0
+
+    def add(name, fmt, size):
+        nonlocal off
+        names.append(name); formats.append(fmt); offsets.append(off)
+        off += size
+    for k in range(8):
+        add(f"rc{k}", ("<f4", 3), 12)
+    for k in range(8):
+        add(f"tr{k}", ("<f4", 2), 8); add(f"tr{k}_sp", ("u1", 2), 2)
+    for k in range(16):
+        add(f"rx{k}_w", ("u1", 4), 4); add(f"rx{k}", ("<f4", 3), 12)
+    for k in range(8):
+        add(f"ef{k}", ("<f4", 2), 8); add(f"ef{k}_sp", ("u1", 2), 2)
+    add("fate", ("<f4", 4), 16)
+    add("tail", ("<f4", 2), 8)
+    assert off == RULESET_FULL_CANONICAL_SIZE
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets, "itemsize": off})
+
+
+CANONICAL_DTYPE = _canonical_dtype()
+
+
+def canonical_rulesets(rules) -> np.ndarray:
+    """Each ruleset's 536-byte canonical record (ruleset_to_canonical_bytes,
+    binary_dump.rs:188-240): receptors {k_half, n_hill, gain}, transport
+    {uptake, secrete, ext, int}, reactions {substrate, product, catalyst,
+    cofactor, k_m, v_max, k_cat}, effectors {threshold, rate, int, ext}, fate,
+    hgt_propensity, mutation_rate — little-endian, packed."""
+    n = len(rules)
+    r = np.zeros(n, CANONICAL_DTYPE)
+    for k in range(8):
+        rc = rules["receptors"][:, k]
+        r[f"rc{k}"] = np.stack([rc["k_half"], rc["n_hill"], rc["gain"]], 1)
+        t = rules["transport"][:, k]
+        r[f"tr{k}"] = np.stack([t["uptake_rate"], t["secrete_rate"]], 1)
+        r[f"tr{k}_sp"] = np.stack([t["ext_species"], t["int_species"]], 1)
+        e = rules["effectors"][:, k]
+        r[f"ef{k}"] = np.stack([e["threshold"], e["rate"]], 1)
+        r[f"ef{k}_sp"] = np.stack([e["int_species"], e["ext_species"]], 1)
+    for k in range(16):
+        x = rules["reactions"][:, k]
+        r[f"rx{k}_w"] = np.stack([x["substrate"], x["product"], x["catalyst"], x["cofactor"]], 1)
+        r[f"rx{k}"] = np.stack([x["k_m"], x["v_max"], x["k_cat"]], 1)
+    r["fate"] = np.stack([rules["division_energy"], rules["death_energy"], rules["quiescence_energy"],
+                          rules["division_prep_ticks"]], 1)
+    r["tail"] = np.stack([rules["hgt_propensity"], rules["mutation_rate"]], 1)
+    return r
+
+
+def _layer_values(records) -> np.ndarray:
+    """The 110 averaged f32 values of a ruleset, in append_ruleset_layer_values
+    order (binary_dump.rs:122-152), as an [n, 110] array."""
+    cols = [records[f"rc{k}"] for k in range(8)] + [records[f"tr{k}"] for k in range(8)] + \
+        [records[f"rx{k}"] for k in range(16)] + [records[f"ef{k}"] for k in range(8)] + \
+        [records["fate"], records["tail"]]
+    return np.concatenate(cols, axis=1)
+
+
+def ruleset_layer_payload(records, z_of_cell, grid_z: int) -> bytes:
+    """Per z: z:u16, 0:u16, count:u32, then the 110 layer means as f32 (FP64
+    sums in cell order, divided, rounded to f32; zeros for an empty layer)."""
+    vals = _layer_values(records).astype(np.float64)
+    z = np.asarray(z_of_cell, np.int64)
+    counts = np.bincount(z, minlength=grid_z)[:grid_z]
+    out = bytearray()
+    sums = np.stack([np.bincount(z, weights=vals[:, k], minlength=grid_z)[:grid_z] for k in range(vals.shape[1])], 1) \
+        if len(z) else np.zeros((grid_z, vals.shape[1]))
+    for zz in range(grid_z):
+        c = int(counts[zz])
+        out += np.array([zz, 0], "<u2").tobytes() + np.array([c], "<u4").tobytes()
+        avg = (sums[zz] / float(c)).astype("<f4") if c else np.zeros(vals.shape[1], "<f4")
+        out += avg.tobytes()
+    return bytes(out)
+
+
+def ruleset_full_payload(records, pos) -> bytes:
+    """MRSF dump: 24-byte header, the distinct canonical rulesets in first-seen
+    order, then per cell (x, y, z: u16, dict_id: u32)."""
+    n = len(records)
+    raw = np.ascontiguousarray(records).view(np.dtype((np.void, RULESET_FULL_CANONICAL_SIZE)))
+    if n:
+        _, first, inverse = np.unique(raw, return_index=True, return_inverse=True)
+        order = np.argsort(first, kind="stable")      # unique rows by first appearance
+        rank = np.empty_like(order)
+        rank[order] = np.arange(len(order))
+        dict_id = rank[inverse.reshape(-1)]
+        dictionary = records[np.sort(first)]
+    else:
+        dict_id = np.zeros(0, np.int64)
+        dictionary = records[:0]
+    hdr = RULESET_FULL_MAGIC + np.array([RULESET_FULL_FORMAT_VERSION, 0, len(dictionary), n,
+                                         RULESET_FULL_CANONICAL_SIZE], "<u4").tobytes()
+    refs = np.zeros(n, np.dtype({"names": ["pos", "id"], "formats": [("<u2", 3), "<u4"], "offsets": [0, 6],
+                                 "itemsize": RULESET_FULL_CELL_REF_STRIDE}))
+    refs["pos"] = pos
+    refs["id"] = dict_id
+    return hdr + dictionary.tobytes() + refs.tobytes()
+
+
+def write_ruleset_layer_dump(rules, pos, grid_z: int, tick: int, out_dir, compression="none", level=3) -> Path:
+    Path(out_dir).mkdir(parents=True, exist_ok=True)
+    path = binary_file_path(out_dir, tick, "ruleset_layers", compression)
+    _write_payload(path, ruleset_layer_payload(canonical_rulesets(rules), np.asarray(pos)[:, 2], grid_z),
+                   compression, level)
+    return path
+
+
+def write_ruleset_full_dump(rules, pos, tick: int, out_dir, compression="none", level=3) -> Path:
+    Path(out_dir).mkdir(parents=True, exist_ok=True)
+    path = binary_file_path(out_dir, tick, "rulesets", compression)
+    _write_payload(path, ruleset_full_payload(canonical_rulesets(rules), pos), compression, level)
+    return path
+
+
+# ---------------------------------------------------------------------------
+# summary.md (≙ DataLogger::write_summary, crates/marl-output/src/data.rs:467-800)
+
+def summary_text(dims, total_ticks: int, runtime_s: float, summary: dict, z_counts, column: dict,
+                 total_div: int, total_death: int, sim) -> str:
+    """The Markdown report.  `summary`/`z_counts` as from MarlGpu.cell_summary;
+    `column` holds the centre-column reads: oxidant/reductant (all z), carbon,
+    waste and light at the surface, middle and deep planes."""
+    X, Y, Z = dims
+    vox = X * Y * Z
+    pop = int(summary["n_cells"])
+    pop_pct = pop / vox * 100.0 if vox > 0 else 0.0
+    rt = float(np.float32(runtime_s))  # the reference carries the runtime as f32
+    tps = total_ticks / rt if rt > 0.0 else 0.0
+    L = ["# Run Summary", "", "## Parameters", "",
+         f"- Grid: {X}x{Y}x{Z} ({vox} voxels)", f"- Ticks: {total_ticks}",
+         f"- Runtime: {rt:.1f}s ({tps:.2f} ticks/sec)", f"- Maintenance rate: {rust_f32(sim.lambda_maintenance)}",
+         f"- Boundary sources: oxidant={rust_f32(sim.source_rate_oxidant)}, carbon={rust_f32(sim.source_rate_carbon)}, "
+         f"reductant={rust_f32(sim.source_rate_reductant)}", "",
+         "## Population", "", f"- Final population: {pop} ({pop_pct:.1f}% capacity)",
+         f"- Total divisions: {total_div}", f"- Total deaths: {total_death}"]
+    turnover = total_death / total_div if total_div > 0 else 0.0
+    L += [f"- Turnover ratio: {turnover:.2f}", "", "## Vertical Zonation", "", "Per-layer cell counts:", "",
+          "| z | cells |", "|---|-------|"]
+    zc = [int(c) for c in z_counts]
+    L += [f"| {z} | {zc[z]} |" for z in range(Z)]
+    third, two = Z // 3, 2 * Z // 3
+    surf, mid, deep = sum(zc[:third]), sum(zc[third:two]), sum(zc[two:])
+    L += ["", f"- Surface zone (z < {third}): {surf} cells", f"- Middle zone ({third} <= z < {two}): {mid} cells",
+          f"- Deep zone (z >= {two}): {deep} cells"]
+    zmax, zmin = max(surf, mid, deep), min(surf, mid, deep)
+    ratio = zmax / zmin if zmin > 0 else (float("inf") if zmax > 0 else 1.0)
+    L += [f"- Zonation ratio (max_zone / min_zone): {ratio:.1f}", "", "## Chemical Gradients (center column)", ""]
+    ox, red = column["oxidant"], column["reductant"]
+    c3, w4, lt = column["carbon"], column["waste"], column["light"]
+    zm = Z // 2
+    L += [f"- Oxidant: surface={ox[0]:.3f}, mid={ox[zm]:.3f}, deep={ox[Z - 1]:.3f}",
+          f"- Reductant: surface={red[0]:.3f}, mid={red[zm]:.3f}, deep={red[Z - 1]:.3f}",
+          f"- Carbon: surface={c3[0]:.3f}, mid={c3[1]:.3f}, deep={c3[2]:.3f}",
+          f"- Organic waste: surface={w4[0]:.3f}, mid={w4[1]:.3f}, deep={w4[2]:.3f}"]
+    f001 = np.float32(0.01)
+    ox_pen = next((z for z in range(Z) if np.float32(ox[z]) < f001), Z)
+    red_pen = next((z for z in reversed(range(Z)) if np.float32(red[z]) < f001), 0)
+    L += [f"- Oxidant penetration depth: z={ox_pen} (first z where oxidant < 0.01)",
+          f"- Reductant penetration depth: z={red_pen} (first z from bottom where reductant < 0.01)",
+          f"- Light: surface={lt[0]:.3f}, mid={lt[1]:.3f}, deep={lt[2]:.3f}", "", "## Metabolic Diversity", ""]
+    if pop > 0:
+        L += [f"- Average active reactions: {summary['sum_active_reactions'] / pop:.1f}",
+              f"- Average energy: {summary['sum_energy'] / pop:.3f}",
+              f"- Average enzyme_A (internal[5]): {summary['sum_enzyme_a'] / pop:.4f}",
+              f"- Average enzyme_B (internal[6]): {summary['sum_enzyme_b'] / pop:.4f}"]
+    else:
+        L += ["- (no living cells — all metrics are zero)"]
+    L += ["", "## Assessment", ""]
+    if pop_pct > 80.0:
+        L.append("- **Grid saturated** — consider larger grid or higher maintenance")
+    elif pop_pct < 5.0:
+        L.append("- **Population sparse** — dynamics may be resource-limited")
+    else:
+        L.append(f"- Population at {pop_pct:.1f}% capacity — within normal operating range")
+    if ratio > 2.0:
+        L.append(f"- **Clear vertical zonation detected** (ratio {ratio:.1f}x)")
+    else:
+        L.append(f"- Weak or no vertical zonation (ratio {ratio:.1f}x)")
+    if ox_pen < Z / 2.0:
+        L.append(f"- **Anoxic zone established** — oxidant penetrates to z={ox_pen}")
+    else:
+        L.append(f"- No clear anoxic zone — oxidant penetrates to z={ox_pen}")
+    if 0.0 < turnover < 0.5:
+        L.append(f"- Low turnover ({turnover:.2f}) — population is growing")
+    elif turnover > 1.5:
+        L.append(f"- High turnover ({turnover:.2f}) — population is declining")
+    elif total_div > 0:
+        L.append(f"- Turnover near unity ({turnover:.2f}) — approximate steady state")
+    L.append("")
+    return "\n".join(L) + "\n"
+
+
+def centre_column(gpu) -> dict:
+    X, Y, Z = gpu.dims
+    cx, cy = X // 2, Y // 2
+    q = [(cx, cy, z, 1) for z in range(Z)] + [(cx, cy, z, 2) for z in range(Z)]
+    depths = (0, Z // 2, Z - 1)
+    q += [(cx, cy, z, 3) for z in depths] + [(cx, cy, z, 4) for z in depths] + [(cx, cy, z, -1) for z in depths]
+    v = gpu.sample(q)
+    return {"oxidant": v[:Z], "reductant": v[Z:2 * Z], "carbon": v[2 * Z:2 * Z + 3],
+            "waste": v[2 * Z + 3:2 * Z + 6], "light": v[2 * Z + 6:2 * Z + 9]}
+
+
+def write_summary(gpu, out_dir, total_ticks: int, runtime_s: float, total_div: int, total_death: int, sim) -> Path:
+    summary, zc = gpu.cell_summary()
+    path = Path(out_dir) / "summary.md"
+    path.write_text(summary_text(gpu.dims, total_ticks, runtime_s, summary, zc, centre_column(gpu), total_div,
+                                 total_death, sim))
+    return path

@@ -223,6 +223,17 @@ int marl_seed_positions(const marl_dims* dims, uint64_t seed, uint64_t band, int
                         int32_t z_lo, int32_t z_hi, int32_t* cell_at, uint16_t* pos_out, uint64_t* lineage_out,
                         int64_t index_base, int64_t* placed);
 
+/* ≙ hgt::transfer_reaction (crates/marl-cell/src/hgt.rs:7-27), which the
+ * reference's run() never calls: for each pair k of `pairs` (n_pairs x
+ * {donor, recipient} cell indices into the resident population) a donor
+ * reaction with |v_max| > 1e-9, drawn uniformly, overwrites the recipient's
+ * first reaction with |v_max| < 1e-9, or a drawn slot when it has none.  Pair k
+ * draws from the Philox stream (seed, 3<<32), entity k: draw 0 picks the donor
+ * reaction, draw 1 the slot.  The pairs are independent transfers: recipients
+ * must be distinct and no recipient may be a donor (else INVALID_INPUT, state
+ * untouched). */
+int marl_hgt_transfer(marl_ctx* ctx, const int32_t* pairs, size_t n_pairs, uint64_t seed);
+
 /* ---- timing helpers (hipEvents on the context's stream) ------------------ */
 
 /* Launch n substeps back to back on the context stream and return the elapsed

@@ -1395,6 +1395,28 @@ int check_cells_host(marl_ctx* ctx, const marl_cell_state* cells, size_t n) {
 
 }  // namespace
 
+// hgt::transfer_reaction (crates/marl-cell/src/hgt.rs:7-27), one pair per
+// thread; the pairs are independent (checked on the host).
+__global__ void k_hgt(const int32_t* __restrict__ pairs, int np, marl_ruleset* __restrict__ rules, uint64_t seed) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const marl_ruleset& d = rules[pairs[2 * k]];
+  marl_ruleset& r = rules[pairs[2 * k + 1]];
+  int active[MARL_R_MAX];
+  int na = 0;
+  for (int q = 0; q < MARL_R_MAX; ++q)
+    if (fabsf(d.reactions[q].v_max) > 1e-9f) active[na++] = q;
+  if (na == 0) return;
+  MarlRng g;
+  g.init(seed, MARL_TAG_HGT << 32, (uint64_t)k);
+  const int src = active[g.range((uint32_t)na)];
+  int tgt = -1;
+  for (int q = 0; q < MARL_R_MAX && tgt < 0; ++q)
+    if (fabsf(r.reactions[q].v_max) < 1e-9f) tgt = q;
+  if (tgt < 0) tgt = (int)g.range(MARL_R_MAX);
+  r.reactions[tgt] = d.reactions[src];
+}
+
 extern "C" {
 
 int marl_upload_cells(marl_ctx* ctx, const marl_cell_state* cells, const marl_ruleset* rules, size_t n) {
@@ -1554,6 +1576,35 @@ int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, in
   }
   if (stats) *stats = st;
   if (host_finished) *host_finished = finished;
+  return MARL_OK;
+}
+
+int marl_hgt_transfer(marl_ctx* ctx, const int32_t* pairs, size_t n_pairs, uint64_t seed) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  if (n_pairs == 0) return MARL_OK;
+  if (!pairs) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "null pairs");
+  const int64_t n = ctx->n_cells;
+  std::vector<uint8_t> role((size_t)n, 0);  // 1 = donor, 2 = recipient
+  for (size_t k = 0; k < n_pairs; ++k) {
+    const int32_t d = pairs[2 * k], r = pairs[2 * k + 1];
+    if (d < 0 || d >= n || r < 0 || r >= n) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "hgt pair out of range");
+    if (d == r) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "hgt donor equals recipient");
+    if (role[r] != 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "hgt recipient repeated or also a donor");
+    role[r] = 2;
+  }
+  for (size_t k = 0; k < n_pairs; ++k)
+    if (role[pairs[2 * k]] == 2) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "hgt recipient repeated or also a donor");
+  int32_t* dp = nullptr;
+  MARL_HIP(ctx, hipMalloc((void**)&dp, sizeof(int32_t) * 2 * n_pairs));
+  hipError_t e = hipMemcpyAsync(dp, pairs, sizeof(int32_t) * 2 * n_pairs, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_hgt, dim3((unsigned)((n_pairs + 255) / 256)), dim3(256), 0, ctx->stream, dp, (int)n_pairs,
+                       ctx->rules[ctx->cbuf], seed);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(dp);
+  MARL_HIP(ctx, e);
   return MARL_OK;
 }
 

@@ -91,6 +91,7 @@ __host__ __device__ inline float marl_logf(float x) {
 
 #define MARL_TAG_SEEDING 1ull
 #define MARL_TAG_DIVISION 2ull
+#define MARL_TAG_HGT 3ull
 #define MARL_TAG_SCENARIO 3ull
 
 __host__ __device__ inline void marl_mulhilo64(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {

@@ -245,6 +245,11 @@ class MarlGpu:
         self._chk(self.lib.marl_download_viewer_cells(self.h, ptr(buf), buf.size, ctypes.byref(got)))
         return buf[: got.value * 25].tobytes()
 
+    def hgt_transfer(self, pairs, seed: int) -> None:
+        """hgt::transfer_reaction for each (donor, recipient) row of pairs."""
+        pr = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        self._chk(self.lib.marl_hgt_transfer(self.h, ptr(pr), len(pr), int(seed)))
+
     def sample(self, queries) -> np.ndarray:
         """Point reads: rows (x, y, z, species), species -1 = light."""
         q = np.ascontiguousarray(np.asarray(queries, np.int32).reshape(-1, 4))

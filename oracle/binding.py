@@ -55,6 +55,7 @@ class Oracle:
             "orc_cell_tick": (ctypes.c_int, [P, P, P, f32, SP, P]),
             "orc_mutate": (None, [P, SP, u64, u64, u64, P]),
             "orc_normal": (f32, [u64, u64, u64, P, f32]),
+            "orc_hgt_transfer": (None, [P, P, i64, u64]),
             "orc_find_empty_neighbor": (ctypes.c_int, [P, P, D, SP, u64, u64, u64, P, P]),
             "orc_compact_indices": (i64, [i64, P, i64, P]),
             "orc_tick": (ctypes.c_int, [D, SP, u64, P, P, P, P, P, P, i64, P, P, ctypes.c_int]),
@@ -173,6 +174,15 @@ class Oracle:
         s, _keep = self._sim(sim)
         self.L.orc_mutate(self._p(r), s, seed, stream, entity, self._p(dr))
         return r[0], int(dr[0])
+
+    def hgt_transfer(self, rules, pairs, seed):
+        """Rulesets after hgt::transfer_reaction for each (donor, recipient) pair."""
+        src = np.ascontiguousarray(rules, dtype=self.A.RULESET_DTYPE)
+        r = np.empty_like(src)
+        r.view(np.uint8)[:] = src.view(np.uint8)  # byte copy: padding included
+        pr = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        self.L.orc_hgt_transfer(self._p(r), self._p(pr), len(pr), seed)
+        return r
 
     def compact_indices(self, n, deaths):
         d = np.ascontiguousarray(sorted(deaths), np.int64)

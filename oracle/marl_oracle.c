@@ -750,3 +750,26 @@ int64_t orc_seed_positions(const marl_dims* d, uint64_t seed, uint64_t band, int
   }
   return seeded;
 }
+
+/* ------------------------------------------------------------------------ */
+/* hgt::transfer_reaction (crates/marl-cell/src/hgt.rs:7-27): a donor's active
+ * reaction (|v_max| > 1e-9), drawn uniformly, overwrites the recipient's first
+ * inactive slot (|v_max| < 1e-9), or a drawn slot when none is inactive.      */
+void orc_hgt_transfer(marl_ruleset* rules, const int32_t* pairs, int64_t n_pairs, uint64_t seed) {
+  for (int64_t k = 0; k < n_pairs; ++k) {
+    const marl_ruleset* d = &rules[pairs[2 * k]];
+    marl_ruleset* r = &rules[pairs[2 * k + 1]];
+    int active[MARL_R_MAX], na = 0;
+    for (int q = 0; q < MARL_R_MAX; ++q)
+      if (fabsf(d->reactions[q].v_max) > 1e-9f) active[na++] = q;
+    if (na == 0) continue;
+    uint64_t draws = 0;
+    rngs rg = {seed, ORC_TAG_HGT << 32, (uint64_t)k, &draws};
+    const int src = active[rs_range(&rg, (uint32_t)na)];
+    int tgt = -1;
+    for (int q = 0; q < MARL_R_MAX && tgt < 0; ++q)
+      if (fabsf(r->reactions[q].v_max) < 1e-9f) tgt = q;
+    if (tgt < 0) tgt = (int)rs_range(&rg, MARL_R_MAX);
+    r->reactions[tgt] = d->reactions[src];
+  }
+}

@@ -23,6 +23,10 @@
  *   mutate                draw-count and clamp invariants only — parity unpinned
  *                         against the reference (its rand 0.9.2 ThreadRng stream
  *                         is OS-seeded, crates/marl-sim/src/lib.rs:35)
+ *   hgt transfer          slot rules (first inactive slot, else a drawn one),
+ *                         donor untouched, the v_max thresholds, the pick's
+ *                         uniformity over active reactions (chi-square) and its
+ *                         widening-multiply mapping of the drawn u32
  */
 #ifndef MARL_ORACLE_H
 #define MARL_ORACLE_H
@@ -38,6 +42,7 @@ extern "C" {
 /* RNG stream tags (key[1] = tag << 32 | sub-stream). */
 #define ORC_TAG_SEEDING 1ull
 #define ORC_TAG_DIVISION 2ull
+#define ORC_TAG_HGT 3ull
 
 void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
 uint32_t orc_draw_u32(uint64_t seed, uint64_t stream, uint64_t entity, uint64_t draw);
@@ -76,6 +81,11 @@ int orc_find_empty_neighbor(const uint16_t pos[3], const int32_t* cell_at, const
 void orc_mutate(marl_ruleset* r, const marl_sim_params* sim, uint64_t seed, uint64_t stream,
                 uint64_t entity, uint64_t* draws);
 float orc_normal(uint64_t seed, uint64_t stream, uint64_t entity, uint64_t* draws, float sd);
+
+/* hgt::transfer_reaction (crates/marl-cell/src/hgt.rs:7-27) for each pair k of
+ * pairs = (donor, recipient) cell indices, in order; pair k draws from entity
+ * k of stream ORC_TAG_HGT << 32 under key seed. */
+void orc_hgt_transfer(marl_ruleset* rules, const int32_t* pairs, int64_t n_pairs, uint64_t seed);
 
 /* swap_remove compaction of deaths (sorted ascending, distinct), sequential
  * emulation; writes the surviving original indices in final order to perm. */

@@ -100,8 +100,25 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
             graph_runs.append((wall, st))
     for (_, a), (_, b) in zip(runs, graph_runs):
         assert all(a[k] == b[k] for k in ("divisions", "deaths", "births_added", "n_cells")), (a, b)
+    # run outputs on the 1M-cell state: GPU reductions / packing incl. the readback
     g.upload_field_aos(field)
     g.upload_cells(cells, rules)
+    g.sync()
+    outputs = {}
+    for name, fn in (("cell_summary", g.cell_summary), ("chemistry_profile", g.chemistry_profile),
+                     ("viewer_records", g.download_viewer_cells)):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            fn()
+        outputs[f"ms_{name}"] = round((time.perf_counter() - t0) * 1e3 / 5, 3)
+    c0 = cells["internal"].astype(np.float64)
+    t0 = time.perf_counter()
+    for k in (0, 5, 6):
+        np.cumsum(c0[:, k])  # the reference's sequential FP64 sums (data.rs:213-240), NumPy on one core
+    (np.abs(rules["reactions"]["v_max"]) > np.float32(1e-9)).sum()
+    np.bincount(cells["pos"][:, 2].astype(np.int64), minlength=dims[2])
+    outputs["ms_cell_summary_numpy_host"] = round((time.perf_counter() - t0) * 1e3, 3)
     traj = []
     t0 = time.perf_counter()
     for t in range(ticks):
@@ -132,6 +149,7 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
         },
         "trajectory": {"ticks": ticks, "ms_per_tick": round(traj_wall * 1e3 / max(ticks, 1), 3),
                        "cells_after": traj[-1][1]["n_cells"] if traj else n0},
+        "outputs": outputs,
     }
     res["roofline_kinetics"]["frac"] = round(res["roofline_kinetics"]["achieved"] / HBM_PEAK_GBS, 4)
     if with_cpu:
@@ -149,6 +167,46 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
             "sample": f"sequential cell pass (env gather, CellState::tick, delta scatter) over the first {sample} "
                       f"cells of the 1M-cell workload, one thread as in the reference ({dt:.2f} s)"}
     log("kinetics:", json.dumps(res))
+    return res
+
+
+def dropin_bench(pkg, reps=10):
+    """The drop-in boundary (GpuFieldDiffuser::diffuse_tick_with_cells,
+    crates/marl-gpu/src/field_diffusion.rs:158-292) at the reference's default
+    grid, 128x128x64, 10 substeps: host AoS field and bool occupancy in, field
+    out, so each call moves 50 MB each way over PCIe.  Beside it the same
+    substeps on the resident field, and the f32 CPU stencil (OpenMP)."""
+    from oracle.binding import Oracle
+
+    dims = (128, 128, 64)
+    rng = np.random.default_rng(1)
+    field = rng.random(dims[0] * dims[1] * dims[2] * 12).astype(np.float32)
+    occ = (rng.random(dims[0] * dims[1] * dims[2]) < 0.02).astype(np.uint8)
+    sim = pkg.SimulationConfig()
+    d = pkg.GpuFieldDiffuser.new(dims)
+    f = field.copy()
+    d.diffuse_tick_with_cells(f, occ, sim)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d.diffuse_tick_with_cells(f, occ, sim)
+    call_ms = (time.perf_counter() - t0) * 1e3 / reps
+    d.close()
+    g = pkg.MarlGpu(dims)
+    g.upload_field_aos(field)
+    g.upload_occupancy(occ)
+    g.time_diffuse(sim, 10)
+    resident_ms = g.time_diffuse(sim, 10 * reps) / reps
+    g.close()
+    orc = Oracle()
+    nthreads = min(os.cpu_count() or 1, 16)
+    t0 = time.perf_counter()
+    orc.diffuse_tick(field, occ, dims, sim, nthreads=nthreads)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    res = {"workload": "GpuFieldDiffuser drop-in at 128x128x64, 10 substeps, 2% occupancy; host buffers in and out",
+           "ms_per_call_pcie_inclusive": round(call_ms, 3), "ms_resident_10_substeps": round(resident_ms, 3),
+           "ms_cpu_oracle_10_substeps": round(cpu_ms, 3), "cpu_threads": nthreads,
+           "bytes_over_pcie_per_call": 2 * field.nbytes + occ.nbytes}
+    log("drop-in:", json.dumps(res))
     return res
 
 
@@ -412,6 +470,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in (host buffers) leg")
     ap.add_argument("--tile", type=int, default=-1, help="use this stencil tile id instead of autotuning")
     ap.add_argument("--kinetics-ticks", type=int, default=10,
                     help="full ticks of the 1M-cell kinetics workload (0 = skip)")
@@ -530,6 +589,8 @@ def main():
         out["kinetics"] = kinetics_bench(pkg, args.kinetics_ticks, not args.no_cpu)
     if args.ensemble_ticks > 0:
         out["ensemble"] = ensemble_bench(pkg, args.ensemble_ticks)
+    if not args.no_dropin:
+        out["dropin"] = dropin_bench(pkg)
     if args.drift_ticks > 0:
         out["drift"] = drift_report(pkg, args.drift_ticks, min(os.cpu_count() or 1, 16))
     if args.surrogate_steps > 0:

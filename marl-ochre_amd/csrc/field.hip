@@ -37,14 +37,29 @@ const TileShape kTiles[] = {
     // 6 species per 4-wave block: the group without species 7 reads it for
     // the niche factor from HBM
     {2, 16, 1, 6, 1, 4},
+    // short z-chunks: enough blocks for small grids
+    {1, 16, 1, 4, 1, 12}, {1, 8, 1, 4, 1, 12},
     // temporal blocking (K1b): two substeps per launch
     {1, 16, 1, 12, 2, 8}, {1, 16, 1, 6, 2, 8},
 };
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
+// Default tile: the quad-group shapes from the longest z-chunk down, taking
+// the first that still gives every CU a block (the autotuner may pick another).
 int marl_default_tile(const marl_dims& d) {
-  (void)d;
-  return 7;  // 2 rows per thread, z-chunk 32, 3 groups of one quad each; the autotuner may pick another
+  static const int order[] = {11, 10, 8, 7, 5, 14, 15};
+  static const int cus = [] {
+    int n = 256, dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+    return n;
+  }();
+  for (int t : order) {
+    const TileShape& s = kTiles[t];
+    const int zc = s.zc < d.z ? s.zc : d.z;
+    const long blocks = (long)((d.x + 63) / 64) * ((d.y + 4 * s.ry - 1) / (4 * s.ry)) * ((d.z + zc - 1) / zc);
+    if (blocks >= cus) return t;
+  }
+  return 15;
 }
 
 // The kernel instantiation of a single-substep tile shape: calls f(TileT<...>)

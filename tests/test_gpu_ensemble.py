@@ -67,7 +67,7 @@ def test_ensemble_bit_identical_to_single_columns(pkg):
     assert events > 50
 
 
-@pytest.mark.parametrize("tile", [0, 1, 4, 7, 9, 13])
+@pytest.mark.parametrize("tile", [0, 1, 4, 7, 9, 13, 15])
 def test_ensemble_every_tile_shape(pkg, tile):
     cols = _columns(pkg)[:3]
     _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=2), 2, tile=tile)

@@ -163,24 +163,52 @@ __global__ void k_level_relax(int n, const int32_t* __restrict__ preds, const ui
   }
 }
 
-// Level histogram and counting-sort scatter.  Counts are aggregated per block
-// in LDS (one global atomic per block and level) — a million same-address
-// global atomics would serialize in L2.
+// Level histogram and counting-sort scatter.  Most cells share a few levels,
+// so per-lane atomics on those counters would serialize: lanes of a wave with
+// the same level are grouped by ballot (one atomic per group, in LDS), blocks
+// aggregate in LDS, and one global atomic per block and level remains.
 constexpr int kLocalLevels = 64;
 
-__global__ void __launch_bounds__(256) k_level_hist(int n, const int32_t* __restrict__ level,
-                                                    int32_t* hist /* [0]=max, [1+L]=count */) {
+// Groups the lanes of a wave by `key` (all 64 lanes must call it).  For each
+// distinct key among the valid lanes, the group's first lane calls
+// per_group(key, count); each valid lane returns that call's result plus its
+// rank among the group's lanes (-1 for invalid lanes).
+template <typename F>
+__device__ __forceinline__ int wave_group(bool valid, int key, F&& per_group) {
+  const int lane = (int)(threadIdx.x & 63);
+  unsigned long long active = __ballot(valid);
+  int out = -1;
+  while (active) {
+    const int ld = __ffsll(active) - 1;
+    const int kl = __shfl(key, ld);
+    const unsigned long long m = __ballot(valid && key == kl);
+    int b = 0;
+    if (lane == ld) b = per_group(kl, __popcll(m));
+    b = __shfl(b, ld);
+    if (valid && key == kl) out = b + __popcll(m & ((1ull << lane) - 1ull));
+    active &= ~m;
+  }
+  return out;
+}
+
+constexpr int kLevelBlocks = 256;  // blocks of the histogram and scatter: few same-address atomics
+
+__device__ __forceinline__ void level_hist_block(int n, const int32_t* __restrict__ level,
+                                                 int32_t* hist /* [0]=max, [1+L]=count */) {
   __shared__ int32_t lh[kLocalLevels];
   __shared__ int32_t lmax;
   for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x) lh[t] = 0;
   if (threadIdx.x == 0) lmax = 0;
   __syncthreads();
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) {
-    const int L = level[j];
-    atomicMax(&lmax, L);
-    if (L < kLocalLevels) atomicAdd(&lh[L], 1);
-    else if (L < kMaxLevels) atomicAdd(&hist[1 + L], 1);
+  for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {  // uniform per block
+    const int j = b0 + threadIdx.x;
+    const bool valid = j < n;
+    wave_group(valid, valid ? level[j] : 0, [&](int L, int c) {
+      atomicMax(&lmax, L);
+      if (L < kLocalLevels) atomicAdd(&lh[L], c);
+      else if (L < kMaxLevels) atomicAdd(&hist[1 + L], c);
+      return 0;
+    });
   }
   __syncthreads();
   for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x)
@@ -188,22 +216,61 @@ __global__ void __launch_bounds__(256) k_level_hist(int n, const int32_t* __rest
   if (threadIdx.x == 0) atomicMax(&hist[0], lmax);
 }
 
-__global__ void __launch_bounds__(256) k_level_scatter(int n, const int32_t* __restrict__ level, int32_t* cursor,
-                                                       int32_t* __restrict__ order) {
+// Counting-sort scatter: block b owns a contiguous run of cells, counts its
+// levels in LDS, reserves one range per level with a single global atomic,
+// then places its cells.
+__device__ __forceinline__ void level_scatter_block(int n, const int32_t* __restrict__ level, int32_t* cursor,
+                                                    int32_t* __restrict__ order) {
   __shared__ int32_t cnt[kLocalLevels], base[kLocalLevels];
+  const int chunk = ((n + 255) / 256 + (int)gridDim.x - 1) / (int)gridDim.x * 256;
+  const int lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
   for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x) cnt[t] = 0;
   __syncthreads();
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int L = j < n ? level[j] : -1;
-  const int local = (L >= 0 && L < kLocalLevels) ? atomicAdd(&cnt[L], 1) : -1;
-  __syncthreads();
-  for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x)
-    base[t] = cnt[t] ? atomicAdd(&cursor[t], cnt[t]) : 0;
-  __syncthreads();
-  if (j < n) {
-    const int pos = (L < kLocalLevels) ? base[L] + local : atomicAdd(&cursor[L], 1);
-    order[pos] = j;
+  // the levels of 8 sub-chunks are loaded together, so their latencies overlap
+  constexpr int U = 8;
+  for (int c0 = lo; c0 < hi; c0 += U * 256) {
+    int Ls[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = c0 + u * 256 + (int)threadIdx.x;
+      Ls[u] = j < hi ? level[j] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      wave_group(Ls[u] >= 0 && Ls[u] < kLocalLevels, Ls[u], [&](int k, int c) { return atomicAdd(&cnt[k], c); });
   }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x) {
+    base[t] = cnt[t] ? atomicAdd(&cursor[t], cnt[t]) : 0;
+    cnt[t] = 0;  // reused as the running count of the placement pass
+  }
+  __syncthreads();
+  for (int c0 = lo; c0 < hi; c0 += U * 256) {
+    int Ls[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = c0 + u * 256 + (int)threadIdx.x;
+      Ls[u] = j < hi ? level[j] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = c0 + u * 256 + (int)threadIdx.x;
+      // deep levels (beyond the LDS counters) take their global slots directly
+      const int pos = wave_group(Ls[u] >= 0, Ls[u], [&](int k, int c) {
+        return k < kLocalLevels ? base[k] + atomicAdd(&cnt[k], c) : atomicAdd(&cursor[k], c);
+      });
+      if (Ls[u] >= 0) order[pos] = j;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_level_hist(int n, const int32_t* __restrict__ level, int32_t* hist) {
+  level_hist_block(n, level, hist);
+}
+
+__global__ void __launch_bounds__(256) k_level_scatter(int n, const int32_t* __restrict__ level, int32_t* cursor,
+                                                       int32_t* __restrict__ order) {
+  level_scatter_block(n, level, cursor, order);
 }
 
 // ---------------------------------------------------------------------------
@@ -689,22 +756,7 @@ __global__ void k_g_relax_check(TickDev* ts, int sweeps) {
 __global__ void __launch_bounds__(256) k_g_level_hist(const TickDev* ts, const int32_t* __restrict__ level,
                                                       int32_t* hist) {
   if (ts->err) return;
-  __shared__ int32_t lh[kLocalLevels];
-  __shared__ int32_t lmax;
-  for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x) lh[t] = 0;
-  if (threadIdx.x == 0) lmax = 0;
-  __syncthreads();
-  const int n = ts->n;
-  GS_LOOP(j, n) {
-    const int L = level[j];
-    atomicMax(&lmax, L);
-    if (L < kLocalLevels) atomicAdd(&lh[L], 1);
-    else if (L < kMaxLevels) atomicAdd(&hist[1 + L], 1);
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x)
-    if (lh[t]) atomicAdd(&hist[1 + t], lh[t]);
-  if (threadIdx.x == 0) atomicMax(&hist[0], lmax);
+  level_hist_block(ts->n, level, hist);
 }
 
 __global__ void k_g_level_offsets(TickDev* ts, const int32_t* __restrict__ hist, int32_t* __restrict__ cursor,
@@ -727,24 +779,11 @@ __global__ void k_g_level_offsets(TickDev* ts, const int32_t* __restrict__ hist,
   ts->levels = nlev;
 }
 
+// Levels here are below kLocalLevels: deeper schedules set err before this kernel.
 __global__ void __launch_bounds__(256) k_g_level_scatter(const TickDev* ts, const int32_t* __restrict__ level,
                                                          int32_t* cursor, int32_t* __restrict__ order) {
   if (ts->err) return;
-  __shared__ int32_t cnt[kLocalLevels], base[kLocalLevels];
-  const int n = ts->n;
-  for (int b0 = blockIdx.x * 256; b0 < n; b0 += gridDim.x * 256) {
-    for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x) cnt[t] = 0;
-    __syncthreads();
-    const int j = b0 + threadIdx.x;
-    const int L = j < n ? level[j] : -1;
-    const int local = L >= 0 ? atomicAdd(&cnt[L], 1) : -1;
-    __syncthreads();
-    for (int t = threadIdx.x; t < kLocalLevels; t += blockDim.x)
-      base[t] = cnt[t] ? atomicAdd(&cursor[t], cnt[t]) : 0;
-    __syncthreads();
-    if (j < n) order[base[L] + local] = j;
-    __syncthreads();
-  }
+  level_scatter_block(ts->n, level, cursor, order);
 }
 
 __global__ void __launch_bounds__(kKinBlock) k_g_kinetics(int L, const TickDev* ts, const int32_t* __restrict__ order,
@@ -1022,7 +1061,8 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
   }
   int32_t* hist = ctx->level_count;
   if (int rc = fill_i32(ctx, hist, 0, kMaxLevels + 1)) return rc;
-  hipLaunchKernelGGL(k_level_hist, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->level, hist);
+  const unsigned lb = std::min<unsigned>(grid1(n), kLevelBlocks);
+  hipLaunchKernelGGL(k_level_hist, dim3(lb), dim3(256), 0, ctx->stream, n, ctx->level, hist);
   MARL_HIP(ctx, hipGetLastError());
   std::vector<int32_t> hh(kMaxLevels + 1);
   if (int rc = d2h_sync(ctx, hh.data(), hist, sizeof(int32_t) * (kMaxLevels + 1))) return rc;
@@ -1032,7 +1072,7 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
   for (int L = 0; L < nlev; ++L) off[L + 1] = off[L] + hh[1 + L];
   int32_t* cursor = ctx->level_count + (kMaxLevels + 1);
   MARL_HIP(ctx, hipMemcpyAsync(cursor, off.data(), sizeof(int32_t) * nlev, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_level_scatter, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->level, cursor, ctx->order);
+  hipLaunchKernelGGL(k_level_scatter, dim3(lb), dim3(256), 0, ctx->stream, n, ctx->level, cursor, ctx->order);
   MARL_HIP(ctx, hipGetLastError());
   MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));  // `off` must outlive the copy
   if (ev_sched) MARL_HIP(ctx, hipEventRecord(ev_sched, ctx->stream));
@@ -1209,9 +1249,11 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
   int32_t* hist = ctx->level_count;
   int32_t* cursor = ctx->level_count + (kMaxLevels + 1);
   if (int rc = fill_i32(ctx, hist, 0, kMaxLevels + 1)) return rc;
-  hipLaunchKernelGGL(k_g_level_hist, dim3(gc), dim3(256), 0, st, ts, ctx->level, hist);
+  hipLaunchKernelGGL(k_g_level_hist, dim3(std::min<unsigned>(gc, kLevelBlocks)), dim3(256), 0, st, ts, ctx->level,
+                     hist);
   hipLaunchKernelGGL(k_g_level_offsets, dim3(1), dim3(1), 0, st, ts, hist, cursor, ctx->graph_levels);
-  hipLaunchKernelGGL(k_g_level_scatter, dim3(gc), dim3(256), 0, st, ts, ctx->level, cursor, ctx->order);
+  hipLaunchKernelGGL(k_g_level_scatter, dim3(std::min<unsigned>(gc, kLevelBlocks)), dim3(256), 0, st, ts,
+                     ctx->level, cursor, ctx->order);
   MARL_HIP(ctx, hipGetLastError());
   // the ordered pass on the AoS view
   if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;

@@ -1041,23 +1041,33 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
   hipLaunchKernelGGL(k_cell_prep, dim3(grid1(n)), dim3(256), 0, ctx->stream, cells, n, ctx->cell_at, ctx->occ,
                      ctx->nbmask, ctx->preds, ctx->npred, ctx->level, p);
   MARL_HIP(ctx, hipGetLastError());
+  // Relaxation sweeps, each with its own change flag, in batches with one host
+  // check each.  The first batch is as long as the previous tick needed (the
+  // sweep that changed nothing included), so a schedule of steady depth costs
+  // one batch and one check.  Extra sweeps after the fixed point change nothing.
   int32_t* changed = ctx->misc;
   int32_t* h = ctx->pinned;
-  for (int sweep = 0;; ++sweep) {
-    if (sweep > kMaxLevels + 2) return marl_fail(ctx, MARL_ERR_TRANSFER, "conflict schedule did not converge");
-    // a few sweeps per host check
-    if (int rc = fill_i32(ctx, changed, 0, 1)) return rc;
-    for (int k = 0; k < 4; ++k)
+  constexpr int kBatchMax = 32;
+  int done = 0, batch = std::min(std::max(ctx->relax_sweeps, 2), kBatchMax);
+  for (;;) {
+    if (done > kMaxLevels + 2) return marl_fail(ctx, MARL_ERR_TRANSFER, "conflict schedule did not converge");
+    if (int rc = fill_i32(ctx, changed, 0, batch)) return rc;
+    for (int k = 0; k < batch; ++k)
       hipLaunchKernelGGL(k_level_relax, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->preds, ctx->npred,
-                         ctx->level, changed);
+                         ctx->level, changed + k);
     MARL_HIP(ctx, hipGetLastError());
-    // if the last of the four sweeps changed nothing the levels are final
-    if (int rc = fill_i32(ctx, changed + 1, 0, 1)) return rc;
-    hipLaunchKernelGGL(k_level_relax, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->preds, ctx->npred,
-                       ctx->level, changed + 1);
-    MARL_HIP(ctx, hipGetLastError());
-    if (int rc = d2h_sync(ctx, h, changed + 1, sizeof(int32_t))) return rc;
-    if (h[0] == 0) break;
+    if (int rc = d2h_sync(ctx, h, changed, sizeof(int32_t) * batch)) return rc;
+    int quiet = -1;
+    for (int k = 0; k < batch && quiet < 0; ++k)
+      if (h[k] == 0) quiet = k;
+    if (quiet >= 0) {
+      ctx->relax_sweeps = done + quiet + 1;
+      break;
+    }
+    // deeper than last tick: a short batch first, then doubling
+    const int next = done == 0 ? 4 : std::min(2 * batch, kBatchMax);
+    done += batch;
+    batch = next;
   }
   int32_t* hist = ctx->level_count;
   if (int rc = fill_i32(ctx, hist, 0, kMaxLevels + 1)) return rc;

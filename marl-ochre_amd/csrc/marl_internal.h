@@ -255,6 +255,7 @@ struct marl_ctx {
   std::vector<uint8_t> graph_key[2];  // inputs each captured graph was recorded for
   int graph_flip[2] = {0, 0};         // the graph swaps the field buffers
   int graph_sweeps = 64, graph_levels = 64;
+  int relax_sweeps = 4;          // sweeps the last eager schedule needed to converge
   marl_sim_params last_sim{};
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -195,15 +195,20 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
 /* The same tick as marl_tick, captured once as a hipGraph and replayed: every
  * count (cells, schedule depth, births, deaths) lives on the device, so a
  * tick costs one graph launch and one small readback instead of a host round
- * trip per phase.  Results are bit-identical to marl_tick.  Requires an even
- * diffusion_substeps, plain kinetics and no harvest.  A tick whose schedule
- * needs more relaxation sweeps or levels than the captured budget (64 each,
- * see marl_set_graph_limits) is finished by the host-driven path;
- * *host_finished (may be NULL) counts such ticks.  stats->ms_field holds the
- * graph's GPU time; the other phase times are 0. */
+ * trip per phase.  Results are bit-identical to marl_tick.  Requires plain
+ * kinetics and no harvest; an odd diffusion_substeps alternates between two
+ * graphs, one per field-buffer parity.  A tick whose schedule needs more
+ * relaxation sweeps or levels than the captured budget (see
+ * marl_set_graph_limits) is finished by the host-driven path; *host_finished
+ * (may be NULL) counts such ticks.  stats->ms_field holds the graph's GPU
+ * time; the other phase times are 0. */
 int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t n_ticks,
                     marl_tick_stats* stats, int32_t* host_finished);
-/* Budget of relaxation sweeps and level launches in the captured tick (1..64). */
+/* Budget of relaxation sweeps and level launches in the captured tick (1..64
+ * each; a change captures the graph again).  (0, 0) selects the default
+ * automatic budget: it starts at 64 each and then follows what the schedule
+ * needed: within 2 of the need it grows to the need + 12, more than 20 above
+ * it shrinks to the need + 8. */
 int marl_set_graph_limits(marl_ctx* ctx, int32_t sweeps, int32_t levels);
 int marl_cell_pass(marl_ctx* ctx, const marl_sim_params* sim, uint8_t* events, size_t n_events, int32_t* levels_out);
 /* FP64 per-species totals: out[0..11] empty voxels, out[12..23] occupied

@@ -1371,6 +1371,23 @@ int marl_graph_fetch(marl_ctx* ctx, hipStream_t st) {
 
 // After the fetch completed: the tick's outcome, finishing the cell phase on
 // the host-driven path if the device skipped it.
+// Automatic schedule budget of the captured tick: follow the sweeps and levels
+// the schedule needed, with slack, so that the graph launches few idle sweep
+// and level kernels.  A budget within 2 of the need grows at once to the need
+// plus 12 (a growing population deepens its schedule by a level or two per
+// tick, and a tick over budget is finished by the host); a budget shrinks, to
+// the need plus 8, only when it is more than 20 above it, so that the graph is
+// rarely captured again.
+static void adapt_graph_limits(marl_ctx* ctx, int need_sweeps, int need_levels) {
+  if (!ctx->graph_auto) return;
+  auto fit = [](int cur, int need, int cap) {
+    if (need + 2 >= cur) return std::min(cap, need + 12);
+    return cur > need + 20 ? std::min(cap, need + 8) : cur;
+  };
+  ctx->graph_sweeps = fit(ctx->graph_sweeps, std::max(need_sweeps, 1), kGraphSweeps);
+  ctx->graph_levels = fit(ctx->graph_levels, std::max(need_levels, 1), kGraphLevels);
+}
+
 int marl_graph_post(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick, marl_tick_stats* st,
                     int32_t* host_finished) {
   const TickDev* hts = host_tickdev(ctx);
@@ -1381,6 +1398,9 @@ int marl_graph_post(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick, mar
     st->deaths = hts->deaths;
     st->births_added = hts->births_added;
     st->levels = hts->levels;
+    int quiet = 0;  // the first sweep that changed nothing (the check found one)
+    while (quiet < ctx->graph_sweeps - 1 && hts->chg[quiet] != 0) ++quiet;
+    adapt_graph_limits(ctx, quiet + 1, hts->levels);
   } else {
     ++*host_finished;
     if (int rc = marl_ensure_cell_capacity(ctx, 2 * ctx->n_cells + 1024)) return rc;
@@ -1390,6 +1410,7 @@ int marl_graph_post(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick, mar
     if (int rc = run_fate(ctx, p, st)) return rc;
     MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     st->levels = levels;
+    adapt_graph_limits(ctx, ctx->relax_sweeps, levels);
   }
   st->n_cells = ctx->n_cells;
   return MARL_OK;
@@ -1596,10 +1617,15 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
 
 int marl_set_graph_limits(marl_ctx* ctx, int32_t sweeps, int32_t levels) {
   if (!ctx) return MARL_ERR_INVALID_INPUT;
+  if (sweeps == 0 && levels == 0) {
+    ctx->graph_auto = true;
+    return MARL_OK;
+  }
   if (sweeps < 1 || sweeps > kGraphSweeps || levels < 1 || levels > kGraphLevels)
     return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "graph limits out of range");
   ctx->graph_sweeps = sweeps;
   ctx->graph_levels = levels;
+  ctx->graph_auto = false;
   return MARL_OK;
 }
 

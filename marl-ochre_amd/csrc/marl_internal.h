@@ -254,7 +254,8 @@ struct marl_ctx {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<uint8_t> graph_key[2];  // inputs each captured graph was recorded for
   int graph_flip[2] = {0, 0};         // the graph swaps the field buffers
-  int graph_sweeps = 64, graph_levels = 64;
+  int graph_sweeps = 64, graph_levels = 64;  // schedule budget of the captured tick
+  bool graph_auto = true;                    // budget follows the schedule's needs
   int relax_sweeps = 4;          // sweeps the last eager schedule needed to converge
   marl_sim_params last_sim{};
 

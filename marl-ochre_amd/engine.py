@@ -205,7 +205,8 @@ class MarlGpu:
         d["host_finished"] = hf.value
         return d
 
-    def set_graph_limits(self, sweeps: int = 64, levels: int = 64) -> None:
+    def set_graph_limits(self, sweeps: int = 0, levels: int = 0) -> None:
+        """Schedule budget of the captured tick; (0, 0) = automatic (the default)."""
         self._chk(self.lib.marl_set_graph_limits(self.h, int(sweeps), int(levels)))
 
     def cell_pass(self, sim) -> tuple[np.ndarray, int]:

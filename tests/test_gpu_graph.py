@@ -107,3 +107,36 @@ def test_graph_tick_odd_substeps(pkg):
     cells, rules = population(A, dims, 3000, 8)
     field = np.random.default_rng(6).random(np.prod(dims) * 12).astype(F)
     _compare_runs(pkg, dims, field, cells, rules, pkg.SimulationConfig(rng_seed=3, diffusion_substeps=3), 4)
+
+
+def test_graph_budget_automatic_after_explicit(pkg):
+    """An explicit one-sweep budget host-finishes every tick; (0, 0) switches
+    the budget back to automatic: the next tick is still finished by the host
+    (the budget it ran with was one sweep), after which the budget fits the
+    schedule and the captured ticks finish on the device."""
+    A = pkg._abi
+    dims = (64, 32, 24)
+    cells, rules = population(A, dims, 5000, 11)
+    field = np.random.default_rng(12).random(np.prod(dims) * 12).astype(F)
+    sim = pkg.SimulationConfig(rng_seed=21, diffusion_substeps=2)
+    with pkg.MarlGpu(dims) as ge, pkg.MarlGpu(dims) as gg:
+        for g in (ge, gg):
+            g.upload_field_aos(field)
+            g.upload_cells(cells, rules)
+        finished = []
+        for t in range(6):
+            if t == 0:
+                gg.set_graph_limits(1, 64)
+            if t == 2:
+                gg.set_graph_limits(0, 0)
+            se = ge.tick(sim, t, 1)
+            sg = gg.tick_graph(sim, t, 1)
+            assert se["levels"] > 1
+            for k in ("divisions", "deaths", "births_added", "n_cells", "levels"):
+                assert sg[k] == se[k], (t, k, sg[k], se[k])
+            finished.append(sg["host_finished"])
+        c_e, r_e = ge.download_cells()
+        c_g, r_g = gg.download_cells()
+        assert_cells_equal(c_g, r_g, c_e, r_e)
+        assert np.array_equal(gg.download_field_aos().view(np.uint32), ge.download_field_aos().view(np.uint32))
+    assert finished == [1, 1, 1, 0, 0, 0], finished

@@ -170,6 +170,71 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
     return res
 
 
+def column_bench(pkg, ticks=200, cpu_ticks=None, with_cpu=True):
+    """The reference's default column run: 128x128x64, ~4,096 cells seeded in
+    three depth bands (one metabolism each), internals U[0,0.1], 16 random
+    reactions (v_max log-U[1e-3,1e-1], k_m log-U[1e-2,1]), linear-gradient
+    field, 10 substeps, `ticks` whole ticks from the uploaded state (the first
+    tick captures the graph).  The same run with the starters' own states and
+    reactions (a population that lives) is timed too.  CPU: the f32 oracle's
+    full tick (diffusion OpenMP over z-layers, light and the cell pass on one
+    thread, as the reference) over the first `cpu_ticks` ticks (default: all),
+    whose end state is compared with the GPU's after the same ticks."""
+    from marl_ochre_amd import scenarios
+
+    cpu_ticks = ticks if cpu_ticks is None else cpu_ticks
+    dims = (128, 128, 64)
+    res = {"workload": "default column 128x128x64, ~4,096 cells in bands [4,12),[24,40),[52,60), internals U[0,0.1], "
+                       "16 random reactions (v_max log-U[1e-3,1e-1], k_m log-U[1e-2,1]), linear gradients, "
+                       f"10 substeps, {ticks} ticks"}
+    for name, starter in (("random_reactions", False), ("starter_states", True)):
+        field, cells, rules, sim = scenarios.winogradsky_column(dims, starter_state=starter)
+        row = {"cells_start": len(cells)}
+        for mode in ("graph", "eager"):
+            g = pkg.MarlGpu(dims)
+            g.upload_field_aos(field)
+            g.upload_cells(cells, rules)
+            g.sync()
+            step = g.tick_graph if mode == "graph" else g.tick
+            t0 = time.perf_counter()
+            step(sim, 0, ticks)
+            g.sync()
+            row[f"ms_per_tick_{mode}"] = round((time.perf_counter() - t0) * 1e3 / ticks, 4)
+            row["cells_end"] = g.cell_count()
+            g.close()
+        row["Gvoxel_species_substeps_per_s_graph"] = round(
+            dims[0] * dims[1] * dims[2] * 12 * sim.diffusion_substeps / (row["ms_per_tick_graph"] * 1e-3) / 1e9, 1)
+        res[name] = row
+    if with_cpu:
+        from oracle.binding import Oracle
+
+        orc = Oracle()
+        threads = len(os.sched_getaffinity(0))
+        field, cells, rules, sim = scenarios.winogradsky_column(dims)
+        f, c, r = field, cells, rules
+        t0 = time.perf_counter()
+        for t in range(cpu_ticks):
+            f, _, c, r, _, _ = orc.tick(dims, sim, t, f, c, r, nthreads=threads)
+        dt = time.perf_counter() - t0
+        g = pkg.MarlGpu(dims)
+        g.upload_field_aos(field)
+        g.upload_cells(cells, rules)
+        g.tick_graph(sim, 0, cpu_ticks)
+        c_g, _ = g.download_cells()
+        same = bool(np.array_equal(g.download_field_aos().view(np.uint32), f.view(np.uint32))
+                    and len(c_g) == len(c)
+                    and np.array_equal(c_g["internal"].view(np.uint32), c["internal"].view(np.uint32))
+                    and np.array_equal(c_g["pos"], c["pos"]))
+        g.close()
+        res["cpu_baseline"] = {
+            "value": round(dt * 1e3 / cpu_ticks, 3), "unit": "ms/tick", "cores": threads, "kind": "port",
+            "sample": f"the first {cpu_ticks} ticks of the random-reaction run, f32 C oracle ({dt:.2f} s)"}
+        res["speedup_vs_cpu"] = round(res["cpu_baseline"]["value"] / res["random_reactions"]["ms_per_tick_graph"], 1)
+        res["gpu_bit_identical_to_oracle"] = {"tick": cpu_ticks, "field_and_cells": same}
+    log("column:", json.dumps(res))
+    return res
+
+
 def dropin_bench(pkg, reps=10):
     """The drop-in boundary (GpuFieldDiffuser::diffuse_tick_with_cells,
     crates/marl-gpu/src/field_diffusion.rs:158-292) at the reference's default
@@ -476,6 +541,8 @@ def main():
     ap.add_argument("--tile", type=int, default=-1, help="use this stencil tile id instead of autotuning")
     ap.add_argument("--kinetics-ticks", type=int, default=10,
                     help="full ticks of the 1M-cell kinetics workload (0 = skip)")
+    ap.add_argument("--column-ticks", type=int, default=200,
+                    help="ticks of the default 128x128x64 column run (0 = skip)")
     ap.add_argument("--ensemble-ticks", type=int, default=5,
                     help="ticks of the 32-column ensemble leg (0 = skip)")
     ap.add_argument("--drift-ticks", type=int, default=0,
@@ -589,6 +656,8 @@ def main():
     g.close()
     if args.kinetics_ticks > 0:
         out["kinetics"] = kinetics_bench(pkg, args.kinetics_ticks, not args.no_cpu)
+    if args.column_ticks > 0:
+        out["column"] = column_bench(pkg, args.column_ticks, with_cpu=not args.no_cpu)
     if args.ensemble_ticks > 0:
         out["ensemble"] = ensemble_bench(pkg, args.ensemble_ticks)
     if not args.no_dropin:

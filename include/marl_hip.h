@@ -59,6 +59,9 @@ enum { MARL_EVENT_NONE = 0, MARL_EVENT_DIVISION = 1, MARL_EVENT_DEATH = 2, MARL_
 /* Kinetics modes for the resident tick. */
 enum { MARL_KINETICS_PLAIN = 0, MARL_KINETICS_SURROGATE = 1, MARL_KINETICS_SHADOW = 2 };
 
+/* Field layout of the plain cell pass (marl_set_cell_layout). */
+enum { MARL_CELL_LAYOUT_AUTO = 0, MARL_CELL_LAYOUT_AOS = 1, MARL_CELL_LAYOUT_QUADS = 2 };
+
 typedef struct marl_ctx marl_ctx;
 typedef struct marl_ensemble marl_ensemble;
 
@@ -204,6 +207,14 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
  * time; the other phase times are 0. */
 int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t n_ticks,
                     marl_tick_stats* stats, int32_t* host_finished);
+/* Field layout the plain cell pass works on; results are identical either way.
+ * MARL_CELL_LAYOUT_AOS: an AoS view (48-byte voxel records) transposed from
+ * and back to the resident species quads around the pass; _QUADS: the
+ * resident quads in place (no transposes, more cache lines per neighbour);
+ * _AUTO (default): quads below about one cell per 13 voxels.  A tuning knob
+ * with no counterpart in the reference (whose field is always AoS,
+ * crates/marl-field/src/field.rs:15-35). */
+int marl_set_cell_layout(marl_ctx* ctx, int32_t layout);
 /* Budget of relaxation sweeps and level launches in the captured tick (1..64
  * each; a change captures the graph again).  (0, 0) selects the default
  * automatic budget: it starts at 64 each and then follows what the schedule

@@ -22,6 +22,7 @@ S_RECEPTORS = S_TRANSPORTERS = S_EFFECTORS = 8
 OK, ERR_NO_DEVICE, ERR_DEVICE_INIT, ERR_TRANSFER, ERR_INVALID_INPUT = 0, 1, 2, 3, 4
 EVENT_NONE, EVENT_DIVISION, EVENT_DEATH, EVENT_QUIESCENCE = 0, 1, 2, 3
 KINETICS_PLAIN, KINETICS_SURROGATE, KINETICS_SHADOW = 0, 1, 2
+CELL_LAYOUT_AUTO, CELL_LAYOUT_AOS, CELL_LAYOUT_QUADS = 0, 1, 2
 
 
 class marl_dims(ctypes.Structure):
@@ -226,6 +227,7 @@ SIGNATURES = {
     "marl_tick_graph": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I64, _I32,
                                        ctypes.POINTER(marl_tick_stats), ctypes.POINTER(_I32)]),
     "marl_set_graph_limits": (ctypes.c_int, [_P, _I32, _I32]),
+    "marl_set_cell_layout": (ctypes.c_int, [_P, _I32]),
     "marl_cell_pass": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _P, _SZ, ctypes.POINTER(_I32)]),
     "marl_mass_budget": (ctypes.c_int, [_P, _P, _SZ]),
     "marl_time_diffuse": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I32, ctypes.POINTER(ctypes.c_float)]),

@@ -53,6 +53,7 @@ CellParams cell_params(const marl_ctx* ctx, const marl_sim_params* s, uint64_t t
   p.Z = ctx->d.z;
   p.seed = s->rng_seed;
   p.stream = (MARL_TAG_DIVISION << 32) | (tick & 0xffffffffull);
+  p.fplane = (uint64_t)ctx->nvox * 4;
   return p;
 }
 
@@ -279,13 +280,15 @@ __global__ void __launch_bounds__(256) k_level_scatter(int n, const int32_t* __r
 // split deltas with the max(0) clamp into those neighbours (spatial.rs:87-110,
 // field.rs:57-62).  The field is read and written through an AoS view (one
 // 48-byte record per voxel, transposed around the pass) so a neighbour costs
-// three float4 accesses instead of twelve scattered floats.  Dynamically
+// three float4 accesses in one or two cache lines; with QUADS (sparse
+// populations, where the two whole-field transposes cost more than the extra
+// lines) the resident species quads are used in place.  Dynamically
 // indexed per-cell arrays live in LDS (one private slot per thread, odd
 // strides), so nothing goes to scratch.  With a harvest sink, sampled cells
 // also append their (features, post-kinetics deltas and fluxes) pair.
 constexpr int kKinBlock = 128;
 
-template <bool HARVEST>
+template <bool HARVEST, bool QUADS>
 __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict__ cells,
                                               const marl_ruleset* __restrict__ rules, float* __restrict__ field,
                                               const float* __restrict__ light, const uint8_t* __restrict__ nbmask,
@@ -297,7 +300,7 @@ __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict
   const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
 #pragma unroll
   for (int k = 0; k < MARL_M_INT; ++k) sI[k] = c.internal[k];
-  gather_env6(field, v, m, p, sE);
+  gather_env6<QUADS>(field, v, m, p, sE);
   const float lt = light[v];
   PtrArr I{sI}, D{sD};
   const PtrArr E{sE};
@@ -334,10 +337,10 @@ __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict
   for (int k = 0; k < MARL_M_INT; ++k) c.internal[k] = sI[k];
   cells[i] = c;
   events[i] = ev;
-  scatter_deltas6(field, v, m, p, sD);
+  scatter_deltas6<QUADS>(field, v, m, p, sD);
 }
 
-template <bool HARVEST>
+template <bool HARVEST, bool QUADS>
 __global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restrict__ order, int count,
                                                       marl_cell_state* __restrict__ cells,
                                                       const marl_ruleset* __restrict__ rules,
@@ -350,8 +353,8 @@ __global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restric
   const int t = blockIdx.x * kKinBlock + threadIdx.x;
   if (t >= count) return;
   const int i = order ? order[t] : t;
-  kinetics_cell<HARVEST>(i, cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17,
-                         sE + threadIdx.x * 13, sD + threadIdx.x * 13);
+  kinetics_cell<HARVEST, QUADS>(i, cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17,
+                                sE + threadIdx.x * 13, sD + threadIdx.x * 13);
 }
 
 // ---------------------------------------------------------------------------
@@ -786,6 +789,7 @@ __global__ void __launch_bounds__(256) k_g_level_scatter(const TickDev* ts, cons
   level_scatter_block(ts->n, level, cursor, order);
 }
 
+template <bool QUADS>
 __global__ void __launch_bounds__(kKinBlock) k_g_kinetics(int L, const TickDev* ts, const int32_t* __restrict__ order,
                                                         marl_cell_state* __restrict__ cells,
                                                         const marl_ruleset* __restrict__ rules,
@@ -799,8 +803,8 @@ __global__ void __launch_bounds__(kKinBlock) k_g_kinetics(int L, const TickDev* 
   const int lo = ts->off[L], cnt = ts->off[L + 1] - lo;
   const HarvestSink hv{};
   GS_LOOP(t, cnt)
-  kinetics_cell<false>(order[lo + t], cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17,
-                       sE + threadIdx.x * 13, sD + threadIdx.x * 13);
+  kinetics_cell<false, QUADS>(order[lo + t], cells, rules, field, light, nbmask, events, p, hv,
+                              sI + threadIdx.x * 17, sE + threadIdx.x * 13, sD + threadIdx.x * 13);
 }
 
 // division targets for i < n; zero flags up to cap for the fixed-size scan
@@ -1025,6 +1029,30 @@ int marl_launch_occupancy_from_cells(marl_ctx* ctx) {
   return MARL_OK;
 }
 
+// Field layout for the plain cell pass.  The AoS view costs two whole-field
+// transposes (4 x 48 bytes moved per voxel, about 32 ps per voxel at
+// 256x256x128); the quads in place cost each cell about 0.43 ns more, since a
+// neighbour's species then span three cache lines instead of one or two
+// (both measured with 1M cells at 256x256x128).  The quads win below about one
+// cell per 13 voxels; a band of +-25 % around that keeps a captured tick from
+// switching layouts back and forth.
+bool marl_choose_cell_layout(marl_ctx* ctx) {
+  if (ctx->cell_layout == MARL_CELL_LAYOUT_AOS) return ctx->kin_quads = false;
+  if (ctx->cell_layout == MARL_CELL_LAYOUT_QUADS) return ctx->kin_quads = true;
+  const double r = 13.0 * (double)ctx->n_cells / (double)std::max<int64_t>(ctx->nvox, 1);
+  if (ctx->kin_quads && r > 1.25) ctx->kin_quads = false;
+  else if (!ctx->kin_quads && r < 0.8) ctx->kin_quads = true;
+  return ctx->kin_quads;
+}
+
+int marl_set_cell_layout(marl_ctx* ctx, int32_t layout) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  if (layout != MARL_CELL_LAYOUT_AUTO && layout != MARL_CELL_LAYOUT_AOS && layout != MARL_CELL_LAYOUT_QUADS)
+    return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "unknown cell-pass layout");
+  ctx->cell_layout = layout;
+  return MARL_OK;
+}
+
 namespace {
 
 // The ordered cell pass (lib.rs:204-217) on the resident field, light and
@@ -1086,29 +1114,40 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
   MARL_HIP(ctx, hipGetLastError());
   MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));  // `off` must outlive the copy
   if (ev_sched) MARL_HIP(ctx, hipEventRecord(ev_sched, ctx->stream));
-  // the pass works on an AoS view of the field (the 48-byte voxel records keep
+  // The pass works on an AoS view of the field (the 48-byte voxel records keep
   // a neighbour's 12 species in one or two cache lines; the resident layout is
-  // species quads, best for the stencil)
-  if (int rc = marl_ensure_staging(ctx)) return rc;
-  if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
-  float* field = ctx->staging;
+  // species quads, best for the stencil), or on the quads in place when the
+  // population is sparse.  The surrogate and the pair harvest use the AoS view.
   const HarvestSink hv = surrogate_harvest_sink(ctx, p);
+  const bool plain = ctx->kin_mode == MARL_KINETICS_PLAIN || !ctx->surr;
+  const bool quads = plain && !hv.count && marl_choose_cell_layout(ctx);
+  float* field = ctx->field[ctx->cur];
+  if (!quads) {
+    if (int rc = marl_ensure_staging(ctx)) return rc;
+    if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
+    field = ctx->staging;
+  }
   for (int L = 0; L < nlev; ++L) {
     const int cnt = off[L + 1] - off[L];
     if (cnt == 0) continue;
-    if (ctx->kin_mode != MARL_KINETICS_PLAIN && ctx->surr) {
-      if (int rc = surrogate_level(ctx, p, ctx->order + off[L], cnt, cells, rules, field)) return rc;
+    const dim3 grid(grid1(cnt, kKinBlock)), block(kKinBlock);
+    const int32_t* ord = ctx->order + off[L];
+    if (!plain) {
+      if (int rc = surrogate_level(ctx, p, ord, cnt, cells, rules, field)) return rc;
+    } else if (hv.count) {
+      hipLaunchKernelGGL((k_kinetics<true, false>), grid, block, 0, ctx->stream, ord, cnt, cells, rules, field,
+                         ctx->light, ctx->nbmask, ctx->events, p, hv);
+    } else if (quads) {
+      hipLaunchKernelGGL((k_kinetics<false, true>), grid, block, 0, ctx->stream, ord, cnt, cells, rules, field,
+                         ctx->light, ctx->nbmask, ctx->events, p, hv);
     } else {
-      if (hv.count)
-        hipLaunchKernelGGL(k_kinetics<true>, dim3(grid1(cnt, kKinBlock)), dim3(kKinBlock), 0, ctx->stream,
-                           ctx->order + off[L], cnt, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p, hv);
-      else
-        hipLaunchKernelGGL(k_kinetics<false>, dim3(grid1(cnt, kKinBlock)), dim3(kKinBlock), 0, ctx->stream,
-                           ctx->order + off[L], cnt, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p, hv);
+      hipLaunchKernelGGL((k_kinetics<false, false>), grid, block, 0, ctx->stream, ord, cnt, cells, rules, field,
+                         ctx->light, ctx->nbmask, ctx->events, p, hv);
     }
   }
   MARL_HIP(ctx, hipGetLastError());
-  if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
+  if (!quads)
+    if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
   if (levels_out) *levels_out = nlev;
   return MARL_OK;
 }
@@ -1265,13 +1304,20 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
   hipLaunchKernelGGL(k_g_level_scatter, dim3(std::min<unsigned>(gc, kLevelBlocks)), dim3(256), 0, st, ts,
                      ctx->level, cursor, ctx->order);
   MARL_HIP(ctx, hipGetLastError());
-  // the ordered pass on the AoS view
-  if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
-  for (int L = 0; L < ctx->graph_levels; ++L)
-    hipLaunchKernelGGL(k_g_kinetics, dim3(gk), dim3(kKinBlock), 0, st, L, ts, ctx->order, cells, rules,
-                       ctx->staging, ctx->light, ctx->nbmask, ctx->events, p);
+  // the ordered pass on the AoS view, or on the quads in place (the layout
+  // chosen when the tick was prepared; it is part of the graph's key)
+  if (ctx->kin_quads) {
+    for (int L = 0; L < ctx->graph_levels; ++L)
+      hipLaunchKernelGGL(k_g_kinetics<true>, dim3(gk), dim3(kKinBlock), 0, st, L, ts, ctx->order, cells, rules,
+                         ctx->field[ctx->cur], ctx->light, ctx->nbmask, ctx->events, p);
+  } else {
+    if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
+    for (int L = 0; L < ctx->graph_levels; ++L)
+      hipLaunchKernelGGL(k_g_kinetics<false>, dim3(gk), dim3(kKinBlock), 0, st, L, ts, ctx->order, cells, rules,
+                         ctx->staging, ctx->light, ctx->nbmask, ctx->events, p);
+    if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
+  }
   MARL_HIP(ctx, hipGetLastError());
-  if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
   // fate, births with mutation, deaths and compaction
   int32_t* target = ctx->flag_scan;
   int32_t* bflag = target + (cap + 1);
@@ -1311,7 +1357,7 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
     marl_sim_params sim;
     int64_t cap;
     const void *cells, *rules, *f0, *f1, *staging;
-    int32_t tile, cur, cbuf, sweeps, levels;
+    int32_t tile, cur, cbuf, sweeps, levels, kin_quads;
   } k;
   memset(&k, 0, sizeof(k));
   k.sim = *sim;
@@ -1326,6 +1372,7 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
   k.cbuf = ctx->cbuf;
   k.sweeps = ctx->graph_sweeps;
   k.levels = ctx->graph_levels;
+  k.kin_quads = ctx->kin_quads ? 1 : 0;
   const uint8_t* b = reinterpret_cast<const uint8_t*>(&k);
   return std::vector<uint8_t>(b, b + sizeof(k));
 }
@@ -1342,6 +1389,7 @@ int marl_graph_prepare(marl_ctx* ctx, const marl_sim_params* sim) {
   if (int rc = marl_ensure_staging(ctx)) return rc;
   if (!ctx->tdev) MARL_HIP(ctx, hipMalloc(&ctx->tdev, sizeof(TickDev)));
   const int64_t n = ctx->n_cells;
+  marl_choose_cell_layout(ctx);
   // room for this tick's births; generous so that growth rarely re-captures
   if (int rc = marl_ensure_cell_capacity(ctx, std::max<int64_t>(2 * n + 1024, 4 * n))) return rc;
   return scan_reserve(ctx, (int)ctx->cap + 1);

@@ -14,6 +14,7 @@ struct CellParams {
   int32_t div_dist;
   int32_t X, Y, Z;
   uint64_t seed, stream;
+  uint64_t fplane;  // floats between species-quad planes of the resident field (4 x voxels)
 };
 
 __host__ __device__ inline float clamp_rs(float x, float lo, float hi) {  // Rust f32::clamp
@@ -64,23 +65,34 @@ __device__ __forceinline__ marl_cell_state load_cell(const marl_cell_state* c) {
   return out;
 }
 
+// A voxel's species quad k (species 4k..4k+3) in the field the cell pass works
+// on: the AoS view (one 48-byte record per voxel, quads 16 bytes apart) or,
+// with QUADS, the resident species quads in place (planes p.fplane floats
+// apart, 16 bytes per voxel in each).
+template <bool QUADS>
+__device__ __forceinline__ float4* field_quad(float* field, size_t w, int k, const CellParams& p) {
+  return QUADS ? reinterpret_cast<float4*>(field + (size_t)k * p.fplane + w * 4)
+               : reinterpret_cast<float4*>(field + w * MARL_S_EXT) + k;
+}
+
 // Mean environment over the free face neighbours (spatial.rs:56-77; faces in
-// FACE_OFFSETS order +x,-x,+y,-y,+z,-z, spatial.rs:18-43) from the resident
-// AoS field (one 48-byte record per voxel): all six face records are requested
-// at once (non-free faces read the cell's own voxel and are ignored), the sum
-// runs over the free faces in face order, then divides; zeros when enclosed.
+// FACE_OFFSETS order +x,-x,+y,-y,+z,-z, spatial.rs:18-43): all six face
+// records are requested at once (non-free faces read the cell's own voxel and
+// are ignored), the sum runs over the free faces in face order, then divides;
+// zeros when enclosed.
+template <bool QUADS>
 __device__ inline void gather_env6(const float* __restrict__ field, size_t v, uint32_t m, const CellParams& p,
                                    float* E) {
   const size_t XY = (size_t)p.X * p.Y;
   const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
+  float* f = const_cast<float*>(field);
   float4 a[6], b[6], cc[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const size_t w = ((m >> k) & 1u) ? (size_t)((int64_t)v + off[k]) : v;
-    const float4* fv = reinterpret_cast<const float4*>(field + w * MARL_S_EXT);
-    a[k] = fv[0];
-    b[k] = fv[1];
-    cc[k] = fv[2];
+    a[k] = *field_quad<QUADS>(f, w, 0, p);
+    b[k] = *field_quad<QUADS>(f, w, 1, p);
+    cc[k] = *field_quad<QUADS>(f, w, 2, p);
   }
   float acc[MARL_S_EXT];
 #pragma unroll
@@ -102,6 +114,7 @@ __device__ inline void gather_env6(const float* __restrict__ field, size_t v, ui
 // Split deltas into the free face neighbours with the max(0) clamp
 // (spatial.rs:87-110, field.rs:57-62), lost when enclosed.  Each free face is
 // a different voxel, so the order of the updates does not matter.
+template <bool QUADS>
 __device__ inline void scatter_deltas6(float* __restrict__ field, size_t v, uint32_t m, const CellParams& p,
                                        const float* D) {
   const int n = __builtin_popcount(m);
@@ -115,15 +128,18 @@ __device__ inline void scatter_deltas6(float* __restrict__ field, size_t v, uint
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     if (!((m >> k) & 1u)) continue;
-    float4* fv = reinterpret_cast<float4*>(field + (size_t)((int64_t)v + off[k]) * MARL_S_EXT);
-    float4 a = fv[0], b = fv[1], cc = fv[2];
+    const size_t w = (size_t)((int64_t)v + off[k]);
+    float4* qa = field_quad<QUADS>(field, w, 0, p);
+    float4* qb = field_quad<QUADS>(field, w, 1, p);
+    float4* qc = field_quad<QUADS>(field, w, 2, p);
+    float4 a = *qa, b = *qb, cc = *qc;
     a.x = fmaxf(a.x + split[0], 0.0f); a.y = fmaxf(a.y + split[1], 0.0f);
     a.z = fmaxf(a.z + split[2], 0.0f); a.w = fmaxf(a.w + split[3], 0.0f);
     b.x = fmaxf(b.x + split[4], 0.0f); b.y = fmaxf(b.y + split[5], 0.0f);
     b.z = fmaxf(b.z + split[6], 0.0f); b.w = fmaxf(b.w + split[7], 0.0f);
     cc.x = fmaxf(cc.x + split[8], 0.0f); cc.y = fmaxf(cc.y + split[9], 0.0f);
     cc.z = fmaxf(cc.z + split[10], 0.0f); cc.w = fmaxf(cc.w + split[11], 0.0f);
-    fv[0] = a; fv[1] = b; fv[2] = cc;
+    *qa = a; *qb = b; *qc = cc;
   }
 }
 

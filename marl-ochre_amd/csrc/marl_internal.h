@@ -256,6 +256,8 @@ struct marl_ctx {
   int graph_flip[2] = {0, 0};         // the graph swaps the field buffers
   int graph_sweeps = 64, graph_levels = 64;  // schedule budget of the captured tick
   bool graph_auto = true;                    // budget follows the schedule's needs
+  int cell_layout = MARL_CELL_LAYOUT_AUTO;    // field layout of the plain cell pass
+  bool kin_quads = false;                    // the layout last chosen: quads in place
   int relax_sweeps = 4;          // sweeps the last eager schedule needed to converge
   marl_sim_params last_sim{};
 
@@ -312,6 +314,8 @@ int marl_planes(marl_ctx* ctx, bool to_quads, float* quads);
 int marl_pack_occ(marl_ctx* ctx);
 int marl_unpack_occ(marl_ctx* ctx);
 int marl_ensure_staging(marl_ctx* ctx);
+// Picks (and records in ctx->kin_quads) the plain cell pass's field layout.
+bool marl_choose_cell_layout(marl_ctx* ctx);
 int marl_launch_copy_probe(marl_ctx* ctx);
 
 // stencil tiles: rows per thread and z-chunk

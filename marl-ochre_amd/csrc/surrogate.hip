@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(kSBlock) k_surr_features(const int32_t* __rest
     float* E = sE + threadIdx.x * 13;
     const uint32_t m = nbmask[i];
     const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
-    gather_env6(field, v, m, p, E);
+    gather_env6<false>(field, v, m, p, E);
     const int cls = class_of(topology_hash(r), cl);
     float f[kFeat];
     make_features(c.internal, c, E, light[v], r, cls < 0 ? 0 : cls, f);
@@ -130,7 +130,7 @@ __device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __rest
   for (int k = 0; k < MARL_M_INT; ++k) I[k] = c.internal[k];
   const uint32_t m = nbmask[i];
   const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
-  gather_env6(field, v, m, p, E);
+  gather_env6<false>(field, v, m, p, E);
   const float lt = light[v];
 
   const float* tmean = norm + 2 * kFeat;
@@ -204,7 +204,7 @@ __device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __rest
   for (int k = 0; k < MARL_M_INT; ++k) c.internal[k] = I[k];
   cells[i] = c;
   events[i] = ev;
-  scatter_deltas6(field, v, m, p, D);
+  scatter_deltas6<false>(field, v, m, p, D);
 }
 
 // Applies one level: surrogate result where eligible and not near a fate

@@ -205,6 +205,13 @@ class MarlGpu:
         d["host_finished"] = hf.value
         return d
 
+    CELL_LAYOUTS = {"auto": 0, "aos": 1, "quads": 2}
+
+    def set_cell_layout(self, layout: str = "auto") -> None:
+        """Field layout of the plain cell pass: "aos" (transposed view),
+        "quads" (resident quads in place) or "auto"; results are identical."""
+        self._chk(self.lib.marl_set_cell_layout(self.h, self.CELL_LAYOUTS[layout]))
+
     def set_graph_limits(self, sweeps: int = 0, levels: int = 0) -> None:
         """Schedule budget of the captured tick; (0, 0) = automatic (the default)."""
         self._chk(self.lib.marl_set_graph_limits(self.h, int(sweeps), int(levels)))

@@ -62,7 +62,12 @@ def assert_cells_equal(got_c, got_r, want_c, want_r):
     assert not bad_fields, "\n".join(bad_fields)
 
 
-def test_cell_pass_bit_identical(pkg, orc):
+LAYOUTS = pytest.mark.parametrize("layout", ["aos", "quads"])
+
+
+@LAYOUTS
+def test_cell_pass_bit_identical(pkg, orc, layout):
+    """The pass on the transposed AoS view and on the resident quads in place."""
     A = pkg._abi
     dims = (64, 48, 32)
     cells, rules = population(A, dims, 12000, 1)
@@ -73,6 +78,7 @@ def test_cell_pass_bit_identical(pkg, orc):
     sim = pkg.SimulationConfig()
     f_w, c_w, ev_w = orc.cell_pass(dims, sim, field, light, cells, rules)
     with pkg.MarlGpu(dims) as g:
+        g.set_cell_layout(layout)
         g.upload_field_aos(field)
         g.upload_cells(cells, rules)
         g.upload_light(light)
@@ -86,9 +92,10 @@ def test_cell_pass_bit_identical(pkg, orc):
     assert np.array_equal(f_g, f_w), np.count_nonzero(f_g != f_w)
 
 
-def _run_ticks(pkg, orc, dims, field, cells, rules, sim, n_ticks, check_every=1):
+def _run_ticks(pkg, orc, dims, field, cells, rules, sim, n_ticks, check_every=1, layout="auto"):
     stats_seen = {"divisions": 0, "deaths": 0}
     with pkg.MarlGpu(dims) as g:
+        g.set_cell_layout(layout)
         g.upload_field_aos(field)
         g.upload_cells(cells, rules)
         f, c, r = field, cells, rules
@@ -108,18 +115,20 @@ def _run_ticks(pkg, orc, dims, field, cells, rules, sim, n_ticks, check_every=1)
     return stats_seen
 
 
-def test_ticks_with_divisions_deaths_and_mutation_bit_identical(pkg, orc):
+@LAYOUTS
+def test_ticks_with_divisions_deaths_and_mutation_bit_identical(pkg, orc, layout):
     A = pkg._abi
     dims = (64, 32, 24)
     cells, rules = population(A, dims, 4000, 2)
     rng = np.random.default_rng(4)
     field = rng.random(np.prod(dims) * 12).astype(F)
     sim = pkg.SimulationConfig(rng_seed=77, diffusion_substeps=4)
-    seen = _run_ticks(pkg, orc, dims, field, cells, rules, sim, 6)
+    seen = _run_ticks(pkg, orc, dims, field, cells, rules, sim, 6, layout=layout)
     assert seen["divisions"] > 50 and seen["deaths"] > 50, seen
 
 
-def test_crowded_population_and_walls(pkg, orc):
+@LAYOUTS
+def test_crowded_population_and_walls(pkg, orc, layout):
     """Dense blocks (enclosed cells lose their deltas), cells on every wall,
     birth collisions (two parents picking one voxel)."""
     A = pkg._abi
@@ -144,7 +153,7 @@ def test_crowded_population_and_walls(pkg, orc):
     cells["prep_remaining"] = 1   # everyone divides on tick 0 unless it dies
     field = rng.random(np.prod(dims) * 12).astype(F)
     sim = pkg.SimulationConfig(rng_seed=5)
-    seen = _run_ticks(pkg, orc, dims, field, cells, rules, sim, 3)
+    seen = _run_ticks(pkg, orc, dims, field, cells, rules, sim, 3, layout=layout)
     assert seen["divisions"] > 10
 
 

@@ -12,9 +12,11 @@ pytestmark = pytest.mark.gpu
 F = np.float32
 
 
-def _compare_runs(pkg, dims, field, cells, rules, sim, n_ticks, limits=None, orc=None):
+def _compare_runs(pkg, dims, field, cells, rules, sim, n_ticks, limits=None, orc=None, layouts=("auto", "auto")):
     seen = {"divisions": 0, "deaths": 0, "host_finished": 0}
     with pkg.MarlGpu(dims) as ge, pkg.MarlGpu(dims) as gg:
+        ge.set_cell_layout(layouts[0])
+        gg.set_cell_layout(layouts[1])
         for g in (ge, gg):
             g.upload_field_aos(field)
             g.upload_cells(cells, rules)
@@ -41,14 +43,16 @@ def _compare_runs(pkg, dims, field, cells, rules, sim, n_ticks, limits=None, orc
     return seen
 
 
-def test_graph_tick_bit_identical_with_divisions_and_deaths(pkg, orc):
+@pytest.mark.parametrize("layouts", [("aos", "quads"), ("quads", "aos")])
+def test_graph_tick_bit_identical_with_divisions_and_deaths(pkg, orc, layouts):
+    """Eager ticks on one field layout of the cell pass, captured ticks on the other."""
     A = pkg._abi
     dims = (64, 32, 24)
     cells, rules = population(A, dims, 4000, 2)
     rng = np.random.default_rng(4)
     field = rng.random(np.prod(dims) * 12).astype(F)
     sim = pkg.SimulationConfig(rng_seed=77, diffusion_substeps=4)
-    seen = _compare_runs(pkg, dims, field, cells, rules, sim, 6, orc=orc)
+    seen = _compare_runs(pkg, dims, field, cells, rules, sim, 6, orc=orc, layouts=layouts)
     assert seen["divisions"] > 50 and seen["deaths"] > 50, seen
     assert seen["host_finished"] == 0
 

@@ -386,6 +386,7 @@ def ensemble_bench(pkg, ticks=5, warm=2):
     try:
         with pkg.MarlEnsemble(gs) as ens:
             tile = ens.autotune(sim)
+            shape = ens.launch_shape()
             ms_sub = ens.time_field(sim, 20)
             for g, (dims, field, cells, rules, _) in zip(gs, cols):
                 g.upload_field_aos(field)
@@ -424,6 +425,7 @@ def ensemble_bench(pkg, ticks=5, warm=2):
     alg = 96.125 * nvox
     res.update({
         "tile": tile,
+        "launch": shape,
         "batched_substep_us": round(ms_sub * 1e3, 1),
         "per_column_substeps_us": round(per_col_sub * 1e3, 1),
         "batched_substep_Gvoxel_species_per_s": round(nvox * 12 / (ms_sub * 1e-3) / 1e9, 1),

@@ -323,10 +323,16 @@ const char* marl_ensemble_last_error(const marl_ensemble* e);
 int marl_ensemble_tick(marl_ensemble* e, const marl_sim_params* sim, const uint64_t* seeds, int64_t tick0,
                        int32_t n_ticks, marl_tick_stats* stats, int32_t* host_finished, float* ms_out);
 int marl_ensemble_set_tile(marl_ensemble* e, int32_t tile);
+/* Grid of the batched stencil launch: 0 = persistent (as many blocks as fit
+ * the chip, each walking its share of the tiles), 1 = one block per tile. */
+int marl_ensemble_set_grid(marl_ensemble* e, int32_t per_tile);
 /* Average time of one batched substep over all columns (fields unchanged). */
 int marl_ensemble_time_field(marl_ensemble* e, const marl_sim_params* sim, int32_t n_substeps, float* ms_out);
-/* Times every stencil tile shape on the resident columns and keeps the fastest. */
+/* Times every stencil tile shape, with a persistent grid and with one block
+ * per tile, on the resident columns and keeps the fastest. */
 int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t* tile_out);
+/* The batched stencil launch: tile shape, grid blocks and logical tiles. */
+int marl_ensemble_launch_shape(const marl_ensemble* e, int32_t* tile, int32_t* grid_blocks, int32_t* stencil_tiles);
 
 /* ---- learned kinetics surrogate ------------------------------------------- */
 /* No reference counterpart: the reference integrates every cell's kinetics

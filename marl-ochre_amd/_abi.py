@@ -254,9 +254,11 @@ SIGNATURES = {
     "marl_ensemble_tick": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _P, _I64, _I32, _P,
                                           ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_float)]),
     "marl_ensemble_set_tile": (ctypes.c_int, [_P, _I32]),
+    "marl_ensemble_set_grid": (ctypes.c_int, [_P, _I32]),
     "marl_ensemble_time_field": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I32,
                                                 ctypes.POINTER(ctypes.c_float)]),
     "marl_ensemble_autotune": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), ctypes.POINTER(_I32)]),
+    "marl_ensemble_launch_shape": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "marl_set_kinetics_mode": (ctypes.c_int, [_P, _I32]),
     "marl_harvest_begin": (ctypes.c_int, [_P, _I64, ctypes.c_float]),
     "marl_harvest_end": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),

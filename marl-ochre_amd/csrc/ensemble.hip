@@ -26,6 +26,7 @@ struct marl_ensemble {
   std::vector<EnsColumn> hcols;
   EnsColumn* dcols = nullptr;
   int tile = 7;
+  int per_tile_grid = 0;  // 1: one block per stencil tile; 0: persistent grid
   int total_tiles = 0, total_light = 0, grid = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -59,13 +60,15 @@ void drop(marl_ensemble* e) {
   e->key.clear();
 }
 
-// The batched kernels' view of the columns for the ensemble's tile shape.
-int build_table(marl_ensemble* e) {
+// The batched kernels' view of the columns for the ensemble's tile shape and
+// grid, on the host.
+void host_table(const marl_ensemble* e, std::vector<EnsColumn>& cols, int* total_tiles, int* total_light,
+                int* grid) {
   const int n = (int)e->cols.size();
-  e->hcols.assign(n, EnsColumn{});
+  cols.assign(n, EnsColumn{});
   for (int c = 0; c < n; ++c) {
     marl_ctx* x = e->cols[c];
-    EnsColumn& t = e->hcols[c];
+    EnsColumn& t = cols[c];
     t.f[0] = x->field[0];
     t.f[1] = x->field[1];
     t.occ = x->occ;
@@ -76,8 +79,14 @@ int build_table(marl_ensemble* e) {
     t.XW = x->xw;
     t.cur = x->cur;
   }
-  marl_ens_layout(e->tile, e->hcols.data(), n, &e->total_tiles, &e->total_light);
-  e->grid = marl_stencil_ens_grid(e->tile, e->total_tiles);
+  marl_ens_layout(e->tile, cols.data(), n, total_tiles, total_light);
+  *grid = e->per_tile_grid ? *total_tiles : marl_stencil_ens_grid(e->tile, *total_tiles);
+}
+
+// ... and its device copy.
+int build_table(marl_ensemble* e) {
+  const int n = (int)e->cols.size();
+  host_table(e, e->hcols, &e->total_tiles, &e->total_light, &e->grid);
   if (!e->dcols) ENS_HIP(e, hipMalloc(&e->dcols, sizeof(EnsColumn) * n));
   ENS_HIP(e, hipMemcpyAsync(e->dcols, e->hcols.data(), sizeof(EnsColumn) * n, hipMemcpyHostToDevice, e->stream));
   ENS_HIP(e, hipStreamSynchronize(e->stream));
@@ -91,6 +100,7 @@ std::vector<uint8_t> ens_key(marl_ensemble* e, const std::vector<marl_sim_params
     k.insert(k.end(), kc.begin(), kc.end());
   }
   k.push_back((uint8_t)e->tile);
+  k.push_back((uint8_t)e->per_tile_grid);
   return k;
 }
 
@@ -203,6 +213,14 @@ int marl_ensemble_set_tile(marl_ensemble* e, int32_t tile) {
   return MARL_OK;
 }
 
+int marl_ensemble_set_grid(marl_ensemble* e, int32_t per_tile) {
+  if (!e) return MARL_ERR_INVALID_INPUT;
+  if (per_tile != 0 && per_tile != 1) return efail(e, MARL_ERR_INVALID_INPUT, "grid mode is 0 or 1");
+  if (per_tile != e->per_tile_grid) drop(e);
+  e->per_tile_grid = per_tile;
+  return MARL_OK;
+}
+
 int marl_ensemble_tick(marl_ensemble* e, const marl_sim_params* sim, const uint64_t* seeds, int64_t tick0,
                        int32_t n_ticks, marl_tick_stats* stats, int32_t* host_finished, float* ms_out) {
   if (!e || !sim) return MARL_ERR_INVALID_INPUT;
@@ -262,25 +280,42 @@ int marl_ensemble_time_field(marl_ensemble* e, const marl_sim_params* sim, int32
   return MARL_OK;
 }
 
-// The ensemble's tile shape, picked by timing every candidate on the resident
-// columns (the fields are left unchanged).
+// The ensemble's tile shape and grid (persistent or one block per tile),
+// picked by timing every candidate on the resident columns (the fields are
+// left unchanged).
 int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t* tile_out) {
   if (!e || !sim) return MARL_ERR_INVALID_INPUT;
-  int best = e->tile;
+  int best = e->tile, best_grid = e->per_tile_grid;
   float best_ms = 1e30f;
   for (int t = 0; t < marl_stencil_tile_count(); ++t) {
     if (!marl_tile_ensemble_ok(t)) continue;
-    e->tile = t;
-    float ms = 0.0f;
-    if (int rc = marl_ensemble_time_field(e, sim, 5, &ms)) return rc;
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = t;
+    for (int g = 0; g < 2; ++g) {
+      e->tile = t;
+      e->per_tile_grid = g;
+      float ms = 0.0f;
+      if (int rc = marl_ensemble_time_field(e, sim, 5, &ms)) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+        best_grid = g;
+      }
     }
   }
   drop(e);
   e->tile = best;
+  e->per_tile_grid = best_grid;
   if (tile_out) *tile_out = best;
+  return MARL_OK;
+}
+
+int marl_ensemble_launch_shape(const marl_ensemble* e, int32_t* tile, int32_t* grid_blocks, int32_t* stencil_tiles) {
+  if (!e) return MARL_ERR_INVALID_INPUT;
+  std::vector<EnsColumn> cols;
+  int total = 0, light = 0, grid = 0;
+  host_table(e, cols, &total, &light, &grid);
+  if (tile) *tile = e->tile;
+  if (grid_blocks) *grid_blocks = grid;
+  if (stencil_tiles) *stencil_tiles = total;
   return MARL_OK;
 }
 

@@ -404,19 +404,30 @@ __global__ void __launch_bounds__(256 * HB) k_stencil(const float* __restrict__ 
                                     p, sh);
 }
 
-// Ensembles: one persistent launch covers every column of the ensemble.  The
-// logical tiles of all columns are numbered back to back (column after
-// column, each in the single-column order), spread over the XCDs by the same
-// remap, and each block walks its share with a grid stride.
+// Ensembles: one launch covers every column of the ensemble.  The logical
+// tiles of all columns are numbered back to back (column after column, each
+// in the single-column order) and spread over the XCDs by the same remap.
+// The grid is either persistent (as many blocks as fit the chip, each walking
+// its share with a grid stride) or one block per tile, whichever the
+// ensemble's autotuner finds faster; the kernel is the same.
 template <int RY, bool SRC, int PD, int SG, int HB>
 __global__ void __launch_bounds__(256 * HB) k_stencil_ens(const EnsColumn* __restrict__ cols, int ncols, int total,
                                                           int parity, StencilParams p) {
   __shared__ StencilLds<RY, SG * HB> sh;
   constexpr int NG = MARL_S_EXT / (SG * HB);
+  int c = 0;  // column of the previous tile (a block's tiles increase along its walk)
   for (int w = blockIdx.x; w < total; w += gridDim.x) {
     int b = xcd_remap(w, total);
-    int c = 0;
-    while (c + 1 < ncols && cols[c + 1].tile_base <= b) ++c;
+    if (b < cols[c].tile_base) c = 0;
+    if (c + 1 < ncols && cols[c + 1].tile_base <= b) {
+      int lo = c + 1, hi = ncols - 1;  // binary search: the last column starting at or before b
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cols[mid].tile_base <= b) lo = mid;
+        else hi = mid - 1;
+      }
+      c = lo;
+    }
     const EnsColumn& col = cols[c];
     b -= col.tile_base;
     const int grp = b % NG;

@@ -448,6 +448,10 @@ class MarlEnsemble:
     def set_tile(self, tile: int) -> None:
         self._chk(self.lib.marl_ensemble_set_tile(self.h, int(tile)))
 
+    def set_grid(self, per_tile: bool) -> None:
+        """Batched stencil grid: persistent (False) or one block per tile (True)."""
+        self._chk(self.lib.marl_ensemble_set_grid(self.h, 1 if per_tile else 0))
+
     def time_field(self, sim, n_substeps: int = 20) -> float:
         p = _params(sim)
         ms = ctypes.c_float()
@@ -459,3 +463,9 @@ class MarlEnsemble:
         t = ctypes.c_int32()
         self._chk(self.lib.marl_ensemble_autotune(self.h, ctypes.byref(p), ctypes.byref(t)))
         return t.value
+
+    def launch_shape(self) -> dict:
+        """The batched stencil launch: tile id, grid blocks, logical tiles."""
+        t, g, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self.lib.marl_ensemble_launch_shape(self.h, ctypes.byref(t), ctypes.byref(g), ctypes.byref(n)))
+        return {"tile": t.value, "grid_blocks": g.value, "stencil_tiles": n.value}

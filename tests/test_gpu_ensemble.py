@@ -28,7 +28,7 @@ def _columns(pkg):
     return cols
 
 
-def _run(pkg, cols, sim, n_ticks, tile=None):
+def _run(pkg, cols, sim, n_ticks, tile=None, per_tile_grid=False):
     seeds = np.array([c[4] for c in cols], np.uint64)
     alone = [pkg.MarlGpu(c[0]) for c in cols]
     batched = [pkg.MarlGpu(c[0]) for c in cols]
@@ -41,6 +41,7 @@ def _run(pkg, cols, sim, n_ticks, tile=None):
         with pkg.MarlEnsemble(batched) as ens:
             if tile is not None:
                 ens.set_tile(tile)
+            ens.set_grid(per_tile_grid)
             for t in range(n_ticks):
                 stats, ms, hf = ens.tick(sim, seeds, t, 1)
                 assert ms > 0 and hf == 0
@@ -71,6 +72,14 @@ def test_ensemble_bit_identical_to_single_columns(pkg):
 def test_ensemble_every_tile_shape(pkg, tile):
     cols = _columns(pkg)[:3]
     _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=2), 2, tile=tile)
+
+
+@pytest.mark.parametrize("tile", [0, 7, 11, 15])
+def test_ensemble_one_block_per_tile(pkg, tile):
+    """The batched stencil with one block per logical tile instead of the
+    persistent grid: same kernel, same results."""
+    cols = _columns(pkg)
+    _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=2), 2, tile=tile, per_tile_grid=True)
 
 
 def test_ensemble_rejects_temporal_tiles(pkg):

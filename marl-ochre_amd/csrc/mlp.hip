@@ -207,88 +207,124 @@ __global__ void k_adam(size_t n, float* __restrict__ w, const float* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: 64 cells per 256-thread block, activations in LDS as bf16
-// [64][K + 8].  Wave w computes output columns [64w, 64w + 64) of every
-// hidden layer (2 x 2 MFMA tiles of 32x32), reading B operands from a packed
-// weight image in which lane l's 8 k-values of a (k-step, n-tile) fragment are
-// one 16-byte word: packed[((nt * KS) + ks) * 64 + l] for layer weights of
-// KS = K/16 k-steps.  The last layer (28 outputs, one 32-wide n-tile) is
-// computed by waves 0-1, one 32-row m-tile each.
-constexpr int kFM = 64;          // cells per block
+// Fused inference: 128 cells per 256-thread block, the activations in one LDS
+// buffer as bf16 [128 cells][K + 8] (66 KB: two blocks per CU, so that one
+// block's epilogue overlaps the other's MFMAs), overwritten layer by layer.  Each layer is computed transposed, D^T = W^T H^T: the
+// packed weight fragments are the MFMA A operand and the activation rows the
+// B operand, so an accumulator lane holds four consecutive features of one
+// cell per register group and the epilogue writes them as one 8-byte LDS
+// store (and, in the last layer, one 16-byte global store).  Wave w computes
+// all 128 cells of a hidden layer for features [64 w, +64) as 4 x 2 MFMA
+// tiles of 32x32, so each weight fragment feeds four MFMAs and each
+// activation fragment two, and the weights are read once per 128 cells.
+// packed[((nt * KS) + ks) * 64 + l] holds lane l's 8 k-values of a (k-step,
+// feature tile) weight fragment for KS = K/16 k-steps; the next k-step's
+// weight fragments are loaded before the current MFMAs.  SiLU uses the
+// hardware reciprocal: the result is rounded to bf16 anyway.
+constexpr int kFM = 128;           // cells per block
+constexpr int kInferThreads = 256;
 constexpr int kLDA = kHidden + 8;  // LDS row pitch (bf16)
 
-__device__ __forceinline__ void store_tile_act(uint16_t* out, int row0, int col0, const f32x16& acc,
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+static_assert(kOut % 4 == 0 && kHidden == 64 * (kInferThreads / 64) && kFM == 128 && kIn % 16 == 0, "inference tiling");
+
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+// accumulator lane l, register j of a transposed 32x32 tile: cell l % 32,
+// feature 8 (j / 4) + 4 (l / 32) + j % 4
+__device__ __forceinline__ void store_tile_act(uint16_t* act, int cell0, int feat0, const f32x16& acc,
                                                const float* __restrict__ bias, int lane) {
-  const int col = col0 + (lane & 31);
-  const float bc = bias[col];
+  const int cell = cell0 + (lane & 31);
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int row = row0 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-    out[row * kLDA + col] = to_bf16_bits(silu(acc[j] + bc));
+  for (int g = 0; g < 4; ++g) {
+    const int f = feat0 + 8 * g + 4 * (lane >> 5);
+    const float4 bc = *reinterpret_cast<const float4*>(bias + f);
+    bf16x4 v;
+    v.x = (__bf16)silu_fast(acc[4 * g + 0] + bc.x);
+    v.y = (__bf16)silu_fast(acc[4 * g + 1] + bc.y);
+    v.z = (__bf16)silu_fast(acc[4 * g + 2] + bc.z);
+    v.w = (__bf16)silu_fast(acc[4 * g + 3] + bc.w);
+    *reinterpret_cast<bf16x4*>(&act[cell * kLDA + f]) = v;
   }
 }
 
 template <int KS>
-__device__ __forceinline__ void hidden_layer(const uint16_t* in, uint16_t* out, const bf16x8* __restrict__ Wp,
+__device__ __forceinline__ void hidden_layer(uint16_t* act, const bf16x8* __restrict__ Wp,
                                              const float* __restrict__ bias, int wave, int lane) {
-  f32x16 acc[2][2] = {};
+  f32x16 acc[4][2] = {};
   const int half = lane >> 5, r = lane & 31;
-#pragma unroll 4
+  const int cell0 = 0, nt0 = 2 * wave;
+  const bf16x8* w0 = Wp + (size_t)nt0 * KS * 64 + lane;
+  const bf16x8* w1 = w0 + (size_t)KS * 64;
+  bf16x8 a0 = w0[0], a1 = w1[0];
+#pragma unroll
   for (int ks = 0; ks < KS; ++ks) {
-    bf16x8 a[2], b[2];
+    bf16x8 na0 = a0, na1 = a1;
+    if (ks + 1 < KS) {
+      na0 = w0[(ks + 1) * 64];
+      na1 = w1[(ks + 1) * 64];
+    }
+    bf16x8 h[4];
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) a[mt] = *reinterpret_cast<const bf16x8*>(&in[(mt * 32 + r) * kLDA + ks * 16 + 8 * half]);
+    for (int mt = 0; mt < 4; ++mt)
+      h[mt] = *reinterpret_cast<const bf16x8*>(&act[(cell0 + mt * 32 + r) * kLDA + ks * 16 + 8 * half]);
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt) b[nt] = Wp[((size_t)(wave * 2 + nt) * KS + ks) * 64 + lane];
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-      for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+    for (int mt = 0; mt < 4; ++mt) {
+      acc[mt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, h[mt], acc[mt][0], 0, 0, 0);
+      acc[mt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, h[mt], acc[mt][1], 0, 0, 0);
+    }
+    a0 = na0;
+    a1 = na1;
   }
+  __syncthreads();  // every wave has read this layer's input
 #pragma unroll
-  for (int mt = 0; mt < 2; ++mt)
+  for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt) store_tile_act(out, mt * 32, (wave * 2 + nt) * 32, acc[mt][nt], bias, lane);
+    for (int nt = 0; nt < 2; ++nt) store_tile_act(act, cell0 + mt * 32, (nt0 + nt) * 32, acc[mt][nt], bias, lane);
+  __syncthreads();
 }
 
-__global__ void __launch_bounds__(256) k_mlp_infer(int M, const uint16_t* __restrict__ X /* [M][kIn] bf16 */,
-                                                   const bf16x8* __restrict__ W0p, const bf16x8* __restrict__ W1p,
-                                                   const bf16x8* __restrict__ W2p, const bf16x8* __restrict__ W3p,
-                                                   const float* __restrict__ b0, const float* __restrict__ b1,
-                                                   const float* __restrict__ b2, const float* __restrict__ b3,
-                                                   float* __restrict__ Y /* [M][kOut] */) {
-  __shared__ uint16_t bufA[kFM * kLDA];
-  __shared__ uint16_t bufB[kFM * kLDA];
+__global__ void __launch_bounds__(kInferThreads) k_mlp_infer(int M, const uint16_t* __restrict__ X /* [M][kIn] bf16 */,
+                                                             const bf16x8* __restrict__ W0p,
+                                                             const bf16x8* __restrict__ W1p,
+                                                             const bf16x8* __restrict__ W2p,
+                                                             const bf16x8* __restrict__ W3p,
+                                                             const float* __restrict__ b0, const float* __restrict__ b1,
+                                                             const float* __restrict__ b2, const float* __restrict__ b3,
+                                                             float* __restrict__ Y /* [M][kOut] */) {
+  __shared__ uint16_t act[kFM * kLDA];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m0 = blockIdx.x * kFM;
-  // load the 64 x 80 input tile (zero rows past M)
-  for (int e = tid; e < kFM * kIn; e += 256) {
-    const int rr = e / kIn, cc = e % kIn;
-    bufA[rr * kLDA + cc] = (m0 + rr < M) ? X[(size_t)(m0 + rr) * kIn + cc] : (uint16_t)0;
+  // the 128 x 80 input tile, 16 bytes per thread and step (zero rows past M);
+  // rows are 160 bytes, so every chunk is 16-byte aligned
+  constexpr int kChunks = kIn / 8;
+  for (int e = tid; e < kFM * kChunks; e += kInferThreads) {
+    const int rr = e / kChunks, cc = (e % kChunks) * 8;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (m0 + rr < M) v = *reinterpret_cast<const uint4*>(X + (size_t)(m0 + rr) * kIn + cc);
+    *reinterpret_cast<uint4*>(&act[rr * kLDA + cc]) = v;
   }
   __syncthreads();
-  hidden_layer<kIn / 16>(bufA, bufB, W0p, b0, wave, lane);
-  __syncthreads();
-  hidden_layer<kHidden / 16>(bufB, bufA, W1p, b1, wave, lane);
-  __syncthreads();
-  hidden_layer<kHidden / 16>(bufA, bufB, W2p, b2, wave, lane);
-  __syncthreads();
-  if (wave < 2) {
-    f32x16 acc = {};
-    const int half = lane >> 5, r = lane & 31;
+  hidden_layer<kIn / 16>(act, W0p, b0, wave, lane);
+  hidden_layer<kHidden / 16>(act, W1p, b1, wave, lane);
+  hidden_layer<kHidden / 16>(act, W2p, b2, wave, lane);
+  // output layer, transposed like the others: wave w takes cells [32w, 32w + 32)
+  f32x16 acc = {};
+  const int half = lane >> 5, r = lane & 31;
 #pragma unroll 4
-    for (int ks = 0; ks < kHidden / 16; ++ks) {
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&bufB[(wave * 32 + r) * kLDA + ks * 16 + 8 * half]);
-      const bf16x8 b = W3p[(size_t)ks * 64 + lane];
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-    }
-    const int col = lane & 31;
-    if (col < kOut) {
-      const float bc = b3[col];
+  for (int ks = 0; ks < kHidden / 16; ++ks) {
+    const bf16x8 h = *reinterpret_cast<const bf16x8*>(&act[(wave * 32 + r) * kLDA + ks * 16 + 8 * half]);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W3p[(size_t)ks * 64 + lane], h, acc, 0, 0, 0);
+  }
+  const int cell = m0 + wave * 32 + r;
+  if (cell < M) {
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int row = m0 + wave * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-        if (row < M) Y[(size_t)row * kOut + col] = acc[j] + bc;
+    for (int g = 0; g < 4; ++g) {
+      const int f = 8 * g + 4 * half;  // features f..f+3 (kOut = 28 is a multiple of 4)
+      if (f < kOut) {
+        const float4 bc = *reinterpret_cast<const float4*>(b3 + f);
+        *reinterpret_cast<float4*>(Y + (size_t)cell * kOut + f) =
+            make_float4(acc[4 * g] + bc.x, acc[4 * g + 1] + bc.y, acc[4 * g + 2] + bc.z, acc[4 * g + 3] + bc.w);
       }
     }
   }
@@ -493,7 +529,12 @@ int mlp_infer(marl_ctx* ctx, MlpState& s, int M, const float* xin, float* yout) 
 int mlp_infer_bf16(marl_ctx* ctx, MlpState& s, int M, const uint16_t* xb, float* yout) {
   if (M <= 0) return MARL_OK;
   const MlpLayout L = mlp_layout();
-  hipLaunchKernelGGL(k_mlp_infer, dim3(g1(M, kFM)), dim3(256), 0, ctx->stream, M, xb,
+  // the fused kernel moves inputs, biases and outputs in 16-byte vectors
+  auto misaligned = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) != 0; };
+  bool bad = misaligned(xb) || misaligned(yout);
+  for (int l = 0; l < 4; ++l) bad = bad || misaligned(s.w + L.b[l]);
+  if (bad) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "surrogate inference buffers must be 16-byte aligned");
+  hipLaunchKernelGGL(k_mlp_infer, dim3(g1(M, kFM)), dim3(kInferThreads), 0, ctx->stream, M, xb,
                      reinterpret_cast<const bf16x8*>(s.pk[0]), reinterpret_cast<const bf16x8*>(s.pk[1]),
                      reinterpret_cast<const bf16x8*>(s.pk[2]), reinterpret_cast<const bf16x8*>(s.pk[3]),
                      s.w + L.b[0], s.w + L.b[1], s.w + L.b[2], s.w + L.b[3], yout);

@@ -42,7 +42,7 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("rows", [64, 1, 200])
+@pytest.mark.parametrize("rows", [64, 1, 200, 700])
 def test_forward_matches_oracle(pkg, net, rows):
     _, params, x0, _ = net
     rng = np.random.default_rng(rows)

@@ -218,9 +218,9 @@ __global__ void k_adam(size_t n, float* __restrict__ w, const float* __restrict_
 // tiles of 32x32, so each weight fragment feeds four MFMAs and each
 // activation fragment two, and the weights are read once per 128 cells.
 // packed[((nt * KS) + ks) * 64 + l] holds lane l's 8 k-values of a (k-step,
-// feature tile) weight fragment for KS = K/16 k-steps; the next k-step's
-// weight fragments are loaded before the current MFMAs.  SiLU uses the
-// hardware reciprocal: the result is rounded to bf16 anyway.
+// feature tile) weight fragment for KS = K/16 k-steps; weight fragments are
+// loaded two k-steps and activation fragments one k-step ahead of the MFMAs.
+// SiLU uses the hardware reciprocal: the result is rounded to bf16 anyway.
 constexpr int kFM = 128;           // cells per block
 constexpr int kInferThreads = 256;
 constexpr int kLDA = kHidden + 8;  // LDS row pitch (bf16)
@@ -256,25 +256,35 @@ __device__ __forceinline__ void hidden_layer(uint16_t* act, const bf16x8* __rest
   const int cell0 = 0, nt0 = 2 * wave;
   const bf16x8* w0 = Wp + (size_t)nt0 * KS * 64 + lane;
   const bf16x8* w1 = w0 + (size_t)KS * 64;
-  bf16x8 a0 = w0[0], a1 = w1[0];
+  const uint16_t* hrow = act + (cell0 + r) * kLDA + 8 * half;
+  // weight fragments two k-steps ahead (a ring of three), activation
+  // fragments one k-step ahead; the loop is unrolled, so ring slots are
+  // registers
+  bf16x8 wa[3][2], h[2][4];
+  wa[0][0] = w0[0];
+  wa[0][1] = w1[0];
+  if (KS > 1) {
+    wa[1][0] = w0[64];
+    wa[1][1] = w1[64];
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) h[0][mt] = *reinterpret_cast<const bf16x8*>(hrow + mt * 32 * kLDA);
 #pragma unroll
   for (int ks = 0; ks < KS; ++ks) {
-    bf16x8 na0 = a0, na1 = a1;
-    if (ks + 1 < KS) {
-      na0 = w0[(ks + 1) * 64];
-      na1 = w1[(ks + 1) * 64];
+    if (ks + 2 < KS) {
+      wa[(ks + 2) % 3][0] = w0[(ks + 2) * 64];
+      wa[(ks + 2) % 3][1] = w1[(ks + 2) * 64];
     }
-    bf16x8 h[4];
+    if (ks + 1 < KS) {
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-      h[mt] = *reinterpret_cast<const bf16x8*>(&act[(cell0 + mt * 32 + r) * kLDA + ks * 16 + 8 * half]);
+      for (int mt = 0; mt < 4; ++mt)
+        h[(ks + 1) & 1][mt] = *reinterpret_cast<const bf16x8*>(hrow + mt * 32 * kLDA + (ks + 1) * 16);
+    }
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-      acc[mt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, h[mt], acc[mt][0], 0, 0, 0);
-      acc[mt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, h[mt], acc[mt][1], 0, 0, 0);
+      acc[mt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[ks % 3][0], h[ks & 1][mt], acc[mt][0], 0, 0, 0);
+      acc[mt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[ks % 3][1], h[ks & 1][mt], acc[mt][1], 0, 0, 0);
     }
-    a0 = na0;
-    a1 = na1;
   }
   __syncthreads();  // every wave has read this layer's input
 #pragma unroll

@@ -31,6 +31,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Gvoxel-species/s stencil; Mcell-updates/s plain vs surrogate at ≤1e-3 rel err"
 HBM_PEAK_GBS = 8000.0
 STENCIL_BYTES_PER_VOXEL = 96.125
+LIGHT_BYTES_PER_VOXEL = 8.125  # s4 read, intensity written, mask bit
 
 
 def log(*a):
@@ -588,28 +589,36 @@ def main():
     copy_gbs = 20 * by.value / (ms.value * 1e-3) / 1e9
     log(f"copy probe: {copy_gbs:.0f} GB/s")
 
+    # one step = one tick's field phase as the tick runs it: 8 substeps, the
+    # light scan fused into the last when the tile's z-chunk spans the column
+    fused = zc >= dims[2] and g.tile_substeps(tile) == 1
     for _ in range(args.warmup):
-        g.time_diffuse(sim, sim.diffusion_substeps)
-        g.time_light(sim, 1)
+        g.time_field_phase(sim, 1)
     g.sync()
 
-    stencil_ms = 0.0
-    light_ms = 0.0
+    phase_ms = 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        stencil_ms += g.time_diffuse(sim, sim.diffusion_substeps)
-        light_ms += g.time_light(sim, 1)
+        phase_ms += g.time_field_phase(sim, 1)
     g.sync()
     wall = time.perf_counter() - t0
 
     n_sub = args.steps * sim.diffusion_substeps
     ms_per_step = wall * 1e3 / args.steps
     value = n_sub * nvox * 12 / wall / 1e9
-    per_launch_s = stencil_ms * 1e-3 / n_sub
-    alg_bytes = STENCIL_BYTES_PER_VOXEL * nvox
+    # the dominant kernel is k_stencil: with the light fused, the step is its
+    # 8 launches, the last also writing the light (4 B per voxel; its s4 and
+    # mask reads are the stencil's own); unfused, the step also holds k_light
+    per_launch_s = phase_ms * 1e-3 / n_sub
+    light_bytes = (4.0 if fused else LIGHT_BYTES_PER_VOXEL) * nvox
+    alg_bytes = STENCIL_BYTES_PER_VOXEL * nvox + light_bytes / sim.diffusion_substeps
     achieved = alg_bytes / per_launch_s / 1e9
-    log(f"stencil {per_launch_s * 1e6:.1f} us/substep ({achieved:.0f} GB/s algorithmic), "
-        f"light {light_ms * 1e3 / args.steps:.1f} us/tick")
+    # the separate light kernel and plain substeps, for reference
+    light_ms = g.time_light(sim, 20) / 20
+    plain_sub_ms = g.time_diffuse(sim, 5 * sim.diffusion_substeps) / (5 * sim.diffusion_substeps)
+    log(f"field phase {phase_ms / args.steps * 1e3:.1f} us/tick (light fused: {fused}), "
+        f"{per_launch_s * 1e6:.1f} us/launch ({achieved:.0f} GB/s algorithmic); plain substep "
+        f"{plain_sub_ms * 1e3:.1f} us, separate light {light_ms * 1e3:.1f} us")
 
     # HBM bytes per launch from the committed rocprofv3 PMC summary, valid for
     # the tile it was measured on
@@ -648,12 +657,16 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "k_stencil (one masked substep)",
+            "kernel": "k_stencil (one masked substep; the last of a tick also writes the light scan)" if fused
+                      else "k_stencil (one masked substep) + k_light per tick",
             "algorithmic_bytes_per_launch": alg_bytes,
             "avg_launch_us": round(per_launch_s * 1e6, 2),
             "measured_copy_GBs": round(copy_gbs, 1),
+            "plain_substep_us": round(plain_sub_ms * 1e3, 2),
+            "plain_substep_frac": round(STENCIL_BYTES_PER_VOXEL * nvox / (plain_sub_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         },
-        "light_us_per_tick": round(light_ms * 1e3 / args.steps, 2),
+        "light_fused": fused,
+        "light_us_per_tick_separate_kernel": round(light_ms * 1e3, 2),
     }
     g.close()
     if args.kinetics_ticks > 0:

@@ -256,6 +256,12 @@ int marl_hgt_transfer(marl_ctx* ctx, const int32_t* pairs, size_t n_pairs, uint6
  * GPU time between hipEvents recorded around them. */
 int marl_time_diffuse(marl_ctx* ctx, const marl_sim_params* sim, int32_t n_substeps, float* ms_out);
 int marl_time_light(marl_ctx* ctx, const marl_sim_params* sim, int32_t n_updates, float* ms_out);
+/* n_ticks field phases as a tick runs them (boundary sources when
+ * with_sources, diffusion_substeps substeps, the light scan; sources fused
+ * into the first substep and, when the tile's z-chunk spans the column, the
+ * light scan into the last), timed like marl_time_diffuse. */
+int marl_time_field_phase(marl_ctx* ctx, const marl_sim_params* sim, int32_t n_ticks, int32_t with_sources,
+                          float* ms_out);
 /* Choose stencil tile shape by timing the candidates on the resident field
  * (the project's autotuner).  Writes the chosen tile id. */
 int marl_autotune_stencil(marl_ctx* ctx, const marl_sim_params* sim, int32_t* tile_out);

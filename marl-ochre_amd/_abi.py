@@ -232,6 +232,8 @@ SIGNATURES = {
     "marl_mass_budget": (ctypes.c_int, [_P, _P, _SZ]),
     "marl_time_diffuse": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I32, ctypes.POINTER(ctypes.c_float)]),
     "marl_time_light": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I32, ctypes.POINTER(ctypes.c_float)]),
+    "marl_time_field_phase": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I32, _I32,
+                                             ctypes.POINTER(ctypes.c_float)]),
     "marl_autotune_stencil": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), ctypes.POINTER(_I32)]),
     "marl_set_stencil_tile": (ctypes.c_int, [_P, _I32]),
     "marl_get_stencil_tile": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),

@@ -1270,14 +1270,10 @@ int marl_enqueue_tick_begin(marl_ctx* ctx, const marl_sim_params* sim) {
   return MARL_OK;
 }
 
-// Substeps with the boundary sources fused into the first, then light.
+// Substeps with the boundary sources fused into the first, then light (fused
+// into the last substep where the tile allows).
 int marl_enqueue_field_phase(marl_ctx* ctx, const marl_sim_params* sim) {
-  if (sim->diffusion_substeps > 0) {
-    if (int rc = marl_launch_substeps(ctx, sim, sim->diffusion_substeps, true, ctx->tile)) return rc;
-  } else {
-    if (int rc = marl_launch_boundary(ctx, sim)) return rc;
-  }
-  return marl_launch_light(ctx, sim);
+  return marl_launch_field_phase(ctx, sim, true);
 }
 
 // Schedule, the ordered pass, fate, births, deaths and compaction.
@@ -1636,14 +1632,10 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
     const CellParams p = cell_params(ctx, sim, (uint64_t)(tick0 + t));
     MARL_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
     // occupancy from the live cells (lib.rs:174-179); sources fused into the
-    // first substep (field.rs:231-250 then :212-217); light (lib.rs:199)
+    // first substep (field.rs:231-250 then :212-217); light (lib.rs:199),
+    // fused into the last substep where the tile allows
     if (int rc = marl_launch_occupancy_from_cells(ctx)) return rc;
-    if (sim->diffusion_substeps > 0) {
-      if (int rc = marl_launch_substeps(ctx, sim, sim->diffusion_substeps, true, ctx->tile)) return rc;
-    } else {
-      if (int rc = marl_launch_boundary(ctx, sim)) return rc;
-    }
-    if (int rc = marl_launch_light(ctx, sim)) return rc;
+    if (int rc = marl_launch_field_phase(ctx, sim, true)) return rc;
     MARL_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
     int32_t levels = 0;
     if (int rc = run_cell_pass(ctx, p, &levels, ev[2])) return rc;

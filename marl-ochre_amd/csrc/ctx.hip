@@ -34,6 +34,9 @@ StencilParams marl_stencil_params(const marl_sim_params* sim, int apply_sources)
   p.src_c = sim->source_rate_carbon;
   p.src_red = sim->source_rate_reductant;
   p.c_max = sim->c_max;
+  p.fuse_light = 0;
+  p.l_i0 = p.l_e_cell = p.l_nchem = p.l_floor = 0.0f;
+  p.light = nullptr;
   return p;
 }
 
@@ -273,6 +276,19 @@ int marl_time_diffuse(marl_ctx* ctx, const marl_sim_params* sim, int32_t n, floa
   if (sim->diffusion_substeps == 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "diffusion_substeps == 0");
   MARL_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (int rc = marl_launch_substeps(ctx, sim, n, false, ctx->tile)) return rc;
+  MARL_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  MARL_HIP(ctx, hipEventSynchronize(ctx->ev1));
+  MARL_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+  return MARL_OK;
+}
+
+int marl_time_field_phase(marl_ctx* ctx, const marl_sim_params* sim, int32_t n_ticks, int32_t with_sources,
+                          float* ms) {
+  if (!ctx || !ms || n_ticks < 0) return MARL_ERR_INVALID_INPUT;
+  if (int rc = marl_check_params(ctx, sim)) return rc;
+  MARL_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int k = 0; k < n_ticks; ++k)
+    if (int rc = marl_launch_field_phase(ctx, sim, with_sources != 0)) return rc;
   MARL_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   MARL_HIP(ctx, hipEventSynchronize(ctx->ev1));
   MARL_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));

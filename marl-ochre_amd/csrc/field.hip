@@ -197,8 +197,10 @@ struct StencilLds {
 // One tile (tx, ty, tz) of block species group grp of one substep.  The
 // block has HB groups of 4 waves; each takes SG species of the block's HB*SG,
 // and all of them share the LDS plane (so a group without species 7 reads the
-// niche factor's species 7 from LDS when the block holds it).
-template <int RY, bool SRC, int PD, int SG, int HB = 1>
+// niche factor's species 7 from LDS when the block holds it).  With LIGHT
+// the tile also runs the light scan on the new field (the chunk must span the
+// whole column).
+template <int RY, bool SRC, int PD, int SG, int HB = 1, bool LIGHT = false>
 __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, float* __restrict__ dst,
                                              const uint32_t* __restrict__ occ, int X, int Y, int Z, int XW, int zc,
                                              int tx, int ty, int tz, int grp, const StencilParams& p,
@@ -215,6 +217,9 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
   const int ls0 = half * SG;        // its first LDS species slot
   const bool has7 = s0 <= 7 && 7 < s0 + SG;
   const bool blk7 = bs0 <= 7 && 7 < bs0 + SG * HB;  // species 7 is in the block's LDS
+  // the fused light scan: the wave group holding species 4 carries each own
+  // row's intensity down the column (the chunk is the whole column)
+  const bool lit = LIGHT && s0 <= 4 && 4 < s0 + SG;
 
   const int tid = threadIdx.x & 255;
   const int lx = tid & 63, wy = tid >> 6;
@@ -284,6 +289,9 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
   float h[R][S];    // halo of plane pl (slots of planes z .. z+PD+1)
   uint32_t hm[R];   // this lane's mask word of plane pl
   float q7[R][RY];  // species 7 of plane pl for the niche factor (blocks without it)
+  float I[RY];      // light intensity entering plane z (fused light scan)
+#pragma unroll
+  for (int r = 0; r < RY; ++r) I[r] = p.l_i0;
 
   constexpr int A = group_vec<SG>();
   auto load_own = [&](int k, int zz) {
@@ -383,6 +391,13 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
         if (valid[r]) {
 #pragma unroll
           for (int s = 0; s < S; s += A) st_vec_nt<A>(rd, &out[s], vo[r], qoff(s0 + s, Qb) + zo);
+          // light.rs:41-55 on the new species 4 (slot 4 % SG, as for species 7)
+          if (lit) {
+            p.light[(size_t)z * XY + (size_t)y * X + x] = I[r];
+            if (bc) I[r] *= p.l_e_cell;
+            I[r] *= marl_expf(p.l_nchem * out[4 % SG]);
+            if (I[r] < p.l_floor) I[r] = 0.0f;
+          }
         }
       }
       __syncthreads();
@@ -390,7 +405,7 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
   }
 }
 
-template <int RY, bool SRC, int PD, int SG, int HB>
+template <int RY, bool SRC, int PD, int SG, int HB, bool LIGHT>
 __global__ void __launch_bounds__(256 * HB) k_stencil(const float* __restrict__ src, float* __restrict__ dst,
                                                       const uint32_t* __restrict__ occ, int X, int Y, int Z, int XW,
                                                       int zc, int ntx, int nty, int ntz, StencilParams p) {
@@ -400,8 +415,8 @@ __global__ void __launch_bounds__(256 * HB) k_stencil(const float* __restrict__ 
   // species group fastest: the groups of one tile run side by side on one XCD
   const int grp = b % NG;
   b /= NG;
-  stencil_tile<RY, SRC, PD, SG, HB>(src, dst, occ, X, Y, Z, XW, zc, b % ntx, (b / ntx) % nty, b / (ntx * nty), grp,
-                                    p, sh);
+  stencil_tile<RY, SRC, PD, SG, HB, LIGHT>(src, dst, occ, X, Y, Z, XW, zc, b % ntx, (b / ntx) % nty,
+                                           b / (ntx * nty), grp, p, sh);
 }
 
 // Ensembles: one launch covers every column of the ensemble.  The logical
@@ -729,8 +744,14 @@ static hipError_t launch_stencil_t(marl_ctx* ctx, const StencilParams& p, int zc
   const int nb = ntx * nty * ntz * (MARL_S_EXT / (SG * HB));
   const float* src = ctx->field[ctx->cur];
   float* dst = ctx->field[ctx->cur ^ 1];
-  hipLaunchKernelGGL((k_stencil<RY, SRC, PD, SG, HB>), dim3(nb), dim3(256 * HB), 0, ctx->stream, src, dst, ctx->occ,
-                     X, Y, Z, ctx->xw, zc, ntx, nty, ntz, p);
+  if (p.fuse_light) {
+    if (zc < Z || !p.light) return hipErrorInvalidValue;  // the scan needs whole columns
+    hipLaunchKernelGGL((k_stencil<RY, SRC, PD, SG, HB, true>), dim3(nb), dim3(256 * HB), 0, ctx->stream, src, dst,
+                       ctx->occ, X, Y, Z, ctx->xw, zc, ntx, nty, ntz, p);
+  } else {
+    hipLaunchKernelGGL((k_stencil<RY, SRC, PD, SG, HB, false>), dim3(nb), dim3(256 * HB), 0, ctx->stream, src, dst,
+                       ctx->occ, X, Y, Z, ctx->xw, zc, ntx, nty, ntz, p);
+  }
   return hipGetLastError();
 }
 
@@ -781,8 +802,14 @@ static int launch_double(marl_ctx* ctx, const StencilParams& p, const TileShape&
   return MARL_OK;
 }
 
-int marl_launch_substeps(marl_ctx* ctx, const marl_sim_params* sim, int n, bool sources_first, int tile,
-                         int* launches) {
+// The light scan can ride on a single-substep tile whose z-chunk spans the
+// column: each thread then walks its (x, y) column from z = 0 down.
+static bool tile_fuses_light(int tile, int Z) {
+  return tile >= 0 && tile < kNumTiles && kTiles[tile].tb == 1 && kTiles[tile].zc >= Z;
+}
+
+static int substeps_impl(marl_ctx* ctx, const marl_sim_params* sim, int n, bool sources_first, int tile,
+                         int* launches, bool fuse_light) {
   if (tile < 0 || tile >= kNumTiles) tile = marl_default_tile(ctx->d);
   const TileShape t = kTiles[tile];
   int k = 0, nl = 0;
@@ -796,11 +823,42 @@ int marl_launch_substeps(marl_ctx* ctx, const marl_sim_params* sim, int n, bool 
   // single substeps (an odd remainder of a temporal tile runs on the default tile)
   const int single = t.tb == 2 ? marl_default_tile(ctx->d) : tile;
   for (; k < n; ++k, ++nl) {
-    if (int rc = marl_launch_substep(ctx, marl_stencil_params(sim, src ? 1 : 0), single)) return rc;
+    StencilParams p = marl_stencil_params(sim, src ? 1 : 0);
+    if (fuse_light && k == n - 1) {
+      p.fuse_light = 1;
+      p.l_i0 = sim->surface_intensity;
+      p.l_e_cell = marl_expf(-sim->cell_absorption);
+      p.l_nchem = -sim->chemical_absorption;
+      p.l_floor = sim->light_floor;
+      p.light = ctx->light;
+    }
+    if (int rc = marl_launch_substep(ctx, p, single)) return rc;
     src = false;
   }
   if (launches) *launches = nl;
   return MARL_OK;
+}
+
+int marl_launch_substeps(marl_ctx* ctx, const marl_sim_params* sim, int n, bool sources_first, int tile,
+                         int* launches) {
+  return substeps_impl(ctx, sim, n, sources_first, tile, launches, false);
+}
+
+int marl_launch_field_phase(marl_ctx* ctx, const marl_sim_params* sim, bool sources) {
+  const int n = sim->diffusion_substeps;
+  if (n <= 0) {
+    if (sources)
+      if (int rc = marl_launch_boundary(ctx, sim)) return rc;
+    return marl_launch_light(ctx, sim);
+  }
+  int tile = ctx->tile;
+  if (tile < 0 || tile >= kNumTiles) tile = marl_default_tile(ctx->d);
+  // the last substep runs on `tile` itself unless it is a temporal tile (then
+  // an odd remainder runs on the default tile, an even count has no single)
+  const int last = kTiles[tile].tb == 2 ? (n % 2 ? marl_default_tile(ctx->d) : -1) : tile;
+  const bool fuse = tile_fuses_light(last, ctx->d.z);
+  if (int rc = substeps_impl(ctx, sim, n, sources, ctx->tile, nullptr, fuse)) return rc;
+  return fuse ? MARL_OK : marl_launch_light(ctx, sim);
 }
 
 template <int RY, int PD, int SG, int HB>

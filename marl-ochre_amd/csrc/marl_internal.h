@@ -171,6 +171,11 @@ struct StencilParams {
   // boundary sources fused into the first substep of a tick (field.rs:231-250)
   int32_t apply_sources;
   float src_ox, src_c, src_red, c_max;
+  // light scan fused into the last substep of a tick (light.rs:29-59); only
+  // for tiles whose z-chunk spans the whole column
+  int32_t fuse_light;
+  float l_i0, l_e_cell, l_nchem, l_floor;
+  float* light;
 };
 
 // ---------------------------------------------------------------------------
@@ -283,6 +288,10 @@ int marl_launch_substep(marl_ctx* ctx, const StencilParams& p, int tile);
 // be null) = kernel launches, i.e. field buffer swaps.
 int marl_launch_substeps(marl_ctx* ctx, const marl_sim_params* sim, int n, bool sources_first, int tile,
                          int* launches = nullptr);
+// A tick's field phase: boundary sources, the substeps and the light scan,
+// with the sources fused into the first substep and, where the tile allows,
+// the light scan into the last.
+int marl_launch_field_phase(marl_ctx* ctx, const marl_sim_params* sim, bool sources);
 int marl_launch_boundary(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim);
 // Ensembles: fill the tile fields of `cols` for a tile id (returns total

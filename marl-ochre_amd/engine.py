@@ -123,6 +123,14 @@ class MarlGpu:
         self._chk(self.lib.marl_time_diffuse(self.h, ctypes.byref(p), int(n), ctypes.byref(ms)))
         return ms.value
 
+    def time_field_phase(self, sim, n_ticks: int = 1, sources: bool = False) -> float:
+        """n_ticks field phases (substeps, light fused where the tile allows)."""
+        p = _params(sim)
+        ms = ctypes.c_float()
+        self._chk(self.lib.marl_time_field_phase(self.h, ctypes.byref(p), int(n_ticks), 1 if sources else 0,
+                                                 ctypes.byref(ms)))
+        return ms.value
+
     def time_light(self, sim, n: int) -> float:
         p = _params(sim)
         ms = ctypes.c_float()

@@ -24,9 +24,9 @@ def _sim(pkg, substeps):
                                 light_floor=2e-3)
 
 
-def _inputs(seed):
+def _inputs(seed, dims=DIMS):
     rng = np.random.default_rng(seed)
-    nv = DIMS[0] * DIMS[1] * DIMS[2]
+    nv = dims[0] * dims[1] * dims[2]
     field = rng.random(nv * 12, dtype=F)
     occ = (rng.random(nv) < 0.12).astype(np.uint8)
     return field, occ
@@ -50,6 +50,25 @@ def test_field_phase_matches_oracle(pkg, orc, tile, substeps):
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), np.count_nonzero(got != want)
     assert np.array_equal(got_light.view(np.uint32), want_light.view(np.uint32)), \
         np.count_nonzero(got_light != want_light)
+
+
+@pytest.mark.parametrize("tile", [11, 14])
+def test_field_phase_ragged_grid(pkg, orc, tile):
+    """X = 96 (a half-filled 64-lane tile), Y = 70 (partial row groups),
+    Z = 37: tile 11 (z-chunk 128) fuses the light, tile 14 (z-chunk 16) does
+    not."""
+    dims = (96, 70, 37)
+    field, occ = _inputs(40 + tile, dims)
+    sim = _sim(pkg, 3)
+    want = orc.diffuse_tick(orc.boundary(field, dims, sim), occ, dims, sim)
+    want_light = orc.light(want, occ, dims, sim)
+    with pkg.MarlGpu(dims) as g:
+        g.tile = tile
+        g.upload_field_aos(field)
+        g.upload_occupancy(occ)
+        g.time_field_phase(sim, 1, sources=True)
+        assert np.array_equal(g.download_field_aos().view(np.uint32), want.view(np.uint32))
+        assert np.array_equal(g.download_light().view(np.uint32), want_light.view(np.uint32))
 
 
 def test_field_phase_without_substeps(pkg, orc):

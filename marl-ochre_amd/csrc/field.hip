@@ -39,15 +39,20 @@ const TileShape kTiles[] = {
     {2, 16, 1, 6, 1, 4},
     // short z-chunks: enough blocks for small grids
     {1, 16, 1, 4, 1, 12}, {1, 8, 1, 4, 1, 12},
+    // small grids (field resident in the last-level cache): one quad per
+    // 4-wave block, so many small blocks per CU hide the per-plane latency;
+    // the quads without species 7 read it for the niche factor
+    {1, 8, 1, 4, 1, 4}, {1, 4, 1, 4, 1, 4}, {1, 16, 1, 4, 1, 4}, {1, 8, 2, 4, 1, 4}, {1, 4, 1, 4, 1, 12},
     // temporal blocking (K1b): two substeps per launch
     {1, 16, 1, 12, 2, 8}, {1, 16, 1, 6, 2, 8},
 };
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 // Default tile: the quad-group shapes from the longest z-chunk down, taking
-// the first that still gives every CU a block (the autotuner may pick another).
+// the first that still gives every CU a block, down to one-quad blocks with
+// 4-plane chunks for tiny grids (the autotuner may pick another).
 int marl_default_tile(const marl_dims& d) {
-  static const int order[] = {11, 10, 8, 7, 5, 14, 15};
+  static const int order[] = {11, 10, 8, 7, 5, 14, 15, 17};
   static const int cus = [] {
     int n = 256, dev = 0;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
@@ -57,9 +62,9 @@ int marl_default_tile(const marl_dims& d) {
     const TileShape& s = kTiles[t];
     const int zc = s.zc < d.z ? s.zc : d.z;
     const long blocks = (long)((d.x + 63) / 64) * ((d.y + 4 * s.ry - 1) / (4 * s.ry)) * ((d.z + zc - 1) / zc);
-    if (blocks >= cus) return t;
+    if (blocks * (MARL_S_EXT / (s.sg * (s.nw / 4))) >= cus) return t;
   }
-  return 15;
+  return 17;
 }
 
 // The kernel instantiation of a single-substep tile shape: calls f(TileT<...>)
@@ -81,6 +86,8 @@ static hipError_t dispatch_tile(const TileShape& t, F&& f) {
   MARL_TILE(1, 2, 4, 3)
   MARL_TILE(2, 1, 4, 3)
   MARL_TILE(2, 1, 6, 1)
+  MARL_TILE(1, 1, 4, 1)
+  MARL_TILE(1, 2, 4, 1)
 #undef MARL_TILE
   return hipErrorInvalidValue;
 }

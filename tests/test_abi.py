@@ -105,7 +105,7 @@ def test_tile_table(pkg):
     sg, nw = ctypes.c_int32(), ctypes.c_int32()
     for t in range(n):
         assert lib.marl_stencil_tile_shape(t, ctypes.byref(ry), ctypes.byref(zc), ctypes.byref(pd)) == 0
-        assert ry.value in (1, 2, 4) and zc.value in (8, 16, 32, 64, 128) and pd.value in (1, 2)
+        assert ry.value in (1, 2, 4) and zc.value in (4, 8, 16, 32, 64, 128) and pd.value in (1, 2)
         assert lib.marl_stencil_tile_split(t, ctypes.byref(sg), ctypes.byref(nw)) == 0
         assert nw.value in (4, 8, 12) and 12 % sg.value == 0
         if subs[t] == 1:  # a block's wave groups split the 12 species evenly

@@ -89,10 +89,12 @@ __global__ void k_scatter_cells(const marl_cell_state* __restrict__ cells, int n
 
 // K4: empty-neighbour mask (spatial.rs:18-43) and the conflict predecessors.
 // Conflicting partner at L1 distance 2 with a lower index that shares at least
-// one free face neighbour.
+// one free face neighbour.  Predecessor q of cell j is preds[q * pstride + j]
+// (pstride = capacity): the relaxation sweeps then read predecessor lists row
+// by row, coalesced across the cells of a wave.
 __device__ __forceinline__ void cell_prep_cell(int j, const marl_cell_state* __restrict__ cells,
                                                const int32_t* __restrict__ cell_at, const uint32_t* __restrict__ occ,
-                                               uint8_t* __restrict__ nbmask, int32_t* __restrict__ preds,
+                                               uint8_t* __restrict__ nbmask, int32_t* __restrict__ preds, size_t pstride,
                                                uint8_t* __restrict__ npred, int32_t* __restrict__ level,
                                                const CellParams& p) {
   const uint16_t* ps = cells[j].pos;
@@ -110,7 +112,7 @@ __device__ __forceinline__ void cell_prep_cell(int j, const marl_cell_state* __r
   }
   nbmask[j] = (uint8_t)m;
   int np = 0;
-  int32_t* pj = preds + (size_t)j * kMaxPreds;
+  int32_t* pj = preds + j;
   // +-2 along one axis: shared voxel is the midpoint (a face neighbour of j)
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
@@ -118,7 +120,7 @@ __device__ __forceinline__ void cell_prep_cell(int j, const marl_cell_state* __r
     const int ox = x + 2 * kFace[k][0], oy = y + 2 * kFace[k][1], oz = z + 2 * kFace[k][2];
     if (!inb(ox, oy, oz, p) || !occupied(ox, oy, oz)) continue;
     const int32_t i = cell_at[vox(ox, oy, oz, p)];
-    if (i >= 0 && i < j) pj[np++] = i;
+    if (i >= 0 && i < j) pj[(size_t)np++ * pstride] = i;
   }
   // one step along two different axes: shared voxels are the two corners
 #pragma unroll
@@ -131,7 +133,7 @@ __device__ __forceinline__ void cell_prep_cell(int j, const marl_cell_state* __r
                 oz = z + kFace[a][2] + kFace[b][2];
       if (!inb(ox, oy, oz, p) || !occupied(ox, oy, oz)) continue;
       const int32_t i = cell_at[vox(ox, oy, oz, p)];
-      if (i >= 0 && i < j) pj[np++] = i;
+      if (i >= 0 && i < j) pj[(size_t)np++ * pstride] = i;
     }
   }
   npred[j] = (uint8_t)np;
@@ -140,24 +142,24 @@ __device__ __forceinline__ void cell_prep_cell(int j, const marl_cell_state* __r
 
 __global__ void k_cell_prep(const marl_cell_state* __restrict__ cells, int n, const int32_t* __restrict__ cell_at,
                             const uint32_t* __restrict__ occ, uint8_t* __restrict__ nbmask,
-                            int32_t* __restrict__ preds, uint8_t* __restrict__ npred, int32_t* __restrict__ level,
-                            CellParams p) {
+                            int32_t* __restrict__ preds, size_t ps, uint8_t* __restrict__ npred,
+                            int32_t* __restrict__ level, CellParams p) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  cell_prep_cell(j, cells, cell_at, occ, nbmask, preds, npred, level, p);
+  cell_prep_cell(j, cells, cell_at, occ, nbmask, preds, ps, npred, level, p);
 }
 
 // Longest-path relaxation on the conflict DAG; monotone, converges to the
 // exact level in (depth + 1) sweeps or fewer.
-__global__ void k_level_relax(int n, const int32_t* __restrict__ preds, const uint8_t* __restrict__ npred,
+__global__ void k_level_relax(int n, const int32_t* __restrict__ preds, size_t ps, const uint8_t* __restrict__ npred,
                               int32_t* level, int32_t* changed) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const int np = npred[j];
   if (np == 0) return;
-  const int32_t* pj = preds + (size_t)j * kMaxPreds;
+  const int32_t* pj = preds + j;
   int L = 0;
-  for (int k = 0; k < np; ++k) L = max(L, level[pj[k]] + 1);
+  for (int k = 0; k < np; ++k) L = max(L, level[pj[(size_t)k * ps]] + 1);
   if (L > level[j]) {
     level[j] = L;
     *changed = 1;
@@ -725,15 +727,15 @@ __global__ void k_g_scatter_cells(const marl_cell_state* __restrict__ cells, con
 
 __global__ void k_g_cell_prep(const marl_cell_state* __restrict__ cells, const TickDev* ts,
                               const int32_t* __restrict__ cell_at, const uint32_t* __restrict__ occ,
-                              uint8_t* __restrict__ nbmask, int32_t* __restrict__ preds, uint8_t* __restrict__ npred,
-                              int32_t* __restrict__ level, CellParams p) {
+                              uint8_t* __restrict__ nbmask, int32_t* __restrict__ preds, size_t ps,
+                              uint8_t* __restrict__ npred, int32_t* __restrict__ level, CellParams p) {
   if (ts->err) return;
   const int n = ts->n;
-  GS_LOOP(j, n) cell_prep_cell((int)j, cells, cell_at, occ, nbmask, preds, npred, level, p);
+  GS_LOOP(j, n) cell_prep_cell((int)j, cells, cell_at, occ, nbmask, preds, ps, npred, level, p);
 }
 
 // sweep k runs only if sweep k-1 changed a level
-__global__ void k_g_level_relax(int k, TickDev* ts, const int32_t* __restrict__ preds,
+__global__ void k_g_level_relax(int k, TickDev* ts, const int32_t* __restrict__ preds, size_t ps,
                                 const uint8_t* __restrict__ npred, int32_t* level) {
   if (ts->err || (k > 0 && ts->chg[k - 1] == 0)) return;
   const int n = ts->n;
@@ -741,9 +743,9 @@ __global__ void k_g_level_relax(int k, TickDev* ts, const int32_t* __restrict__ 
   GS_LOOP(j, n) {
     const int np = npred[j];
     if (np == 0) continue;
-    const int32_t* pj = preds + (size_t)j * kMaxPreds;
+    const int32_t* pj = preds + j;
     int L = 0;
-    for (int q = 0; q < np; ++q) L = max(L, level[pj[q]] + 1);
+    for (int q = 0; q < np; ++q) L = max(L, level[pj[(size_t)q * ps]] + 1);
     if (L > level[j]) {
       level[j] = L;
       any = true;
@@ -1067,7 +1069,7 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
   marl_cell_state* cells = ctx->cells[ctx->cbuf];
   const marl_ruleset* rules = ctx->rules[ctx->cbuf];
   hipLaunchKernelGGL(k_cell_prep, dim3(grid1(n)), dim3(256), 0, ctx->stream, cells, n, ctx->cell_at, ctx->occ,
-                     ctx->nbmask, ctx->preds, ctx->npred, ctx->level, p);
+                     ctx->nbmask, ctx->preds, (size_t)ctx->cap, ctx->npred, ctx->level, p);
   MARL_HIP(ctx, hipGetLastError());
   // Relaxation sweeps, each with its own change flag, in batches with one host
   // check each.  The first batch is as long as the previous tick needed (the
@@ -1081,8 +1083,8 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
     if (done > kMaxLevels + 2) return marl_fail(ctx, MARL_ERR_TRANSFER, "conflict schedule did not converge");
     if (int rc = fill_i32(ctx, changed, 0, batch)) return rc;
     for (int k = 0; k < batch; ++k)
-      hipLaunchKernelGGL(k_level_relax, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->preds, ctx->npred,
-                         ctx->level, changed + k);
+      hipLaunchKernelGGL(k_level_relax, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->preds,
+                         (size_t)ctx->cap, ctx->npred, ctx->level, changed + k);
     MARL_HIP(ctx, hipGetLastError());
     if (int rc = d2h_sync(ctx, h, changed, sizeof(int32_t) * batch)) return rc;
     int quiet = -1;
@@ -1287,9 +1289,10 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
   const unsigned gc = gs_grid(cap), gk = gs_grid(cap, kKinBlock, 4096), gf = gs_grid(cap + 1);
   // schedule
   hipLaunchKernelGGL(k_g_cell_prep, dim3(gc), dim3(256), 0, st, cells, ts, ctx->cell_at, ctx->occ, ctx->nbmask,
-                     ctx->preds, ctx->npred, ctx->level, p);
+                     ctx->preds, (size_t)cap, ctx->npred, ctx->level, p);
   for (int k = 0; k < ctx->graph_sweeps; ++k)
-    hipLaunchKernelGGL(k_g_level_relax, dim3(gc), dim3(256), 0, st, k, ts, ctx->preds, ctx->npred, ctx->level);
+    hipLaunchKernelGGL(k_g_level_relax, dim3(gc), dim3(256), 0, st, k, ts, ctx->preds, (size_t)cap, ctx->npred,
+                       ctx->level);
   hipLaunchKernelGGL(k_g_relax_check, dim3(1), dim3(1), 0, st, ts, ctx->graph_sweeps);
   int32_t* hist = ctx->level_count;
   int32_t* cursor = ctx->level_count + (kMaxLevels + 1);

@@ -232,7 +232,7 @@ struct marl_ctx {
   // per-cell scratch (capacity cap)
   uint8_t* nbmask = nullptr;   // 6-bit empty-neighbour mask
   int32_t* level = nullptr;
-  int32_t* preds = nullptr;    // conflict predecessors, 18 slots per cell
+  int32_t* preds = nullptr;    // conflict predecessors: 18 rows of cap entries (row q = q-th predecessor)
   uint8_t* npred = nullptr;
   int32_t* order = nullptr;    // cells sorted by level
   uint8_t* events = nullptr;

@@ -60,6 +60,21 @@ CellParams cell_params(const marl_ctx* ctx, const marl_sim_params* s, uint64_t t
 // FACE_OFFSETS (spatial.rs:8-15): +x, -x, +y, -y, +z, -z
 __constant__ int8_t kFace[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
 
+// Conflict candidates at L1 distance 2 (the cells that can share a free face
+// neighbour with a cell): +-2 along each axis in face order, then one step
+// along each pair of faces (a, b), a < b, on different axes.  kCandFaces: the
+// faces of this cell that lie between it and the candidate (the midpoint, or
+// the two corners).
+constexpr int kCand = 18;
+__constant__ int8_t kCandOff[kCand][3] = {
+    {2, 0, 0},  {-2, 0, 0}, {0, 2, 0},  {0, -2, 0}, {0, 0, 2},  {0, 0, -2},                          // axes
+    {1, 1, 0},  {1, -1, 0}, {1, 0, 1},  {1, 0, -1}, {-1, 1, 0}, {-1, -1, 0}, {-1, 0, 1}, {-1, 0, -1},  // x with y, z
+    {0, 1, 1},  {0, 1, -1}, {0, -1, 1}, {0, -1, -1}};                                                  // y with z
+__constant__ uint8_t kCandFaces[kCand] = {
+    1, 2, 4, 8, 16, 32,
+    1 | 4, 1 | 8, 1 | 16, 1 | 32, 2 | 4, 2 | 8, 2 | 16, 2 | 32,
+    4 | 16, 4 | 32, 8 | 16, 8 | 32};
+
 __device__ inline bool inb(int x, int y, int z, const CellParams& p) {
   return x >= 0 && x < p.X && y >= 0 && y < p.Y && z >= 0 && z < p.Z;
 }
@@ -99,43 +114,49 @@ __device__ __forceinline__ void cell_prep_cell(int j, const marl_cell_state* __r
                                                const CellParams& p) {
   const uint16_t* ps = cells[j].pos;
   const int x = ps[0], y = ps[1], z = ps[2];
-  // emptiness from the 1-bit mask (L2-resident); cell_at only for occupied voxels
-  auto occupied = [&](int ox, int oy, int oz) -> bool {
-    const int64_t w = vox(ox, oy, oz, p);
-    return (occ[w >> 5] >> (w & 31)) & 1u;
-  };
-  uint32_t m = 0;
+  // Emptiness from the 1-bit mask (L2-resident), cell_at only for occupied
+  // candidates.  Every mask word (6 faces, 18 candidates) is requested before
+  // any is used, and the candidates' cell_at loads likewise, so a cell waits
+  // for two rounds of loads, not one per candidate; out-of-grid positions read
+  // word 0 and are masked off.
+  uint32_t fw[6], fb[6];
+  bool fin[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const int nx = x + kFace[k][0], ny = y + kFace[k][1], nz = z + kFace[k][2];
-    if (inb(nx, ny, nz, p) && !occupied(nx, ny, nz)) m |= 1u << k;
+    fin[k] = inb(nx, ny, nz, p);
+    const int64_t w = fin[k] ? vox(nx, ny, nz, p) : 0;
+    fw[k] = occ[w >> 5];
+    fb[k] = (uint32_t)(w & 31);
   }
+  int64_t cv[kCand];
+  uint32_t cw[kCand];
+  bool cin[kCand];
+#pragma unroll
+  for (int c = 0; c < kCand; ++c) {
+    const int ox = x + kCandOff[c][0], oy = y + kCandOff[c][1], oz = z + kCandOff[c][2];
+    cin[c] = inb(ox, oy, oz, p);
+    cv[c] = cin[c] ? vox(ox, oy, oz, p) : 0;
+    cw[c] = occ[cv[c] >> 5];
+  }
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    if (fin[k] && !((fw[k] >> fb[k]) & 1u)) m |= 1u << k;
   nbmask[j] = (uint8_t)m;
+  // a candidate conflicts when it is occupied and one of the faces it shares
+  // with this cell (kCandFaces) is free
+  int32_t ci[kCand];
+#pragma unroll
+  for (int c = 0; c < kCand; ++c) {
+    const bool want = cin[c] && ((cw[c] >> (cv[c] & 31)) & 1u) && (m & kCandFaces[c]);
+    ci[c] = want ? cell_at[cv[c]] : -1;
+  }
   int np = 0;
   int32_t* pj = preds + j;
-  // +-2 along one axis: shared voxel is the midpoint (a face neighbour of j)
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    if (!((m >> k) & 1u)) continue;
-    const int ox = x + 2 * kFace[k][0], oy = y + 2 * kFace[k][1], oz = z + 2 * kFace[k][2];
-    if (!inb(ox, oy, oz, p) || !occupied(ox, oy, oz)) continue;
-    const int32_t i = cell_at[vox(ox, oy, oz, p)];
-    if (i >= 0 && i < j) pj[(size_t)np++ * pstride] = i;
-  }
-  // one step along two different axes: shared voxels are the two corners
-#pragma unroll
-  for (int a = 0; a < 6; ++a) {
-#pragma unroll
-    for (int b = a + 1; b < 6; ++b) {
-      if ((a >> 1) == (b >> 1)) continue;  // same axis
-      if (!(((m >> a) | (m >> b)) & 1u)) continue;  // neither corner free
-      const int ox = x + kFace[a][0] + kFace[b][0], oy = y + kFace[a][1] + kFace[b][1],
-                oz = z + kFace[a][2] + kFace[b][2];
-      if (!inb(ox, oy, oz, p) || !occupied(ox, oy, oz)) continue;
-      const int32_t i = cell_at[vox(ox, oy, oz, p)];
-      if (i >= 0 && i < j) pj[(size_t)np++ * pstride] = i;
-    }
-  }
+  for (int c = 0; c < kCand; ++c)
+    if (ci[c] >= 0 && ci[c] < j) pj[(size_t)np++ * pstride] = ci[c];
   npred[j] = (uint8_t)np;
   level[j] = 0;
 }

@@ -60,6 +60,14 @@ def cpu_baseline(field, occ, dims, sim, substeps=40):
     }
 
 
+def _records_equal(a, b):
+    """Structured arrays equal bit for bit in every named field (padding and
+    reserved fields, whose names start with '_', are not compared)."""
+    if a.dtype.names is None:
+        return a.shape == b.shape and a.tobytes() == b.tobytes()
+    return all(_records_equal(a[k], b[k]) for k in a.dtype.names if not k.startswith("_"))
+
+
 KINETICS_BYTES_PER_CELL = 1350.0  # state r/w + ruleset + neighbour gather/RMW at 12% occupancy
 
 
@@ -101,6 +109,7 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
             graph_runs.append((wall, st))
     for (_, a), (_, b) in zip(runs, graph_runs):
         assert all(a[k] == b[k] for k in ("divisions", "deaths", "births_added", "n_cells")), (a, b)
+    first_tick = (g.download_field_aos(), *g.download_cells()) if with_cpu else None
     # run outputs on the 1M-cell state: GPU reductions / packing incl. the readback
     g.upload_field_aos(field)
     g.upload_cells(cells, rules)
@@ -157,6 +166,19 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
         from oracle.binding import Oracle
 
         orc = Oracle()
+        threads = len(os.sched_getaffinity(0))
+        t0 = time.perf_counter()
+        f_o, _, c_o, r_o, _, _ = orc.tick(dims, sim, 0, field, cells, rules, nthreads=threads)
+        dt = time.perf_counter() - t0
+        f_g, c_g, r_g = first_tick
+        same = bool(np.array_equal(f_g.view(np.uint32), f_o.view(np.uint32)) and len(c_g) == len(c_o)
+                    and _records_equal(c_g, c_o) and _records_equal(r_g, r_o))
+        res["cpu_baseline_full_tick"] = {
+            "value": round(dt * 1e3, 1), "unit": "ms/tick", "cores": threads, "kind": "port",
+            "sample": "the first full tick of the 1M-cell workload, f32 C oracle (diffusion OpenMP over z-layers; "
+                      "light, cell pass and fate on one thread, as the reference)",
+            "gpu_tick_bit_identical": same}
+        res["speedup_full_tick_vs_cpu"] = round(dt * 1e3 / tick_ms, 1)
         nv = dims[0] * dims[1] * dims[2]
         sample = 200_000
         light = np.ones(nv, np.float32)
@@ -542,7 +564,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in (host buffers) leg")
     ap.add_argument("--tile", type=int, default=-1, help="use this stencil tile id instead of autotuning")
-    ap.add_argument("--kinetics-ticks", type=int, default=10,
+    ap.add_argument("--kinetics-ticks", type=int, default=100,
                     help="full ticks of the 1M-cell kinetics workload (0 = skip)")
     ap.add_argument("--column-ticks", type=int, default=200,
                     help="ticks of the default 128x128x64 column run (0 = skip)")

@@ -1437,6 +1437,71 @@ int marl_graph_fetch(marl_ctx* ctx, hipStream_t st) {
   return MARL_OK;
 }
 
+namespace {
+
+// Batched staging: (counters, live count, tick) of up to 64 columns by value.
+struct StageBatch {
+  TickDev* ts[64];
+  int32_t n[64];
+  uint64_t tick;
+  int32_t count;
+};
+
+__global__ void k_stage_batch(StageBatch b) {
+  const int i = threadIdx.x;
+  if (i < b.count) {
+    b.ts[i]->n = b.n[i];
+    b.ts[i]->tick = b.tick;
+  }
+}
+
+// One block per column: its counters into slot c of a contiguous array.
+__global__ void k_gather_tickdevs(TickDev* const* __restrict__ src, TickDev* __restrict__ out) {
+  constexpr int kWords = sizeof(TickDev) / 4;
+  const int32_t* s = reinterpret_cast<const int32_t*>(src[blockIdx.x]);
+  int32_t* d = reinterpret_cast<int32_t*>(out + blockIdx.x);
+  for (int k = threadIdx.x; k < kWords; k += blockDim.x) d[k] = s[k];
+}
+static_assert(sizeof(TickDev) % 4 == 0, "TickDev words");
+
+}  // namespace
+
+size_t marl_tickdev_bytes() { return sizeof(TickDev); }
+
+int marl_graph_stage_many(marl_ctx* const* cols, int n, hipStream_t st, int64_t tick) {
+  for (int c0 = 0; c0 < n; c0 += 64) {
+    StageBatch b{};
+    b.count = std::min(64, n - c0);
+    b.tick = (uint64_t)tick;
+    for (int i = 0; i < b.count; ++i) {
+      marl_ctx* ctx = cols[c0 + i];
+      b.ts[i] = static_cast<TickDev*>(ctx->tdev);
+      b.n[i] = (int32_t)ctx->n_cells;
+      TickDev* hts = host_tickdev(ctx);  // kept in step with the device for the host-finish path
+      hts->n = b.n[i];
+      hts->tick = b.tick;
+    }
+    hipLaunchKernelGGL(k_stage_batch, dim3(1), dim3(64), 0, st, b);
+    MARL_HIP(cols[c0], hipGetLastError());
+  }
+  return MARL_OK;
+}
+
+int marl_graph_fetch_many(marl_ctx* const* cols, int n, hipStream_t st, void* const* dev_srcs, void* dev_out,
+                          void* host_out) {
+  if (n <= 0) return MARL_OK;
+  hipLaunchKernelGGL(k_gather_tickdevs, dim3(n), dim3(64), 0, st, reinterpret_cast<TickDev* const*>(dev_srcs),
+                     static_cast<TickDev*>(dev_out));
+  MARL_HIP(cols[0], hipGetLastError());
+  MARL_HIP(cols[0], hipMemcpyAsync(host_out, dev_out, sizeof(TickDev) * (size_t)n, hipMemcpyDeviceToHost, st));
+  return MARL_OK;
+}
+
+void marl_graph_fetch_finish_many(marl_ctx* const* cols, int n, const void* host_out) {
+  const TickDev* h = static_cast<const TickDev*>(host_out);
+  for (int c = 0; c < n; ++c) memcpy(host_tickdev(cols[c]), h + c, sizeof(TickDev));
+}
+
 // After the fetch completed: the tick's outcome, finishing the cell phase on
 // the host-driven path if the device skipped it.
 // Automatic schedule budget of the captured tick: follow the sweeps and levels

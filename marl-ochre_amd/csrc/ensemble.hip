@@ -32,6 +32,12 @@ struct marl_ensemble {
   hipGraphExec_t gexec = nullptr;
   std::vector<uint8_t> key;
   std::string err;
+  // batched counter transfers: the columns' device counter pointers, the
+  // gathered counters on the device and their pinned host copy
+  std::vector<void*> tdevs;
+  void** d_tdevs = nullptr;
+  void* d_counters = nullptr;
+  void* h_counters = nullptr;
 };
 
 namespace {
@@ -89,6 +95,26 @@ int build_table(marl_ensemble* e) {
   host_table(e, e->hcols, &e->total_tiles, &e->total_light, &e->grid);
   if (!e->dcols) ENS_HIP(e, hipMalloc(&e->dcols, sizeof(EnsColumn) * n));
   ENS_HIP(e, hipMemcpyAsync(e->dcols, e->hcols.data(), sizeof(EnsColumn) * n, hipMemcpyHostToDevice, e->stream));
+  ENS_HIP(e, hipStreamSynchronize(e->stream));
+  return MARL_OK;
+}
+
+// The buffers of the batched counter transfers, rebuilt when a column's
+// device counters moved (they are allocated by the column's first prepare).
+int ensure_counter_buffers(marl_ensemble* e) {
+  const int n = (int)e->cols.size();
+  bool same = (int)e->tdevs.size() == n;
+  for (int c = 0; same && c < n; ++c) same = e->tdevs[c] == e->cols[c]->tdev;
+  if (same && e->d_tdevs) return MARL_OK;
+  e->tdevs.resize(n);
+  for (int c = 0; c < n; ++c) e->tdevs[c] = e->cols[c]->tdev;
+  const size_t bytes = marl_tickdev_bytes() * (size_t)n;
+  if (!e->d_tdevs) {
+    ENS_HIP(e, hipMalloc((void**)&e->d_tdevs, sizeof(void*) * n));
+    ENS_HIP(e, hipMalloc(&e->d_counters, bytes));
+    ENS_HIP(e, hipHostMalloc(&e->h_counters, bytes, hipHostMallocDefault));
+  }
+  ENS_HIP(e, hipMemcpyAsync(e->d_tdevs, e->tdevs.data(), sizeof(void*) * n, hipMemcpyHostToDevice, e->stream));
   ENS_HIP(e, hipStreamSynchronize(e->stream));
   return MARL_OK;
 }
@@ -191,6 +217,9 @@ void marl_ensemble_destroy(marl_ensemble* e) {
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   drop(e);
   if (e->dcols) (void)hipFree(e->dcols);
+  if (e->d_tdevs) (void)hipFree(e->d_tdevs);
+  if (e->d_counters) (void)hipFree(e->d_counters);
+  if (e->h_counters) (void)hipHostFree(e->h_counters);
   for (auto f : e->fork)
     if (f) (void)hipEventDestroy(f);
   for (auto j : e->join)
@@ -237,14 +266,17 @@ int marl_ensemble_tick(marl_ensemble* e, const marl_sim_params* sim, const uint6
       if (int rc = marl_graph_prepare(e->cols[c], &sims[c])) return col_fail(e, c, rc);
     if (e->key != ens_key(e, sims))
       if (int rc = capture(e, sims)) return rc;
-    for (int c = 0; c < n; ++c)
-      if (int rc = marl_graph_stage(e->cols[c], e->stream, tick0 + t)) return col_fail(e, c, rc);
+    if (int rc = ensure_counter_buffers(e)) return rc;
+    // every column's live count and tick in one launch, every column's
+    // counters back in one gather and one copy
+    if (int rc = marl_graph_stage_many(e->cols.data(), n, e->stream, tick0 + t)) return col_fail(e, 0, rc);
     ENS_HIP(e, hipEventRecord(e->ev0, e->stream));
     ENS_HIP(e, hipGraphLaunch(e->gexec, e->stream));
     ENS_HIP(e, hipEventRecord(e->ev1, e->stream));
-    for (int c = 0; c < n; ++c)
-      if (int rc = marl_graph_fetch(e->cols[c], e->stream)) return col_fail(e, c, rc);
+    if (int rc = marl_graph_fetch_many(e->cols.data(), n, e->stream, e->d_tdevs, e->d_counters, e->h_counters))
+      return col_fail(e, 0, rc);
     ENS_HIP(e, hipStreamSynchronize(e->stream));
+    marl_graph_fetch_finish_many(e->cols.data(), n, e->h_counters);
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, e->ev0, e->ev1);
     ms_total += ms;

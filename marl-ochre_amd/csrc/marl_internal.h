@@ -312,6 +312,16 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim);
 std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* sim);
 int marl_graph_stage(marl_ctx* ctx, hipStream_t st, int64_t tick);
 int marl_graph_fetch(marl_ctx* ctx, hipStream_t st);
+// The same for many contexts with one launch each way (ensembles): the live
+// counts and tick go to every column's device counters in one kernel; the
+// counters come back gathered into `dev_out` (device, n x marl_tickdev_bytes())
+// and copied to `host_out` (pinned) in one transfer; after the stream has
+// synchronised, marl_graph_fetch_finish_many hands each column its copy.
+size_t marl_tickdev_bytes();
+int marl_graph_stage_many(marl_ctx* const* cols, int n, hipStream_t st, int64_t tick);
+int marl_graph_fetch_many(marl_ctx* const* cols, int n, hipStream_t st, void* const* dev_srcs, void* dev_out,
+                          void* host_out);
+void marl_graph_fetch_finish_many(marl_ctx* const* cols, int n, const void* host_out);
 int marl_graph_post(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick, marl_tick_stats* st,
                     int32_t* host_finished);
 int marl_launch_occupancy_from_cells(marl_ctx* ctx);

@@ -218,8 +218,9 @@ int marl_set_cell_layout(marl_ctx* ctx, int32_t layout);
 /* Budget of relaxation sweeps and level launches in the captured tick (1..64
  * each; a change captures the graph again).  (0, 0) selects the default
  * automatic budget: it starts at 64 each and then follows what the schedule
- * needed: within 2 of the need it grows to the need + 12, more than 20 above
- * it shrinks to the need + 8. */
+ * needed, with a slack of need / 2 + 2 (3 to 8): within 1 of the need it
+ * grows to the need + 2 slacks, more than 3 slacks above it shrinks to the
+ * need + 1 slack. */
 int marl_set_graph_limits(marl_ctx* ctx, int32_t sweeps, int32_t levels);
 int marl_cell_pass(marl_ctx* ctx, const marl_sim_params* sim, uint8_t* events, size_t n_events, int32_t* levels_out);
 /* FP64 per-species totals: out[0..11] empty voxels, out[12..23] occupied

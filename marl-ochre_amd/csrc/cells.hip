@@ -1506,16 +1506,19 @@ void marl_graph_fetch_finish_many(marl_ctx* const* cols, int n, const void* host
 // the host-driven path if the device skipped it.
 // Automatic schedule budget of the captured tick: follow the sweeps and levels
 // the schedule needed, with slack, so that the graph launches few idle sweep
-// and level kernels.  A budget within 2 of the need grows at once to the need
-// plus 12 (a growing population deepens its schedule by a level or two per
-// tick, and a tick over budget is finished by the host); a budget shrinks, to
-// the need plus 8, only when it is more than 20 above it, so that the graph is
-// rarely captured again.
+// and level kernels (each costs a launch even when it exits at once).  The
+// slack is need / 2 + 2, between 3 and 8.  A budget within 1 of the need
+// grows at once to the need plus twice the slack (a growing population
+// deepens its schedule by a level or two per tick, and a tick over budget is
+// finished by the host); a budget more than three slacks above the need
+// shrinks to the need plus one slack, so that the graph is rarely captured
+// again.
 static void adapt_graph_limits(marl_ctx* ctx, int need_sweeps, int need_levels) {
   if (!ctx->graph_auto) return;
   auto fit = [](int cur, int need, int cap) {
-    if (need + 2 >= cur) return std::min(cap, need + 12);
-    return cur > need + 20 ? std::min(cap, need + 8) : cur;
+    const int slack = std::min(8, std::max(3, need / 2 + 2));
+    if (need + 1 >= cur) return std::min(cap, need + 2 * slack);
+    return cur > need + 3 * slack ? std::min(cap, need + slack) : cur;
   };
   ctx->graph_sweeps = fit(ctx->graph_sweeps, std::max(need_sweeps, 1), kGraphSweeps);
   ctx->graph_levels = fit(ctx->graph_levels, std::max(need_levels, 1), kGraphLevels);

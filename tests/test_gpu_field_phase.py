@@ -86,14 +86,15 @@ def test_field_phase_without_substeps(pkg, orc):
         assert np.array_equal(g.download_light(), want_light)
 
 
-@pytest.mark.parametrize("tile", [11, 13])
-def test_ticks_with_fused_light(pkg, orc, tile):
+@pytest.mark.parametrize("tile,substeps", [(11, 4), (13, 4), (11, 3)])
+def test_ticks_with_fused_light(pkg, orc, tile, substeps):
     """Whole ticks, host-driven and captured, against the oracle: the fused
-    light feeds the cell pass of the same tick."""
+    light feeds the cell pass of the same tick (an odd substep count
+    alternates the captured tick between two graphs)."""
     A = pkg._abi
     cells, rules = population(A, DIMS, 5000, 21)
     field, _ = _inputs(22)
-    sim = _sim(pkg, 4).replace(rng_seed=31)
+    sim = _sim(pkg, substeps).replace(rng_seed=31)
     with pkg.MarlGpu(DIMS) as ge, pkg.MarlGpu(DIMS) as gg:
         for g in (ge, gg):
             g.tile = tile

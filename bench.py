@@ -298,7 +298,7 @@ def dropin_bench(pkg, reps=10):
     return res
 
 
-def drift_report(pkg, n_ticks, nthreads):
+def drift_report(pkg, n_ticks, nthreads, tile=-1):
     """Long-run accuracy on the default 128x128x64 column (~4,096 cells).
 
     (1) Full ticks on the GPU and in the FP32 oracle in lockstep for n_ticks:
@@ -314,9 +314,12 @@ def drift_report(pkg, n_ticks, nthreads):
     field, cells, rules, sim = scenarios.winogradsky_column(dims, starter_state=True)
     checks = sorted({c for c in (1, 10, 100, 1000, n_ticks) if c <= n_ticks})
     out = {"workload": "default column 128x128x64 with the starters' own states, %d cells at start, 10 substeps/tick, "
-                       "GPU ticks through the captured graph" % len(cells), "ticks": n_ticks}
+                       "GPU ticks through the captured graph" % len(cells), "ticks": n_ticks,
+           "stencil_tile": tile if tile >= 0 else "default"}
     # (1) full ticks, FP32 GPU vs FP32 oracle
     g = pkg.MarlGpu(dims)
+    if tile >= 0:
+        g.tile = tile
     g.upload_field_aos(field)
     g.upload_cells(cells, rules)
     f_o, c_o, r_o = field.copy(), cells.copy(), rules.copy()
@@ -353,6 +356,8 @@ def drift_report(pkg, n_ticks, nthreads):
     p = cells["pos"].astype(np.int64)
     occ[(p[:, 2] * dims[1] + p[:, 1]) * dims[0] + p[:, 0]] = 1
     g = pkg.MarlGpu(dims)
+    if tile >= 0:
+        g.tile = tile
     g.upload_field_aos(field)
     g.upload_occupancy(occ)
     f64 = field.astype(np.float64)
@@ -705,7 +710,7 @@ def main():
     if not args.no_dropin:
         out["dropin"] = dropin_bench(pkg)
     if args.drift_ticks > 0:
-        out["drift"] = drift_report(pkg, args.drift_ticks, min(os.cpu_count() or 1, 16))
+        out["drift"] = drift_report(pkg, args.drift_ticks, min(os.cpu_count() or 1, 16), args.tile)
     if args.surrogate_steps > 0:
         out["surrogate"] = surrogate_bench(pkg, args.surrogate_steps, args.harvest_ticks)
     if not args.no_cpu:

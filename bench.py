@@ -583,8 +583,8 @@ def main():
                     help="plain ticks whose pairs train the surrogate")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the grid would shard)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path (ensembles would shard "
+              "by column, a single grid by y-slabs with one halo-plane exchange per substep)", file=sys.stderr)
         sys.exit(2)
 
     from __graft_entry__ import load_package

@@ -68,13 +68,13 @@ def test_ensemble_bit_identical_to_single_columns(pkg):
     assert events > 50
 
 
-@pytest.mark.parametrize("tile", [0, 1, 4, 7, 9, 13, 15])
+@pytest.mark.parametrize("tile", [0, 1, 4, 7, 9, 13, 15, 16, 17, 18, 19, 20])
 def test_ensemble_every_tile_shape(pkg, tile):
     cols = _columns(pkg)[:3]
     _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=2), 2, tile=tile)
 
 
-@pytest.mark.parametrize("tile", [0, 7, 11, 15])
+@pytest.mark.parametrize("tile", [0, 7, 11, 15, 17, 18])
 def test_ensemble_one_block_per_tile(pkg, tile):
     """The batched stencil with one block per logical tile instead of the
     persistent grid: same kernel, same results."""

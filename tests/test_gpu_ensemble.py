@@ -110,7 +110,15 @@ def test_ensemble_autotune_keeps_fields(pkg):
         with pkg.MarlEnsemble(gs) as ens:
             tile = ens.autotune(pkg.SimulationConfig())
             assert 0 <= tile < pkg.load_library().marl_stencil_tile_count()
+            assert ens.launch_shape()["tile"] == tile
             assert ens.time_field(pkg.SimulationConfig(), 4) > 0
+            # one block per logical tile, or a persistent grid of at most that many blocks
+            ens.set_grid(True)
+            shape = ens.launch_shape()
+            assert shape["grid_blocks"] == shape["stencil_tiles"] > 0
+            ens.set_grid(False)
+            shape = ens.launch_shape()
+            assert 0 < shape["grid_blocks"] <= shape["stencil_tiles"]
             with pytest.raises(pkg.InvalidInput):
                 ens.tick(pkg.SimulationConfig(diffusion_substeps=3))
         for g, b in zip(gs, before):

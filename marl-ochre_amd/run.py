@@ -435,6 +435,18 @@ def _ruleset_dumps(g, tick, o: OutputConfig, warn) -> None:
             warn(f"Warning: failed to write binary ruleset full dump at tick {tick}: {e}")
 
 
+USAGE = """usage: python marl-ochre_amd [--config PATH] [--ticks N] [--stats N] [--snapshot N]
+                          [--ruleset-interval N] [--images N] [--seed N] [--output DIR]
+                          [--grid XxYxZ] [--rng-seed N] [--eager] [--gpu-diffusion]
+
+Runs the simulation on the GPU with the reference engine's flags and marl.toml (defaults, then the
+TOML file, then the flags; a flag without a parsable value is skipped). --help prints this text."""
+
+
 def main(argv=None) -> int:
-    cfg = load(sys.argv[1:] if argv is None else argv)
+    args = sys.argv[1:] if argv is None else list(argv)
+    if "--help" in args or "-h" in args:  # the reference has no help flag; this one only prints
+        print(USAGE)
+        return 0
+    cfg = load(args)
     return run(cfg)

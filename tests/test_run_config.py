@@ -37,6 +37,13 @@ def test_output_defaults_match_reference(run):
     assert run.load([], cwd="/nonexistent").output.output_dir == "output/run_128x128x64"
 
 
+def test_help_prints_usage_and_runs_nothing(run, tmp_path, monkeypatch, capsys):
+    monkeypatch.chdir(tmp_path)
+    assert run.main(["--ticks", "3", "--help"]) == 0
+    assert "usage:" in capsys.readouterr().out
+    assert list(tmp_path.iterdir()) == []
+
+
 def test_toml_overrides_then_cli(run, tmp_path):
     (tmp_path / "marl.toml").write_text("""
 [simulation]

@@ -31,13 +31,21 @@ def build() -> None:
     subprocess.run(["make", "-B", "-C", str(ORACLE_DIR)], check=True, capture_output=True)
 
 
+def build_planted(plant: int, out: Path) -> Path:
+    """The oracle built with planted mistake `plant` (ORC_PLANT in marl_oracle.c),
+    for tests that show a check notices that mistake."""
+    subprocess.run(["make", "-C", str(ORACLE_DIR), "plant", f"PLANT={int(plant)}", f"PLANT_OUT={out}"], check=True,
+                   capture_output=True)
+    return out
+
+
 class Oracle:
-    def __init__(self):
-        if not LIB.exists():
+    def __init__(self, lib: Path | None = None):
+        if lib is None and not LIB.exists():
             build()
         A = _abi()
         self.A = A
-        L = ctypes.CDLL(str(LIB))
+        L = ctypes.CDLL(str(lib or LIB))
         P, D, SP = ctypes.c_void_p, ctypes.POINTER(A.marl_dims), ctypes.POINTER(A.marl_sim_params)
         u64, i64, f32 = ctypes.c_uint64, ctypes.c_int64, ctypes.c_float
         sig = {
@@ -183,6 +191,17 @@ class Oracle:
         pr = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
         self.L.orc_hgt_transfer(self._p(r), self._p(pr), len(pr), seed)
         return r
+
+    def find_empty_neighbor(self, pos, cell_at, dims, sim, seed, stream, entity, draws=0):
+        """(target (x, y, z) or None, draws after) of find_empty_neighbor."""
+        p = np.ascontiguousarray(pos, np.uint16).reshape(3)
+        ca = np.ascontiguousarray(cell_at, np.int32)
+        d = np.array([draws], np.uint64)
+        out = np.zeros(3, np.uint16)
+        s, _keep = self._sim(sim)
+        ok = self.L.orc_find_empty_neighbor(self._p(p), self._p(ca), self._dims(dims), s, seed, stream, entity,
+                                            self._p(d), self._p(out))
+        return (tuple(int(v) for v in out) if ok else None), int(d[0])
 
     def compact_indices(self, n, deaths):
         d = np.ascontiguousarray(sorted(deaths), np.int64)

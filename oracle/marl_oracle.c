@@ -25,6 +25,24 @@
 #include <omp.h>
 #endif
 
+/* Planted mistakes.  ORC_PLANT is 0 in the oracle that tests check against.
+ * tests/test_oracle_pins.py builds this file again with each nonzero value to
+ * show that its independent restatements notice that mistake; no product or
+ * bench path ever builds a planted variant. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+#define PLANT_SKIP_PARENT_HALVING 1   /* tick: the parent keeps its internals  */
+#define PLANT_COUNT_SURVIVING_BIRTHS 2 /* tick: dropped births not counted    */
+#define PLANT_KEEP_LAST_DUPLICATE 3   /* tick: the later duplicate birth wins  */
+#define PLANT_ASCENDING_SWAP_REMOVE 4 /* tick: deaths removed low index first  */
+#define PLANT_DAUGHTER_KEEPS_AGE 5    /* tick: the daughter inherits age       */
+#define PLANT_STRUCTURAL_AT_RATE 6    /* mutate: rate, not rate * mult         */
+#define PLANT_META_UNSCALED 7         /* mutate: meta step not * meta rate     */
+#define PLANT_DONOR_AFTER_LOOP 8      /* mutate: active list read live         */
+#define PLANT_SEED_LINEAGE_SWAP 9     /* seeding: lineage halves swapped       */
+#define PLANT_SEED_FEWER_ATTEMPTS 10  /* seeding: count * 4 attempts           */
+
 #define S_EXT 12
 #define M_INT 16
 #define R_MAX 16
@@ -537,7 +555,11 @@ void orc_mutate(marl_ruleset* r, const marl_sim_params* sim, uint64_t seed, uint
   rngs rg = {seed, stream, entity, draws};
   const float rate = r->mutation_rate;
   const float sd = sim->mutation_stddev;
+#if ORC_PLANT == PLANT_STRUCTURAL_AT_RATE
+  const float sr = rate;
+#else
   const float sr = rate * sim->structural_mutation_rate_mult;
+#endif
 
   for (int i = 0; i < MARL_S_RECEPTORS; ++i) { /* :419-426 */
     marl_receptor* rc = &r->receptors[i];
@@ -565,6 +587,11 @@ void orc_mutate(marl_ruleset* r, const marl_sim_params* sim, uint64_t seed, uint
     for (int field = 0; field < 3; ++field) {
       if (rs_u01(&rg) < sr) {
         uint32_t idx = rs_range(&rg, (uint32_t)(na > 1 ? na : 1));
+#if ORC_PLANT == PLANT_DONOR_AFTER_LOOP
+        na = 0;
+        for (int q = 0; q < R_MAX; ++q)
+          if (fabsf(r->reactions[q].v_max) > sim->active_reaction_threshold) active[na++] = q;
+#endif
         if ((int)idx < na) {
           int donor = active[idx];
           if (donor != i) {
@@ -592,7 +619,11 @@ void orc_mutate(marl_ruleset* r, const marl_sim_params* sim, uint64_t seed, uint
   r->division_prep_ticks = fmaxf(r->division_prep_ticks, 1.0f);
   maybe_mutate(&r->hgt_propensity, rate, sd, &rg);
   if (rs_u01(&rg) < sim->meta_mutation_rate) { /* :518-523 */
+#if ORC_PLANT == PLANT_META_UNSCALED
+    r->mutation_rate += rs_normal(&rg, sd);
+#else
     r->mutation_rate += rs_normal(&rg, sd) * sim->meta_mutation_rate;
+#endif
     r->mutation_rate = clampf_rs(r->mutation_rate, sim->meta_mutation_clamp_low, sim->meta_mutation_clamp_high);
   }
 }
@@ -673,20 +704,26 @@ int orc_tick(const marl_dims* d, const marl_sim_params* sim, uint64_t tick, floa
         marl_cell_state dau = cells[i];
         marl_ruleset dr = rules[i];
         dau.pos[0] = tgt[0]; dau.pos[1] = tgt[1]; dau.pos[2] = tgt[2];
+#if ORC_PLANT != PLANT_DAUGHTER_KEEPS_AGE
         dau.age = 0;
+#endif
         dau.prep_remaining = 0;
         uint32_t lo = orc_draw_u32(sim->rng_seed, stream, (uint64_t)i, draws++);
         uint32_t hi = orc_draw_u32(sim->rng_seed, stream, (uint64_t)i, draws++);
         dau.lineage_id = (uint64_t)lo | ((uint64_t)hi << 32);
         for (int k = 0; k < M_INT; ++k) {
           dau.internal[k] *= 0.5f;
+#if ORC_PLANT != PLANT_SKIP_PARENT_HALVING
           cells[i].internal[k] *= 0.5f;
+#endif
         }
         orc_mutate(&dr, sim, sim->rng_seed, stream, (uint64_t)i, &draws);
         bc[nb] = dau;
         br[nb] = dr;
         ++nb;
+#if ORC_PLANT != PLANT_COUNT_SURVIVING_BIRTHS
         stats.divisions++;
+#endif
       }
     } else if (events[i] == MARL_EVENT_DEATH) {
       deaths[nd++] = i;
@@ -695,7 +732,12 @@ int orc_tick(const marl_dims* d, const marl_sim_params* sim, uint64_t tick, floa
   }
 
   /* deaths: sort (already ascending), dedup (distinct), swap_remove descending, :257-267 */
+#if ORC_PLANT == PLANT_ASCENDING_SWAP_REMOVE
+  for (int64_t j0 = 0; j0 < nd; ++j0) {
+    const int64_t j = j0;
+#else
   for (int64_t j = nd - 1; j >= 0; --j) {
+#endif
     const int64_t i = deaths[j];
     cell_at[vidx(d, cells[i].pos[0], cells[i].pos[1], cells[i].pos[2])] = -1;
     cells[i] = cells[n - 1];
@@ -707,6 +749,12 @@ int orc_tick(const marl_dims* d, const marl_sim_params* sim, uint64_t tick, floa
   int rc = 0;
   for (int64_t j = 0; j < nb; ++j) {
     const size_t v = vidx(d, bc[j].pos[0], bc[j].pos[1], bc[j].pos[2]);
+#if ORC_PLANT == PLANT_KEEP_LAST_DUPLICATE
+    int later = 0; /* a later birth at the same voxel takes it instead */
+    for (int64_t q = j + 1; q < nb; ++q)
+      later |= bc[q].pos[0] == bc[j].pos[0] && bc[q].pos[1] == bc[j].pos[1] && bc[q].pos[2] == bc[j].pos[2];
+    if (later) continue;
+#endif
     if (cell_at[v] < 0) {
       if (n >= capacity) { rc = -1; break; }
       cell_at[v] = (int32_t)n;
@@ -714,6 +762,9 @@ int orc_tick(const marl_dims* d, const marl_sim_params* sim, uint64_t tick, floa
       rules[n] = br[j];
       ++n;
       stats.births_added++;
+#if ORC_PLANT == PLANT_COUNT_SURVIVING_BIRTHS
+      stats.divisions++;
+#endif
     }
   }
   stats.n_cells = n;
@@ -733,7 +784,11 @@ int64_t orc_seed_positions(const marl_dims* d, uint64_t seed, uint64_t band, int
   rngs rg = {seed, (ORC_TAG_SEEDING << 32) | band, 0, &draws};
   int64_t seeded = 0;
   const int32_t zh = z_hi < d->z ? z_hi : d->z;
+#if ORC_PLANT == PLANT_SEED_FEWER_ATTEMPTS
+  for (int64_t a = 0; a < count * 4; ++a) {
+#else
   for (int64_t a = 0; a < count * 8; ++a) {
+#endif
     if (seeded >= count) break;
     const int x = margin + (int)rs_range(&rg, (uint32_t)(d->x - 2 * margin));
     const int y = margin + (int)rs_range(&rg, (uint32_t)(d->y - 2 * margin));
@@ -741,6 +796,9 @@ int64_t orc_seed_positions(const marl_dims* d, uint64_t seed, uint64_t band, int
     const size_t v = vidx(d, x, y, z);
     if (cell_at[v] >= 0) continue;
     uint32_t lo = rs_u32(&rg), hi = rs_u32(&rg);
+#if ORC_PLANT == PLANT_SEED_LINEAGE_SWAP
+    { const uint32_t t = lo; lo = hi; hi = t; }
+#endif
     cell_at[v] = (int32_t)(index_base + seeded);
     pos_out[3 * seeded + 0] = (uint16_t)x;
     pos_out[3 * seeded + 1] = (uint16_t)y;

@@ -19,10 +19,18 @@
  *   light                 FP64 closed form, geometric decay, floor switch
  *   cell tick             hand-derived known answers, invariants, fate truth table
  *   neighbours / fate /
- *   compaction            brute-force sequential emulation (Python) on tiny grids
- *   mutate                draw-count and clamp invariants only — parity unpinned
- *                         against the reference (its rand 0.9.2 ThreadRng stream
- *                         is OS-seeded, crates/marl-sim/src/lib.rs:35)
+ *   compaction            brute-force sequential emulation (Python) on tiny grids;
+ *                         the tick's fate, birth, death and compaction block
+ *                         against a literal Python restatement of
+ *                         crates/marl-sim/src/lib.rs:220-277 (tests/restate.py)
+ *   mutate                a draw-order Python restatement of cell.rs:406-524
+ *                         (structural mult != 1, meta mutation), the structural
+ *                         rate rate * mult and the meta step's spread measured,
+ *                         the normal sampler's moments and KS distance
+ *   seeding               a Python restatement of seeding.rs:28-58
+ *   (each of the last three also catches mistakes planted with ORC_PLANT);
+ *   the draw streams themselves stay unpinned against the reference, whose
+ *   rand 0.9.2 ThreadRng is OS-seeded (crates/marl-sim/src/lib.rs:35)
  *   hgt transfer          slot rules (first inactive slot, else a drawn one),
  *                         donor untouched, the v_max thresholds, the pick's
  *                         uniformity over active reactions (chi-square) and its

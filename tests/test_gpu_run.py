@@ -24,6 +24,50 @@ def _config(run, tmp_path, name, extra=()):
     return cfg
 
 
+def _primed_field(dims, sim):
+    """init_field_boundaries (crates/marl-sim/src/seeding.rs:9-26): the top
+    boundary_prime_layers hold oxidant (1) and carbon (3), the bottom ones
+    reductant (2)."""
+    X, Y, Z = dims
+    f = np.zeros((Z, Y, X, 12), np.float32)
+    for z in range(sim.boundary_prime_layers):
+        f[z, :, :, 1] = sim.boundary_prime_oxidant
+        f[z, :, :, 3] = sim.boundary_prime_carbon
+    for z in range(Z - sim.boundary_prime_layers, Z):
+        f[z, :, :, 2] = sim.boundary_prime_reductant
+    return f.reshape(-1)
+
+
+def _oracle_population(orc, dims, sim, count, seed):
+    """run()'s three seed_cells calls (crates/marl-sim/src/lib.rs:59-104) with the
+    oracle's seeding: depth bands from the canonical 200-layer depths scaled by
+    Z/200 in f32 and cast to u16, widened to lo+1 / lo+3 / lo+3."""
+    from marl_ochre_amd import starters as st
+
+    A = orc.A
+    f32 = np.float32
+    z_scale = f32(dims[2]) / f32(200.0)
+    bands = []
+    for lo_f, hi_f, widen in ((sim.phototroph_z_lo, sim.phototroph_z_hi, 1.0),
+                              (sim.chemolithotroph_z_lo, sim.chemolithotroph_z_hi, 3.0),
+                              (sim.anaerobe_z_lo, sim.anaerobe_z_hi, 3.0)):
+        lo = int(f32(lo_f) * z_scale)
+        bands.append((lo, int(max(f32(hi_f) * z_scale, f32(lo) + f32(widen)))))
+    ca = None
+    cells, rules = [], []
+    for kind, (lo, hi) in enumerate(bands):
+        pos, lin, ca = orc.seed_positions(dims, seed, kind, count, sim.seed_margin, lo, hi, ca,
+                                          index_base=sum(len(x) for x in cells))
+        c = np.zeros(len(pos), A.CELL_DTYPE)
+        r = np.zeros(len(pos), A.RULESET_DTYPE)
+        for k in range(len(pos)):
+            c[k] = st.make_cell(kind, pos[k], int(lin[k]))
+            r[k] = st.make_ruleset(kind)
+        cells.append(c)
+        rules.append(r)
+    return np.concatenate(cells), np.concatenate(rules)
+
+
 def _seq(x):
     x = np.asarray(x, np.float64).reshape(-1)
     return float(np.cumsum(x)[-1]) if x.size else 0.0
@@ -41,11 +85,11 @@ def test_run_matches_oracle_run(pkg, orc, tmp_path):
     for t in (0, 10, 20, TICKS - 1):
         assert f"t={t:>5} | " in text, t
 
-    # the same run through the oracle
+    # the same run through the oracle, set up without the product's helpers
     dims = cfg.grid
     sim = cfg.simulation.replace(rng_seed=5)
-    f = run.primed_field(dims, sim)
-    c, r = run.seed_population(dims, sim, 40, 5)
+    f = _primed_field(dims, sim)
+    c, r = _oracle_population(orc, dims, sim, 40, 5)
     assert len(c) > 60
     rows, events = [], 0
     for t in range(TICKS):

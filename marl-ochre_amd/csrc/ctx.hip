@@ -57,18 +57,39 @@ int marl_device_count(void) {
   return n;
 }
 
-const char* marl_last_error(const marl_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+namespace {
+// Why the calling thread's last marl_create failed (there is no context to
+// hold the message).
+thread_local std::string t_create_err;
+}  // namespace
+
+const char* marl_last_error(const marl_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  return t_create_err.empty() ? "null context" : t_create_err.c_str();
+}
 
 int marl_create(const marl_dims* dims, marl_ctx** out) {
-  if (!out) return MARL_ERR_INVALID_INPUT;
+  t_create_err.clear();
+  if (!out) {
+    t_create_err = "marl_create: null output pointer";
+    return MARL_ERR_INVALID_INPUT;
+  }
   *out = nullptr;
   if (!dims || dims->x <= 0 || dims->y <= 0 || dims->z <= 0 || dims->x % 32 != 0 || dims->x > 65535 ||
-      dims->y > 65535 || dims->z > 65535)
+      dims->y > 65535 || dims->z > 65535) {
+    t_create_err = "marl_create: grid must be positive, at most 65535 per axis, with x a multiple of 32";
     return MARL_ERR_INVALID_INPUT;
+  }
   // kernels address one field buffer with 32-bit byte offsets
-  if ((uint64_t)dims->x * dims->y * dims->z * MARL_S_EXT * 4ull > 0x7fffffffull) return MARL_ERR_INVALID_INPUT;
+  if ((uint64_t)dims->x * dims->y * dims->z * MARL_S_EXT * 4ull > 0x7fffffffull) {
+    t_create_err = "marl_create: one field buffer would exceed 2 GiB";
+    return MARL_ERR_INVALID_INPUT;
+  }
   int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return MARL_ERR_NO_DEVICE;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    t_create_err = "marl_create: no GPU visible";
+    return MARL_ERR_NO_DEVICE;
+  }
   marl_ctx* c = new marl_ctx();
   c->d = *dims;
   c->nvox = (size_t)dims->x * dims->y * dims->z;
@@ -76,8 +97,7 @@ int marl_create(const marl_dims* dims, marl_ctx** out) {
   c->xw = dims->x / 32;
   c->occ_words = c->nvox / 32;
   auto fail = [&](hipError_t e, const char* what) {
-    (void)e;
-    (void)what;
+    t_create_err = std::string("marl_create: ") + what + ": " + hipGetErrorString(e);
     marl_destroy(c);
     return MARL_ERR_DEVICE_INIT;
   };

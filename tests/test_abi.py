@@ -63,6 +63,20 @@ def test_invalid_dims_rejected_without_device(pkg):
     for dims in [(100, 8, 8), (0, 8, 8), (64, -1, 8)]:
         d = pkg._abi.marl_dims(*dims)
         assert lib.marl_create(ctypes.byref(d), ctypes.byref(h)) == pkg._abi.ERR_INVALID_INPUT
+        # a failed create leaves its reason for marl_last_error(NULL)
+        assert b"multiple of 32" in lib.marl_last_error(None)
+
+
+def test_failed_create_keeps_its_message(pkg):
+    lib = pkg.load_library()
+    h = ctypes.c_void_p()
+    d = pkg._abi.marl_dims(1024, 1024, 1024)   # a 48 GiB field buffer
+    assert lib.marl_create(ctypes.byref(d), ctypes.byref(h)) == pkg._abi.ERR_INVALID_INPUT
+    assert b"2 GiB" in lib.marl_last_error(None)
+    if lib.marl_device_count() == 0:
+        d = pkg._abi.marl_dims(64, 8, 8)
+        assert lib.marl_create(ctypes.byref(d), ctypes.byref(h)) == pkg._abi.ERR_NO_DEVICE
+        assert b"no GPU" in lib.marl_last_error(None)
 
 
 def test_sim_params_defaults_match_reference(pkg):

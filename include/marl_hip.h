@@ -300,6 +300,16 @@ typedef struct {
 } marl_population_summary;
 /* z_counts (may be NULL): cells per z-layer, nz >= Z. */
 int marl_cell_summary(marl_ctx* ctx, marl_population_summary* out, int64_t* z_counts, size_t nz);
+/* marl_cell_summary without allocation.  No counterpart in the reference; it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same inputs,
+ * outputs and results, bit for bit, as marl_cell_summary, which allocates and
+ * frees its partial sums on every call; this variant keeps them in a fixed
+ * 72 KiB device array per GPU, set up by marl_create, and calls no allocator.
+ * Grids deeper than 4096 layers are rejected (MARL_ERR_INVALID_INPUT).  Every
+ * context on one GPU shares that array: calls that overlap on one GPU (from
+ * several host threads) race. */
+int marl_cell_summary_noalloc(marl_ctx* ctx, marl_population_summary* out, int64_t* z_counts, size_t nz);
 /* ≙ DataLogger::snapshot_chemistry (data.rs:292-335): per z-layer, the FP64 mean
  * over the XY plane of each of the 12 species, then of light: out[z*13 + c]. */
 int marl_chemistry_profile(marl_ctx* ctx, double* out, size_t n);

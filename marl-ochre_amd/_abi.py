@@ -247,6 +247,7 @@ SIGNATURES = {
                                            _I32, _I32, _P, _P, _P, _I64, ctypes.POINTER(_I64)]),
     "marl_sync": (ctypes.c_int, [_P]),
     "marl_cell_summary": (ctypes.c_int, [_P, ctypes.POINTER(marl_population_summary), _P, _SZ]),
+    "marl_cell_summary_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(marl_population_summary), _P, _SZ]),
     "marl_chemistry_profile": (ctypes.c_int, [_P, _P, _SZ]),
     "marl_download_viewer_cells": (ctypes.c_int, [_P, _P, _SZ, ctypes.POINTER(_I64)]),
     "marl_sample": (ctypes.c_int, [_P, _P, _SZ, _P]),

@@ -103,6 +103,7 @@ int marl_create(const marl_dims* dims, marl_ctx** out) {
   };
   hipError_t e;
   if ((e = hipSetDevice(0)) != hipSuccess) return fail(e, "hipSetDevice");
+  if (marl_noalloc_init_device(0) != MARL_OK) return fail(hipErrorInvalidSymbol, "scratch symbol address");
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
   for (int i = 0; i < 2; ++i)
     if ((e = hipMalloc(&c->field[i], c->nfield * sizeof(float))) != hipSuccess) return fail(e, "field");

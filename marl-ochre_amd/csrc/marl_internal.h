@@ -278,6 +278,14 @@ int marl_hip_fail(marl_ctx* ctx, hipError_t e, const char* what);
     if (_e != hipSuccess) return marl_hip_fail((ctx), _e, #expr); \
   } while (0)
 
+// Allocation-free population summary (output.hip): the file-scope scratch
+// array's size and the deepest grid it serves.
+constexpr int kNoAllocSummaryMaxZ = 4096;
+constexpr size_t kNoAllocScratchBytes = 72 * 1024;
+// Looks up the scratch array's address on `device` (which must be current);
+// called once per device by marl_create.
+int marl_noalloc_init_device(int device);
+
 StencilParams marl_stencil_params(const marl_sim_params* sim, int apply_sources);
 int marl_check_params(marl_ctx* ctx, const marl_sim_params* sim);
 

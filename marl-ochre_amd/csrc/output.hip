@@ -9,6 +9,7 @@
 //                        crates/marl-format/src/lib.rs:350-364 (ViewerCellRecord, 25 bytes packed)
 //   point samples        crates/marl-sim/src/stats.rs:51-56, 86-103 (centre-column reads)
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "marl_internal.h"
@@ -123,7 +124,66 @@ int dev_scratch(marl_ctx* ctx, T** p, size_t count) {
   return MARL_OK;
 }
 
+// ---- the population summary without allocation ------------------------------
+// marl_cell_summary_noalloc keeps its partial sums, counters and result record
+// in one file-scope device array instead of memory it allocates per call.
+// Each GPU holds its own copy of the array at its own address; host code may
+// use only the address hipGetSymbolAddress returns for the current device,
+// looked up once per device at context creation (marl_noalloc_init_device).
+struct NoAllocSummary {
+  double partial[kSumBlocks * 3];                    // FP64 block partials
+  unsigned long long counts[4 + kNoAllocSummaryMaxZ];  // active, thirds, per-layer
+  marl_population_summary result;                    // summed on the device
+  int64_t z_counts[kNoAllocSummaryMaxZ];
+};
+static_assert(sizeof(NoAllocSummary) <= kNoAllocScratchBytes, "summary fits the scratch array");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Counters and partials to zero (a fill kernel, never a memset node, so that
+// the call stays capturable).
+__global__ void k_summary_clear(NoAllocSummary* s, int nz) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 4 + nz; k += gridDim.x * blockDim.x) s->counts[k] = 0ull;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < kSumBlocks * 3; k += gridDim.x * blockDim.x)
+    s->partial[k] = 0.0;
+}
+
+// The partials summed in block order, one thread: the same FP64 additions in
+// the same order as marl_cell_summary's host loop, so the sums are identical.
+__global__ void k_summary_finish(NoAllocSummary* s, int64_t n, int nz) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    marl_population_summary r{};
+    r.n_cells = n;
+    for (int b = 0; b < kSumBlocks; ++b) {
+      r.sum_energy += s->partial[b * 3];
+      r.sum_enzyme_a += s->partial[b * 3 + 1];
+      r.sum_enzyme_b += s->partial[b * 3 + 2];
+    }
+    r.sum_active_reactions = (int64_t)s->counts[0];
+    for (int k = 0; k < 3; ++k) r.z_thirds[k] = (int64_t)s->counts[1 + k];
+    s->result = r;
+  }
+  for (int z = blockIdx.x * blockDim.x + threadIdx.x; z < nz; z += gridDim.x * blockDim.x)
+    s->z_counts[z] = (int64_t)s->counts[4 + z];
+}
+
 }  // namespace
+
+int marl_noalloc_init_device(int device) {
+  if (device < 0 || device >= kMaxDevices) return MARL_ERR_DEVICE_INIT;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return MARL_OK;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != device) return MARL_ERR_DEVICE_INIT;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return MARL_ERR_DEVICE_INIT;
+  g_scratch_ptr[device] = ptr;
+  return MARL_OK;
+}
 
 extern "C" {
 
@@ -172,6 +232,33 @@ int marl_cell_summary(marl_ctx* ctx, marl_population_summary* out, int64_t* z_co
   *out = s;
   if (z_counts)
     for (int z = 0; z < Z; ++z) z_counts[z] = (int64_t)hc[4 + z];
+  return MARL_OK;
+}
+
+int marl_cell_summary_noalloc(marl_ctx* ctx, marl_population_summary* out, int64_t* z_counts, size_t nz) {
+  if (!ctx || !out) return MARL_ERR_INVALID_INPUT;
+  const int Z = ctx->d.z;
+  if (z_counts && nz < (size_t)Z) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "z_counts shorter than Z");
+  if (Z > kNoAllocSummaryMaxZ)
+    return marl_fail(ctx, MARL_ERR_INVALID_INPUT,
+                     "marl_cell_summary_noalloc: Z above the scratch limit of " +
+                         std::to_string(kNoAllocSummaryMaxZ) + " layers");
+  void* base = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+  if (!base) return marl_fail(ctx, MARL_ERR_DEVICE_INIT, "marl_cell_summary_noalloc: scratch address not set up");
+  NoAllocSummary* s = static_cast<NoAllocSummary*>(base);
+  const int n = (int)ctx->n_cells;
+  hipLaunchKernelGGL(k_summary_clear, dim3(8), dim3(256), 0, ctx->stream, s, Z);
+  if (n > 0)
+    hipLaunchKernelGGL(k_cell_summary, dim3(kSumBlocks), dim3(256), 0, ctx->stream, ctx->cells[ctx->cbuf],
+                       ctx->rules[ctx->cbuf], n, Z, s->partial, s->counts, s->counts + 4);
+  hipLaunchKernelGGL(k_summary_finish, dim3(8), dim3(256), 0, ctx->stream, s, (int64_t)n, Z);
+  MARL_HIP(ctx, hipGetLastError());
+  MARL_HIP(ctx, hipMemcpyAsync(out, &s->result, sizeof(marl_population_summary), hipMemcpyDeviceToHost,
+                               ctx->stream));
+  if (z_counts)
+    MARL_HIP(ctx, hipMemcpyAsync(z_counts, s->z_counts, sizeof(int64_t) * (size_t)Z, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MARL_OK;
 }
 

@@ -215,6 +215,16 @@ int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, in
  * with no counterpart in the reference (whose field is always AoS,
  * crates/marl-field/src/field.rs:15-35). */
 int marl_set_cell_layout(marl_ctx* ctx, int32_t layout);
+/* Threads per block of the plain cell pass: 64, 128 (default) or 256; results
+ * are identical for every size.  A tuning knob with no reference counterpart. */
+int marl_set_kinetics_block(marl_ctx* ctx, int32_t block);
+int marl_get_kinetics_block(marl_ctx* ctx, int32_t* block);
+/* The project's kinetics autotuner: times the plain cell pass (the ordered
+ * per-level launches with the field layout's transposes) on the resident
+ * field, light and population with each block size, keeps the fastest and
+ * writes it to *block_out.  Field and cells are saved first and restored after
+ * every trial, so the resident state is unchanged. */
+int marl_autotune_kinetics(marl_ctx* ctx, const marl_sim_params* sim, int32_t* block_out);
 /* Budget of relaxation sweeps and level launches in the captured tick (1..64
  * each; a change captures the graph again).  (0, 0) selects the default
  * automatic budget: it starts at 64 each and then follows what the schedule

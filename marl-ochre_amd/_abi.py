@@ -226,6 +226,9 @@ SIGNATURES = {
     "marl_tick": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I64, _I32, ctypes.POINTER(marl_tick_stats)]),
     "marl_tick_graph": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I64, _I32,
                                        ctypes.POINTER(marl_tick_stats), ctypes.POINTER(_I32)]),
+    "marl_set_kinetics_block": (ctypes.c_int, [_P, _I32]),
+    "marl_get_kinetics_block": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
+    "marl_autotune_kinetics": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), ctypes.POINTER(ctypes.c_int32)]),
     "marl_set_graph_limits": (ctypes.c_int, [_P, _I32, _I32]),
     "marl_set_cell_layout": (ctypes.c_int, [_P, _I32]),
     "marl_cell_pass": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _P, _SZ, ctypes.POINTER(_I32)]),

@@ -309,8 +309,10 @@ __global__ void __launch_bounds__(256) k_level_scatter(int n, const int32_t* __r
 // indexed per-cell arrays live in LDS (one private slot per thread, odd
 // strides), so nothing goes to scratch.  With a harvest sink, sampled cells
 // also append their (features, post-kinetics deltas and fluxes) pair.
-constexpr int kKinBlock = 128;
+constexpr int kKinBlock = 128;  // default block of the plain pass; marl_autotune_kinetics picks 64, 128 or 256
 
+// The six face records gathered for the environment are kept for the
+// scatter, which then writes without reading them again.
 template <bool HARVEST, bool QUADS>
 __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict__ cells,
                                               const marl_ruleset* __restrict__ rules, float* __restrict__ field,
@@ -323,11 +325,14 @@ __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict
   const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
 #pragma unroll
   for (int k = 0; k < MARL_M_INT; ++k) sI[k] = c.internal[k];
-  gather_env6<QUADS>(field, v, m, p, sE);
+  float4 keep[18];
+  gather_env6_keep<QUADS>(field, v, m, p, sE, keep);
   const float lt = light[v];
   PtrArr I{sI}, D{sD};
   const PtrArr E{sE};
-  cell_kinetics(c, r, I, E, D, lt, p);
+  const AosRules ar{&r};
+  cell_kinetics(c, ar, I, E, D, lt, p);
+  const float4 fate = ar.fate();
   // one counter update per wave: a per-cell atomic would serialise on one address
   const bool take = HARVEST && (uint64_t)sample_hash(hv.tick, (uint32_t)i) < hv.threshold;
   unsigned long long slot = 0;
@@ -355,27 +360,27 @@ __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict
       hv.hash[slot] = topology_hash(r);
     }
   }
-  const uint8_t ev = cell_fate(c, r, sI[0], p);
+  const uint8_t ev = cell_fate(c, fate, sI[0], p);
 #pragma unroll
   for (int k = 0; k < MARL_M_INT; ++k) c.internal[k] = sI[k];
   cells[i] = c;
   events[i] = ev;
-  scatter_deltas6<QUADS>(field, v, m, p, sD);
+  scatter_from_kept<QUADS>(field, v, m, p, sD, keep);
 }
 
-template <bool HARVEST, bool QUADS>
-__global__ void __launch_bounds__(kKinBlock) k_kinetics(const int32_t* __restrict__ order, int count,
-                                                      marl_cell_state* __restrict__ cells,
-                                                      const marl_ruleset* __restrict__ rules,
-                                                      float* __restrict__ field, const float* __restrict__ light,
-                                                      const uint8_t* __restrict__ nbmask,
-                                                      uint8_t* __restrict__ events, CellParams p, HarvestSink hv) {
-  __shared__ float sI[kKinBlock * 17];
-  __shared__ float sE[kKinBlock * 13];
-  __shared__ float sD[kKinBlock * 13];
-  const int t = blockIdx.x * kKinBlock + threadIdx.x;
+// One level: `order` points at the level's first cell.
+template <bool HARVEST, bool QUADS, int B = kKinBlock>
+__global__ void __launch_bounds__(B) k_kinetics(const int32_t* __restrict__ order, int count,
+                                              marl_cell_state* __restrict__ cells,
+                                              const marl_ruleset* __restrict__ rules, float* __restrict__ field,
+                                              const float* __restrict__ light, const uint8_t* __restrict__ nbmask,
+                                              uint8_t* __restrict__ events, CellParams p, HarvestSink hv) {
+  __shared__ float sI[B * 17];
+  __shared__ float sE[B * 13];
+  __shared__ float sD[B * 13];
+  const int t = blockIdx.x * B + threadIdx.x;
   if (t >= count) return;
-  const int i = order ? order[t] : t;
+  const int i = order[t];
   kinetics_cell<HARVEST, QUADS>(i, cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17,
                                 sE + threadIdx.x * 13, sD + threadIdx.x * 13);
 }
@@ -812,17 +817,16 @@ __global__ void __launch_bounds__(256) k_g_level_scatter(const TickDev* ts, cons
   level_scatter_block(ts->n, level, cursor, order);
 }
 
-template <bool QUADS>
-__global__ void __launch_bounds__(kKinBlock) k_g_kinetics(int L, const TickDev* ts, const int32_t* __restrict__ order,
-                                                        marl_cell_state* __restrict__ cells,
-                                                        const marl_ruleset* __restrict__ rules,
-                                                        float* __restrict__ field, const float* __restrict__ light,
-                                                        const uint8_t* __restrict__ nbmask,
-                                                        uint8_t* __restrict__ events, CellParams p) {
+template <bool QUADS, int B>
+__global__ void __launch_bounds__(B) k_g_kinetics(int L, const TickDev* ts, const int32_t* __restrict__ order,
+                                                marl_cell_state* __restrict__ cells,
+                                                const marl_ruleset* __restrict__ rules, float* __restrict__ field,
+                                                const float* __restrict__ light, const uint8_t* __restrict__ nbmask,
+                                                uint8_t* __restrict__ events, CellParams p) {
   if (ts->err || L >= ts->nlev) return;
-  __shared__ float sI[kKinBlock * 17];
-  __shared__ float sE[kKinBlock * 13];
-  __shared__ float sD[kKinBlock * 13];
+  __shared__ float sI[B * 17];
+  __shared__ float sE[B * 13];
+  __shared__ float sD[B * 13];
   const int lo = ts->off[L], cnt = ts->off[L + 1] - lo;
   const HarvestSink hv{};
   GS_LOOP(t, cnt)
@@ -1080,6 +1084,32 @@ namespace {
 
 // The ordered cell pass (lib.rs:204-217) on the resident field, light and
 // occupancy.  Leaves events in ctx->events; *levels_out = schedule depth.
+// One level of the plain pass with the context's kinetics block size.
+template <bool QUADS>
+void launch_plain_level_q(marl_ctx* ctx, const int32_t* ord, int cnt, marl_cell_state* cells,
+                          const marl_ruleset* rules, float* field, const CellParams& p) {
+  const HarvestSink hv{};
+  switch (ctx->kin_block) {
+    case 64:
+      hipLaunchKernelGGL((k_kinetics<false, QUADS, 64>), dim3(grid1(cnt, 64)), dim3(64), 0, ctx->stream, ord, cnt,
+                         cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p, hv);
+      break;
+    case 256:
+      hipLaunchKernelGGL((k_kinetics<false, QUADS, 256>), dim3(grid1(cnt, 256)), dim3(256), 0, ctx->stream, ord,
+                         cnt, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p, hv);
+      break;
+    default:
+      hipLaunchKernelGGL((k_kinetics<false, QUADS, 128>), dim3(grid1(cnt, 128)), dim3(128), 0, ctx->stream, ord,
+                         cnt, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p, hv);
+  }
+}
+
+void launch_plain_level(marl_ctx* ctx, bool quads, const int32_t* ord, int cnt, marl_cell_state* cells,
+                        const marl_ruleset* rules, float* field, const CellParams& p) {
+  if (quads) launch_plain_level_q<true>(ctx, ord, cnt, cells, rules, field, p);
+  else launch_plain_level_q<false>(ctx, ord, cnt, cells, rules, field, p);
+}
+
 int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEvent_t ev_sched) {
   const int n = (int)ctx->n_cells;
   if (levels_out) *levels_out = 0;
@@ -1160,12 +1190,8 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
     } else if (hv.count) {
       hipLaunchKernelGGL((k_kinetics<true, false>), grid, block, 0, ctx->stream, ord, cnt, cells, rules, field,
                          ctx->light, ctx->nbmask, ctx->events, p, hv);
-    } else if (quads) {
-      hipLaunchKernelGGL((k_kinetics<false, true>), grid, block, 0, ctx->stream, ord, cnt, cells, rules, field,
-                         ctx->light, ctx->nbmask, ctx->events, p, hv);
     } else {
-      hipLaunchKernelGGL((k_kinetics<false, false>), grid, block, 0, ctx->stream, ord, cnt, cells, rules, field,
-                         ctx->light, ctx->nbmask, ctx->events, p, hv);
+      launch_plain_level(ctx, quads, ord, cnt, cells, rules, field, p);
     }
   }
   MARL_HIP(ctx, hipGetLastError());
@@ -1307,7 +1333,8 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
   hipStream_t st = ctx->stream;
   marl_cell_state* cells = ctx->cells[ctx->cbuf];
   marl_ruleset* rules = ctx->rules[ctx->cbuf];
-  const unsigned gc = gs_grid(cap), gk = gs_grid(cap, kKinBlock, 4096), gf = gs_grid(cap + 1);
+  const int kb = ctx->kin_block;
+  const unsigned gc = gs_grid(cap), gk = gs_grid(cap, kb, 16384 * 64 / kb), gf = gs_grid(cap + 1);
   // schedule
   hipLaunchKernelGGL(k_g_cell_prep, dim3(gc), dim3(256), 0, st, cells, ts, ctx->cell_at, ctx->occ, ctx->nbmask,
                      ctx->preds, (size_t)cap, ctx->npred, ctx->level, p);
@@ -1326,15 +1353,28 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
   MARL_HIP(ctx, hipGetLastError());
   // the ordered pass on the AoS view, or on the quads in place (the layout
   // chosen when the tick was prepared; it is part of the graph's key)
+  auto level = [&](int L, float* field, bool quads) {
+    if (quads) {
+      if (kb == 64) hipLaunchKernelGGL((k_g_kinetics<true, 64>), dim3(gk), dim3(64), 0, st, L, ts, ctx->order,
+                                       cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p);
+      else if (kb == 256) hipLaunchKernelGGL((k_g_kinetics<true, 256>), dim3(gk), dim3(256), 0, st, L, ts,
+                                             ctx->order, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p);
+      else hipLaunchKernelGGL((k_g_kinetics<true, 128>), dim3(gk), dim3(128), 0, st, L, ts, ctx->order, cells,
+                              rules, field, ctx->light, ctx->nbmask, ctx->events, p);
+    } else {
+      if (kb == 64) hipLaunchKernelGGL((k_g_kinetics<false, 64>), dim3(gk), dim3(64), 0, st, L, ts, ctx->order,
+                                       cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p);
+      else if (kb == 256) hipLaunchKernelGGL((k_g_kinetics<false, 256>), dim3(gk), dim3(256), 0, st, L, ts,
+                                             ctx->order, cells, rules, field, ctx->light, ctx->nbmask, ctx->events, p);
+      else hipLaunchKernelGGL((k_g_kinetics<false, 128>), dim3(gk), dim3(128), 0, st, L, ts, ctx->order, cells,
+                              rules, field, ctx->light, ctx->nbmask, ctx->events, p);
+    }
+  };
   if (ctx->kin_quads) {
-    for (int L = 0; L < ctx->graph_levels; ++L)
-      hipLaunchKernelGGL(k_g_kinetics<true>, dim3(gk), dim3(kKinBlock), 0, st, L, ts, ctx->order, cells, rules,
-                         ctx->field[ctx->cur], ctx->light, ctx->nbmask, ctx->events, p);
+    for (int L = 0; L < ctx->graph_levels; ++L) level(L, ctx->field[ctx->cur], true);
   } else {
     if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
-    for (int L = 0; L < ctx->graph_levels; ++L)
-      hipLaunchKernelGGL(k_g_kinetics<false>, dim3(gk), dim3(kKinBlock), 0, st, L, ts, ctx->order, cells, rules,
-                         ctx->staging, ctx->light, ctx->nbmask, ctx->events, p);
+    for (int L = 0; L < ctx->graph_levels; ++L) level(L, ctx->staging, false);
     if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
   }
   MARL_HIP(ctx, hipGetLastError());
@@ -1377,7 +1417,7 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
     marl_sim_params sim;
     int64_t cap;
     const void *cells, *rules, *f0, *f1, *staging;
-    int32_t tile, cur, cbuf, sweeps, levels, kin_quads;
+    int32_t tile, cur, cbuf, sweeps, levels, kin_quads, kin_block;
   } k;
   memset(&k, 0, sizeof(k));
   k.sim = *sim;
@@ -1393,6 +1433,7 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
   k.sweeps = ctx->graph_sweeps;
   k.levels = ctx->graph_levels;
   k.kin_quads = ctx->kin_quads ? 1 : 0;
+  k.kin_block = ctx->kin_block;
   const uint8_t* b = reinterpret_cast<const uint8_t*>(&k);
   return std::vector<uint8_t>(b, b + sizeof(k));
 }
@@ -1786,6 +1827,90 @@ int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, in
   }
   if (stats) *stats = st;
   if (host_finished) *host_finished = finished;
+  return MARL_OK;
+}
+
+int marl_set_kinetics_block(marl_ctx* ctx, int32_t block) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  if (block != 64 && block != 128 && block != 256)
+    return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "kinetics block must be 64, 128 or 256 threads");
+  ctx->kin_block = block;
+  return MARL_OK;
+}
+
+int marl_get_kinetics_block(marl_ctx* ctx, int32_t* block) {
+  if (!ctx || !block) return MARL_ERR_INVALID_INPUT;
+  *block = ctx->kin_block;
+  return MARL_OK;
+}
+
+// Times the plain cell pass on the resident field, light and population with
+// each block size and keeps the fastest; the field and the cells are saved
+// before the trials and restored after each, so the state is unchanged.
+int marl_autotune_kinetics(marl_ctx* ctx, const marl_sim_params* sim, int32_t* block_out) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  if (int rc = marl_check_params(ctx, sim)) return rc;
+  const int64_t n = ctx->n_cells;
+  if (n == 0) {
+    if (block_out) *block_out = ctx->kin_block;
+    return MARL_OK;
+  }
+  ctx->last_sim = *sim;
+  if (int rc = ensure_voxel_buffers(ctx)) return rc;
+  if (int rc = marl_ensure_cell_capacity(ctx, 2 * n + 1024)) return rc;
+  const size_t fbytes = ctx->nfield * sizeof(float), cbytes = (size_t)n * sizeof(marl_cell_state);
+  void* save_f = nullptr;
+  void* save_c = nullptr;
+  MARL_HIP(ctx, hipMalloc(&save_f, fbytes));
+  if (hipMalloc(&save_c, cbytes) != hipSuccess) {
+    (void)hipFree(save_f);
+    return marl_fail(ctx, MARL_ERR_TRANSFER, "marl_autotune_kinetics: allocation failed");
+  }
+  struct Free {
+    void* a;
+    void* b;
+    ~Free() {
+      (void)hipFree(a);
+      (void)hipFree(b);
+    }
+  } fr{save_f, save_c};
+  float* field = ctx->field[ctx->cur];
+  marl_cell_state* cells = ctx->cells[ctx->cbuf];
+  MARL_HIP(ctx, hipMemcpyAsync(save_f, field, fbytes, hipMemcpyDeviceToDevice, ctx->stream));
+  MARL_HIP(ctx, hipMemcpyAsync(save_c, cells, cbytes, hipMemcpyDeviceToDevice, ctx->stream));
+  if (int rc = marl_launch_occupancy_from_cells(ctx)) return rc;
+  const CellParams p = cell_params(ctx, sim, 0);
+  const int keep = ctx->kin_block, kin_mode = ctx->kin_mode;
+  ctx->kin_mode = MARL_KINETICS_PLAIN;
+  const int cands[3] = {64, 128, 256};
+  float best_ms = 1e30f;
+  int best = keep, rc = MARL_OK;
+  for (int rep = 0; rep < 2 && rc == MARL_OK; ++rep)
+    for (int b : cands) {
+      ctx->kin_block = b;
+      int32_t levels = 0;
+      hipEvent_t ev = nullptr;
+      rc = hipEventCreate(&ev) == hipSuccess ? MARL_OK : MARL_ERR_TRANSFER;
+      if (rc == MARL_OK) rc = run_cell_pass(ctx, p, &levels, ev);  // ev marks the end of the schedule
+      if (rc == MARL_OK && hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess) rc = MARL_ERR_TRANSFER;
+      if (rc == MARL_OK && hipEventSynchronize(ctx->ev1) != hipSuccess) rc = MARL_ERR_TRANSFER;
+      float ms = 0.0f;
+      if (rc == MARL_OK) (void)hipEventElapsedTime(&ms, ev, ctx->ev1);
+      if (ev) (void)hipEventDestroy(ev);
+      if (rc == MARL_OK && rep > 0 && ms < best_ms) {
+        best_ms = ms;
+        best = b;
+      }
+      if (hipMemcpyAsync(field, save_f, fbytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
+          hipMemcpyAsync(cells, save_c, cbytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+        rc = MARL_ERR_TRANSFER;
+      if (rc != MARL_OK) break;
+    }
+  ctx->kin_mode = kin_mode;
+  ctx->kin_block = rc == MARL_OK ? best : keep;
+  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (rc != MARL_OK) return marl_fail(ctx, rc, "marl_autotune_kinetics: a trial pass failed");
+  if (block_out) *block_out = ctx->kin_block;
   return MARL_OK;
 }
 

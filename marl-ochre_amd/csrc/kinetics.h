@@ -65,6 +65,19 @@ __device__ __forceinline__ marl_cell_state load_cell(const marl_cell_state* c) {
   return out;
 }
 
+// The parameters the tick reads from a cell's ruleset, through accessors so
+// that cell_kinetics does not depend on the record layout.
+struct AosRules {
+  const marl_ruleset* r;
+  __device__ __forceinline__ marl_transport transport(int q) const { return r->transport[q]; }
+  __device__ __forceinline__ marl_reaction reaction(int q) const { return r->reactions[q]; }
+  __device__ __forceinline__ marl_effector effector(int q) const { return r->effectors[q]; }
+  // (division, death, quiescence energies, division prep ticks)
+  __device__ __forceinline__ float4 fate() const {
+    return make_float4(r->division_energy, r->death_energy, r->quiescence_energy, r->division_prep_ticks);
+  }
+};
+
 // A voxel's species quad k (species 4k..4k+3) in the field the cell pass works
 // on: the AoS view (one 48-byte record per voxel, quads 16 bytes apart) or,
 // with QUADS, the resident species quads in place (planes p.fplane floats
@@ -111,6 +124,70 @@ __device__ inline void gather_env6(const float* __restrict__ field, size_t v, ui
   for (int s = 0; s < MARL_S_EXT; ++s) E[s] = n ? acc[s] / (float)n : 0.0f;
 }
 
+// gather_env6 that also keeps the six face records (18 quads) for the
+// scatter: within a level no other cell touches these voxels, so the values
+// the scatter updates are exactly the ones gathered, and the scatter then
+// writes without reading the field again.
+template <bool QUADS>
+__device__ inline void gather_env6_keep(const float* __restrict__ field, size_t v, uint32_t m, const CellParams& p,
+                                        float* E, float4 (&keep)[18]) {
+  const size_t XY = (size_t)p.X * p.Y;
+  const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
+  float* f = const_cast<float*>(field);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const size_t w = ((m >> k) & 1u) ? (size_t)((int64_t)v + off[k]) : v;
+    keep[3 * k] = *field_quad<QUADS>(f, w, 0, p);
+    keep[3 * k + 1] = *field_quad<QUADS>(f, w, 1, p);
+    keep[3 * k + 2] = *field_quad<QUADS>(f, w, 2, p);
+  }
+  float acc[MARL_S_EXT];
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) acc[s] = 0.0f;
+  int n = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if ((m >> k) & 1u) {
+      ++n;
+      const float4 a = keep[3 * k], b = keep[3 * k + 1], cc = keep[3 * k + 2];
+      acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
+      acc[4] += b.x; acc[5] += b.y; acc[6] += b.z; acc[7] += b.w;
+      acc[8] += cc.x; acc[9] += cc.y; acc[10] += cc.z; acc[11] += cc.w;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) E[s] = n ? acc[s] / (float)n : 0.0f;
+}
+
+// scatter_deltas6 from the records gather_env6_keep kept (same arithmetic).
+template <bool QUADS>
+__device__ inline void scatter_from_kept(float* __restrict__ field, size_t v, uint32_t m, const CellParams& p,
+                                         const float* D, const float4 (&keep)[18]) {
+  const int n = __builtin_popcount(m);
+  if (n == 0) return;
+  const size_t XY = (size_t)p.X * p.Y;
+  const int64_t off[6] = {1, -1, (int64_t)p.X, -(int64_t)p.X, (int64_t)XY, -(int64_t)XY};
+  const float nf = (float)n;
+  float split[MARL_S_EXT];
+#pragma unroll
+  for (int s = 0; s < MARL_S_EXT; ++s) split[s] = D[s] / nf;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (!((m >> k) & 1u)) continue;
+    const size_t w = (size_t)((int64_t)v + off[k]);
+    float4 a = keep[3 * k], b = keep[3 * k + 1], cc = keep[3 * k + 2];
+    a.x = fmaxf(a.x + split[0], 0.0f); a.y = fmaxf(a.y + split[1], 0.0f);
+    a.z = fmaxf(a.z + split[2], 0.0f); a.w = fmaxf(a.w + split[3], 0.0f);
+    b.x = fmaxf(b.x + split[4], 0.0f); b.y = fmaxf(b.y + split[5], 0.0f);
+    b.z = fmaxf(b.z + split[6], 0.0f); b.w = fmaxf(b.w + split[7], 0.0f);
+    cc.x = fmaxf(cc.x + split[8], 0.0f); cc.y = fmaxf(cc.y + split[9], 0.0f);
+    cc.z = fmaxf(cc.z + split[10], 0.0f); cc.w = fmaxf(cc.w + split[11], 0.0f);
+    *field_quad<QUADS>(field, w, 0, p) = a;
+    *field_quad<QUADS>(field, w, 1, p) = b;
+    *field_quad<QUADS>(field, w, 2, p) = cc;
+  }
+}
+
 // Split deltas into the free face neighbours with the max(0) clamp
 // (spatial.rs:87-110, field.rs:57-62), lost when enclosed.  Each free face is
 // a different voxel, so the order of the updates does not matter.
@@ -144,21 +221,21 @@ __device__ inline void scatter_deltas6(float* __restrict__ field, size_t v, uint
 }
 
 // Phase 5 (cell.rs:332-366) from the post-effector energy.
-template <typename R>
-__device__ inline uint8_t cell_fate(marl_cell_state& c, const R& r, float energy, const CellParams& p) {
+// fate = (division, death, quiescence energies, division prep ticks).
+__device__ inline uint8_t cell_fate(marl_cell_state& c, const float4 fate, float energy, const CellParams& p) {
   c.age += 1;
-  const float eff_death = fmaxf(r.death_energy, p.hard_death_floor);
+  const float eff_death = fmaxf(fate.y, p.hard_death_floor);
   if (energy < eff_death) {
     c.prep_remaining = 0;
     return MARL_EVENT_DEATH;
   } else if (c.prep_remaining > 0) {
     c.prep_remaining -= 1;
     return c.prep_remaining == 0 ? MARL_EVENT_DIVISION : MARL_EVENT_NONE;
-  } else if (energy > r.division_energy) {
-    c.prep_remaining = f32_as_u16(fmaxf(r.division_prep_ticks, 1.0f));
+  } else if (energy > fate.x) {
+    c.prep_remaining = f32_as_u16(fmaxf(fate.w, 1.0f));
     c.quiescent = 0;
     return MARL_EVENT_NONE;
-  } else if (energy < r.quiescence_energy) {
+  } else if (energy < fate.z) {
     c.quiescent = 1;
     return MARL_EVENT_QUIESCENCE;
   }
@@ -211,7 +288,7 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const R& r, AI& I
   // transport (cell.rs:223-238)
 #pragma unroll
   for (int q = 0; q < MARL_S_TRANSPORTERS; ++q) {
-    const marl_transport tp = r.transport[q];
+    const marl_transport tp = r.transport(q);
     const int e = tp.ext_species, k = tp.int_species;
     if (e >= MARL_S_EXT || k >= MARL_M_INT) continue;
     const float ex = E.get(e);
@@ -226,7 +303,7 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const R& r, AI& I
   int active = 0;
 #pragma unroll
   for (int q = 0; q < MARL_R_MAX; ++q) {
-    const marl_reaction rx = r.reactions[q];
+    const marl_reaction rx = r.reaction(q);
     if (fabsf(rx.v_max) > p.active_reaction_threshold && rx.substrate != rx.product) ++active;
     if (fabsf(rx.v_max) < p.active_reaction_threshold) continue;
     if (rx.substrate == rx.product) continue;
@@ -253,7 +330,7 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const R& r, AI& I
   // maintenance (cell.rs:291-311)
   float pm = 1.0f;
   if (c.prep_remaining > 0) {
-    const float evolved = fmaxf(r.division_prep_ticks, 1.0f);
+    const float evolved = fmaxf(r.fate().w, 1.0f);
     const float rush = fmaxf(p.base_division_prep - evolved, 0.0f);
     pm = p.prep_maintenance_multiplier + rush * p.rush_penalty_rate;
   }
@@ -265,7 +342,7 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const R& r, AI& I
   if (!c.quiescent) {
 #pragma unroll
     for (int q = 0; q < MARL_S_EFFECTORS; ++q) {
-      const marl_effector ef = r.effectors[q];
+      const marl_effector ef = r.effector(q);
       const int k = ef.int_species, e = ef.ext_species;
       if (k >= MARL_M_INT || e >= MARL_S_EXT) continue;
       const float ik = I.get(k);

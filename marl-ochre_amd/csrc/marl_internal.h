@@ -263,6 +263,7 @@ struct marl_ctx {
   bool graph_auto = true;                    // budget follows the schedule's needs
   int cell_layout = MARL_CELL_LAYOUT_AUTO;    // field layout of the plain cell pass
   bool kin_quads = false;                    // the layout last chosen: quads in place
+  int kin_block = 128;                       // threads per block of the plain pass (64, 128 or 256)
   int relax_sweeps = 4;          // sweeps the last eager schedule needed to converge
   marl_sim_params last_sim{};
 

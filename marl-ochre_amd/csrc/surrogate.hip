@@ -183,12 +183,13 @@ __device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __rest
   if (!use_s || shadow) {
     PtrArr aI{I}, aD{D};
     const PtrArr aE{E};
-    cell_kinetics(c, r, aI, aE, aD, lt, p);
+    cell_kinetics(c, AosRules{&r}, aI, aE, aD, lt, p);
   }
   if (shadow) {
     record_error(Ip, I, sst);
     marl_cell_state cs = c, cp = c;
-    const uint8_t es = cell_fate(cs, r, Ip[0], p), ep = cell_fate(cp, r, I[0], p);
+    const float4 fate = AosRules{&r}.fate();
+    const uint8_t es = cell_fate(cs, fate, Ip[0], p), ep = cell_fate(cp, fate, I[0], p);
     if (es == ep && cs.prep_remaining == cp.prep_remaining && cs.quiescent == cp.quiescent)
       atomicAdd(&sst[ST_FATE_AGREE], 1u);
   }
@@ -199,7 +200,7 @@ __device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __rest
 #pragma unroll
     for (int e = 0; e < MARL_S_EXT; ++e) D[e] = Dp[e];
   }
-  const uint8_t ev = cell_fate(c, r, I[0], p);
+  const uint8_t ev = cell_fate(c, AosRules{&r}.fate(), I[0], p);
 #pragma unroll
   for (int k = 0; k < MARL_M_INT; ++k) c.internal[k] = I[k];
   cells[i] = c;

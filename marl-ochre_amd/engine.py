@@ -220,6 +220,24 @@ class MarlGpu:
         "quads" (resident quads in place) or "auto"; results are identical."""
         self._chk(self.lib.marl_set_cell_layout(self.h, self.CELL_LAYOUTS[layout]))
 
+    @property
+    def kinetics_block(self) -> int:
+        b = ctypes.c_int32()
+        self._chk(self.lib.marl_get_kinetics_block(self.h, ctypes.byref(b)))
+        return b.value
+
+    @kinetics_block.setter
+    def kinetics_block(self, block: int) -> None:
+        self._chk(self.lib.marl_set_kinetics_block(self.h, int(block)))
+
+    def autotune_kinetics(self, sim) -> int:
+        """marl_autotune_kinetics: the fastest plain-pass block size on the
+        resident state (left unchanged)."""
+        p = _params(sim)
+        b = ctypes.c_int32()
+        self._chk(self.lib.marl_autotune_kinetics(self.h, ctypes.byref(p), ctypes.byref(b)))
+        return b.value
+
     def set_graph_limits(self, sweeps: int = 0, levels: int = 0) -> None:
         """Schedule budget of the captured tick; (0, 0) = automatic (the default)."""
         self._chk(self.lib.marl_set_graph_limits(self.h, int(sweeps), int(levels)))

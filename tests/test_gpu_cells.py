@@ -92,10 +92,12 @@ def test_cell_pass_bit_identical(pkg, orc, layout):
     assert np.array_equal(f_g, f_w), np.count_nonzero(f_g != f_w)
 
 
-def _run_ticks(pkg, orc, dims, field, cells, rules, sim, n_ticks, check_every=1, layout="auto"):
+def _run_ticks(pkg, orc, dims, field, cells, rules, sim, n_ticks, check_every=1, layout="auto", block=None):
     stats_seen = {"divisions": 0, "deaths": 0}
     with pkg.MarlGpu(dims) as g:
         g.set_cell_layout(layout)
+        if block is not None:
+            g.kinetics_block = block
         g.upload_field_aos(field)
         g.upload_cells(cells, rules)
         f, c, r = field, cells, rules
@@ -125,6 +127,35 @@ def test_ticks_with_divisions_deaths_and_mutation_bit_identical(pkg, orc, layout
     sim = pkg.SimulationConfig(rng_seed=77, diffusion_substeps=4)
     seen = _run_ticks(pkg, orc, dims, field, cells, rules, sim, 6, layout=layout)
     assert seen["divisions"] > 50 and seen["deaths"] > 50, seen
+
+
+@pytest.mark.parametrize("block", [64, 256])
+@LAYOUTS
+def test_ticks_every_kinetics_block(pkg, orc, layout, block):
+    """The plain pass with each block size the kinetics autotuner chooses from."""
+    A = pkg._abi
+    dims = (64, 32, 24)
+    cells, rules = population(A, dims, 4000, 12)
+    field = np.random.default_rng(13).random(np.prod(dims) * 12).astype(F)
+    sim = pkg.SimulationConfig(rng_seed=78, diffusion_substeps=2)
+    _run_ticks(pkg, orc, dims, field, cells, rules, sim, 3, layout=layout, block=block)
+
+
+def test_kinetics_autotune_keeps_state(pkg):
+    A = pkg._abi
+    dims = (64, 48, 32)
+    cells, rules = population(A, dims, 12000, 14)
+    field = np.random.default_rng(15).random(np.prod(dims) * 12).astype(F)
+    with pkg.MarlGpu(dims) as g:
+        g.upload_field_aos(field)
+        g.upload_cells(cells, rules)
+        b = g.autotune_kinetics(pkg.SimulationConfig())
+        assert b in (64, 128, 256) and g.kinetics_block == b
+        assert np.array_equal(g.download_field_aos(), field)
+        c2, r2 = g.download_cells()
+        assert_cells_equal(c2, r2, cells, rules)
+        with pytest.raises(pkg.InvalidInput):
+            g.kinetics_block = 96
 
 
 @LAYOUTS

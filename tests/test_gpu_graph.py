@@ -12,11 +12,13 @@ pytestmark = pytest.mark.gpu
 F = np.float32
 
 
-def _compare_runs(pkg, dims, field, cells, rules, sim, n_ticks, limits=None, orc=None, layouts=("auto", "auto")):
+def _compare_runs(pkg, dims, field, cells, rules, sim, n_ticks, limits=None, orc=None, layouts=("auto", "auto"),
+                  blocks=(128, 128)):
     seen = {"divisions": 0, "deaths": 0, "host_finished": 0}
     with pkg.MarlGpu(dims) as ge, pkg.MarlGpu(dims) as gg:
         ge.set_cell_layout(layouts[0])
         gg.set_cell_layout(layouts[1])
+        ge.kinetics_block, gg.kinetics_block = blocks
         for g in (ge, gg):
             g.upload_field_aos(field)
             g.upload_cells(cells, rules)
@@ -55,6 +57,17 @@ def test_graph_tick_bit_identical_with_divisions_and_deaths(pkg, orc, layouts):
     seen = _compare_runs(pkg, dims, field, cells, rules, sim, 6, orc=orc, layouts=layouts)
     assert seen["divisions"] > 50 and seen["deaths"] > 50, seen
     assert seen["host_finished"] == 0
+
+
+@pytest.mark.parametrize("block", [64, 256])
+def test_graph_tick_every_kinetics_block(pkg, orc, block):
+    """The captured tick's level kernels with the other block sizes."""
+    A = pkg._abi
+    dims = (64, 32, 24)
+    cells, rules = population(A, dims, 4000, 22)
+    field = np.random.default_rng(23).random(np.prod(dims) * 12).astype(F)
+    sim = pkg.SimulationConfig(rng_seed=79, diffusion_substeps=2)
+    _compare_runs(pkg, dims, field, cells, rules, sim, 3, orc=orc, blocks=(128, block))
 
 
 def test_graph_tick_host_finishes_over_budget_ticks(pkg):

@@ -87,6 +87,17 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
     field, cells, rules, sim = scenarios.kinetics_at_scale(dims, 1_000_000, seed=0)
     n0 = len(cells)
     g = pkg.MarlGpu(dims)
+    # the kinetics autotuner on the resident state, then each block size's pass for the record
+    g.upload_field_aos(field)
+    g.upload_cells(cells, rules)
+    chosen = g.autotune_kinetics(sim)
+    by_block = {}
+    for b in (64, 128, 256):
+        g.kinetics_block = b
+        g.upload_field_aos(field)
+        g.upload_cells(cells, rules)
+        by_block[str(b)] = round(g.tick(sim, 0, 1)["ms_kinetics"], 4)
+    g.kinetics_block = chosen
     runs = []
     for rep in range(reps + 1):
         g.upload_field_aos(field)
@@ -151,6 +162,7 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
         "ms_breakdown": {k: round(sum(st[k] for _, st in runs) / reps, 3)
                          for k in ("ms_field", "ms_schedule", "ms_kinetics", "ms_fate")},
         "levels": runs[-1][1]["levels"],
+        "kinetics_autotune": {"block": chosen, "ms_kinetics_by_block": by_block},
         "events_first_tick": {k: runs[-1][1][k] for k in ("divisions", "deaths", "births_added", "n_cells")},
         "roofline_kinetics": {
             "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,

@@ -38,7 +38,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(field, occ, dims, sim, substeps=40):
+def cpu_baseline(field, occ, dims, sim, substeps=400):
     from oracle.binding import Oracle
 
     orc = Oracle()
@@ -56,7 +56,8 @@ def cpu_baseline(field, occ, dims, sim, substeps=40):
         "cores": threads,
         "kind": "port",
         "sample": f"{substeps} masked substeps of the 256x256x128x12 stress grid, f32 C oracle, "
-                  f"OpenMP over z-layers ({dt:.2f} s)",
+                  f"OpenMP over z-layers, dense bool occupancy as the reference's ({dt:.2f} s)",
+        "cpu_model": cpu_model(),
     }
 
 
@@ -67,6 +68,26 @@ def _records_equal(a, b):
         return a.shape == b.shape and a.tobytes() == b.tobytes()
     return all(_records_equal(a[k], b[k]) for k in a.dtype.names if not k.startswith("_"))
 
+
+def cpu_model() -> str:
+    """The host CPU's model name (the `lscpu` "Model name" field, read from /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+# The oracle's cell pass and light scan look occupancy up in dense arrays (a
+# voxel -> cell index map and a byte mask) where the reference probes its
+# HashMap<[u16; 3], usize> (crates/marl-field/src/light.rs:44,
+# crates/marl-sim/src/spatial.rs:18-43); the CPU figures below are therefore
+# those of a dense-array variant, faster than the reference's own lookups.
+DENSE_NOTE = "dense-array occupancy lookups in place of the reference's HashMap probes"
 
 KINETICS_BYTES_PER_CELL = 1350.0  # state r/w + ruleset + neighbour gather/RMW at 12% occupancy
 
@@ -188,7 +209,8 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
         res["cpu_baseline_full_tick"] = {
             "value": round(dt * 1e3, 1), "unit": "ms/tick", "cores": threads, "kind": "port",
             "sample": "the first full tick of the 1M-cell workload, f32 C oracle (diffusion OpenMP over z-layers; "
-                      "light, cell pass and fate on one thread, as the reference)",
+                      f"light, cell pass and fate on one thread, as the reference; {DENSE_NOTE})",
+            "cpu_model": cpu_model(),
             "gpu_tick_bit_identical": same}
         res["speedup_full_tick_vs_cpu"] = round(dt * 1e3 / tick_ms, 1)
         nv = dims[0] * dims[1] * dims[2]
@@ -200,12 +222,13 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
         res["cpu_baseline_cell_pass"] = {
             "value": round(sample / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
             "sample": f"sequential cell pass (env gather, CellState::tick, delta scatter) over the first {sample} "
-                      f"cells of the 1M-cell workload, one thread as in the reference ({dt:.2f} s)"}
+                      f"cells of the 1M-cell workload, one thread as in the reference, {DENSE_NOTE} ({dt:.2f} s)",
+            "cpu_model": cpu_model()}
     log("kinetics:", json.dumps(res))
     return res
 
 
-def column_bench(pkg, ticks=200, cpu_ticks=None, with_cpu=True):
+def column_bench(pkg, ticks=200, cpu_ticks=None, with_cpu=True, modes=("graph", "eager")):
     """The reference's default column run: 128x128x64, ~4,096 cells seeded in
     three depth bands (one metabolism each), internals U[0,0.1], 16 random
     reactions (v_max log-U[1e-3,1e-1], k_m log-U[1e-2,1]), linear-gradient
@@ -225,7 +248,7 @@ def column_bench(pkg, ticks=200, cpu_ticks=None, with_cpu=True):
     for name, starter in (("random_reactions", False), ("starter_states", True)):
         field, cells, rules, sim = scenarios.winogradsky_column(dims, starter_state=starter)
         row = {"cells_start": len(cells)}
-        for mode in ("graph", "eager"):
+        for mode in modes:
             g = pkg.MarlGpu(dims)
             g.upload_field_aos(field)
             g.upload_cells(cells, rules)
@@ -237,8 +260,10 @@ def column_bench(pkg, ticks=200, cpu_ticks=None, with_cpu=True):
             row[f"ms_per_tick_{mode}"] = round((time.perf_counter() - t0) * 1e3 / ticks, 4)
             row["cells_end"] = g.cell_count()
             g.close()
-        row["Gvoxel_species_substeps_per_s_graph"] = round(
-            dims[0] * dims[1] * dims[2] * 12 * sim.diffusion_substeps / (row["ms_per_tick_graph"] * 1e-3) / 1e9, 1)
+        ref_mode = modes[0]
+        row[f"Gvoxel_species_substeps_per_s_{ref_mode}"] = round(
+            dims[0] * dims[1] * dims[2] * 12 * sim.diffusion_substeps / (row[f"ms_per_tick_{ref_mode}"] * 1e-3) / 1e9,
+            1)
         res[name] = row
     if with_cpu:
         from oracle.binding import Oracle
@@ -254,7 +279,7 @@ def column_bench(pkg, ticks=200, cpu_ticks=None, with_cpu=True):
         g = pkg.MarlGpu(dims)
         g.upload_field_aos(field)
         g.upload_cells(cells, rules)
-        g.tick_graph(sim, 0, cpu_ticks)
+        (g.tick_graph if modes[0] == "graph" else g.tick)(sim, 0, cpu_ticks)
         c_g, _ = g.download_cells()
         same = bool(np.array_equal(g.download_field_aos().view(np.uint32), f.view(np.uint32))
                     and len(c_g) == len(c)
@@ -263,8 +288,11 @@ def column_bench(pkg, ticks=200, cpu_ticks=None, with_cpu=True):
         g.close()
         res["cpu_baseline"] = {
             "value": round(dt * 1e3 / cpu_ticks, 3), "unit": "ms/tick", "cores": threads, "kind": "port",
-            "sample": f"the first {cpu_ticks} ticks of the random-reaction run, f32 C oracle ({dt:.2f} s)"}
-        res["speedup_vs_cpu"] = round(res["cpu_baseline"]["value"] / res["random_reactions"]["ms_per_tick_graph"], 1)
+            "sample": f"the first {cpu_ticks} ticks of the random-reaction run, f32 C oracle, {DENSE_NOTE} "
+                      f"({dt:.2f} s)",
+            "cpu_model": cpu_model()}
+        res["speedup_vs_cpu"] = round(res["cpu_baseline"]["value"] / res["random_reactions"][f"ms_per_tick_{modes[0]}"],
+                                      1)
         res["gpu_bit_identical_to_oracle"] = {"tick": cpu_ticks, "field_and_cells": same}
     log("column:", json.dumps(res))
     return res
@@ -585,6 +613,8 @@ def main():
                     help="full ticks of the 1M-cell kinetics workload (0 = skip)")
     ap.add_argument("--column-ticks", type=int, default=200,
                     help="ticks of the default 128x128x64 column run (0 = skip)")
+    ap.add_argument("--column-modes", default="graph,eager",
+                    help="tick paths of the column run: graph (marl_tick_graph), eager (marl_tick)")
     ap.add_argument("--ensemble-ticks", type=int, default=5,
                     help="ticks of the 32-column ensemble leg (0 = skip)")
     ap.add_argument("--drift-ticks", type=int, default=0,
@@ -716,7 +746,8 @@ def main():
     if args.kinetics_ticks > 0:
         out["kinetics"] = kinetics_bench(pkg, args.kinetics_ticks, not args.no_cpu)
     if args.column_ticks > 0:
-        out["column"] = column_bench(pkg, args.column_ticks, with_cpu=not args.no_cpu)
+        out["column"] = column_bench(pkg, args.column_ticks, with_cpu=not args.no_cpu,
+                                     modes=tuple(m for m in args.column_modes.split(",") if m))
     if args.ensemble_ticks > 0:
         out["ensemble"] = ensemble_bench(pkg, args.ensemble_ticks)
     if not args.no_dropin:

@@ -16,7 +16,12 @@
  *     marl_last_error() returns a message;
  *   - host buffers are owned by the caller, device memory by the context;
  *   - one host thread per context; calls block until the GPU work is done
- *     unless the name says otherwise (the `_async` timing helpers).
+ *     unless the name says otherwise (the `_async` timing helpers);
+ *   - for tests of a caller's fallback, the environment variable
+ *     MARL_HIP_FAULT=init makes marl_create fail with MARL_ERR_DEVICE_INIT,
+ *     and MARL_HIP_FAULT=tick:<n> makes the n-th marl_diffuse_tick or
+ *     marl_tick call of each context fail with MARL_ERR_TRANSFER before it
+ *     changes any state (read at every call).
  *
  * Layouts at the boundary
  *   - field AoS: ((z*Y + y)*X + x)*12 + s, f32 (crates/marl-field/src/field.rs:33-35);
@@ -237,6 +242,15 @@ int marl_cell_pass(marl_ctx* ctx, const marl_sim_params* sim, uint8_t* events, s
  * voxels, out[24..39] cell internals (no reference counterpart: the reference
  * keeps no mass budget, docs/INFO.md:276). */
 int marl_mass_budget(marl_ctx* ctx, double* out, size_t n_out);
+/* Per-phase mass budget of the host-driven tick (no reference counterpart):
+ * with it on, every marl_tick records marl_mass_budget's 40 totals at four
+ * points of the tick — after the occupancy rebuild (before the sources), after
+ * the field phase (sources, substeps, light), after the cell pass, and after
+ * fate and compaction (against the new occupancy) — so that the matter each
+ * phase adds or removes can be read off.  Costs four FP64 reductions a tick. */
+int marl_set_phase_budget(marl_ctx* ctx, int32_t on);
+/* The last such tick's budgets, out[phase*40 + k] (n_out >= 160). */
+int marl_tick_phase_budget(marl_ctx* ctx, double* out, size_t n_out);
 
 /* ---- setup ----------------------------------------------------------------- */
 

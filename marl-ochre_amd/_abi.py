@@ -229,6 +229,8 @@ SIGNATURES = {
     "marl_set_kinetics_block": (ctypes.c_int, [_P, _I32]),
     "marl_get_kinetics_block": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
     "marl_autotune_kinetics": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), ctypes.POINTER(ctypes.c_int32)]),
+    "marl_set_phase_budget": (ctypes.c_int, [_P, _I32]),
+    "marl_tick_phase_budget": (ctypes.c_int, [_P, _P, _SZ]),
     "marl_set_graph_limits": (ctypes.c_int, [_P, _I32, _I32]),
     "marl_set_cell_layout": (ctypes.c_int, [_P, _I32]),
     "marl_cell_pass": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _P, _SZ, ctypes.POINTER(_I32)]),

@@ -938,6 +938,12 @@ __global__ void k_g_finish(TickDev* ts) {
 
 constexpr int kBudgetBlocks = 512;
 
+// The 40 FP64 totals of the resident state (empty-voxel field, occupied-voxel
+// field, cell internals; occupancy from ctx->occ) reduced into `dst` (40
+// doubles), or into the budget buffer's final slot when dst is null.  Fill
+// kernels clear the partials, so the sequence can run inside a tick.
+int enqueue_budget(marl_ctx* ctx, double* dst);
+
 inline unsigned grid1(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 // ---------------------------------------------------------------------------
@@ -978,6 +984,20 @@ int fill_i32(marl_ctx* ctx, int32_t* a, int32_t v, size_t n) {
 int d2h_sync(marl_ctx* ctx, void* host, const void* dev, size_t bytes) {
   MARL_HIP(ctx, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
   MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MARL_OK;
+}
+
+// Every block writes every one of its 40 partial slots, so nothing needs
+// clearing between reductions.
+int enqueue_budget(marl_ctx* ctx, double* dst) {
+  if (!ctx->budget) MARL_HIP(ctx, hipMalloc(&ctx->budget, sizeof(double) * (kBudgetBlocks * kBudget + kBudget)));
+  hipLaunchKernelGGL(k_budget_field, dim3(kBudgetBlocks), dim3(256), 0, ctx->stream, ctx->field[ctx->cur], ctx->occ,
+                     ctx->nvox, ctx->budget);
+  hipLaunchKernelGGL(k_budget_cells, dim3(kBudgetBlocks), dim3(256), 0, ctx->stream, ctx->cells[ctx->cbuf],
+                     (int)ctx->n_cells, ctx->budget);
+  hipLaunchKernelGGL(k_budget_final, dim3(1), dim3(64), 0, ctx->stream, ctx->budget, kBudgetBlocks,
+                     dst ? dst : ctx->budget + kBudgetBlocks * kBudget);
+  MARL_HIP(ctx, hipGetLastError());
   return MARL_OK;
 }
 
@@ -1747,8 +1767,11 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
   if (!ctx) return MARL_ERR_INVALID_INPUT;
   if (int rc = marl_check_params(ctx, sim)) return rc;
   if (n_ticks < 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "negative tick count");
+  if (int rc = marl_fault_tick(ctx)) return rc;
   ctx->last_sim = *sim;
   if (int rc = ensure_voxel_buffers(ctx)) return rc;
+  if (ctx->phase_budget && !ctx->phase_budget_dev)
+    MARL_HIP(ctx, hipMalloc(&ctx->phase_budget_dev, sizeof(double) * 4 * kBudget));
   hipEvent_t ev[5] = {};
   for (auto& e : ev) MARL_HIP(ctx, hipEventCreate(&e));
   struct Guard {
@@ -1768,13 +1791,23 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
     // first substep (field.rs:231-250 then :212-217); light (lib.rs:199),
     // fused into the last substep where the tile allows
     if (int rc = marl_launch_occupancy_from_cells(ctx)) return rc;
+    if (ctx->phase_budget)
+      if (int rc = enqueue_budget(ctx, ctx->phase_budget_dev)) return rc;
     if (int rc = marl_launch_field_phase(ctx, sim, true)) return rc;
+    if (ctx->phase_budget)
+      if (int rc = enqueue_budget(ctx, ctx->phase_budget_dev + kBudget)) return rc;
     MARL_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
     int32_t levels = 0;
     if (int rc = run_cell_pass(ctx, p, &levels, ev[2])) return rc;
     if (ctx->n_cells == 0) MARL_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+    if (ctx->phase_budget)
+      if (int rc = enqueue_budget(ctx, ctx->phase_budget_dev + 2 * kBudget)) return rc;
     MARL_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
     if (int rc = run_fate(ctx, p, &st)) return rc;
+    if (ctx->phase_budget) {  // against the post-compaction occupancy
+      if (int rc = marl_launch_occupancy_from_cells(ctx)) return rc;
+      if (int rc = enqueue_budget(ctx, ctx->phase_budget_dev + 3 * kBudget)) return rc;
+    }
     MARL_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
     MARL_HIP(ctx, hipEventSynchronize(ev[4]));
     st.levels = levels;
@@ -1976,19 +2009,25 @@ int marl_seed_positions(const marl_dims* d, uint64_t seed, uint64_t band, int64_
 int marl_mass_budget(marl_ctx* ctx, double* out, size_t n_out) {
   if (!ctx || !out) return MARL_ERR_INVALID_INPUT;
   if (n_out < (size_t)kBudget) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "mass budget needs 40 doubles");
-  if (!ctx->budget) MARL_HIP(ctx, hipMalloc(&ctx->budget, sizeof(double) * (kBudgetBlocks * kBudget + kBudget)));
-  MARL_HIP(ctx, hipMemsetAsync(ctx->budget, 0, sizeof(double) * kBudgetBlocks * kBudget, ctx->stream));
-  hipLaunchKernelGGL(k_budget_field, dim3(kBudgetBlocks), dim3(256), 0, ctx->stream, ctx->field[ctx->cur], ctx->occ,
-                     ctx->nvox, ctx->budget);
-  MARL_HIP(ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_budget_cells, dim3(kBudgetBlocks), dim3(256), 0, ctx->stream, ctx->cells[ctx->cbuf],
-                     (int)ctx->n_cells, ctx->budget);
-  MARL_HIP(ctx, hipGetLastError());
-  double* fin = ctx->budget + kBudgetBlocks * kBudget;
-  hipLaunchKernelGGL(k_budget_final, dim3(1), dim3(64), 0, ctx->stream, ctx->budget, kBudgetBlocks, fin);
-  MARL_HIP(ctx, hipGetLastError());
+  if (int rc = enqueue_budget(ctx, nullptr)) return rc;
   double tmp[kBudget];
-  if (int rc = d2h_sync(ctx, tmp, fin, sizeof(tmp))) return rc;
+  if (int rc = d2h_sync(ctx, tmp, ctx->budget + kBudgetBlocks * kBudget, sizeof(tmp))) return rc;
+  memcpy(out, tmp, sizeof(tmp));
+  return MARL_OK;
+}
+
+int marl_set_phase_budget(marl_ctx* ctx, int32_t on) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  ctx->phase_budget = on != 0;
+  return MARL_OK;
+}
+
+int marl_tick_phase_budget(marl_ctx* ctx, double* out, size_t n_out) {
+  if (!ctx || !out) return MARL_ERR_INVALID_INPUT;
+  if (n_out < 4 * (size_t)kBudget) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "phase budget needs 160 doubles");
+  if (!ctx->phase_budget_dev) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "no tick ran with the phase budget on");
+  double tmp[4 * kBudget];
+  if (int rc = d2h_sync(ctx, tmp, ctx->phase_budget_dev, sizeof(tmp))) return rc;
   memcpy(out, tmp, sizeof(tmp));
   return MARL_OK;
 }

@@ -2,6 +2,7 @@
 //
 // Reference seam: crates/marl-gpu/src/field_diffusion.rs:31-268 (GpuFieldDiffuser)
 // and crates/marl-gpu/src/context.rs:4-54 (GpuError, GpuContext).
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -47,6 +48,34 @@ int marl_check_params(marl_ctx* ctx, const marl_sim_params* sim) {
   return MARL_OK;
 }
 
+namespace {
+// Why the calling thread's last marl_create failed (there is no context to
+// hold the message).
+thread_local std::string t_create_err;
+}  // namespace
+
+// Fault injection for tests of a caller's fallback path (the reference's
+// engine falls back to CPU diffusion when the GPU seam fails,
+// crates/marl-sim/src/lib.rs:133-146, 180-191).  MARL_HIP_FAULT=init makes
+// marl_create fail with DEVICE_INIT before it touches a device;
+// MARL_HIP_FAULT=tick:<n> makes the n-th marl_diffuse_tick or marl_tick call
+// of each context fail with TRANSFER before it changes any state, so the
+// strong guarantee (host buffers and resident state untouched) can be tested.
+// The variable is read at every call.
+bool marl_fault_init() {
+  const char* e = getenv("MARL_HIP_FAULT");
+  return e && strcmp(e, "init") == 0;
+}
+
+int marl_fault_tick(marl_ctx* ctx) {
+  const char* e = getenv("MARL_HIP_FAULT");
+  if (!e || strncmp(e, "tick:", 5) != 0) return MARL_OK;
+  const long n = strtol(e + 5, nullptr, 10);
+  if (n > 0 && ++ctx->fault_calls == n)
+    return marl_fail(ctx, MARL_ERR_TRANSFER, "injected fault (MARL_HIP_FAULT=" + std::string(e) + ")");
+  return MARL_OK;
+}
+
 extern "C" {
 
 const char* marl_version(void) { return MARL_VERSION; }
@@ -57,11 +86,6 @@ int marl_device_count(void) {
   return n;
 }
 
-namespace {
-// Why the calling thread's last marl_create failed (there is no context to
-// hold the message).
-thread_local std::string t_create_err;
-}  // namespace
 
 const char* marl_last_error(const marl_ctx* ctx) {
   if (ctx) return ctx->err.c_str();
@@ -84,6 +108,10 @@ int marl_create(const marl_dims* dims, marl_ctx** out) {
   if ((uint64_t)dims->x * dims->y * dims->z * MARL_S_EXT * 4ull > 0x7fffffffull) {
     t_create_err = "marl_create: one field buffer would exceed 2 GiB";
     return MARL_ERR_INVALID_INPUT;
+  }
+  if (marl_fault_init()) {
+    t_create_err = "marl_create: injected fault (MARL_HIP_FAULT=init)";
+    return MARL_ERR_DEVICE_INIT;
   }
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
@@ -135,7 +163,7 @@ void marl_destroy(marl_ctx* c) {
   void* bufs[] = {c->field[0], c->field[1], c->occ, c->light, c->staging, c->occ_bytes, c->cells[0], c->cells[1],
                   c->rules[0], c->rules[1], c->cell_at, c->nbmask, c->level, c->order, c->events, c->flag_scan,
                   c->flag_scan2, c->misc, c->level_count, c->claim, c->birth_cells, c->birth_rules,
-                  c->birth_flag, c->scan_tmp, c->budget, c->preds, c->npred};
+                  c->birth_flag, c->scan_tmp, c->budget, c->preds, c->npred, c->phase_budget_dev};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->pinned) (void)hipHostFree(c->pinned);
@@ -273,6 +301,7 @@ int marl_diffuse_tick(marl_ctx* ctx, float* aos, size_t n_floats, const uint8_t*
   if (int rc = check_len(ctx, n_voxels, ctx->nvox, "occupancy")) return rc;    // :170-175
   if (int rc = marl_check_params(ctx, sim)) return rc;
   if (sim->diffusion_substeps == 0) return MARL_OK;                             // :176-178
+  if (int rc = marl_fault_tick(ctx)) return rc;
   if (int rc = marl_ensure_staging(ctx)) return rc;
   MARL_HIP(ctx, hipMemcpyAsync(ctx->staging, aos, n_floats * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   MARL_HIP(ctx, hipMemcpyAsync(ctx->occ_bytes, occ, n_voxels, hipMemcpyHostToDevice, ctx->stream));

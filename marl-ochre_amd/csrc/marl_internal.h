@@ -264,6 +264,9 @@ struct marl_ctx {
   int cell_layout = MARL_CELL_LAYOUT_AUTO;    // field layout of the plain cell pass
   bool kin_quads = false;                    // the layout last chosen: quads in place
   int kin_block = 128;                       // threads per block of the plain pass (64, 128 or 256)
+  long fault_calls = 0;                      // drop-in / tick calls counted for MARL_HIP_FAULT=tick:<n>
+  bool phase_budget = false;                 // FP64 mass budget at the tick's phase boundaries
+  double* phase_budget_dev = nullptr;        // [4][40] of the last tick
   int relax_sweeps = 4;          // sweeps the last eager schedule needed to converge
   marl_sim_params last_sim{};
 
@@ -286,6 +289,9 @@ constexpr size_t kNoAllocScratchBytes = 72 * 1024;
 // Looks up the scratch array's address on `device` (which must be current);
 // called once per device by marl_create.
 int marl_noalloc_init_device(int device);
+// MARL_HIP_FAULT hooks (ctx.hip).
+bool marl_fault_init();
+int marl_fault_tick(marl_ctx* ctx);
 
 StencilParams marl_stencil_params(const marl_sim_params* sim, int apply_sources);
 int marl_check_params(marl_ctx* ctx, const marl_sim_params* sim);

@@ -42,7 +42,9 @@ class MarlGpu:
                 _abi.ERR_INVALID_INPUT: f"unsupported grid {self.dims} (x must be a multiple of 32, "
                                         f"field under 2 GiB)",
             }
-            check(rc, what=msgs.get(rc, "failed to request GPU device"))
+            why = self.lib.marl_last_error(None)
+            what = msgs.get(rc, "failed to request GPU device")
+            check(rc, what=what + (": " + why.decode(errors="replace") if why else ""))
         self.h = h
 
     # -- lifecycle --------------------------------------------------------
@@ -256,6 +258,16 @@ class MarlGpu:
         return out
 
     # -- outputs ------------------------------------------------------------------
+    def set_phase_budget(self, on: bool = True) -> None:
+        self._chk(self.lib.marl_set_phase_budget(self.h, 1 if on else 0))
+
+    def tick_phase_budget(self) -> np.ndarray:
+        """[4, 40] FP64 budgets of the last tick: after the occupancy rebuild,
+        the field phase, the cell pass, and fate with compaction."""
+        out = np.zeros((4, 40), np.float64)
+        self._chk(self.lib.marl_tick_phase_budget(self.h, ptr(out), out.size))
+        return out
+
     def cell_summary(self) -> tuple[dict, np.ndarray]:
         """Population sums (FP64) and z-thirds, plus cells per z-layer."""
         st = _abi.marl_population_summary()

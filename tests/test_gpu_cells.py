@@ -224,3 +224,43 @@ def test_mass_budget_matches_numpy_f64(pkg):
     assert np.allclose(mb[:12], f[~occ].sum(0), rtol=1e-12)
     assert np.allclose(mb[12:24], f[occ].sum(0), rtol=1e-12)
     assert np.allclose(mb[24:40], cells["internal"].astype(np.float64).sum(0), rtol=1e-12)
+
+
+def test_phase_budget_matches_numpy_f64_at_each_phase(pkg, orc):
+    """marl_tick's per-phase FP64 budgets against numpy FP64 sums of the
+    oracle's state at the same points of the tick: after the occupancy
+    rebuild, after the field phase, after the cell pass, after fate."""
+    A = pkg._abi
+    dims = (64, 32, 16)
+    cells, rules = population(A, dims, 3000, 31)
+    rng = np.random.default_rng(32)
+    field = rng.random(np.prod(dims) * 12).astype(F)
+    sim = pkg.SimulationConfig(rng_seed=33, diffusion_substeps=2)
+
+    def budget(f, c):
+        occ = np.zeros(np.prod(dims), bool)
+        for x, y, z in c["pos"].astype(np.int64):
+            occ[(z * dims[1] + y) * dims[0] + x] = True
+        f = f.reshape(-1, 12).astype(np.float64)
+        return np.concatenate([f[~occ].sum(0), f[occ].sum(0), c["internal"].astype(np.float64).sum(0)]), occ
+
+    b0, occ = budget(field, cells)
+    f1 = orc.diffuse_tick(orc.boundary(field, dims, sim), occ.astype(np.uint8), dims, sim)
+    lt = orc.light(f1, occ.astype(np.uint8), dims, sim)
+    b1, _ = budget(f1, cells)
+    f2, c2, _ = orc.cell_pass(dims, sim, f1, lt, cells, rules)
+    b2, _ = budget(f2, c2)
+    f3, _, c3, _, _, st = orc.tick(dims, sim, 0, field, cells, rules)
+    b3, _ = budget(f3, c3)
+    assert st["divisions"] + st["deaths"] > 0
+    with pkg.MarlGpu(dims) as g:
+        g.upload_field_aos(field)
+        g.upload_cells(cells, rules)
+        g.set_phase_budget(True)
+        g.tick(sim, 0, 1)
+        got = g.tick_phase_budget()
+    for k, want in enumerate((b0, b1, b2, b3)):
+        assert np.allclose(got[k], want, rtol=1e-12, atol=1e-9), (k, np.abs(got[k] - want).max())
+    # the field phase adds the boundary sources: species 1, 2 and 3 gain, the
+    # cells' internals are untouched; the cell pass moves matter, fate removes it
+    assert np.array_equal(got[1, 24:], got[0, 24:])

@@ -513,40 +513,62 @@ SURR_FLOP_PER_CELL = 2.0 * (80 * 256 + 256 * 256 * 2 + 256 * 28)  # forward mult
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3):
-    """The learned kinetics surrogate on the 1M-cell workload with the reaction
-    wiring shared per metabolism (three topologies; rates per cell).
+def _shadow_summary(sh):
+    """Shadow statistics: the within-1e-3 share (all of internals 0-14, and per
+    species), the median bin of the per-cell max relative error, fate agreement
+    and the fate event rates under the exact update and under the prediction."""
+    n = max(sh["shadow"], 1)
+    hist = sh["err_hist"]
+    med = next((b for b in range(24) if sum(hist[:b + 1]) * 2 >= n), 23)
+    ev = ("none", "division", "death", "quiescence")
+    return {
+        "cells": sh["shadow"],
+        "within_1e-3_rel": round(sh["shadow_within_1e3"] / n, 5),
+        "within_1e-3_rel_per_species": [round(w / n, 4) for w in sh["shadow_within_1e3_species"][:15]],
+        "median_max_rel_err_bin": [10 ** (med / 2 - 9), 10 ** ((med + 1) / 2 - 9)],
+        "max_rel_err": sh["shadow_max_rel_err"],
+        "fate_agreement": round(sh["shadow_fate_agree"] / n, 5),
+        "fate_rate_exact": {e: round(v / n, 5) for e, v in zip(ev, sh["fate_exact"])},
+        "fate_rate_surrogate": {e: round(v / n, 5) for e, v in zip(ev, sh["fate_surrogate"])},
+        "err_hist_half_decades_from_1e-9": hist,
+    }
 
-    Pairs are harvested from `harvest_ticks` plain ticks of the trajectory, the
-    network is trained on the GPU (batch 8192), then the first tick is run
-    `reps` times from the same state in plain mode and in surrogate mode
-    (kinetics launches timed with hipEvents), once in shadow mode (plain
-    authoritative, every eligible cell also predicted) for the error
-    distribution, and the fused inference kernel is timed alone on 1M rows."""
+
+def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3, pair_target=10_000_000):
+    """The learned kinetics surrogate on the 1M-cell workload, every cell with
+    its own random reaction wiring.
+
+    Pairs (explicit-step features, exact-minus-explicit residuals) are harvested
+    from plain ticks of the trajectory until `pair_target` pairs or
+    `harvest_ticks` ticks.  For each training-set size (1M, 3M and all pairs)
+    the network is trained from scratch on the GPU (batch 8192) and the first
+    tick is run in shadow mode (plain authoritative, every eligible cell also
+    predicted) for its error distribution; the explicit step alone (residual 0)
+    is the curve's zero point.  With the network trained on all pairs the
+    first tick is timed `reps` times in plain and in surrogate mode (kinetics
+    launches, hipEvents), and the fused inference kernel alone on 1M rows."""
     from marl_ochre_amd import scenarios
 
     A = pkg._abi
     dims = (256, 256, 128)
-    log("generating 1M-cell surrogate workload")
-    field, cells, rules, sim = scenarios.kinetics_at_scale(dims, 1_000_000, seed=0, shared_wiring=True)
+    log("generating 1M-cell surrogate workload (per-cell wiring)")
+    field, cells, rules, sim = scenarios.kinetics_at_scale(dims, 1_000_000, seed=0)
     n0 = len(cells)
     g = pkg.MarlGpu(dims)
     try:
         g.upload_field_aos(field)
         g.upload_cells(cells, rules)
-        g.harvest_begin(harvest_ticks * n0 + 1, 1.0)
+        g.harvest_begin(pair_target, 1.0)   # pairs beyond the capacity are not kept
         t0 = time.perf_counter()
-        for t in range(harvest_ticks):
-            g.tick(sim, t, 1)
+        ticks = harvest_ticks
+        g.tick(sim, 0, ticks)
         n_pairs = g.harvest_end()
         harvest_s = time.perf_counter() - t0
-        log(f"harvested {n_pairs} pairs in {harvest_s:.2f} s; training {train_steps} steps")
-        rep = g.surrogate_train(steps=train_steps, batch=8192, lr=2e-3, lr_final=1e-5, seed=1, reinit=True)
-        log("train:", json.dumps(rep))
+        log(f"harvested {n_pairs} pairs in {ticks} ticks ({harvest_s:.2f} s)")
 
         def first_tick(mode, shadow_rate=0.0):
             out = []
-            for r in range(reps + 1):
+            for r in range(reps + 1 if mode != A.KINETICS_SHADOW else 1):
                 g.set_kinetics_mode(A.KINETICS_PLAIN)
                 g.upload_field_aos(field)
                 g.upload_cells(cells, rules)
@@ -555,28 +577,43 @@ def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3):
                     g.set_kinetics_mode(mode)
                 g.surrogate_reset_stats()
                 st = g.tick(sim, 0, 1)
-                if r > 0:
+                if r > 0 or mode == A.KINETICS_SHADOW:
                     out.append(st)
             stats = g.surrogate_stats()
             g.set_kinetics_mode(A.KINETICS_PLAIN)
             return out, stats
 
+        curve = []
+        for size in [m for m in (1_000_000, 3_000_000) if m < n_pairs] + [n_pairs]:
+            rep = g.surrogate_train(steps=train_steps, batch=8192, lr=2e-3, lr_final=1e-5, seed=1, reinit=True,
+                                    max_pairs=size)
+            _, sh = first_tick(A.KINETICS_SHADOW)
+            curve.append({"train_pairs": rep["pairs"], "val_loss": round(rep["val_loss"], 5),
+                          "train_ms": round(rep["ms"], 1), "shadow": _shadow_summary(sh)})
+            log("surrogate curve point:", json.dumps(curve[-1]["shadow"])[:300])
+        # the explicit step alone: last layer and target means zeroed, so the residual is 0
+        w, norm = g.surrogate_get_params()
+        w0, nm0 = w.copy(), norm.copy()
+        w0[-(256 * 28 + 28):] = 0.0
+        nm0[160:188] = 0.0
+        g.surrogate_set_params(w0, nm0)
+        _, sh0 = first_tick(A.KINETICS_SHADOW)
+        g.surrogate_set_params(w, norm)
         plain, _ = first_tick(A.KINETICS_PLAIN)
         surr, surr_stats = first_tick(A.KINETICS_SURROGATE)
-        shadow, shadow_stats = first_tick(A.KINETICS_SHADOW)
         infer_ms = g.surrogate_time_infer(n0, 10)
     finally:
         g.close()
-    kin_plain = sum(s["ms_kinetics"] for s in plain) / reps
-    kin_surr = sum(s["ms_kinetics"] for s in surr) / reps
-    sh = shadow_stats
-    hist = sh["err_hist"]
-    med_bin = next(b for b in range(24) if sum(hist[:b + 1]) * 2 >= max(sh["shadow"], 1))
+    kin_plain = sum(st["ms_kinetics"] for st in plain) / reps
+    kin_surr = sum(st["ms_kinetics"] for st in surr) / reps
     res = {
-        "workload": "surrogate: 256x256x128, 1,000,000 cells, 16 reactions with wiring drawn once per metabolism "
-                    "(3 topologies), v_max/k_m per cell log-uniform [1e-3,1]; first full tick",
-        "pairs_harvested": n_pairs, "harvest_ticks": harvest_ticks, "train": rep,
-        "train_pairs_per_s": round(train_steps * 8192 / (rep["ms"] * 1e-3), 0),
+        "workload": "surrogate: 256x256x128, 1,000,000 cells, 16 random reactions per cell with its own wiring, "
+                    "v_max/k_m log-uniform [1e-3,1], p=0.2 light-catalysed; first full tick",
+        "pairs_harvested": n_pairs, "harvest_ticks": ticks,
+        "train": {k: curve[-1][k] for k in ("train_pairs", "val_loss", "train_ms")},
+        "train_pairs_per_s": round(train_steps * 8192 / (curve[-1]["train_ms"] * 1e-3), 0),
+        "error_vs_training_pairs": [{"train_pairs": 0, "note": "explicit step alone", "shadow": _shadow_summary(sh0)}]
+        + curve,
         "Mcell_updates_per_s_plain": round(n0 / (kin_plain * 1e-3) / 1e6, 3),
         "Mcell_updates_per_s_surrogate": round(n0 / (kin_surr * 1e-3) / 1e6, 3),
         "speedup_vs_plain": round(kin_plain / kin_surr, 3),
@@ -585,11 +622,6 @@ def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3):
         "fallbacks": {k: surr_stats[k] for k in ("fallback_topology", "fallback_ood", "fallback_margin")},
         "fate_events_plain": {k: plain[-1][k] for k in ("divisions", "deaths", "births_added", "n_cells")},
         "fate_events_surrogate": {k: surr[-1][k] for k in ("divisions", "deaths", "births_added", "n_cells")},
-        "shadow": {"cells": sh["shadow"],
-                   "within_1e-3_rel": round(sh["shadow_within_1e3"] / max(sh["shadow"], 1), 5),
-                   "fate_agreement": round(sh["shadow_fate_agree"] / max(sh["shadow"], 1), 5),
-                   "median_max_rel_err_bin": [10 ** (med_bin / 2 - 9), 10 ** ((med_bin + 1) / 2 - 9)],
-                   "max_rel_err": sh["shadow_max_rel_err"], "err_hist_half_decades_from_1e-9": hist},
         "infer_kernel": {"rows": n0, "ms": round(infer_ms, 4),
                          "TFLOPs": round(SURR_FLOP_PER_CELL * n0 / (infer_ms * 1e-3) / 1e12, 1),
                          "mfma_frac": round(SURR_FLOP_PER_CELL * n0 / (infer_ms * 1e-3) / 1e12

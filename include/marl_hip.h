@@ -379,10 +379,18 @@ int marl_ensemble_launch_shape(const marl_ensemble* e, int32_t* tile, int32_t* g
 /* No reference counterpart: the reference integrates every cell's kinetics
  * exactly (crates/marl-cell/src/cell.rs:223-329).  The surrogate is an MLP
  * (80 -> 256 -> 256 -> 256 -> 28, SiLU) trained on the GPU from pairs the plain
- * kinetics pass harvests, then used for cells whose reaction topology is one
- * of the trained classes, whose standardised features lie within zmax and
- * whose predicted energy is at least energy_margin from every fate
- * threshold; all other cells run the plain path. */
+ * kinetics pass harvests.  It predicts the residual between the exact update
+ * and one explicit step of the same rate laws from the pre-tick state, routed
+ * through the cell's own wiring; the explicit step and its per-reaction flux
+ * ratios are among the inputs, so a cell of any wiring is eligible.  A cell
+ * uses the prediction when its standardised features lie within zmax and its
+ * predicted energy is at least energy_margin from every fate threshold; all
+ * other cells run the plain path.
+ * Features (80): internal[0..14], quiescent, environment[12], light, in-prep,
+ * explicit internal deltas [0..14], explicit exchange fluxes [12], each
+ * reaction's explicit flux / (its substrate + 1e-6) [16], 7 zeros.
+ * Targets (28): exact minus explicit internal deltas [16] (slot 15 is 0: the
+ * tick sets internal 15 to the light) and exchange fluxes [12]. */
 
 typedef struct {
   int64_t steps;       /* Adam steps */
@@ -392,19 +400,19 @@ typedef struct {
   float zmax;          /* out-of-distribution bound (<= 0: 6) */
   float energy_margin; /* fate margin on the predicted energy (< 0: 0.05) */
   uint64_t seed;
+  int64_t max_pairs;   /* train on the first max_pairs harvested pairs (<= 0: all) */
 } marl_train_config;
 
 typedef struct {
   float loss_first, loss_last, val_loss; /* MSE on standardised targets */
   float ms;                              /* wall time of the call */
   int64_t pairs, pairs_train;
-  int32_t classes, _pad;
 } marl_train_report;
 
 typedef struct {
   int64_t cells;              /* cells that reached the surrogate pass */
   int64_t surrogate;          /* advanced by the surrogate */
-  int64_t fallback_topology;  /* unknown reaction topology */
+  int64_t fallback_topology;  /* always 0: every wiring is eligible */
   int64_t fallback_ood;       /* feature outside zmax */
   int64_t fallback_margin;    /* predicted energy near a fate threshold */
   int64_t shadow;             /* cells run both ways and compared */
@@ -413,6 +421,9 @@ typedef struct {
   float shadow_max_rel_err;
   int32_t _pad;
   int64_t err_hist[24];       /* log10 max relative error, half-decade bins from 1e-9 */
+  int64_t shadow_within_1e3_species[16]; /* shadow cells whose internal k is within 1e-3 (k < 15) */
+  int64_t fate_exact[4];      /* shadow cells' fate events (MARL_EVENT_*) from the exact update */
+  int64_t fate_surrogate[4];  /* ... and from the prediction */
 } marl_surr_stats;
 
 /* MARL_KINETICS_PLAIN (exact), _SURROGATE (learned, with fallback) or
@@ -430,9 +441,7 @@ int marl_surrogate_param_count(void);
 /* Flat FP32 parameters: per layer W [in x out] row-major then b [out];
  * norm = feature mean[80], std[80], target mean[28], std[28]. */
 int marl_surrogate_get_params(marl_ctx* ctx, float* params, size_t n, float* norm, size_t n_norm);
-int marl_surrogate_set_params(marl_ctx* ctx, const float* params, size_t n, const float* norm, size_t n_norm,
-                              const uint64_t* classes, int32_t n_classes);
-int marl_surrogate_classes(marl_ctx* ctx, uint64_t* classes, int32_t* n);
+int marl_surrogate_set_params(marl_ctx* ctx, const float* params, size_t n, const float* norm, size_t n_norm);
 /* Network output for standardised inputs x [rows][80] -> y [rows][28]:
  * fused = 1 the inference kernel, 0 the training forward. */
 int marl_surrogate_forward(marl_ctx* ctx, const float* x, size_t rows, float* y, int32_t fused);

@@ -92,6 +92,7 @@ class marl_train_config(ctypes.Structure):
         ("zmax", ctypes.c_float),
         ("energy_margin", ctypes.c_float),
         ("seed", ctypes.c_uint64),
+        ("max_pairs", ctypes.c_int64),
     ]
 
 
@@ -103,8 +104,6 @@ class marl_train_report(ctypes.Structure):
         ("ms", ctypes.c_float),
         ("pairs", ctypes.c_int64),
         ("pairs_train", ctypes.c_int64),
-        ("classes", ctypes.c_int32),
-        ("_pad", ctypes.c_int32),
     ]
 
 
@@ -121,6 +120,9 @@ class marl_surr_stats(ctypes.Structure):
         ("shadow_max_rel_err", ctypes.c_float),
         ("_pad", ctypes.c_int32),
         ("err_hist", ctypes.c_int64 * 24),
+        ("shadow_within_1e3_species", ctypes.c_int64 * 16),
+        ("fate_exact", ctypes.c_int64 * 4),
+        ("fate_surrogate", ctypes.c_int64 * 4),
     ]
 
 
@@ -275,8 +277,7 @@ SIGNATURES = {
     "marl_surrogate_train": (ctypes.c_int, [_P, ctypes.POINTER(marl_train_config), ctypes.POINTER(marl_train_report)]),
     "marl_surrogate_param_count": (ctypes.c_int, []),
     "marl_surrogate_get_params": (ctypes.c_int, [_P, _P, _SZ, _P, _SZ]),
-    "marl_surrogate_set_params": (ctypes.c_int, [_P, _P, _SZ, _P, _SZ, _P, _I32]),
-    "marl_surrogate_classes": (ctypes.c_int, [_P, _P, ctypes.POINTER(_I32)]),
+    "marl_surrogate_set_params": (ctypes.c_int, [_P, _P, _SZ, _P, _SZ]),
     "marl_surrogate_forward": (ctypes.c_int, [_P, _P, _SZ, _P, _I32]),
     "marl_surrogate_grad": (ctypes.c_int, [_P, _P, _P, _SZ, ctypes.POINTER(ctypes.c_float), _P]),
     "marl_surrogate_adam_step": (ctypes.c_int, [_P, _P, ctypes.c_float]),

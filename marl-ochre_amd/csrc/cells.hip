@@ -318,7 +318,8 @@ __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict
                                               const marl_ruleset* __restrict__ rules, float* __restrict__ field,
                                               const float* __restrict__ light, const uint8_t* __restrict__ nbmask,
                                               uint8_t* __restrict__ events, const CellParams& p,
-                                              const HarvestSink& hv, float* sI, float* sE, float* sD) {
+                                              const HarvestSink& hv, float* sI, float* sE, float* sD, float* sQ,
+                                              float* sR) {
   marl_cell_state c = load_cell(cells + i);
   const marl_ruleset& r = rules[i];
   const uint32_t m = nbmask[i];
@@ -331,10 +332,16 @@ __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict
   PtrArr I{sI}, D{sD};
   const PtrArr E{sE};
   const AosRules ar{&r};
+  // a sampled cell of the harvest also records the explicit step from its pre-tick state
+  const bool take = HARVEST && (uint64_t)sample_hash(hv.tick, (uint32_t)i) < hv.threshold;
+  float frel[MARL_R_MAX];
+  if (take) {
+    PtrArr dI{sQ}, dD{sR};
+    explicit_step(c, ar, PtrArr{sI}, E, dI, dD, lt, p, frel);
+  }
   cell_kinetics(c, ar, I, E, D, lt, p);
   const float4 fate = ar.fate();
   // one counter update per wave: a per-cell atomic would serialise on one address
-  const bool take = HARVEST && (uint64_t)sample_hash(hv.tick, (uint32_t)i) < hv.threshold;
   unsigned long long slot = 0;
   if (HARVEST) {
     const unsigned long long mask = __ballot(take);
@@ -348,15 +355,16 @@ __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict
   if (take) {
     if ((int64_t)slot < hv.cap) {
       float f[kFeat];
-      make_features(c.internal, c, sE, lt, r, 0, f);
+      make_features(c.internal, c, sE, lt, sQ, sR, frel, f);
       float* fo = hv.feat + slot * kFeat;
 #pragma unroll
       for (int k = 0; k < kFeat; ++k) fo[k] = f[k];
       float* to = hv.targ + slot * kTarg;
 #pragma unroll
-      for (int k = 0; k < MARL_M_INT; ++k) to[k] = sI[k] - c.internal[k];
+      for (int k = 0; k < MARL_M_INT - 1; ++k) to[k] = (sI[k] - c.internal[k]) - sQ[k];
+      to[MARL_M_INT - 1] = 0.0f;
 #pragma unroll
-      for (int s2 = 0; s2 < MARL_S_EXT; ++s2) to[16 + s2] = sD[s2];
+      for (int s2 = 0; s2 < MARL_S_EXT; ++s2) to[16 + s2] = sD[s2] - sR[s2];
       hv.hash[slot] = topology_hash(r);
     }
   }
@@ -378,11 +386,14 @@ __global__ void __launch_bounds__(B) k_kinetics(const int32_t* __restrict__ orde
   __shared__ float sI[B * 17];
   __shared__ float sE[B * 13];
   __shared__ float sD[B * 13];
+  __shared__ float sQ[HARVEST ? B * 17 : 1];  // the harvest's explicit-step deltas
+  __shared__ float sR[HARVEST ? B * 13 : 1];
   const int t = blockIdx.x * B + threadIdx.x;
   if (t >= count) return;
   const int i = order[t];
   kinetics_cell<HARVEST, QUADS>(i, cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17,
-                                sE + threadIdx.x * 13, sD + threadIdx.x * 13);
+                                sE + threadIdx.x * 13, sD + threadIdx.x * 13, HARVEST ? sQ + threadIdx.x * 17 : sQ,
+                                HARVEST ? sR + threadIdx.x * 13 : sR);
 }
 
 // ---------------------------------------------------------------------------
@@ -831,7 +842,7 @@ __global__ void __launch_bounds__(B) k_g_kinetics(int L, const TickDev* ts, cons
   const HarvestSink hv{};
   GS_LOOP(t, cnt)
   kinetics_cell<false, QUADS>(order[lo + t], cells, rules, field, light, nbmask, events, p, hv,
-                              sI + threadIdx.x * 17, sE + threadIdx.x * 13, sD + threadIdx.x * 13);
+                              sI + threadIdx.x * 17, sE + threadIdx.x * 13, sD + threadIdx.x * 13, nullptr, nullptr);
 }
 
 // division targets for i < n; zero flags up to cap for the fixed-size scan
@@ -1481,15 +1492,12 @@ static TickDev* host_tickdev(marl_ctx* ctx) {
   return reinterpret_cast<TickDev*>(reinterpret_cast<uint8_t*>(ctx->pinned) + 2048);
 }
 
-// Before a replay: the live count and tick index into the device counters.
+// Before a replay: the live count and tick index into the device counters,
+// as kernel arguments (k_stage_batch with one column; the host copy of the
+// counters is kept in step for the host-finish path).
 int marl_graph_stage(marl_ctx* ctx, hipStream_t st, int64_t tick) {
-  TickDev* hts = host_tickdev(ctx);
-  TickDev* ts = static_cast<TickDev*>(ctx->tdev);
-  hts->n = (int32_t)ctx->n_cells;
-  hts->tick = (uint64_t)tick;
-  MARL_HIP(ctx, hipMemcpyAsync(&ts->n, &hts->n, sizeof(int32_t), hipMemcpyHostToDevice, st));
-  MARL_HIP(ctx, hipMemcpyAsync(&ts->tick, &hts->tick, sizeof(uint64_t), hipMemcpyHostToDevice, st));
-  return MARL_OK;
+  marl_ctx* one[1] = {ctx};
+  return marl_graph_stage_many(one, 1, st, tick);
 }
 
 // After a replay: the counters back (the caller synchronises st).

@@ -356,20 +356,97 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const R& r, AI& I
 }
 
 // ---------------------------------------------------------------------------
-// Surrogate features (80): internal[0..14], quiescent, environment[12], light,
-// in-prep, then per reaction (log v_max, log k_m, catalyst == light), then the
-// topology class as two one-hot slots (class 0 = both zero).  The first 77
-// follow the workload's feature list with internal[15] (overwritten by light
-// every tick) and the prep flag replaced by the two state bits the tick
-// depends on; the last two identify which known reaction topology applies.
+// Surrogate inputs and targets.
+//
+// The surrogate predicts the residual between the exact update (cell_kinetics:
+// transporters, reactions and effectors applied one after another, each seeing
+// the previous ones' results, with their clamps) and one explicit step of the
+// same rate laws evaluated from the pre-tick state and routed through the
+// cell's own wiring (explicit_step).  The explicit step carries every cell's
+// wiring into the features, so any wiring is eligible; the network learns what
+// the sequential order and the clamps change.
+// Features (80): internal[0..14], quiescent, environment[12], light, in-prep,
+// the explicit internal deltas [0..14], the explicit exchange fluxes [12], each
+// reaction's explicit flux relative to its substrate [16], then 7 zeros.
+// Targets (28): exact minus explicit internal deltas [16] (slot 15, which the
+// tick overwrites with the voxel's light, is not learned) and exact minus
+// explicit exchange fluxes [12].
 constexpr int kFeat = 80;
 constexpr int kTarg = 28;
+constexpr int kFeatExplicit = 30;  // first explicit feature
 
-__device__ inline float safe_log(float v) { return __logf(fmaxf(fabsf(v), 1e-12f)); }
+// One explicit step of cell.rs:223-329's rate laws from the pre-tick
+// internals I0 and environment E: transport, reactions (light in slot 15, as
+// the reactions see it), maintenance on the pre-tick energy and effectors, all
+// evaluated at the same state and summed into dI[16] / dD[12] without clamps;
+// frel[q] = reaction q's flux / (its substrate + 1e-6).
+template <typename R, typename AI, typename AE, typename AQ, typename AD>
+__device__ inline void explicit_step(const marl_cell_state& c, const R& r, const AI& I0, const AE& E, AQ& dI, AD& dD,
+                                     float lt, const CellParams& p, float* frel) {
+  const float dt = p.dt;
+#pragma unroll
+  for (int k = 0; k < MARL_M_INT; ++k) dI.set(k, 0.0f);
+#pragma unroll
+  for (int e = 0; e < MARL_S_EXT; ++e) dD.set(e, 0.0f);
+#pragma unroll
+  for (int q = 0; q < MARL_S_TRANSPORTERS; ++q) {
+    const marl_transport tp = r.transport(q);
+    const int e = tp.ext_species, k = tp.int_species;
+    if (e >= MARL_S_EXT || k >= MARL_M_INT) continue;
+    const float ex = E.get(e), ik = I0.get(k);
+    const float up = tp.uptake_rate * ex / (1.0f + ex);
+    const float sec = tp.secrete_rate * ik / (1.0f + ik);
+    dI.set(k, dI.get(k) + (up - sec) * dt);
+    dD.set(e, dD.get(e) + (sec - up) * dt);
+  }
+  auto val = [&](int i) { return i == MARL_M_INT - 1 ? lt : I0.get(i); };
+  int active = 0;
+#pragma unroll
+  for (int q = 0; q < MARL_R_MAX; ++q) {
+    const marl_reaction rx = r.reaction(q);
+    frel[q] = 0.0f;
+    if (fabsf(rx.v_max) > p.active_reaction_threshold && rx.substrate != rx.product) ++active;
+    if (fabsf(rx.v_max) < p.active_reaction_threshold || rx.substrate == rx.product) continue;
+    const int si = rx.substrate, pi = rx.product, ci = rx.catalyst;
+    if (si >= MARL_M_INT || pi >= MARL_M_INT || ci >= MARL_M_INT) continue;
+    const float sv = val(si), cv = val(ci);
+    float rate = rx.v_max * (sv / (rx.k_m + sv + 1.1920929e-07f)) *
+                 (p.epsilon + cv / (rx.k_cat + cv + 1.1920929e-07f));
+    if (rx.cofactor != 0xFF && rx.cofactor < MARL_M_INT) {
+      const int fi = rx.cofactor;
+      const float cof = val(fi);
+      rate *= cof / (1.0f + cof);
+      dI.set(fi, dI.get(fi) - 0.5f * rate * dt);
+    }
+    const float flux = rate * dt;
+    dI.set(si, dI.get(si) - flux);
+    dI.set(pi, dI.get(pi) + flux);
+    frel[q] = flux / (sv + 1e-6f);
+  }
+  float pm = 1.0f;
+  if (c.prep_remaining > 0) {
+    const float evolved = fmaxf(r.fate().w, 1.0f);
+    pm = p.prep_maintenance_multiplier + fmaxf(p.base_division_prep - evolved, 0.0f) * p.rush_penalty_rate;
+  }
+  dI.set(0, dI.get(0) - p.lambda_maintenance * pm * dt * I0.get(0) - (float)active * p.reaction_maintenance * dt);
+  if (!c.quiescent) {
+#pragma unroll
+    for (int q = 0; q < MARL_S_EFFECTORS; ++q) {
+      const marl_effector ef = r.effector(q);
+      const int k = ef.int_species, e = ef.ext_species;
+      if (k >= MARL_M_INT || e >= MARL_S_EXT) continue;
+      const float ik = I0.get(k);
+      if (ik > ef.threshold) {
+        const float amount = ef.rate * ik / (1.0f + ik) * dt;
+        dI.set(k, dI.get(k) - amount);
+        dD.set(e, dD.get(e) + amount);
+      }
+    }
+  }
+}
 
-template <typename R>
 __device__ inline void make_features(const float* I_pre, const marl_cell_state& c, const float* E, float lt,
-                                     const R& r, int cls, float* f) {
+                                     const float* dI, const float* dD, const float* frel, float* f) {
 #pragma unroll
   for (int k = 0; k < 15; ++k) f[k] = I_pre[k];
   f[15] = c.quiescent ? 1.0f : 0.0f;
@@ -377,15 +454,14 @@ __device__ inline void make_features(const float* I_pre, const marl_cell_state& 
   for (int s = 0; s < MARL_S_EXT; ++s) f[16 + s] = E[s];
   f[28] = lt;
   f[29] = c.prep_remaining > 0 ? 1.0f : 0.0f;
-#pragma unroll 4
-  for (int q = 0; q < MARL_R_MAX; ++q) {
-    const marl_reaction rx = r.reactions[q];
-    f[30 + 3 * q] = safe_log(rx.v_max);
-    f[31 + 3 * q] = safe_log(rx.k_m);
-    f[32 + 3 * q] = rx.catalyst == 15 ? 1.0f : 0.0f;
-  }
-  f[78] = cls == 1 ? 1.0f : 0.0f;
-  f[79] = cls == 2 ? 1.0f : 0.0f;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) f[kFeatExplicit + k] = dI[k];
+#pragma unroll
+  for (int e = 0; e < MARL_S_EXT; ++e) f[kFeatExplicit + 15 + e] = dD[e];
+#pragma unroll
+  for (int q = 0; q < MARL_R_MAX; ++q) f[kFeatExplicit + 27 + q] = frel[q];
+#pragma unroll
+  for (int k = kFeatExplicit + 43; k < kFeat; ++k) f[k] = 0.0f;
 }
 
 // 64-bit FNV-1a over the ruleset's integer wiring (reaction species,

@@ -4,7 +4,6 @@
 #include "kinetics.h"
 #include "mlp.h"
 
-constexpr int kMaxClasses = 3;
 constexpr int kErrBins = 24;  // log10 relative error histogram, bins of 0.5 decade from 1e-9
 
 struct SurrogateState {
@@ -18,8 +17,6 @@ struct SurrogateState {
   uint64_t hthreshold = 0;
   bool harvesting = false;
   // model
-  uint64_t classes[kMaxClasses] = {0, 0, 0};
-  int n_classes = 0;
   bool trained = false;
   float zmax = 6.0f;           // out-of-distribution bound on |standardised feature|
   float margin = 0.05f;        // fate margin on the predicted energy
@@ -28,6 +25,7 @@ struct SurrogateState {
   // per-level scratch
   uint16_t* featb = nullptr;
   float* out = nullptr;
+  float* expl = nullptr;       // explicit-step deltas [16] and fluxes [12] per cell of the level
   uint8_t* elig = nullptr;
   int64_t scratch_cap = 0;
   // statistics (device): counters as u64, error histogram, max error bits

@@ -21,7 +21,8 @@ namespace {
 
 enum StatSlot {
   ST_CELLS = 0, ST_SURR, ST_FB_TOPO, ST_FB_OOD, ST_FB_MARGIN, ST_SHADOW, ST_WITHIN, ST_FATE_AGREE, ST_MAXERR,
-  ST_HIST, ST_COUNT = ST_HIST + kErrBins
+  ST_HIST, ST_SPECIES = ST_HIST + kErrBins, ST_FATE_EXACT = ST_SPECIES + 16, ST_FATE_SURR = ST_FATE_EXACT + 4,
+  ST_COUNT = ST_FATE_SURR + 4
 };
 
 __device__ __forceinline__ uint16_t bf16_bits(float f) {
@@ -30,19 +31,8 @@ __device__ __forceinline__ uint16_t bf16_bits(float f) {
   return (uint16_t)(u >> 16);
 }
 
-struct Classes {
-  uint64_t h[kMaxClasses];
-  int n;
-};
-
-__device__ inline int class_of(uint64_t h, const Classes& cl) {
-  for (int j = 0; j < cl.n; ++j)
-    if (cl.h[j] == h) return j;
-  return -1;
-}
-
-// binary feature slots (left unnormalised)
-__device__ __host__ inline bool binary_feature(int k) { return k == 15 || k == 29 || k >= 78 || (k >= 30 && (k - 32) % 3 == 0); }
+// binary feature slots (left unnormalised): quiescent, in-prep
+__device__ __host__ inline bool binary_feature(int k) { return k == 15 || k == 29; }
 
 constexpr int kSBlock = 128;
 
@@ -62,32 +52,46 @@ __device__ __forceinline__ void stats_flush(const unsigned int* s, unsigned long
   }
 }
 
+// Features (standardised, BF16) and the explicit step of a level's cells;
+// a cell with a continuous feature beyond zmax standard deviations of the
+// training pairs is out of distribution and runs the plain path.
 __global__ void __launch_bounds__(kSBlock) k_surr_features(const int32_t* __restrict__ order, int count,
                                                          const marl_cell_state* __restrict__ cells,
                                                          const marl_ruleset* __restrict__ rules,
                                                          const float* __restrict__ field,
                                                          const float* __restrict__ light,
                                                          const uint8_t* __restrict__ nbmask, CellParams p,
-                                                         Classes cl, const float* __restrict__ norm,
+                                                         const float* __restrict__ norm,
                                                          const float* __restrict__ inv, float zmax,
-                                                         uint16_t* __restrict__ featb, uint8_t* __restrict__ elig,
+                                                         uint16_t* __restrict__ featb, float* __restrict__ expl,
+                                                         uint8_t* __restrict__ elig,
                                                          unsigned long long* __restrict__ stats) {
+  __shared__ float sI[kSBlock * 17];
   __shared__ float sE[kSBlock * 13];
+  __shared__ float sQ[kSBlock * 17];
+  __shared__ float sR[kSBlock * 13];
   __shared__ unsigned int sst[ST_COUNT];
   stats_init(sst);
   const int t = blockIdx.x * kSBlock + threadIdx.x;
   if (t < count) {
     const int i = order[t];
     const marl_cell_state c = load_cell(cells + i);
-    const marl_ruleset& r = rules[i];
+    float* I0 = sI + threadIdx.x * 17;
     float* E = sE + threadIdx.x * 13;
+    float* dI = sQ + threadIdx.x * 17;
+    float* dD = sR + threadIdx.x * 13;
+#pragma unroll
+    for (int k = 0; k < MARL_M_INT; ++k) I0[k] = c.internal[k];
     const uint32_t m = nbmask[i];
     const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
     gather_env6<false>(field, v, m, p, E);
-    const int cls = class_of(topology_hash(r), cl);
+    const float lt = light[v];
+    float frel[MARL_R_MAX];
+    PtrArr aQ{dI}, aR{dD};
+    explicit_step(c, AosRules{&rules[i]}, PtrArr{I0}, PtrArr{E}, aQ, aR, lt, p, frel);
     float f[kFeat];
-    make_features(c.internal, c, E, light[v], r, cls < 0 ? 0 : cls, f);
-    bool ok = cls >= 0;
+    make_features(c.internal, c, E, lt, dI, dD, frel, f);
+    bool ok = true;
     uint16_t* fo = featb + (size_t)t * kFeat;
 #pragma unroll
     for (int k = 0; k < kFeat; ++k) {
@@ -95,18 +99,29 @@ __global__ void __launch_bounds__(kSBlock) k_surr_features(const int32_t* __rest
       if (!binary_feature(k) && !(fabsf(z) <= zmax)) ok = false;
       fo[k] = bf16_bits(z);
     }
+    float* xo = expl + (size_t)t * kTarg;
+#pragma unroll
+    for (int k = 0; k < MARL_M_INT; ++k) xo[k] = dI[k];
+#pragma unroll
+    for (int e = 0; e < MARL_S_EXT; ++e) xo[16 + e] = dD[e];
     elig[t] = ok ? 1 : 0;
     atomicAdd(&sst[ST_CELLS], 1u);
-    if (cls < 0) atomicAdd(&sst[ST_FB_TOPO], 1u);
-    else if (!ok) atomicAdd(&sst[ST_FB_OOD], 1u);
+    if (!ok) atomicAdd(&sst[ST_FB_OOD], 1u);
   }
   stats_flush(sst, stats);
 }
 
+// Shadow error of a cell: the largest relative error over internals 0-14
+// (denominator max(|exact|, 1e-3)) into the histogram, and per species
+// whether it is within 1e-3.  Is: surrogate, Ip: exact.
 __device__ inline void record_error(const float* Is, const float* Ip, unsigned int* sst) {
   float err = 0.0f;
 #pragma unroll
-  for (int k = 0; k < 15; ++k) err = fmaxf(err, fabsf(Is[k] - Ip[k]) / fmaxf(fabsf(Ip[k]), 1e-3f));
+  for (int k = 0; k < 15; ++k) {
+    const float ek = fabsf(Is[k] - Ip[k]) / fmaxf(fabsf(Ip[k]), 1e-3f);
+    if (ek <= 1e-3f) atomicAdd(&sst[ST_SPECIES + k], 1u);
+    err = fmaxf(err, ek);
+  }
   atomicAdd(&sst[ST_SHADOW], 1u);
   if (err <= 1e-3f) atomicAdd(&sst[ST_WITHIN], 1u);
   atomicMax(&sst[ST_MAXERR], __builtin_bit_cast(uint32_t, err));
@@ -115,23 +130,29 @@ __device__ inline void record_error(const float* Is, const float* Ip, unsigned i
   atomicAdd(&sst[ST_HIST + bin], 1u);
 }
 
-// One cell of a surrogate level (see k_surr_apply).
+// One cell of a surrogate level (see k_surr_apply).  The prediction is the
+// explicit step plus the network's residual, clamped as the exact update
+// clamps internals ([0, c_max]); internal 15 is the voxel's light, as the tick
+// sets it.  The gathered face records are kept for the scatter, as in the
+// plain pass.
 __device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __restrict__ cells,
                                            const marl_ruleset* __restrict__ rules, float* __restrict__ field,
                                            const float* __restrict__ light, const uint8_t* __restrict__ nbmask,
                                            uint8_t* __restrict__ events, const CellParams& p,
-                                           const float* __restrict__ out, const uint8_t* __restrict__ elig,
-                                           const float* __restrict__ norm, int mode, float margin,
-                                           uint64_t shadow_threshold, unsigned int* sst, float* I, float* E,
-                                           float* D) {
+                                           const float* __restrict__ out, const float* __restrict__ expl,
+                                           const uint8_t* __restrict__ elig, const float* __restrict__ norm,
+                                           int mode, float margin, uint64_t shadow_threshold, unsigned int* sst,
+                                           float* I, float* E, float* D) {
   marl_cell_state c = load_cell(cells + i);
-  const marl_ruleset& r = rules[i];
+  const AosRules ar{&rules[i]};
 #pragma unroll
   for (int k = 0; k < MARL_M_INT; ++k) I[k] = c.internal[k];
   const uint32_t m = nbmask[i];
   const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
-  gather_env6<false>(field, v, m, p, E);
+  float4 keep[18];
+  gather_env6_keep<false>(field, v, m, p, E, keep);
   const float lt = light[v];
+  const float4 fate = ar.fate();
 
   const float* tmean = norm + 2 * kFeat;
   const float* tstd = tmean + kTarg;
@@ -139,39 +160,15 @@ __device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __rest
   bool have = elig[t] != 0, use_s = false;
   if (have) {
     const float* y = out + (size_t)t * kTarg;
+    const float* x = expl + (size_t)t * kTarg;
 #pragma unroll
-    for (int k = 0; k < 15; ++k) Ip[k] = clamp_rs(c.internal[k] + (y[k] * tstd[k] + tmean[k]), 0.0f, p.c_max);
+    for (int k = 0; k < 15; ++k)
+      Ip[k] = clamp_rs(c.internal[k] + (x[k] + (y[k] * tstd[k] + tmean[k])), 0.0f, p.c_max);
     Ip[15] = lt;
 #pragma unroll
-    for (int e = 0; e < MARL_S_EXT; ++e) Dp[e] = y[16 + e] * tstd[16 + e] + tmean[16 + e];
-    // mass projection: the tick never creates matter (transport moves it,
-    // reactions convert 1:1 or destroy, maintenance destroys), so scale the
-    // positive parts down if the prediction's net change is positive
-    float gain = 0.0f, pos = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 15; ++k) {
-      const float d = Ip[k] - c.internal[k];
-      gain += d;
-      pos += fmaxf(d, 0.0f);
-    }
-#pragma unroll
-    for (int e = 0; e < MARL_S_EXT; ++e) {
-      gain += Dp[e];
-      pos += fmaxf(Dp[e], 0.0f);
-    }
-    if (gain > 0.0f && pos > 0.0f) {
-      const float keep = fmaxf(1.0f - gain / pos, 0.0f);
-#pragma unroll
-      for (int k = 0; k < 15; ++k) {
-        const float d = Ip[k] - c.internal[k];
-        if (d > 0.0f) Ip[k] = c.internal[k] + d * keep;
-      }
-#pragma unroll
-      for (int e = 0; e < MARL_S_EXT; ++e)
-        if (Dp[e] > 0.0f) Dp[e] *= keep;
-    }
+    for (int e = 0; e < MARL_S_EXT; ++e) Dp[e] = x[16 + e] + (y[16 + e] * tstd[16 + e] + tmean[16 + e]);
     const float e1 = Ip[0];
-    const float thr[3] = {fmaxf(r.death_energy, p.hard_death_floor), r.division_energy, r.quiescence_energy};
+    const float thr[3] = {fmaxf(fate.y, p.hard_death_floor), fate.x, fate.z};
     bool near = false;
 #pragma unroll
     for (int q = 0; q < 3; ++q) near |= fabsf(e1 - thr[q]) < margin;
@@ -183,13 +180,14 @@ __device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __rest
   if (!use_s || shadow) {
     PtrArr aI{I}, aD{D};
     const PtrArr aE{E};
-    cell_kinetics(c, AosRules{&r}, aI, aE, aD, lt, p);
+    cell_kinetics(c, ar, aI, aE, aD, lt, p);
   }
   if (shadow) {
     record_error(Ip, I, sst);
     marl_cell_state cs = c, cp = c;
-    const float4 fate = AosRules{&r}.fate();
     const uint8_t es = cell_fate(cs, fate, Ip[0], p), ep = cell_fate(cp, fate, I[0], p);
+    atomicAdd(&sst[ST_FATE_SURR + es], 1u);
+    atomicAdd(&sst[ST_FATE_EXACT + ep], 1u);
     if (es == ep && cs.prep_remaining == cp.prep_remaining && cs.quiescent == cp.quiescent)
       atomicAdd(&sst[ST_FATE_AGREE], 1u);
   }
@@ -200,12 +198,12 @@ __device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __rest
 #pragma unroll
     for (int e = 0; e < MARL_S_EXT; ++e) D[e] = Dp[e];
   }
-  const uint8_t ev = cell_fate(c, AosRules{&r}.fate(), I[0], p);
+  const uint8_t ev = cell_fate(c, fate, I[0], p);
 #pragma unroll
   for (int k = 0; k < MARL_M_INT; ++k) c.internal[k] = I[k];
   cells[i] = c;
   events[i] = ev;
-  scatter_deltas6<false>(field, v, m, p, D);
+  scatter_from_kept<false>(field, v, m, p, D, keep);
 }
 
 // Applies one level: surrogate result where eligible and not near a fate
@@ -216,7 +214,8 @@ __global__ void __launch_bounds__(kSBlock) k_surr_apply(const int32_t* __restric
                                                       float* __restrict__ field, const float* __restrict__ light,
                                                       const uint8_t* __restrict__ nbmask,
                                                       uint8_t* __restrict__ events, CellParams p,
-                                                      const float* __restrict__ out, const uint8_t* __restrict__ elig,
+                                                      const float* __restrict__ out, const float* __restrict__ expl,
+                                                      const uint8_t* __restrict__ elig,
                                                       const float* __restrict__ norm, int mode, float margin,
                                                       uint64_t shadow_threshold, unsigned long long* __restrict__ stats) {
   __shared__ float sI[kSBlock * 17];
@@ -225,8 +224,8 @@ __global__ void __launch_bounds__(kSBlock) k_surr_apply(const int32_t* __restric
   __shared__ unsigned int sst[ST_COUNT];
   stats_init(sst);
   const int t = blockIdx.x * kSBlock + threadIdx.x;
-  if (t < count) apply_cell(order[t], t, cells, rules, field, light, nbmask, events, p, out, elig, norm, mode, margin,
-                            shadow_threshold, sst, sI + threadIdx.x * 17, sE + threadIdx.x * 13,
+  if (t < count) apply_cell(order[t], t, cells, rules, field, light, nbmask, events, p, out, expl, elig, norm, mode,
+                            margin, shadow_threshold, sst, sI + threadIdx.x * 17, sE + threadIdx.x * 13,
                             sD + threadIdx.x * 13);
   stats_flush(sst, stats);
 }
@@ -234,15 +233,9 @@ __global__ void __launch_bounds__(kSBlock) k_surr_apply(const int32_t* __restric
 // ---------------------------------------------------------------------------
 // training helpers
 
-__device__ inline void class_slots(float* f, int cls) {
-  f[78] = cls == 1 ? 1.0f : 0.0f;
-  f[79] = cls == 2 ? 1.0f : 0.0f;
-}
-
 // column sums and sums of squares (double) of features (with class slots) and targets
 __global__ void __launch_bounds__(256) k_colstats(const int32_t* __restrict__ idx, int n, const float* __restrict__ F,
-                                                  const float* __restrict__ T, const uint64_t* __restrict__ H,
-                                                  Classes cl, double* __restrict__ acc) {
+                                                  const float* __restrict__ T, double* __restrict__ acc) {
   constexpr int C = kFeat + kTarg;
   __shared__ double s1[C], s2[C];
   for (int k = threadIdx.x; k < C; k += blockDim.x) s1[k] = s2[k] = 0.0;
@@ -251,7 +244,6 @@ __global__ void __launch_bounds__(256) k_colstats(const int32_t* __restrict__ id
     const int row = idx[j];
     float f[kFeat];
     for (int k = 0; k < kFeat; ++k) f[k] = F[(size_t)row * kFeat + k];
-    class_slots(f, class_of(H[row], cl));
     for (int k = 0; k < C; ++k) {
       const double x = k < kFeat ? f[k] : T[(size_t)row * kTarg + (k - kFeat)];
       atomicAdd(&s1[k], x);
@@ -268,8 +260,7 @@ __global__ void __launch_bounds__(256) k_colstats(const int32_t* __restrict__ id
 // minibatch: rows drawn from idx by a per-(step, slot) counter hash, normalised
 __global__ void k_gather_batch(const int32_t* __restrict__ idx, int n, uint64_t seed, uint64_t step, int B,
                                const float* __restrict__ F, const float* __restrict__ T,
-                               const uint64_t* __restrict__ H, Classes cl, const float* __restrict__ norm,
-                               const float* __restrict__ inv,
+                               const float* __restrict__ norm, const float* __restrict__ inv,
                                float* __restrict__ X, float* __restrict__ Yt, int sequential) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
@@ -284,7 +275,6 @@ __global__ void k_gather_batch(const int32_t* __restrict__ idx, int n, uint64_t 
   const int row = idx[j];
   float f[kFeat];
   for (int k = 0; k < kFeat; ++k) f[k] = F[(size_t)row * kFeat + k];
-  class_slots(f, class_of(H[row], cl));
   for (int k = 0; k < kFeat; ++k) X[(size_t)b * kFeat + k] = (f[k] - norm[k]) * inv[k];
   const float* tm = norm + 2 * kFeat;
   const float* ts = tm + kTarg;
@@ -319,13 +309,16 @@ int ensure_level_scratch(marl_ctx* ctx, int64_t n) {
   if (n <= s->scratch_cap) return MARL_OK;
   if (s->featb) (void)hipFree(s->featb);
   if (s->out) (void)hipFree(s->out);
+  if (s->expl) (void)hipFree(s->expl);
   if (s->elig) (void)hipFree(s->elig);
   s->featb = nullptr;
   s->out = nullptr;
+  s->expl = nullptr;
   s->elig = nullptr;
   const int64_t cap = std::max<int64_t>(n, 65536);
   MARL_HIP(ctx, hipMalloc(&s->featb, sizeof(uint16_t) * cap * kFeat));
   MARL_HIP(ctx, hipMalloc(&s->out, sizeof(float) * cap * kTarg));
+  MARL_HIP(ctx, hipMalloc(&s->expl, sizeof(float) * cap * kTarg));
   MARL_HIP(ctx, hipMalloc(&s->elig, cap));
   s->scratch_cap = cap;
   return MARL_OK;
@@ -342,12 +335,6 @@ int upload_norm(marl_ctx* ctx, SurrogateState* s, const std::vector<float>& nm) 
   return MARL_OK;
 }
 
-Classes classes_of(const SurrogateState* s) {
-  Classes cl;
-  cl.n = s->n_classes;
-  for (int j = 0; j < kMaxClasses; ++j) cl.h[j] = s->classes[j];
-  return cl;
-}
 
 // deterministic standardised-looking bf16 inputs for timing the network alone
 __global__ void k_fill_timing_input(size_t n, uint16_t* __restrict__ x) {
@@ -360,7 +347,7 @@ __global__ void k_fill_timing_input(size_t n, uint16_t* __restrict__ x) {
 void surrogate_free(SurrogateState* s) {
   if (!s) return;
   mlp_free(s->mlp);
-  void* bufs[] = {s->hfeat, s->htarg, s->hhash, s->hcount, s->featb, s->out, s->elig, s->stats, s->inv};
+  void* bufs[] = {s->hfeat, s->htarg, s->hhash, s->hcount, s->featb, s->out, s->expl, s->elig, s->stats, s->inv};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete s;
@@ -386,12 +373,13 @@ int surrogate_level(marl_ctx* ctx, const CellParams& p, const int32_t* order, in
   if (!s->trained) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "surrogate mode needs a trained surrogate");
   if (int rc = ensure_level_scratch(ctx, cnt)) return rc;
   hipLaunchKernelGGL(k_surr_features, dim3(gs(cnt, kSBlock)), dim3(kSBlock), 0, ctx->stream, order, cnt, cells,
-                     rules, field, ctx->light, ctx->nbmask, p, classes_of(s), s->mlp.norm, s->inv, s->zmax, s->featb,
+                     rules, field, ctx->light, ctx->nbmask, p, s->mlp.norm, s->inv, s->zmax, s->featb, s->expl,
                      s->elig, s->stats);
   MARL_HIP(ctx, hipGetLastError());
   if (int rc = mlp_infer_bf16(ctx, s->mlp, cnt, s->featb, s->out)) return rc;
   hipLaunchKernelGGL(k_surr_apply, dim3(gs(cnt, kSBlock)), dim3(kSBlock), 0, ctx->stream, order, cnt, cells, rules,
-                     field, ctx->light, ctx->nbmask, ctx->events, p, s->out, s->elig, s->mlp.norm, ctx->kin_mode,
+                     field, ctx->light, ctx->nbmask, ctx->events, p, s->out, s->expl, s->elig, s->mlp.norm,
+                     ctx->kin_mode,
                      s->margin, s->shadow_threshold, s->stats);
   MARL_HIP(ctx, hipGetLastError());
   return MARL_OK;
@@ -488,28 +476,12 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
     return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "bad training configuration");
   const auto t0 = std::chrono::steady_clock::now();
   SurrogateState* s = ctx->surr;
-  const int64_t n = harvested(ctx);
+  int64_t n = harvested(ctx);
   if (n <= 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "no harvested pairs");
-  // topology classes: the most frequent hashes holding >= 0.5% of the pairs
-  std::vector<uint64_t> hh(n);
-  MARL_HIP(ctx, xfer(ctx, hh.data(), s->hhash, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
-  std::unordered_map<uint64_t, int64_t> freq;
-  for (uint64_t h : hh) freq[h]++;
-  std::vector<std::pair<int64_t, uint64_t>> byf;
-  for (auto& kv : freq) byf.push_back({kv.second, kv.first});
-  std::sort(byf.begin(), byf.end(), [](auto& a, auto& b) { return a.first > b.first || (a.first == b.first && a.second < b.second); });
-  s->n_classes = 0;
-  for (auto& e : byf)
-    if (s->n_classes < kMaxClasses && e.first * 200 >= n) s->classes[s->n_classes++] = e.second;
-  if (s->n_classes == 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "no reaction topology holds 0.5% of the pairs");
-  // training / validation indices (every 20th known-topology pair validates)
+  if (cfg->max_pairs > 0) n = std::min(n, cfg->max_pairs);
+  // training / validation indices: every 20th pair validates
   std::vector<int32_t> tr, va;
-  for (int64_t j = 0; j < n; ++j) {
-    bool known = false;
-    for (int c = 0; c < s->n_classes; ++c) known |= hh[j] == s->classes[c];
-    if (!known) continue;
-    ((tr.size() + va.size()) % 20 == 19 ? va : tr).push_back((int32_t)j);
-  }
+  for (int64_t j = 0; j < n; ++j) (j % 20 == 19 ? va : tr).push_back((int32_t)j);
   if (tr.empty()) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "no training pairs");
   if (va.empty()) va.push_back(tr.back());
   int32_t *dtr = nullptr, *dva = nullptr;
@@ -533,9 +505,8 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
   if (int rc = mlp_alloc(ctx, s->mlp, B)) return rc;
   // normalisation from the training pairs
   MARL_HIP(ctx, hipMemsetAsync(dacc, 0, sizeof(double) * 2 * (kFeat + kTarg), ctx->stream));
-  const Classes cl = classes_of(s);
   hipLaunchKernelGGL(k_colstats, dim3(std::min<int64_t>(gs(tr.size()), 1024)), dim3(256), 0, ctx->stream, dtr,
-                     (int)tr.size(), s->hfeat, s->htarg, s->hhash, cl, dacc);
+                     (int)tr.size(), s->hfeat, s->htarg, dacc);
   MARL_HIP(ctx, hipGetLastError());
   std::vector<double> acc(2 * (kFeat + kTarg));
   MARL_HIP(ctx, xfer(ctx, acc.data(), dacc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost));
@@ -579,7 +550,7 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
   float first = -1.0f, last = 0.0f;
   for (int64_t st = 0; st < cfg->steps; ++st) {
     hipLaunchKernelGGL(k_gather_batch, dim3(gs(B)), dim3(256), 0, ctx->stream, dtr, (int)tr.size(), cfg->seed,
-                       (uint64_t)st, B, s->hfeat, s->htarg, s->hhash, cl, s->mlp.norm, s->inv, s->mlp.x, s->mlp.t, 0);
+                       (uint64_t)st, B, s->hfeat, s->htarg, s->mlp.norm, s->inv, s->mlp.x, s->mlp.t, 0);
     MARL_HIP(ctx, hipGetLastError());
     if (int rc = mlp_forward_train(ctx, s->mlp, B)) return rc;
     const bool report = st == 0 || st == cfg->steps - 1;
@@ -597,7 +568,7 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
   // validation loss on up to one batch of held-out pairs (in order)
   const int VB = (int)std::min<size_t>(va.size(), (size_t)B);
   hipLaunchKernelGGL(k_gather_batch, dim3(gs(VB)), dim3(256), 0, ctx->stream, dva, (int)va.size(), cfg->seed, 0ull,
-                     VB, s->hfeat, s->htarg, s->hhash, cl, s->mlp.norm, s->inv, s->mlp.x, s->mlp.t, 1);
+                     VB, s->hfeat, s->htarg, s->mlp.norm, s->inv, s->mlp.x, s->mlp.t, 1);
   MARL_HIP(ctx, hipGetLastError());
   if (int rc = mlp_forward_train(ctx, s->mlp, VB)) return rc;
   float vloss = 0.0f;
@@ -613,7 +584,6 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
     rep->val_loss = vloss;
     rep->pairs = n;
     rep->pairs_train = (int64_t)tr.size();
-    rep->classes = s->n_classes;
     rep->ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   return MARL_OK;
@@ -633,12 +603,10 @@ int marl_surrogate_get_params(marl_ctx* ctx, float* params, size_t n, float* nor
   return MARL_OK;
 }
 
-int marl_surrogate_set_params(marl_ctx* ctx, const float* params, size_t n, const float* norm, size_t n_norm,
-                              const uint64_t* classes, int32_t n_classes) {
+int marl_surrogate_set_params(marl_ctx* ctx, const float* params, size_t n, const float* norm, size_t n_norm) {
   if (!ctx || !params) return MARL_ERR_INVALID_INPUT;
   const MlpLayout L = mlp_layout();
   if (n != L.total) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "parameter count mismatch");
-  if (n_classes < 0 || n_classes > kMaxClasses) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "bad class count");
   if (int rc = ensure_surr(ctx)) return rc;
   SurrogateState* s = ctx->surr;
   if (int rc = mlp_alloc(ctx, s->mlp, std::max(s->mlp.batch, 64))) return rc;
@@ -654,8 +622,6 @@ int marl_surrogate_set_params(marl_ctx* ctx, const float* params, size_t n, cons
     std::copy(norm, norm + n_norm, nm.begin());
   }
   if (int rc = upload_norm(ctx, s, nm)) return rc;
-  s->n_classes = n_classes;
-  for (int j = 0; j < n_classes; ++j) s->classes[j] = classes[j];
   if (int rc = mlp_pack(ctx, s->mlp)) return rc;
   MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
   s->trained = true;
@@ -757,6 +723,11 @@ int marl_surrogate_stats(marl_ctx* ctx, marl_surr_stats* out) {
   const uint32_t mb = (uint32_t)st[ST_MAXERR];
   memcpy(&out->shadow_max_rel_err, &mb, 4);
   for (int b = 0; b < kErrBins; ++b) out->err_hist[b] = (int64_t)st[ST_HIST + b];
+  for (int k = 0; k < 16; ++k) out->shadow_within_1e3_species[k] = (int64_t)st[ST_SPECIES + k];
+  for (int e = 0; e < 4; ++e) {
+    out->fate_exact[e] = (int64_t)st[ST_FATE_EXACT + e];
+    out->fate_surrogate[e] = (int64_t)st[ST_FATE_SURR + e];
+  }
   return MARL_OK;
 }
 
@@ -768,11 +739,5 @@ int marl_surrogate_reset_stats(marl_ctx* ctx) {
   return MARL_OK;
 }
 
-int marl_surrogate_classes(marl_ctx* ctx, uint64_t* classes, int32_t* n) {
-  if (!ctx || !classes || !n) return MARL_ERR_INVALID_INPUT;
-  *n = ctx->surr ? ctx->surr->n_classes : 0;
-  for (int j = 0; j < *n; ++j) classes[j] = ctx->surr->classes[j];
-  return MARL_OK;
-}
 
 }  // extern "C"

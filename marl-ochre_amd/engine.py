@@ -334,9 +334,11 @@ class MarlGpu:
         self._chk(self.lib.marl_harvest_upload(self.h, ptr(f), ptr(t), ptr(h), h.size))
 
     def surrogate_train(self, steps: int, batch: int = 4096, lr: float = 2e-3, lr_final: float = 1e-5,
-                        seed: int = 1, reinit: bool = False, zmax: float = 6.0, energy_margin: float = 0.05) -> dict:
+                        seed: int = 1, reinit: bool = False, zmax: float = 6.0, energy_margin: float = 0.05,
+                        max_pairs: int = 0) -> dict:
         cfg = _abi.marl_train_config(steps=int(steps), batch=int(batch), reinit=int(reinit), lr=lr,
-                                     lr_final=lr_final, zmax=zmax, energy_margin=energy_margin, seed=int(seed))
+                                     lr_final=lr_final, zmax=zmax, energy_margin=energy_margin, seed=int(seed),
+                                     max_pairs=int(max_pairs))
         rep = _abi.marl_train_report()
         self._chk(self.lib.marl_surrogate_train(self.h, ctypes.byref(cfg), ctypes.byref(rep)))
         return {k: getattr(rep, k) for k, _ in rep._fields_ if not k.startswith("_")}
@@ -351,19 +353,11 @@ class MarlGpu:
         self._chk(self.lib.marl_surrogate_get_params(self.h, ptr(w), w.size, ptr(nm), nm.size))
         return w, nm
 
-    def surrogate_set_params(self, params: np.ndarray, norm: np.ndarray | None = None, classes=()) -> None:
+    def surrogate_set_params(self, params: np.ndarray, norm: np.ndarray | None = None) -> None:
         w = np.ascontiguousarray(params, np.float32)
-        cl = np.ascontiguousarray(np.asarray(classes, np.uint64).reshape(-1))
         nm = None if norm is None else np.ascontiguousarray(norm, np.float32)
         self._chk(self.lib.marl_surrogate_set_params(self.h, ptr(w), w.size, None if nm is None else ptr(nm),
-                                                     0 if nm is None else nm.size,
-                                                     ptr(cl) if cl.size else None, cl.size))
-
-    def surrogate_classes(self) -> np.ndarray:
-        out = np.zeros(8, np.uint64)
-        n = ctypes.c_int32()
-        self._chk(self.lib.marl_surrogate_classes(self.h, ptr(out), ctypes.byref(n)))
-        return out[: n.value]
+                                                     0 if nm is None else nm.size))
 
     def surrogate_forward(self, x: np.ndarray, fused: bool = True) -> np.ndarray:
         xx = np.ascontiguousarray(x, np.float32)
@@ -395,8 +389,10 @@ class MarlGpu:
     def surrogate_stats(self) -> dict:
         st = _abi.marl_surr_stats()
         self._chk(self.lib.marl_surrogate_stats(self.h, ctypes.byref(st)))
-        d = {k: getattr(st, k) for k, _ in st._fields_ if not k.startswith("_") and k != "err_hist"}
-        d["err_hist"] = list(st.err_hist)
+        arrays = ("err_hist", "shadow_within_1e3_species", "fate_exact", "fate_surrogate")
+        d = {k: getattr(st, k) for k, _ in st._fields_ if not k.startswith("_") and k not in arrays}
+        for k in arrays:
+            d[k] = list(getattr(st, k))
         return d
 
     def surrogate_reset_stats(self) -> None:

@@ -119,3 +119,73 @@ def adam(params, grads, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
     c2 = f(1.0) - f(b2) ** f(step)
     w = np.asarray(params, f) - f(lr) * (m / c1) / (np.sqrt(v / c2) + f(eps))
     return w.astype(f), m.astype(f), v.astype(f)
+
+
+# ---------------------------------------------------------------------------
+# The surrogate's input map (test infrastructure): one explicit step of the
+# cell tick's rate laws (crates/marl-cell/src/cell.rs:223-329) from the
+# pre-tick state, routed through the cell's wiring, in float32 with one IEEE
+# operation per step in the order the HIP kernel uses.
+
+F32 = np.float32
+F32_EPS = F32(1.1920929e-07)
+
+
+def explicit_step(I0, E, lt, rule, quiescent, prep_remaining, sim):
+    """(dI[16], dD[12], frel[16]): the explicit internal deltas, exchange fluxes
+    and per-reaction flux / (substrate + 1e-6) of one cell.  I0: pre-tick
+    internals [16], E: environment [12], lt: the voxel's light."""
+    dt = F32(sim.dt)
+    dI = [F32(0)] * 16
+    dD = [F32(0)] * 12
+    frel = [F32(0)] * 16
+    one = F32(1)
+    for tp in rule["transport"]:
+        e, k = int(tp["ext_species"]), int(tp["int_species"])
+        if e >= 12 or k >= 16:
+            continue
+        ex, ik = F32(E[e]), F32(I0[k])
+        up = F32(tp["uptake_rate"]) * ex / (one + ex)
+        sec = F32(tp["secrete_rate"]) * ik / (one + ik)
+        dI[k] = dI[k] + (up - sec) * dt
+        dD[e] = dD[e] + (sec - up) * dt
+
+    def val(i):
+        return F32(lt) if i == 15 else F32(I0[i])
+
+    thr = F32(sim.active_reaction_threshold)
+    active = 0
+    for q, rx in enumerate(rule["reactions"]):
+        v, si, pi, ci, co = F32(rx["v_max"]), int(rx["substrate"]), int(rx["product"]), int(rx["catalyst"]), \
+            int(rx["cofactor"])
+        if abs(v) > thr and si != pi:
+            active += 1
+        if abs(v) < thr or si == pi or si >= 16 or pi >= 16 or ci >= 16:
+            continue
+        sv, cv = val(si), val(ci)
+        rate = v * (sv / (F32(rx["k_m"]) + sv + F32_EPS)) * (F32(sim.epsilon) + cv / (F32(rx["k_cat"]) + cv + F32_EPS))
+        if co != 0xFF and co < 16:
+            cof = val(co)
+            rate = rate * (cof / (one + cof))
+            dI[co] = dI[co] - F32(0.5) * rate * dt
+        flux = rate * dt
+        dI[si] = dI[si] - flux
+        dI[pi] = dI[pi] + flux
+        frel[q] = flux / (sv + F32(1e-6))
+    pm = one
+    if prep_remaining > 0:
+        evolved = max(F32(rule["division_prep_ticks"]), one)
+        pm = F32(sim.prep_maintenance_multiplier) + max(F32(sim.base_division_prep) - evolved, F32(0)) * \
+            F32(sim.rush_penalty_rate)
+    dI[0] = dI[0] - F32(sim.lambda_maintenance) * pm * dt * F32(I0[0]) - F32(active) * F32(sim.reaction_maintenance) * dt
+    if not quiescent:
+        for ef in rule["effectors"]:
+            k, e = int(ef["int_species"]), int(ef["ext_species"])
+            if k >= 16 or e >= 12:
+                continue
+            ik = F32(I0[k])
+            if ik > F32(ef["threshold"]):
+                amount = F32(ef["rate"]) * ik / (one + ik) * dt
+                dI[k] = dI[k] - amount
+                dD[e] = dD[e] + amount
+    return np.array(dI, F32), np.array(dD, F32), np.array(frel, F32)

@@ -130,11 +130,15 @@ def _population(A, dims, n, seed, shared=True):
 
 
 def test_harvest_pairs_match_oracle(pkg, orc):
+    """Harvested pairs: the state features as each cell saw them, the explicit
+    step restated on the host from those inputs (bit for bit), and targets equal
+    to the oracle's post-kinetics internals minus the pre-tick ones minus the
+    explicit deltas."""
     from marl_ochre_amd import scenarios
 
     A = pkg._abi
     dims = (64, 32, 32)
-    cells, rules, field, light = _population(A, dims, 3000, 7)
+    cells, rules, field, light = _population(A, dims, 3000, 7, shared=False)
     sim = pkg.SimulationConfig()
     _, c_w, _ = orc.cell_pass(dims, sim, field, light, cells, rules)
     with pkg.MarlGpu(dims) as g:
@@ -155,23 +159,27 @@ def test_harvest_pairs_match_oracle(pkg, orc):
     np.testing.assert_array_equal(feat[:, 29], (cells["prep_remaining"][idx] > 0).astype(F))
     vox = (cells["pos"][:, 2].astype(np.int64) * dims[1] + cells["pos"][:, 1]) * dims[0] + cells["pos"][:, 0]
     np.testing.assert_array_equal(feat[:, 28], light[vox[idx]])
-    rx = rules["reactions"][idx]
-    np.testing.assert_allclose(feat[:, 30:78:3], np.log(np.maximum(np.abs(rx["v_max"]), 1e-12)), rtol=0, atol=1e-5)
-    np.testing.assert_allclose(feat[:, 31:78:3], np.log(np.maximum(np.abs(rx["k_m"]), 1e-12)), rtol=0, atol=1e-5)
-    np.testing.assert_array_equal(feat[:, 32:78:3], (rx["catalyst"] == 15).astype(F))
-    # internal-delta targets: post-kinetics internals minus pre, bit for bit in f32
-    want = c_w["internal"][idx] - cells["internal"][idx]
-    np.testing.assert_array_equal(targ[:, :16], want)
+    np.testing.assert_array_equal(feat[:, 73:], 0)
+    for j in range(0, n, 7):   # every 7th pair through the host restatement
+        i = idx[j]
+        dI, dD, frel = mlp_ref.explicit_step(cells["internal"][i], feat[j, 16:28], feat[j, 28], rules[i],
+                                             bool(cells["quiescent"][i]), int(cells["prep_remaining"][i]), sim)
+        np.testing.assert_array_equal(feat[j, 30:45], dI[:15])
+        np.testing.assert_array_equal(feat[j, 45:57], dD)
+        np.testing.assert_array_equal(feat[j, 57:73], frel)
+        want = (c_w["internal"][i] - cells["internal"][i]) - dI
+        np.testing.assert_array_equal(targ[j, :15], want[:15])
+        assert targ[j, 15] == 0
 
 
 @pytest.fixture(scope="module")
 def trained(pkg):
     """A context whose network was trained on pairs harvested from two plain
-    passes over 200k cells of three reaction topologies, plus the same network
-    before training (parameters and normalisation)."""
+    passes over 200k cells, each with its own random reaction wiring, plus the
+    same network before training (parameters and normalisation)."""
     A = pkg._abi
     dims = (128, 128, 64)
-    cells, rules, field, light = _population(A, dims, 200000, 11)
+    cells, rules, field, light = _population(A, dims, 200000, 11, shared=False)
     sim = pkg.SimulationConfig()
     g = pkg.MarlGpu(dims)
     g.upload_field_aos(field)
@@ -184,60 +192,51 @@ def trained(pkg):
     g.surrogate_train(steps=0, batch=4096, seed=3, reinit=True)
     untrained = g.surrogate_get_params()
     rep = g.surrogate_train(steps=800, batch=4096, lr=2e-3, seed=3)
-    yield dict(g=g, rep=rep, n_pairs=n_pairs, untrained=untrained, dims=dims, rules=rules, sim=sim)
+    yield dict(g=g, rep=rep, n_pairs=n_pairs, untrained=untrained, dims=dims, sim=sim)
     g.close()
 
 
-def _held_out(pkg, dims, rules, n, seed):
-    """A fresh population whose cells reuse the trained reaction topologies."""
-    c2, r2, f2, l2 = _population(pkg._abi, dims, n, seed)
-    for name in ("substrate", "product", "catalyst", "cofactor"):
-        r2["reactions"][name] = rules["reactions"][name][:n]
-    r2["transport"] = rules["transport"][:n]
-    r2["effectors"] = rules["effectors"][:n]
-    return c2, r2, f2, l2
+def _held_out(pkg, dims, n, seed):
+    """A fresh population with fresh per-cell wiring."""
+    return _population(pkg._abi, dims, n, seed, shared=False)
 
 
-def test_training_beats_untrained_and_no_reaction(pkg, trained):
+def test_training_beats_untrained_and_explicit_step(pkg, trained):
+    """Held-out cells of unseen wiring, in raw units: the internal-delta RMS
+    error of explicit step + trained residual against the explicit step alone
+    (residual 0) and against explicit step + untrained network."""
     g, rep = trained["g"], trained["rep"]
-    assert trained["n_pairs"] == 400000
-    assert rep["classes"] == 3 and rep["pairs"] == 400000
-    assert rep["loss_last"] < 0.1 * rep["loss_first"]
-    # standardised targets: predicting the mean scores 1.0
-    assert rep["val_loss"] < 0.1, rep
-    # held-out cells in raw units: internal-delta RMS error of the trained
-    # network, the untrained one and "no reaction" (zero deltas)
-    c2, r2, f2, l2 = _held_out(pkg, trained["dims"], trained["rules"], 20000, 12)
+    assert trained["n_pairs"] == 400000 and rep["pairs"] == 400000
+    assert rep["loss_last"] < 0.5 * rep["loss_first"]
+    assert rep["val_loss"] < 0.5, rep   # standardised targets: predicting the mean scores 1.0
+    c2, r2, f2, l2 = _held_out(pkg, trained["dims"], 20000, 12)
     g.upload_field_aos(f2)
     g.upload_light(l2)
     g.upload_cells(c2, r2)
     g.harvest_begin(30000, 1.0)
     g.cell_pass(trained["sim"])
     g.harvest_end()
-    feat, targ, hsh = g.harvest_download()
-    classes = g.surrogate_classes()
-    cls = np.select([hsh == classes[1], hsh == classes[2]], [1, 2], 0)
-    feat[:, 78] = cls == 1
-    feat[:, 79] = cls == 2
+    feat, targ, _ = g.harvest_download()
 
     def rms_err(w, norm):
         with pkg.MarlGpu((32, 32, 32)) as h:
-            h.surrogate_set_params(w, norm, classes)
+            h.surrogate_set_params(w, norm)
             z = (feat - norm[:80]) / norm[80:160]
             pred = h.surrogate_forward(z, fused=True) * norm[188:216] + norm[160:188]
         return np.sqrt(np.mean((pred[:, :15] - targ[:, :15]) ** 2))
 
     err_net = rms_err(*g.surrogate_get_params())
     err_untrained = rms_err(*trained["untrained"])
-    err_zero = np.sqrt(np.mean(targ[:, :15] ** 2))
-    assert err_net < 0.5 * err_zero, (err_net, err_zero)
-    assert err_net < 0.4 * err_untrained, (err_net, err_untrained)
+    err_explicit = np.sqrt(np.mean(targ[:, :15] ** 2))   # the targets are the explicit step's error
+    print("rms internal-delta error: net", err_net, "explicit step", err_explicit, "untrained", err_untrained)
+    assert err_net < 0.6 * err_explicit, (err_net, err_explicit)
+    assert err_net < 0.5 * err_untrained, (err_net, err_untrained)
 
 
 def test_surrogate_mode_accounting_and_bounds(pkg, trained):
     A = pkg._abi
     g, sim = trained["g"], trained["sim"]
-    c2, r2, f2, l2 = _held_out(pkg, trained["dims"], trained["rules"], 50000, 13)
+    c2, r2, f2, l2 = _held_out(pkg, trained["dims"], 50000, 13)
     g.upload_field_aos(f2)
     g.upload_light(l2)
     g.upload_cells(c2, r2)
@@ -260,6 +259,8 @@ def test_surrogate_mode_accounting_and_bounds(pkg, trained):
     assert 0.45 * st["surrogate"] < st["shadow"] < 0.55 * st["surrogate"]
     assert sum(st["err_hist"]) == st["shadow"] and st["shadow_within_1e3"] <= st["shadow"]
     assert 0.9 * st["shadow"] <= st["shadow_fate_agree"] <= st["shadow"]
+    assert all(st["shadow_within_1e3"] <= w <= st["shadow"] for w in st["shadow_within_1e3_species"][:15])
+    assert sum(st["fate_exact"]) == sum(st["fate_surrogate"]) == st["shadow"]
     ci = c["internal"]
     assert np.isfinite(ci).all() and (ci >= 0).all() and (ci <= sim.c_max).all()
     assert np.isfinite(f).all() and (f >= 0).all()
@@ -273,16 +274,15 @@ def test_fallback_and_shadow_are_bit_exact(pkg):
     cells, rules, field, light = _population(A, dims, 12000, 21)
     sim = pkg.SimulationConfig()
 
-    def run(mode, classes=None, margin=0.05, shadow=0.0):
+    def run(mode, zmax=1e9, margin=0.05, shadow=0.0):
         with pkg.MarlGpu(dims) as g:
             g.upload_field_aos(field)
             g.upload_light(light)
             g.upload_cells(cells, rules)
             if mode != A.KINETICS_PLAIN:
                 rng = np.random.default_rng(0)
-                g.surrogate_set_params(rng.standard_normal(g.surrogate_param_count).astype(F) * F(0.05),
-                                       None, classes)
-                g.surrogate_configure(zmax=1e9, energy_margin=margin, shadow_rate=shadow)
+                g.surrogate_set_params(rng.standard_normal(g.surrogate_param_count).astype(F) * F(0.05), None)
+                g.surrogate_configure(zmax=zmax, energy_margin=margin, shadow_rate=shadow)
                 g.set_kinetics_mode(mode)
             ev, _ = g.cell_pass(sim)
             c, r = g.download_cells()
@@ -290,16 +290,15 @@ def test_fallback_and_shadow_are_bit_exact(pkg):
             return ev, c, g.download_field_aos(), st
 
     ev0, c0, f0, _ = run(A.KINETICS_PLAIN)
-    known = np.unique(scenarios.topology_hash(rules))
-    assert known.size == 3
-    # unknown topology: every cell falls back
-    ev1, c1, f1, st1 = run(A.KINETICS_SURROGATE, classes=[12345])
-    assert st1["cells"] == len(cells) and st1["fallback_topology"] == len(cells) and st1["surrogate"] == 0
-    # known topologies but an energy margin wider than any energy: every cell falls back
-    ev2, c2, f2, st2 = run(A.KINETICS_SURROGATE, classes=known, margin=1e9)
+    # a vanishing out-of-distribution bound: every cell falls back
+    ev1, c1, f1, st1 = run(A.KINETICS_SURROGATE, zmax=1e-20)
+    assert st1["cells"] == len(cells) and st1["fallback_ood"] == len(cells) and st1["surrogate"] == 0
+    assert st1["fallback_topology"] == 0
+    # an energy margin wider than any energy: every cell falls back
+    ev2, c2, f2, st2 = run(A.KINETICS_SURROGATE, margin=1e9)
     assert st2["fallback_margin"] == len(cells) and st2["surrogate"] == 0
     # shadow: exact path authoritative, every eligible cell compared
-    ev3, c3, f3, st3 = run(A.KINETICS_SHADOW, classes=known)
+    ev3, c3, f3, st3 = run(A.KINETICS_SHADOW)
     assert st3["shadow"] == len(cells) and st3["surrogate"] == 0
     assert sum(st3["err_hist"]) == st3["shadow"]
     for ev, c, f in ((ev1, c1, f1), (ev2, c2, f2), (ev3, c3, f3)):

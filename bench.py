@@ -456,6 +456,7 @@ def ensemble_bench(pkg, ticks=5, warm=2):
             tile = ens.autotune(sim)
             shape = ens.launch_shape()
             ms_sub = ens.time_field(sim, 20)
+            ms_phase = ens.time_field_phase(sim, 5)
             for g, (dims, field, cells, rules, _) in zip(gs, cols):
                 g.upload_field_aos(field)
                 g.upload_cells(cells, rules)
@@ -498,6 +499,12 @@ def ensemble_bench(pkg, ticks=5, warm=2):
         "per_column_substeps_us": round(per_col_sub * 1e3, 1),
         "batched_substep_Gvoxel_species_per_s": round(nvox * 12 / (ms_sub * 1e-3) / 1e9, 1),
         "batched_substep_hbm_frac": round(alg / (ms_sub * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        # a tick's field phase: 10 substeps and the light scan (4 B/voxel write when fused into the
+        # last substep, 8.125 B/voxel as its own launch)
+        "batched_field_phase_us": round(ms_phase * 1e3, 1),
+        "batched_field_phase_hbm_frac": round(
+            (alg * sim.diffusion_substeps + (4.0 if shape["whole_columns"] else LIGHT_BYTES_PER_VOXEL) * nvox)
+            / (ms_phase * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "ms_per_tick_ensemble_graph": round(ens_wall, 3),
         "ms_per_tick_ensemble_graph_gpu": round(gpu_ms / ticks, 3),
         "ms_per_tick_columns_one_by_one_eager": round(seq["eager"], 3),

@@ -369,11 +369,24 @@ int marl_ensemble_set_tile(marl_ensemble* e, int32_t tile);
 int marl_ensemble_set_grid(marl_ensemble* e, int32_t per_tile);
 /* Average time of one batched substep over all columns (fields unchanged). */
 int marl_ensemble_time_field(marl_ensemble* e, const marl_sim_params* sim, int32_t n_substeps, float* ms_out);
-/* Times every stencil tile shape, with a persistent grid and with one block
- * per tile, on the resident columns and keeps the fastest. */
+/* Average time of a tick's batched field phase (the substeps, sources in the
+ * first, and the light scan) over n_ticks, reading the current buffers each
+ * time: the fields are unchanged, the light planes are rewritten. */
+int marl_ensemble_time_field_phase(marl_ensemble* e, const marl_sim_params* sim, int32_t n_ticks, float* ms_out);
+/* Per-column z-chunks: 0 = the tile shape's chunk (cut at each column's
+ * depth) with the light scan as its own batched launch; 1 = every column one
+ * chunk deep, which lets the last substep of the tick carry each column's
+ * light scan (no separate light launch).  Results are identical. */
+int marl_ensemble_set_whole_columns(marl_ensemble* e, int32_t whole);
+/* Times a tick's batched field phase (substeps and light) for every stencil
+ * tile shape, with a persistent grid and with one block per tile, and with
+ * both z-chunk modes, on the resident columns and keeps the fastest (fields
+ * and light planes are left unchanged). */
 int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t* tile_out);
-/* The batched stencil launch: tile shape, grid blocks and logical tiles. */
-int marl_ensemble_launch_shape(const marl_ensemble* e, int32_t* tile, int32_t* grid_blocks, int32_t* stencil_tiles);
+/* The batched stencil launch: tile shape, grid blocks, logical tiles and the
+ * z-chunk mode (each may be NULL). */
+int marl_ensemble_launch_shape(const marl_ensemble* e, int32_t* tile, int32_t* grid_blocks, int32_t* stencil_tiles,
+                               int32_t* whole_columns);
 
 /* ---- learned kinetics surrogate ------------------------------------------- */
 /* No reference counterpart: the reference integrates every cell's kinetics

@@ -268,7 +268,11 @@ SIGNATURES = {
     "marl_ensemble_time_field": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I32,
                                                 ctypes.POINTER(ctypes.c_float)]),
     "marl_ensemble_autotune": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), ctypes.POINTER(_I32)]),
-    "marl_ensemble_launch_shape": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
+    "marl_ensemble_launch_shape": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32),
+                                                  ctypes.POINTER(_I32)]),
+    "marl_ensemble_time_field_phase": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I32,
+                                                      ctypes.POINTER(ctypes.c_float)]),
+    "marl_ensemble_set_whole_columns": (ctypes.c_int, [_P, _I32]),
     "marl_set_kinetics_mode": (ctypes.c_int, [_P, _I32]),
     "marl_harvest_begin": (ctypes.c_int, [_P, _I64, ctypes.c_float]),
     "marl_harvest_end": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),

@@ -27,6 +27,7 @@ struct marl_ensemble {
   EnsColumn* dcols = nullptr;
   int tile = 7;
   int per_tile_grid = 0;  // 1: one block per stencil tile; 0: persistent grid
+  int whole = 0;          // 1: every column's z-chunk is its whole depth and the light scan rides the last substep
   int total_tiles = 0, total_light = 0, grid = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -85,7 +86,7 @@ void host_table(const marl_ensemble* e, std::vector<EnsColumn>& cols, int* total
     t.XW = x->xw;
     t.cur = x->cur;
   }
-  marl_ens_layout(e->tile, cols.data(), n, total_tiles, total_light);
+  marl_ens_layout(e->tile, cols.data(), n, total_tiles, total_light, e->whole != 0);
   *grid = e->per_tile_grid ? *total_tiles : marl_stencil_ens_grid(e->tile, *total_tiles);
 }
 
@@ -127,7 +128,31 @@ std::vector<uint8_t> ens_key(marl_ensemble* e, const std::vector<marl_sim_params
   }
   k.push_back((uint8_t)e->tile);
   k.push_back((uint8_t)e->per_tile_grid);
+  k.push_back((uint8_t)e->whole);
   return k;
+}
+
+// The batched field phase of one tick: the substeps (sources fused into the
+// first) and the light scan, fused into the last substep when every column is
+// one z-chunk deep, else its own launch.  parity_fixed: every launch reads the
+// current buffers (timing without changing the fields).
+int enqueue_field_phase(marl_ensemble* e, const marl_sim_params& sim, bool parity_fixed) {
+  const int n = (int)e->cols.size();
+  const int S = sim.diffusion_substeps;
+  for (int k = 0; k < S; ++k) {
+    StencilParams p = marl_stencil_params(&sim, k == 0 ? 1 : 0);
+    if (k == S - 1) {  // the light scan itself, or the compact species 4 the light launch reads
+      p.fuse_light = e->whole ? 1 : 2;
+      p.l_i0 = sim.surface_intensity;
+      p.l_e_cell = marl_expf(-sim.cell_absorption);
+      p.l_nchem = -sim.chemical_absorption;
+      p.l_floor = sim.light_floor;
+    }
+    ENS_HIP(e, marl_launch_substep_ens(e->stream, e->dcols, n, e->total_tiles, parity_fixed ? 0 : (k & 1), p,
+                                       e->tile, e->grid));
+  }
+  if (!e->whole) ENS_HIP(e, marl_launch_light_ens(e->stream, e->dcols, n, e->total_light, &sim, true));
+  return MARL_OK;
 }
 
 int enqueue_ensemble_tick(marl_ensemble* e, const std::vector<marl_sim_params>& sims) {
@@ -141,10 +166,7 @@ int enqueue_ensemble_tick(marl_ensemble* e, const std::vector<marl_sim_params>& 
     ENS_HIP(e, hipEventRecord(e->join[2 * c], x->stream));
     ENS_HIP(e, hipStreamWaitEvent(e->stream, e->join[2 * c], 0));
   }
-  for (int k = 0; k < sim.diffusion_substeps; ++k)
-    ENS_HIP(e, marl_launch_substep_ens(e->stream, e->dcols, n, e->total_tiles, k & 1,
-                                       marl_stencil_params(&sim, k == 0 ? 1 : 0), e->tile, e->grid));
-  ENS_HIP(e, marl_launch_light_ens(e->stream, e->dcols, n, e->total_light, &sim));
+  if (int rc = enqueue_field_phase(e, sim, false)) return rc;
   ENS_HIP(e, hipEventRecord(e->fork[1], e->stream));
   for (int c = 0; c < n; ++c) {
     marl_ctx* x = e->cols[c];
@@ -317,31 +339,90 @@ int marl_ensemble_time_field(marl_ensemble* e, const marl_sim_params* sim, int32
 // left unchanged).
 int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t* tile_out) {
   if (!e || !sim) return MARL_ERR_INVALID_INPUT;
-  int best = e->tile, best_grid = e->per_tile_grid;
+  if (sim->diffusion_substeps <= 0) return efail(e, MARL_ERR_INVALID_INPUT, "autotune needs substeps");
+  const int n = (int)e->cols.size();
+  // the fused light writes the columns' light planes: keep copies to restore
+  std::vector<float*> saved(n, nullptr);
+  struct Free {
+    std::vector<float*>& p;
+    ~Free() {
+      for (float* q : p)
+        if (q) (void)hipFree(q);
+    }
+  } fr{saved};
+  for (int c = 0; c < n; ++c) {
+    const size_t bytes = e->cols[c]->nvox * sizeof(float);
+    ENS_HIP(e, hipMalloc(&saved[c], bytes));
+    ENS_HIP(e, hipMemcpyAsync(saved[c], e->cols[c]->light, bytes, hipMemcpyDeviceToDevice, e->stream));
+  }
+  int best = e->tile, best_grid = e->per_tile_grid, best_whole = e->whole;
   float best_ms = 1e30f;
   for (int t = 0; t < marl_stencil_tile_count(); ++t) {
     if (!marl_tile_ensemble_ok(t)) continue;
-    for (int g = 0; g < 2; ++g) {
-      e->tile = t;
-      e->per_tile_grid = g;
-      float ms = 0.0f;
-      if (int rc = marl_ensemble_time_field(e, sim, 5, &ms)) return rc;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = t;
-        best_grid = g;
+    for (int g = 0; g < 2; ++g)
+      for (int w = 0; w < 2; ++w) {
+        e->tile = t;
+        e->per_tile_grid = g;
+        e->whole = w;
+        drop(e);
+        if (int rc = build_table(e)) return rc;
+        float total = 0.0f;
+        for (int rep = 0; rep < 3; ++rep) {  // the first is a warm-up
+          ENS_HIP(e, hipEventRecord(e->ev0, e->stream));
+          if (int rc = enqueue_field_phase(e, *sim, true)) return rc;
+          ENS_HIP(e, hipEventRecord(e->ev1, e->stream));
+          ENS_HIP(e, hipEventSynchronize(e->ev1));
+          float ms = 0.0f;
+          ENS_HIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+          if (rep > 0) total += ms;
+        }
+        if (total < best_ms) {
+          best_ms = total;
+          best = t;
+          best_grid = g;
+          best_whole = w;
+        }
       }
-    }
   }
+  for (int c = 0; c < n; ++c)
+    ENS_HIP(e, hipMemcpyAsync(e->cols[c]->light, saved[c], e->cols[c]->nvox * sizeof(float),
+                              hipMemcpyDeviceToDevice, e->stream));
+  ENS_HIP(e, hipStreamSynchronize(e->stream));
   drop(e);
   e->tile = best;
   e->per_tile_grid = best_grid;
+  e->whole = best_whole;
   if (tile_out) *tile_out = best;
   return MARL_OK;
 }
 
-int marl_ensemble_launch_shape(const marl_ensemble* e, int32_t* tile, int32_t* grid_blocks, int32_t* stencil_tiles) {
+int marl_ensemble_time_field_phase(marl_ensemble* e, const marl_sim_params* sim, int32_t n_ticks, float* ms_out) {
+  if (!e || !sim || n_ticks <= 0 || !ms_out || sim->diffusion_substeps <= 0) return MARL_ERR_INVALID_INPUT;
+  drop(e);
+  if (int rc = build_table(e)) return rc;
+  if (int rc = enqueue_field_phase(e, *sim, true)) return rc;  // warm-up
+  ENS_HIP(e, hipEventRecord(e->ev0, e->stream));
+  for (int t = 0; t < n_ticks; ++t)
+    if (int rc = enqueue_field_phase(e, *sim, true)) return rc;
+  ENS_HIP(e, hipEventRecord(e->ev1, e->stream));
+  ENS_HIP(e, hipEventSynchronize(e->ev1));
+  float ms = 0.0f;
+  ENS_HIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+  *ms_out = ms / (float)n_ticks;
+  return MARL_OK;
+}
+
+int marl_ensemble_set_whole_columns(marl_ensemble* e, int32_t whole) {
   if (!e) return MARL_ERR_INVALID_INPUT;
+  e->whole = whole != 0;
+  drop(e);
+  return MARL_OK;
+}
+
+int marl_ensemble_launch_shape(const marl_ensemble* e, int32_t* tile, int32_t* grid_blocks, int32_t* stencil_tiles,
+                               int32_t* whole_columns) {
+  if (!e) return MARL_ERR_INVALID_INPUT;
+  if (whole_columns) *whole_columns = e->whole;
   std::vector<EnsColumn> cols;
   int total = 0, light = 0, grid = 0;
   host_table(e, cols, &total, &light, &grid);

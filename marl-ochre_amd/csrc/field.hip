@@ -204,14 +204,14 @@ struct StencilLds {
 // One tile (tx, ty, tz) of block species group grp of one substep.  The
 // block has HB groups of 4 waves; each takes SG species of the block's HB*SG,
 // and all of them share the LDS plane (so a group without species 7 reads the
-// niche factor's species 7 from LDS when the block holds it).  With LIGHT
+// niche factor's species 7 from LDS when the block holds it).  With LM = 1
 // the tile also runs the light scan on the new field (the chunk must span the
 // whole column).
-template <int RY, bool SRC, int PD, int SG, int HB = 1, bool LIGHT = false>
+template <int RY, bool SRC, int PD, int SG, int HB = 1, int LM = 0>
 __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, float* __restrict__ dst,
                                              const uint32_t* __restrict__ occ, int X, int Y, int Z, int XW, int zc,
                                              int tx, int ty, int tz, int grp, const StencilParams& p,
-                                             StencilLds<RY, SG * HB>& sh) {
+                                             StencilLds<RY, SG * HB>& sh, float* __restrict__ light_out) {
   constexpr int TY = 4 * RY;
   constexpr int S = SG;              // species handled by this thread
   constexpr int LW = 66;
@@ -226,7 +226,7 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
   const bool blk7 = bs0 <= 7 && 7 < bs0 + SG * HB;  // species 7 is in the block's LDS
   // the fused light scan: the wave group holding species 4 carries each own
   // row's intensity down the column (the chunk is the whole column)
-  const bool lit = LIGHT && s0 <= 4 && 4 < s0 + SG;
+  const bool lit = LM != 0 && s0 <= 4 && 4 < s0 + SG;
 
   const int tid = threadIdx.x & 255;
   const int lx = tid & 63, wy = tid >> 6;
@@ -398,12 +398,18 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
         if (valid[r]) {
 #pragma unroll
           for (int s = 0; s < S; s += A) st_vec_nt<A>(rd, &out[s], vo[r], qoff(s0 + s, Qb) + zo);
-          // light.rs:41-55 on the new species 4 (slot 4 % SG, as for species 7)
+          // LM 1: light.rs:41-55 on the new species 4 (slot 4 % SG, as for
+          // species 7); LM 2: the new species 4 into the light plane, compact,
+          // for the separate light scan to read in place
           if (lit) {
-            p.light[(size_t)z * XY + (size_t)y * X + x] = I[r];
-            if (bc) I[r] *= p.l_e_cell;
-            I[r] *= marl_expf(p.l_nchem * out[4 % SG]);
-            if (I[r] < p.l_floor) I[r] = 0.0f;
+            if (LM == 1) {
+              light_out[(size_t)z * XY + (size_t)y * X + x] = I[r];
+              if (bc) I[r] *= p.l_e_cell;
+              I[r] *= marl_expf(p.l_nchem * out[4 % SG]);
+              if (I[r] < p.l_floor) I[r] = 0.0f;
+            } else {
+              light_out[(size_t)z * XY + (size_t)y * X + x] = out[4 % SG];
+            }
           }
         }
       }
@@ -412,7 +418,7 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
   }
 }
 
-template <int RY, bool SRC, int PD, int SG, int HB, bool LIGHT>
+template <int RY, bool SRC, int PD, int SG, int HB, int LM>
 __global__ void __launch_bounds__(256 * HB) k_stencil(const float* __restrict__ src, float* __restrict__ dst,
                                                       const uint32_t* __restrict__ occ, int X, int Y, int Z, int XW,
                                                       int zc, int ntx, int nty, int ntz, StencilParams p) {
@@ -422,8 +428,8 @@ __global__ void __launch_bounds__(256 * HB) k_stencil(const float* __restrict__ 
   // species group fastest: the groups of one tile run side by side on one XCD
   const int grp = b % NG;
   b /= NG;
-  stencil_tile<RY, SRC, PD, SG, HB, LIGHT>(src, dst, occ, X, Y, Z, XW, zc, b % ntx, (b / ntx) % nty,
-                                           b / (ntx * nty), grp, p, sh);
+  stencil_tile<RY, SRC, PD, SG, HB, LM>(src, dst, occ, X, Y, Z, XW, zc, b % ntx, (b / ntx) % nty,
+                                           b / (ntx * nty), grp, p, sh, p.light);
 }
 
 // Ensembles: one launch covers every column of the ensemble.  The logical
@@ -432,7 +438,9 @@ __global__ void __launch_bounds__(256 * HB) k_stencil(const float* __restrict__ 
 // The grid is either persistent (as many blocks as fit the chip, each walking
 // its share with a grid stride) or one block per tile, whichever the
 // ensemble's autotuner finds faster; the kernel is the same.
-template <int RY, bool SRC, int PD, int SG, int HB>
+// LM as for k_stencil, into each column's own light plane: 1 (every column's
+// z-chunk spans it) runs the light scan, 2 writes the compact species 4.
+template <int RY, bool SRC, int PD, int SG, int HB, int LM = 0>
 __global__ void __launch_bounds__(256 * HB) k_stencil_ens(const EnsColumn* __restrict__ cols, int ncols, int total,
                                                           int parity, StencilParams p) {
   __shared__ StencilLds<RY, SG * HB> sh;
@@ -455,8 +463,9 @@ __global__ void __launch_bounds__(256 * HB) k_stencil_ens(const EnsColumn* __res
     const int grp = b % NG;
     b /= NG;
     const int cur = col.cur ^ parity;
-    stencil_tile<RY, SRC, PD, SG, HB>(col.f[cur], col.f[cur ^ 1], col.occ, col.X, col.Y, col.Z, col.XW, col.zc,
-                                  b % col.ntx, (b / col.ntx) % col.nty, b / (col.ntx * col.nty), grp, p, sh);
+    stencil_tile<RY, SRC, PD, SG, HB, LM>(col.f[cur], col.f[cur ^ 1], col.occ, col.X, col.Y, col.Z, col.XW,
+                                             col.zc, b % col.ntx, (b / col.ntx) % col.nty, b / (col.ntx * col.nty),
+                                             grp, p, sh, col.light);
     __syncthreads();  // the next tile reuses the LDS
   }
 }
@@ -751,12 +760,16 @@ static hipError_t launch_stencil_t(marl_ctx* ctx, const StencilParams& p, int zc
   const int nb = ntx * nty * ntz * (MARL_S_EXT / (SG * HB));
   const float* src = ctx->field[ctx->cur];
   float* dst = ctx->field[ctx->cur ^ 1];
-  if (p.fuse_light) {
+  if (p.fuse_light == 1) {
     if (zc < Z || !p.light) return hipErrorInvalidValue;  // the scan needs whole columns
-    hipLaunchKernelGGL((k_stencil<RY, SRC, PD, SG, HB, true>), dim3(nb), dim3(256 * HB), 0, ctx->stream, src, dst,
+    hipLaunchKernelGGL((k_stencil<RY, SRC, PD, SG, HB, 1>), dim3(nb), dim3(256 * HB), 0, ctx->stream, src, dst,
+                       ctx->occ, X, Y, Z, ctx->xw, zc, ntx, nty, ntz, p);
+  } else if (p.fuse_light == 2) {
+    if (!p.light) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_stencil<RY, SRC, PD, SG, HB, 2>), dim3(nb), dim3(256 * HB), 0, ctx->stream, src, dst,
                        ctx->occ, X, Y, Z, ctx->xw, zc, ntx, nty, ntz, p);
   } else {
-    hipLaunchKernelGGL((k_stencil<RY, SRC, PD, SG, HB, false>), dim3(nb), dim3(256 * HB), 0, ctx->stream, src, dst,
+    hipLaunchKernelGGL((k_stencil<RY, SRC, PD, SG, HB, 0>), dim3(nb), dim3(256 * HB), 0, ctx->stream, src, dst,
                        ctx->occ, X, Y, Z, ctx->xw, zc, ntx, nty, ntz, p);
   }
   return hipGetLastError();
@@ -815,8 +828,10 @@ static bool tile_fuses_light(int tile, int Z) {
   return tile >= 0 && tile < kNumTiles && kTiles[tile].tb == 1 && kTiles[tile].zc >= Z;
 }
 
+// light_mode: what the last substep does for the tick's light scan (0
+// nothing, 1 the scan itself, 2 the compact species-4 copy the scan reads).
 static int substeps_impl(marl_ctx* ctx, const marl_sim_params* sim, int n, bool sources_first, int tile,
-                         int* launches, bool fuse_light) {
+                         int* launches, int light_mode) {
   if (tile < 0 || tile >= kNumTiles) tile = marl_default_tile(ctx->d);
   const TileShape t = kTiles[tile];
   int k = 0, nl = 0;
@@ -831,8 +846,8 @@ static int substeps_impl(marl_ctx* ctx, const marl_sim_params* sim, int n, bool 
   const int single = t.tb == 2 ? marl_default_tile(ctx->d) : tile;
   for (; k < n; ++k, ++nl) {
     StencilParams p = marl_stencil_params(sim, src ? 1 : 0);
-    if (fuse_light && k == n - 1) {
-      p.fuse_light = 1;
+    if (light_mode && k == n - 1) {
+      p.fuse_light = light_mode;
       p.l_i0 = sim->surface_intensity;
       p.l_e_cell = marl_expf(-sim->cell_absorption);
       p.l_nchem = -sim->chemical_absorption;
@@ -863,20 +878,31 @@ int marl_launch_field_phase(marl_ctx* ctx, const marl_sim_params* sim, bool sour
   // the last substep runs on `tile` itself unless it is a temporal tile (then
   // an odd remainder runs on the default tile, an even count has no single)
   const int last = kTiles[tile].tb == 2 ? (n % 2 ? marl_default_tile(ctx->d) : -1) : tile;
-  const bool fuse = tile_fuses_light(last, ctx->d.z);
-  if (int rc = substeps_impl(ctx, sim, n, sources, ctx->tile, nullptr, fuse)) return rc;
-  return fuse ? MARL_OK : marl_launch_light(ctx, sim);
+  const int mode = tile_fuses_light(last, ctx->d.z) ? 1 : (last >= 0 ? 2 : 0);
+  if (int rc = substeps_impl(ctx, sim, n, sources, ctx->tile, nullptr, mode)) return rc;
+  if (mode == 1) return MARL_OK;
+  return mode == 2 ? marl_launch_light_compact(ctx, sim) : marl_launch_light(ctx, sim);
 }
 
 template <int RY, int PD, int SG, int HB>
 static hipError_t launch_stencil_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total, int parity,
                                      const StencilParams& p, int grid) {
-  if (p.apply_sources)
-    hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG, HB>), dim3(grid), dim3(256 * HB), 0, st, dcols, ncols, total,
-                       parity, p);
-  else
-    hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG, HB>), dim3(grid), dim3(256 * HB), 0, st, dcols, ncols,
-                       total, parity, p);
+  const dim3 g(grid), b(256 * HB);
+  if (p.fuse_light == 1) {
+    if (p.apply_sources)
+      hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG, HB, 1>), g, b, 0, st, dcols, ncols, total, parity, p);
+    else
+      hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG, HB, 1>), g, b, 0, st, dcols, ncols, total, parity, p);
+  } else if (p.fuse_light == 2) {
+    if (p.apply_sources)
+      hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG, HB, 2>), g, b, 0, st, dcols, ncols, total, parity, p);
+    else
+      hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG, HB, 2>), g, b, 0, st, dcols, ncols, total, parity, p);
+  } else if (p.apply_sources) {
+    hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG, HB>), g, b, 0, st, dcols, ncols, total, parity, p);
+  } else {
+    hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG, HB>), g, b, 0, st, dcols, ncols, total, parity, p);
+  }
   return hipGetLastError();
 }
 
@@ -885,14 +911,17 @@ bool marl_tile_ensemble_ok(int tile) {
   return tile >= 0 && tile < kNumTiles && kTiles[tile].tb == 1;
 }
 
-void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int* total_light_blocks) {
+// whole_columns: every column's z-chunk is its whole depth (whatever the
+// tile's own chunk), which lets the last substep carry the light scan.
+void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int* total_light_blocks,
+                     bool whole_columns) {
   if (!marl_tile_ensemble_ok(tile)) tile = 7;
   const TileShape t = kTiles[tile];
   const int NG = MARL_S_EXT / (t.sg * (t.nw / 4));
   int base = 0, lbase = 0;
   for (int c = 0; c < ncols; ++c) {
     EnsColumn& e = cols[c];
-    e.zc = t.zc < e.Z ? t.zc : e.Z;
+    e.zc = (whole_columns || t.zc >= e.Z) ? e.Z : t.zc;
     e.ntx = (e.X + 63) / 64;
     e.nty = (e.Y + 4 * t.ry - 1) / (4 * t.ry);
     e.ntz = (e.Z + e.zc - 1) / e.zc;
@@ -964,9 +993,12 @@ int marl_launch_boundary(marl_ctx* ctx, const marl_sim_params* sim) {
 // sequential multiply order of the reference is kept per column.  s4 points at
 // species 4 of voxel 0 in the quad field (quad plane 1, lane 0): voxel v's
 // value is s4[4 v].
-__device__ __forceinline__ void light_column(const float* __restrict__ s4, const uint32_t* __restrict__ occ,
-                                             float* __restrict__ light, int x, int y, int X, int Y, int Z, int XW,
-                                             float I0, float e_cell, float nchem, float floor_) {
+// ST = 1: s4 is the compact copy the last substep wrote into the light plane
+// itself (s4 == light; each lane reads a plane's value before writing it).
+template <int ST = 4>
+__device__ __forceinline__ void light_column(const float* s4, const uint32_t* __restrict__ occ, float* light, int x,
+                                             int y, int X, int Y, int Z, int XW, float I0, float e_cell, float nchem,
+                                             float floor_) {
   const size_t XY = (size_t)X * Y;
   const size_t col = (size_t)y * X + x;
   const uint32_t bit = 1u << (x & 31);
@@ -979,7 +1011,7 @@ __device__ __forceinline__ void light_column(const float* __restrict__ s4, const
     uint32_t w[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      a[u] = s4[4 * ((size_t)(z + u) * XY + col)];
+      a[u] = s4[ST * ((size_t)(z + u) * XY + col)];
       w[u] = occ[(size_t)(z + u) * Y * XW + wcol];
     }
 #pragma unroll
@@ -991,22 +1023,25 @@ __device__ __forceinline__ void light_column(const float* __restrict__ s4, const
     }
   }
   for (; z < Z; ++z) {
+    const float a = s4[ST * ((size_t)z * XY + col)];  // read before the write: s4 may be the light plane
     light[(size_t)z * XY + col] = I;
     if (occ[(size_t)z * Y * XW + wcol] & bit) I *= e_cell;
-    I *= marl_expf(nchem * s4[4 * ((size_t)z * XY + col)]);
+    I *= marl_expf(nchem * a);
     if (I < floor_) I = 0.0f;
   }
 }
 
-__global__ void __launch_bounds__(64) k_light(const float* __restrict__ s4, const uint32_t* __restrict__ occ,
-                                              float* __restrict__ light, int X, int Y, int Z, int XW, float I0,
-                                              float e_cell, float nchem, float floor_) {
+template <int ST>
+__global__ void __launch_bounds__(64) k_light(const float* s4, const uint32_t* __restrict__ occ, float* light, int X,
+                                              int Y, int Z, int XW, float I0, float e_cell, float nchem,
+                                              float floor_) {
   const int x = blockIdx.x * 64 + threadIdx.x;
   if (x >= X) return;
-  light_column(s4, occ, light, x, blockIdx.y, X, Y, Z, XW, I0, e_cell, nchem, floor_);
+  light_column<ST>(s4, occ, light, x, blockIdx.y, X, Y, Z, XW, I0, e_cell, nchem, floor_);
 }
 
 // Ensembles: one launch over the (x, y) columns of every column of the ensemble.
+template <int ST>
 __global__ void __launch_bounds__(64) k_light_ens(const EnsColumn* __restrict__ cols, int ncols, float I0,
                                                   float e_cell, float nchem, float floor_) {
   int c = 0;
@@ -1017,15 +1052,19 @@ __global__ void __launch_bounds__(64) k_light_ens(const EnsColumn* __restrict__ 
   const int x = (local % e.lbx) * 64 + threadIdx.x;
   if (x >= e.X) return;
   const size_t nvox = (size_t)e.X * e.Y * e.Z;
-  light_column(e.f[e.cur] + 4 * nvox, e.occ, e.light, x, local / e.lbx, e.X, e.Y, e.Z, e.XW, I0, e_cell, nchem,
-               floor_);
+  light_column<ST>(ST == 1 ? e.light : e.f[e.cur] + 4 * nvox, e.occ, e.light, x, local / e.lbx, e.X, e.Y, e.Z, e.XW,
+                   I0, e_cell, nchem, floor_);
 }
 
 hipError_t marl_launch_light_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total_blocks,
-                                 const marl_sim_params* sim) {
+                                 const marl_sim_params* sim, bool compact) {
   const float e_cell = marl_expf(-sim->cell_absorption);
-  hipLaunchKernelGGL(k_light_ens, dim3(total_blocks), dim3(64), 0, st, dcols, ncols, sim->surface_intensity, e_cell,
-                     -sim->chemical_absorption, sim->light_floor);
+  if (compact)
+    hipLaunchKernelGGL(k_light_ens<1>, dim3(total_blocks), dim3(64), 0, st, dcols, ncols, sim->surface_intensity,
+                       e_cell, -sim->chemical_absorption, sim->light_floor);
+  else
+    hipLaunchKernelGGL(k_light_ens<4>, dim3(total_blocks), dim3(64), 0, st, dcols, ncols, sim->surface_intensity,
+                       e_cell, -sim->chemical_absorption, sim->light_floor);
   return hipGetLastError();
 }
 
@@ -1033,8 +1072,19 @@ int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim) {
   const int X = ctx->d.x, Y = ctx->d.y, Z = ctx->d.z;
   const float* s4 = ctx->field[ctx->cur] + (size_t)4 * ctx->nvox;
   const float e_cell = marl_expf(-sim->cell_absorption);
-  hipLaunchKernelGGL(k_light, dim3((X + 63) / 64, Y), dim3(64), 0, ctx->stream, s4, ctx->occ, ctx->light, X, Y, Z,
-                     ctx->xw, sim->surface_intensity, e_cell, -sim->chemical_absorption, sim->light_floor);
+  hipLaunchKernelGGL(k_light<4>, dim3((X + 63) / 64, Y), dim3(64), 0, ctx->stream, s4, ctx->occ, ctx->light, X, Y,
+                     Z, ctx->xw, sim->surface_intensity, e_cell, -sim->chemical_absorption, sim->light_floor);
+  MARL_HIP(ctx, hipGetLastError());
+  return MARL_OK;
+}
+
+// The light scan on the compact species 4 the tick's last substep left in the
+// light plane (read and overwritten in place).
+int marl_launch_light_compact(marl_ctx* ctx, const marl_sim_params* sim) {
+  const int X = ctx->d.x, Y = ctx->d.y, Z = ctx->d.z;
+  const float e_cell = marl_expf(-sim->cell_absorption);
+  hipLaunchKernelGGL(k_light<1>, dim3((X + 63) / 64, Y), dim3(64), 0, ctx->stream, ctx->light, ctx->occ, ctx->light,
+                     X, Y, Z, ctx->xw, sim->surface_intensity, e_cell, -sim->chemical_absorption, sim->light_floor);
   MARL_HIP(ctx, hipGetLastError());
   return MARL_OK;
 }

@@ -173,7 +173,7 @@ struct StencilParams {
   float src_ox, src_c, src_red, c_max;
   // light scan fused into the last substep of a tick (light.rs:29-59); only
   // for tiles whose z-chunk spans the whole column
-  int32_t fuse_light;
+  int32_t fuse_light;  // last substep's light work: 0 none, 1 the scan, 2 compact species 4 into the light plane
   float l_i0, l_e_cell, l_nchem, l_floor;
   float* light;
 };
@@ -309,13 +309,15 @@ int marl_launch_substeps(marl_ctx* ctx, const marl_sim_params* sim, int n, bool 
 int marl_launch_field_phase(marl_ctx* ctx, const marl_sim_params* sim, bool sources);
 int marl_launch_boundary(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim);
+int marl_launch_light_compact(marl_ctx* ctx, const marl_sim_params* sim);
 // Ensembles: fill the tile fields of `cols` for a tile id (returns total
 // logical tiles and light blocks), and launch one batched substep / light scan.
-void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int* total_light_blocks);
+void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int* total_light_blocks,
+                     bool whole_columns);
 hipError_t marl_launch_substep_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total_tiles, int parity,
                                    const StencilParams& p, int tile, int grid);
 hipError_t marl_launch_light_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total_blocks,
-                                 const marl_sim_params* sim);
+                                 const marl_sim_params* sim, bool compact);
 int marl_stencil_ens_grid(int tile, int total_tiles);
 
 // Device-driven tick pieces (cells.hip), shared by the single-column graph and

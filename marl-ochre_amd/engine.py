@@ -499,7 +499,19 @@ class MarlEnsemble:
         return t.value
 
     def launch_shape(self) -> dict:
-        """The batched stencil launch: tile id, grid blocks, logical tiles."""
-        t, g, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        self._chk(self.lib.marl_ensemble_launch_shape(self.h, ctypes.byref(t), ctypes.byref(g), ctypes.byref(n)))
-        return {"tile": t.value, "grid_blocks": g.value, "stencil_tiles": n.value}
+        """The batched stencil launch: tile id, grid blocks, logical tiles, z-chunk mode."""
+        t, g, n, w = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self.lib.marl_ensemble_launch_shape(self.h, ctypes.byref(t), ctypes.byref(g), ctypes.byref(n),
+                                                      ctypes.byref(w)))
+        return {"tile": t.value, "grid_blocks": g.value, "stencil_tiles": n.value, "whole_columns": bool(w.value)}
+
+    def time_field_phase(self, sim, n_ticks: int = 5) -> float:
+        """Milliseconds per batched field phase (substeps + light) of a tick."""
+        p = _params(sim)
+        ms = ctypes.c_float()
+        self._chk(self.lib.marl_ensemble_time_field_phase(self.h, ctypes.byref(p), int(n_ticks), ctypes.byref(ms)))
+        return ms.value
+
+    def set_whole_columns(self, whole: bool) -> None:
+        """Every column one z-chunk deep, the light scan fused into the last substep."""
+        self._chk(self.lib.marl_ensemble_set_whole_columns(self.h, 1 if whole else 0))

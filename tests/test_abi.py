@@ -99,7 +99,8 @@ def test_entry_points_validate_arguments_without_device(pkg):
     assert lib.marl_ensemble_create(arr, 2, ctypes.byref(out)) == A.ERR_INVALID_INPUT
     assert lib.marl_ensemble_tick(None, None, None, 0, 1, None, None, None) == A.ERR_INVALID_INPUT
     assert lib.marl_ensemble_set_grid(None, 1) == A.ERR_INVALID_INPUT
-    assert lib.marl_ensemble_launch_shape(None, None, None, None) == A.ERR_INVALID_INPUT
+    assert lib.marl_ensemble_launch_shape(None, None, None, None, None) == A.ERR_INVALID_INPUT
+    assert lib.marl_ensemble_set_whole_columns(None, 1) == A.ERR_INVALID_INPUT
     assert lib.marl_set_graph_limits(None, 8, 8) == A.ERR_INVALID_INPUT
     assert lib.marl_set_cell_layout(None, A.CELL_LAYOUT_AUTO) == A.ERR_INVALID_INPUT
     assert lib.marl_time_field_phase(None, None, 1, 0, None) == A.ERR_INVALID_INPUT

@@ -28,7 +28,7 @@ def _columns(pkg):
     return cols
 
 
-def _run(pkg, cols, sim, n_ticks, tile=None, per_tile_grid=False):
+def _run(pkg, cols, sim, n_ticks, tile=None, per_tile_grid=False, whole=False):
     seeds = np.array([c[4] for c in cols], np.uint64)
     alone = [pkg.MarlGpu(c[0]) for c in cols]
     batched = [pkg.MarlGpu(c[0]) for c in cols]
@@ -42,6 +42,7 @@ def _run(pkg, cols, sim, n_ticks, tile=None, per_tile_grid=False):
             if tile is not None:
                 ens.set_tile(tile)
             ens.set_grid(per_tile_grid)
+            ens.set_whole_columns(whole)
             for t in range(n_ticks):
                 stats, ms, hf = ens.tick(sim, seeds, t, 1)
                 assert ms > 0 and hf == 0
@@ -80,6 +81,14 @@ def test_ensemble_one_block_per_tile(pkg, tile):
     persistent grid: same kernel, same results."""
     cols = _columns(pkg)
     _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=2), 2, tile=tile, per_tile_grid=True)
+
+
+@pytest.mark.parametrize("tile,per_tile", [(0, False), (11, True), (15, False), (18, True)])
+def test_ensemble_whole_columns_with_fused_light(pkg, tile, per_tile):
+    """Every column one z-chunk deep, the light scan carried by the last
+    substep of the batched launch: same fields, light and cells."""
+    cols = _columns(pkg)
+    _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=2), 2, tile=tile, per_tile_grid=per_tile, whole=True)
 
 
 def test_ensemble_rejects_temporal_tiles(pkg):

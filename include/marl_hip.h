@@ -216,7 +216,7 @@ int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, in
  * MARL_CELL_LAYOUT_AOS: an AoS view (48-byte voxel records) transposed from
  * and back to the resident species quads around the pass; _QUADS: the
  * resident quads in place (no transposes, more cache lines per neighbour);
- * _AUTO (default): quads below about one cell per 13 voxels.  A tuning knob
+ * _AUTO (default): quads below about one cell per 10 voxels.  A tuning knob
  * with no counterpart in the reference (whose field is always AoS,
  * crates/marl-field/src/field.rs:15-35). */
 int marl_set_cell_layout(marl_ctx* ctx, int32_t layout);

@@ -1089,15 +1089,15 @@ int marl_launch_occupancy_from_cells(marl_ctx* ctx) {
 
 // Field layout for the plain cell pass.  The AoS view costs two whole-field
 // transposes (4 x 48 bytes moved per voxel, about 32 ps per voxel at
-// 256x256x128); the quads in place cost each cell about 0.43 ns more, since a
+// 256x256x128); the quads in place cost each cell about 0.33 ns more, since a
 // neighbour's species then span three cache lines instead of one or two
 // (both measured with 1M cells at 256x256x128).  The quads win below about one
-// cell per 13 voxels; a band of +-25 % around that keeps a captured tick from
+// cell per 10 voxels; a band of +-25 % around that keeps a captured tick from
 // switching layouts back and forth.
 bool marl_choose_cell_layout(marl_ctx* ctx) {
   if (ctx->cell_layout == MARL_CELL_LAYOUT_AOS) return ctx->kin_quads = false;
   if (ctx->cell_layout == MARL_CELL_LAYOUT_QUADS) return ctx->kin_quads = true;
-  const double r = 13.0 * (double)ctx->n_cells / (double)std::max<int64_t>(ctx->nvox, 1);
+  const double r = 10.0 * (double)ctx->n_cells / (double)std::max<int64_t>(ctx->nvox, 1);
   if (ctx->kin_quads && r > 1.25) ctx->kin_quads = false;
   else if (!ctx->kin_quads && r < 0.8) ctx->kin_quads = true;
   return ctx->kin_quads;

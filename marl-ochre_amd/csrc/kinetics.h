@@ -29,31 +29,6 @@ __device__ inline uint16_t f32_as_u16(float f) {  // Rust `as u16`: truncate, sa
   return (uint16_t)f;
 }
 
-// The parts of a ruleset the tick reads (transporters .. fate thresholds; the
-// receptors' result is unused), copied into registers with 29 independent
-// 16-byte loads at the start of a cell's update so that their latency is paid
-// once instead of once per reaction.
-struct RuleRegs {
-  marl_transport transport[MARL_S_TRANSPORTERS];
-  marl_reaction reactions[MARL_R_MAX];
-  marl_effector effectors[MARL_S_EFFECTORS];
-  float division_energy, death_energy, quiescence_energy, division_prep_ticks;
-};
-static_assert(sizeof(RuleRegs) == 464 && sizeof(RuleRegs) % 16 == 0, "RuleRegs layout");
-static_assert(offsetof(marl_ruleset, transport) == 96 && offsetof(marl_ruleset, hgt_propensity) == 96 + 464,
-              "RuleRegs mirrors marl_ruleset from the transporters on");
-
-__device__ __forceinline__ RuleRegs load_rules(const marl_ruleset* r) {
-  constexpr int N = sizeof(RuleRegs) / 16;
-  const uint4* s = reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(r) + 96);
-  uint4 buf[N];
-#pragma unroll
-  for (int k = 0; k < N; ++k) buf[k] = s[k];
-  RuleRegs out;
-  __builtin_memcpy(&out, buf, sizeof(out));
-  return out;
-}
-
 __device__ __forceinline__ marl_cell_state load_cell(const marl_cell_state* c) {
   constexpr int N = sizeof(marl_cell_state) / 16;
   const uint4* s = reinterpret_cast<const uint4*>(c);
@@ -276,6 +251,29 @@ struct RegArr {
   }
 };
 
+// The ruleset's transporter, reaction and effector records are read in
+// batches of kRuleBatch: a batch's loads are issued together, before any of its
+// records is used, so a cell's update waits for one memory round trip per
+// batch instead of one per record (the rules are read-only during a pass).
+#ifndef MARL_KIN_BATCH
+#define MARL_KIN_BATCH 16
+#endif
+constexpr int kRuleBatch = MARL_KIN_BATCH;
+
+template <int N, typename T, typename Load, typename Body>
+__device__ __forceinline__ void batched_loop(Load load, Body body) {
+  constexpr int B = kRuleBatch < 1 ? 1 : (kRuleBatch < N ? kRuleBatch : N);
+  static_assert(N % B == 0, "batch divides the record count");
+#pragma unroll
+  for (int q0 = 0; q0 < N; q0 += B) {
+    T buf[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) buf[k] = load(q0 + k);
+#pragma unroll
+    for (int k = 0; k < B; ++k) body(q0 + k, buf[k]);
+  }
+}
+
 // Phases 2-4 of CellState::tick (cell.rs:223-329) on I[16] (updated in place)
 // with environment E[12], accumulating exchange fluxes into D[12].  The
 // receptor pass (:206-220) only writes an unused array and is omitted.
@@ -286,29 +284,27 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const R& r, AI& I
 #pragma unroll
   for (int s = 0; s < MARL_S_EXT; ++s) D.set(s, 0.0f);
   // transport (cell.rs:223-238)
-#pragma unroll
-  for (int q = 0; q < MARL_S_TRANSPORTERS; ++q) {
-    const marl_transport tp = r.transport(q);
+  batched_loop<MARL_S_TRANSPORTERS, marl_transport>([&](int q) { return r.transport(q); },
+                                                       [&](int, const marl_transport& tp) {
     const int e = tp.ext_species, k = tp.int_species;
-    if (e >= MARL_S_EXT || k >= MARL_M_INT) continue;
+    if (e >= MARL_S_EXT || k >= MARL_M_INT) return;
     const float ex = E.get(e);
     const float ik = I.get(k);
     const float up = tp.uptake_rate * ex / (1.0f + ex);
     const float sec = tp.secrete_rate * ik / (1.0f + ik);
     I.set(k, clamp_rs(ik + (up - sec) * dt, 0.0f, p.c_max));
     D.set(e, D.get(e) + (sec - up) * dt);
-  }
+  });
   I.set(MARL_M_INT - 1, lt);  // :243
   // reactions (cell.rs:246-285)
   int active = 0;
-#pragma unroll
-  for (int q = 0; q < MARL_R_MAX; ++q) {
-    const marl_reaction rx = r.reaction(q);
+  batched_loop<MARL_R_MAX, marl_reaction>([&](int q) { return r.reaction(q); },
+                                             [&](int, const marl_reaction& rx) {
     if (fabsf(rx.v_max) > p.active_reaction_threshold && rx.substrate != rx.product) ++active;
-    if (fabsf(rx.v_max) < p.active_reaction_threshold) continue;
-    if (rx.substrate == rx.product) continue;
+    if (fabsf(rx.v_max) < p.active_reaction_threshold) return;
+    if (rx.substrate == rx.product) return;
     const int si = rx.substrate, pi = rx.product, ci = rx.catalyst;
-    if (si >= MARL_M_INT || pi >= MARL_M_INT || ci >= MARL_M_INT) continue;
+    if (si >= MARL_M_INT || pi >= MARL_M_INT || ci >= MARL_M_INT) return;
     const float s = I.get(si);
     const float cc = I.get(ci);
     const float st = s / (rx.k_m + s + 1.1920929e-07f);
@@ -326,7 +322,7 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const R& r, AI& I
     const float flux = fminf(rate * dt, I.get(si));
     I.set(si, I.get(si) - flux);
     I.set(pi, fminf(I.get(pi) + flux, p.c_max));
-  }
+  });
   // maintenance (cell.rs:291-311)
   float pm = 1.0f;
   if (c.prep_remaining > 0) {
@@ -340,18 +336,17 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const R& r, AI& I
   I.set(0, e0);
   // effectors (cell.rs:314-329)
   if (!c.quiescent) {
-#pragma unroll
-    for (int q = 0; q < MARL_S_EFFECTORS; ++q) {
-      const marl_effector ef = r.effector(q);
+    batched_loop<MARL_S_EFFECTORS, marl_effector>([&](int q) { return r.effector(q); },
+                                                     [&](int, const marl_effector& ef) {
       const int k = ef.int_species, e = ef.ext_species;
-      if (k >= MARL_M_INT || e >= MARL_S_EXT) continue;
+      if (k >= MARL_M_INT || e >= MARL_S_EXT) return;
       const float ik = I.get(k);
       if (ik > ef.threshold) {
         const float amount = ef.rate * ik / (1.0f + ik) * dt;
         I.set(k, fmaxf(ik - amount, 0.0f));
         D.set(e, D.get(e) + amount);
       }
-    }
+    });
   }
 }
 

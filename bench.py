@@ -455,6 +455,8 @@ def ensemble_bench(pkg, ticks=5, warm=2):
         with pkg.MarlEnsemble(gs) as ens:
             tile = ens.autotune(sim)
             shape = ens.launch_shape()
+            shape["chunks_by_depth"] = {str(d[2]): z for (d, *_), z in sorted(zip(cols, ens.chunks()),
+                                                                                key=lambda t: t[0][0][2])}
             ms_sub = ens.time_field(sim, 20)
             ms_phase = ens.time_field_phase(sim, 5)
             for g, (dims, field, cells, rules, _) in zip(gs, cols):

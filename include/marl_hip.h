@@ -365,8 +365,14 @@ int marl_ensemble_tick(marl_ensemble* e, const marl_sim_params* sim, const uint6
                        int32_t n_ticks, marl_tick_stats* stats, int32_t* host_finished, float* ms_out);
 int marl_ensemble_set_tile(marl_ensemble* e, int32_t tile);
 /* Grid of the batched stencil launch: 0 = persistent (as many blocks as fit
- * the chip, each walking its share of the tiles), 1 = one block per tile. */
+ * the chip, each walking its share of the tiles), 1 = one block per tile,
+ * 2 = persistent over balanced pieces (a multiple of 8 blocks; per XCD a run
+ * of neighbouring tiles of equal work, each tile marching its column's whole
+ * depth except in the last, partial round, whose tiles are cut into z-chunks;
+ * the z-chunk settings then do not apply and the light scan is its own
+ * launch).  Results are identical in every mode. */
 int marl_ensemble_set_grid(marl_ensemble* e, int32_t per_tile);
+int marl_ensemble_get_grid(const marl_ensemble* e, int32_t* mode);
 /* Average time of one batched substep over all columns (fields unchanged). */
 int marl_ensemble_time_field(marl_ensemble* e, const marl_sim_params* sim, int32_t n_substeps, float* ms_out);
 /* Average time of a tick's batched field phase (the substeps, sources in the
@@ -378,13 +384,23 @@ int marl_ensemble_time_field_phase(marl_ensemble* e, const marl_sim_params* sim,
  * chunk deep, which lets the last substep of the tick carry each column's
  * light scan (no separate light launch).  Results are identical. */
 int marl_ensemble_set_whole_columns(marl_ensemble* e, int32_t whole);
+/* Per-column z-chunk lengths when not whole columns: chunks[c] planes for
+ * column c (0 = the tile shape's chunk), cut at the column's depth; n = the
+ * column count, or 0 to clear.  Results are identical for every choice. */
+int marl_ensemble_set_chunks(marl_ensemble* e, const int32_t* chunks, int32_t n);
+/* The z-chunk each column's batched stencil tiles use in grid modes 0 and 1
+ * (n = the column count). */
+int marl_ensemble_get_chunks(const marl_ensemble* e, int32_t* chunks, int32_t n);
 /* Times a tick's batched field phase (substeps and light) for every stencil
- * tile shape, with a persistent grid and with one block per tile, and with
- * both z-chunk modes, on the resident columns and keeps the fastest (fields
- * and light planes are left unchanged). */
+ * tile shape, with a persistent grid (both z-chunk modes), one block per tile
+ * (both z-chunk modes) and balanced pieces, on the resident columns; then, for the three fastest
+ * shapes without whole columns, the z-chunk of each group of equally deep
+ * columns (4, 8, 16, 24, 32, 48, 64 planes or the whole depth, deepest group
+ * first); keeps the fastest.  Fields and light planes are left unchanged. */
 int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t* tile_out);
-/* The batched stencil launch: tile shape, grid blocks, logical tiles and the
- * z-chunk mode (each may be NULL). */
+/* The batched stencil launch: tile shape, grid blocks, logical tiles (the
+ * pieces in grid mode 2) and whether the light scan rides the last substep
+ * (each may be NULL). */
 int marl_ensemble_launch_shape(const marl_ensemble* e, int32_t* tile, int32_t* grid_blocks, int32_t* stencil_tiles,
                                int32_t* whole_columns);
 

@@ -273,6 +273,9 @@ SIGNATURES = {
     "marl_ensemble_time_field_phase": (ctypes.c_int, [_P, ctypes.POINTER(marl_sim_params), _I32,
                                                       ctypes.POINTER(ctypes.c_float)]),
     "marl_ensemble_set_whole_columns": (ctypes.c_int, [_P, _I32]),
+    "marl_ensemble_set_chunks": (ctypes.c_int, [_P, ctypes.POINTER(_I32), _I32]),
+    "marl_ensemble_get_grid": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
+    "marl_ensemble_get_chunks": (ctypes.c_int, [_P, ctypes.POINTER(_I32), _I32]),
     "marl_set_kinetics_mode": (ctypes.c_int, [_P, _I32]),
     "marl_harvest_begin": (ctypes.c_int, [_P, _I64, ctypes.c_float]),
     "marl_harvest_end": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),

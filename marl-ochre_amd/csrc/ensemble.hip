@@ -11,7 +11,9 @@
 //   - fork: each column's cell phase (schedule, ordered pass, fate, compaction)
 //     on its own stream, so small columns run side by side; join.
 // Results are bit-identical to ticking each column alone.
+#include <algorithm>
 #include <chrono>
+#include <functional>
 #include <cstring>
 #include <vector>
 
@@ -26,8 +28,11 @@ struct marl_ensemble {
   std::vector<EnsColumn> hcols;
   EnsColumn* dcols = nullptr;
   int tile = 7;
-  int per_tile_grid = 0;  // 1: one block per stencil tile; 0: persistent grid
+  int per_tile_grid = 0;  // 0: persistent grid; 1: one block per stencil tile; 2: persistent, balanced pieces
+  uint32_t* dpieces = nullptr;  // device piece table of grid mode 2
+  size_t dpieces_cap = 0;       // its capacity in words
   int whole = 0;          // 1: every column's z-chunk is its whole depth and the light scan rides the last substep
+  std::vector<int> chunks;  // per column: z-chunk planes (0: the tile shape's), when not whole
   int total_tiles = 0, total_light = 0, grid = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -86,8 +91,15 @@ void host_table(const marl_ensemble* e, std::vector<EnsColumn>& cols, int* total
     t.XW = x->xw;
     t.cur = x->cur;
   }
-  marl_ens_layout(e->tile, cols.data(), n, total_tiles, total_light, e->whole != 0);
-  *grid = e->per_tile_grid ? *total_tiles : marl_stencil_ens_grid(e->tile, *total_tiles);
+  const bool pieces = e->per_tile_grid == 2;
+  marl_ens_layout(e->tile, cols.data(), n, total_tiles, total_light, e->whole != 0 && !pieces,
+                  (int)e->chunks.size() == n && !pieces ? e->chunks.data() : nullptr);
+  if (pieces) {  // a multiple of 8 blocks, at least one per XCD
+    const int g = marl_stencil_ens_grid(e->tile, 1 << 30);
+    *grid = std::max(8, g / 8 * 8);
+  } else {
+    *grid = e->per_tile_grid ? *total_tiles : marl_stencil_ens_grid(e->tile, *total_tiles);
+  }
 }
 
 // ... and its device copy.
@@ -96,9 +108,27 @@ int build_table(marl_ensemble* e) {
   host_table(e, e->hcols, &e->total_tiles, &e->total_light, &e->grid);
   if (!e->dcols) ENS_HIP(e, hipMalloc(&e->dcols, sizeof(EnsColumn) * n));
   ENS_HIP(e, hipMemcpyAsync(e->dcols, e->hcols.data(), sizeof(EnsColumn) * n, hipMemcpyHostToDevice, e->stream));
+  if (e->per_tile_grid == 2) {
+    std::vector<uint32_t> pieces;
+    marl_ens_pieces(e->tile, e->hcols.data(), n, e->grid, pieces);
+    if (pieces.size() > e->dpieces_cap) {
+      ENS_HIP(e, hipStreamSynchronize(e->stream));
+      if (e->dpieces) (void)hipFree(e->dpieces);
+      e->dpieces = nullptr;
+      e->dpieces_cap = 0;
+      ENS_HIP(e, hipMalloc(&e->dpieces, pieces.size() * sizeof(uint32_t)));
+      e->dpieces_cap = pieces.size();
+    }
+    ENS_HIP(e, hipMemcpyAsync(e->dpieces, pieces.data(), pieces.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              e->stream));
+    ENS_HIP(e, hipStreamSynchronize(e->stream));  // `pieces` must outlive the copy
+  }
   ENS_HIP(e, hipStreamSynchronize(e->stream));
   return MARL_OK;
 }
+
+// The piece table the batched stencil walks, or nullptr.
+const uint32_t* launch_pieces(const marl_ensemble* e) { return e->per_tile_grid == 2 ? e->dpieces : nullptr; }
 
 // The buffers of the batched counter transfers, rebuilt when a column's
 // device counters moved (they are allocated by the column's first prepare).
@@ -129,8 +159,16 @@ std::vector<uint8_t> ens_key(marl_ensemble* e, const std::vector<marl_sim_params
   k.push_back((uint8_t)e->tile);
   k.push_back((uint8_t)e->per_tile_grid);
   k.push_back((uint8_t)e->whole);
+  for (int z : e->chunks) {  // chunk lengths fit 16 bits (a field is under 2 GiB)
+    k.push_back((uint8_t)(z & 0xff));
+    k.push_back((uint8_t)((z >> 8) & 0xff));
+  }
   return k;
 }
+
+// The light scan rides the last substep when every column is one z-chunk
+// deep (whole columns; the balanced pieces may cut a column into chunks).
+bool whole_light(const marl_ensemble* e) { return e->whole && e->per_tile_grid != 2; }
 
 // The batched field phase of one tick: the substeps (sources fused into the
 // first) and the light scan, fused into the last substep when every column is
@@ -142,16 +180,16 @@ int enqueue_field_phase(marl_ensemble* e, const marl_sim_params& sim, bool parit
   for (int k = 0; k < S; ++k) {
     StencilParams p = marl_stencil_params(&sim, k == 0 ? 1 : 0);
     if (k == S - 1) {  // the light scan itself, or the compact species 4 the light launch reads
-      p.fuse_light = e->whole ? 1 : 2;
+      p.fuse_light = whole_light(e) ? 1 : 2;
       p.l_i0 = sim.surface_intensity;
       p.l_e_cell = marl_expf(-sim.cell_absorption);
       p.l_nchem = -sim.chemical_absorption;
       p.l_floor = sim.light_floor;
     }
     ENS_HIP(e, marl_launch_substep_ens(e->stream, e->dcols, n, e->total_tiles, parity_fixed ? 0 : (k & 1), p,
-                                       e->tile, e->grid));
+                                       e->tile, e->grid, launch_pieces(e)));
   }
-  if (!e->whole) ENS_HIP(e, marl_launch_light_ens(e->stream, e->dcols, n, e->total_light, &sim, true));
+  if (!whole_light(e)) ENS_HIP(e, marl_launch_light_ens(e->stream, e->dcols, n, e->total_light, &sim, true));
   return MARL_OK;
 }
 
@@ -239,6 +277,7 @@ void marl_ensemble_destroy(marl_ensemble* e) {
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   drop(e);
   if (e->dcols) (void)hipFree(e->dcols);
+  if (e->dpieces) (void)hipFree(e->dpieces);
   if (e->d_tdevs) (void)hipFree(e->d_tdevs);
   if (e->d_counters) (void)hipFree(e->d_counters);
   if (e->h_counters) (void)hipHostFree(e->h_counters);
@@ -266,7 +305,7 @@ int marl_ensemble_set_tile(marl_ensemble* e, int32_t tile) {
 
 int marl_ensemble_set_grid(marl_ensemble* e, int32_t per_tile) {
   if (!e) return MARL_ERR_INVALID_INPUT;
-  if (per_tile != 0 && per_tile != 1) return efail(e, MARL_ERR_INVALID_INPUT, "grid mode is 0 or 1");
+  if (per_tile < 0 || per_tile > 2) return efail(e, MARL_ERR_INVALID_INPUT, "grid mode is 0, 1 or 2");
   if (per_tile != e->per_tile_grid) drop(e);
   e->per_tile_grid = per_tile;
   return MARL_OK;
@@ -322,10 +361,12 @@ int marl_ensemble_time_field(marl_ensemble* e, const marl_sim_params* sim, int32
   const StencilParams sp = marl_stencil_params(sim, 0);
   // every launch reads each column's current buffer and writes the other one,
   // so the resident fields are left unchanged
-  ENS_HIP(e, marl_launch_substep_ens(e->stream, e->dcols, n, e->total_tiles, 0, sp, e->tile, e->grid));
+  ENS_HIP(e, marl_launch_substep_ens(e->stream, e->dcols, n, e->total_tiles, 0, sp, e->tile, e->grid,
+                                     launch_pieces(e)));
   ENS_HIP(e, hipEventRecord(e->ev0, e->stream));
   for (int k = 0; k < n_substeps; ++k)
-    ENS_HIP(e, marl_launch_substep_ens(e->stream, e->dcols, n, e->total_tiles, 0, sp, e->tile, e->grid));
+    ENS_HIP(e, marl_launch_substep_ens(e->stream, e->dcols, n, e->total_tiles, 0, sp, e->tile, e->grid,
+                                       launch_pieces(e)));
   ENS_HIP(e, hipEventRecord(e->ev1, e->stream));
   ENS_HIP(e, hipEventSynchronize(e->ev1));
   float ms = 0.0f;
@@ -334,9 +375,32 @@ int marl_ensemble_time_field(marl_ensemble* e, const marl_sim_params* sim, int32
   return MARL_OK;
 }
 
+namespace {
+
+// Two timed field phases of the current launch shape after a warm-up, summed
+// (every launch reads the current buffers; the light planes are rewritten).
+int time_phase(marl_ensemble* e, const marl_sim_params& sim, float* ms_out) {
+  drop(e);
+  if (int rc = build_table(e)) return rc;
+  float total = 0.0f;
+  for (int rep = 0; rep < 3; ++rep) {  // the first is a warm-up
+    ENS_HIP(e, hipEventRecord(e->ev0, e->stream));
+    if (int rc = enqueue_field_phase(e, sim, true)) return rc;
+    ENS_HIP(e, hipEventRecord(e->ev1, e->stream));
+    ENS_HIP(e, hipEventSynchronize(e->ev1));
+    float ms = 0.0f;
+    ENS_HIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    if (rep > 0) total += ms;
+  }
+  *ms_out = total;
+  return MARL_OK;
+}
+
+}  // namespace
+
 // The ensemble's tile shape and grid (persistent or one block per tile),
 // picked by timing every candidate on the resident columns (the fields are
-// left unchanged).
+// left unchanged), then the z-chunk of each group of columns of one depth.
 int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t* tile_out) {
   if (!e || !sim) return MARL_ERR_INVALID_INPUT;
   if (sim->diffusion_substeps <= 0) return efail(e, MARL_ERR_INVALID_INPUT, "autotune needs substeps");
@@ -355,44 +419,78 @@ int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t
     ENS_HIP(e, hipMalloc(&saved[c], bytes));
     ENS_HIP(e, hipMemcpyAsync(saved[c], e->cols[c]->light, bytes, hipMemcpyDeviceToDevice, e->stream));
   }
-  int best = e->tile, best_grid = e->per_tile_grid, best_whole = e->whole;
-  float best_ms = 1e30f;
+  // Every tile shape, grid mode and chunk mode with the tile shape's chunks.
+  struct Cand {
+    float ms;
+    int tile, grid, whole;
+  };
+  std::vector<Cand> cands;
+  e->chunks.clear();
   for (int t = 0; t < marl_stencil_tile_count(); ++t) {
     if (!marl_tile_ensemble_ok(t)) continue;
-    for (int g = 0; g < 2; ++g)
-      for (int w = 0; w < 2; ++w) {
+    for (int g = 0; g < 3; ++g)
+      for (int w = 0; w < (g == 2 ? 1 : 2); ++w) {
         e->tile = t;
         e->per_tile_grid = g;
         e->whole = w;
-        drop(e);
-        if (int rc = build_table(e)) return rc;
-        float total = 0.0f;
-        for (int rep = 0; rep < 3; ++rep) {  // the first is a warm-up
-          ENS_HIP(e, hipEventRecord(e->ev0, e->stream));
-          if (int rc = enqueue_field_phase(e, *sim, true)) return rc;
-          ENS_HIP(e, hipEventRecord(e->ev1, e->stream));
-          ENS_HIP(e, hipEventSynchronize(e->ev1));
-          float ms = 0.0f;
-          ENS_HIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
-          if (rep > 0) total += ms;
-        }
-        if (total < best_ms) {
-          best_ms = total;
-          best = t;
-          best_grid = g;
-          best_whole = w;
-        }
+        float ms = 0.0f;
+        if (int rc = time_phase(e, *sim, &ms)) return rc;
+        cands.push_back({ms, t, g, w});
       }
   }
+  std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.ms < b.ms; });
+  Cand best = cands.front();
+  std::vector<int> best_chunks(n, 0);
+  // Then, for the three fastest shapes without whole columns, the z-chunk of
+  // each group of equally deep columns (deepest first, one pass): a column's
+  // chunk trades the halo planes every chunk re-reads against how evenly the
+  // tiles fill the chip.
+  std::vector<int> depths;
+  for (marl_ctx* x : e->cols) depths.push_back(x->d.z);
+  std::sort(depths.begin(), depths.end(), std::greater<int>());
+  depths.erase(std::unique(depths.begin(), depths.end()), depths.end());
+  const int opts[] = {4, 8, 16, 24, 32, 48, 64};
+  int tried = 0;
+  for (const Cand& c0 : cands) {
+    if (c0.whole || c0.grid == 2 || tried == 3) continue;  // the pieces set their own chunks
+    ++tried;
+    e->tile = c0.tile;
+    e->per_tile_grid = c0.grid;
+    e->whole = 0;
+    e->chunks.assign(n, 0);
+    float cur_ms = c0.ms;
+    for (int depth : depths) {
+      int keep_zc = 0;
+      for (int k = -1; k < (int)(sizeof(opts) / sizeof(opts[0])); ++k) {
+        const int zc = k < 0 ? depth : opts[k];  // k = -1: one chunk per column
+        if (k >= 0 && zc >= depth) continue;
+        for (int c = 0; c < n; ++c)
+          if (e->cols[c]->d.z == depth) e->chunks[c] = zc;
+        float ms = 0.0f;
+        if (int rc = time_phase(e, *sim, &ms)) return rc;
+        if (ms < cur_ms) {
+          cur_ms = ms;
+          keep_zc = zc;
+        }
+      }
+      for (int c = 0; c < n; ++c)
+        if (e->cols[c]->d.z == depth) e->chunks[c] = keep_zc;
+    }
+    if (cur_ms < best.ms) {
+      best = {cur_ms, c0.tile, c0.grid, 0};
+      best_chunks = e->chunks;
+    }
+  }
+  e->tile = best.tile;
+  e->per_tile_grid = best.grid;
+  e->whole = best.whole;
+  e->chunks = best.whole ? std::vector<int>(n, 0) : best_chunks;
   for (int c = 0; c < n; ++c)
     ENS_HIP(e, hipMemcpyAsync(e->cols[c]->light, saved[c], e->cols[c]->nvox * sizeof(float),
                               hipMemcpyDeviceToDevice, e->stream));
   ENS_HIP(e, hipStreamSynchronize(e->stream));
   drop(e);
-  e->tile = best;
-  e->per_tile_grid = best_grid;
-  e->whole = best_whole;
-  if (tile_out) *tile_out = best;
+  if (tile_out) *tile_out = best.tile;
   return MARL_OK;
 }
 
@@ -419,16 +517,49 @@ int marl_ensemble_set_whole_columns(marl_ensemble* e, int32_t whole) {
   return MARL_OK;
 }
 
-int marl_ensemble_launch_shape(const marl_ensemble* e, int32_t* tile, int32_t* grid_blocks, int32_t* stencil_tiles,
-                               int32_t* whole_columns) {
+int marl_ensemble_set_chunks(marl_ensemble* e, const int32_t* chunks, int32_t n) {
   if (!e) return MARL_ERR_INVALID_INPUT;
-  if (whole_columns) *whole_columns = e->whole;
+  const int nc = (int)e->cols.size();
+  if (n != 0 && (n != nc || !chunks))
+    return efail(e, MARL_ERR_INVALID_INPUT, "marl_ensemble_set_chunks: one entry per column, or n = 0");
+  for (int c = 0; c < n; ++c)
+    if (chunks[c] < 0 || chunks[c] > 65535) return efail(e, MARL_ERR_INVALID_INPUT, "z-chunk out of range");
+  if (n == 0) e->chunks.assign(nc, 0);
+  else e->chunks.assign(chunks, chunks + n);
+  drop(e);
+  return MARL_OK;
+}
+
+int marl_ensemble_get_chunks(const marl_ensemble* e, int32_t* chunks, int32_t n) {
+  if (!e || !chunks || n != (int)e->cols.size()) return MARL_ERR_INVALID_INPUT;
   std::vector<EnsColumn> cols;
   int total = 0, light = 0, grid = 0;
   host_table(e, cols, &total, &light, &grid);
+  for (int c = 0; c < n; ++c) chunks[c] = cols[c].zc;
+  return MARL_OK;
+}
+
+int marl_ensemble_launch_shape(const marl_ensemble* e, int32_t* tile, int32_t* grid_blocks, int32_t* stencil_tiles,
+                               int32_t* whole_columns) {
+  if (!e) return MARL_ERR_INVALID_INPUT;
+  if (whole_columns) *whole_columns = whole_light(e);
+  std::vector<EnsColumn> cols;
+  int total = 0, light = 0, grid = 0;
+  host_table(e, cols, &total, &light, &grid);
+  if (e->per_tile_grid == 2) {  // the units of the walk are the pieces
+    std::vector<uint32_t> pieces;
+    marl_ens_pieces(e->tile, cols.data(), (int)cols.size(), grid, pieces);
+    total = (int)pieces[8];
+  }
   if (tile) *tile = e->tile;
   if (grid_blocks) *grid_blocks = grid;
   if (stencil_tiles) *stencil_tiles = total;
+  return MARL_OK;
+}
+
+int marl_ensemble_get_grid(const marl_ensemble* e, int32_t* mode) {
+  if (!e || !mode) return MARL_ERR_INVALID_INPUT;
+  *mode = e->per_tile_grid;
   return MARL_OK;
 }
 

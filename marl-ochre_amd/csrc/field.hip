@@ -6,6 +6,9 @@
 //             crates/marl-gpu/src/shaders/field_diffuse.wgsl:55-95)
 //   sources   crates/marl-field/src/field.rs:231-250
 //   light     crates/marl-field/src/light.rs:29-59
+#include <algorithm>
+#include <vector>
+
 #include "marl_internal.h"
 
 // ---------------------------------------------------------------------------
@@ -150,23 +153,34 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // buffer load to one dword when its elements are re-packed into a vector.
 // Offsets stay inside the field (planes are clamped, out-of-grid lanes read
 // voxel 0), so no buffer range check is needed.
+// The field pointers are cast to the global address space: a pointer that
+// reaches the kernel through memory (an ensemble's column table) would
+// otherwise be accessed with FLAT instructions, which also count against the
+// LDS wait counter, so every LDS wait of the z-loop would wait for the
+// prefetches in flight as well.
+#define MARL_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const MARL_GLOBAL T* as_global(const void* p) {
+  return (const MARL_GLOBAL T*)(p);
+}
+
 template <bool SRC, int A>
 __device__ __forceinline__ void ld_vec(const float* __restrict__ base, const StencilParams& p, int s, uint32_t Qb,
                                        uint32_t XYq, int Z, uint32_t voff, int zz, float* out) {
   const uint32_t so = qoff(s, Qb) + (uint32_t)zz * XYq;
   const char* a = reinterpret_cast<const char*>(base) + so;
   if constexpr (A == 4) {
-    const f32x4 t = *reinterpret_cast<const f32x4*>(a + voff);
+    const f32x4 t = *as_global<f32x4>(a + voff);
     out[0] = t.x;
     out[1] = t.y;
     out[2] = t.z;
     out[3] = t.w;
   } else if constexpr (A == 2) {
-    const f32x2 t = *reinterpret_cast<const f32x2*>(a + voff);
+    const f32x2 t = *as_global<f32x2>(a + voff);
     out[0] = t.x;
     out[1] = t.y;
   } else {
-    out[0] = *reinterpret_cast<const float*>(a + voff);
+    out[0] = *as_global<float>(a + voff);
   }
 #pragma unroll
   for (int k = 0; k < A; ++k) out[k] = with_sources<SRC>(out[k], p, s + k, Z, zz);
@@ -402,13 +416,14 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
           // species 7); LM 2: the new species 4 into the light plane, compact,
           // for the separate light scan to read in place
           if (lit) {
+            MARL_GLOBAL float* lo = (MARL_GLOBAL float*)light_out;
             if (LM == 1) {
-              light_out[(size_t)z * XY + (size_t)y * X + x] = I[r];
+              lo[(size_t)z * XY + (size_t)y * X + x] = I[r];
               if (bc) I[r] *= p.l_e_cell;
               I[r] *= marl_expf(p.l_nchem * out[4 % SG]);
               if (I[r] < p.l_floor) I[r] = 0.0f;
             } else {
-              light_out[(size_t)z * XY + (size_t)y * X + x] = out[4 % SG];
+              lo[(size_t)z * XY + (size_t)y * X + x] = out[4 % SG];
             }
           }
         }
@@ -442,30 +457,54 @@ __global__ void __launch_bounds__(256 * HB) k_stencil(const float* __restrict__ 
 // z-chunk spans it) runs the light scan, 2 writes the compact species 4.
 template <int RY, bool SRC, int PD, int SG, int HB, int LM = 0>
 __global__ void __launch_bounds__(256 * HB) k_stencil_ens(const EnsColumn* __restrict__ cols, int ncols, int total,
-                                                          int parity, StencilParams p) {
+                                                          int parity, StencilParams p,
+                                                          const uint32_t* __restrict__ pieces) {
   __shared__ StencilLds<RY, SG * HB> sh;
   constexpr int NG = MARL_S_EXT / (SG * HB);
+  // With a piece table (marl_ens_pieces): block b runs on XCD b % 8 and walks
+  // that XCD's pieces [off[x], off[x+1]) with a stride of the XCD's blocks.
+  const int x8 = blockIdx.x & 7;
+  const bool walk_pieces = pieces != nullptr;
+  int w = walk_pieces ? (int)pieces[x8] + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+  const int wend = walk_pieces ? (int)pieces[x8 + 1] : total;
+  const int wstep = walk_pieces ? (int)(gridDim.x >> 3) : (int)gridDim.x;
+  const uint32_t* table = pieces + kEnsPieceHeader;
   int c = 0;  // column of the previous tile (a block's tiles increase along its walk)
-  for (int w = blockIdx.x; w < total; w += gridDim.x) {
-    int b = xcd_remap(w, total);
-    if (b < cols[c].tile_base) c = 0;
-    if (c + 1 < ncols && cols[c + 1].tile_base <= b) {
-      int lo = c + 1, hi = ncols - 1;  // binary search: the last column starting at or before b
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (cols[mid].tile_base <= b) lo = mid;
-        else hi = mid - 1;
+  for (; w < wend; w += wstep) {
+    int tx, ty, tz, zc, grp;
+    if (walk_pieces) {
+      const uint32_t w0 = table[3 * w], w1 = table[3 * w + 1], w2 = table[3 * w + 2];
+      c = (int)(w0 & 0xffffu);
+      grp = (int)(w0 >> 16);
+      tx = (int)(w1 & 0xffffu);
+      ty = (int)(w1 >> 16);
+      tz = (int)(w2 & 0xffffu);
+      zc = (int)(w2 >> 16);
+    } else {
+      int b = xcd_remap(w, total);
+      if (b < cols[c].tile_base) c = 0;
+      if (c + 1 < ncols && cols[c + 1].tile_base <= b) {
+        int lo = c + 1, hi = ncols - 1;  // binary search: the last column starting at or before b
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (cols[mid].tile_base <= b) lo = mid;
+          else hi = mid - 1;
+        }
+        c = lo;
       }
-      c = lo;
+      const EnsColumn& cb = cols[c];
+      b -= cb.tile_base;
+      grp = b % NG;
+      b /= NG;
+      tx = b % cb.ntx;
+      ty = (b / cb.ntx) % cb.nty;
+      tz = b / (cb.ntx * cb.nty);
+      zc = cb.zc;
     }
     const EnsColumn& col = cols[c];
-    b -= col.tile_base;
-    const int grp = b % NG;
-    b /= NG;
     const int cur = col.cur ^ parity;
-    stencil_tile<RY, SRC, PD, SG, HB, LM>(col.f[cur], col.f[cur ^ 1], col.occ, col.X, col.Y, col.Z, col.XW,
-                                             col.zc, b % col.ntx, (b / col.ntx) % col.nty, b / (col.ntx * col.nty),
-                                             grp, p, sh, col.light);
+    stencil_tile<RY, SRC, PD, SG, HB, LM>(col.f[cur], col.f[cur ^ 1], col.occ, col.X, col.Y, col.Z, col.XW, zc, tx,
+                                             ty, tz, grp, p, sh, col.light);
     __syncthreads();  // the next tile reuses the LDS
   }
 }
@@ -886,22 +925,22 @@ int marl_launch_field_phase(marl_ctx* ctx, const marl_sim_params* sim, bool sour
 
 template <int RY, int PD, int SG, int HB>
 static hipError_t launch_stencil_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total, int parity,
-                                     const StencilParams& p, int grid) {
+                                     const StencilParams& p, int grid, const uint32_t* pieces) {
   const dim3 g(grid), b(256 * HB);
   if (p.fuse_light == 1) {
     if (p.apply_sources)
-      hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG, HB, 1>), g, b, 0, st, dcols, ncols, total, parity, p);
+      hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG, HB, 1>), g, b, 0, st, dcols, ncols, total, parity, p, pieces);
     else
-      hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG, HB, 1>), g, b, 0, st, dcols, ncols, total, parity, p);
+      hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG, HB, 1>), g, b, 0, st, dcols, ncols, total, parity, p, pieces);
   } else if (p.fuse_light == 2) {
     if (p.apply_sources)
-      hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG, HB, 2>), g, b, 0, st, dcols, ncols, total, parity, p);
+      hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG, HB, 2>), g, b, 0, st, dcols, ncols, total, parity, p, pieces);
     else
-      hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG, HB, 2>), g, b, 0, st, dcols, ncols, total, parity, p);
+      hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG, HB, 2>), g, b, 0, st, dcols, ncols, total, parity, p, pieces);
   } else if (p.apply_sources) {
-    hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG, HB>), g, b, 0, st, dcols, ncols, total, parity, p);
+    hipLaunchKernelGGL((k_stencil_ens<RY, true, PD, SG, HB>), g, b, 0, st, dcols, ncols, total, parity, p, pieces);
   } else {
-    hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG, HB>), g, b, 0, st, dcols, ncols, total, parity, p);
+    hipLaunchKernelGGL((k_stencil_ens<RY, false, PD, SG, HB>), g, b, 0, st, dcols, ncols, total, parity, p, pieces);
   }
   return hipGetLastError();
 }
@@ -914,14 +953,15 @@ bool marl_tile_ensemble_ok(int tile) {
 // whole_columns: every column's z-chunk is its whole depth (whatever the
 // tile's own chunk), which lets the last substep carry the light scan.
 void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int* total_light_blocks,
-                     bool whole_columns) {
+                     bool whole_columns, const int* chunks) {
   if (!marl_tile_ensemble_ok(tile)) tile = 7;
   const TileShape t = kTiles[tile];
   const int NG = MARL_S_EXT / (t.sg * (t.nw / 4));
   int base = 0, lbase = 0;
   for (int c = 0; c < ncols; ++c) {
     EnsColumn& e = cols[c];
-    e.zc = (whole_columns || t.zc >= e.Z) ? e.Z : t.zc;
+    const int zc = (chunks && chunks[c] > 0) ? chunks[c] : t.zc;
+    e.zc = (whole_columns || zc >= e.Z) ? e.Z : zc;
     e.ntx = (e.X + 63) / 64;
     e.nty = (e.Y + 4 * t.ry - 1) / (4 * t.ry);
     e.ntz = (e.Z + e.zc - 1) / e.zc;
@@ -956,12 +996,71 @@ int marl_stencil_ens_grid(int tile, int total_tiles) {
 }
 
 hipError_t marl_launch_substep_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total, int parity,
-                                   const StencilParams& p, int tile, int grid) {
+                                   const StencilParams& p, int tile, int grid, const uint32_t* pieces) {
   if (!marl_tile_ensemble_ok(tile)) tile = 7;
   return dispatch_tile(kTiles[tile], [&](auto T) {
     using TT = decltype(T);
-    return launch_stencil_ens<TT::RY, TT::PD, TT::SG, TT::HB>(st, dcols, ncols, total, parity, p, grid);
+    return launch_stencil_ens<TT::RY, TT::PD, TT::SG, TT::HB>(st, dcols, ncols, total, parity, p, grid, pieces);
   });
+}
+
+// Balanced pieces for a persistent grid of `grid` blocks (a multiple of 8,
+// block b on XCD b % 8).  The stencil tiles of all columns (deepest columns
+// first; in each column y, then x, then species group, fastest) are cut into
+// 8 runs of equal work (planes), one per XCD, so that the tiles an XCD runs
+// side by side are neighbours and share their halo rows in its L2.  In a run,
+// the whole-depth tiles that fill complete rounds of the XCD's blocks come
+// first, each one piece marching its column's whole depth (no z-chunk halo,
+// one pipeline prologue per tile); the tiles of the last, partial round are
+// cut into z-chunks so that that round, too, keeps every block busy.
+void marl_ens_pieces(int tile, const EnsColumn* cols, int ncols, int grid, std::vector<uint32_t>& out) {
+  if (!marl_tile_ensemble_ok(tile)) tile = 7;
+  const TileShape t = kTiles[tile];
+  const int NG = MARL_S_EXT / (t.sg * (t.nw / 4));
+  std::vector<int> ord(ncols);
+  for (int c = 0; c < ncols; ++c) ord[c] = c;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cols[a].Z > cols[b].Z; });
+  struct Tile {
+    int c, tx, ty, grp;
+  };
+  std::vector<Tile> L;
+  int64_t W = 0;
+  for (int c : ord) {
+    const EnsColumn& e = cols[c];
+    const int nty = (e.Y + 4 * t.ry - 1) / (4 * t.ry), ntx = (e.X + 63) / 64;
+    for (int ty = 0; ty < nty; ++ty)
+      for (int tx = 0; tx < ntx; ++tx)
+        for (int g = 0; g < NG; ++g) {
+          L.push_back({c, tx, ty, g});
+          W += e.Z;
+        }
+  }
+  const int B = grid / 8 > 0 ? grid / 8 : 1;
+  out.assign(kEnsPieceHeader, 0u);
+  auto emit = [&](const Tile& tl, int tz, int zc) {
+    out.push_back((uint32_t)tl.c | (uint32_t)tl.grp << 16);
+    out.push_back((uint32_t)tl.tx | (uint32_t)tl.ty << 16);
+    out.push_back((uint32_t)tz | (uint32_t)zc << 16);
+  };
+  size_t pos = 0;
+  int64_t acc = 0;
+  for (int x = 0; x < 8; ++x) {
+    out[x] = (uint32_t)((out.size() - kEnsPieceHeader) / 3);
+    const int64_t target = W * (x + 1) / 8;
+    const size_t s0 = pos;
+    while (pos < L.size() && (x == 7 || acc < target)) acc += cols[L[pos++].c].Z;
+    const size_t n = pos - s0, full = n / B * B, rem = n - full;
+    for (size_t i = s0; i < s0 + full; ++i) emit(L[i], 0, cols[L[i].c].Z);
+    if (rem) {
+      const int k = std::max<int>(1, (int)(B / rem));
+      for (size_t i = s0 + full; i < pos; ++i) {
+        const int Z = cols[L[i].c].Z;
+        const int zc = (Z + k - 1) / k;
+        for (int tz = 0; tz * zc < Z; ++tz) emit(L[i], tz, zc);
+      }
+    }
+  }
+  out[8] = (uint32_t)((out.size() - kEnsPieceHeader) / 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -999,6 +1098,10 @@ template <int ST = 4>
 __device__ __forceinline__ void light_column(const float* s4, const uint32_t* __restrict__ occ, float* light, int x,
                                              int y, int X, int Y, int Z, int XW, float I0, float e_cell, float nchem,
                                              float floor_) {
+  // global address space: the ensemble's pointers come from its column table
+  const MARL_GLOBAL float* s4g = as_global<float>(s4);
+  const MARL_GLOBAL uint32_t* occg = as_global<uint32_t>(occ);
+  MARL_GLOBAL float* lg = (MARL_GLOBAL float*)light;
   const size_t XY = (size_t)X * Y;
   const size_t col = (size_t)y * X + x;
   const uint32_t bit = 1u << (x & 31);
@@ -1011,21 +1114,21 @@ __device__ __forceinline__ void light_column(const float* s4, const uint32_t* __
     uint32_t w[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      a[u] = s4[ST * ((size_t)(z + u) * XY + col)];
-      w[u] = occ[(size_t)(z + u) * Y * XW + wcol];
+      a[u] = s4g[ST * ((size_t)(z + u) * XY + col)];
+      w[u] = occg[(size_t)(z + u) * Y * XW + wcol];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      light[(size_t)(z + u) * XY + col] = I;
+      lg[(size_t)(z + u) * XY + col] = I;
       if (w[u] & bit) I *= e_cell;
       I *= marl_expf(nchem * a[u]);
       if (I < floor_) I = 0.0f;
     }
   }
   for (; z < Z; ++z) {
-    const float a = s4[ST * ((size_t)z * XY + col)];  // read before the write: s4 may be the light plane
-    light[(size_t)z * XY + col] = I;
-    if (occ[(size_t)z * Y * XW + wcol] & bit) I *= e_cell;
+    const float a = s4g[ST * ((size_t)z * XY + col)];  // read before the write: s4 may be the light plane
+    lg[(size_t)z * XY + col] = I;
+    if (occg[(size_t)z * Y * XW + wcol] & bit) I *= e_cell;
     I *= marl_expf(nchem * a);
     if (I < floor_) I = 0.0f;
   }

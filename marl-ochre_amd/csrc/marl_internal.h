@@ -312,10 +312,20 @@ int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_launch_light_compact(marl_ctx* ctx, const marl_sim_params* sim);
 // Ensembles: fill the tile fields of `cols` for a tile id (returns total
 // logical tiles and light blocks), and launch one batched substep / light scan.
+// A column's z-chunk is its whole depth with whole_columns, else chunks[c]
+// planes when chunks is given and chunks[c] > 0, else the tile shape's
+// chunk, cut at the column's depth.
 void marl_ens_layout(int tile, EnsColumn* cols, int ncols, int* total_tiles, int* total_light_blocks,
-                     bool whole_columns);
+                     bool whole_columns, const int* chunks = nullptr);
+// pieces: nullptr (the tile numbering of the layout) or a device copy of a
+// marl_ens_pieces table for a persistent grid of `grid` blocks.
 hipError_t marl_launch_substep_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total_tiles, int parity,
-                                   const StencilParams& p, int tile, int grid);
+                                   const StencilParams& p, int tile, int grid, const uint32_t* pieces = nullptr);
+// Piece table of the balanced persistent walk: words 0..8 are the first
+// piece of each XCD's run (and the end), then 3 words per piece: column |
+// species group << 16, tile x | tile y << 16, z-chunk index | chunk planes << 16.
+constexpr int kEnsPieceHeader = 12;
+void marl_ens_pieces(int tile, const EnsColumn* cols, int ncols, int grid, std::vector<uint32_t>& out);
 hipError_t marl_launch_light_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total_blocks,
                                  const marl_sim_params* sim, bool compact);
 int marl_stencil_ens_grid(int tile, int total_tiles);

@@ -482,9 +482,15 @@ class MarlEnsemble:
     def set_tile(self, tile: int) -> None:
         self._chk(self.lib.marl_ensemble_set_tile(self.h, int(tile)))
 
-    def set_grid(self, per_tile: bool) -> None:
-        """Batched stencil grid: persistent (False) or one block per tile (True)."""
-        self._chk(self.lib.marl_ensemble_set_grid(self.h, 1 if per_tile else 0))
+    def set_grid(self, mode) -> None:
+        """Batched stencil grid: persistent (False / 0), one block per tile
+        (True / 1) or persistent over balanced whole-depth pieces (2)."""
+        self._chk(self.lib.marl_ensemble_set_grid(self.h, int(mode)))
+
+    def grid_mode(self) -> int:
+        m = ctypes.c_int32()
+        self._chk(self.lib.marl_ensemble_get_grid(self.h, ctypes.byref(m)))
+        return m.value
 
     def time_field(self, sim, n_substeps: int = 20) -> float:
         p = _params(sim)
@@ -503,7 +509,8 @@ class MarlEnsemble:
         t, g, n, w = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         self._chk(self.lib.marl_ensemble_launch_shape(self.h, ctypes.byref(t), ctypes.byref(g), ctypes.byref(n),
                                                       ctypes.byref(w)))
-        return {"tile": t.value, "grid_blocks": g.value, "stencil_tiles": n.value, "whole_columns": bool(w.value)}
+        return {"tile": t.value, "grid_blocks": g.value, "stencil_tiles": n.value, "whole_columns": bool(w.value),
+                "grid_mode": self.grid_mode()}
 
     def time_field_phase(self, sim, n_ticks: int = 5) -> float:
         """Milliseconds per batched field phase (substeps + light) of a tick."""
@@ -515,3 +522,18 @@ class MarlEnsemble:
     def set_whole_columns(self, whole: bool) -> None:
         """Every column one z-chunk deep, the light scan fused into the last substep."""
         self._chk(self.lib.marl_ensemble_set_whole_columns(self.h, 1 if whole else 0))
+
+    def set_chunks(self, chunks=None) -> None:
+        """Per-column z-chunk planes (0: the tile shape's); None clears."""
+        if chunks is None:
+            self._chk(self.lib.marl_ensemble_set_chunks(self.h, None, 0))
+            return
+        a = (ctypes.c_int32 * len(chunks))(*[int(z) for z in chunks])
+        self._chk(self.lib.marl_ensemble_set_chunks(self.h, a, len(chunks)))
+
+    def chunks(self) -> list:
+        """The z-chunk each column's batched stencil tiles use."""
+        n = len(self.columns)
+        a = (ctypes.c_int32 * n)()
+        self._chk(self.lib.marl_ensemble_get_chunks(self.h, a, n))
+        return list(a)

@@ -28,7 +28,7 @@ def _columns(pkg):
     return cols
 
 
-def _run(pkg, cols, sim, n_ticks, tile=None, per_tile_grid=False, whole=False):
+def _run(pkg, cols, sim, n_ticks, tile=None, per_tile_grid=False, whole=False, chunks=None):
     seeds = np.array([c[4] for c in cols], np.uint64)
     alone = [pkg.MarlGpu(c[0]) for c in cols]
     batched = [pkg.MarlGpu(c[0]) for c in cols]
@@ -43,6 +43,8 @@ def _run(pkg, cols, sim, n_ticks, tile=None, per_tile_grid=False, whole=False):
                 ens.set_tile(tile)
             ens.set_grid(per_tile_grid)
             ens.set_whole_columns(whole)
+            if chunks is not None:
+                ens.set_chunks(chunks)
             for t in range(n_ticks):
                 stats, ms, hf = ens.tick(sim, seeds, t, 1)
                 assert ms > 0 and hf == 0
@@ -83,12 +85,53 @@ def test_ensemble_one_block_per_tile(pkg, tile):
     _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=2), 2, tile=tile, per_tile_grid=True)
 
 
+@pytest.mark.parametrize("tile", [0, 6, 7, 11, 17, 18])
+def test_ensemble_balanced_pieces(pkg, tile):
+    """Grid mode 2: a persistent grid over balanced whole-depth pieces (the
+    last round's tiles cut into z-chunks): same fields, light and cells."""
+    cols = _columns(pkg)
+    _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=2), 2, tile=tile, per_tile_grid=2)
+
+
 @pytest.mark.parametrize("tile,per_tile", [(0, False), (11, True), (15, False), (18, True)])
 def test_ensemble_whole_columns_with_fused_light(pkg, tile, per_tile):
     """Every column one z-chunk deep, the light scan carried by the last
     substep of the batched launch: same fields, light and cells."""
     cols = _columns(pkg)
     _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=2), 2, tile=tile, per_tile_grid=per_tile, whole=True)
+
+
+@pytest.mark.parametrize("tile,per_tile", [(18, True), (7, False), (0, True)])
+def test_ensemble_per_column_chunks(pkg, tile, per_tile):
+    """Columns of one batched launch with z-chunks of their own (4, the tile
+    shape's, 32, 8, and one longer than the column): same fields, light and
+    cells as each column ticked alone."""
+    cols = _columns(pkg)
+    assert [c[0][2] for c in cols] == [24, 16, 40, 32, 48]
+    _run(pkg, cols, pkg.SimulationConfig(diffusion_substeps=4), 2, tile=tile, per_tile_grid=per_tile,
+         chunks=[4, 0, 32, 8, 64])
+
+
+def test_ensemble_chunks_reported_and_checked(pkg):
+    cols = _columns(pkg)
+    gs = [pkg.MarlGpu(c[0]) for c in cols]
+    try:
+        with pkg.MarlEnsemble(gs) as ens:
+            ens.set_tile(18)  # 16-plane chunks
+            ens.set_chunks([4, 0, 32, 8, 64])
+            assert ens.chunks() == [4, 16, 32, 8, 48]
+            ens.set_whole_columns(True)
+            assert ens.chunks() == [24, 16, 40, 32, 48]
+            ens.set_whole_columns(False)
+            ens.set_chunks(None)
+            assert ens.chunks() == [16, 16, 16, 16, 16]
+            with pytest.raises(pkg.InvalidInput):
+                ens.set_chunks([4, 4])
+            with pytest.raises(pkg.InvalidInput):
+                ens.set_chunks([4, -1, 4, 4, 4])
+    finally:
+        for g in gs:
+            g.close()
 
 
 def test_ensemble_rejects_temporal_tiles(pkg):
@@ -120,6 +163,8 @@ def test_ensemble_autotune_keeps_fields(pkg):
             tile = ens.autotune(pkg.SimulationConfig())
             assert 0 <= tile < pkg.load_library().marl_stencil_tile_count()
             assert ens.launch_shape()["tile"] == tile
+            zc = ens.chunks()
+            assert all(0 < z <= c[0][2] for z, c in zip(zc, cols)), zc
             assert ens.time_field(pkg.SimulationConfig(), 4) > 0
             # one block per logical tile, or a persistent grid of at most that many blocks
             ens.set_grid(True)
@@ -128,6 +173,11 @@ def test_ensemble_autotune_keeps_fields(pkg):
             ens.set_grid(False)
             shape = ens.launch_shape()
             assert 0 < shape["grid_blocks"] <= shape["stencil_tiles"]
+            ens.set_grid(2)
+            shape = ens.launch_shape()
+            assert shape["grid_mode"] == 2 and shape["grid_blocks"] % 8 == 0 and shape["stencil_tiles"] > 0
+            with pytest.raises(pkg.InvalidInput):
+                ens.set_grid(3)
             with pytest.raises(pkg.InvalidInput):
                 ens.tick(pkg.SimulationConfig(diffusion_substeps=3))
         for g, b in zip(gs, before):

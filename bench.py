@@ -176,12 +176,19 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
                     f"{reps} repetitions from the same state",
         "cells": n0,
         "Mcell_updates_per_s_kinetics": round(n0 / (kin_ms * 1e-3) / 1e6, 3),
+        # ms_kinetics is the pass on the tick's critical path: the transpose into
+        # the AoS view and the levels; the transpose back runs beside fate
+        "ms_kinetics_with_transpose_back": round(
+            kin_ms + sum(st["ms_transpose_back"] for _, st in runs) / reps, 3),
         "Mcell_updates_per_s_full_tick": round(n0 / (tick_ms * 1e-3) / 1e6, 3),
         "ms_full_tick": round(tick_ms, 3),
         "ms_full_tick_graph": round(sum(w for w, _ in graph_runs) * 1e3 / reps, 3),
         "ms_full_tick_graph_gpu": round(sum(st["ms_field"] for _, st in graph_runs) / reps, 3),
+        # the schedule runs beside the field phase and the transpose back beside fate,
+        # so the phases overlap; ms_tick_gpu is the tick's own span
         "ms_breakdown": {k: round(sum(st[k] for _, st in runs) / reps, 3)
-                         for k in ("ms_field", "ms_schedule", "ms_kinetics", "ms_fate")},
+                         for k in ("ms_field", "ms_schedule", "ms_kinetics", "ms_fate", "ms_transpose_back")},
+        "ms_tick_gpu": round(sum(st["ms_tick"] for _, st in runs) / reps, 3),
         "levels": runs[-1][1]["levels"],
         "kinetics_autotune": {"block": chosen, "ms_kinetics_by_block": by_block},
         "events_first_tick": {k: runs[-1][1][k] for k in ("divisions", "deaths", "births_added", "n_cells")},
@@ -593,6 +600,7 @@ def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3, pair_target=10_000_
             return out, stats
 
         curve = []
+        g.surrogate_set_dead_zone(0.2)   # the shipped operating point
         for size in [m for m in (1_000_000, 3_000_000) if m < n_pairs] + [n_pairs]:
             rep = g.surrogate_train(steps=train_steps, batch=8192, lr=2e-3, lr_final=1e-5, seed=1, reinit=True,
                                     max_pairs=size)
@@ -608,6 +616,14 @@ def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3, pair_target=10_000_
         g.surrogate_set_params(w0, nm0)
         _, sh0 = first_tick(A.KINETICS_SHADOW)
         g.surrogate_set_params(w, norm)
+        # the network trained on all pairs with corrections below a dead zone dropped
+        dead_zone = []
+        for dz in (0.0, 0.05, 0.1, 0.2, 0.5, 1.0):
+            g.surrogate_set_dead_zone(dz)
+            _, shz = first_tick(A.KINETICS_SHADOW)
+            dead_zone.append({"dead_zone": dz, "shadow": _shadow_summary(shz)})
+            log("surrogate dead zone:", json.dumps(dead_zone[-1])[:300])
+        g.surrogate_set_dead_zone(0.2)   # the shipped operating point
         plain, _ = first_tick(A.KINETICS_PLAIN)
         surr, surr_stats = first_tick(A.KINETICS_SURROGATE)
         infer_ms = g.surrogate_time_infer(n0, 10)
@@ -623,6 +639,7 @@ def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3, pair_target=10_000_
         "train_pairs_per_s": round(train_steps * 8192 / (curve[-1]["train_ms"] * 1e-3), 0),
         "error_vs_training_pairs": [{"train_pairs": 0, "note": "explicit step alone", "shadow": _shadow_summary(sh0)}]
         + curve,
+        "error_vs_dead_zone": dead_zone,
         "Mcell_updates_per_s_plain": round(n0 / (kin_plain * 1e-3) / 1e6, 3),
         "Mcell_updates_per_s_surrogate": round(n0 / (kin_surr * 1e-3) / 1e6, 3),
         "speedup_vs_plain": round(kin_plain / kin_surr, 3),

@@ -135,9 +135,11 @@ typedef struct {
   int32_t levels;         /* depth of the conflict schedule of the cell pass */
   int32_t _pad;
   float ms_field;         /* occupancy rebuild, sources, substeps, light (hipEvents) */
-  float ms_schedule;      /* empty-neighbour masks and the conflict levels */
-  float ms_kinetics;      /* the per-level kinetics launches */
-  float ms_fate;          /* divisions, mutation, deaths, compaction */
+  float ms_schedule;      /* empty-neighbour masks and the conflict levels (beside the field phase) */
+  float ms_kinetics;      /* the pass's field view, then the per-level kinetics launches */
+  float ms_fate;          /* divisions, mutation, deaths, compaction (and the transpose back beside them) */
+  float ms_tick;          /* the whole tick, first launch to last */
+  float ms_transpose_back;/* the pass's field view back to the resident layout, beside fate */
 } marl_tick_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -480,6 +482,10 @@ int marl_surrogate_grad(marl_ctx* ctx, const float* x, const float* t, size_t ro
 int marl_surrogate_adam_step(marl_ctx* ctx, const float* grads, float lr);
 /* zmax, energy margin, share of surrogate cells also run exactly for monitoring. */
 int marl_surrogate_configure(marl_ctx* ctx, float zmax, float energy_margin, float shadow_rate);
+/* Corrections whose magnitude in target-scale units (residual over the
+ * output's gross explicit activity) is at most dead_zone are dropped, so the
+ * explicit step stands where the network is not confident (default 0.2). */
+int marl_surrogate_set_dead_zone(marl_ctx* ctx, float dead_zone);
 int marl_surrogate_stats(marl_ctx* ctx, marl_surr_stats* out);
 /* Average time of the fused inference kernel over `rows` synthetic rows
  * (hipEvents around `reps` back-to-back launches on the context stream). */

@@ -79,6 +79,8 @@ class marl_tick_stats(ctypes.Structure):
         ("ms_schedule", ctypes.c_float),
         ("ms_kinetics", ctypes.c_float),
         ("ms_fate", ctypes.c_float),
+        ("ms_tick", ctypes.c_float),
+        ("ms_transpose_back", ctypes.c_float),
     ]
 
 
@@ -289,6 +291,7 @@ SIGNATURES = {
     "marl_surrogate_grad": (ctypes.c_int, [_P, _P, _P, _SZ, ctypes.POINTER(ctypes.c_float), _P]),
     "marl_surrogate_adam_step": (ctypes.c_int, [_P, _P, ctypes.c_float]),
     "marl_surrogate_configure": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "marl_surrogate_set_dead_zone": (ctypes.c_int, [_P, ctypes.c_float]),
     "marl_surrogate_stats": (ctypes.c_int, [_P, ctypes.POINTER(marl_surr_stats)]),
     "marl_surrogate_reset_stats": (ctypes.c_int, [_P]),
     "marl_surrogate_time_infer": (ctypes.c_int, [_P, _I64, _I32, ctypes.POINTER(ctypes.c_float)]),

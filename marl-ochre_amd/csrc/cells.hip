@@ -319,7 +319,7 @@ __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict
                                               const float* __restrict__ light, const uint8_t* __restrict__ nbmask,
                                               uint8_t* __restrict__ events, const CellParams& p,
                                               const HarvestSink& hv, float* sI, float* sE, float* sD, float* sQ,
-                                              float* sR) {
+                                              float* sR, float* sG, float* sH) {
   marl_cell_state c = load_cell(cells + i);
   const marl_ruleset& r = rules[i];
   const uint32_t m = nbmask[i];
@@ -336,8 +336,8 @@ __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict
   const bool take = HARVEST && (uint64_t)sample_hash(hv.tick, (uint32_t)i) < hv.threshold;
   float frel[MARL_R_MAX];
   if (take) {
-    PtrArr dI{sQ}, dD{sR};
-    explicit_step(c, ar, PtrArr{sI}, E, dI, dD, lt, p, frel);
+    PtrArr dI{sQ}, dD{sR}, gI{sG}, gD{sH};
+    explicit_step(c, ar, PtrArr{sI}, E, dI, dD, lt, p, frel, gI, gD);
   }
   cell_kinetics(c, ar, I, E, D, lt, p);
   const float4 fate = ar.fate();
@@ -360,11 +360,13 @@ __device__ __forceinline__ void kinetics_cell(int i, marl_cell_state* __restrict
 #pragma unroll
       for (int k = 0; k < kFeat; ++k) fo[k] = f[k];
       float* to = hv.targ + slot * kTarg;
+      float sc[kTarg];
+      target_scales(sG, sH, sc);
 #pragma unroll
-      for (int k = 0; k < MARL_M_INT - 1; ++k) to[k] = (sI[k] - c.internal[k]) - sQ[k];
+      for (int k = 0; k < MARL_M_INT - 1; ++k) to[k] = scaled_target((sI[k] - c.internal[k]) - sQ[k], sc[k]);
       to[MARL_M_INT - 1] = 0.0f;
 #pragma unroll
-      for (int s2 = 0; s2 < MARL_S_EXT; ++s2) to[16 + s2] = sD[s2] - sR[s2];
+      for (int s2 = 0; s2 < MARL_S_EXT; ++s2) to[16 + s2] = scaled_target(sD[s2] - sR[s2], sc[16 + s2]);
       hv.hash[slot] = topology_hash(r);
     }
   }
@@ -388,12 +390,15 @@ __global__ void __launch_bounds__(B) k_kinetics(const int32_t* __restrict__ orde
   __shared__ float sD[B * 13];
   __shared__ float sQ[HARVEST ? B * 17 : 1];  // the harvest's explicit-step deltas
   __shared__ float sR[HARVEST ? B * 13 : 1];
+  __shared__ float sG[HARVEST ? B * 17 : 1];  // and their gross activities
+  __shared__ float sH[HARVEST ? B * 13 : 1];
   const int t = blockIdx.x * B + threadIdx.x;
   if (t >= count) return;
   const int i = order[t];
   kinetics_cell<HARVEST, QUADS>(i, cells, rules, field, light, nbmask, events, p, hv, sI + threadIdx.x * 17,
                                 sE + threadIdx.x * 13, sD + threadIdx.x * 13, HARVEST ? sQ + threadIdx.x * 17 : sQ,
-                                HARVEST ? sR + threadIdx.x * 13 : sR);
+                                HARVEST ? sR + threadIdx.x * 13 : sR, HARVEST ? sG + threadIdx.x * 17 : sG,
+                                HARVEST ? sH + threadIdx.x * 13 : sH);
 }
 
 // ---------------------------------------------------------------------------
@@ -724,6 +729,7 @@ __global__ void k_budget_final(const double* __restrict__ partial, int nblocks, 
 // ones exit at once); a tick that would exceed it, or the cell capacity, sets
 // `err`, skips the cell phase and is finished by the host-driven path.
 constexpr int kGraphSweeps = 64;
+constexpr int64_t kGraphOverlapCells = 131072;  // captured ticks overlap phases from this population up
 constexpr int kGraphLevels = 64;  // <= kLocalLevels: every level is block-aggregated
 static_assert(kGraphLevels <= kLocalLevels, "graph levels");
 
@@ -842,7 +848,8 @@ __global__ void __launch_bounds__(B) k_g_kinetics(int L, const TickDev* ts, cons
   const HarvestSink hv{};
   GS_LOOP(t, cnt)
   kinetics_cell<false, QUADS>(order[lo + t], cells, rules, field, light, nbmask, events, p, hv,
-                              sI + threadIdx.x * 17, sE + threadIdx.x * 13, sD + threadIdx.x * 13, nullptr, nullptr);
+                              sI + threadIdx.x * 17, sE + threadIdx.x * 13, sD + threadIdx.x * 13, nullptr, nullptr,
+                              nullptr, nullptr);
 }
 
 // division targets for i < n; zero flags up to cap for the fixed-size scan
@@ -1141,15 +1148,16 @@ void launch_plain_level(marl_ctx* ctx, bool quads, const int32_t* ord, int cnt, 
   else launch_plain_level_q<false>(ctx, ord, cnt, cells, rules, field, p);
 }
 
-int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEvent_t ev_sched) {
+// The conflict schedule of the ordered pass (lib.rs:204-217): empty-neighbour
+// masks, conflict levels and the cells sorted by level.  Reads only the cells,
+// the occupancy and the voxel map, so it may run beside the field phase.
+// On return off[L]..off[L+1] index level L's cells in ctx->order (off has
+// depth + 1 entries; {0} when there are no cells).
+int run_schedule(marl_ctx* ctx, const CellParams& p, std::vector<int32_t>& off) {
   const int n = (int)ctx->n_cells;
-  if (levels_out) *levels_out = 0;
-  if (n == 0) {
-    if (ev_sched) MARL_HIP(ctx, hipEventRecord(ev_sched, ctx->stream));
-    return MARL_OK;
-  }
+  off.assign(1, 0);
+  if (n == 0) return MARL_OK;
   marl_cell_state* cells = ctx->cells[ctx->cbuf];
-  const marl_ruleset* rules = ctx->rules[ctx->cbuf];
   hipLaunchKernelGGL(k_cell_prep, dim3(grid1(n)), dim3(256), 0, ctx->stream, cells, n, ctx->cell_at, ctx->occ,
                      ctx->nbmask, ctx->preds, (size_t)ctx->cap, ctx->npred, ctx->level, p);
   MARL_HIP(ctx, hipGetLastError());
@@ -1190,18 +1198,37 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
   if (int rc = d2h_sync(ctx, hh.data(), hist, sizeof(int32_t) * (kMaxLevels + 1))) return rc;
   const int nlev = hh[0] + 1;
   if (nlev > kMaxLevels) return marl_fail(ctx, MARL_ERR_TRANSFER, "cell-pass schedule deeper than 4096 levels");
-  std::vector<int32_t> off(nlev + 1, 0);
+  off.assign(nlev + 1, 0);
   for (int L = 0; L < nlev; ++L) off[L + 1] = off[L] + hh[1 + L];
   int32_t* cursor = ctx->level_count + (kMaxLevels + 1);
   MARL_HIP(ctx, hipMemcpyAsync(cursor, off.data(), sizeof(int32_t) * nlev, hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_level_scatter, dim3(lb), dim3(256), 0, ctx->stream, n, ctx->level, cursor, ctx->order);
   MARL_HIP(ctx, hipGetLastError());
-  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));  // `off` must outlive the copy
-  if (ev_sched) MARL_HIP(ctx, hipEventRecord(ev_sched, ctx->stream));
-  // The pass works on an AoS view of the field (the 48-byte voxel records keep
-  // a neighbour's 12 species in one or two cache lines; the resident layout is
-  // species quads, best for the stencil), or on the quads in place when the
-  // population is sparse.  The surrogate and the pair harvest use the AoS view.
+  MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the copy reads `off` (pageable, staged at once) and `hh`
+  return MARL_OK;
+}
+
+// The per-level launches of the ordered pass, after the schedule.  Leaves the
+// events in ctx->events.  The pass works on an AoS view of the field (the
+// 48-byte voxel records keep a neighbour's 12 species in one or two cache
+// lines; the resident layout is species quads, best for the stencil), or on
+// the quads in place when the population is sparse.  The surrogate and the
+// pair harvest use the AoS view.  With back_done, the view is written back to
+// the quads on ctx->aux after the levels, between back_start and back_done
+// recorded there (the caller joins back_done before the field is next read);
+// otherwise on ctx->stream.
+int run_levels(marl_ctx* ctx, const CellParams& p, const std::vector<int32_t>& off, hipEvent_t back_start,
+               hipEvent_t back_done) {
+  const int nlev = (int)off.size() - 1;
+  if (nlev <= 0) {
+    if (back_done) {
+      MARL_HIP(ctx, hipEventRecord(back_start, ctx->aux));
+      MARL_HIP(ctx, hipEventRecord(back_done, ctx->aux));
+    }
+    return MARL_OK;
+  }
+  marl_cell_state* cells = ctx->cells[ctx->cbuf];
+  const marl_ruleset* rules = ctx->rules[ctx->cbuf];
   const HarvestSink hv = surrogate_harvest_sink(ctx, p);
   const bool plain = ctx->kin_mode == MARL_KINETICS_PLAIN || !ctx->surr;
   const bool quads = plain && !hv.count && marl_choose_cell_layout(ctx);
@@ -1226,9 +1253,31 @@ int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEv
     }
   }
   MARL_HIP(ctx, hipGetLastError());
-  if (!quads)
+  if (back_done) {
+    MARL_HIP(ctx, hipEventRecord(ctx->xev[0], ctx->stream));
+    MARL_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->xev[0], 0));
+    MARL_HIP(ctx, hipEventRecord(back_start, ctx->aux));
+    if (!quads) {
+      OnStream on(ctx, ctx->aux);
+      if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
+    }
+    MARL_HIP(ctx, hipEventRecord(back_done, ctx->aux));
+  } else if (!quads) {
     if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
-  if (levels_out) *levels_out = nlev;
+  }
+  return MARL_OK;
+}
+
+// The ordered cell pass (lib.rs:204-217) on the resident field, light and
+// occupancy, on ctx->stream.  Leaves events in ctx->events; *levels_out =
+// schedule depth; ev_sched (if given) is recorded after the schedule.
+int run_cell_pass(marl_ctx* ctx, const CellParams& p, int32_t* levels_out, hipEvent_t ev_sched) {
+  if (levels_out) *levels_out = 0;
+  std::vector<int32_t> off;
+  if (int rc = run_schedule(ctx, p, off)) return rc;
+  if (ev_sched) MARL_HIP(ctx, hipEventRecord(ev_sched, ctx->stream));
+  if (int rc = run_levels(ctx, p, off, nullptr, nullptr)) return rc;
+  if (levels_out) *levels_out = (int)off.size() - 1;
   return MARL_OK;
 }
 
@@ -1358,15 +1407,18 @@ int marl_enqueue_field_phase(marl_ctx* ctx, const marl_sim_params* sim) {
 
 // Schedule, the ordered pass, fate, births, deaths and compaction.
 int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
+  if (int rc = marl_enqueue_schedule(ctx, sim)) return rc;
+  return marl_enqueue_pass_and_fate(ctx, sim, false);
+}
+
+// The conflict schedule (masks, levels, level lists) on ctx->stream.
+int marl_enqueue_schedule(marl_ctx* ctx, const marl_sim_params* sim) {
   TickDev* ts = static_cast<TickDev*>(ctx->tdev);
   const int64_t cap = ctx->cap;
-  const CellParams p = cell_params(ctx, sim, 0);  // the RNG stream comes from ts->tick
+  const CellParams p = cell_params(ctx, sim, 0);
   hipStream_t st = ctx->stream;
   marl_cell_state* cells = ctx->cells[ctx->cbuf];
-  marl_ruleset* rules = ctx->rules[ctx->cbuf];
-  const int kb = ctx->kin_block;
-  const unsigned gc = gs_grid(cap), gk = gs_grid(cap, kb, 16384 * 64 / kb), gf = gs_grid(cap + 1);
-  // schedule
+  const unsigned gc = gs_grid(cap);
   hipLaunchKernelGGL(k_g_cell_prep, dim3(gc), dim3(256), 0, st, cells, ts, ctx->cell_at, ctx->occ, ctx->nbmask,
                      ctx->preds, (size_t)cap, ctx->npred, ctx->level, p);
   for (int k = 0; k < ctx->graph_sweeps; ++k)
@@ -1382,6 +1434,22 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
   hipLaunchKernelGGL(k_g_level_scatter, dim3(std::min<unsigned>(gc, kLevelBlocks)), dim3(256), 0, st, ts,
                      ctx->level, cursor, ctx->order);
   MARL_HIP(ctx, hipGetLastError());
+  return MARL_OK;
+}
+
+// The ordered pass, fate, births, deaths and compaction on ctx->stream, after
+// the schedule and the field phase.  With fork_back the AoS view goes back to
+// the quads on ctx->aux while fate runs (fate reads no field), joined on
+// ctx->stream before the part ends.
+int marl_enqueue_pass_and_fate(marl_ctx* ctx, const marl_sim_params* sim, bool fork_back) {
+  TickDev* ts = static_cast<TickDev*>(ctx->tdev);
+  const int64_t cap = ctx->cap;
+  const CellParams p = cell_params(ctx, sim, 0);  // the RNG stream comes from ts->tick
+  hipStream_t st = ctx->stream;
+  marl_cell_state* cells = ctx->cells[ctx->cbuf];
+  marl_ruleset* rules = ctx->rules[ctx->cbuf];
+  const int kb = ctx->kin_block;
+  const unsigned gc = gs_grid(cap), gk = gs_grid(cap, kb, 16384 * 64 / kb), gf = gs_grid(cap + 1);
   // the ordered pass on the AoS view, or on the quads in place (the layout
   // chosen when the tick was prepared; it is part of the graph's key)
   auto level = [&](int L, float* field, bool quads) {
@@ -1406,7 +1474,15 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
   } else {
     if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
     for (int L = 0; L < ctx->graph_levels; ++L) level(L, ctx->staging, false);
-    if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
+    if (fork_back) {
+      MARL_HIP(ctx, hipEventRecord(ctx->xev[2], st));
+      MARL_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->xev[2], 0));
+      OnStream on(ctx, ctx->aux);
+      if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) return rc;
+      MARL_HIP(ctx, hipEventRecord(ctx->xev[3], ctx->aux));
+    } else if (int rc = marl_transpose(ctx, true, ctx->field[ctx->cur])) {
+      return rc;
+    }
   }
   MARL_HIP(ctx, hipGetLastError());
   // fate, births with mutation, deaths and compaction
@@ -1439,6 +1515,7 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
                      ctx->birth_cells, ctx->birth_rules, cells, rules, win);
   hipLaunchKernelGGL(k_g_finish, dim3(1), dim3(1), 0, st, ts);
   MARL_HIP(ctx, hipGetLastError());
+  if (fork_back && !ctx->kin_quads) MARL_HIP(ctx, hipStreamWaitEvent(st, ctx->xev[3], 0));
   return MARL_OK;
 }
 
@@ -1448,7 +1525,7 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
     marl_sim_params sim;
     int64_t cap;
     const void *cells, *rules, *f0, *f1, *staging;
-    int32_t tile, cur, cbuf, sweeps, levels, kin_quads, kin_block;
+    int32_t tile, cur, cbuf, sweeps, levels, kin_quads, kin_block, overlap;
   } k;
   memset(&k, 0, sizeof(k));
   k.sim = *sim;
@@ -1465,6 +1542,7 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
   k.levels = ctx->graph_levels;
   k.kin_quads = ctx->kin_quads ? 1 : 0;
   k.kin_block = ctx->kin_block;
+  k.overlap = ctx->graph_overlap ? 1 : 0;
   const uint8_t* b = reinterpret_cast<const uint8_t*>(&k);
   return std::vector<uint8_t>(b, b + sizeof(k));
 }
@@ -1639,10 +1717,30 @@ int capture_tick(marl_ctx* ctx, const marl_sim_params* sim) {
   const int slot = ctx->cur;
   drop_graph(ctx, slot);
   MARL_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  // counters and occupancy; then the schedule on ctx->aux beside the field
+  // phase (the schedule reads only cells and occupancy); the pass after both
   int rc = marl_enqueue_tick_begin(ctx, sim);
+  const bool overlap = ctx->graph_overlap;
+  bool forked = false;
+  if (rc == MARL_OK && overlap) {
+    if (hipEventRecord(ctx->xev[1], ctx->stream) == hipSuccess &&
+        hipStreamWaitEvent(ctx->aux, ctx->xev[1], 0) == hipSuccess) {
+      forked = true;
+      OnStream on(ctx, ctx->aux);
+      rc = marl_enqueue_schedule(ctx, sim);
+    } else {
+      rc = marl_fail(ctx, MARL_ERR_TRANSFER, "capture: fork of the schedule stream failed");
+    }
+  }
   if (rc == MARL_OK) rc = marl_enqueue_field_phase(ctx, sim);
   const int cur_after = ctx->cur;
-  if (rc == MARL_OK) rc = marl_enqueue_cell_phase(ctx, sim);
+  if (forked) {  // join the schedule stream on every path, so that the capture can end
+    const hipError_t j1 = hipEventRecord(ctx->xev[0], ctx->aux);
+    const hipError_t j2 = j1 == hipSuccess ? hipStreamWaitEvent(ctx->stream, ctx->xev[0], 0) : j1;
+    if (rc == MARL_OK && j2 != hipSuccess) rc = marl_hip_fail(ctx, j2, "capture: join of the schedule stream");
+  }
+  if (rc == MARL_OK && !overlap) rc = marl_enqueue_schedule(ctx, sim);
+  if (rc == MARL_OK) rc = marl_enqueue_pass_and_fate(ctx, sim, overlap);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
   ctx->cur = slot;
@@ -1780,20 +1878,27 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
   if (int rc = ensure_voxel_buffers(ctx)) return rc;
   if (ctx->phase_budget && !ctx->phase_budget_dev)
     MARL_HIP(ctx, hipMalloc(&ctx->phase_budget_dev, sizeof(double) * 4 * kBudget));
-  hipEvent_t ev[5] = {};
+  if (int rc = marl_ensure_aux(ctx)) return rc;
+  constexpr int kEv = 9;
+  hipEvent_t ev[kEv] = {};
   for (auto& e : ev) MARL_HIP(ctx, hipEventCreate(&e));
   struct Guard {
     hipEvent_t* e;
     ~Guard() {
-      for (int k = 0; k < 5; ++k)
+      for (int k = 0; k < kEv; ++k)
         if (e[k]) (void)hipEventDestroy(e[k]);
     }
   } guard{ev};
   marl_tick_stats st{};
+  std::vector<int32_t> off;
   for (int t = 0; t < n_ticks; ++t) {
     st = marl_tick_stats{};
     if (int rc = marl_ensure_cell_capacity(ctx, 2 * ctx->n_cells + 1024)) return rc;
     const CellParams p = cell_params(ctx, sim, (uint64_t)(tick0 + t));
+    // With the phase budget on, each phase finishes before the next begins (the
+    // budget reads the field after the cell pass); otherwise the schedule runs
+    // on ctx->aux beside the field phase and the transpose back beside fate.
+    const bool overlap = !ctx->phase_budget;
     MARL_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
     // occupancy from the live cells (lib.rs:174-179); sources fused into the
     // first substep (field.rs:231-250 then :212-217); light (lib.rs:199),
@@ -1801,29 +1906,42 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
     if (int rc = marl_launch_occupancy_from_cells(ctx)) return rc;
     if (ctx->phase_budget)
       if (int rc = enqueue_budget(ctx, ctx->phase_budget_dev)) return rc;
+    if (overlap) {
+      MARL_HIP(ctx, hipEventRecord(ctx->xev[1], ctx->stream));
+      MARL_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->xev[1], 0));
+    }
     if (int rc = marl_launch_field_phase(ctx, sim, true)) return rc;
     if (ctx->phase_budget)
       if (int rc = enqueue_budget(ctx, ctx->phase_budget_dev + kBudget)) return rc;
     MARL_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
-    int32_t levels = 0;
-    if (int rc = run_cell_pass(ctx, p, &levels, ev[2])) return rc;
-    if (ctx->n_cells == 0) MARL_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+    {
+      OnStream on(ctx, overlap ? ctx->aux : ctx->stream);
+      MARL_HIP(ctx, hipEventRecord(ev[5], ctx->stream));
+      if (int rc = run_schedule(ctx, p, off)) return rc;
+      MARL_HIP(ctx, hipEventRecord(ev[6], ctx->stream));
+    }
+    if (overlap) MARL_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev[6], 0));
+    MARL_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+    if (int rc = run_levels(ctx, p, off, overlap ? ev[7] : nullptr, overlap ? ev[8] : nullptr)) return rc;
     if (ctx->phase_budget)
       if (int rc = enqueue_budget(ctx, ctx->phase_budget_dev + 2 * kBudget)) return rc;
     MARL_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
     if (int rc = run_fate(ctx, p, &st)) return rc;
+    if (overlap) MARL_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev[8], 0));
     if (ctx->phase_budget) {  // against the post-compaction occupancy
       if (int rc = marl_launch_occupancy_from_cells(ctx)) return rc;
       if (int rc = enqueue_budget(ctx, ctx->phase_budget_dev + 3 * kBudget)) return rc;
     }
     MARL_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
     MARL_HIP(ctx, hipEventSynchronize(ev[4]));
-    st.levels = levels;
+    st.levels = (int32_t)off.size() - 1;
     st.n_cells = ctx->n_cells;
     (void)hipEventElapsedTime(&st.ms_field, ev[0], ev[1]);
-    (void)hipEventElapsedTime(&st.ms_schedule, ev[1], ev[2]);
+    (void)hipEventElapsedTime(&st.ms_schedule, ev[5], ev[6]);
     (void)hipEventElapsedTime(&st.ms_kinetics, ev[2], ev[3]);
     (void)hipEventElapsedTime(&st.ms_fate, ev[3], ev[4]);
+    (void)hipEventElapsedTime(&st.ms_tick, ev[0], ev[4]);
+    if (overlap) (void)hipEventElapsedTime(&st.ms_transpose_back, ev[7], ev[8]);
   }
   if (stats) *stats = st;
   return MARL_OK;
@@ -1851,6 +1969,15 @@ int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, in
   int32_t finished = 0;
   for (int t = 0; t < n_ticks; ++t) {
     if (int rc = marl_graph_prepare(ctx, sim)) return rc;
+    // Overlap inside the captured tick (schedule beside the field phase,
+    // transpose back beside fate) pays for large populations only: each
+    // branch of the graph costs a cross-queue wait that small ticks cannot
+    // hide.  (Ensembles capture their columns without it.)
+    const int64_t n = ctx->n_cells;
+    if (ctx->graph_overlap && n < kGraphOverlapCells * 3 / 4) ctx->graph_overlap = false;
+    else if (!ctx->graph_overlap && n >= kGraphOverlapCells) ctx->graph_overlap = true;
+    if (ctx->graph_overlap)
+      if (int rc = marl_ensure_aux(ctx)) return rc;
     const int slot = ctx->cur;
     if (ctx->graph_key[slot] != marl_graph_key(ctx, sim))
       if (int rc = capture_tick(ctx, sim)) return rc;

@@ -154,6 +154,7 @@ void marl_destroy(marl_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->aux) (void)hipStreamSynchronize(c->aux);
   surrogate_free(c->surr);
   for (int k = 0; k < 2; ++k) {
     if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
@@ -169,6 +170,9 @@ void marl_destroy(marl_ctx* c) {
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (auto x : c->xev)
+    if (x) (void)hipEventDestroy(x);
+  if (c->aux) (void)hipStreamDestroy(c->aux);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -180,6 +184,16 @@ int marl_sync(marl_ctx* ctx) {
 }
 
 }  // extern "C"
+
+// The second stream and its fork/join events, created on first use (contexts
+// that only ever tick inside an ensemble never need them).
+int marl_ensure_aux(marl_ctx* ctx) {
+  if (ctx->aux) return MARL_OK;
+  for (auto& x : ctx->xev)
+    if (!x) MARL_HIP(ctx, hipEventCreateWithFlags(&x, hipEventDisableTiming));
+  MARL_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+  return MARL_OK;
+}
 
 int marl_ensure_staging(marl_ctx* ctx) {
   if (!ctx->staging) MARL_HIP(ctx, hipMalloc(&ctx->staging, ctx->nfield * sizeof(float)));

@@ -365,24 +365,57 @@ __device__ inline void cell_kinetics(const marl_cell_state& c, const R& r, AI& I
 // reaction's explicit flux relative to its substrate [16], then 7 zeros.
 // Targets (28): exact minus explicit internal deltas [16] (slot 15, which the
 // tick overwrites with the voxel's light, is not learned) and exact minus
-// explicit exchange fluxes [12].
+// explicit exchange fluxes [12], each divided by its output's scale, the
+// output's gross explicit activity (the sum of the magnitudes of the explicit
+// terms that touch it) plus kGrossEps, and clipped to +-kTargClip.  A slow
+// output then gets a correction as small as its own activity, so the
+// network's error cannot swamp outputs the explicit step already gets right.
+// kGrossEps keeps the f32 rounding of the residual itself (about 6e-8 of an
+// internal of order one) from turning the scaled targets of the slowest
+// outputs into noise.
+// An output that no explicit term touches has scale 0: its target is 0 and it
+// gets no correction.  What the clip cuts off (an output reached only through
+// a chain of reactions within the tick) stays an error of the prediction.
 constexpr int kFeat = 80;
 constexpr int kTarg = 28;
 constexpr int kFeatExplicit = 30;  // first explicit feature
+constexpr float kGrossEps = 1e-5f;
+constexpr float kTargClip = 4.0f;
+
+// Target scales from the gross activities: s[0..15] internals, s[16..27]
+// exchange fluxes.
+__device__ __forceinline__ void target_scales(const float* gI, const float* gD, float* s) {
+#pragma unroll
+  for (int k = 0; k < MARL_M_INT; ++k) s[k] = gI[k] > 0.0f ? gI[k] + kGrossEps : 0.0f;
+#pragma unroll
+  for (int e = 0; e < MARL_S_EXT; ++e) s[16 + e] = gD[e] > 0.0f ? gD[e] + kGrossEps : 0.0f;
+}
+
+// A residual over its scale, clipped (0 where the scale is 0).
+__device__ __forceinline__ float scaled_target(float resid, float scale) {
+  return scale > 0.0f ? clamp_rs(resid / scale, -kTargClip, kTargClip) : 0.0f;
+}
 
 // One explicit step of cell.rs:223-329's rate laws from the pre-tick
 // internals I0 and environment E: transport, reactions (light in slot 15, as
 // the reactions see it), maintenance on the pre-tick energy and effectors, all
 // evaluated at the same state and summed into dI[16] / dD[12] without clamps;
-// frel[q] = reaction q's flux / (its substrate + 1e-6).
-template <typename R, typename AI, typename AE, typename AQ, typename AD>
+// frel[q] = reaction q's flux / (its substrate + 1e-6); gI[16] / gD[12] = the
+// gross activity of each output (sum of the magnitudes of its terms).
+template <typename R, typename AI, typename AE, typename AQ, typename AD, typename AG, typename AH>
 __device__ inline void explicit_step(const marl_cell_state& c, const R& r, const AI& I0, const AE& E, AQ& dI, AD& dD,
-                                     float lt, const CellParams& p, float* frel) {
+                                     float lt, const CellParams& p, float* frel, AG& gI, AH& gD) {
   const float dt = p.dt;
 #pragma unroll
-  for (int k = 0; k < MARL_M_INT; ++k) dI.set(k, 0.0f);
+  for (int k = 0; k < MARL_M_INT; ++k) {
+    dI.set(k, 0.0f);
+    gI.set(k, 0.0f);
+  }
 #pragma unroll
-  for (int e = 0; e < MARL_S_EXT; ++e) dD.set(e, 0.0f);
+  for (int e = 0; e < MARL_S_EXT; ++e) {
+    dD.set(e, 0.0f);
+    gD.set(e, 0.0f);
+  }
 #pragma unroll
   for (int q = 0; q < MARL_S_TRANSPORTERS; ++q) {
     const marl_transport tp = r.transport(q);
@@ -393,6 +426,9 @@ __device__ inline void explicit_step(const marl_cell_state& c, const R& r, const
     const float sec = tp.secrete_rate * ik / (1.0f + ik);
     dI.set(k, dI.get(k) + (up - sec) * dt);
     dD.set(e, dD.get(e) + (sec - up) * dt);
+    const float gross = (fabsf(up) + fabsf(sec)) * dt;
+    gI.set(k, gI.get(k) + gross);
+    gD.set(e, gD.get(e) + gross);
   }
   auto val = [&](int i) { return i == MARL_M_INT - 1 ? lt : I0.get(i); };
   int active = 0;
@@ -412,10 +448,13 @@ __device__ inline void explicit_step(const marl_cell_state& c, const R& r, const
       const float cof = val(fi);
       rate *= cof / (1.0f + cof);
       dI.set(fi, dI.get(fi) - 0.5f * rate * dt);
+      gI.set(fi, gI.get(fi) + fabsf(0.5f * rate * dt));
     }
     const float flux = rate * dt;
     dI.set(si, dI.get(si) - flux);
     dI.set(pi, dI.get(pi) + flux);
+    gI.set(si, gI.get(si) + fabsf(flux));
+    gI.set(pi, gI.get(pi) + fabsf(flux));
     frel[q] = flux / (sv + 1e-6f);
   }
   float pm = 1.0f;
@@ -423,7 +462,9 @@ __device__ inline void explicit_step(const marl_cell_state& c, const R& r, const
     const float evolved = fmaxf(r.fate().w, 1.0f);
     pm = p.prep_maintenance_multiplier + fmaxf(p.base_division_prep - evolved, 0.0f) * p.rush_penalty_rate;
   }
-  dI.set(0, dI.get(0) - p.lambda_maintenance * pm * dt * I0.get(0) - (float)active * p.reaction_maintenance * dt);
+  const float decay = p.lambda_maintenance * pm * dt * I0.get(0), upkeep = (float)active * p.reaction_maintenance * dt;
+  dI.set(0, dI.get(0) - decay - upkeep);
+  gI.set(0, gI.get(0) + fabsf(decay) + fabsf(upkeep));
   if (!c.quiescent) {
 #pragma unroll
     for (int q = 0; q < MARL_S_EFFECTORS; ++q) {
@@ -435,6 +476,8 @@ __device__ inline void explicit_step(const marl_cell_state& c, const R& r, const
         const float amount = ef.rate * ik / (1.0f + ik) * dt;
         dI.set(k, dI.get(k) - amount);
         dD.set(e, dD.get(e) + amount);
+        gI.set(k, gI.get(k) + fabsf(amount));
+        gD.set(e, gD.get(e) + fabsf(amount));
       }
     }
   }

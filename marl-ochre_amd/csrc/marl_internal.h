@@ -263,6 +263,7 @@ struct marl_ctx {
   bool graph_auto = true;                    // budget follows the schedule's needs
   int cell_layout = MARL_CELL_LAYOUT_AUTO;    // field layout of the plain cell pass
   bool kin_quads = false;                    // the layout last chosen: quads in place
+  bool graph_overlap = false;                // the captured tick overlaps phases on ctx->aux
   int kin_block = 128;                       // threads per block of the plain pass (64, 128 or 256)
   long fault_calls = 0;                      // drop-in / tick calls counted for MARL_HIP_FAULT=tick:<n>
   bool phase_budget = false;                 // FP64 mass budget at the tick's phase boundaries
@@ -271,6 +272,22 @@ struct marl_ctx {
   marl_sim_params last_sim{};
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // A second stream for work that overlaps the tick's critical path (the
+  // conflict schedule beside the field phase, the transpose back beside
+  // fate), and untimed events to fork and join it.
+  hipStream_t aux = nullptr;
+  hipEvent_t xev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+// Runs the enclosing scope's launches on another stream: helpers that enqueue
+// on ctx->stream then use that stream; the previous one is restored on exit.
+struct OnStream {
+  marl_ctx* c;
+  hipStream_t prev;
+  OnStream(marl_ctx* ctx, hipStream_t s) : c(ctx), prev(ctx->stream) { c->stream = s; }
+  ~OnStream() { c->stream = prev; }
+  OnStream(const OnStream&) = delete;
+  OnStream& operator=(const OnStream&) = delete;
 };
 
 // helpers implemented in ctx.hip
@@ -336,6 +353,13 @@ int marl_graph_prepare(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_enqueue_tick_begin(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_enqueue_field_phase(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim);
+// The cell phase in two parts: the conflict schedule, which reads only the
+// cells and the occupancy and so may run beside the field phase, and the
+// ordered pass with fate and compaction after both (with fork_back the
+// transpose back of the pass's field view forks onto ctx->aux and joins before
+// the part ends).
+int marl_enqueue_schedule(marl_ctx* ctx, const marl_sim_params* sim);
+int marl_enqueue_pass_and_fate(marl_ctx* ctx, const marl_sim_params* sim, bool fork_back);
 std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* sim);
 int marl_graph_stage(marl_ctx* ctx, hipStream_t st, int64_t tick);
 int marl_graph_fetch(marl_ctx* ctx, hipStream_t st);
@@ -360,6 +384,7 @@ int marl_planes(marl_ctx* ctx, bool to_quads, float* quads);
 int marl_pack_occ(marl_ctx* ctx);
 int marl_unpack_occ(marl_ctx* ctx);
 int marl_ensure_staging(marl_ctx* ctx);
+int marl_ensure_aux(marl_ctx* ctx);
 // Picks (and records in ctx->kin_quads) the plain cell pass's field layout.
 bool marl_choose_cell_layout(marl_ctx* ctx);
 int marl_launch_copy_probe(marl_ctx* ctx);

@@ -20,12 +20,14 @@ struct SurrogateState {
   bool trained = false;
   float zmax = 6.0f;           // out-of-distribution bound on |standardised feature|
   float margin = 0.05f;        // fate margin on the predicted energy
+  float dead_zone = 0.2f;      // corrections up to this (in target-scale units) are dropped
   uint64_t shadow_threshold = 0;  // sampled surrogate cells also run the plain path for monitoring
   float* inv = nullptr;        // 1 / feature std (device, kFeat), kept with the normalisation
   // per-level scratch
   uint16_t* featb = nullptr;
   float* out = nullptr;
-  float* expl = nullptr;       // explicit-step deltas [16] and fluxes [12] per cell of the level
+  float* expl = nullptr;       // per cell of the level: explicit-step deltas [16] and fluxes [12], then
+                               // their target scales [28] (target_scales)
   uint8_t* elig = nullptr;
   int64_t scratch_cap = 0;
   // statistics (device): counters as u64, error histogram, max error bits

@@ -70,6 +70,8 @@ __global__ void __launch_bounds__(kSBlock) k_surr_features(const int32_t* __rest
   __shared__ float sE[kSBlock * 13];
   __shared__ float sQ[kSBlock * 17];
   __shared__ float sR[kSBlock * 13];
+  __shared__ float sG[kSBlock * 17];
+  __shared__ float sH[kSBlock * 13];
   __shared__ unsigned int sst[ST_COUNT];
   stats_init(sst);
   const int t = blockIdx.x * kSBlock + threadIdx.x;
@@ -86,9 +88,11 @@ __global__ void __launch_bounds__(kSBlock) k_surr_features(const int32_t* __rest
     const size_t v = ((size_t)c.pos[2] * p.Y + c.pos[1]) * p.X + c.pos[0];
     gather_env6<false>(field, v, m, p, E);
     const float lt = light[v];
+    float* gI = sG + threadIdx.x * 17;
+    float* gD = sH + threadIdx.x * 13;
     float frel[MARL_R_MAX];
-    PtrArr aQ{dI}, aR{dD};
-    explicit_step(c, AosRules{&rules[i]}, PtrArr{I0}, PtrArr{E}, aQ, aR, lt, p, frel);
+    PtrArr aQ{dI}, aR{dD}, aG{gI}, aH{gD};
+    explicit_step(c, AosRules{&rules[i]}, PtrArr{I0}, PtrArr{E}, aQ, aR, lt, p, frel, aG, aH);
     float f[kFeat];
     make_features(c.internal, c, E, lt, dI, dD, frel, f);
     bool ok = true;
@@ -99,7 +103,8 @@ __global__ void __launch_bounds__(kSBlock) k_surr_features(const int32_t* __rest
       if (!binary_feature(k) && !(fabsf(z) <= zmax)) ok = false;
       fo[k] = bf16_bits(z);
     }
-    float* xo = expl + (size_t)t * kTarg;
+    float* xo = expl + (size_t)t * 2 * kTarg;
+    target_scales(gI, gD, xo + kTarg);
 #pragma unroll
     for (int k = 0; k < MARL_M_INT; ++k) xo[k] = dI[k];
 #pragma unroll
@@ -131,9 +136,10 @@ __device__ inline void record_error(const float* Is, const float* Ip, unsigned i
 }
 
 // One cell of a surrogate level (see k_surr_apply).  The prediction is the
-// explicit step plus the network's residual, clamped as the exact update
-// clamps internals ([0, c_max]); internal 15 is the voxel's light, as the tick
-// sets it.  The gathered face records are kept for the scatter, as in the
+// explicit step plus the network's residual (de-standardised, times the
+// output's target scale), clamped as the exact update clamps
+// internals ([0, c_max]); internal 15 is the voxel's light, as the tick sets
+// it.  The gathered face records are kept for the scatter, as in the
 // plain pass.
 __device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __restrict__ cells,
                                            const marl_ruleset* __restrict__ rules, float* __restrict__ field,
@@ -141,7 +147,8 @@ __device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __rest
                                            uint8_t* __restrict__ events, const CellParams& p,
                                            const float* __restrict__ out, const float* __restrict__ expl,
                                            const uint8_t* __restrict__ elig, const float* __restrict__ norm,
-                                           int mode, float margin, uint64_t shadow_threshold, unsigned int* sst,
+                                           int mode, float margin, float dead_zone, uint64_t shadow_threshold,
+                                           unsigned int* sst,
                                            float* I, float* E, float* D) {
   marl_cell_state c = load_cell(cells + i);
   const AosRules ar{&rules[i]};
@@ -160,13 +167,18 @@ __device__ __forceinline__ void apply_cell(int i, int t, marl_cell_state* __rest
   bool have = elig[t] != 0, use_s = false;
   if (have) {
     const float* y = out + (size_t)t * kTarg;
-    const float* x = expl + (size_t)t * kTarg;
+    const float* x = expl + (size_t)t * 2 * kTarg;
+    const float* g = x + kTarg;
+    // the correction in target-scale units, dropped inside the dead zone
+    auto corr = [&](int k) {
+      const float r = y[k] * tstd[k] + tmean[k];
+      return fabsf(r) > dead_zone ? r * g[k] : 0.0f;
+    };
 #pragma unroll
-    for (int k = 0; k < 15; ++k)
-      Ip[k] = clamp_rs(c.internal[k] + (x[k] + (y[k] * tstd[k] + tmean[k])), 0.0f, p.c_max);
+    for (int k = 0; k < 15; ++k) Ip[k] = clamp_rs(c.internal[k] + (x[k] + corr(k)), 0.0f, p.c_max);
     Ip[15] = lt;
 #pragma unroll
-    for (int e = 0; e < MARL_S_EXT; ++e) Dp[e] = x[16 + e] + (y[16 + e] * tstd[16 + e] + tmean[16 + e]);
+    for (int e = 0; e < MARL_S_EXT; ++e) Dp[e] = x[16 + e] + corr(16 + e);
     const float e1 = Ip[0];
     const float thr[3] = {fmaxf(fate.y, p.hard_death_floor), fate.x, fate.z};
     bool near = false;
@@ -217,7 +229,8 @@ __global__ void __launch_bounds__(kSBlock) k_surr_apply(const int32_t* __restric
                                                       const float* __restrict__ out, const float* __restrict__ expl,
                                                       const uint8_t* __restrict__ elig,
                                                       const float* __restrict__ norm, int mode, float margin,
-                                                      uint64_t shadow_threshold, unsigned long long* __restrict__ stats) {
+                                                      float dead_zone, uint64_t shadow_threshold,
+                                                      unsigned long long* __restrict__ stats) {
   __shared__ float sI[kSBlock * 17];
   __shared__ float sE[kSBlock * 13];
   __shared__ float sD[kSBlock * 13];
@@ -225,7 +238,7 @@ __global__ void __launch_bounds__(kSBlock) k_surr_apply(const int32_t* __restric
   stats_init(sst);
   const int t = blockIdx.x * kSBlock + threadIdx.x;
   if (t < count) apply_cell(order[t], t, cells, rules, field, light, nbmask, events, p, out, expl, elig, norm, mode,
-                            margin, shadow_threshold, sst, sI + threadIdx.x * 17, sE + threadIdx.x * 13,
+                            margin, dead_zone, shadow_threshold, sst, sI + threadIdx.x * 17, sE + threadIdx.x * 13,
                             sD + threadIdx.x * 13);
   stats_flush(sst, stats);
 }
@@ -318,7 +331,7 @@ int ensure_level_scratch(marl_ctx* ctx, int64_t n) {
   const int64_t cap = std::max<int64_t>(n, 65536);
   MARL_HIP(ctx, hipMalloc(&s->featb, sizeof(uint16_t) * cap * kFeat));
   MARL_HIP(ctx, hipMalloc(&s->out, sizeof(float) * cap * kTarg));
-  MARL_HIP(ctx, hipMalloc(&s->expl, sizeof(float) * cap * kTarg));
+  MARL_HIP(ctx, hipMalloc(&s->expl, sizeof(float) * cap * 2 * kTarg));
   MARL_HIP(ctx, hipMalloc(&s->elig, cap));
   s->scratch_cap = cap;
   return MARL_OK;
@@ -380,7 +393,7 @@ int surrogate_level(marl_ctx* ctx, const CellParams& p, const int32_t* order, in
   hipLaunchKernelGGL(k_surr_apply, dim3(gs(cnt, kSBlock)), dim3(kSBlock), 0, ctx->stream, order, cnt, cells, rules,
                      field, ctx->light, ctx->nbmask, ctx->events, p, s->out, s->expl, s->elig, s->mlp.norm,
                      ctx->kin_mode,
-                     s->margin, s->shadow_threshold, s->stats);
+                     s->margin, s->dead_zone, s->shadow_threshold, s->stats);
   MARL_HIP(ctx, hipGetLastError());
   return MARL_OK;
 }
@@ -702,6 +715,15 @@ int marl_surrogate_time_infer(marl_ctx* ctx, int64_t rows, int32_t reps, float* 
   float ms = 0.0f;
   MARL_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   *ms_out = ms / (float)reps;
+  return MARL_OK;
+}
+
+int marl_surrogate_set_dead_zone(marl_ctx* ctx, float dead_zone) {
+  if (!ctx) return MARL_ERR_INVALID_INPUT;
+  if (!(dead_zone >= 0.0f) || !std::isfinite(dead_zone))
+    return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "dead zone must be finite and >= 0");
+  if (int rc = ensure_surr(ctx)) return rc;
+  ctx->surr->dead_zone = dead_zone;
   return MARL_OK;
 }
 

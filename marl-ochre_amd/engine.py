@@ -386,6 +386,11 @@ class MarlGpu:
     def surrogate_configure(self, zmax: float = 6.0, energy_margin: float = 0.05, shadow_rate: float = 0.0) -> None:
         self._chk(self.lib.marl_surrogate_configure(self.h, float(zmax), float(energy_margin), float(shadow_rate)))
 
+    def surrogate_set_dead_zone(self, dead_zone: float) -> None:
+        """Drop corrections of at most `dead_zone` target-scale units (the
+        explicit step stands there)."""
+        self._chk(self.lib.marl_surrogate_set_dead_zone(self.h, float(dead_zone)))
+
     def surrogate_stats(self) -> dict:
         st = _abi.marl_surr_stats()
         self._chk(self.lib.marl_surrogate_stats(self.h, ctypes.byref(st)))

@@ -131,13 +131,17 @@ F32 = np.float32
 F32_EPS = F32(1.1920929e-07)
 
 
-def explicit_step(I0, E, lt, rule, quiescent, prep_remaining, sim):
+def explicit_step(I0, E, lt, rule, quiescent, prep_remaining, sim, gross=False):
     """(dI[16], dD[12], frel[16]): the explicit internal deltas, exchange fluxes
     and per-reaction flux / (substrate + 1e-6) of one cell.  I0: pre-tick
-    internals [16], E: environment [12], lt: the voxel's light."""
+    internals [16], E: environment [12], lt: the voxel's light.  With gross,
+    also (gI[16], gD[12]): each output's gross activity, the sum of the
+    magnitudes of the terms that touch it."""
     dt = F32(sim.dt)
     dI = [F32(0)] * 16
     dD = [F32(0)] * 12
+    gI = [F32(0)] * 16
+    gD = [F32(0)] * 12
     frel = [F32(0)] * 16
     one = F32(1)
     for tp in rule["transport"]:
@@ -149,6 +153,9 @@ def explicit_step(I0, E, lt, rule, quiescent, prep_remaining, sim):
         sec = F32(tp["secrete_rate"]) * ik / (one + ik)
         dI[k] = dI[k] + (up - sec) * dt
         dD[e] = dD[e] + (sec - up) * dt
+        g = (abs(up) + abs(sec)) * dt
+        gI[k] = gI[k] + g
+        gD[e] = gD[e] + g
 
     def val(i):
         return F32(lt) if i == 15 else F32(I0[i])
@@ -168,16 +175,22 @@ def explicit_step(I0, E, lt, rule, quiescent, prep_remaining, sim):
             cof = val(co)
             rate = rate * (cof / (one + cof))
             dI[co] = dI[co] - F32(0.5) * rate * dt
+            gI[co] = gI[co] + abs(F32(0.5) * rate * dt)
         flux = rate * dt
         dI[si] = dI[si] - flux
         dI[pi] = dI[pi] + flux
+        gI[si] = gI[si] + abs(flux)
+        gI[pi] = gI[pi] + abs(flux)
         frel[q] = flux / (sv + F32(1e-6))
     pm = one
     if prep_remaining > 0:
         evolved = max(F32(rule["division_prep_ticks"]), one)
         pm = F32(sim.prep_maintenance_multiplier) + max(F32(sim.base_division_prep) - evolved, F32(0)) * \
             F32(sim.rush_penalty_rate)
-    dI[0] = dI[0] - F32(sim.lambda_maintenance) * pm * dt * F32(I0[0]) - F32(active) * F32(sim.reaction_maintenance) * dt
+    decay = F32(sim.lambda_maintenance) * pm * dt * F32(I0[0])
+    upkeep = F32(active) * F32(sim.reaction_maintenance) * dt
+    dI[0] = dI[0] - decay - upkeep
+    gI[0] = gI[0] + abs(decay) + abs(upkeep)
     if not quiescent:
         for ef in rule["effectors"]:
             k, e = int(ef["int_species"]), int(ef["ext_species"])
@@ -188,4 +201,22 @@ def explicit_step(I0, E, lt, rule, quiescent, prep_remaining, sim):
                 amount = F32(ef["rate"]) * ik / (one + ik) * dt
                 dI[k] = dI[k] - amount
                 dD[e] = dD[e] + amount
-    return np.array(dI, F32), np.array(dD, F32), np.array(frel, F32)
+                gI[k] = gI[k] + abs(amount)
+                gD[e] = gD[e] + abs(amount)
+    out = (np.array(dI, F32), np.array(dD, F32), np.array(frel, F32))
+    return out + (np.array(gI, F32), np.array(gD, F32)) if gross else out
+
+
+def target_scales(gI, gD):
+    """The 28 target scales of a cell: each output's gross activity plus 1e-5
+    (f32), 0 for an output with no gross activity."""
+    return np.array([F32(g) + F32(1e-5) if F32(g) > 0 else F32(0) for g in list(gI) + list(gD)], F32)
+
+
+def scaled_targets(resid, scales):
+    """Residuals over their scales, clipped to +-4, 0 where the scale is 0."""
+    resid = np.asarray(resid, F32)
+    out = np.zeros_like(resid)
+    nz = scales > 0
+    out[nz] = np.clip(resid[nz] / scales[nz], F32(-4), F32(4))
+    return out

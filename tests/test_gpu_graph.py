@@ -59,6 +59,27 @@ def test_graph_tick_bit_identical_with_divisions_and_deaths(pkg, orc, layouts):
     assert seen["host_finished"] == 0
 
 
+_LARGE = {}
+
+
+def _large_population(pkg, dims):
+    if dims not in _LARGE:
+        _LARGE[dims] = population(pkg._abi, dims, 150_000, 31)
+    return _LARGE[dims]
+
+
+@pytest.mark.parametrize("layouts", [("auto", "aos"), ("aos", "quads")])
+def test_graph_tick_overlapped_phases_large_population(pkg, orc, layouts):
+    """From 131,072 cells up the captured tick runs the schedule on a second
+    stream beside the field phase and writes the AoS view back beside fate."""
+    dims = (128, 128, 64)
+    cells, rules = _large_population(pkg, dims)
+    field = np.random.default_rng(32).random(np.prod(dims) * 12).astype(F)
+    sim = pkg.SimulationConfig(rng_seed=81, diffusion_substeps=4)
+    seen = _compare_runs(pkg, dims, field, cells, rules, sim, 2, orc=orc, layouts=layouts)
+    assert seen["deaths"] > 0 and seen["host_finished"] == 0, seen
+
+
 @pytest.mark.parametrize("block", [64, 256])
 def test_graph_tick_every_kinetics_block(pkg, orc, block):
     """The captured tick's level kernels with the other block sizes."""

@@ -133,7 +133,9 @@ def test_harvest_pairs_match_oracle(pkg, orc):
     """Harvested pairs: the state features as each cell saw them, the explicit
     step restated on the host from those inputs (bit for bit), and targets equal
     to the oracle's post-kinetics internals minus the pre-tick ones minus the
-    explicit deltas."""
+    explicit deltas, divided by each output's target scale (its gross explicit
+    activity plus 1e-5; 0, with target 0, for an output no explicit term
+    touches) and clipped to +-4."""
     from marl_ochre_amd import scenarios
 
     A = pkg._abi
@@ -162,12 +164,14 @@ def test_harvest_pairs_match_oracle(pkg, orc):
     np.testing.assert_array_equal(feat[:, 73:], 0)
     for j in range(0, n, 7):   # every 7th pair through the host restatement
         i = idx[j]
-        dI, dD, frel = mlp_ref.explicit_step(cells["internal"][i], feat[j, 16:28], feat[j, 28], rules[i],
-                                             bool(cells["quiescent"][i]), int(cells["prep_remaining"][i]), sim)
+        dI, dD, frel, gI, gD = mlp_ref.explicit_step(cells["internal"][i], feat[j, 16:28], feat[j, 28], rules[i],
+                                                     bool(cells["quiescent"][i]), int(cells["prep_remaining"][i]),
+                                                     sim, gross=True)
         np.testing.assert_array_equal(feat[j, 30:45], dI[:15])
         np.testing.assert_array_equal(feat[j, 45:57], dD)
         np.testing.assert_array_equal(feat[j, 57:73], frel)
-        want = (c_w["internal"][i] - cells["internal"][i]) - dI
+        want = mlp_ref.scaled_targets((c_w["internal"][i] - cells["internal"][i]) - dI,
+                                      mlp_ref.target_scales(gI, gD)[:16])
         np.testing.assert_array_equal(targ[j, :15], want[:15])
         assert targ[j, 15] == 0
 
@@ -191,7 +195,7 @@ def trained(pkg):
     n_pairs = g.harvest_end()
     g.surrogate_train(steps=0, batch=4096, seed=3, reinit=True)
     untrained = g.surrogate_get_params()
-    rep = g.surrogate_train(steps=800, batch=4096, lr=2e-3, seed=3)
+    rep = g.surrogate_train(steps=3000, batch=4096, lr=2e-3, seed=3)
     yield dict(g=g, rep=rep, n_pairs=n_pairs, untrained=untrained, dims=dims, sim=sim)
     g.close()
 
@@ -202,8 +206,9 @@ def _held_out(pkg, dims, n, seed):
 
 
 def test_training_beats_untrained_and_explicit_step(pkg, trained):
-    """Held-out cells of unseen wiring, in raw units: the internal-delta RMS
-    error of explicit step + trained residual against the explicit step alone
+    """Held-out cells of unseen wiring, in target-scale units (residual over
+    each output's gross explicit activity): the internal-delta RMS error of
+    explicit step + trained residual against the explicit step alone
     (residual 0) and against explicit step + untrained network."""
     g, rep = trained["g"], trained["rep"]
     assert trained["n_pairs"] == 400000 and rep["pairs"] == 400000
@@ -229,8 +234,47 @@ def test_training_beats_untrained_and_explicit_step(pkg, trained):
     err_untrained = rms_err(*trained["untrained"])
     err_explicit = np.sqrt(np.mean(targ[:, :15] ** 2))   # the targets are the explicit step's error
     print("rms internal-delta error: net", err_net, "explicit step", err_explicit, "untrained", err_untrained)
-    assert err_net < 0.6 * err_explicit, (err_net, err_explicit)
-    assert err_net < 0.5 * err_untrained, (err_net, err_untrained)
+    assert err_net < 0.97 * err_explicit, (err_net, err_explicit)
+    assert err_net < 0.8 * err_untrained, (err_net, err_untrained)
+
+
+def test_operating_point_keeps_explicit_accuracy_and_fates(pkg, trained):
+    """The shipped operating point (dead zone 0.2) in shadow mode on held-out
+    cells, against the explicit step alone (a dead zone no correction
+    passes): per species, the share of cells within 1e-3 stays within 15 % of
+    the explicit step's, the fate decisions agree on >= 95 % of cells, and no
+    more cells land above a relative error of 1 than under the explicit step
+    alone, within 25 %."""
+    A = pkg._abi
+    g, sim = trained["g"], trained["sim"]
+    c2, r2, f2, l2 = _held_out(pkg, trained["dims"], 50000, 14)
+
+    def shadow(dead_zone):
+        g.upload_field_aos(f2)
+        g.upload_light(l2)
+        g.upload_cells(c2, r2)
+        g.surrogate_configure(zmax=6.0, energy_margin=0.05, shadow_rate=0.0)
+        g.surrogate_set_dead_zone(dead_zone)
+        g.surrogate_reset_stats()
+        g.set_kinetics_mode(A.KINETICS_SHADOW)
+        try:
+            g.cell_pass(sim)
+            return g.surrogate_stats()
+        finally:
+            g.set_kinetics_mode(A.KINETICS_PLAIN)
+            g.surrogate_set_dead_zone(0.2)
+
+    op, ex = shadow(0.2), shadow(1e30)
+    assert op["shadow"] == ex["shadow"] > 0.5 * len(c2)
+    n = op["shadow"]
+    w_op = np.array(op["shadow_within_1e3_species"][:15]) / n
+    w_ex = np.array(ex["shadow_within_1e3_species"][:15]) / n
+    print("within 1e-3 per species: operating point", np.round(w_op, 3), "explicit step", np.round(w_ex, 3))
+    assert (w_op >= 0.85 * w_ex).all(), (w_op, w_ex)
+    assert op["shadow_fate_agree"] >= 0.95 * n
+    big_op, big_ex = sum(op["err_hist"][18:]), sum(ex["err_hist"][18:])   # bins from relative error 1 up
+    print("cells above relative error 1: operating point", big_op, "explicit step", big_ex)
+    assert big_op <= 1.25 * big_ex + 0.01 * n, (big_op, big_ex)
 
 
 def test_surrogate_mode_accounting_and_bounds(pkg, trained):

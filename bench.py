@@ -526,6 +526,8 @@ def ensemble_bench(pkg, ticks=5, warm=2):
 
 
 SURR_FLOP_PER_CELL = 2.0 * (80 * 256 + 256 * 256 * 2 + 256 * 28)  # forward multiply-adds x 2
+# a training pair: the forward, the input gradients of layers 4..2 and every weight gradient
+SURR_TRAIN_FLOP_PER_PAIR = SURR_FLOP_PER_CELL + 2.0 * (28 * 256 + 256 * 256 * 2) + SURR_FLOP_PER_CELL
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
@@ -637,6 +639,9 @@ def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3, pair_target=10_000_
         "pairs_harvested": n_pairs, "harvest_ticks": ticks,
         "train": {k: curve[-1][k] for k in ("train_pairs", "val_loss", "train_ms")},
         "train_pairs_per_s": round(train_steps * 8192 / (curve[-1]["train_ms"] * 1e-3), 0),
+        "train_TFLOPs": round(train_steps * 8192 * SURR_TRAIN_FLOP_PER_PAIR / (curve[-1]["train_ms"] * 1e-3) / 1e12, 2),
+        "train_mfma_frac": round(train_steps * 8192 * SURR_TRAIN_FLOP_PER_PAIR / (curve[-1]["train_ms"] * 1e-3) / 1e12
+                                 / MFMA_BF16_PEAK_TFLOPS, 5),
         "error_vs_training_pairs": [{"train_pairs": 0, "note": "explicit step alone", "shadow": _shadow_summary(sh0)}]
         + curve,
         "error_vs_dead_zone": dead_zone,

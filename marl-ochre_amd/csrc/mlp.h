@@ -29,6 +29,7 @@ struct MlpState {
   float* z[3] = {nullptr, nullptr, nullptr};
   float* h[3] = {nullptr, nullptr, nullptr};
   float *dh = nullptr, *dz = nullptr, *loss_part = nullptr;
+  float* dzpart = nullptr;  // column partials of dZ per block of rows (bias gradients)
   uint16_t* xb = nullptr;
   float* yinf = nullptr;
   size_t infer_cap = 0;

@@ -36,9 +36,16 @@ __device__ __forceinline__ float silu_grad(float z) {
 // ---------------------------------------------------------------------------
 // GEMM: C[M x N] = op(A)[M x K] * op(B)[K x N], f32 in memory, bf16 in MFMA.
 // op(A)[m][k] = TA ? A[k*lda + m] : A[m*lda + k];  op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n].
-// Block 256 threads = 2x2 waves, block tile 64x64, K tile 32; each wave one
-// 32x32 MFMA tile.  LDS holds the A tile as [m][k] and the B tile as [n][k]
-// so that every lane's 8 consecutive-k operands are one 16-byte LDS read.
+// Block 256 threads = 2x2 waves, block tile 64x64; each wave one 32x32 MFMA
+// tile.  K is staged 128 at a time: each thread fetches its 8 + 8 float4 of
+// the next stage into registers while the waves run the current stage's 8
+// MFMAs each, so a product with K <= 128 costs one memory round trip and
+// K = 256 two (the previous design staged 32 at a time, one round trip each,
+// with nothing in flight during the MFMAs).  LDS holds the stage as bf16
+// [m][k] and [n][k], so every lane's 8 consecutive-k operands are one 16-byte
+// LDS read.  The float4 loads run along each operand's contiguous dimension;
+// the feature dimensions (and lda, ldb) must be multiples of 4, the batch
+// dimension (M when !TA, K when TA) may be anything.
 //
 // MFMA 32x32x16 bf16 lane layout: A lane l holds A[l%32][8*(l/32) + i],
 // B lane l holds B[8*(l/32) + i][l%32], D lane l element j holds
@@ -53,7 +60,9 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const float* 
                                               const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
                                               const float* __restrict__ bias, float* __restrict__ Zout,
                                               int kchunk) {
-  constexpr int BM = 64, BN = 64, BK = 32, LP = BK + 8;  // +8 bf16 pad per row
+  constexpr int BM = 64, BN = 64, BK = 128, LP = BK + 8;  // +8 bf16 pad per row
+  constexpr int NV = (BM * BK / 4) / 256;                 // float4 per thread per operand and stage (8)
+  static_assert(BM == BN && NV == 8, "tile shape");
   __shared__ uint16_t As[BM][LP];
   __shared__ uint16_t Bs[BN][LP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -61,36 +70,75 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const float* 
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int k_lo = blockIdx.z * kchunk, k_hi = min(K, k_lo + kchunk);
   C += (size_t)blockIdx.z * M * ldc;
-  f32x16 acc = {};
-  for (int k0 = k_lo; k0 < k_hi; k0 += BK) {
-    // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
+  const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 ra[NV], rb[NV];
+  // Element (i, k) of a stage: A rows i = m, B rows i = n.  A float4 covers
+  // 4 consecutive k (row-major in k) or 4 consecutive rows (k-major).
+  auto load = [&](int k0) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
+    for (int e = 0; e < NV; ++e) {
       const int idx = tid + 256 * e;
-      int mm, kk;
-      if (TA) { kk = idx / BM; mm = idx % BM; }   // walk m fastest: A[k][m] rows are contiguous
-      else { mm = idx / BK; kk = idx % BK; }
-      const int gm = m0 + mm, gk = k0 + kk;
-      float v = 0.0f;
-      if (gm < M && gk < k_hi) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-      As[mm][kk] = to_bf16_bits(v);
-      int nn, kb;
-      if (TB) { nn = idx / BK; kb = idx % BK; }   // B[n][k] rows contiguous in k
-      else { kb = idx / BN; nn = idx % BN; }
-      const int gn = n0 + nn, gkb = k0 + kb;
-      float w = 0.0f;
-      if (gn < N && gkb < k_hi) w = TB ? B[(size_t)gn * ldb + gkb] : B[(size_t)gkb * ldb + gn];
-      Bs[nn][kb] = to_bf16_bits(w);
+      if (!TA) {  // A[m][k]: 4 consecutive k
+        const int mm = idx / (BK / 4), gk = k0 + 4 * (idx % (BK / 4)), gm = m0 + mm;
+        ra[e] = (gm < M && gk < k_hi) ? *reinterpret_cast<const float4*>(A + (size_t)gm * lda + gk) : zero;
+      } else {    // A[k][m]: 4 consecutive m
+        const int gk = k0 + idx / (BM / 4), gm = m0 + 4 * (idx % (BM / 4));
+        ra[e] = (gk < k_hi && gm < M) ? *reinterpret_cast<const float4*>(A + (size_t)gk * lda + gm) : zero;
+      }
+      if (TB) {   // B[n][k]: 4 consecutive k
+        const int nn = idx / (BK / 4), gk = k0 + 4 * (idx % (BK / 4)), gn = n0 + nn;
+        rb[e] = (gn < N && gk < k_hi) ? *reinterpret_cast<const float4*>(B + (size_t)gn * ldb + gk) : zero;
+      } else {    // B[k][n]: 4 consecutive n
+        const int gk = k0 + idx / (BN / 4), gn = n0 + 4 * (idx % (BN / 4));
+        rb[e] = (gk < k_hi && gn < N) ? *reinterpret_cast<const float4*>(B + (size_t)gk * ldb + gn) : zero;
+      }
     }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int e = 0; e < NV; ++e) {
+      const int idx = tid + 256 * e;
+      const float va[4] = {ra[e].x, ra[e].y, ra[e].z, ra[e].w};
+      const float vb[4] = {rb[e].x, rb[e].y, rb[e].z, rb[e].w};
+      if (!TA) {
+        const int mm = idx / (BK / 4), kk = 4 * (idx % (BK / 4));
+        const uint2 w = make_uint2((uint32_t)to_bf16_bits(va[0]) | ((uint32_t)to_bf16_bits(va[1]) << 16),
+                                   (uint32_t)to_bf16_bits(va[2]) | ((uint32_t)to_bf16_bits(va[3]) << 16));
+        *reinterpret_cast<uint2*>(&As[mm][kk]) = w;
+      } else {
+        const int kk = idx / (BM / 4), mm = 4 * (idx % (BM / 4));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) As[mm + j][kk] = to_bf16_bits(va[j]);
+      }
+      if (TB) {
+        const int nn = idx / (BK / 4), kk = 4 * (idx % (BK / 4));
+        const uint2 w = make_uint2((uint32_t)to_bf16_bits(vb[0]) | ((uint32_t)to_bf16_bits(vb[1]) << 16),
+                                   (uint32_t)to_bf16_bits(vb[2]) | ((uint32_t)to_bf16_bits(vb[3]) << 16));
+        *reinterpret_cast<uint2*>(&Bs[nn][kk]) = w;
+      } else {
+        const int kk = idx / (BN / 4), nn = 4 * (idx % (BN / 4));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Bs[nn + j][kk] = to_bf16_bits(vb[j]);
+      }
+    }
+  };
+  f32x16 acc = {};
+  if (k_lo < k_hi) load(k_lo);
+  for (int k0 = k_lo; k0 < k_hi; k0 += BK) {
+    __syncthreads();  // the previous stage's MFMAs have read LDS
+    stash();
     __syncthreads();
+    if (k0 + BK < k_hi) load(k0 + BK);  // in flight during this stage's MFMAs
+    const int kn = min(BK, k_hi - k0);   // zero-filled beyond k_hi: whole 16-k steps only
+    const int half = lane >> 5, r = lane & 31;
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 16) {
-      const int half = lane >> 5, r = lane & 31;
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + r][kk + 8 * half]);
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + r][kk + 8 * half]);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      if (kk < kn) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + r][kk + 8 * half]);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + r][kk + 8 * half]);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      }
     }
-    __syncthreads();
   }
   const int col = n0 + wn * 32 + (lane & 31);
 #pragma unroll
@@ -108,68 +156,107 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const float* 
   }
 }
 
+// The float4 loads need the feature dimensions and leading dimensions to be
+// multiples of 4 (and the operands 16-byte aligned).
+template <bool TA, bool TB>
+bool gemm_shape_ok(int M, int N, int K, const float* A, int lda, const float* B, int ldb) {
+  const bool a_ok = (TA ? M : K) % 4 == 0 && lda % 4 == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+  const bool b_ok = (TB ? K : N) % 4 == 0 && ldb % 4 == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0;
+  return a_ok && b_ok;
+}
+
 template <bool TA, bool TB, int EPI>
 hipError_t gemm(hipStream_t st, int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
                 int ldc, const float* bias = nullptr, float* Z = nullptr) {
+  if (!gemm_shape_ok<TA, TB>(M, N, K, A, lda, B, ldb)) return hipErrorInvalidValue;
   dim3 grid((N + 63) / 64, (M + 63) / 64);
   hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, bias, Z, K);
   return hipGetLastError();
 }
 
-// C[M x N] (ldc = N) = sum over splits of the partials in ws, in split order
-__global__ void k_sum_splits(size_t mn, int splits, const float* __restrict__ ws, float* __restrict__ C) {
+// Finishing kernel of a weight gradient: threads [0, mn) sum the split-K
+// partials of C[M x N] (ldc = N) in split order (none when splits == 0: the
+// product went straight to C); threads [mn, mn + nb) sum the bias gradient's
+// per-block column partials (k_dz_partials) in block order.  The loads of a
+// thread are issued eight at a time ahead of the adds, which keep their order.
+__global__ void k_finish_grads(size_t mn, int splits, const float* __restrict__ ws, float* __restrict__ C, int nb,
+                               int parts, const float* __restrict__ part, float* __restrict__ db) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= mn) return;
-  float acc = ws[i];
-  for (int z = 1; z < splits; ++z) acc += ws[(size_t)z * mn + i];
-  C[i] = acc;
+  auto sum = [](const float* p, size_t stride, int count) {
+    float acc = 0.0f;
+    int z = 0;
+    for (; z + 8 <= count; z += 8) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = p[(size_t)(z + u) * stride];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = (z + u == 0) ? v[u] : acc + v[u];
+    }
+    for (; z < count; ++z) acc = z == 0 ? p[(size_t)z * stride] : acc + p[(size_t)z * stride];
+    return acc;
+  };
+  if (i < mn) {
+    if (splits > 0) C[i] = sum(ws + i, mn, splits);
+  } else if (i < mn + (size_t)nb) {
+    const int n = (int)(i - mn);
+    db[n] = sum(part + n, (size_t)nb, parts);
+  }
 }
 
 constexpr int kMaxSplits = 64;
+constexpr int kDzRows = 64;  // rows per block of k_dz_partials
 
 // Weight gradients have a small output (<= 256 x 256) and K = the batch: the
 // K range is split over enough blocks to fill the chip, partials summed in a
-// fixed order (deterministic).
+// fixed order (deterministic).  The bias gradient's column partials (nb
+// columns, `parts` blocks of k_dz_partials) are summed in the same finishing
+// launch.
 template <bool TA, bool TB>
 hipError_t gemm_splitk(hipStream_t st, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
-                       float* C, float* ws) {
+                       float* C, float* ws, int nb, int parts, const float* part, float* db) {
   const int tiles = ((N + 63) / 64) * ((M + 63) / 64);
   int splits = std::min(kMaxSplits, std::max(1, 1024 / tiles));
   int kchunk = (K + splits - 1) / splits;
   kchunk = (kchunk + 31) / 32 * 32;
   splits = (K + kchunk - 1) / kchunk;
-  if (splits <= 1) return gemm<TA, TB, EPI_STORE>(st, M, N, K, A, lda, B, ldb, C, N);
-  dim3 grid((N + 63) / 64, (M + 63) / 64, splits);
-  hipLaunchKernelGGL((k_gemm<TA, TB, EPI_STORE>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, ws, N, nullptr,
-                     nullptr, kchunk);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  const size_t mn = (size_t)M * N;
-  hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, mn, splits, ws, C);
+  hipError_t e;
+  if (splits <= 1) {
+    if ((e = gemm<TA, TB, EPI_STORE>(st, M, N, K, A, lda, B, ldb, C, N)) != hipSuccess) return e;
+    splits = 0;
+  } else {
+    if (!gemm_shape_ok<TA, TB>(M, N, K, A, lda, B, ldb)) return hipErrorInvalidValue;
+    dim3 grid((N + 63) / 64, (M + 63) / 64, splits);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI_STORE>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, ws, N, nullptr,
+                       nullptr, kchunk);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  const size_t mn = (size_t)M * N, total = mn + (size_t)nb;
+  hipLaunchKernelGGL(k_finish_grads, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, mn, splits, ws, C, nb,
+                     parts, part, db);
   return hipGetLastError();
 }
 
-// dZ = dH * silu'(Z), elementwise
-__global__ void k_silu_back(size_t n, const float* __restrict__ dH, const float* __restrict__ Z,
-                            float* __restrict__ dZ) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dZ[i] = dH[i] * silu_grad(Z[i]);
-}
-
-// db[n] = sum_m dZ[m][n]: one block per column group, fixed-order reduction
-__global__ void __launch_bounds__(256) k_bias_grad(int M, int N, const float* __restrict__ dZ,
-                                                   float* __restrict__ db) {
-  const int n = blockIdx.x;
-  float s = 0.0f;
-  for (int m = threadIdx.x; m < M; m += 256) s += dZ[(size_t)m * N + n];
-  __shared__ float red[256];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+// Column partials of dZ over blocks of kDzRows rows (one thread per column,
+// N <= 256): with SILU, dZ = dH * silu'(Z) is also written; without it dZ is
+// dH itself (the output layer).  part[b][n] = sum of the block's rows in order.
+template <bool SILU>
+__global__ void __launch_bounds__(256) k_dz_partials(int M, int N, const float* __restrict__ dH,
+                                                     const float* __restrict__ Z, float* __restrict__ dZ,
+                                                     float* __restrict__ part) {
+  const int n = threadIdx.x;
+  if (n >= N) return;
+  const int r0 = blockIdx.x * kDzRows, r1 = min(M, r0 + kDzRows);
+  float acc = 0.0f;
+  for (int r = r0; r < r1; ++r) {
+    const size_t i = (size_t)r * N + n;
+    float v = dH[i];
+    if (SILU) {
+      v *= silu_grad(Z[i]);
+      dZ[i] = v;
+    }
+    acc = r == r0 ? v : acc + v;
   }
-  if (threadIdx.x == 0) db[n] = red[0];
+  part[(size_t)blockIdx.x * N + n] = acc;
 }
 
 // MSE on normalised targets: dY = 2 (Y - T) / (M * O), loss partials per block
@@ -417,6 +504,7 @@ int mlp_alloc(marl_ctx* ctx, MlpState& s, int batch) {
     if (s.dh) (void)hipFree(s.dh);
     if (s.dz) (void)hipFree(s.dz);
     if (s.loss_part) (void)hipFree(s.loss_part);
+    if (s.dzpart) (void)hipFree(s.dzpart);
     MARL_HIP(ctx, hipMalloc(&s.x, sizeof(float) * batch * kIn));
     MARL_HIP(ctx, hipMalloc(&s.t, sizeof(float) * batch * kOut));
     MARL_HIP(ctx, hipMalloc(&s.y, sizeof(float) * batch * kOut));
@@ -428,13 +516,14 @@ int mlp_alloc(marl_ctx* ctx, MlpState& s, int batch) {
     MARL_HIP(ctx, hipMalloc(&s.dh, sizeof(float) * batch * kHidden));
     MARL_HIP(ctx, hipMalloc(&s.dz, sizeof(float) * batch * kHidden));
     MARL_HIP(ctx, hipMalloc(&s.loss_part, sizeof(float) * (g1((size_t)batch * kOut) + 1)));
+    MARL_HIP(ctx, hipMalloc(&s.dzpart, sizeof(float) * ((batch + kDzRows - 1) / kDzRows) * kHidden));
     s.batch = batch;
   }
   return MARL_OK;
 }
 
 void mlp_free(MlpState& s) {
-  void* bufs[] = {s.w, s.g, s.m, s.v, s.packed, s.norm, s.ws, s.x, s.t, s.y, s.dy, s.dh, s.dz, s.loss_part,
+  void* bufs[] = {s.w, s.g, s.m, s.v, s.packed, s.norm, s.ws, s.x, s.t, s.y, s.dy, s.dh, s.dz, s.loss_part, s.dzpart,
                   s.z[0], s.z[1], s.z[2], s.h[0], s.h[1], s.h[2], s.xb, s.yinf};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -460,22 +549,24 @@ int mlp_forward_train(marl_ctx* ctx, MlpState& s, int M) {
 // backward from s.dy (dL/dY) into s.g
 int mlp_backward(marl_ctx* ctx, MlpState& s, int M) {
   const MlpLayout L = mlp_layout();
-  // layer 3
+  const int parts = (M + kDzRows - 1) / kDzRows;
+  // layer 3: dY's column partials, then dW4 with db4 in its finishing launch
+  hipLaunchKernelGGL(k_dz_partials<false>, dim3(parts), dim3(256), 0, ctx->stream, M, kOut, s.dy, nullptr, nullptr,
+                     s.dzpart);
+  MARL_HIP(ctx, hipGetLastError());
   MARL_HIP(ctx, (gemm_splitk<true, false>(ctx->stream, kHidden, kOut, M, s.h[2], kHidden, s.dy, kOut,
-                                          s.g + L.w[3], s.ws)));
-  hipLaunchKernelGGL(k_bias_grad, dim3(kOut), dim3(256), 0, ctx->stream, M, kOut, s.dy, s.g + L.b[3]);
+                                          s.g + L.w[3], s.ws, kOut, parts, s.dzpart, s.g + L.b[3])));
   MARL_HIP(ctx, (gemm<false, true, EPI_STORE>(ctx->stream, M, kHidden, kOut, s.dy, kOut, s.w + L.w[3], kOut, s.dh,
                                               kHidden)));
   for (int l = 2; l >= 0; --l) {
-    const size_t n = (size_t)M * kHidden;
-    hipLaunchKernelGGL(k_silu_back, dim3(g1(n)), dim3(256), 0, ctx->stream, n, s.dh, s.z[l], s.dz);
+    // dZ = dH * silu'(Z) and its column partials in one pass
+    hipLaunchKernelGGL(k_dz_partials<true>, dim3(parts), dim3(256), 0, ctx->stream, M, kHidden, s.dh, s.z[l], s.dz,
+                       s.dzpart);
     MARL_HIP(ctx, hipGetLastError());
     const float* hin = l > 0 ? s.h[l - 1] : s.x;
     const int K = l > 0 ? kHidden : kIn;
-    MARL_HIP(ctx, (gemm_splitk<true, false>(ctx->stream, K, kHidden, M, hin, K, s.dz, kHidden, s.g + L.w[l],
-                                            s.ws)));
-    hipLaunchKernelGGL(k_bias_grad, dim3(kHidden), dim3(256), 0, ctx->stream, M, kHidden, s.dz, s.g + L.b[l]);
-    MARL_HIP(ctx, hipGetLastError());
+    MARL_HIP(ctx, (gemm_splitk<true, false>(ctx->stream, K, kHidden, M, hin, K, s.dz, kHidden, s.g + L.w[l], s.ws,
+                                            kHidden, parts, s.dzpart, s.g + L.b[l])));
     if (l > 0)
       MARL_HIP(ctx, (gemm<false, true, EPI_STORE>(ctx->stream, M, K, kHidden, s.dz, kHidden, s.w + L.w[l], kHidden,
                                                   s.dh, kHidden)));

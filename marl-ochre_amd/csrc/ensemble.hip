@@ -74,13 +74,37 @@ void drop(marl_ensemble* e) {
 
 // The batched kernels' view of the columns for the ensemble's tile shape and
 // grid, on the host.
+// The table lists the columns longest stencil tiles first (z-chunk, then
+// row length), so that the first blocks of a launch, whether one block per
+// tile or a persistent grid taking tiles round-robin, get the longest tiles
+// and the shortest ones fill the end.  Only the tile numbering depends on the
+// order; every column's arithmetic is the same.
+// order (if given): table entry k holds column order[k].
 void host_table(const marl_ensemble* e, std::vector<EnsColumn>& cols, int* total_tiles, int* total_light,
-                int* grid) {
+                int* grid, std::vector<int>* order = nullptr) {
   const int n = (int)e->cols.size();
+  const bool has_chunks = (int)e->chunks.size() == n;
+  auto chunk_of = [&](int c) {
+    const int Z = e->cols[c]->d.z, zc = kTiles[e->tile].zc;
+    if (e->whole) return Z;
+    if (has_chunks && e->chunks[c] > 0) return std::min(e->chunks[c], Z);
+    return std::min(zc, Z);
+  };
+  std::vector<int> ord(n);
+  for (int c = 0; c < n; ++c) ord[c] = c;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+    const int za = chunk_of(a), zb = chunk_of(b);
+    if (za != zb) return za > zb;
+    return e->cols[a]->d.x > e->cols[b]->d.x;
+  });
+  if (order) *order = ord;
+  std::vector<int> chunks(n, 0);
   cols.assign(n, EnsColumn{});
-  for (int c = 0; c < n; ++c) {
+  for (int k = 0; k < n; ++k) {
+    const int c = ord[k];
+    if (has_chunks) chunks[k] = e->chunks[c];
     marl_ctx* x = e->cols[c];
-    EnsColumn& t = cols[c];
+    EnsColumn& t = cols[k];
     t.f[0] = x->field[0];
     t.f[1] = x->field[1];
     t.occ = x->occ;
@@ -93,7 +117,7 @@ void host_table(const marl_ensemble* e, std::vector<EnsColumn>& cols, int* total
   }
   const bool pieces = e->per_tile_grid == 2;
   marl_ens_layout(e->tile, cols.data(), n, total_tiles, total_light, e->whole != 0 && !pieces,
-                  (int)e->chunks.size() == n && !pieces ? e->chunks.data() : nullptr);
+                  has_chunks && !pieces ? chunks.data() : nullptr);
   if (pieces) {  // a multiple of 8 blocks, at least one per XCD
     const int g = marl_stencil_ens_grid(e->tile, 1 << 30);
     *grid = std::max(8, g / 8 * 8);
@@ -533,9 +557,10 @@ int marl_ensemble_set_chunks(marl_ensemble* e, const int32_t* chunks, int32_t n)
 int marl_ensemble_get_chunks(const marl_ensemble* e, int32_t* chunks, int32_t n) {
   if (!e || !chunks || n != (int)e->cols.size()) return MARL_ERR_INVALID_INPUT;
   std::vector<EnsColumn> cols;
+  std::vector<int> order;
   int total = 0, light = 0, grid = 0;
-  host_table(e, cols, &total, &light, &grid);
-  for (int c = 0; c < n; ++c) chunks[c] = cols[c].zc;
+  host_table(e, cols, &total, &light, &grid, &order);
+  for (int k = 0; k < n; ++k) chunks[order[k]] = cols[k].zc;
   return MARL_OK;
 }
 

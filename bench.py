@@ -447,11 +447,15 @@ def ensemble_bench(pkg, ticks=5, warm=2):
     nvox = sum(d[0] * d[1] * d[2] for d, *_ in cols)
     ncells = sum(len(c[2]) for c in cols)
 
+    blocks = []   # each column's plain-pass block size, as the ensemble autotuner picked it
+
     def fresh():
         gs = [pkg.MarlGpu(d) for d, *_ in cols]
         for g, (dims, field, cells, rules, _) in zip(gs, cols):
             g.upload_field_aos(field)
             g.upload_cells(cells, rules)
+        for g, b in zip(gs, blocks):
+            g.kinetics_block = b
         return gs
 
     gs = fresh()
@@ -461,7 +465,9 @@ def ensemble_bench(pkg, ticks=5, warm=2):
     try:
         with pkg.MarlEnsemble(gs) as ens:
             tile = ens.autotune(sim)
+            blocks[:] = [g.kinetics_block for g in gs]
             shape = ens.launch_shape()
+            shape["kinetics_blocks"] = {str(b): blocks.count(b) for b in sorted(set(blocks))}
             shape["chunks_by_depth"] = {str(d[2]): z for (d, *_), z in sorted(zip(cols, ens.chunks()),
                                                                                 key=lambda t: t[0][0][2])}
             ms_sub = ens.time_field(sim, 20)

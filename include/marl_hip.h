@@ -398,7 +398,9 @@ int marl_ensemble_get_chunks(const marl_ensemble* e, int32_t* chunks, int32_t n)
  * (both z-chunk modes) and balanced pieces, on the resident columns; then, for the three fastest
  * shapes without whole columns, the z-chunk of each group of equally deep
  * columns (4, 8, 16, 24, 32, 48, 64 planes or the whole depth, deepest group
- * first); keeps the fastest.  Fields and light planes are left unchanged. */
+ * first); keeps the fastest.  Then each column's plain cell-pass block size
+ * (marl_autotune_kinetics on its own population).  Fields, light planes and
+ * cells are left unchanged. */
 int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t* tile_out);
 /* The batched stencil launch: tile shape, grid blocks, logical tiles (the
  * pieces in grid mode 2) and whether the light scan rides the last substep

@@ -513,6 +513,10 @@ int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t
     ENS_HIP(e, hipMemcpyAsync(e->cols[c]->light, saved[c], e->cols[c]->nvox * sizeof(float),
                               hipMemcpyDeviceToDevice, e->stream));
   ENS_HIP(e, hipStreamSynchronize(e->stream));
+  // each column's plain-pass block size on its own population (the column's
+  // state is restored after every trial pass)
+  for (int c = 0; c < n; ++c)
+    if (int rc = marl_autotune_kinetics(e->cols[c], sim, nullptr)) return col_fail(e, c, rc);
   drop(e);
   if (tile_out) *tile_out = best.tile;
   return MARL_OK;

@@ -1087,86 +1087,113 @@ int marl_launch_boundary(marl_ctx* ctx, const marl_sim_params* sim) {
 }
 
 // ---------------------------------------------------------------------------
-// K2: Beer-Lambert light (light.rs:29-59).  One lane per (x, y) column,
-// coalesced across x; the z loop carries the intensity in a register.  The
-// sequential multiply order of the reference is kept per column.  s4 points at
-// species 4 of voxel 0 in the quad field (quad plane 1, lane 0): voxel v's
-// value is s4[4 v].
+// K2: Beer-Lambert light (light.rs:29-59).  One 256-thread block per 64
+// (x, y) columns of one row y.  The per-voxel factors do not depend on the
+// intensity, so they are computed in parallel: the block's waves evaluate
+// expf(-k s4) and the occupancy bit of a chunk of kLightZ planes into LDS,
+// and wave 0 then walks the chunk down each column carrying the intensity in
+// a register, with the reference's operations in its order per voxel (store
+// the incident intensity, times the cell factor if occupied, times the
+// chemical factor, floor).  While wave 0 walks chunk k, waves 1-3 fill chunk
+// k + 1 into the other LDS buffer.  s4 points at species 4 of voxel 0 in the
+// quad field (quad plane 1, lane 0): voxel v's value is s4[4 v].
 // ST = 1: s4 is the compact copy the last substep wrote into the light plane
-// itself (s4 == light; each lane reads a plane's value before writing it).
-template <int ST = 4>
-__device__ __forceinline__ void light_column(const float* s4, const uint32_t* __restrict__ occ, float* light, int x,
-                                             int y, int X, int Y, int Z, int XW, float I0, float e_cell, float nchem,
-                                             float floor_) {
+// itself (s4 == light): a chunk's values are read before any of its planes is
+// written, and the walk writes only planes already read.
+constexpr int kLightZ = 32;
+constexpr int kLightThreads = 256;
+
+template <int ST>
+__device__ __forceinline__ void light_block(const float* s4, const uint32_t* __restrict__ occ, float* light, int x0,
+                                            int y, int X, int Y, int Z, int XW, float I0, float e_cell, float nchem,
+                                            float floor_) {
+  __shared__ float ex[2][kLightZ][64];
+  __shared__ uint8_t ob[2][kLightZ][64];
   // global address space: the ensemble's pointers come from its column table
   const MARL_GLOBAL float* s4g = as_global<float>(s4);
   const MARL_GLOBAL uint32_t* occg = as_global<uint32_t>(occ);
   MARL_GLOBAL float* lg = (MARL_GLOBAL float*)light;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int x = x0 + lane;
+  const bool xin = x < X;
+  const int xc = xin ? x : x0;  // lanes past the row read its first column and write nothing
   const size_t XY = (size_t)X * Y;
-  const size_t col = (size_t)y * X + x;
-  const uint32_t bit = 1u << (x & 31);
-  const size_t wcol = (size_t)y * XW + (x >> 5);
+  const size_t col = (size_t)y * X + xc;
+  const uint32_t bit = 1u << (xc & 31);
+  const size_t wcol = (size_t)y * XW + (xc >> 5);
+  // planes z0 + j, j = the wave's offset from w0, stepping by the nw waves
+  // that fill (4 or 3, so at most kPer planes per wave): every load is
+  // issued, from a clamped address, before any value is used
+  constexpr int kPer = (kLightZ + 2) / 3;
+  auto fill = [&](int buf, int z0, int w0, int nw) {
+    float a[kPer];
+    uint32_t w[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int z = min(z0 + (wv - w0) + u * nw, Z - 1);
+      a[u] = s4g[ST * ((size_t)z * XY + col)];
+      w[u] = occg[(size_t)z * Y * XW + wcol];
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int j = (wv - w0) + u * nw;
+      if (j < kLightZ && z0 + j < Z) {
+        ex[buf][j][lane] = marl_expf(nchem * a[u]);
+        ob[buf][j][lane] = (w[u] & bit) ? 1 : 0;
+      }
+    }
+  };
+  const int nchunks = (Z + kLightZ - 1) / kLightZ;
+  fill(0, 0, 0, kLightThreads / 64);
+  __syncthreads();
   float I = I0;
-  constexpr int U = 8;
-  int z = 0;
-  for (; z + U <= Z; z += U) {
-    float a[U];
-    uint32_t w[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      a[u] = s4g[ST * ((size_t)(z + u) * XY + col)];
-      w[u] = occg[(size_t)(z + u) * Y * XW + wcol];
+  for (int k = 0; k < nchunks; ++k) {
+    const int cur = k & 1;
+    if (wv == 0) {
+      const int z0 = k * kLightZ, zn = min(kLightZ, Z - z0);
+#pragma unroll 8
+      for (int j = 0; j < zn; ++j) {
+        if (xin) lg[(size_t)(z0 + j) * XY + col] = I;
+        if (ob[cur][j][lane]) I *= e_cell;
+        I *= ex[cur][j][lane];
+        if (I < floor_) I = 0.0f;
+      }
+    } else if (k + 1 < nchunks) {
+      fill(cur ^ 1, (k + 1) * kLightZ, 1, kLightThreads / 64 - 1);
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      lg[(size_t)(z + u) * XY + col] = I;
-      if (w[u] & bit) I *= e_cell;
-      I *= marl_expf(nchem * a[u]);
-      if (I < floor_) I = 0.0f;
-    }
-  }
-  for (; z < Z; ++z) {
-    const float a = s4g[ST * ((size_t)z * XY + col)];  // read before the write: s4 may be the light plane
-    lg[(size_t)z * XY + col] = I;
-    if (occg[(size_t)z * Y * XW + wcol] & bit) I *= e_cell;
-    I *= marl_expf(nchem * a);
-    if (I < floor_) I = 0.0f;
+    __syncthreads();
   }
 }
 
 template <int ST>
-__global__ void __launch_bounds__(64) k_light(const float* s4, const uint32_t* __restrict__ occ, float* light, int X,
-                                              int Y, int Z, int XW, float I0, float e_cell, float nchem,
-                                              float floor_) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  if (x >= X) return;
-  light_column<ST>(s4, occ, light, x, blockIdx.y, X, Y, Z, XW, I0, e_cell, nchem, floor_);
+__global__ void __launch_bounds__(kLightThreads) k_light(const float* s4, const uint32_t* __restrict__ occ,
+                                                         float* light, int X, int Y, int Z, int XW, float I0,
+                                                         float e_cell, float nchem, float floor_) {
+  light_block<ST>(s4, occ, light, blockIdx.x * 64, blockIdx.y, X, Y, Z, XW, I0, e_cell, nchem, floor_);
 }
 
 // Ensembles: one launch over the (x, y) columns of every column of the ensemble.
 template <int ST>
-__global__ void __launch_bounds__(64) k_light_ens(const EnsColumn* __restrict__ cols, int ncols, float I0,
-                                                  float e_cell, float nchem, float floor_) {
+__global__ void __launch_bounds__(kLightThreads) k_light_ens(const EnsColumn* __restrict__ cols, int ncols, float I0,
+                                                             float e_cell, float nchem, float floor_) {
   int c = 0;
   const int b = blockIdx.x;
   while (c + 1 < ncols && cols[c + 1].light_base <= b) ++c;
   const EnsColumn& e = cols[c];
   const int local = b - e.light_base;
-  const int x = (local % e.lbx) * 64 + threadIdx.x;
-  if (x >= e.X) return;
   const size_t nvox = (size_t)e.X * e.Y * e.Z;
-  light_column<ST>(ST == 1 ? e.light : e.f[e.cur] + 4 * nvox, e.occ, e.light, x, local / e.lbx, e.X, e.Y, e.Z, e.XW,
-                   I0, e_cell, nchem, floor_);
+  light_block<ST>(ST == 1 ? e.light : e.f[e.cur] + 4 * nvox, e.occ, e.light, (local % e.lbx) * 64, local / e.lbx,
+                  e.X, e.Y, e.Z, e.XW, I0, e_cell, nchem, floor_);
 }
 
 hipError_t marl_launch_light_ens(hipStream_t st, const EnsColumn* dcols, int ncols, int total_blocks,
                                  const marl_sim_params* sim, bool compact) {
   const float e_cell = marl_expf(-sim->cell_absorption);
   if (compact)
-    hipLaunchKernelGGL(k_light_ens<1>, dim3(total_blocks), dim3(64), 0, st, dcols, ncols, sim->surface_intensity,
+    hipLaunchKernelGGL(k_light_ens<1>, dim3(total_blocks), dim3(kLightThreads), 0, st, dcols, ncols, sim->surface_intensity,
                        e_cell, -sim->chemical_absorption, sim->light_floor);
   else
-    hipLaunchKernelGGL(k_light_ens<4>, dim3(total_blocks), dim3(64), 0, st, dcols, ncols, sim->surface_intensity,
+    hipLaunchKernelGGL(k_light_ens<4>, dim3(total_blocks), dim3(kLightThreads), 0, st, dcols, ncols, sim->surface_intensity,
                        e_cell, -sim->chemical_absorption, sim->light_floor);
   return hipGetLastError();
 }
@@ -1175,7 +1202,7 @@ int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim) {
   const int X = ctx->d.x, Y = ctx->d.y, Z = ctx->d.z;
   const float* s4 = ctx->field[ctx->cur] + (size_t)4 * ctx->nvox;
   const float e_cell = marl_expf(-sim->cell_absorption);
-  hipLaunchKernelGGL(k_light<4>, dim3((X + 63) / 64, Y), dim3(64), 0, ctx->stream, s4, ctx->occ, ctx->light, X, Y,
+  hipLaunchKernelGGL(k_light<4>, dim3((X + 63) / 64, Y), dim3(kLightThreads), 0, ctx->stream, s4, ctx->occ, ctx->light, X, Y,
                      Z, ctx->xw, sim->surface_intensity, e_cell, -sim->chemical_absorption, sim->light_floor);
   MARL_HIP(ctx, hipGetLastError());
   return MARL_OK;
@@ -1186,7 +1213,7 @@ int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim) {
 int marl_launch_light_compact(marl_ctx* ctx, const marl_sim_params* sim) {
   const int X = ctx->d.x, Y = ctx->d.y, Z = ctx->d.z;
   const float e_cell = marl_expf(-sim->cell_absorption);
-  hipLaunchKernelGGL(k_light<1>, dim3((X + 63) / 64, Y), dim3(64), 0, ctx->stream, ctx->light, ctx->occ, ctx->light,
+  hipLaunchKernelGGL(k_light<1>, dim3((X + 63) / 64, Y), dim3(kLightThreads), 0, ctx->stream, ctx->light, ctx->occ, ctx->light,
                      X, Y, Z, ctx->xw, sim->surface_intensity, e_cell, -sim->chemical_absorption, sim->light_floor);
   MARL_HIP(ctx, hipGetLastError());
   return MARL_OK;

@@ -395,10 +395,10 @@ int marl_ensemble_set_chunks(marl_ensemble* e, const int32_t* chunks, int32_t n)
 int marl_ensemble_get_chunks(const marl_ensemble* e, int32_t* chunks, int32_t n);
 /* Times a tick's batched field phase (substeps and light) for every stencil
  * tile shape, with a persistent grid (both z-chunk modes), one block per tile
- * (both z-chunk modes) and balanced pieces, on the resident columns; then, for the three fastest
+ * (both z-chunk modes) and balanced pieces, on the resident columns; then, for the five fastest
  * shapes without whole columns, the z-chunk of each group of equally deep
  * columns (4, 8, 16, 24, 32, 48, 64 planes or the whole depth, deepest group
- * first); keeps the fastest.  Then each column's plain cell-pass block size
+ * first, two passes); keeps the fastest.  Then each column's plain cell-pass block size
  * (marl_autotune_kinetics on its own population).  Fields, light planes and
  * cells are left unchanged. */
 int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t* tile_out);

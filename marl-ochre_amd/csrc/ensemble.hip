@@ -465,10 +465,10 @@ int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t
   std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.ms < b.ms; });
   Cand best = cands.front();
   std::vector<int> best_chunks(n, 0);
-  // Then, for the three fastest shapes without whole columns, the z-chunk of
-  // each group of equally deep columns (deepest first, one pass): a column's
-  // chunk trades the halo planes every chunk re-reads against how evenly the
-  // tiles fill the chip.
+  // Then, for the five fastest shapes without whole columns, the z-chunk of
+  // each group of equally deep columns (deepest first, two passes): a
+  // column's chunk trades the halo planes every chunk re-reads against how
+  // evenly the tiles fill the chip.
   std::vector<int> depths;
   for (marl_ctx* x : e->cols) depths.push_back(x->d.z);
   std::sort(depths.begin(), depths.end(), std::greater<int>());
@@ -476,15 +476,18 @@ int marl_ensemble_autotune(marl_ensemble* e, const marl_sim_params* sim, int32_t
   const int opts[] = {4, 8, 16, 24, 32, 48, 64};
   int tried = 0;
   for (const Cand& c0 : cands) {
-    if (c0.whole || c0.grid == 2 || tried == 3) continue;  // the pieces set their own chunks
+    if (c0.whole || c0.grid == 2 || tried == 5) continue;  // the pieces set their own chunks
     ++tried;
     e->tile = c0.tile;
     e->per_tile_grid = c0.grid;
     e->whole = 0;
     e->chunks.assign(n, 0);
     float cur_ms = c0.ms;
+    for (int pass = 0; pass < 2; ++pass)
     for (int depth : depths) {
       int keep_zc = 0;
+      for (int c = 0; c < n; ++c)  // the second pass starts from the first pass's choice
+        if (e->cols[c]->d.z == depth) keep_zc = e->chunks[c];
       for (int k = -1; k < (int)(sizeof(opts) / sizeof(opts[0])); ++k) {
         const int zc = k < 0 ? depth : opts[k];  // k = -1: one chunk per column
         if (k >= 0 && zc >= depth) continue;

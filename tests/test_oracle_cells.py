@@ -59,6 +59,25 @@ def test_transport_is_sequential_and_clamped(pkg, orc, A):
     assert D[1] == F(-1.0) and D[5] == pytest.approx(1.0, rel=1e-6)
 
 
+def test_secretion_past_the_clamp_creates_matter(pkg, orc, A):
+    # cell.rs:231-237: the internal clamps at 0 while the full secretion * dt still
+    # leaves as an exchange flux, so one tick can add matter (why the surrogate
+    # applies no "net change <= 0" projection)
+    c, r = blank(A)
+    c["internal"][0] = 1.0
+    c["internal"][3] = 0.5
+    r["transport"][0] = (0.0, 3.0, 6, 3)   # secretes 3 * 0.5 / 1.5 = 1.0 per unit time
+    sim = sim_no_maint(pkg)
+    dt = F(sim.dt)
+    sec = F(3.0) * F(0.5) / (F(1.0) + F(0.5))
+    assert sec * dt > F(0.5)               # more than the cell holds
+    c2, D, _ = orc.cell_tick(c, r, np.zeros(12, F), 0.0, sim)
+    assert c2["internal"][3] == F(0.0)
+    assert D[6] == sec * dt
+    created = (float(c2["internal"][:15].sum()) - float(c["internal"][:15].sum())) + float(D.sum())
+    assert created == pytest.approx(float(sec * dt) - 0.5, rel=1e-6) and created > 0
+
+
 def test_reaction_known_answer_and_epsilon_background(pkg, orc, A):
     # cell.rs:261-284: no catalyst -> rate = vmax * s/(km+s+eps) * epsilon
     c, r = blank(A)

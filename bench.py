@@ -108,6 +108,13 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
     field, cells, rules, sim = scenarios.kinetics_at_scale(dims, 1_000_000, seed=0)
     n0 = len(cells)
     g = pkg.MarlGpu(dims)
+    # the random-access ceiling the cell pass works against: 64-byte sectors at
+    # hashed positions of a 1 GiB buffer
+    import ctypes
+    gms, gby = ctypes.c_float(), ctypes.c_double()
+    g._chk(g.lib.marl_gather_probe(g.h, 10, ctypes.byref(gms), ctypes.byref(gby)))
+    gather_gbs = 10 * gby.value / (gms.value * 1e-3) / 1e9
+    log(f"gather probe: {gather_gbs:.0f} GB/s of random 64-byte sectors")
     # the kinetics autotuner on the resident state, then each block size's pass for the record
     g.upload_field_aos(field)
     g.upload_cells(cells, rules)
@@ -196,6 +203,8 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
             "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
             "achieved": round(KINETICS_BYTES_PER_CELL * n0 / (kin_ms * 1e-3) / 1e9, 1),
             "bytes_per_cell": KINETICS_BYTES_PER_CELL,
+            # what scattered 64-byte reads reach on this box (the pass reads scattered records)
+            "random_sector_ceiling_GBs": round(gather_gbs, 1),
         },
         "trajectory": {"ticks": ticks, "ms_per_tick": round(traj_wall * 1e3 / max(ticks, 1), 3),
                        "cells_after": traj[-1][1]["n_cells"] if traj else n0},

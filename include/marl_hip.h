@@ -308,6 +308,11 @@ int marl_stencil_tile_split(int32_t tile, int32_t* species_per_thread, int32_t* 
  * float4 loads/stores): the HBM calibration kernel for roofline and counter
  * scaling.  *bytes_out = bytes read + written per rep. */
 int marl_copy_probe(marl_ctx* ctx, int32_t n_reps, float* ms_out, double* bytes_out);
+/* Random-access calibration (no reference counterpart): n_reps launches in
+ * which every 4 lanes read one 64-byte sector at a hashed position of a 1 GiB
+ * buffer the call allocates and frees, 4 sectors per thread; the ceiling of
+ * kernels that read scattered records.  *bytes_out = sector bytes read per rep. */
+int marl_gather_probe(marl_ctx* ctx, int32_t n_reps, float* ms_out, double* bytes_out);
 int marl_sync(marl_ctx* ctx);
 
 /* ---- outputs: reductions and packing on the resident state ---------------- */

@@ -383,6 +383,40 @@ int marl_copy_probe(marl_ctx* ctx, int32_t n, float* ms, double* bytes) {
   return MARL_OK;
 }
 
+// Sectors of a 1 GiB buffer (four times the last-level cache), allocated for
+// the measurement and freed after it; contents are irrelevant.
+int marl_gather_probe(marl_ctx* ctx, int32_t n, float* ms, double* bytes) {
+  if (!ctx || !ms || n <= 0) return MARL_ERR_INVALID_INPUT;
+  constexpr size_t kSrcBytes = (size_t)1 << 30, kThreads = (size_t)1 << 24;
+  void* src = nullptr;
+  float* out = nullptr;
+  MARL_HIP(ctx, hipMalloc(&src, kSrcBytes));
+  if (hipMalloc(&out, kThreads * sizeof(float)) != hipSuccess) {
+    (void)hipFree(src);
+    return marl_fail(ctx, MARL_ERR_TRANSFER, "gather probe: allocation failed");
+  }
+  struct Free {
+    void* a;
+    void* b;
+    ~Free() {
+      (void)hipFree(a);
+      (void)hipFree(b);
+    }
+  } fr{src, out};
+  const uint32_t nsec = (uint32_t)(kSrcBytes / 64);
+  double per = 0.0;
+  if (int rc = marl_launch_gather_probe(ctx, src, nsec, out, kThreads, 0x9e3779b9u, &per)) return rc;  // warm-up
+  MARL_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int k = 0; k < n; ++k)
+    if (int rc = marl_launch_gather_probe(ctx, src, nsec, out, kThreads, (uint32_t)k * 0x85ebca6bu + 1u, &per))
+      return rc;
+  MARL_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  MARL_HIP(ctx, hipEventSynchronize(ctx->ev1));
+  MARL_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+  if (bytes) *bytes = per;
+  return MARL_OK;
+}
+
 int marl_stencil_tile_count(void) { return kNumTiles; }
 
 int marl_stencil_tile_substeps(int32_t tile) {

@@ -1338,6 +1338,46 @@ __global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ a
   if (i < n4) b[i] = a[i];
 }
 
+// Gather probe: the random-access ceiling for scattered-record kernels (the
+// cell pass).  Every 4 consecutive lanes read one 64-byte sector of the
+// resident field at a hashed position, four sectors per thread, and each
+// thread writes one float of the spare buffer so the reads are kept.
+// Positions are a hash modulo the sector count, so every read is in bounds.
+__device__ __forceinline__ uint32_t probe_hash(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+constexpr int kGatherPerThread = 4;
+
+__global__ void __launch_bounds__(256) k_gather_probe(const float4* __restrict__ a, uint32_t nsectors,
+                                                      float* __restrict__ out, uint32_t salt) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  float acc = 0.0f;
+#pragma unroll
+  for (int r = 0; r < kGatherPerThread; ++r) {
+    const uint32_t s = probe_hash(((t >> 2) * kGatherPerThread + r) ^ salt) % nsectors;
+    const float4 v = a[(size_t)s * 4 + (t & 3)];
+    acc += v.x + v.y + v.z + v.w;
+  }
+  out[t] = acc;
+}
+
+// src: nsectors 64-byte sectors; out: one float per thread (threads a
+// multiple of 256).
+int marl_launch_gather_probe(marl_ctx* ctx, const void* src, uint32_t nsectors, float* out, size_t threads,
+                             uint32_t salt, double* sector_bytes) {
+  hipLaunchKernelGGL(k_gather_probe, dim3((unsigned)(threads / 256)), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<const float4*>(src), nsectors, out, salt);
+  MARL_HIP(ctx, hipGetLastError());
+  if (sector_bytes) *sector_bytes = (double)(threads / 4) * kGatherPerThread * 64.0;
+  return MARL_OK;
+}
+
 int marl_launch_copy_probe(marl_ctx* ctx) {
   const size_t n4 = ctx->nfield / 4;
   hipLaunchKernelGGL(k_copy_probe, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, ctx->stream,

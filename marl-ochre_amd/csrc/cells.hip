@@ -1889,6 +1889,13 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
         if (e[k]) (void)hipEventDestroy(e[k]);
     }
   } guard{ev};
+  // every exit, error paths included, waits for the second stream, so that no
+  // transpose of this call is still writing the field when the call returns
+  // (after a normal tick it has finished already)
+  struct AuxJoin {
+    hipStream_t aux;
+    ~AuxJoin() { (void)hipStreamSynchronize(aux); }
+  } aux_join{ctx->aux};
   marl_tick_stats st{};
   std::vector<int32_t> off;
   for (int t = 0; t < n_ticks; ++t) {

@@ -7,6 +7,8 @@
 - plain kinetics at scale, 1,000,000 cells on 256x256x128: two full ticks
   (crates/marl-sim/src/lib.rs:160-277) against the oracle's tick — field,
   light, every cell and ruleset field, the tick's counters;
+- the same workload through the captured graph (phases overlapped on a
+  second stream at this population);
 - the 32-column uneven ensemble (x = y in {64, 96, 128, 192}, z = x / 2),
   autotuned and advanced in one captured graph per tick: two ticks, each
   column against the oracle's tick with that column's seed.
@@ -71,6 +73,31 @@ def test_kinetics_at_scale_two_ticks(pkg, orc):
             for k in ("divisions", "deaths", "births_added", "n_cells"):
                 assert gs[k] == st[k], (t, k, gs[k], st[k])
             assert st["divisions"] + st["deaths"] > 0, st
+            cg, rg = g.download_cells()
+            assert_cells_equal(cg, rg, c, r)
+            assert _bits_equal(g.download_field_aos(), f), t
+            assert _bits_equal(g.download_light(), lt), t
+
+
+def test_kinetics_at_scale_captured_ticks(pkg, orc):
+    """The 1,000,000-cell workload through the captured graph, which from
+    131,072 cells up runs the schedule on a second stream beside the field
+    phase and the transpose back beside fate: two ticks against the oracle."""
+    from marl_ochre_amd import scenarios
+
+    dims = (256, 256, 128)
+    field, cells, rules, sim = scenarios.kinetics_at_scale(dims, 1_000_000, seed=0)
+    with pkg.MarlGpu(dims) as g:
+        g.upload_field_aos(field)
+        g.upload_cells(cells, rules)
+        f, c, r = field, cells, rules
+        for t in range(2):
+            print("1M-cell captured tick", t, flush=True)
+            f, lt, c, r, ev, st = orc.tick(dims, sim, t, f, c, r, nthreads=THREADS)
+            gs = g.tick_graph(sim, t, 1)
+            assert gs["host_finished"] == 0
+            for k in ("divisions", "deaths", "births_added", "n_cells"):
+                assert gs[k] == st[k], (t, k, gs[k], st[k])
             cg, rg = g.download_cells()
             assert_cells_equal(cg, rg, c, r)
             assert _bits_equal(g.download_field_aos(), f), t

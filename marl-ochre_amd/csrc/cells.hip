@@ -172,10 +172,19 @@ __global__ void k_cell_prep(const marl_cell_state* __restrict__ cells, int n, co
 
 // Longest-path relaxation on the conflict DAG; monotone, converges to the
 // exact level in (depth + 1) sweeps or fewer.
-__global__ void k_level_relax(int n, const int32_t* __restrict__ preds, size_t ps, const uint8_t* __restrict__ npred,
-                              int32_t* level, int32_t* changed) {
+// Sweep k (0-based) of the monotone relaxation.  Levels only rise and never
+// pass the longest-path depth; after k sweeps every level is at least
+// min(depth, k) (the synchronous bound, which in-place updates only beat), so
+// a level below k is already the depth and the cell is skipped without
+// reading its predecessors' levels: the sweeps' scattered reads shrink with
+// the frontier instead of costing every cell every sweep.
+__device__ __forceinline__ bool relax_settled(const int32_t* level, int j, int k) { return level[j] < k; }
+
+__global__ void k_level_relax(int k, int n, const int32_t* __restrict__ preds, size_t ps,
+                              const uint8_t* __restrict__ npred, int32_t* level, int32_t* changed) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
+  if (relax_settled(level, j, k)) return;
   const int np = npred[j];
   if (np == 0) return;
   const int32_t* pj = preds + j;
@@ -784,6 +793,7 @@ __global__ void k_g_level_relax(int k, TickDev* ts, const int32_t* __restrict__ 
   const int n = ts->n;
   bool any = false;
   GS_LOOP(j, n) {
+    if (relax_settled(level, (int)j, k)) continue;
     const int np = npred[j];
     if (np == 0) continue;
     const int32_t* pj = preds + j;
@@ -1173,7 +1183,7 @@ int run_schedule(marl_ctx* ctx, const CellParams& p, std::vector<int32_t>& off) 
     if (done > kMaxLevels + 2) return marl_fail(ctx, MARL_ERR_TRANSFER, "conflict schedule did not converge");
     if (int rc = fill_i32(ctx, changed, 0, batch)) return rc;
     for (int k = 0; k < batch; ++k)
-      hipLaunchKernelGGL(k_level_relax, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->preds,
+      hipLaunchKernelGGL(k_level_relax, dim3(grid1(n)), dim3(256), 0, ctx->stream, done + k, n, ctx->preds,
                          (size_t)ctx->cap, ctx->npred, ctx->level, changed + k);
     MARL_HIP(ctx, hipGetLastError());
     if (int rc = d2h_sync(ctx, h, changed, sizeof(int32_t) * batch)) return rc;

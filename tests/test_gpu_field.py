@@ -187,3 +187,21 @@ def test_temporal_tiles_bit_identical(pkg, orc, dims):
             g.upload_field_aos(f)
             g.tick_graph(sim, 0, 1)
             assert np.array_equal(g.download_field_aos(), fw), t
+
+
+def test_calibration_probes_report_their_bytes(pkg):
+    """The copy probe moves the field's bytes twice per rep; the gather probe
+    reads 2^24 / 4 groups x 4 sectors x 64 B per rep; both take measurable time
+    and leave the resident field as it was."""
+    import ctypes
+
+    dims = (128, 128, 64)
+    field = np.random.default_rng(3).random(np.prod(dims) * 12).astype(np.float32)
+    with pkg.MarlGpu(dims) as g:
+        g.upload_field_aos(field)
+        ms, by = ctypes.c_float(), ctypes.c_double()
+        g._chk(g.lib.marl_copy_probe(g.h, 3, ctypes.byref(ms), ctypes.byref(by)))
+        assert ms.value > 0 and by.value == 2.0 * field.size * 4
+        g._chk(g.lib.marl_gather_probe(g.h, 3, ctypes.byref(ms), ctypes.byref(by)))
+        assert ms.value > 0 and by.value == (1 << 24) // 4 * 4 * 64.0
+        assert np.array_equal(g.download_field_aos(), field)

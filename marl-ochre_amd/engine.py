@@ -315,6 +315,12 @@ class MarlGpu:
         self._chk(self.lib.marl_harvest_end(self.h, ctypes.byref(n)))
         return n.value
 
+    def harvest_count(self) -> int:
+        """Pairs kept so far (at most the capacity), without ending the harvest."""
+        n = ctypes.c_int64()
+        self._chk(self.lib.marl_harvest_download(self.h, None, None, None, 0, ctypes.byref(n)))
+        return n.value
+
     def harvest_download(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         n = ctypes.c_int64()
         self._chk(self.lib.marl_harvest_download(self.h, None, None, None, 0, ctypes.byref(n)))

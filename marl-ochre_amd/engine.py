@@ -26,6 +26,21 @@ def _params(sim) -> _abi.marl_sim_params:
     return sim.to_params()
 
 
+def _records(a, dt: np.dtype) -> np.ndarray:
+    """`a` as contiguous records of the C layout `dt`.  Records of another
+    layout (numpy packs the padding out of structured arrays that pass through
+    np.concatenate) are copied field by field into a zero-filled array, so the
+    reserved and padding bytes the library stores are zeros, not whatever an
+    uninitialised conversion buffer held."""
+    a = np.asarray(a)
+    if a.dtype == dt and a.flags.c_contiguous:
+        return a
+    out = np.zeros(a.shape, dt)
+    for name in dt.names:
+        out[name] = a[name]
+    return out
+
+
 class MarlGpu:
     """One GPU context (≙ GpuFieldDiffuser::new + the resident state)."""
 
@@ -178,8 +193,8 @@ class MarlGpu:
 
     # -- cells ----------------------------------------------------------------
     def upload_cells(self, cells: np.ndarray, rules: np.ndarray) -> None:
-        c = np.ascontiguousarray(cells, dtype=CELL_DTYPE)
-        r = np.ascontiguousarray(rules, dtype=RULESET_DTYPE)
+        c = _records(cells, CELL_DTYPE)
+        r = _records(rules, RULESET_DTYPE)
         if c.shape != r.shape:
             raise _abi.InvalidInput("cells and rulesets differ in length")
         self._chk(self.lib.marl_upload_cells(self.h, ptr(c), ptr(r), c.size))

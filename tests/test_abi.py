@@ -142,3 +142,41 @@ def test_topology_hash_groups_wiring(pkg):
     assert h[0] == h[3] and len(set(h[:3].tolist())) == 3
     r["reactions"]["substrate"][3, 5] = (int(r["reactions"]["substrate"][3, 5]) + 1) % 15
     assert scenarios.topology_hash(r)[3] != h[0]
+
+
+def test_uploaded_records_carry_zero_padding(pkg):
+    """np.concatenate packs the padding out of structured records (the default
+    column is built that way).  The upload's conversion back to the C layout
+    copies field by field into zeros, so padding and reserved bytes reach the
+    device as zeros on every run; records already in the C layout pass through
+    without a copy."""
+    from marl_ochre_amd import engine, scenarios
+
+    _, cells, rules, _ = scenarios.winogradsky_column((128, 128, 64), starter_state=True)
+    A = pkg._abi
+    assert cells.dtype != A.CELL_DTYPE and rules.dtype != A.RULESET_DTYPE   # packed by np.concatenate
+    c = engine._records(cells, A.CELL_DTYPE)
+    r = engine._records(rules, A.RULESET_DTYPE)
+    assert c.dtype == A.CELL_DTYPE and r.dtype == A.RULESET_DTYPE
+    cb = c.view(np.uint8).reshape(len(c), -1)
+    rb = r.view(np.uint8).reshape(len(r), -1)
+    named = np.zeros(A.RULESET_DTYPE.itemsize, bool)
+    for name, (sub, off) in A.RULESET_DTYPE.fields.items():
+        base, count = (sub.base, int(np.prod(sub.shape))) if sub.shape else (sub, 1)
+        for k in range(count):
+            o = off + k * base.itemsize
+            if base.names:
+                for fname, (fd, fo) in base.fields.items():
+                    named[o + fo:o + fo + fd.itemsize] = True
+            else:
+                named[o:o + base.itemsize] = True
+    assert (~named).sum() == 8 * 2 + 8 * 2 + 8   # transport and effector pads, reserved tail
+    assert not cb[:, 86:].any() and not rb[:, ~named].any()
+    for name in cells.dtype.names:
+        assert np.array_equal(c[name], cells[name]), name
+    for name in rules.dtype.names:
+        sub = A.RULESET_DTYPE[name].base
+        for f in (sub.names or [None]):
+            got, want = (r[name], rules[name]) if f is None else (r[name][f], rules[name][f])
+            assert np.array_equal(got, want), (name, f)
+    assert engine._records(c, A.CELL_DTYPE) is c

@@ -36,6 +36,8 @@ def _compare_runs(pkg, dims, field, cells, rules, sim, n_ticks, limits=None, orc
             c_e, r_e = ge.download_cells()
             c_g, r_g = gg.download_cells()
             assert_cells_equal(c_g, r_g, c_e, r_e)
+            # every byte, padding and reserved words included (both uploads zero them)
+            assert c_g.tobytes() == c_e.tobytes() and r_g.tobytes() == r_e.tobytes(), t
             assert np.array_equal(gg.download_field_aos().view(np.uint32), ge.download_field_aos().view(np.uint32)), t
             assert np.array_equal(gg.download_light().view(np.uint32), ge.download_light().view(np.uint32)), t
             if orc is not None:

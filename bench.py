@@ -658,6 +658,12 @@ def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3, pair_target=10_000_
         plain, _ = first_tick(A.KINETICS_PLAIN)
         surr, surr_stats = first_tick(A.KINETICS_SURROGATE)
         infer_ms = g.surrogate_time_infer(n0, 10)
+        import ctypes
+
+        mms, mfl = ctypes.c_float(), ctypes.c_double()
+        g._chk(g.lib.marl_mfma_probe(g.h, 5, ctypes.byref(mms), ctypes.byref(mfl)))
+        mfma_meas = 5 * mfl.value / (mms.value * 1e-3) / 1e12
+        log(f"mfma probe: {mfma_meas:.0f} TFLOP/s dense BF16")
     finally:
         g.close()
     kin_plain = sum(st["ms_kinetics"] for st in plain) / reps
@@ -687,7 +693,11 @@ def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3, pair_target=10_000_
                          "TFLOPs": round(SURR_FLOP_PER_CELL * n0 / (infer_ms * 1e-3) / 1e12, 1),
                          "mfma_frac": round(SURR_FLOP_PER_CELL * n0 / (infer_ms * 1e-3) / 1e12
                                             / MFMA_BF16_PEAK_TFLOPS, 4),
+                         "mfma_frac_of_measured": round(SURR_FLOP_PER_CELL * n0 / (infer_ms * 1e-3) / 1e12
+                                                        / mfma_meas, 4),
                          "flop_per_cell": SURR_FLOP_PER_CELL, "peak_TFLOPs": MFMA_BF16_PEAK_TFLOPS},
+        "mfma_probe": {"TFLOPs": round(mfma_meas, 1), "frac_of_vendor_peak": round(mfma_meas / MFMA_BF16_PEAK_TFLOPS, 4),
+                       "kernel": "register-only v_mfma_f32_32x32x16_bf16 chains, 4 per wave, 4 blocks of 256 per CU"},
     }
     log("surrogate:", json.dumps(res))
     return res

@@ -313,6 +313,11 @@ int marl_copy_probe(marl_ctx* ctx, int32_t n_reps, float* ms_out, double* bytes_
  * buffer the call allocates and frees, 4 sectors per thread; the ceiling of
  * kernels that read scattered records.  *bytes_out = sector bytes read per rep. */
 int marl_gather_probe(marl_ctx* ctx, int32_t n_reps, float* ms_out, double* bytes_out);
+/* Dense BF16 MFMA calibration (no reference counterpart): n_reps launches in
+ * which every wave of 4 resident 256-thread blocks per CU runs 4 independent
+ * chains of 4,096 v_mfma_f32_32x32x16_bf16 on register operands; the measured
+ * denominator beside the vendor's 2.5 PFLOP/s.  *flop_out = FLOP per rep. */
+int marl_mfma_probe(marl_ctx* ctx, int32_t n_reps, float* ms_out, double* flop_out);
 int marl_sync(marl_ctx* ctx);
 
 /* ---- outputs: reductions and packing on the resident state ---------------- */

@@ -252,6 +252,7 @@ SIGNATURES = {
     "marl_stencil_tile_split": (ctypes.c_int, [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "marl_copy_probe": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]),
     "marl_gather_probe": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]),
+    "marl_mfma_probe": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]),
     "marl_hgt_transfer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64]),
     "marl_seed_positions": (ctypes.c_int, [ctypes.POINTER(marl_dims), ctypes.c_uint64, ctypes.c_uint64, _I64, _I32,
                                            _I32, _I32, _P, _P, _P, _I64, ctypes.POINTER(_I64)]),

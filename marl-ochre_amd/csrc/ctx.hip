@@ -417,6 +417,32 @@ int marl_gather_probe(marl_ctx* ctx, int32_t n, float* ms, double* bytes) {
   return MARL_OK;
 }
 
+// Dense BF16 MFMA rate: register-only chains of 32x32x16 products on every
+// CU; the output buffer (one float per thread) is allocated for the
+// measurement and freed after it.
+int marl_mfma_probe(marl_ctx* ctx, int32_t n, float* ms, double* flop) {
+  if (!ctx || !ms || n <= 0) return MARL_ERR_INVALID_INPUT;
+  int cus = 0;
+  MARL_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  if (cus <= 0) return marl_fail(ctx, MARL_ERR_TRANSFER, "mfma probe: no compute units reported");
+  float* out = nullptr;
+  MARL_HIP(ctx, hipMalloc(&out, marl_mfma_probe_floats(cus) * sizeof(float)));
+  struct Free {
+    float* p;
+    ~Free() { (void)hipFree(p); }
+  } fr{out};
+  double per = 0.0;
+  if (int rc = marl_launch_mfma_probe(ctx, cus, out, &per)) return rc;  // warm-up
+  MARL_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int k = 0; k < n; ++k)
+    if (int rc = marl_launch_mfma_probe(ctx, cus, out, &per)) return rc;
+  MARL_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  MARL_HIP(ctx, hipEventSynchronize(ctx->ev1));
+  MARL_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+  if (flop) *flop = per;
+  return MARL_OK;
+}
+
 int marl_stencil_tile_count(void) { return kNumTiles; }
 
 int marl_stencil_tile_substeps(int32_t tile) {

@@ -388,6 +388,8 @@ int marl_ensure_aux(marl_ctx* ctx);
 // Picks (and records in ctx->kin_quads) the plain cell pass's field layout.
 bool marl_choose_cell_layout(marl_ctx* ctx);
 int marl_launch_copy_probe(marl_ctx* ctx);
+int marl_launch_mfma_probe(marl_ctx* ctx, int cus, float* out, double* flop);
+size_t marl_mfma_probe_floats(int cus);
 int marl_launch_gather_probe(marl_ctx* ctx, const void* src, uint32_t nsectors, float* out, size_t threads,
                              uint32_t salt, double* sector_bytes);
 

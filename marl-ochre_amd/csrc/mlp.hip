@@ -451,6 +451,38 @@ __global__ void k_to_bf16(size_t n, const float* __restrict__ x, uint16_t* __res
 
 inline unsigned g1(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
+// MFMA calibration: each wave runs kMfmaChains independent chains of
+// v_mfma_f32_32x32x16_bf16 on register operands for `iters` steps, with no
+// memory traffic but one store per lane at the end (which keeps the products
+// live): the dense BF16 rate the chip sustains, the denominator measured next
+// to the vendor's.
+constexpr int kMfmaChains = 4;
+constexpr int kMfmaProbeIters = 4096;
+constexpr int kMfmaProbeBlocksPerCu = 4;  // 256-thread blocks: 4 waves per SIMD, all resident at once
+
+__global__ void __launch_bounds__(256) k_mfma_probe(int iters, float seed, float* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  bf16x8 a, b;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    a[i] = (__bf16)(seed * (float)(lane + i + 1));
+    b[i] = (__bf16)(seed * (float)(64 - lane + i));
+  }
+  f32x16 acc[kMfmaChains];
+#pragma unroll
+  for (int c = 0; c < kMfmaChains; ++c) acc[c] = f32x16{};
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int c = 0; c < kMfmaChains; ++c) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[c], 0, 0, 0);
+  }
+  float sum = 0.0f;
+#pragma unroll
+  for (int c = 0; c < kMfmaChains; ++c)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) sum += acc[c][j];
+  out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = sum;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -642,3 +674,15 @@ int mlp_infer_bf16(marl_ctx* ctx, MlpState& s, int M, const uint16_t* xb, float*
   MARL_HIP(ctx, hipGetLastError());
   return MARL_OK;
 }
+
+// out: kMfmaProbeBlocksPerCu * cus * 256 floats.  *flop = FLOP per launch.
+int marl_launch_mfma_probe(marl_ctx* ctx, int cus, float* out, double* flop) {
+  const int blocks = kMfmaProbeBlocksPerCu * cus;
+  hipLaunchKernelGGL(k_mfma_probe, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, kMfmaProbeIters, 1.0f / 64.0f,
+                     out);
+  MARL_HIP(ctx, hipGetLastError());
+  if (flop) *flop = (double)blocks * 4.0 * kMfmaProbeIters * kMfmaChains * (2.0 * 32 * 32 * 16);
+  return MARL_OK;
+}
+
+size_t marl_mfma_probe_floats(int cus) { return (size_t)kMfmaProbeBlocksPerCu * cus * 256; }

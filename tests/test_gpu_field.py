@@ -205,3 +205,19 @@ def test_calibration_probes_report_their_bytes(pkg):
         g._chk(g.lib.marl_gather_probe(g.h, 3, ctypes.byref(ms), ctypes.byref(by)))
         assert ms.value > 0 and by.value == (1 << 24) // 4 * 4 * 64.0
         assert np.array_equal(g.download_field_aos(), field)
+
+
+def test_mfma_probe_reports_a_plausible_rate(pkg):
+    """The dense BF16 calibration: FLOP per rep = 4 blocks per CU x 4 waves x
+    4,096 steps x 4 chains x 32,768 FLOP, and a rate between a tenth of the
+    vendor's 2.5 PFLOP/s and that peak plus a margin."""
+    import ctypes
+    with pkg.MarlGpu((64, 64, 32)) as g:
+        ms, fl = ctypes.c_float(), ctypes.c_double()
+        g._chk(g.lib.marl_mfma_probe(g.h, 2, ctypes.byref(ms), ctypes.byref(fl)))
+        per_cu = 4 * 4 * 4096 * 4 * 32768.0
+        cus = fl.value / per_cu
+        assert cus == int(cus) and cus >= 1
+        tflops = 2 * fl.value / (ms.value * 1e-3) / 1e12
+        print("mfma probe", round(tflops, 1), "TFLOP/s on", int(cus), "CUs", flush=True)
+        assert 250.0 < tflops < 3000.0

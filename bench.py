@@ -568,6 +568,62 @@ def _shadow_summary(sh):
     }
 
 
+XGMI_LINK_GBS = 153.0  # one MI355X xGMI link, per direction
+
+
+def slab_bench(pkg, slabs=8, reps=4):
+    """One GPU's share of a single large grid split into `slabs` y-slabs (the
+    split that keeps every light column on one GPU): the stress grid's
+    256 x 256/slabs x 128 slab with its own autotuned stencil tile, and the
+    1M-cell workload's share (1,000,000/slabs cells at the same density) on
+    that slab.  No halo exchange runs (one GPU); the per-substep halo plane a
+    neighbour would send (256 x 128 voxels x 12 species x 4 B per face) is
+    priced at one xGMI link and set against the slab's substep time."""
+    from marl_ochre_amd import scenarios
+
+    dims = (256, 256 // slabs, 128)
+    nvox = dims[0] * dims[1] * dims[2]
+    field, occ, sim = scenarios.stencil_stress(dims, seed=0)
+    with pkg.MarlGpu(dims) as g:
+        g.upload_field_aos(field)
+        g.upload_occupancy(occ)
+        tile = g.autotune(sim)
+        g.upload_field_aos(field)
+        g.time_field_phase(sim, 3)
+        n = 40
+        phase_ms = g.time_field_phase(sim, n) / n
+        sub_ms = g.time_diffuse(sim, 10 * sim.diffusion_substeps) / (10 * sim.diffusion_substeps)
+        zc = g.tile_shape(tile)[1]
+    kfield, kcells, krules, ksim = scenarios.kinetics_at_scale(dims, 1_000_000 // slabs, seed=0)
+    with pkg.MarlGpu(dims) as g:
+        g.kinetics_block = 64
+        ticks = []
+        for rep in range(reps + 1):
+            g.upload_field_aos(kfield)
+            g.upload_cells(kcells, krules)
+            st = g.tick(ksim, 0, 1)
+            if rep:
+                ticks.append(st)
+    med = lambda k: float(np.median([st[k] for st in ticks]))
+    halo_bytes = dims[0] * dims[2] * 12 * 4
+    halo_us = halo_bytes / (XGMI_LINK_GBS * 1e9) * 1e6
+    return {
+        "workload": f"one of {slabs} y-slabs: {dims[0]}x{dims[1]}x{dims[2]} of the stress grid, and "
+                    f"{len(kcells):,} cells of the 1M-cell workload on it; no halo exchange (one GPU)",
+        "slab": list(dims), "stencil_tile": tile, "z_chunk": zc,
+        "substep_us": round(sub_ms * 1e3, 2),
+        "substep_hbm_frac": round(STENCIL_BYTES_PER_VOXEL * nvox / (sub_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "field_phase_us": round(phase_ms * 1e3, 1),
+        "field_phase_Gvoxel_species_per_s": round(nvox * 12 * sim.diffusion_substeps / (phase_ms * 1e-3) / 1e9, 1),
+        "halo_plane_bytes_per_face": halo_bytes,
+        "halo_plane_us_per_face_one_xgmi_link": round(halo_us, 2),
+        "halo_over_substep": round(halo_us / (sub_ms * 1e3), 3),
+        "cells": len(kcells),
+        "ms_tick": round(med("ms_tick"), 4), "ms_kinetics": round(med("ms_kinetics"), 4),
+        "ms_field": round(med("ms_field"), 4), "ms_fate": round(med("ms_fate"), 4), "levels": int(med("levels")),
+    }
+
+
 def surrogate_bench(pkg, train_steps, harvest_ticks, reps=3, pair_target=10_000_000, harvest_seeds=4):
     """The learned kinetics surrogate on the 1M-cell workload, every cell with
     its own random reaction wiring.
@@ -718,6 +774,8 @@ def main():
                     help="ticks of the default 128x128x64 column run (0 = skip)")
     ap.add_argument("--column-modes", default="graph,eager",
                     help="tick paths of the column run: graph (marl_tick_graph), eager (marl_tick)")
+    ap.add_argument("--slabs", type=int, default=8,
+                    help="time one GPU's y-slab of the stress grid and 1M-cell workload split this many ways (<= 1 = skip)")
     ap.add_argument("--ensemble-ticks", type=int, default=5,
                     help="ticks of the 32-column ensemble leg (0 = skip)")
     ap.add_argument("--drift-ticks", type=int, default=0,
@@ -857,6 +915,8 @@ def main():
         out["dropin"] = dropin_bench(pkg)
     if args.drift_ticks > 0:
         out["drift"] = drift_report(pkg, args.drift_ticks, min(os.cpu_count() or 1, 16), args.tile)
+    if args.slabs > 1:
+        out["slab"] = slab_bench(pkg, args.slabs)
     if args.surrogate_steps > 0:
         out["surrogate"] = surrogate_bench(pkg, args.surrogate_steps, args.harvest_ticks)
     if not args.no_cpu:

@@ -962,6 +962,210 @@ __global__ void k_g_finish(TickDev* ts) {
   ts->tick += 1;
 }
 
+// ---------------------------------------------------------------------------
+// The cell phase of a small population in one workgroup.  For a few hundred
+// cells the device-driven phase above is some forty launches of a few
+// microseconds each, most of them over almost no work; here one block of
+// kSmallThreads runs the same steps with barriers between them: the masks and
+// predecessors (cell_prep_cell), the relaxation to its fixed point with the
+// levels in LDS, a counting sort of the cells by level, the levels in order
+// (kinetics_cell on the resident quads), then fate, births with mutation,
+// deaths and compaction with block scans over LDS.  Per cell it calls the
+// same functions in the same order as the multi-launch phase, so the results
+// are bit-identical.  A population above kSmallCells, a schedule deeper than
+// kGraphLevels or one that does not settle within kGraphSweeps sets `err`
+// (the host finishes the tick).  Global writes are read back only by this
+// block after a barrier, so workgroup scope orders them.
+constexpr int kSmallThreads = 512;
+constexpr int kSmallCells = 512;
+
+// out[k] = in[0] + ... + in[k-1] for k in [0, m]; in[m] need not exist.
+// All threads of the block call it; s_w holds kSmallThreads / 64 + 1 ints.
+__device__ void block_exscan(const int32_t* in, int32_t* out, int m, int32_t* s_w) {
+  constexpr int B = kSmallThreads, W = B / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int chunk = (m + 1 + B - 1) / B;
+  const int lo = tid * chunk, hi = min(m, lo + chunk);  // out covers [lo, lo + chunk) within [0, m]
+  int sum = 0;
+  for (int k = lo; k < hi; ++k) sum += in[k];
+  int v = sum;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(v, d);
+    if (lane >= d) v += y;
+  }
+  if (lane == 63) s_w[wave] = v;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int w = 0; w < W; ++w) {
+      const int t = s_w[w];
+      s_w[w] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+  int run = s_w[wave] + v - sum;
+  const int top = min(m + 1, lo + chunk);
+  for (int k = lo; k < top; ++k) {
+    out[k] = run;
+    if (k < m) run += in[k];
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kSmallThreads) k_g_cell_phase_small(
+    TickDev* ts, marl_cell_state* __restrict__ cells, marl_ruleset* __restrict__ rules, float* __restrict__ field,
+    const float* __restrict__ light, const int32_t* __restrict__ cell_at, const uint32_t* __restrict__ occ,
+    uint8_t* __restrict__ nbmask, int32_t* __restrict__ preds, size_t ps, uint8_t* __restrict__ npred,
+    uint8_t* __restrict__ events, marl_cell_state* __restrict__ bcells, marl_ruleset* __restrict__ brules,
+    uint32_t* __restrict__ claim, CellParams p) {
+  constexpr int B = kSmallThreads, NC = kSmallCells;
+  __shared__ float sI[B * 17];
+  __shared__ float sE[B * 13];
+  __shared__ float sD[B * 13];
+  __shared__ int32_t s_level[NC + 1];        // levels; then the birth flags' ranks (brank)
+  __shared__ int32_t s_order[NC + 1];        // cells by level; then the death flags
+  __shared__ int32_t s_target[NC + 1];       // division targets; then the hole sources
+  __shared__ int32_t s_flag[NC + 1];         // birth flags; then the winners
+  __shared__ int32_t s_P[NC + 1];            // death ranks
+  __shared__ int32_t s_btarget[NC + 1];      // each birth's voxel; then the hole destinations
+  __shared__ int32_t s_wrank[NC + 1];        // winner ranks; then the birth slots
+  __shared__ int32_t s_cnt[kGraphLevels + 1], s_cur[kGraphLevels + 1];
+  __shared__ int32_t s_w[B / 64 + 1];
+  __shared__ int32_t s_flagv, s_depth;
+  const int tid = threadIdx.x;
+  if (ts->err) return;  // uniform: the whole block leaves
+  const int n = ts->n;
+  if (n > NC) {
+    if (tid == 0) ts->err = 1;
+    return;
+  }
+  // masks and predecessors (levels start at 0)
+  for (int j = tid; j < n; j += B) cell_prep_cell(j, cells, cell_at, occ, nbmask, preds, ps, npred, s_level, p);
+  __syncthreads();
+  // relaxation to the fixed point; sweep k skips levels below k (settled)
+  int quiet = -1;
+  for (int k = 0; k < kGraphSweeps; ++k) {
+    if (tid == 0) s_flagv = 0;
+    __syncthreads();
+    for (int j = tid; j < n; j += B) {
+      const int lj = s_level[j];
+      const int np = npred[j];
+      if (lj < k || np == 0) continue;
+      int L = 0;
+      for (int q = 0; q < np; ++q) L = max(L, s_level[preds[(size_t)q * ps + j]] + 1);
+      if (L > lj) {
+        s_level[j] = L;
+        s_flagv = 1;
+      }
+    }
+    __syncthreads();
+    const int changed = s_flagv;
+    if (tid == 0) ts->chg[k] = changed;
+    __syncthreads();
+    if (!changed) {
+      quiet = k;
+      break;
+    }
+  }
+  if (quiet < 0) {
+    if (tid == 0) ts->err = 1;
+    return;
+  }
+  // counting sort by level
+  if (tid == 0) s_depth = 0;
+  for (int L = tid; L <= kGraphLevels; L += B) s_cnt[L] = 0;
+  __syncthreads();
+  for (int j = tid; j < n; j += B) {
+    const int L = s_level[j];
+    atomicMax(&s_depth, L + 1);
+    if (L < kGraphLevels) atomicAdd(&s_cnt[L], 1);
+  }
+  __syncthreads();
+  const int depth = s_depth;
+  if (depth > kGraphLevels) {
+    if (tid == 0) ts->err = 1;
+    return;
+  }
+  if (tid == 0) {
+    int run = 0;
+    for (int L = 0; L < depth; ++L) {
+      const int c = s_cnt[L];
+      s_cnt[L] = run;
+      s_cur[L] = run;
+      run += c;
+    }
+    s_cnt[depth] = run;
+    ts->nlev = depth;
+    ts->levels = depth;
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += B) s_order[atomicAdd(&s_cur[s_level[j]], 1)] = j;
+  __syncthreads();
+  // the levels in order, on the resident quads
+  const HarvestSink hv{};
+  for (int L = 0; L < depth; ++L) {
+    const int lo = s_cnt[L], hi = s_cnt[L + 1];
+    for (int t = lo + tid; t < hi; t += B)
+      kinetics_cell<false, true>(s_order[t], cells, rules, field, light, nbmask, events, p, hv, sI + tid * 17,
+                                 sE + tid * 13, sD + tid * 13, nullptr, nullptr, nullptr, nullptr);
+    __syncthreads();
+  }
+  // fate (lib.rs:220-277): division targets in the pre-birth map
+  CellParams pd = p;
+  pd.stream = div_stream(ts);
+  for (int i = tid; i <= n; i += B) {
+    if (i < n) {
+      div_target_cell(i, cells, events, cell_at, s_target, s_flag, pd);
+    } else {
+      s_target[i] = -1;
+      s_flag[i] = 0;
+    }
+  }
+  __syncthreads();
+  int32_t* brank = s_level;
+  block_exscan(s_flag, brank, n, s_w);
+  for (int i = tid; i < n; i += B) birth_cell(i, cells, rules, s_target, brank, bcells, brules, s_btarget, claim, pd);
+  int32_t* dflag = s_order;
+  for (int i = tid; i < n; i += B) dflag[i] = events[i] == MARL_EVENT_DEATH ? 1 : 0;
+  __syncthreads();
+  block_exscan(dflag, s_P, n, s_w);
+  const int nb = brank[n], nd = s_P[n], m0 = n - nd;
+  const int holes = (nd > 0 && m0 > 0) ? s_P[m0] : 0;
+  // the first birth claiming a voxel wins; then the claims are cleared
+  int32_t* win = s_flag;
+  for (int r = tid; r <= nb; r += B) win[r] = (r < nb && claim[s_btarget[r]] == (uint32_t)r) ? 1 : 0;
+  __syncthreads();
+  for (int r = tid; r < nb; r += B) claim[s_btarget[r]] = 0xffffffffu;
+  block_exscan(win, s_wrank, nb, s_w);
+  const int nw = s_wrank[nb];
+  // deaths: head holes filled from the tail (closed form)
+  int32_t* hole_dst = s_btarget;
+  int32_t* hole_src = s_target;
+  if (holes > 0) {
+    for (int s0 = tid; s0 < m0; s0 += B) hole_source(s0, n, m0, events, s_P, hole_dst, hole_src);
+    __syncthreads();
+    for (int t = tid; t < holes * kRecChunks; t += B)
+      move_chunk(t, hole_dst, hole_src, cells, rules, cells, rules, nullptr);
+  }
+  __syncthreads();
+  // births appended after the survivors, in event order
+  if (nw > 0) {
+    int32_t* slot = s_wrank;  // each entry turns into its birth's slot in place
+    for (int r = tid; r < nb; r += B) slot[r] = win[r] ? m0 + s_wrank[r] : 0;
+    __syncthreads();
+    for (int t = tid; t < nb * kRecChunks; t += B) move_chunk(t, slot, nullptr, bcells, brules, cells, rules, win);
+  }
+  if (tid == 0) {
+    ts->nb = nb;
+    ts->nd = nd;
+    ts->m0 = m0;
+    ts->holes = holes;
+    ts->nw = nw;
+  }
+}
+
 #undef GS_LOOP
 
 constexpr int kBudgetBlocks = 512;
@@ -1415,8 +1619,21 @@ int marl_enqueue_field_phase(marl_ctx* ctx, const marl_sim_params* sim) {
   return marl_launch_field_phase(ctx, sim, true);
 }
 
-// Schedule, the ordered pass, fate, births, deaths and compaction.
+// Schedule, the ordered pass, fate, births, deaths and compaction: one
+// workgroup for a small population (ctx->graph_small), else the multi-launch
+// sequence.
 int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
+  if (ctx->graph_small) {
+    TickDev* ts = static_cast<TickDev*>(ctx->tdev);
+    const CellParams p = cell_params(ctx, sim, 0);  // the RNG stream comes from ts->tick
+    hipLaunchKernelGGL(k_g_cell_phase_small, dim3(1), dim3(kSmallThreads), 0, ctx->stream, ts,
+                       ctx->cells[ctx->cbuf], ctx->rules[ctx->cbuf], ctx->field[ctx->cur], ctx->light, ctx->cell_at,
+                       ctx->occ, ctx->nbmask, ctx->preds, (size_t)ctx->cap, ctx->npred, ctx->events,
+                       ctx->birth_cells, ctx->birth_rules, ctx->claim, p);
+    hipLaunchKernelGGL(k_g_finish, dim3(1), dim3(1), 0, ctx->stream, ts);
+    MARL_HIP(ctx, hipGetLastError());
+    return MARL_OK;
+  }
   if (int rc = marl_enqueue_schedule(ctx, sim)) return rc;
   return marl_enqueue_pass_and_fate(ctx, sim, false);
 }
@@ -1535,7 +1752,7 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
     marl_sim_params sim;
     int64_t cap;
     const void *cells, *rules, *f0, *f1, *staging;
-    int32_t tile, cur, cbuf, sweeps, levels, kin_quads, kin_block, overlap;
+    int32_t tile, cur, cbuf, sweeps, levels, kin_quads, kin_block, overlap, small;
   } k;
   memset(&k, 0, sizeof(k));
   k.sim = *sim;
@@ -1553,6 +1770,7 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
   k.kin_quads = ctx->kin_quads ? 1 : 0;
   k.kin_block = ctx->kin_block;
   k.overlap = ctx->graph_overlap ? 1 : 0;
+  k.small = ctx->graph_small ? 1 : 0;
   const uint8_t* b = reinterpret_cast<const uint8_t*>(&k);
   return std::vector<uint8_t>(b, b + sizeof(k));
 }
@@ -1570,6 +1788,12 @@ int marl_graph_prepare(marl_ctx* ctx, const marl_sim_params* sim) {
   if (!ctx->tdev) MARL_HIP(ctx, hipMalloc(&ctx->tdev, sizeof(TickDev)));
   const int64_t n = ctx->n_cells;
   marl_choose_cell_layout(ctx);
+  // the one-workgroup cell phase for small populations (it works on the
+  // resident quads), with hysteresis so that a population near the bound is
+  // not captured again every tick; a tick that starts above kSmallCells is
+  // finished by the host
+  if (ctx->graph_small && (n > kSmallCells * 3 / 4 || !ctx->kin_quads)) ctx->graph_small = false;
+  else if (!ctx->graph_small && n <= kSmallCells / 2 && ctx->kin_quads) ctx->graph_small = true;
   // room for this tick's births; generous so that growth rarely re-captures
   if (int rc = marl_ensure_cell_capacity(ctx, std::max<int64_t>(2 * n + 1024, 4 * n))) return rc;
   return scan_reserve(ctx, (int)ctx->cap + 1);
@@ -1749,8 +1973,7 @@ int capture_tick(marl_ctx* ctx, const marl_sim_params* sim) {
     const hipError_t j2 = j1 == hipSuccess ? hipStreamWaitEvent(ctx->stream, ctx->xev[0], 0) : j1;
     if (rc == MARL_OK && j2 != hipSuccess) rc = marl_hip_fail(ctx, j2, "capture: join of the schedule stream");
   }
-  if (rc == MARL_OK && !overlap) rc = marl_enqueue_schedule(ctx, sim);
-  if (rc == MARL_OK) rc = marl_enqueue_pass_and_fate(ctx, sim, overlap);
+  if (rc == MARL_OK) rc = overlap ? marl_enqueue_pass_and_fate(ctx, sim, true) : marl_enqueue_cell_phase(ctx, sim);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
   ctx->cur = slot;

@@ -264,6 +264,7 @@ struct marl_ctx {
   int cell_layout = MARL_CELL_LAYOUT_AUTO;    // field layout of the plain cell pass
   bool kin_quads = false;                    // the layout last chosen: quads in place
   bool graph_overlap = false;                // the captured tick overlaps phases on ctx->aux
+  bool graph_small = false;                  // the captured cell phase runs in one workgroup
   int kin_block = 128;                       // threads per block of the plain pass (64, 128 or 256)
   long fault_calls = 0;                      // drop-in / tick calls counted for MARL_HIP_FAULT=tick:<n>
   bool phase_budget = false;                 // FP64 mass budget at the tick's phase boundaries

@@ -93,6 +93,58 @@ def test_graph_tick_every_kinetics_block(pkg, orc, block):
     _compare_runs(pkg, dims, field, cells, rules, sim, 3, orc=orc, blocks=(128, block))
 
 
+def test_graph_tick_small_population_one_workgroup(pkg, orc):
+    """Populations of at most 256 cells (on the resident quads) run the
+    captured cell phase in one workgroup (k_g_cell_phase_small), up to 384
+    once there.  A small population that divides and dies for eight ticks
+    (240 -> 365 -> 267 cells): every tick equals the eager tick and the oracle."""
+    A = pkg._abi
+    dims = (64, 32, 24)
+    cells, rules = population(A, dims, 240, 41, energy_scale=6.0)
+    field = np.random.default_rng(42).random(np.prod(dims) * 12).astype(F) * 4.0
+    sim = pkg.SimulationConfig(rng_seed=43, diffusion_substeps=3)
+    seen = _compare_runs(pkg, dims, field, cells, rules, sim, 8, orc=orc, layouts=("aos", "auto"))
+    print("small population", seen, flush=True)
+    assert seen["divisions"] > 100 and seen["deaths"] > 100 and seen["host_finished"] == 0, seen
+
+
+def test_graph_tick_switches_between_small_and_multi_launch_phases(pkg):
+    """The default column with random reactions falls from 4,098 cells to 17
+    in its first tick, so the captured tick switches to the one-workgroup
+    phase; a population uploaded above 384 cells switches it back (its first
+    tick may outgrow the schedule budget fitted to the small one and be
+    finished by the host), and a small one again to the one-workgroup phase.
+    Each tick equals the eager tick."""
+    from marl_ochre_amd import scenarios
+
+    A = pkg._abi
+    field, cells, rules, sim = scenarios.winogradsky_column(starter_state=False)
+    seen = _compare_runs(pkg, (128, 128, 64), field, cells, rules, sim, 4)
+    assert seen["deaths"] > 4000 and seen["host_finished"] == 0, seen
+    dims = (64, 32, 24)
+    small_c, small_r = population(A, dims, 200, 51)
+    big_c, big_r = population(A, dims, 3000, 52)
+    fld = np.random.default_rng(53).random(np.prod(dims) * 12).astype(F)
+    sim2 = pkg.SimulationConfig(rng_seed=54, diffusion_substeps=2)
+    with pkg.MarlGpu(dims) as ge, pkg.MarlGpu(dims) as gg:
+        t = 0
+        for c, r in ((small_c, small_r), (big_c, big_r), (small_c, small_r)):
+            for g in (ge, gg):
+                g.upload_field_aos(fld)
+                g.upload_cells(c, r)
+            for _ in range(2):
+                se = ge.tick(sim2, t, 1)
+                sg = gg.tick_graph(sim2, t, 1)
+                t += 1
+                assert sg["host_finished"] == 0 or len(c) > 384, (t, len(c))
+                for k in ("divisions", "deaths", "births_added", "n_cells", "levels"):
+                    assert sg[k] == se[k], (t, k, sg[k], se[k])
+                c_e, r_e = ge.download_cells()
+                c_g, r_g = gg.download_cells()
+                assert c_g.tobytes() == c_e.tobytes() and r_g.tobytes() == r_e.tobytes(), t
+                assert np.array_equal(gg.download_field_aos().view(np.uint32), ge.download_field_aos().view(np.uint32))
+
+
 def test_graph_tick_host_finishes_over_budget_ticks(pkg):
     """Budgets too small for the schedule force the skip-and-finish path:
     one sweep (relaxation not known to converge) and two levels (too deep)."""

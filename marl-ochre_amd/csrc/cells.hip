@@ -972,12 +972,14 @@ __global__ void k_g_finish(TickDev* ts) {
 // (kinetics_cell on the resident quads), then fate, births with mutation,
 // deaths and compaction with block scans over LDS.  Per cell it calls the
 // same functions in the same order as the multi-launch phase, so the results
-// are bit-identical.  A population above kSmallCells, a schedule deeper than
-// kGraphLevels or one that does not settle within kGraphSweeps sets `err`
-// (the host finishes the tick).  Global writes are read back only by this
-// block after a barrier, so workgroup scope orders them.
+// are bit-identical.  Only a population above kSmallCells sets `err` (the host
+// finishes the tick): the relaxation runs to its fixed point (at most n + 1
+// sweeps) and every level is kept, however deep, so a run of small ticks can be
+// queued without the host looking at each.  Global writes are read back only
+// by this block after a barrier, so workgroup scope orders them.
 constexpr int kSmallThreads = 512;
 constexpr int kSmallCells = 512;
+constexpr int kSmallRun = 16;  // captured small ticks queued per host look
 
 // out[k] = in[0] + ... + in[k-1] for k in [0, m]; in[m] need not exist.
 // All threads of the block call it; s_w holds kSmallThreads / 64 + 1 ints.
@@ -1031,7 +1033,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_g_cell_phase_small(
   __shared__ int32_t s_P[NC + 1];            // death ranks
   __shared__ int32_t s_btarget[NC + 1];      // each birth's voxel; then the hole destinations
   __shared__ int32_t s_wrank[NC + 1];        // winner ranks; then the birth slots
-  __shared__ int32_t s_cnt[kGraphLevels + 1], s_cur[kGraphLevels + 1];
+  __shared__ int32_t s_cnt[NC + 1], s_cur[NC + 1];  // per level (depth <= n)
   __shared__ int32_t s_w[B / 64 + 1];
   __shared__ int32_t s_flagv, s_depth;
   const int tid = threadIdx.x;
@@ -1044,9 +1046,10 @@ __global__ void __launch_bounds__(kSmallThreads) k_g_cell_phase_small(
   // masks and predecessors (levels start at 0)
   for (int j = tid; j < n; j += B) cell_prep_cell(j, cells, cell_at, occ, nbmask, preds, ps, npred, s_level, p);
   __syncthreads();
-  // relaxation to the fixed point; sweep k skips levels below k (settled)
+  // relaxation to the fixed point (levels stay below n, so sweep n changes
+  // nothing); sweep k skips levels below k (settled)
   int quiet = -1;
-  for (int k = 0; k < kGraphSweeps; ++k) {
+  for (int k = 0; k <= n; ++k) {
     if (tid == 0) s_flagv = 0;
     __syncthreads();
     for (int j = tid; j < n; j += B) {
@@ -1062,32 +1065,28 @@ __global__ void __launch_bounds__(kSmallThreads) k_g_cell_phase_small(
     }
     __syncthreads();
     const int changed = s_flagv;
-    if (tid == 0) ts->chg[k] = changed;
+    if (tid == 0 && k < kGraphSweeps) ts->chg[k] = changed;
     __syncthreads();
     if (!changed) {
       quiet = k;
       break;
     }
   }
-  if (quiet < 0) {
+  if (quiet < 0) {  // not reached: a level never exceeds n - 1
     if (tid == 0) ts->err = 1;
     return;
   }
   // counting sort by level
   if (tid == 0) s_depth = 0;
-  for (int L = tid; L <= kGraphLevels; L += B) s_cnt[L] = 0;
+  for (int L = tid; L <= NC; L += B) s_cnt[L] = 0;
   __syncthreads();
   for (int j = tid; j < n; j += B) {
     const int L = s_level[j];
     atomicMax(&s_depth, L + 1);
-    if (L < kGraphLevels) atomicAdd(&s_cnt[L], 1);
+    atomicAdd(&s_cnt[L], 1);
   }
   __syncthreads();
   const int depth = s_depth;
-  if (depth > kGraphLevels) {
-    if (tid == 0) ts->err = 1;
-    return;
-  }
   if (tid == 0) {
     int run = 0;
     for (int L = 0; L < depth; ++L) {
@@ -1765,8 +1764,8 @@ std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* 
   k.tile = ctx->tile;
   k.cur = ctx->cur;
   k.cbuf = ctx->cbuf;
-  k.sweeps = ctx->graph_sweeps;
-  k.levels = ctx->graph_levels;
+  k.sweeps = ctx->graph_small ? 0 : ctx->graph_sweeps;  // the one-workgroup phase has no budget
+  k.levels = ctx->graph_small ? 0 : ctx->graph_levels;
   k.kin_quads = ctx->kin_quads ? 1 : 0;
   k.kin_block = ctx->kin_block;
   k.overlap = ctx->graph_overlap ? 1 : 0;
@@ -1795,7 +1794,11 @@ int marl_graph_prepare(marl_ctx* ctx, const marl_sim_params* sim) {
   if (ctx->graph_small && (n > kSmallCells * 3 / 4 || !ctx->kin_quads)) ctx->graph_small = false;
   else if (!ctx->graph_small && n <= kSmallCells / 2 && ctx->kin_quads) ctx->graph_small = true;
   // room for this tick's births; generous so that growth rarely re-captures
-  if (int rc = marl_ensure_cell_capacity(ctx, std::max<int64_t>(2 * n + 1024, 4 * n))) return rc;
+  // (in small mode for any population up to kSmallCells, so that a run of
+  // ticks never outgrows it)
+  int64_t room = std::max<int64_t>(2 * n + 1024, 4 * n);
+  if (ctx->graph_small) room = std::max<int64_t>(room, 2 * kSmallCells + 1024);
+  if (int rc = marl_ensure_cell_capacity(ctx, room)) return rc;
   return scan_reserve(ctx, (int)ctx->cap + 1);
 }
 
@@ -2207,7 +2210,7 @@ int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, in
   if (n_ticks < 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "negative tick count");
   marl_tick_stats st{};
   int32_t finished = 0;
-  for (int t = 0; t < n_ticks; ++t) {
+  for (int t = 0; t < n_ticks;) {
     if (int rc = marl_graph_prepare(ctx, sim)) return rc;
     // Overlap inside the captured tick (schedule beside the field phase,
     // transpose back beside fate) pays for large populations only: each
@@ -2221,17 +2224,45 @@ int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, in
     const int slot = ctx->cur;
     if (ctx->graph_key[slot] != marl_graph_key(ctx, sim))
       if (int rc = capture_tick(ctx, sim)) return rc;
+    // A run of replays with no look from the host in between: in small mode
+    // the device defers a tick to the host only above kSmallCells cells, a
+    // population at most doubles per tick, the capacity covers kSmallCells,
+    // and with an even number of buffer swaps per tick every replay is this
+    // graph.  The device carries the live count and the tick index from one
+    // replay to the next (k_g_finish).
+    int run = 1;
+    if (ctx->graph_small && !ctx->graph_flip[slot]) {
+      const int64_t n0 = std::max<int64_t>(ctx->n_cells, 1);
+      while (run < kSmallRun && run < n_ticks - t && (n0 << run) <= kSmallCells) ++run;
+    }
+    TickDev* ring = nullptr;
+    if (run > 1) {
+      if (!ctx->tick_ring) MARL_HIP(ctx, hipHostMalloc(&ctx->tick_ring, sizeof(TickDev) * kSmallRun, hipHostMallocDefault));
+      ring = static_cast<TickDev*>(ctx->tick_ring);
+    }
     if (int rc = marl_graph_stage(ctx, ctx->stream, tick0 + t)) return rc;
     MARL_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    MARL_HIP(ctx, hipGraphLaunch(ctx->gexec[slot], ctx->stream));
+    for (int i = 0; i < run; ++i) {
+      MARL_HIP(ctx, hipGraphLaunch(ctx->gexec[slot], ctx->stream));
+      if (ring) MARL_HIP(ctx, hipMemcpyAsync(ring + i, ctx->tdev, sizeof(TickDev), hipMemcpyDeviceToHost, ctx->stream));
+    }
     MARL_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    if (int rc = marl_graph_fetch(ctx, ctx->stream)) return rc;
+    if (!ring)
+      if (int rc = marl_graph_fetch(ctx, ctx->stream)) return rc;
     MARL_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->cur ^= ctx->graph_flip[slot];
+    if (!ring) ctx->cur ^= ctx->graph_flip[slot];
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-    if (int rc = marl_graph_post(ctx, sim, tick0 + t, &st, &finished)) return rc;
-    st.ms_field = ms;
+    for (int i = 0; i < run; ++i) {
+      if (ring) {
+        if (ring[i].err && i + 1 < run)
+          return marl_fail(ctx, MARL_ERR_TRANSFER, "captured tick deferred to the host inside a run of ticks");
+        memcpy(host_tickdev(ctx), ring + i, sizeof(TickDev));
+      }
+      if (int rc = marl_graph_post(ctx, sim, tick0 + t + i, &st, &finished)) return rc;
+      st.ms_field = ms / (float)run;
+    }
+    t += run;
   }
   if (stats) *stats = st;
   if (host_finished) *host_finished = finished;

@@ -168,6 +168,7 @@ void marl_destroy(marl_ctx* c) {
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->tick_ring) (void)hipHostFree(c->tick_ring);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (auto x : c->xev)

@@ -253,6 +253,7 @@ struct marl_ctx {
 
   // device-driven tick captured as a hipGraph
   void* tdev = nullptr;            // per-tick counters on the device (TickDev)
+  void* tick_ring = nullptr;       // pinned host copies of a run of captured ticks' counters
   // one captured tick per field parity (a tick with an odd number of stencil
   // launches alternates between them)
   hipGraph_t graph[2] = {nullptr, nullptr};

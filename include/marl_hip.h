@@ -135,8 +135,11 @@ typedef struct {
   int32_t levels;         /* depth of the conflict schedule of the cell pass */
   int32_t _pad;
   float ms_field;         /* occupancy rebuild, sources, substeps, light (hipEvents) */
-  float ms_schedule;      /* empty-neighbour masks and the conflict levels (beside the field phase) */
-  float ms_kinetics;      /* the pass's field view, then the per-level kinetics launches */
+  float ms_schedule;      /* empty-neighbour masks and the conflict levels (beside the field phase);
+                             0 when a population of at most 384 cells runs its whole cell phase
+                             in one workgroup */
+  float ms_kinetics;      /* the pass's field view, then the per-level kinetics launches (with the
+                             one-workgroup cell phase: all of it, schedule and fate included) */
   float ms_fate;          /* divisions, mutation, deaths, compaction (and the transpose back beside them) */
   float ms_tick;          /* the whole tick, first launch to last */
   float ms_transpose_back;/* the pass's field view back to the resident layout, beside fate */

@@ -1622,19 +1622,23 @@ int marl_enqueue_field_phase(marl_ctx* ctx, const marl_sim_params* sim) {
 // workgroup for a small population (ctx->graph_small), else the multi-launch
 // sequence.
 int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
-  if (ctx->graph_small) {
-    TickDev* ts = static_cast<TickDev*>(ctx->tdev);
-    const CellParams p = cell_params(ctx, sim, 0);  // the RNG stream comes from ts->tick
-    hipLaunchKernelGGL(k_g_cell_phase_small, dim3(1), dim3(kSmallThreads), 0, ctx->stream, ts,
-                       ctx->cells[ctx->cbuf], ctx->rules[ctx->cbuf], ctx->field[ctx->cur], ctx->light, ctx->cell_at,
-                       ctx->occ, ctx->nbmask, ctx->preds, (size_t)ctx->cap, ctx->npred, ctx->events,
-                       ctx->birth_cells, ctx->birth_rules, ctx->claim, p);
-    hipLaunchKernelGGL(k_g_finish, dim3(1), dim3(1), 0, ctx->stream, ts);
-    MARL_HIP(ctx, hipGetLastError());
-    return MARL_OK;
-  }
+  if (ctx->graph_small) return marl_enqueue_cell_phase_small(ctx, sim);
   if (int rc = marl_enqueue_schedule(ctx, sim)) return rc;
   return marl_enqueue_pass_and_fate(ctx, sim, false);
+}
+
+// The one-workgroup cell phase and the counters' finish, on ctx->stream
+// (TickDev staged; the resident quads hold the field).
+int marl_enqueue_cell_phase_small(marl_ctx* ctx, const marl_sim_params* sim) {
+  TickDev* ts = static_cast<TickDev*>(ctx->tdev);
+  const CellParams p = cell_params(ctx, sim, 0);  // the RNG stream comes from ts->tick
+  hipLaunchKernelGGL(k_g_cell_phase_small, dim3(1), dim3(kSmallThreads), 0, ctx->stream, ts, ctx->cells[ctx->cbuf],
+                     ctx->rules[ctx->cbuf], ctx->field[ctx->cur], ctx->light, ctx->cell_at, ctx->occ, ctx->nbmask,
+                     ctx->preds, (size_t)ctx->cap, ctx->npred, ctx->events, ctx->birth_cells, ctx->birth_rules,
+                     ctx->claim, p);
+  hipLaunchKernelGGL(k_g_finish, dim3(1), dim3(1), 0, ctx->stream, ts);
+  MARL_HIP(ctx, hipGetLastError());
+  return MARL_OK;
 }
 
 // The conflict schedule (masks, levels, level lists) on ctx->stream.
@@ -2157,6 +2161,35 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
     if (ctx->phase_budget)
       if (int rc = enqueue_budget(ctx, ctx->phase_budget_dev + kBudget)) return rc;
     MARL_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+    // a small population (plain kinetics on the resident quads, no harvest or
+    // phase budget) runs its cell phase in one workgroup, as the captured
+    // tick does: one launch and one readback instead of some thirty launches
+    // and several host round trips; ms_kinetics then covers the whole cell
+    // phase and ms_schedule is 0
+    const bool small = !ctx->phase_budget && ctx->kin_mode == MARL_KINETICS_PLAIN &&
+                       !(ctx->surr && ctx->surr->harvesting) && ctx->n_cells <= kSmallCells * 3 / 4 &&
+                       marl_choose_cell_layout(ctx);
+    if (small) {
+      if (!ctx->tdev) MARL_HIP(ctx, hipMalloc(&ctx->tdev, sizeof(TickDev)));
+      TickDev* ts = static_cast<TickDev*>(ctx->tdev);
+      MARL_HIP(ctx, hipEventRecord(ev[5], ctx->stream));
+      MARL_HIP(ctx, hipEventRecord(ev[6], ctx->stream));
+      if (int rc = marl_graph_stage(ctx, ctx->stream, tick0 + t)) return rc;
+      hipLaunchKernelGGL(k_g_begin, dim3(1), dim3(64), 0, ctx->stream, ts, ctx->cap);
+      MARL_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+      if (int rc = marl_enqueue_cell_phase_small(ctx, sim)) return rc;
+      MARL_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+      if (int rc = marl_graph_fetch(ctx, ctx->stream)) return rc;
+      MARL_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
+      MARL_HIP(ctx, hipEventSynchronize(ev[4]));
+      int32_t finished = 0;  // a tick above kSmallCells would be finished here on the multi-launch path
+      if (int rc = marl_graph_post(ctx, sim, tick0 + t, &st, &finished)) return rc;
+      (void)hipEventElapsedTime(&st.ms_field, ev[0], ev[1]);
+      (void)hipEventElapsedTime(&st.ms_kinetics, ev[2], ev[3]);
+      (void)hipEventElapsedTime(&st.ms_fate, ev[3], ev[4]);
+      (void)hipEventElapsedTime(&st.ms_tick, ev[0], ev[4]);
+      continue;
+    }
     {
       OnStream on(ctx, overlap ? ctx->aux : ctx->stream);
       MARL_HIP(ctx, hipEventRecord(ev[5], ctx->stream));

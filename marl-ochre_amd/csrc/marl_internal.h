@@ -361,6 +361,7 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim);
 // transpose back of the pass's field view forks onto ctx->aux and joins before
 // the part ends).
 int marl_enqueue_schedule(marl_ctx* ctx, const marl_sim_params* sim);
+int marl_enqueue_cell_phase_small(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_enqueue_pass_and_fate(marl_ctx* ctx, const marl_sim_params* sim, bool fork_back);
 std::vector<uint8_t> marl_graph_key(const marl_ctx* ctx, const marl_sim_params* sim);
 int marl_graph_stage(marl_ctx* ctx, hipStream_t st, int64_t tick);

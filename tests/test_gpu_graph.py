@@ -159,7 +159,11 @@ def test_graph_tick_host_finishes_over_budget_ticks(pkg):
         assert seen["host_finished"] == 3, (limits, seen)
 
 
-def test_graph_tick_crowded_and_empty(pkg):
+def test_graph_tick_crowded_and_empty(pkg, orc):
+    """158 cells in a crowded block and at the walls (birth collisions,
+    enclosed cells), and an empty population: small enough that both the
+    captured and the host-driven tick run the one-workgroup cell phase, so
+    the oracle is the independent check."""
     A = pkg._abi
     from marl_ochre_amd import starters as st
 
@@ -176,7 +180,7 @@ def test_graph_tick_crowded_and_empty(pkg):
     cells["internal"] = (rng.random((n, 16)) * 2).astype(F)
     cells["prep_remaining"] = 1
     field = rng.random(np.prod(dims) * 12).astype(F)
-    seen = _compare_runs(pkg, dims, field, cells, rules, pkg.SimulationConfig(rng_seed=5), 3)
+    seen = _compare_runs(pkg, dims, field, cells, rules, pkg.SimulationConfig(rng_seed=5), 3, orc=orc)
     assert seen["divisions"] > 10
     empty_c = np.zeros(0, A.CELL_DTYPE)
     empty_r = np.zeros(0, A.RULESET_DTYPE)

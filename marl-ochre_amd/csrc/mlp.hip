@@ -204,7 +204,8 @@ __global__ void k_finish_grads(size_t mn, int splits, const float* __restrict__ 
 }
 
 constexpr int kMaxSplits = 64;
-constexpr int kDzRows = 64;  // rows per block of k_dz_partials
+constexpr int kDzRows = 64;  // rows per part of k_dz_partials
+static_assert(kDzRows % 4 == 0, "four waves share a part");
 
 // Weight gradients have a small output (<= 256 x 256) and K = the batch: the
 // K range is split over enough blocks to fill the chip, partials summed in a
@@ -236,27 +237,46 @@ hipError_t gemm_splitk(hipStream_t st, int M, int N, int K, const float* A, int 
   return hipGetLastError();
 }
 
-// Column partials of dZ over blocks of kDzRows rows (one thread per column,
-// N <= 256): with SILU, dZ = dH * silu'(Z) is also written; without it dZ is
-// dH itself (the output layer).  part[b][n] = sum of the block's rows in order.
+// Column partials of dZ over parts of kDzRows rows.  A 256-thread block
+// takes 64 columns of one part, each of its four waves a quarter of the rows:
+// with SILU it writes dZ = dH * silu'(Z) (without it dZ is dH itself, the
+// output layer), every value also into LDS; then wave 0 sums each column's
+// values in row order, so part[b][n] = sum of the part's rows in order.
 template <bool SILU>
 __global__ void __launch_bounds__(256) k_dz_partials(int M, int N, const float* __restrict__ dH,
                                                      const float* __restrict__ Z, float* __restrict__ dZ,
                                                      float* __restrict__ part) {
-  const int n = threadIdx.x;
-  if (n >= N) return;
-  const int r0 = blockIdx.x * kDzRows, r1 = min(M, r0 + kDzRows);
-  float acc = 0.0f;
-  for (int r = r0; r < r1; ++r) {
-    const size_t i = (size_t)r * N + n;
-    float v = dH[i];
-    if (SILU) {
-      v *= silu_grad(Z[i]);
-      dZ[i] = v;
+  constexpr int RPW = kDzRows / 4;  // rows per wave
+  __shared__ float sv[kDzRows][65];
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int n = blockIdx.y * 64 + c;
+  const int r0 = blockIdx.x * kDzRows;
+  if (n < N) {
+    float v[RPW], z[RPW];
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+      const int r = r0 + w * RPW + u;
+      v[u] = r < M ? dH[(size_t)r * N + n] : 0.0f;
+      if (SILU) z[u] = r < M ? Z[(size_t)r * N + n] : 0.0f;
     }
-    acc = r == r0 ? v : acc + v;
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+      const int r = r0 + w * RPW + u;
+      float x = v[u];
+      if (SILU) {
+        x *= silu_grad(z[u]);
+        if (r < M) dZ[(size_t)r * N + n] = x;
+      }
+      sv[w * RPW + u][c] = x;
+    }
   }
-  part[(size_t)blockIdx.x * N + n] = acc;
+  __syncthreads();
+  if (w == 0 && n < N) {
+    const int rows = min(kDzRows, M - r0);
+    float acc = 0.0f;
+    for (int u = 0; u < rows; ++u) acc = u == 0 ? sv[u][c] : acc + sv[u][c];
+    part[(size_t)blockIdx.x * N + n] = acc;
+  }
 }
 
 // MSE on normalised targets: dY = 2 (Y - T) / (M * O), loss partials per block
@@ -583,8 +603,8 @@ int mlp_backward(marl_ctx* ctx, MlpState& s, int M) {
   const MlpLayout L = mlp_layout();
   const int parts = (M + kDzRows - 1) / kDzRows;
   // layer 3: dY's column partials, then dW4 with db4 in its finishing launch
-  hipLaunchKernelGGL(k_dz_partials<false>, dim3(parts), dim3(256), 0, ctx->stream, M, kOut, s.dy, nullptr, nullptr,
-                     s.dzpart);
+  hipLaunchKernelGGL(k_dz_partials<false>, dim3(parts, (kOut + 63) / 64), dim3(256), 0, ctx->stream, M, kOut, s.dy,
+                     nullptr, nullptr, s.dzpart);
   MARL_HIP(ctx, hipGetLastError());
   MARL_HIP(ctx, (gemm_splitk<true, false>(ctx->stream, kHidden, kOut, M, s.h[2], kHidden, s.dy, kOut,
                                           s.g + L.w[3], s.ws, kOut, parts, s.dzpart, s.g + L.b[3])));
@@ -592,8 +612,8 @@ int mlp_backward(marl_ctx* ctx, MlpState& s, int M) {
                                               kHidden)));
   for (int l = 2; l >= 0; --l) {
     // dZ = dH * silu'(Z) and its column partials in one pass
-    hipLaunchKernelGGL(k_dz_partials<true>, dim3(parts), dim3(256), 0, ctx->stream, M, kHidden, s.dh, s.z[l], s.dz,
-                       s.dzpart);
+    hipLaunchKernelGGL(k_dz_partials<true>, dim3(parts, (kHidden + 63) / 64), dim3(256), 0, ctx->stream, M, kHidden,
+                       s.dh, s.z[l], s.dz, s.dzpart);
     MARL_HIP(ctx, hipGetLastError());
     const float* hin = l > 0 ? s.h[l - 1] : s.x;
     const int K = l > 0 ? kHidden : kIn;

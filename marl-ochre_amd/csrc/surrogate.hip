@@ -270,29 +270,60 @@ __global__ void __launch_bounds__(256) k_colstats(const int32_t* __restrict__ id
   }
 }
 
-// minibatch: rows drawn from idx by a per-(step, slot) counter hash, normalised
-__global__ void k_gather_batch(const int32_t* __restrict__ idx, int n, uint64_t seed, uint64_t step, int B,
-                               const float* __restrict__ F, const float* __restrict__ T,
-                               const float* __restrict__ norm, const float* __restrict__ inv,
-                               float* __restrict__ X, float* __restrict__ Yt, int sequential) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  int j;
-  if (sequential) {
-    j = b % n;
-  } else {
-    uint64_t c[4] = {step, (uint64_t)b, 0, 0};
-    marl_philox4x64(c, seed, 0x5EEDull);
-    j = (int)(c[0] % (uint64_t)n);
+// minibatch: rows drawn from idx by a per-(step, slot) counter hash,
+// normalised.  A block takes kGatherRows slots: their rows are drawn first
+// (one lane each), then the block copies the rows' features and targets with
+// consecutive lanes on consecutive elements.
+constexpr int kGatherRows = 32;  // the kernel's block is 256 threads
+__global__ void __launch_bounds__(256) k_gather_batch(const int32_t* __restrict__ idx, int n, uint64_t seed,
+                                                      uint64_t step, int B, const float* __restrict__ F,
+                                                      const float* __restrict__ T, const float* __restrict__ norm,
+                                                      const float* __restrict__ inv, float* __restrict__ X,
+                                                      float* __restrict__ Yt, int sequential) {
+  __shared__ int32_t rows[kGatherRows];
+  const int b0 = blockIdx.x * kGatherRows;
+  if (threadIdx.x < kGatherRows) {
+    const int b = b0 + (int)threadIdx.x;
+    int row = 0;
+    if (b < B) {
+      int j;
+      if (sequential) {
+        j = b % n;
+      } else {
+        uint64_t c[4] = {step, (uint64_t)b, 0, 0};
+        marl_philox4x64(c, seed, 0x5EEDull);
+        j = (int)(c[0] % (uint64_t)n);
+      }
+      row = idx[j];
+    }
+    rows[threadIdx.x] = row;
   }
-  const int row = idx[j];
-  float f[kFeat];
-  for (int k = 0; k < kFeat; ++k) f[k] = F[(size_t)row * kFeat + k];
-  for (int k = 0; k < kFeat; ++k) X[(size_t)b * kFeat + k] = (f[k] - norm[k]) * inv[k];
+  __syncthreads();
   const float* tm = norm + 2 * kFeat;
   const float* ts = tm + kTarg;
-  for (int k = 0; k < kTarg; ++k)
-    Yt[(size_t)b * kTarg + k] = ts[k] > 0.0f ? (T[(size_t)row * kTarg + k] - tm[k]) / ts[k] : 0.0f;
+  // every element's load is issued before any is stored
+  constexpr int kF = (kGatherRows * kFeat + 255) / 256, kT = (kGatherRows * kTarg + 255) / 256;
+  float fv[kF], tv[kT];
+#pragma unroll
+  for (int u = 0; u < kF; ++u) {
+    const int e = (int)threadIdx.x + 256 * u, r = e / kFeat, k = e - r * kFeat;
+    fv[u] = (e < kGatherRows * kFeat && b0 + r < B) ? F[(size_t)rows[r] * kFeat + k] : 0.0f;
+  }
+#pragma unroll
+  for (int u = 0; u < kT; ++u) {
+    const int e = (int)threadIdx.x + 256 * u, r = e / kTarg, k = e - r * kTarg;
+    tv[u] = (e < kGatherRows * kTarg && b0 + r < B) ? T[(size_t)rows[r] * kTarg + k] : 0.0f;
+  }
+#pragma unroll
+  for (int u = 0; u < kF; ++u) {
+    const int e = (int)threadIdx.x + 256 * u, r = e / kFeat, k = e - r * kFeat, b = b0 + r;
+    if (e < kGatherRows * kFeat && b < B) X[(size_t)b * kFeat + k] = (fv[u] - norm[k]) * inv[k];
+  }
+#pragma unroll
+  for (int u = 0; u < kT; ++u) {
+    const int e = (int)threadIdx.x + 256 * u, r = e / kTarg, k = e - r * kTarg, b = b0 + r;
+    if (e < kGatherRows * kTarg && b < B) Yt[(size_t)b * kTarg + k] = ts[k] > 0.0f ? (tv[u] - tm[k]) / ts[k] : 0.0f;
+  }
 }
 
 inline unsigned gs(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -562,7 +593,7 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
   }
   float first = -1.0f, last = 0.0f;
   for (int64_t st = 0; st < cfg->steps; ++st) {
-    hipLaunchKernelGGL(k_gather_batch, dim3(gs(B)), dim3(256), 0, ctx->stream, dtr, (int)tr.size(), cfg->seed,
+    hipLaunchKernelGGL(k_gather_batch, dim3(gs(B, kGatherRows)), dim3(256), 0, ctx->stream, dtr, (int)tr.size(), cfg->seed,
                        (uint64_t)st, B, s->hfeat, s->htarg, s->mlp.norm, s->inv, s->mlp.x, s->mlp.t, 0);
     MARL_HIP(ctx, hipGetLastError());
     if (int rc = mlp_forward_train(ctx, s->mlp, B)) return rc;
@@ -580,7 +611,7 @@ int marl_surrogate_train(marl_ctx* ctx, const marl_train_config* cfg, marl_train
   }
   // validation loss on up to one batch of held-out pairs (in order)
   const int VB = (int)std::min<size_t>(va.size(), (size_t)B);
-  hipLaunchKernelGGL(k_gather_batch, dim3(gs(VB)), dim3(256), 0, ctx->stream, dva, (int)va.size(), cfg->seed, 0ull,
+  hipLaunchKernelGGL(k_gather_batch, dim3(gs(VB, kGatherRows)), dim3(256), 0, ctx->stream, dva, (int)va.size(), cfg->seed, 0ull,
                      VB, s->hfeat, s->htarg, s->mlp.norm, s->inv, s->mlp.x, s->mlp.t, 1);
   MARL_HIP(ctx, hipGetLastError());
   if (int rc = mlp_forward_train(ctx, s->mlp, VB)) return rc;

@@ -537,12 +537,14 @@ __device__ __forceinline__ void birth_cell(int i, marl_cell_state* __restrict__ 
                                            const int32_t* __restrict__ target, const int32_t* __restrict__ brank,
                                            marl_cell_state* __restrict__ bcells, marl_ruleset* __restrict__ brules,
                                            int32_t* __restrict__ btarget, uint32_t* __restrict__ claim,
-                                           const CellParams& p) {
+                                           const CellParams& p, const uint64_t* tab = nullptr, int tab_n = 0) {
   const int32_t tgt = target[i];
   if (tgt < 0) return;
   const int32_t rk = brank[i];
   MarlRng rng;
   rng.init(p.seed, p.stream, (uint64_t)i, 1);  // draw 0 picked the target
+  rng.tab = tab;
+  rng.tab_n = (uint32_t)tab_n;
   marl_cell_state d = cells[i];
   const int64_t XY = (int64_t)p.X * p.Y;
   d.pos[0] = (uint16_t)(tgt % p.X);
@@ -566,13 +568,60 @@ __device__ __forceinline__ void birth_cell(int i, marl_cell_state* __restrict__ 
   atomicMin(&claim[tgt], (uint32_t)rk);
 }
 
-__global__ void k_births(int n, marl_cell_state* __restrict__ cells, const marl_ruleset* __restrict__ rules,
-                         const int32_t* __restrict__ target, const int32_t* __restrict__ brank,
-                         marl_cell_state* __restrict__ bcells, marl_ruleset* __restrict__ brules,
-                         int32_t* __restrict__ btarget, uint32_t* __restrict__ claim, CellParams p) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  birth_cell(i, cells, rules, target, brank, bcells, brules, btarget, claim, p);
+// A division's daughter draws some two hundred numbers in a data-dependent
+// order (mutate's draws depend on earlier outcomes), one Philox block per
+// eight.  One-wave blocks look at 64 cells at a time, find the dividing ones
+// by ballot, and for each its 64 lanes compute the cell's first 64 blocks
+// together (draws 0-511, more than a mutation uses), then lane 0 walks the
+// draws from that table; the stream and its order are unchanged.
+constexpr int kBirthBlocks = 64;  // one per lane
+constexpr int kBirthGrid = 4096;  // one-wave blocks of the births kernels
+
+__device__ __forceinline__ void birth_cell_wave(int i, marl_cell_state* __restrict__ cells,
+                                                const marl_ruleset* __restrict__ rules,
+                                                const int32_t* __restrict__ target, const int32_t* __restrict__ brank,
+                                                marl_cell_state* __restrict__ bcells, marl_ruleset* __restrict__ brules,
+                                                int32_t* __restrict__ btarget, uint32_t* __restrict__ claim,
+                                                const CellParams& p, uint64_t* tab) {
+  const int lane = (int)threadIdx.x;  // i is the same in every lane: one cell per block of one wave
+  uint64_t c[4] = {(uint64_t)lane, (uint64_t)i, 0, 0};
+  marl_philox4x64(c, p.seed, p.stream);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) tab[4 * lane + k] = c[k];
+  __syncthreads();
+  if (lane == 0) birth_cell(i, cells, rules, target, brank, bcells, brules, btarget, claim, p, tab, kBirthBlocks);
+  __syncthreads();  // the table is reused for the block's next cell
+}
+
+__device__ __forceinline__ void births_scan(int n, marl_cell_state* __restrict__ cells,
+                                            const marl_ruleset* __restrict__ rules,
+                                            const int32_t* __restrict__ target, const int32_t* __restrict__ brank,
+                                            marl_cell_state* __restrict__ bcells, marl_ruleset* __restrict__ brules,
+                                            int32_t* __restrict__ btarget, uint32_t* __restrict__ claim,
+                                            const CellParams& p, uint64_t* tab) {
+  // block b takes cells b, b + G, b + 2G, ... (G blocks), 64 of them per
+  // ballot: dividing cells, which come in runs of neighbouring indices, are
+  // spread over the blocks
+  const int G = (int)gridDim.x, b = (int)blockIdx.x;
+  for (int64_t k0 = 0; b + k0 * G < n; k0 += 64) {
+    const int64_t i = b + (k0 + (int64_t)threadIdx.x) * G;
+    unsigned long long m = __ballot(i < n && target[i] >= 0);
+    while (m) {
+      const int l = __ffsll((long long)m) - 1;
+      m &= m - 1;
+      birth_cell_wave((int)(b + (k0 + l) * G), cells, rules, target, brank, bcells, brules, btarget, claim, p, tab);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) k_births(int n, marl_cell_state* __restrict__ cells,
+                                               const marl_ruleset* __restrict__ rules,
+                                               const int32_t* __restrict__ target, const int32_t* __restrict__ brank,
+                                               marl_cell_state* __restrict__ bcells, marl_ruleset* __restrict__ brules,
+                                               int32_t* __restrict__ btarget, uint32_t* __restrict__ claim,
+                                               CellParams p) {
+  __shared__ uint64_t tab[kBirthBlocks * 4];
+  births_scan(n, cells, rules, target, brank, bcells, brules, btarget, claim, p, tab);
 }
 
 __global__ void k_birth_winners(int nb, const int32_t* __restrict__ btarget, const uint32_t* __restrict__ claim,
@@ -878,15 +927,17 @@ __global__ void k_g_div_target(const TickDev* ts, int64_t cap, const marl_cell_s
   }
 }
 
-__global__ void k_g_births(const TickDev* ts, marl_cell_state* __restrict__ cells,
-                           const marl_ruleset* __restrict__ rules, const int32_t* __restrict__ target,
-                           const int32_t* __restrict__ brank, marl_cell_state* __restrict__ bcells,
-                           marl_ruleset* __restrict__ brules, int32_t* __restrict__ btarget,
-                           uint32_t* __restrict__ claim, CellParams p) {
+__global__ void __launch_bounds__(64) k_g_births(const TickDev* ts, marl_cell_state* __restrict__ cells,
+                                                 const marl_ruleset* __restrict__ rules,
+                                                 const int32_t* __restrict__ target,
+                                                 const int32_t* __restrict__ brank, marl_cell_state* __restrict__ bcells,
+                                                 marl_ruleset* __restrict__ brules, int32_t* __restrict__ btarget,
+                                                 uint32_t* __restrict__ claim, CellParams p) {
+  __shared__ uint64_t tab[kBirthBlocks * 4];
   if (ts->err) return;
   const int n = ts->n;
   p.stream = div_stream(ts);
-  GS_LOOP(i, n) birth_cell((int)i, cells, rules, target, brank, bcells, brules, btarget, claim, p);
+  births_scan(n, cells, rules, target, brank, bcells, brules, btarget, claim, p, tab);
 }
 
 __global__ void k_g_death_flags(const TickDev* ts, int64_t cap, const uint8_t* __restrict__ events,
@@ -1514,8 +1565,8 @@ int run_fate(marl_ctx* ctx, const CellParams& p, marl_tick_stats* st) {
   MARL_HIP(ctx, hipGetLastError());
   if (int rc = fill_i32(ctx, bflag + n, 0, 1)) return rc;
   if (int rc = scan_i32(ctx, bflag, brank, n + 1)) return rc;  // brank[n] = number of births
-  hipLaunchKernelGGL(k_births, dim3(grid1(n, 64)), dim3(64), 0, ctx->stream, n, cells, rules, target, brank,
-                     ctx->birth_cells, ctx->birth_rules, btarget, ctx->claim, p);
+  hipLaunchKernelGGL(k_births, dim3((unsigned)std::min(n, kBirthGrid)), dim3(64), 0, ctx->stream, n, cells,
+                     rules, target, brank, ctx->birth_cells, ctx->birth_rules, btarget, ctx->claim, p);
   MARL_HIP(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_death_flags, dim3(grid1(n)), dim3(256), 0, ctx->stream, n, ctx->events, win);
   MARL_HIP(ctx, hipGetLastError());
@@ -1726,8 +1777,8 @@ int marl_enqueue_pass_and_fate(marl_ctx* ctx, const marl_sim_params* sim, bool f
   hipLaunchKernelGGL(k_g_div_target, dim3(gf), dim3(256), 0, st, ts, cap, cells, ctx->events, ctx->cell_at, target,
                      bflag, p);
   if (int rc = scan_i32(ctx, bflag, brank, (int)cap + 1)) return rc;
-  hipLaunchKernelGGL(k_g_births, dim3(gs_grid(cap, 64, 8192)), dim3(64), 0, st, ts, cells, rules, target, brank,
-                     ctx->birth_cells, ctx->birth_rules, btarget, ctx->claim, p);
+  hipLaunchKernelGGL(k_g_births, dim3((unsigned)std::min<int64_t>(std::max<int64_t>(cap, 1), kBirthGrid)), dim3(64), 0, st,
+                     ts, cells, rules, target, brank, ctx->birth_cells, ctx->birth_rules, btarget, ctx->claim, p);
   hipLaunchKernelGGL(k_g_death_flags, dim3(gf), dim3(256), 0, st, ts, cap, ctx->events, win);
   if (int rc = scan_i32(ctx, win, P, (int)cap + 1)) return rc;
   hipLaunchKernelGGL(k_g_counts, dim3(1), dim3(1), 0, st, ts, brank, P);

@@ -125,14 +125,22 @@ struct MarlRng {
   uint64_t seed, stream, entity, draw;
   uint64_t blk[4];
   uint64_t blk_id;
+  // optional: the stream's blocks [0, tab_n) computed ahead (4 words each,
+  // e.g. by the other lanes of a wave); later blocks are computed here
+  const uint64_t* tab;
+  uint32_t tab_n;
   __host__ __device__ void init(uint64_t s, uint64_t st, uint64_t e, uint64_t d0 = 0) {
-    seed = s; stream = st; entity = e; draw = d0; blk_id = ~0ull;
+    seed = s; stream = st; entity = e; draw = d0; blk_id = ~0ull; tab = nullptr; tab_n = 0;
   }
   __host__ __device__ uint32_t u32() {
     uint64_t b = draw >> 3;
     if (b != blk_id) {
-      blk[0] = b; blk[1] = entity; blk[2] = 0; blk[3] = 0;
-      marl_philox4x64(blk, seed, stream);
+      if (b < tab_n) {
+        for (int k = 0; k < 4; ++k) blk[k] = tab[4 * b + k];
+      } else {
+        blk[0] = b; blk[1] = entity; blk[2] = 0; blk[3] = 0;
+        marl_philox4x64(blk, seed, stream);
+      }
       blk_id = b;
     }
     uint64_t w = blk[(draw >> 1) & 3];

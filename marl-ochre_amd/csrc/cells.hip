@@ -1374,6 +1374,14 @@ bool marl_choose_cell_layout(marl_ctx* ctx) {
   return ctx->kin_quads;
 }
 
+// Whether the plain or learned pass of this tick runs on the AoS view (the
+// choice run_levels makes; the population is the same until the pass).
+static bool pass_uses_view(marl_ctx* ctx, const CellParams& p) {
+  const HarvestSink hv = surrogate_harvest_sink(ctx, p);
+  const bool plain = ctx->kin_mode == MARL_KINETICS_PLAIN || !ctx->surr;
+  return !(plain && !hv.count && marl_choose_cell_layout(ctx));
+}
+
 int marl_set_cell_layout(marl_ctx* ctx, int32_t layout) {
   if (!ctx) return MARL_ERR_INVALID_INPUT;
   if (layout != MARL_CELL_LAYOUT_AUTO && layout != MARL_CELL_LAYOUT_AOS && layout != MARL_CELL_LAYOUT_QUADS)
@@ -1484,7 +1492,14 @@ int run_schedule(marl_ctx* ctx, const CellParams& p, std::vector<int32_t>& off) 
 int run_levels(marl_ctx* ctx, const CellParams& p, const std::vector<int32_t>& off, hipEvent_t back_start,
                hipEvent_t back_done) {
   const int nlev = (int)off.size() - 1;
-  if (nlev <= 0) {
+  // the field phase may have left the new field in the view already
+  const bool fresh = ctx->view_fresh;
+  ctx->view_fresh = false;
+  const HarvestSink hv = surrogate_harvest_sink(ctx, p);
+  const bool plain = ctx->kin_mode == MARL_KINETICS_PLAIN || !ctx->surr;
+  const bool quads = plain && !hv.count && marl_choose_cell_layout(ctx);
+  if (fresh && quads) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "cell pass on the quads after a field phase into the view");
+  if (nlev <= 0 && !fresh) {
     if (back_done) {
       MARL_HIP(ctx, hipEventRecord(back_start, ctx->aux));
       MARL_HIP(ctx, hipEventRecord(back_done, ctx->aux));
@@ -1493,13 +1508,11 @@ int run_levels(marl_ctx* ctx, const CellParams& p, const std::vector<int32_t>& o
   }
   marl_cell_state* cells = ctx->cells[ctx->cbuf];
   const marl_ruleset* rules = ctx->rules[ctx->cbuf];
-  const HarvestSink hv = surrogate_harvest_sink(ctx, p);
-  const bool plain = ctx->kin_mode == MARL_KINETICS_PLAIN || !ctx->surr;
-  const bool quads = plain && !hv.count && marl_choose_cell_layout(ctx);
   float* field = ctx->field[ctx->cur];
   if (!quads) {
     if (int rc = marl_ensure_staging(ctx)) return rc;
-    if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
+    if (!fresh)
+      if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
     field = ctx->staging;
   }
   for (int L = 0; L < nlev; ++L) {
@@ -1681,6 +1694,7 @@ int marl_enqueue_cell_phase(marl_ctx* ctx, const marl_sim_params* sim) {
 // The one-workgroup cell phase and the counters' finish, on ctx->stream
 // (TickDev staged; the resident quads hold the field).
 int marl_enqueue_cell_phase_small(marl_ctx* ctx, const marl_sim_params* sim) {
+  if (ctx->view_fresh) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "cell pass on the quads after a field phase into the view");
   TickDev* ts = static_cast<TickDev*>(ctx->tdev);
   const CellParams p = cell_params(ctx, sim, 0);  // the RNG stream comes from ts->tick
   hipLaunchKernelGGL(k_g_cell_phase_small, dim3(1), dim3(kSmallThreads), 0, ctx->stream, ts, ctx->cells[ctx->cbuf],
@@ -1750,10 +1764,14 @@ int marl_enqueue_pass_and_fate(marl_ctx* ctx, const marl_sim_params* sim, bool f
                               rules, field, ctx->light, ctx->nbmask, ctx->events, p);
     }
   };
+  const bool fresh = ctx->view_fresh;  // the field phase wrote the view
+  ctx->view_fresh = false;
   if (ctx->kin_quads) {
+    if (fresh) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "cell pass on the quads after a field phase into the view");
     for (int L = 0; L < ctx->graph_levels; ++L) level(L, ctx->field[ctx->cur], true);
   } else {
-    if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
+    if (!fresh)
+      if (int rc = marl_transpose(ctx, false, ctx->field[ctx->cur])) return rc;
     for (int L = 0; L < ctx->graph_levels; ++L) level(L, ctx->staging, false);
     if (fork_back) {
       MARL_HIP(ctx, hipEventRecord(ctx->xev[2], st));
@@ -2024,7 +2042,8 @@ int capture_tick(marl_ctx* ctx, const marl_sim_params* sim) {
       rc = marl_fail(ctx, MARL_ERR_TRANSFER, "capture: fork of the schedule stream failed");
     }
   }
-  if (rc == MARL_OK) rc = marl_enqueue_field_phase(ctx, sim);
+  // a pass on the AoS view takes the last substep's output directly
+  if (rc == MARL_OK) rc = marl_launch_field_phase(ctx, sim, true, ctx->kin_quads ? nullptr : ctx->staging);
   const int cur_after = ctx->cur;
   if (forked) {  // join the schedule stream on every path, so that the capture can end
     const hipError_t j1 = hipEventRecord(ctx->xev[0], ctx->aux);
@@ -2035,6 +2054,7 @@ int capture_tick(marl_ctx* ctx, const marl_sim_params* sim) {
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
   ctx->cur = slot;
+  ctx->view_fresh = false;  // nothing ran: the captured work starts from the quads
   if (rc != MARL_OK) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
@@ -2187,6 +2207,17 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
     hipStream_t aux;
     ~AuxJoin() { (void)hipStreamSynchronize(aux); }
   } aux_join{ctx->aux};
+  // a tick that fails between a field phase into the view and the pass's
+  // transpose back restores the quads from the view (the field phase's result)
+  struct ViewBack {
+    marl_ctx* c;
+    ~ViewBack() {
+      if (!c->view_fresh) return;
+      c->view_fresh = false;
+      (void)marl_transpose(c, true, c->field[c->cur]);
+      (void)hipStreamSynchronize(c->stream);
+    }
+  } view_back{ctx};
   marl_tick_stats st{};
   std::vector<int32_t> off;
   for (int t = 0; t < n_ticks; ++t) {
@@ -2208,7 +2239,14 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
       MARL_HIP(ctx, hipEventRecord(ctx->xev[1], ctx->stream));
       MARL_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->xev[1], 0));
     }
-    if (int rc = marl_launch_field_phase(ctx, sim, true)) return rc;
+    // a pass on the AoS view takes the last substep's output directly (not
+    // with the phase budget, which reads the quads between the phases)
+    float* view = nullptr;
+    if (!ctx->phase_budget && pass_uses_view(ctx, p)) {
+      if (int rc = marl_ensure_staging(ctx)) return rc;
+      view = ctx->staging;
+    }
+    if (int rc = marl_launch_field_phase(ctx, sim, true, view)) return rc;
     if (ctx->phase_budget)
       if (int rc = enqueue_budget(ctx, ctx->phase_budget_dev + kBudget)) return rc;
     MARL_HIP(ctx, hipEventRecord(ev[1], ctx->stream));

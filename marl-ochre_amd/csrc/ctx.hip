@@ -38,6 +38,7 @@ StencilParams marl_stencil_params(const marl_sim_params* sim, int apply_sources)
   p.fuse_light = 0;
   p.l_i0 = p.l_e_cell = p.l_nchem = p.l_floor = 0.0f;
   p.light = nullptr;
+  p.view = nullptr;
   return p;
 }
 

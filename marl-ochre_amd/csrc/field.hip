@@ -186,18 +186,18 @@ __device__ __forceinline__ void ld_vec(const float* __restrict__ base, const Ste
   for (int k = 0; k < A; ++k) out[k] = with_sources<SRC>(out[k], p, s + k, Z, zz);
 }
 
-// Non-temporal store of species s .. s+A-1 of one voxel.
-template <int A>
+// Store of species s .. s+A-1 of one voxel with cache policy CP (2: non-temporal).
+template <int A, int CP = 2>
 __device__ __forceinline__ void st_vec_nt(rsrc_t r, const float* v, uint32_t voff, uint32_t soff) {
   if constexpr (A == 4) {
     const u32x4 t = {__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1]),
                      __builtin_bit_cast(uint32_t, v[2]), __builtin_bit_cast(uint32_t, v[3])};
-    __builtin_amdgcn_raw_buffer_store_b128(t, r, (int)voff, (int)soff, 2);
+    __builtin_amdgcn_raw_buffer_store_b128(t, r, (int)voff, (int)soff, CP);
   } else if constexpr (A == 2) {
     const u32x2 t = {__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1])};
-    __builtin_amdgcn_raw_buffer_store_b64(t, r, (int)voff, (int)soff, 2);
+    __builtin_amdgcn_raw_buffer_store_b64(t, r, (int)voff, (int)soff, CP);
   } else {
-    bstore_nt(r, v[0], voff, soff);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v[0]), r, (int)voff, (int)soff, CP);
   }
 }
 
@@ -410,8 +410,18 @@ __device__ __forceinline__ void stencil_tile(const float* __restrict__ src, floa
                                                  (__builtin_bit_cast(uint32_t, fmaxf(nc, 0.0f)) & ~keep));
         }
         if (valid[r]) {
+          if (LM != 0 && p.view) {
+            // into the cell pass's AoS view: species k of voxel v at byte
+            // 48 v + 4 k (vo = 16 v, zo = 16 XY z); the block's species groups
+            // fill each record's three quads side by side, so the lines are
+            // completed in L2 (plain stores, not non-temporal)
+            const rsrc_t rv = make_rsrc(p.view, Qb * 3u);
 #pragma unroll
-          for (int s = 0; s < S; s += A) st_vec_nt<A>(rd, &out[s], vo[r], qoff(s0 + s, Qb) + zo);
+            for (int s = 0; s < S; s += A) st_vec_nt<A, 0>(rv, &out[s], vo[r] * 3u, zo * 3u + (uint32_t)(s0 + s) * 4u);
+          } else {
+#pragma unroll
+            for (int s = 0; s < S; s += A) st_vec_nt<A>(rd, &out[s], vo[r], qoff(s0 + s, Qb) + zo);
+          }
           // LM 1: light.rs:41-55 on the new species 4 (slot 4 % SG, as for
           // species 7); LM 2: the new species 4 into the light plane, compact,
           // for the separate light scan to read in place
@@ -869,8 +879,9 @@ static bool tile_fuses_light(int tile, int Z) {
 
 // light_mode: what the last substep does for the tick's light scan (0
 // nothing, 1 the scan itself, 2 the compact species-4 copy the scan reads).
+// With light_mode set, `view` (if not null) takes the last substep's output.
 static int substeps_impl(marl_ctx* ctx, const marl_sim_params* sim, int n, bool sources_first, int tile,
-                         int* launches, int light_mode) {
+                         int* launches, int light_mode, float* view = nullptr) {
   if (tile < 0 || tile >= kNumTiles) tile = marl_default_tile(ctx->d);
   const TileShape t = kTiles[tile];
   int k = 0, nl = 0;
@@ -892,6 +903,7 @@ static int substeps_impl(marl_ctx* ctx, const marl_sim_params* sim, int n, bool 
       p.l_nchem = -sim->chemical_absorption;
       p.l_floor = sim->light_floor;
       p.light = ctx->light;
+      p.view = view;
     }
     if (int rc = marl_launch_substep(ctx, p, single)) return rc;
     src = false;
@@ -905,8 +917,9 @@ int marl_launch_substeps(marl_ctx* ctx, const marl_sim_params* sim, int n, bool 
   return substeps_impl(ctx, sim, n, sources_first, tile, launches, false);
 }
 
-int marl_launch_field_phase(marl_ctx* ctx, const marl_sim_params* sim, bool sources) {
+int marl_launch_field_phase(marl_ctx* ctx, const marl_sim_params* sim, bool sources, float* view) {
   const int n = sim->diffusion_substeps;
+  ctx->view_fresh = false;
   if (n <= 0) {
     if (sources)
       if (int rc = marl_launch_boundary(ctx, sim)) return rc;
@@ -918,7 +931,9 @@ int marl_launch_field_phase(marl_ctx* ctx, const marl_sim_params* sim, bool sour
   // an odd remainder runs on the default tile, an even count has no single)
   const int last = kTiles[tile].tb == 2 ? (n % 2 ? marl_default_tile(ctx->d) : -1) : tile;
   const int mode = tile_fuses_light(last, ctx->d.z) ? 1 : (last >= 0 ? 2 : 0);
-  if (int rc = substeps_impl(ctx, sim, n, sources, ctx->tile, nullptr, mode)) return rc;
+  if (view && !mode) view = nullptr;  // no single last substep to write it
+  if (int rc = substeps_impl(ctx, sim, n, sources, ctx->tile, nullptr, mode, view)) return rc;
+  ctx->view_fresh = view != nullptr;
   if (mode == 1) return MARL_OK;
   return mode == 2 ? marl_launch_light_compact(ctx, sim) : marl_launch_light(ctx, sim);
 }

@@ -184,6 +184,9 @@ struct StencilParams {
   int32_t fuse_light;  // last substep's light work: 0 none, 1 the scan, 2 compact species 4 into the light plane
   float l_i0, l_e_cell, l_nchem, l_floor;
   float* light;
+  // last substep of a tick: the new field goes into this AoS view (48 B per
+  // voxel, the cell pass's layout) instead of the quads, when not null
+  float* view;
 };
 
 // ---------------------------------------------------------------------------
@@ -228,6 +231,9 @@ struct marl_ctx {
   uint32_t* occ = nullptr;
   float* light = nullptr;
   float* staging = nullptr;  // AoS staging (nfield floats), allocated on first use
+  // the last field phase left the new field in the AoS view (staging) only;
+  // the quads field[cur] are stale until the cell pass transposes the view back
+  bool view_fresh = false;
   uint8_t* occ_bytes = nullptr;
   int tile = -1;             // stencil tile id; -1 = default
 
@@ -333,7 +339,11 @@ int marl_launch_substeps(marl_ctx* ctx, const marl_sim_params* sim, int n, bool 
 // A tick's field phase: boundary sources, the substeps and the light scan,
 // with the sources fused into the first substep and, where the tile allows,
 // the light scan into the last.
-int marl_launch_field_phase(marl_ctx* ctx, const marl_sim_params* sim, bool sources);
+// With `view` (ctx->staging) the last substep writes the new field into that
+// AoS view instead of the quads and ctx->view_fresh is set, so the cell pass
+// skips its transpose in; a phase that cannot do so (no single last substep)
+// writes the quads and clears the flag.
+int marl_launch_field_phase(marl_ctx* ctx, const marl_sim_params* sim, bool sources, float* view = nullptr);
 int marl_launch_boundary(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_launch_light(marl_ctx* ctx, const marl_sim_params* sim);
 int marl_launch_light_compact(marl_ctx* ctx, const marl_sim_params* sim);

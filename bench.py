@@ -183,8 +183,9 @@ def kinetics_bench(pkg, ticks, with_cpu, reps=5):
                     f"{reps} repetitions from the same state",
         "cells": n0,
         "Mcell_updates_per_s_kinetics": round(n0 / (kin_ms * 1e-3) / 1e6, 3),
-        # ms_kinetics is the pass on the tick's critical path: the transpose into
-        # the AoS view and the levels; the transpose back runs beside fate
+        # ms_kinetics is the pass on the tick's critical path: the levels on the
+        # AoS view, which the field phase's last substep writes; the transpose
+        # back to the quads runs beside fate
         "ms_kinetics_with_transpose_back": round(
             kin_ms + sum(st["ms_transpose_back"] for _, st in runs) / reps, 3),
         "Mcell_updates_per_s_full_tick": round(n0 / (tick_ms * 1e-3) / 1e6, 3),

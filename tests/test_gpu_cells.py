@@ -129,6 +129,45 @@ def test_ticks_with_divisions_deaths_and_mutation_bit_identical(pkg, orc, layout
     assert seen["divisions"] > 50 and seen["deaths"] > 50, seen
 
 
+@pytest.mark.parametrize("case", ["temporal_even", "temporal_odd", "split_species", "phase_budget"])
+def test_ticks_on_the_view_for_every_kind_of_last_substep(pkg, orc, case):
+    """The pass on the AoS view, which the tick's last substep writes when it is a
+    single-substep launch (temporal tile with an odd count, a tile whose blocks hold
+    some of the species) and which is transposed from the quads otherwise (a
+    temporal tile with an even count, the per-phase mass budget): each against the
+    oracle over ticks with divisions and deaths."""
+    A = pkg._abi
+    dims = (64, 32, 24)
+    cells, rules = population(A, dims, 4000, 21)
+    field = np.random.default_rng(22).random(np.prod(dims) * 12).astype(F)
+    sim = pkg.SimulationConfig(rng_seed=79, diffusion_substeps=5 if case == "temporal_odd" else 4)
+    with pkg.MarlGpu(dims) as g:
+        n = g.lib.marl_stencil_tile_count()
+        if case.startswith("temporal"):
+            g.tile = next(t for t in range(n) if g.tile_substeps(t) == 2)
+        elif case == "split_species":
+            g.tile = next(t for t in range(n)
+                          if g.tile_substeps(t) == 1 and g.tile_split(t)[0] * (g.tile_split(t)[1] // 4) < 12)
+        else:
+            g.set_phase_budget(True)
+        g.set_cell_layout("aos")
+        g.upload_field_aos(field)
+        g.upload_cells(cells, rules)
+        f, c, r = field, cells, rules
+        seen = 0
+        for t in range(3):
+            f, lt, c, r, ev, st = orc.tick(dims, sim, t, f, c, r)
+            gs = g.tick(sim, t, 1)
+            assert gs["n_cells"] == st["n_cells"], (t, gs["n_cells"], st["n_cells"])
+            seen += st["divisions"] + st["deaths"]
+            c_g, r_g = g.download_cells()
+            assert_cells_equal(c_g, r_g, c, r)
+            f_g = g.download_field_aos()
+            assert np.array_equal(f_g, f), (t, np.count_nonzero(f_g != f))
+            assert np.array_equal(g.download_light(), lt), t
+    assert seen > 50
+
+
 @pytest.mark.parametrize("block", [64, 256])
 @LAYOUTS
 def test_ticks_every_kinetics_block(pkg, orc, layout, block):

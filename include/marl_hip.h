@@ -218,8 +218,9 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
 int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t n_ticks,
                     marl_tick_stats* stats, int32_t* host_finished);
 /* Field layout the plain cell pass works on; results are identical either way.
- * MARL_CELL_LAYOUT_AOS: an AoS view (48-byte voxel records) transposed from
- * and back to the resident species quads around the pass; _QUADS: the
+ * MARL_CELL_LAYOUT_AOS: an AoS view (48-byte voxel records) that a tick's
+ * last substep writes (a pass outside a tick transposes it from the quads),
+ * transposed back to the resident species quads after the pass; _QUADS: the
  * resident quads in place (no transposes, more cache lines per neighbour);
  * _AUTO (default): quads below about one cell per 10 voxels.  A tuning knob
  * with no counterpart in the reference (whose field is always AoS,

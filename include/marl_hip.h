@@ -222,7 +222,8 @@ int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, in
  * last substep writes (a pass outside a tick transposes it from the quads),
  * transposed back to the resident species quads after the pass; _QUADS: the
  * resident quads in place (no transposes, more cache lines per neighbour);
- * _AUTO (default): quads below about one cell per 10 voxels.  A tuning knob
+ * _AUTO (default): quads below about one cell per 20 voxels (per 10 for
+ * ensemble columns, which transpose both ways).  A tuning knob
  * with no counterpart in the reference (whose field is always AoS,
  * crates/marl-field/src/field.rs:15-35). */
 int marl_set_cell_layout(marl_ctx* ctx, int32_t layout);

@@ -1358,19 +1358,25 @@ int marl_launch_occupancy_from_cells(marl_ctx* ctx) {
   return MARL_OK;
 }
 
-// Field layout for the plain cell pass.  The AoS view costs two whole-field
-// transposes (4 x 48 bytes moved per voxel, about 32 ps per voxel at
-// 256x256x128); the quads in place cost each cell about 0.33 ns more, since a
-// neighbour's species then span three cache lines instead of one or two
-// (both measured with 1M cells at 256x256x128).  The quads win below about one
-// cell per 10 voxels; a band of +-25 % around that keeps a captured tick from
+// Field layout for the plain cell pass.  The quads in place cost each cell
+// about 0.33 ns more than the AoS view, since a neighbour's species then span
+// three cache lines instead of one or two.  The view costs whole-field
+// transposes (48 bytes each way per voxel, about 15 ps per voxel at
+// 256x256x128).  In a single-column tick the last substep writes the view
+// (about +6 ps per voxel on that substep) and only the transpose back remains,
+// beside fate: the tick is as fast either way near one cell per 20 voxels
+// (measured at 256x256x128: 200k cells 1.958 vs 1.948 ms with quads, 400k
+// 2.146 vs 2.141, 550k 2.289 vs 2.356, 1M 2.682 vs 2.997).  An ensemble
+// column transposes both ways, which moves the crossover to about one cell
+// per 10 voxels.  A band around the crossover keeps a captured tick from
 // switching layouts back and forth.
 bool marl_choose_cell_layout(marl_ctx* ctx) {
   if (ctx->cell_layout == MARL_CELL_LAYOUT_AOS) return ctx->kin_quads = false;
   if (ctx->cell_layout == MARL_CELL_LAYOUT_QUADS) return ctx->kin_quads = true;
   const double r = 10.0 * (double)ctx->n_cells / (double)std::max<int64_t>(ctx->nvox, 1);
-  if (ctx->kin_quads && r > 1.25) ctx->kin_quads = false;
-  else if (!ctx->kin_quads && r < 0.8) ctx->kin_quads = true;
+  const double to_view = ctx->layout_view ? 0.6 : 1.25, to_quads = ctx->layout_view ? 0.35 : 0.8;
+  if (ctx->kin_quads && r > to_view) ctx->kin_quads = false;
+  else if (!ctx->kin_quads && r < to_quads) ctx->kin_quads = true;
   return ctx->kin_quads;
 }
 
@@ -2186,6 +2192,7 @@ int marl_tick(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, int32_t 
   if (n_ticks < 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "negative tick count");
   if (int rc = marl_fault_tick(ctx)) return rc;
   ctx->last_sim = *sim;
+  ctx->layout_view = true;
   if (int rc = ensure_voxel_buffers(ctx)) return rc;
   if (ctx->phase_budget && !ctx->phase_budget_dev)
     MARL_HIP(ctx, hipMalloc(&ctx->phase_budget_dev, sizeof(double) * 4 * kBudget));
@@ -2330,6 +2337,7 @@ int marl_tick_graph(marl_ctx* ctx, const marl_sim_params* sim, int64_t tick0, in
                     marl_tick_stats* stats, int32_t* host_finished) {
   if (!ctx) return MARL_ERR_INVALID_INPUT;
   if (n_ticks < 0) return marl_fail(ctx, MARL_ERR_INVALID_INPUT, "negative tick count");
+  ctx->layout_view = true;
   marl_tick_stats st{};
   int32_t finished = 0;
   for (int t = 0; t < n_ticks;) {

@@ -347,8 +347,10 @@ int marl_ensemble_tick(marl_ensemble* e, const marl_sim_params* sim, const uint6
   float ms_total = 0.0f;
   std::vector<marl_tick_stats> st(n);
   for (int t = 0; t < n_ticks; ++t) {
-    for (int c = 0; c < n; ++c)
+    for (int c = 0; c < n; ++c) {
+      e->cols[c]->layout_view = false;  // the batched field phase writes quads only
       if (int rc = marl_graph_prepare(e->cols[c], &sims[c])) return col_fail(e, c, rc);
+    }
     if (e->key != ens_key(e, sims))
       if (int rc = capture(e, sims)) return rc;
     if (int rc = ensure_counter_buffers(e)) return rc;

@@ -278,6 +278,9 @@ struct marl_ctx {
   bool graph_auto = true;                    // budget follows the schedule's needs
   int cell_layout = MARL_CELL_LAYOUT_AUTO;    // field layout of the plain cell pass
   bool kin_quads = false;                    // the layout last chosen: quads in place
+  // which crossover marl_choose_cell_layout uses: a single-column tick, whose
+  // last substep writes the view, or an ensemble column, which transposes both ways
+  bool layout_view = true;
   bool graph_overlap = false;                // the captured tick overlaps phases on ctx->aux
   bool graph_small = false;                  // the captured cell phase runs in one workgroup
   int kin_block = 128;                       // threads per block of the plain pass (64, 128 or 256)
